@@ -1,0 +1,190 @@
+/*
+ * osiris_amd.h -- C ABI of the MI355X (gfx950) PIC hot path.
+ *
+ * Plain C types only (pointers, sizes, status codes).  Every entry point returns an int
+ * status (OSR_OK = 0) unless it mirrors a reference entry point that returns nothing;
+ * the message of the last failure of the calling thread is osr_last_error().
+ *
+ * Three groups of entry points:
+ *
+ *  1. Stateless drop-ins for the reference's native fused kernels.  Same argument list,
+ *     meaning and layout as FNAME(vadvance_deposit_{2d,3d}_s{1,2,3}) in
+ *     simd/os-spec-push-avx2.c:471-475 (2D) and :761-765 (3D), called from
+ *     spec/os-spec-vpush.f03:144-169 / :294-318 with every argument by reference.
+ *     They copy the caller's host arrays to the GPU, run gather + Boris push + position
+ *     advance + charge-conserving deposit, and copy the results back.
+ *
+ *  2. A stateless Yee advance on the caller's guarded field arrays, replacing
+ *     t_emf_solver%advance for periodic boundaries
+ *     (emf/solver/os-emf-solver-define.f03:26-34, os-emf-solver-yee.f03:542-568).
+ *
+ *  3. The resident fast path: a simulation handle whose fields and particles live in
+ *     HBM; osr_sim_run advances whole time steps (gather, push, deposit, current fold,
+ *     Yee, energy diagnostics) without host round trips.  This replaces, per time
+ *     step, t_particles%advance_deposit (particles/os-particles-define.f03:256-269)
+ *     followed by jay%update_boundary and t_emf%advance (os-simulation.f03:588-661).
+ *
+ * Array conventions of the reference (spec/os-spec-define.f03:391-412,
+ * vdf/os-vdf-define.f03:35-116): particle ix(D,N) are 1-based cell indices, x(D,N) the
+ * position in [-0.5,0.5) relative to the cell, u(3,N) proper velocities, q(N) charges;
+ * grids are f(3, 1-gcl:nx+gcu, 1-gcl:ny+gcu[, ...]) stored component fastest.
+ */
+#ifndef OSIRIS_AMD_H
+#define OSIRIS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  OSR_OK = 0,
+  OSR_ERR_ARG = 1,      /* invalid argument */
+  OSR_ERR_HIP = 2,      /* HIP runtime error (no device, launch failure, ...) */
+  OSR_ERR_NOMEM = 3,    /* device allocation failed */
+  OSR_ERR_STATE = 4,    /* call not valid in the current state */
+  OSR_ERR_LAYOUT = 5    /* a particle was found outside its tile (internal invariant) */
+};
+
+enum { OSR_BORIS = 0, OSR_VAY = 1 };
+
+/* Message of the last error on this thread ("" when none). */
+const char *osr_last_error(void);
+/* ABI version: major*10000 + minor*100 + patch. */
+int osr_version(void);
+/* Number of visible GPUs (0 when none); never initialises a context on failure. */
+int osr_device_count(void);
+
+/* ---------------------------------------------------------------------------------
+ * 1. Stateless drop-ins (reference: simd/os-spec-push-avx2.c:471-475, 761-765).
+ *
+ * osr_advance_deposit is the status-returning core; pusher = OSR_BORIS or OSR_VAY
+ * (the reference's SIMD kernels are Boris only).  The particles i0..i1 (1-based,
+ * inclusive) are pushed; ix, x, u are updated in place; the current of their motion is
+ * ADDED to j (the caller zeroes it, particles/os-particles.f03:1426,1453); *ene is
+ * incremented by the time-centred Sum q*u^2/(gamma+1) (not normalised), as the
+ * reference does.  emf_size/j_size are the array extents in cells (guards included),
+ * emf_offset/j_offset the number of lower guard cells per dimension.
+ * Differences from the reference: no dummy particles are written past i1, and the
+ * current is accumulated exactly (int64 fixed point) so the result does not depend on
+ * thread scheduling.
+ * ------------------------------------------------------------------------------- */
+int osr_advance_deposit(int dims, int order, int pusher, int *ix, float *x, float *u,
+                        const float *q, int i0, int i1, float rqm, const float *efield,
+                        const float *bfield, const int *emf_size, const int *emf_offset,
+                        float *j, const int *j_size, const int *j_offset, const double *dx,
+                        double dt, double *ene);
+
+/* Fortran-callable twins of FNAME(vadvance_deposit_*_sN): every argument by reference,
+ * no return value.  On failure they print the error and exit(1), as the reference's
+ * kernels do on a violated precondition (simd/os-spec-push-avx2.c:537-547). */
+void hip_vadvance_deposit_2d_s1_(int *ix, float *x, float *u, float *q, int *i0, int *i1,
+                                 float *rqm, float *efield, float *bfield, int *emf_size,
+                                 int *emf_offset, float *j, int *j_size, int *j_offset,
+                                 double *dx, double *dt, double *ene);
+void hip_vadvance_deposit_2d_s2_(int *ix, float *x, float *u, float *q, int *i0, int *i1,
+                                 float *rqm, float *efield, float *bfield, int *emf_size,
+                                 int *emf_offset, float *j, int *j_size, int *j_offset,
+                                 double *dx, double *dt, double *ene);
+void hip_vadvance_deposit_2d_s3_(int *ix, float *x, float *u, float *q, int *i0, int *i1,
+                                 float *rqm, float *efield, float *bfield, int *emf_size,
+                                 int *emf_offset, float *j, int *j_size, int *j_offset,
+                                 double *dx, double *dt, double *ene);
+void hip_vadvance_deposit_3d_s1_(int *ix, float *x, float *u, float *q, int *i0, int *i1,
+                                 float *rqm, float *efield, float *bfield, int *emf_size,
+                                 int *emf_offset, float *j, int *j_size, int *j_offset,
+                                 double *dx, double *dt, double *ene);
+void hip_vadvance_deposit_3d_s2_(int *ix, float *x, float *u, float *q, int *i0, int *i1,
+                                 float *rqm, float *efield, float *bfield, int *emf_size,
+                                 int *emf_offset, float *j, int *j_size, int *j_offset,
+                                 double *dx, double *dt, double *ene);
+void hip_vadvance_deposit_3d_s3_(int *ix, float *x, float *u, float *q, int *i0, int *i1,
+                                 float *rqm, float *efield, float *bfield, int *emf_size,
+                                 int *emf_offset, float *j, int *j_size, int *j_offset,
+                                 double *dx, double *dt, double *ene);
+
+/* ---------------------------------------------------------------------------------
+ * 2. Stateless Yee advance, periodic boundaries (os-emf-solver-yee.f03:542-568):
+ * B += -dt/2 curl E, E += dt curl B - dt J, B += -dt/2 curl E, guard cells refreshed
+ * with the periodic images after each sub-step (os-emf-boundary.f90:785-786).
+ * e, b, jay share the extents ld[] (cells incl. guards) and lower guards gcl[]; nx[]
+ * are the interior cells.  jay must already be folded (jay%update_boundary).
+ * ------------------------------------------------------------------------------- */
+int osr_yee_advance(int dims, const int *nx, const int *ld, const int *gcl, const double *dx,
+                    double dt, float *e, float *b, const float *jay);
+
+/* ---------------------------------------------------------------------------------
+ * 3. Resident simulation
+ * ------------------------------------------------------------------------------- */
+typedef struct osr_sim osr_sim;
+
+typedef struct {
+  int dims;       /* 2 or 3 */
+  int nx[3];      /* interior cells per dimension (unused entries ignored) */
+  double dx[3];   /* cell size */
+  double dt;      /* time step (Courant condition of the Yee scheme is checked) */
+  int order;      /* interpolation / deposit order 1..3 */
+  int tile[3];    /* tile size in cells; 0 = default (2D 16x16, 3D 8x8x4) */
+  int reserved[4];
+} osr_sim_desc;
+
+int osr_sim_create(const osr_sim_desc *desc, osr_sim **out);
+int osr_sim_destroy(osr_sim *sim);
+
+/* Adds a particle species (rqm = charge/mass ratio, pusher OSR_BORIS/OSR_VAY);
+ * returns its id in *sid.  At most 8 species. */
+int osr_sim_add_species(osr_sim *sim, float rqm, int pusher, int *sid);
+
+/* Replaces the particles of a species with np particles in the reference layout.
+ * Positions must satisfy -0.5 <= x < 0.5 and 1 <= ix <= nx. */
+int osr_species_set(osr_sim *sim, int sid, int64_t np, const int *ix, const float *x,
+                    const float *u, const float *q);
+/* Number of particles of a species. */
+int osr_species_count(osr_sim *sim, int sid, int64_t *np);
+/* Copies the particles back in the order they were set (or generated). */
+int osr_species_get(osr_sim *sim, int sid, int *ix, float *x, float *u, float *q);
+
+/* Generates particles on the device: ppc[d] particles per cell along each dimension
+ * (regular lattice as spec/psource/os-psource-std.f03:822-869 when random_pos == 0,
+ * uniform random in the cell otherwise), density n (q = sign(rqm) * n / ppc), momenta
+ * u = ufl + uth * N(0,1) per component, counter-based RNG seeded with seed. */
+int osr_species_generate(osr_sim *sim, int sid, const int *ppc, double density,
+                         const double *ufl, const double *uth, uint64_t seed, int random_pos);
+
+/* Fields in the reference layout with ld[] extents and gcl[] lower guard cells; only the
+ * interior cells are read (set) or written (get; guards filled with periodic images). */
+int osr_fields_set(osr_sim *sim, const float *e, const float *b, const int *ld,
+                   const int *gcl);
+int osr_fields_get(osr_sim *sim, float *e, float *b, const int *ld, const int *gcl);
+/* Uniform initial fields (e.g. type_init_b = "uniform", os-emf-gridval.f03). */
+int osr_fields_fill(osr_sim *sim, const double *e0, const double *b0);
+
+/* Advances nsteps full time steps (blocking). */
+int osr_sim_run(osr_sim *sim, int nsteps);
+/* Completed time steps. */
+int osr_sim_steps(osr_sim *sim, int64_t *steps);
+/* Energy history rows first..first+count-1: fld[6*count] = (B1,B2,B3,E1,E2,E3) energies
+ * of the fields the particles were pushed with, par[nspecies*count] = per-species
+ * time-centred kinetic energy, normalised as HIST/fld_ene and HIST/par_ene
+ * (os-emf-diagnostics.f03:44-55, os-spec-define.f03:1030-1086). */
+int osr_sim_history(osr_sim *sim, int64_t first, int64_t count, double *fld, double *par);
+/* Gauss-law residual div E - rho (FP64) on the interior cell corners; res may be NULL.
+ * *maxabs receives max |res|. */
+int osr_sim_gauss(osr_sim *sim, double *res, double *maxabs);
+/* Particle re-layouts done because a tile ran out of capacity. */
+int osr_sim_relayouts(osr_sim *sim, int64_t *count);
+
+/* Per-kernel timing of nsteps steps, measured with HIP events on the simulation's
+ * stream (the steps are really advanced).  ms[k] = mean duration per step of kernel
+ * class k: 0 push+deposit, 1 tile count, 2 tile fill, 3 Yee (3 sub-steps),
+ * 4 energy finalisation; ms[5] = mean wall time per step between events. */
+int osr_sim_time_kernels(osr_sim *sim, int nsteps, double *ms);
+/* Bytes of device memory held by a simulation. */
+int osr_sim_footprint(osr_sim *sim, int64_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OSIRIS_AMD_H */

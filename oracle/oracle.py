@@ -1,0 +1,405 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's cpu_baseline leg use this
+module, and only as the checker.  It wraps two libraries built by ``oracle/Makefile``:
+
+* ``lib/liboracle_f64.so`` / ``lib/liboracle_f32.so`` -- the C restatement of the
+  reference Fortran hot path (pic_oracle.c), in FP64 and FP32;
+* ``_ref/libosiris_simd_{s,d}.so`` -- the reference's own AVX2 fused push+deposit
+  kernels (simd/os-spec-push-avx2{,d}.c), compiled from the reference sources.
+
+Array conventions follow the reference: grids are ``(3, nx+gc, ny+gc[, nz+gc])`` in
+Fortran order (component fastest), stored here as numpy arrays of shape
+``(ld_z, ld_y, ld_x, 3)`` (C order, same memory); particle arrays are ``ix (N, D)``
+int32 with 1-based cell indices, ``x (N, D)``, ``u (N, 3)``, ``q (N,)``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBS: dict = {}
+
+
+class _Grid(C.Structure):
+    _fields_ = [("dims", C.c_int), ("nx", C.c_int * 3), ("gcl", C.c_int * 3),
+                ("gcu", C.c_int * 3)]
+
+
+def _lib(prec: str):
+    """Load the oracle restatement for 'f64' or 'f32'."""
+    if prec not in _LIBS:
+        path = os.path.join(_HERE, "lib", f"liboracle_{prec}.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle`")
+        L = C.CDLL(path)
+        P = C.c_void_p
+        L.orc_advance_deposit.restype = C.c_double
+        real = C.c_double if prec == "f64" else C.c_float
+        L.orc_advance_deposit.argtypes = [P, C.c_int, C.c_int, real, P, C.c_double, P, P, P,
+                                          P, P, P, P, C.c_int]
+        L.orc_dudt_boris.restype = C.c_double
+        L.orc_dudt_boris.argtypes = [real, C.c_double, P, P, P, P, C.c_int]
+        L.orc_dudt_vay.restype = C.c_double
+        L.orc_dudt_vay.argtypes = [real, C.c_double, P, P, P, P, C.c_int]
+        L.orc_get_emf.argtypes = [P, C.c_int, P, P, P, P, C.c_int, P, P]
+        L.orc_spline.argtypes = [C.c_int, real, P]
+        L.orc_splineh.argtypes = [C.c_int, real, C.c_int, P]
+        L.orc_wl.argtypes = [C.c_int, real, real, real, P]
+        L.orc_ntrim.argtypes = [real]
+        L.orc_split.argtypes = [C.c_int, C.c_int, P, P, P, P, P, P, P, P, P]
+        L.orc_split.restype = C.c_int
+        L.orc_update_periodic.argtypes = [P, P, C.c_int, C.c_int]
+        L.orc_particles_periodic.argtypes = [P, P, C.c_int]
+        L.orc_yee_advance.argtypes = [P, P, C.c_double, P, P, P]
+        L.orc_field_energy.argtypes = [P, P, P, C.c_double, P]
+        L.orc_kinetic_energy.restype = C.c_double
+        L.orc_kinetic_energy.argtypes = [P, P, C.c_int]
+        L.orc_deposit_rho.argtypes = [P, C.c_int, P, P, P, C.c_int, P]
+        L.orc_div.argtypes = [P, P, P, P]
+        L.orc_step.argtypes = [P, C.c_int, P, C.c_double, P, P, P, C.c_int, P, P, P, P, P, P,
+                               P, P, P]
+        L._real = np.float64 if prec == "f64" else np.float32
+        _LIBS[prec] = L
+    return _LIBS[prec]
+
+
+def ref_available(prec: str = "s") -> bool:
+    return os.path.exists(os.path.join(_HERE, "_ref", f"libosiris_simd_{prec}.so")) and \
+        _cpu_has_avx2()
+
+
+def _cpu_has_avx2() -> bool:
+    try:
+        with open("/proc/cpuinfo") as f:
+            flags = f.read()
+        return " avx2" in flags and " fma" in flags
+    except OSError:
+        return False
+
+
+def _ref(prec: str):
+    key = "ref_" + prec
+    if key not in _LIBS:
+        path = os.path.join(_HERE, "_ref", f"libosiris_simd_{prec}.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle ref` where the "
+                                    "reference sources are present")
+        _LIBS[key] = C.CDLL(path)
+    return _LIBS[key]
+
+
+def _aligned(shape, dtype, fill=0, align=64):
+    """Zero/fill-initialised array whose data pointer is ``align``-byte aligned (the
+    reference SIMD kernels use aligned AVX loads and stores)."""
+    dtype = np.dtype(dtype)
+    nbytes = int(np.prod(shape)) * dtype.itemsize
+    raw = np.empty(nbytes + align, np.uint8)
+    off = (-raw.ctypes.data) % align
+    a = raw[off:off + nbytes].view(dtype).reshape(shape)
+    a[...] = fill
+    return a
+
+
+def _p(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"], "arrays must be C-contiguous"
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# --------------------------------------------------------------------------------------
+# Grid helpers (reference layout)
+# --------------------------------------------------------------------------------------
+@dataclass
+class Grid:
+    dims: int
+    nx: tuple
+    gcl: tuple
+    gcu: tuple
+
+    def c(self) -> _Grid:
+        g = _Grid()
+        g.dims = self.dims
+        for d in range(3):
+            g.nx[d] = self.nx[d] if d < self.dims else 1
+            g.gcl[d] = self.gcl[d] if d < self.dims else 0
+            g.gcu[d] = self.gcu[d] if d < self.dims else 0
+        return g
+
+    @property
+    def ld(self):
+        return tuple(self.nx[d] + self.gcl[d] + self.gcu[d] for d in range(self.dims))
+
+    def zeros(self, nc=3, dtype=np.float64):
+        return np.zeros(tuple(reversed(self.ld)) + (nc,), dtype=dtype)
+
+    def interior(self, f):
+        """View of the interior cells [1..nx] of a grid array."""
+        sl = tuple(slice(self.gcl[d], self.gcl[d] + self.nx[d]) for d in reversed(range(self.dims)))
+        return f[sl]
+
+
+def guard_cells(order: int):
+    """Guard cells (lower, upper) required by interpolation and deposit of a given order:
+    particles/os-particles.f03:321-360 (linear (1,2), quadratic (2,3), cubic (3,4))."""
+    return {1: (1, 2), 2: (2, 3), 3: (3, 4)}[order]
+
+
+def make_grid(dims, nx, order):
+    lo, hi = guard_cells(order)
+    return Grid(dims, tuple(nx), (lo,) * dims, (hi,) * dims)
+
+
+# --------------------------------------------------------------------------------------
+# Elementary pieces
+# --------------------------------------------------------------------------------------
+def spline(order, x, prec="f64"):
+    L = _lib(prec)
+    s = np.zeros(4, L._real)
+    L.orc_spline(order, x, _p(s))
+    return s[:order + 1]
+
+
+def splineh(order, x, h, prec="f64"):
+    L = _lib(prec)
+    s = np.zeros(4, L._real)
+    L.orc_splineh(order, x, h, _p(s))
+    return s[:order + 1]
+
+
+def wl(order, qn, x0, x1, prec="f64"):
+    L = _lib(prec)
+    s = np.zeros(4, L._real)
+    L.orc_wl(order, qn, x0, x1, _p(s))
+    return s[:order + 1]
+
+
+def ntrim(x, prec="f64"):
+    return _lib(prec).orc_ntrim(x)
+
+
+def split(dims, dxi, xnew, ixold, xold, q, rgamma, u, prec="f64"):
+    L = _lib(prec)
+    r = L._real
+    n = len(q)
+    dxi = np.ascontiguousarray(dxi, np.int32)
+    ixold = np.ascontiguousarray(ixold, np.int32)
+    xnew, xold = (np.ascontiguousarray(a, r) for a in (xnew, xold))
+    q, rgamma, u = (np.ascontiguousarray(a, r) for a in (q, rgamma, u))
+    nf = 6 if dims == 2 else 7
+    vpf = np.zeros((4 * max(n, 1), nf), r)
+    vpi = np.zeros((4 * max(n, 1), dims), np.int32)
+    ns = L.orc_split(dims, n, _p(dxi), _p(xnew), _p(ixold), _p(xold), _p(q), _p(rgamma), _p(u),
+                     _p(vpf), _p(vpi))
+    return vpf[:ns], vpi[:ns]
+
+
+def get_emf(grid: Grid, order, e, b, ix, x, prec="f64"):
+    L = _lib(prec)
+    n = len(ix)
+    ep = np.zeros((n, 3), L._real)
+    bp = np.zeros((n, 3), L._real)
+    g = grid.c()
+    L.orc_get_emf(C.byref(g), order, _p(e), _p(b), _p(ix), _p(x), n, _p(ep), _p(bp))
+    return ep, bp
+
+
+def dudt(pusher, rqm, dt, u, q, ep, bp, prec="f64"):
+    """Momentum update only; returns the un-normalised energy sum. u is updated in place."""
+    L = _lib(prec)
+    f = L.orc_dudt_boris if pusher == 0 else L.orc_dudt_vay
+    ep = np.array(ep, L._real)
+    bp = np.array(bp, L._real)
+    return f(rqm, dt, _p(u), _p(q), _p(ep), _p(bp), len(q))
+
+
+def advance_deposit(grid: Grid, order, pusher, rqm, dx, dt, e, b, j, ix, x, u, q, prec="f64"):
+    """Fortran advance_deposit_{2d,3d}: updates ix/x/u in place, accumulates j."""
+    L = _lib(prec)
+    g = grid.c()
+    dxa = np.array(list(dx) + [1.0] * (3 - len(dx)), np.float64)
+    return L.orc_advance_deposit(C.byref(g), order, pusher, rqm, _p(dxa), dt, _p(e), _p(b),
+                                 _p(j), _p(ix), _p(x), _p(u), _p(q), len(q))
+
+
+def update_periodic(grid: Grid, f, mode, prec="f64"):
+    L = _lib(prec)
+    g = grid.c()
+    L.orc_update_periodic(C.byref(g), _p(f), f.shape[-1], mode)
+
+
+def particles_periodic(grid: Grid, ix, prec="f64"):
+    L = _lib(prec)
+    g = grid.c()
+    L.orc_particles_periodic(C.byref(g), _p(ix), len(ix))
+
+
+def yee_advance(grid: Grid, dx, dt, e, b, j, prec="f64"):
+    L = _lib(prec)
+    g = grid.c()
+    dxa = np.array(list(dx) + [1.0] * (3 - len(dx)), np.float64)
+    L.orc_yee_advance(C.byref(g), _p(dxa), dt, _p(e), _p(b), _p(j))
+
+
+def field_energy(grid: Grid, e, b, dvol, prec="f64"):
+    L = _lib(prec)
+    out = np.zeros(6)
+    g = grid.c()
+    L.orc_field_energy(C.byref(g), _p(e), _p(b), dvol, _p(out))
+    return out
+
+
+def kinetic_energy(u, q, prec="f64"):
+    return _lib(prec).orc_kinetic_energy(_p(u), _p(q), len(q))
+
+
+def deposit_rho(grid: Grid, order, ix, x, q, prec="f64"):
+    L = _lib(prec)
+    rho = grid.zeros(1, L._real)
+    g = grid.c()
+    L.orc_deposit_rho(C.byref(g), order, _p(ix), _p(x), _p(q), len(q), _p(rho))
+    return rho
+
+
+def div(grid: Grid, dx, f, prec="f64"):
+    L = _lib(prec)
+    out = grid.zeros(1, L._real)
+    g = grid.c()
+    dxa = np.array(list(dx) + [1.0] * (3 - len(dx)), np.float64)
+    L.orc_div(C.byref(g), _p(dxa), _p(f), _p(out))
+    return out
+
+
+# --------------------------------------------------------------------------------------
+# Reference AVX2 kernels (simd/os-spec-push-avx2{,d}.c:471-760 and 3D variant)
+# --------------------------------------------------------------------------------------
+def _copy_aligned(a, dtype):
+    out = _aligned(a.shape, dtype)
+    out[...] = a
+    return out
+
+
+def ref_vadvance_deposit(dims, order, ix, x, u, q, rqm, e, b, emf_lo, j, j_lo, dx, dt,
+                         prec="s"):
+    """Call FNAME(vadvance_deposit_{2d,3d}_s<order>) of the reference on copies.
+
+    ``e``/``b``/``j`` are grids in the reference layout with ``emf_lo``/``j_lo`` lower guard
+    cells.  Particle arrays are padded to a multiple of the SIMD width (8 single / 4
+    double) because the reference writes dummy particles past i1
+    (os-spec-push-avx2.c:523-542).  Returns (ix, x, u, j, ene)."""
+    L = _ref(prec)
+    real = np.float32 if prec == "s" else np.float64
+    fn = getattr(L, f"vadvance_deposit_{dims}d_s{order}_")
+    n = len(q)
+    pad = n + 16
+    ixp = _aligned((pad, dims), np.int32, 1)
+    ixp[:n] = ix
+    xp = _aligned((pad, dims), real)
+    xp[:n] = x
+    up = _aligned((pad, 3), real)
+    up[:n] = u
+    qp = _aligned((pad,), real)
+    qp[:n] = q
+    e = _copy_aligned(e, real)
+    b = _copy_aligned(b, real)
+    jj = _copy_aligned(j, real)
+    emf_size = np.array(list(reversed(e.shape[:-1])), np.int32)
+    j_size = np.array(list(reversed(jj.shape[:-1])), np.int32)
+    emf_off = np.array([emf_lo] * dims, np.int32)
+    j_off = np.array([j_lo] * dims, np.int32)
+    i0 = C.c_int(1)
+    i1 = C.c_int(n)
+    rq = C.c_float(rqm) if prec == "s" else C.c_double(rqm)
+    dxa = np.array(list(dx), np.float64)
+    dtc = C.c_double(dt)
+    ene = C.c_double(0.0)
+    fn(_p(ixp), _p(xp), _p(up), _p(qp), C.byref(i0), C.byref(i1), C.byref(rq), _p(e), _p(b),
+       _p(emf_size), _p(emf_off), _p(jj), _p(j_size), _p(j_off), _p(dxa), C.byref(dtc),
+       C.byref(ene))
+    return ixp[:n].copy(), xp[:n].copy(), up[:n].copy(), jj, ene.value
+
+
+# --------------------------------------------------------------------------------------
+# Whole-simulation driver (os-simulation.f03:498-666 order, periodic)
+# --------------------------------------------------------------------------------------
+@dataclass
+class Species:
+    rqm: float
+    pusher: int  # 0 Boris, 1 Vay
+    ix: np.ndarray
+    x: np.ndarray
+    u: np.ndarray
+    q: np.ndarray
+
+
+@dataclass
+class Sim:
+    """FP64 (or FP32) oracle simulation in the reference layout."""
+    dims: int
+    nx: tuple
+    dx: tuple
+    dt: float
+    order: int
+    prec: str = "f64"
+    species: list = field(default_factory=list)
+
+    def __post_init__(self):
+        self.grid = make_grid(self.dims, self.nx, self.order)
+        r = _lib(self.prec)._real
+        self.e = self.grid.zeros(3, r)
+        self.b = self.grid.zeros(3, r)
+        self.j = self.grid.zeros(3, r)
+        self.fld_hist = []
+        self.par_hist = []
+
+    @property
+    def real(self):
+        return _lib(self.prec)._real
+
+    def add_species(self, rqm, pusher, ix, x, u, q):
+        r = self.real
+        self.species.append(Species(rqm, pusher, np.ascontiguousarray(ix, np.int32),
+                                    np.ascontiguousarray(x, r), np.ascontiguousarray(u, r),
+                                    np.ascontiguousarray(q, r)))
+
+    def set_interior(self, which, arr):
+        """Set interior E or B from an array of shape (nz, ny, nx, 3) (C order)."""
+        f = self.e if which == "e" else self.b
+        self.grid.interior(f)[...] = arr
+
+    def step(self, n=1):
+        L = _lib(self.prec)
+        g = self.grid.c()
+        ns = len(self.species)
+        dxa = np.array(list(self.dx) + [1.0] * (3 - self.dims), np.float64)
+        IA = C.c_void_p * ns
+        pus = np.array([s.pusher for s in self.species], np.int32)
+        rqm = np.array([s.rqm for s in self.species], self.real)
+        npa = np.array([len(s.q) for s in self.species], np.int32)
+        ixs = IA(*[s.ix.ctypes.data for s in self.species])
+        xs = IA(*[s.x.ctypes.data for s in self.species])
+        us = IA(*[s.u.ctypes.data for s in self.species])
+        qs = IA(*[s.q.ctypes.data for s in self.species])
+        for _ in range(n):
+            fe = np.zeros(6)
+            pe = np.zeros(ns)
+            L.orc_step(C.byref(g), self.order, _p(dxa), self.dt, _p(self.e), _p(self.b),
+                       _p(self.j), ns, _p(pus), _p(rqm), ixs, xs, us, qs, _p(npa), _p(fe),
+                       _p(pe))
+            self.fld_hist.append(fe)
+            self.par_hist.append(pe)
+
+    def gauss_residual(self):
+        """div E - rho (interior nodes) with rho from all species (no ion background)."""
+        rho = self.grid.zeros(1, self.real)
+        for s in self.species:
+            rho += deposit_rho(self.grid, self.order, s.ix, s.x, s.q, self.prec)
+        update_periodic(self.grid, rho, 1, self.prec)
+        e = self.e.copy()
+        update_periodic(self.grid, e, 0, self.prec)
+        dv = div(self.grid, self.dx, e, self.prec)
+        return self.grid.interior(dv)[..., 0].astype(np.float64) - \
+            self.grid.interior(rho)[..., 0].astype(np.float64)

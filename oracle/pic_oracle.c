@@ -1,0 +1,1036 @@
+/*
+ * pic_oracle.c -- CPU restatement of the OSIRIS per-timestep PIC hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP path: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The product
+ * library never links it and never falls back to it.
+ *
+ * It restates the reference Fortran algorithms in plain C, operation by operation, so
+ * that a build with REAL=double reproduces the FP64 reference build and a build with
+ * REAL=float (and -ffp-contract=off) reproduces the Fortran single precision build.
+ * Every function cites the reference file:line it follows (paths relative to
+ * source/ of osiris-code/osiris).
+ *
+ * Conventions (identical to the reference):
+ *   - grids are stored like t_vdf%f2 / f3: component fastest, then x, y, z, with
+ *     Fortran indices running from 1-gcl to nx+gcu (vdf/os-vdf-define.f03:35-116);
+ *   - particle cell indices ix(d,N) are 1-based Fortran indices, positions x(d,N) lie in
+ *     [-0.5,0.5) relative to the cell, momenta u(3,N) are proper velocities, q(N) charges
+ *     (spec/os-spec-define.f03:391-412).
+ *
+ * Pinning (see the tests): splines/wl against closed forms and partition of unity; the
+ * fused Boris advance/deposit against the reference's own AVX2 kernels compiled from
+ * simd/ (oracle/_ref) on identical inputs; deposit against discrete continuity with the
+ * independent rho deposit; Vay against the exact ExB drift; Yee against the closed-form
+ * numerical dispersion of a vacuum plane wave and energy conservation.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef REAL
+#define REAL double
+#endif
+typedef REAL real;
+#define R(v) ((real)(v))
+
+/* os-param.f90:227 */
+#define P_CACHE_SIZE 1024
+
+typedef struct {
+  int dims;   /* 2 or 3 */
+  int nx[3];  /* interior cells */
+  int gcl[3]; /* lower guard cells */
+  int gcu[3]; /* upper guard cells */
+} orc_grid;
+
+static inline long ld_(const orc_grid *g, int d) {
+  return (long)g->nx[d] + g->gcl[d] + g->gcu[d];
+}
+
+/* linear cell offset of Fortran index (i,j,k) in a t_vdf with 1-gcl lower bound */
+static inline long cidx(const orc_grid *g, int i, int j, int k) {
+  long o = (long)(i - 1 + g->gcl[0]);
+  if (g->dims > 1) o += ld_(g, 0) * (long)(j - 1 + g->gcl[1]);
+  if (g->dims > 2) o += ld_(g, 0) * ld_(g, 1) * (long)(k - 1 + g->gcl[2]);
+  return o;
+}
+#define FLD(f, g, c, i, j, k) ((f)[3 * cidx((g), (i), (j), (k)) + (c)])
+
+long orc_grid_cells(const orc_grid *g) {
+  long n = ld_(g, 0);
+  if (g->dims > 1) n *= ld_(g, 1);
+  if (g->dims > 2) n *= ld_(g, 2);
+  return n;
+}
+
+int orc_real_size(void) { return (int)sizeof(real); }
+
+/* ------------------------------------------------------------------------------------
+ * Splines.  emf/os-emf-interpolate.f03:171-293 (spline_s1..s3, splineh_s1..s3) and
+ * spec/os-spec-current.f03:488-550 (same formulas).  s[k] holds the weight of stencil
+ * point LP+k, LP = 0/-1/-1 for orders 1/2/3 (os-emf-interpolate.f03:385-461).
+ * ------------------------------------------------------------------------------------ */
+static inline int lp_of(int order) { return order == 1 ? 0 : -1; }
+
+void orc_spline(int order, real x, real *s) {
+  switch (order) {
+  case 1:
+    s[0] = R(0.5) - x;
+    s[1] = R(0.5) + x;
+    break;
+  case 2: {
+    real t0 = R(0.5) - x, t1 = R(0.5) + x;
+    s[0] = R(0.5) * (t0 * t0);
+    s[1] = R(0.5) + t0 * t1;
+    s[2] = R(0.5) * (t1 * t1);
+  } break;
+  case 3: {
+    const real c1_6 = R(1.0) / R(6.0), c2_3 = R(2.0) / R(3.0);
+    real t0 = R(0.5) - x, t1 = R(0.5) + x;
+    real t2 = t0 * t0, t3 = t1 * t1;
+    t0 = t0 * t2;
+    t1 = t1 * t3;
+    s[0] = c1_6 * t0;
+    s[1] = (c2_3 - t3) + R(0.5) * t1;
+    s[2] = (c2_3 - t2) + R(0.5) * t0;
+    s[3] = c1_6 * t1;
+  } break;
+  }
+}
+
+/* half-cell shifted spline, os-emf-interpolate.f03:183-193, 215-232, 264-292 */
+void orc_splineh(int order, real x, int h, real *s) {
+  real t0 = (real)(1 - h) - x, t1 = (real)h + x;
+  switch (order) {
+  case 1:
+    s[0] = t0;
+    s[1] = t1;
+    break;
+  case 2:
+    s[0] = R(0.5) * (t0 * t0);
+    s[1] = R(0.5) + t0 * t1;
+    s[2] = R(0.5) * (t1 * t1);
+    break;
+  case 3: {
+    const real c1_6 = R(1.0) / R(6.0), c2_3 = R(2.0) / R(3.0);
+    real t2 = t0 * t0, t3 = t1 * t1;
+    t0 = t0 * t2;
+    t1 = t1 * t3;
+    s[0] = c1_6 * t0;
+    s[1] = (c2_3 - t3) + R(0.5) * t1;
+    s[2] = (c2_3 - t2) + R(0.5) * t0;
+    s[3] = c1_6 * t1;
+  } break;
+  }
+}
+
+/* Longitudinal current weights, spec/os-spec-current.f03:383-441 (wl_s1..wl_s3).
+ * wl[k] is the weight of point P0+k; the entry P1 is zeroed as in DEP_CURRENT
+ * (os-spec-current.f03:2752-2753). */
+void orc_wl(int order, real qn, real x0, real x1, real *wl) {
+  switch (order) {
+  case 1:
+    wl[0] = qn * (x1 - x0);
+    wl[1] = 0;
+    break;
+  case 2: {
+    real d = x1 - x0, s1_2 = R(0.5) - x0, p1_2 = R(0.5) + x0;
+    wl[0] = s1_2 - R(0.5) * d;
+    wl[1] = p1_2 + R(0.5) * d;
+    real n = qn * d;
+    wl[0] = n * wl[0];
+    wl[1] = n * wl[1];
+    wl[2] = 0;
+  } break;
+  case 3: {
+    const real c1_3 = R(1.0) / R(3.0);
+    real d = x1 - x0, s1_2 = R(0.5) - x0, p1_2 = R(0.5) + x0, d_3 = c1_3 * d;
+    wl[0] = R(0.5) * (s1_2 * s1_2 - d * (s1_2 - d_3));
+    wl[1] = (R(0.75) - x0 * x0) - d * (x0 + d_3);
+    wl[2] = R(0.5) * (p1_2 * p1_2 + d * (p1_2 + d_3));
+    real n = qn * d;
+    wl[0] = n * wl[0];
+    wl[1] = n * wl[1];
+    wl[2] = n * wl[2];
+    wl[3] = 0;
+  } break;
+  }
+}
+
+/* ------------------------------------------------------------------------------------
+ * Field gather.  emf/os-emf-interpolate.f03:554-648 (GET_EMF_2D), 652-779 (GET_EMF_3D);
+ * signbit :358-371.
+ * ------------------------------------------------------------------------------------ */
+static void get_emf_2d(const orc_grid *g, int order, const real *e, const real *b,
+                       const int *ix, const real *x, int np, real *ep, real *bp) {
+  const int n = order + 1, lp = lp_of(order);
+  real w1[4], w1h[4], w2[4], w2h[4];
+  for (int l = 0; l < np; l++) {
+    int i1 = ix[2 * l], i2 = ix[2 * l + 1];
+    real dx1 = x[2 * l], dx2 = x[2 * l + 1];
+    int h1 = dx1 < 0 ? 1 : 0, h2 = dx2 < 0 ? 1 : 0;
+    int i1h = i1 - h1, i2h = i2 - h2;
+    orc_spline(order, dx1, w1);
+    orc_splineh(order, dx1, h1, w1h);
+    orc_spline(order, dx2, w2);
+    orc_splineh(order, dx2, h2, w2h);
+    real f1 = 0, f2 = 0, f3 = 0;
+    for (int k2 = 0; k2 < n; k2++) {
+      real l1 = 0, l2 = 0, l3 = 0;
+      for (int k1 = 0; k1 < n; k1++) {
+        l1 = l1 + FLD(e, g, 0, i1h + lp + k1, i2 + lp + k2, 1) * w1h[k1];
+        l2 = l2 + FLD(e, g, 1, i1 + lp + k1, i2h + lp + k2, 1) * w1[k1];
+        l3 = l3 + FLD(e, g, 2, i1 + lp + k1, i2 + lp + k2, 1) * w1[k1];
+      }
+      f1 = f1 + l1 * w2[k2];
+      f2 = f2 + l2 * w2h[k2];
+      f3 = f3 + l3 * w2[k2];
+    }
+    ep[3 * l] = f1;
+    ep[3 * l + 1] = f2;
+    ep[3 * l + 2] = f3;
+    f1 = f2 = f3 = 0;
+    for (int k2 = 0; k2 < n; k2++) {
+      real l1 = 0, l2 = 0, l3 = 0;
+      for (int k1 = 0; k1 < n; k1++) {
+        l1 = l1 + FLD(b, g, 0, i1 + lp + k1, i2h + lp + k2, 1) * w1[k1];
+        l2 = l2 + FLD(b, g, 1, i1h + lp + k1, i2 + lp + k2, 1) * w1h[k1];
+        l3 = l3 + FLD(b, g, 2, i1h + lp + k1, i2h + lp + k2, 1) * w1h[k1];
+      }
+      f1 = f1 + l1 * w2h[k2];
+      f2 = f2 + l2 * w2[k2];
+      f3 = f3 + l3 * w2h[k2];
+    }
+    bp[3 * l] = f1;
+    bp[3 * l + 1] = f2;
+    bp[3 * l + 2] = f3;
+  }
+}
+
+static void get_emf_3d(const orc_grid *g, int order, const real *e, const real *b,
+                       const int *ix, const real *x, int np, real *ep, real *bp) {
+  const int n = order + 1, lp = lp_of(order);
+  real w1[4], w1h[4], w2[4], w2h[4], w3[4], w3h[4];
+  for (int l = 0; l < np; l++) {
+    int i1 = ix[3 * l], i2 = ix[3 * l + 1], i3 = ix[3 * l + 2];
+    real dx1 = x[3 * l], dx2 = x[3 * l + 1], dx3 = x[3 * l + 2];
+    int h1 = dx1 < 0, h2 = dx2 < 0, h3 = dx3 < 0;
+    int i1h = i1 - h1, i2h = i2 - h2, i3h = i3 - h3;
+    orc_spline(order, dx1, w1);
+    orc_splineh(order, dx1, h1, w1h);
+    orc_spline(order, dx2, w2);
+    orc_splineh(order, dx2, h2, w2h);
+    orc_spline(order, dx3, w3);
+    orc_splineh(order, dx3, h3, w3h);
+    real f1 = 0, f2 = 0, f3 = 0;
+    for (int k3 = 0; k3 < n; k3++) {
+      real p1 = 0, p2 = 0, p3 = 0;
+      for (int k2 = 0; k2 < n; k2++) {
+        real l1 = 0, l2 = 0, l3 = 0;
+        for (int k1 = 0; k1 < n; k1++) {
+          l1 = l1 + FLD(e, g, 0, i1h + lp + k1, i2 + lp + k2, i3 + lp + k3) * w1h[k1];
+          l2 = l2 + FLD(e, g, 1, i1 + lp + k1, i2h + lp + k2, i3 + lp + k3) * w1[k1];
+          l3 = l3 + FLD(e, g, 2, i1 + lp + k1, i2 + lp + k2, i3h + lp + k3) * w1[k1];
+        }
+        p1 = p1 + l1 * w2[k2];
+        p2 = p2 + l2 * w2h[k2];
+        p3 = p3 + l3 * w2[k2];
+      }
+      f1 = f1 + p1 * w3[k3];
+      f2 = f2 + p2 * w3[k3];
+      f3 = f3 + p3 * w3h[k3];
+    }
+    ep[3 * l] = f1;
+    ep[3 * l + 1] = f2;
+    ep[3 * l + 2] = f3;
+    f1 = f2 = f3 = 0;
+    for (int k3 = 0; k3 < n; k3++) {
+      real p1 = 0, p2 = 0, p3 = 0;
+      for (int k2 = 0; k2 < n; k2++) {
+        real l1 = 0, l2 = 0, l3 = 0;
+        for (int k1 = 0; k1 < n; k1++) {
+          l1 = l1 + FLD(b, g, 0, i1 + lp + k1, i2h + lp + k2, i3h + lp + k3) * w1[k1];
+          l2 = l2 + FLD(b, g, 1, i1h + lp + k1, i2 + lp + k2, i3h + lp + k3) * w1h[k1];
+          l3 = l3 + FLD(b, g, 2, i1h + lp + k1, i2h + lp + k2, i3 + lp + k3) * w1h[k1];
+        }
+        p1 = p1 + l1 * w2h[k2];
+        p2 = p2 + l2 * w2[k2];
+        p3 = p3 + l3 * w2h[k2];
+      }
+      f1 = f1 + p1 * w3h[k3];
+      f2 = f2 + p2 * w3h[k3];
+      f3 = f3 + p3 * w3[k3];
+    }
+    bp[3 * l] = f1;
+    bp[3 * l + 1] = f2;
+    bp[3 * l + 2] = f3;
+  }
+}
+
+void orc_get_emf(const orc_grid *g, int order, const real *e, const real *b, const int *ix,
+                 const real *x, int np, real *ep, real *bp) {
+  if (g->dims == 2)
+    get_emf_2d(g, order, e, b, ix, x, np, ep, bp);
+  else
+    get_emf_3d(g, order, e, b, ix, x, np, ep, bp);
+}
+
+/* ------------------------------------------------------------------------------------
+ * Momentum pushers on pre-gathered fields.
+ * Boris: spec/os-spec-push.f03:356-632, non radiation-reaction path 466-574.
+ *   tem = real(0.5*(dt/rqm)) :396; energy accumulated per p_cache_size chunk :488-510.
+ * Vay:   spec/os-spec-push.f03:636-796.
+ * Both return the chunk-accumulated sum of q*u^2/(gamma+1) in double.
+ * ------------------------------------------------------------------------------------ */
+double orc_dudt_boris(real rqm, double dt, real *u, const real *q, real *ep, real *bp,
+                      int np) {
+  real tem = (real)(0.5 * (dt / (double)rqm));
+  double energy = 0;
+  for (int c0 = 0; c0 < np; c0 += P_CACHE_SIZE) {
+    int c1 = c0 + P_CACHE_SIZE < np ? c0 + P_CACHE_SIZE : np;
+    double loc_ene = 0;
+    for (int i = c0; i < c1; i++) {
+      real *p = u + 3 * i, *e = ep + 3 * i, *b = bp + 3 * i;
+      e[0] = e[0] * tem;
+      e[1] = e[1] * tem;
+      e[2] = e[2] * tem;
+      real ut1 = p[0] + e[0], ut2 = p[1] + e[1], ut3 = p[2] + e[2];
+      real u2 = (ut1 * ut1 + ut2 * ut2) + ut3 * ut3;
+      real gamma = (real)sqrt(u2 + 1);
+      loc_ene = loc_ene + (double)(q[i] * u2) / ((double)gamma + 1.0);
+      real gam_tem = tem / gamma;
+      b[0] = b[0] * gam_tem;
+      b[1] = b[1] * gam_tem;
+      b[2] = b[2] * gam_tem;
+      real p1 = ut1 + ut2 * b[2], p2 = ut2 + ut3 * b[0], p3 = ut3 + ut1 * b[1];
+      p1 = p1 - ut3 * b[1];
+      p2 = p2 - ut1 * b[2];
+      p3 = p3 - ut2 * b[0];
+      real otsq = R(2.0) / (((R(1.0) + b[0] * b[0]) + b[1] * b[1]) + b[2] * b[2]);
+      b[0] = b[0] * otsq;
+      b[1] = b[1] * otsq;
+      b[2] = b[2] * otsq;
+      ut1 = ut1 + p2 * b[2];
+      ut2 = ut2 + p3 * b[0];
+      ut3 = ut3 + p1 * b[1];
+      ut1 = ut1 - p3 * b[1];
+      ut2 = ut2 - p1 * b[2];
+      ut3 = ut3 - p2 * b[0];
+      p[0] = ut1 + e[0];
+      p[1] = ut2 + e[1];
+      p[2] = ut3 + e[2];
+    }
+    energy = energy + loc_ene;
+  }
+  return energy;
+}
+
+double orc_dudt_vay(real rqm, double dt, real *u, const real *q, real *ep, real *bp, int np) {
+  real tem = (real)(0.5 * dt / (double)rqm);
+  double energy = 0;
+  for (int c0 = 0; c0 < np; c0 += P_CACHE_SIZE) {
+    int c1 = c0 + P_CACHE_SIZE < np ? c0 + P_CACHE_SIZE : np;
+    double loc_ene = 0;
+    for (int i = c0; i < c1; i++) {
+      real *p = u + 3 * i, *e = ep + 3 * i, *b = bp + 3 * i;
+      e[0] *= tem;
+      e[1] *= tem;
+      e[2] *= tem;
+      b[0] *= tem;
+      b[1] *= tem;
+      b[2] *= tem;
+      real rgam = R(1.0) / (real)sqrt(R(1.0) + p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+      real v1 = p[0] * rgam, v2 = p[1] * rgam, v3 = p[2] * rgam;
+      real bpsq = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+      real ut1 = p[0] + e[0] + v2 * b[2] - v3 * b[1];
+      real ut2 = p[1] + e[1] + v3 * b[0] - v1 * b[2];
+      real ut3 = p[2] + e[2] + v1 * b[1] - v2 * b[0];
+      /* temp_vec = u' */
+      real w1 = ut1 + e[0], w2 = ut2 + e[1], w3 = ut3 + e[2];
+      real ustar = w1 * b[0] + w2 * b[1] + w3 * b[2];
+      real sigma = R(1.0) + w1 * w1 + w2 * w2 + w3 * w3 - bpsq;
+      rgam = R(1.0) /
+             (real)sqrt(R(0.5) * (sigma + (real)sqrt(sigma * sigma +
+                                                      R(4.0) * (bpsq + ustar * ustar))));
+      b[0] *= rgam;
+      b[1] *= rgam;
+      b[2] *= rgam;
+      real spar = R(1.0) / (R(1.0) + b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+      real uptp = w1 * b[0] + w2 * b[1] + w3 * b[2];
+      real o1 = p[0], o2 = p[1], o3 = p[2];
+      p[0] = spar * (w1 + uptp * b[0] + w2 * b[2] - w3 * b[1]);
+      p[1] = spar * (w2 + uptp * b[1] + w3 * b[0] - w1 * b[2]);
+      p[2] = spar * (w3 + uptp * b[2] + w1 * b[1] - w2 * b[0]);
+      real uc1 = R(0.5) * (o1 + p[0]), uc2 = R(0.5) * (o2 + p[1]), uc3 = R(0.5) * (o3 + p[2]);
+      real uc2s = uc1 * uc1 + uc2 * uc2 + uc3 * uc3;
+      real gamma_c = (real)sqrt(uc2s + 1);
+      loc_ene = loc_ene + (double)(q[i] * uc2s / (gamma_c + 1));
+    }
+    energy = energy + loc_ene;
+  }
+  return energy;
+}
+
+/* ------------------------------------------------------------------------------------
+ * ntrim, spec/os-spec-push.f03:1332-1359
+ * ------------------------------------------------------------------------------------ */
+static inline int ntrim(real x) { return (x < R(-0.5) ? -1 : 0) + (x >= R(0.5) ? 1 : 0); }
+int orc_ntrim(real x) { return ntrim(x); }
+
+/* ------------------------------------------------------------------------------------
+ * Virtual particles and trajectory splitting.
+ * t_vp2D / t_vp3D: spec/os-spec-current.f03:52-66
+ * split_2d: os-spec-current.f03:659-888;  split_3d: :895-1601
+ * ------------------------------------------------------------------------------------ */
+typedef struct {
+  real x0, y0, x1, y1, q, vz;
+  int i, j;
+} vp2d;
+typedef struct {
+  real x0, y0, z0, x1, y1, z1, q;
+  int i, j, k;
+} vp3d;
+
+static int split_2d(const int *dxi, const real *xnew, const int *ixold, const real *xold,
+                    const real *q, const real *rgamma, const real *u, int np, vp2d *vp) {
+  int k = 0;
+  for (int i = 0; i < np; i++) {
+    const int d1 = dxi[2 * i], d2 = dxi[2 * i + 1];
+    const real xo = xold[2 * i], yo = xold[2 * i + 1];
+    const real xn = xnew[2 * i], yn = xnew[2 * i + 1];
+    const int io = ixold[2 * i], jo = ixold[2 * i + 1];
+    const int cross = abs(d1) + 2 * abs(d2);
+    real xint, yint, xint2, yint2, delta, vz, vzint;
+    switch (cross) {
+    case 0:
+      vp[k] = (vp2d){xo, yo, xn, yn, q[i], u[3 * i + 2] * rgamma[i], io, jo};
+      k++;
+      break;
+    case 1:
+      xint = R(0.5) * d1;
+      delta = (xint - xo) / (xn - xo);
+      yint = yo + (yn - yo) * delta;
+      vz = u[3 * i + 2] * rgamma[i];
+      vp[k++] = (vp2d){xo, yo, xint, yint, q[i], vz * delta, io, jo};
+      vp[k++] = (vp2d){-xint, yint, xn - d1, yn, q[i], vz * (1 - delta), io + d1, jo};
+      break;
+    case 2:
+      yint = R(0.5) * d2;
+      delta = (yint - yo) / (yn - yo);
+      xint = xo + (xn - xo) * delta;
+      vz = u[3 * i + 2] * rgamma[i];
+      vp[k++] = (vp2d){xo, yo, xint, yint, q[i], vz * delta, io, jo};
+      vp[k++] = (vp2d){xint, -yint, xn, yn - d2, q[i], vz * (1 - delta), io, jo + d2};
+      break;
+    case 3:
+      xint = R(0.5) * d1;
+      delta = (xint - xo) / (xn - xo);
+      yint = yo + (yn - yo) * delta;
+      vz = u[3 * i + 2] * rgamma[i];
+      if ((yint >= R(-0.5)) && (yint < R(0.5))) {
+        vp[k++] = (vp2d){xo, yo, xint, yint, q[i], vz * delta, io, jo};
+        vzint = vz * (1 - delta);
+        yint2 = R(0.5) * d2;
+        delta = (yint2 - yint) / (yn - yint);
+        xint2 = -xint + (xn - xint) * delta;
+        vp[k++] = (vp2d){-xint, yint, xint2, yint2, q[i], vzint * delta, io + d1, jo};
+        vp[k++] = (vp2d){xint2, -yint2, xn - d1, yn - d2, q[i], vzint * (1 - delta), io + d1,
+                         jo + d2};
+      } else {
+        vzint = vz * delta;
+        yint2 = R(0.5) * d2;
+        delta = (yint2 - yo) / (yint - yo);
+        xint2 = xo + (xint - xo) * delta;
+        vp[k++] = (vp2d){xo, yo, xint2, yint2, q[i], vzint * delta, io, jo};
+        vp[k++] = (vp2d){xint2, -yint2, xint, yint - d2, q[i], vzint * (1 - delta), io, jo + d2};
+        vp[k++] = (vp2d){-xint, yint - d2, xn - d1, yn - d2, q[i], vz - vzint, io + d1, jo + d2};
+      }
+      break;
+    }
+  }
+  return k;
+}
+
+static inline vp3d mk3(real x0, real y0, real z0, real x1, real y1, real z1, real q, int i,
+                       int j, int k) {
+  vp3d v = {x0, y0, z0, x1, y1, z1, q, i, j, k};
+  return v;
+}
+
+static int split_3d(const int *dxi, const real *xnew, const int *ixold, const real *xold,
+                    const real *q, int np, vp3d *vp) {
+  int k = 0;
+  for (int i = 0; i < np; i++) {
+    const int d1 = dxi[3 * i], d2 = dxi[3 * i + 1], d3 = dxi[3 * i + 2];
+    const real xo = xold[3 * i], yo = xold[3 * i + 1], zo = xold[3 * i + 2];
+    const real xn = xnew[3 * i], yn = xnew[3 * i + 1], zn = xnew[3 * i + 2];
+    const int io = ixold[3 * i], jo = ixold[3 * i + 1], ko = ixold[3 * i + 2];
+    const real qq = q[i];
+    const int cross = abs(d1) + 2 * abs(d2) + 4 * abs(d3);
+    real xint, yint, zint, xint2, yint2, zint2, delta;
+    switch (cross) {
+    case 0:
+      vp[k++] = mk3(xo, yo, zo, xn, yn, zn, qq, io, jo, ko);
+      break;
+    case 1: /* :940-975 */
+      xint = R(0.5) * d1;
+      delta = (xint - xo) / (xn - xo);
+      yint = yo + (yn - yo) * delta;
+      zint = zo + (zn - zo) * delta;
+      vp[k++] = mk3(xo, yo, zo, xint, yint, zint, qq, io, jo, ko);
+      vp[k++] = mk3(-xint, yint, zint, xn - d1, yn, zn, qq, io + d1, jo, ko);
+      break;
+    case 2: /* :977-1012 */
+      yint = R(0.5) * d2;
+      delta = (yint - yo) / (yn - yo);
+      xint = xo + (xn - xo) * delta;
+      zint = zo + (zn - zo) * delta;
+      vp[k++] = mk3(xo, yo, zo, xint, yint, zint, qq, io, jo, ko);
+      vp[k++] = mk3(xint, -yint, zint, xn, yn - d2, zn, qq, io, jo + d2, ko);
+      break;
+    case 3: /* :1015-1133 */
+      xint = R(0.5) * d1;
+      delta = (xint - xo) / (xn - xo);
+      yint = yo + (yn - yo) * delta;
+      zint = zo + (zn - zo) * delta;
+      if ((yint >= R(-0.5)) && (yint < R(0.5))) {
+        vp[k++] = mk3(xo, yo, zo, xint, yint, zint, qq, io, jo, ko);
+        yint2 = R(0.5) * d2;
+        delta = (yint2 - yint) / (yn - yint);
+        xint2 = -xint + (xn - xint) * delta;
+        zint2 = zint + (zn - zint) * delta;
+        vp[k++] = mk3(-xint, yint, zint, xint2, yint2, zint2, qq, io + d1, jo, ko);
+        vp[k++] = mk3(xint2, -yint2, zint2, xn - d1, yn - d2, zn, qq, io + d1, jo + d2, ko);
+      } else {
+        yint2 = R(0.5) * d2;
+        delta = (yint2 - yo) / (yint - yo);
+        xint2 = xo + (xint - xo) * delta;
+        zint2 = zo + (zint - zo) * delta;
+        vp[k++] = mk3(xo, yo, zo, xint2, yint2, zint2, qq, io, jo, ko);
+        vp[k++] = mk3(xint2, -yint2, zint2, xint, yint - d2, zint, qq, io, jo + d2, ko);
+        vp[k++] = mk3(-xint, yint - d2, zint, xn - d1, yn - d2, zn, qq, io + d1, jo + d2, ko);
+      }
+      break;
+    case 4: /* :1135-1170 */
+      zint = R(0.5) * d3;
+      delta = (zint - zo) / (zn - zo);
+      xint = xo + (xn - xo) * delta;
+      yint = yo + (yn - yo) * delta;
+      vp[k++] = mk3(xo, yo, zo, xint, yint, zint, qq, io, jo, ko);
+      vp[k++] = mk3(xint, yint, -zint, xn, yn, zn - d3, qq, io, jo, ko + d3);
+      break;
+    case 5: /* :1173-1300 */
+      xint = R(0.5) * d1;
+      delta = (xint - xo) / (xn - xo);
+      yint = yo + (yn - yo) * delta;
+      zint = zo + (zn - zo) * delta;
+      zint2 = R(0.5) * d3;
+      if ((zint >= R(-0.5)) && (zint < R(0.5))) {
+        vp[k++] = mk3(xo, yo, zo, xint, yint, zint, qq, io, jo, ko);
+        delta = (zint2 - zint) / (zn - zint);
+        xint2 = -xint + (xn - xint) * delta;
+        yint2 = yint + (yn - yint) * delta;
+        vp[k++] = mk3(-xint, yint, zint, xint2, yint2, zint2, qq, io + d1, jo, ko);
+        vp[k++] = mk3(xint2, yint2, -zint2, xn - d1, yn, zn - d3, qq, io + d1, jo, ko + d3);
+      } else {
+        delta = (zint2 - zo) / (zint - zo);
+        xint2 = xo + (xint - xo) * delta;
+        yint2 = yo + (yint - yo) * delta;
+        vp[k++] = mk3(xo, yo, zo, xint2, yint2, zint2, qq, io, jo, ko);
+        vp[k++] = mk3(xint2, yint2, -zint2, xint, yint, zint - d3, qq, io, jo, ko + d3);
+        vp[k++] = mk3(-xint, yint, zint - d3, xn - d1, yn, zn - d3, qq, io + d1, jo, ko + d3);
+      }
+      break;
+    case 6: /* :1302-1428 */
+      yint = R(0.5) * d2;
+      delta = (yint - yo) / (yn - yo);
+      xint = xo + (xn - xo) * delta;
+      zint = zo + (zn - zo) * delta;
+      if ((zint >= R(-0.5)) && (zint < R(0.5))) {
+        vp[k++] = mk3(xo, yo, zo, xint, yint, zint, qq, io, jo, ko);
+        zint2 = R(0.5) * d3;
+        delta = (zint2 - zint) / (zn - zint);
+        xint2 = xint + (xn - xint) * delta;
+        yint2 = -yint + (yn - yint) * delta;
+        vp[k++] = mk3(xint, -yint, zint, xint2, yint2, zint2, qq, io, jo + d2, ko);
+        vp[k++] = mk3(xint2, yint2, -zint2, xn, yn - d2, zn - d3, qq, io, jo + d2, ko + d3);
+      } else {
+        zint2 = R(0.5) * d3;
+        delta = (zint2 - zo) / (zint - zo);
+        xint2 = xo + (xint - xo) * delta;
+        yint2 = yo + (yint - yo) * delta;
+        vp[k++] = mk3(xo, yo, zo, xint2, yint2, zint2, qq, io, jo, ko);
+        vp[k++] = mk3(xint2, yint2, -zint2, xint, yint, zint - d3, qq, io, jo, ko + d3);
+        vp[k++] = mk3(xint, -yint, zint - d3, xn, yn - d2, zn - d3, qq, io, jo + d2, ko + d3);
+      }
+      break;
+    case 7: { /* :1431-1592 */
+      xint = R(0.5) * d1;
+      delta = (xint - xo) / (xn - xo);
+      yint = yo + (yn - yo) * delta;
+      zint = zo + (zn - zo) * delta;
+      if ((yint >= R(-0.5)) && (yint < R(0.5))) {
+        vp[k++] = mk3(xo, yo, zo, xint, yint, zint, qq, io, jo, ko);
+        yint2 = R(0.5) * d2;
+        delta = (yint2 - yint) / (yn - yint);
+        xint2 = -xint + (xn - xint) * delta;
+        zint2 = zint + (zn - zint) * delta;
+        vp[k++] = mk3(-xint, yint, zint, xint2, yint2, zint2, qq, io + d1, jo, ko);
+        vp[k++] = mk3(xint2, -yint2, zint2, xn - d1, yn - d2, zn, qq, io + d1, jo + d2, ko);
+      } else {
+        yint2 = R(0.5) * d2;
+        delta = (yint2 - yo) / (yint - yo);
+        xint2 = xo + (xint - xo) * delta;
+        zint2 = zo + (zint - zo) * delta;
+        vp[k++] = mk3(xo, yo, zo, xint2, yint2, zint2, qq, io, jo, ko);
+        vp[k++] = mk3(xint2, -yint2, zint2, xint, yint - d2, zint, qq, io, jo + d2, ko);
+        vp[k++] = mk3(-xint, yint - d2, zint, xn - d1, yn - d2, zn, qq, io + d1, jo + d2, ko);
+      }
+      /* one of the 3 vp requires an additional z split, :1561-1590 */
+      zint = R(0.5) * d3;
+      for (int vpidx = k - 3; vpidx < k; vpidx++) {
+        if ((vp[vpidx].z1 < R(-0.5)) || (vp[vpidx].z1 >= R(0.5))) {
+          vp3d *o = &vp[vpidx];
+          delta = (zint - o->z0) / (o->z1 - o->z0);
+          xint = o->x0 + (o->x1 - o->x0) * delta;
+          yint = o->y0 + (o->y1 - o->y0) * delta;
+          vp[k] = mk3(xint, yint, -zint, o->x1, o->y1, o->z1 - d3, o->q, o->i, o->j, o->k + d3);
+          k++;
+          o->x1 = xint;
+          o->y1 = yint;
+          o->z1 = zint;
+          for (int jj = vpidx + 1; jj < k - 1; jj++) {
+            vp[jj].z0 = vp[jj].z0 - d3;
+            vp[jj].z1 = vp[jj].z1 - d3;
+            vp[jj].k = vp[jj].k + d3;
+          }
+          break;
+        }
+      }
+    } break;
+    }
+  }
+  return k;
+}
+
+/* ------------------------------------------------------------------------------------
+ * Current deposition from virtual particles.
+ * DEP_CURRENT_2D: spec/os-spec-current.f03:2691-2796; DEP_CURRENT_3D: :2800-2905
+ * P0/P1 template params :2495-2533.  jnorm = real(dx/dt/2) (2D), real(dx/dt/3) (3D).
+ * ------------------------------------------------------------------------------------ */
+static void dep_vp_2d(const orc_grid *g, int order, real *j, const vp2d *vp, int nsplit,
+                      real jn1, real jn2) {
+  const int n = order + 1, p0 = lp_of(order);
+  const real c1_3 = R(1.0) / R(3.0);
+  real S0x[4], S1x[4], S0y[4], S1y[4], wl1[4], wl2[4], wp1[4], wp2[4];
+  for (int i = 0; i < nsplit; i++) {
+    const vp2d *v = &vp[i];
+    real qnx = v->q * jn1, qny = v->q * jn2, qvz = c1_3 * (v->q * v->vz);
+    orc_spline(order, v->x0, S0x);
+    orc_spline(order, v->x1, S1x);
+    orc_spline(order, v->y0, S0y);
+    orc_spline(order, v->y1, S1y);
+    orc_wl(order, qnx, v->x0, v->x1, wl1);
+    orc_wl(order, qny, v->y0, v->y1, wl2);
+    for (int k = 0; k < n; k++) {
+      wp1[k] = S0y[k] + S1y[k];
+      wp2[k] = S0x[k] + S1x[k];
+    }
+    for (int k2 = 0; k2 < n; k2++)
+      for (int k1 = 0; k1 < n; k1++) {
+        real *jj = &FLD(j, g, 0, v->i + p0 + k1, v->j + p0 + k2, 1);
+        real j1 = jj[0], j2 = jj[1], j3 = jj[2];
+        j1 = j1 + wl1[k1] * wp1[k2];
+        j2 = j2 + wp2[k1] * wl2[k2];
+        real tmp1 = S0x[k1] * S0y[k2] + S1x[k1] * S1y[k2];
+        real tmp2 = S0x[k1] * S1y[k2] + S1x[k1] * S0y[k2];
+        j3 = j3 + qvz * (tmp1 + R(0.5) * tmp2);
+        jj[0] = j1;
+        jj[1] = j2;
+        jj[2] = j3;
+      }
+  }
+}
+
+static void dep_vp_3d(const orc_grid *g, int order, real *j, const vp3d *vp, int nsplit,
+                      real jn1, real jn2, real jn3) {
+  const int n = order + 1, p0 = lp_of(order);
+  real S0x[4], S1x[4], S0y[4], S1y[4], S0z[4], S1z[4], wl1[4], wl2[4], wl3[4];
+  real wp1[4][4], wp2[4][4], wp3[4][4];
+  for (int i = 0; i < nsplit; i++) {
+    const vp3d *v = &vp[i];
+    real qnx = v->q * jn1, qny = v->q * jn2, qnz = v->q * jn3;
+    orc_spline(order, v->x0, S0x);
+    orc_spline(order, v->x1, S1x);
+    orc_spline(order, v->y0, S0y);
+    orc_spline(order, v->y1, S1y);
+    orc_spline(order, v->z0, S0z);
+    orc_spline(order, v->z1, S1z);
+    orc_wl(order, qnx, v->x0, v->x1, wl1);
+    orc_wl(order, qny, v->y0, v->y1, wl2);
+    orc_wl(order, qnz, v->z0, v->z1, wl3);
+    for (int k2 = 0; k2 < n; k2++)
+      for (int k1 = 0; k1 < n; k1++) {
+        wp1[k1][k2] = (S0y[k1] * S0z[k2] + S1y[k1] * S1z[k2]) +
+                      R(0.5) * (S0y[k1] * S1z[k2] + S1y[k1] * S0z[k2]);
+        wp2[k1][k2] = (S0x[k1] * S0z[k2] + S1x[k1] * S1z[k2]) +
+                      R(0.5) * (S0x[k1] * S1z[k2] + S1x[k1] * S0z[k2]);
+        wp3[k1][k2] = (S0x[k1] * S0y[k2] + S1x[k1] * S1y[k2]) +
+                      R(0.5) * (S0x[k1] * S1y[k2] + S1x[k1] * S0y[k2]);
+      }
+    for (int k3 = 0; k3 < n; k3++)
+      for (int k2 = 0; k2 < n; k2++)
+        for (int k1 = 0; k1 < n; k1++) {
+          real *jj = &FLD(j, g, 0, v->i + p0 + k1, v->j + p0 + k2, v->k + p0 + k3);
+          real j1 = jj[0], j2 = jj[1], j3 = jj[2];
+          j1 = j1 + wl1[k1] * wp1[k2][k3];
+          j2 = j2 + wl2[k2] * wp2[k1][k3];
+          j3 = j3 + wl3[k3] * wp3[k1][k2];
+          jj[0] = j1;
+          jj[1] = j2;
+          jj[2] = j3;
+        }
+  }
+}
+
+/* ------------------------------------------------------------------------------------
+ * advance_deposit_2d / 3d: spec/os-spec-push.f03:1438-1518 / 1522-1605, called from the
+ * dispatcher advance_deposit :1748-1810 (dtcycle = real(dt, p_k_part) :1780).
+ * pusher: 0 = Boris, 1 = Vay.  Updates ix, x, u in place, accumulates into j (caller
+ * zeroes it), returns the un-normalised time-centred energy sum (Sum q*u^2/(gamma+1)).
+ * ------------------------------------------------------------------------------------ */
+double orc_advance_deposit(const orc_grid *g, int order, int pusher, real rqm,
+                           const double *dx, double dt_in, const real *e, const real *b,
+                           real *j, int *ix, real *x, real *u, const real *q, int np) {
+  const int D = g->dims;
+  const double gdt = (double)(real)dt_in;
+  real *ep = malloc(sizeof(real) * 3 * P_CACHE_SIZE);
+  real *bp = malloc(sizeof(real) * 3 * P_CACHE_SIZE);
+  double energy = 0;
+
+  /* dudt over the whole range in cache-size chunks (get_emf per chunk) */
+  for (int c0 = 0; c0 < np; c0 += P_CACHE_SIZE) {
+    int nc = np - c0 < P_CACHE_SIZE ? np - c0 : P_CACHE_SIZE;
+    orc_get_emf(g, order, e, b, ix + D * c0, x + D * c0, nc, ep, bp);
+    if (pusher == 0)
+      energy += orc_dudt_boris(rqm, gdt, u + 3 * c0, q + c0, ep, bp, nc);
+    else
+      energy += orc_dudt_vay(rqm, gdt, u + 3 * c0, q + c0, ep, bp, nc);
+  }
+  free(ep);
+  free(bp);
+
+  real dt_dx[3];
+  for (int d = 0; d < D; d++) dt_dx[d] = (real)(gdt / dx[d]);
+  real *xbuf = malloc(sizeof(real) * D * P_CACHE_SIZE);
+  int *dxi = malloc(sizeof(int) * D * P_CACHE_SIZE);
+  real *rgamma = malloc(sizeof(real) * P_CACHE_SIZE);
+  void *vpb = malloc(D == 2 ? sizeof(vp2d) * 3 * P_CACHE_SIZE : sizeof(vp3d) * 4 * P_CACHE_SIZE);
+  for (int c0 = 0; c0 < np; c0 += P_CACHE_SIZE) {
+    int nc = np - c0 < P_CACHE_SIZE ? np - c0 : P_CACHE_SIZE;
+    for (int i = 0; i < nc; i++) {
+      const real *p = u + 3 * (c0 + i);
+      rgamma[i] = R(1.0) / (real)sqrt(((R(1.0) + p[0] * p[0]) + p[1] * p[1]) + p[2] * p[2]);
+    }
+    for (int i = 0; i < nc; i++) {
+      const real *p = u + 3 * (c0 + i);
+      for (int d = 0; d < D; d++) {
+        xbuf[D * i + d] = x[D * (c0 + i) + d] + (p[d] * rgamma[i]) * dt_dx[d];
+        dxi[D * i + d] = ntrim(xbuf[D * i + d]);
+      }
+    }
+    if (D == 2) {
+      int ns = split_2d(dxi, xbuf, ix + 2 * c0, x + 2 * c0, q + c0, rgamma, u + 3 * c0, nc,
+                        (vp2d *)vpb);
+      dep_vp_2d(g, order, j, (vp2d *)vpb, ns, (real)(dx[0] / gdt / 2), (real)(dx[1] / gdt / 2));
+    } else {
+      int ns = split_3d(dxi, xbuf, ix + 3 * c0, x + 3 * c0, q + c0, nc, (vp3d *)vpb);
+      dep_vp_3d(g, order, j, (vp3d *)vpb, ns, (real)(dx[0] / gdt / 3), (real)(dx[1] / gdt / 3),
+                (real)(dx[2] / gdt / 3));
+    }
+    for (int i = 0; i < nc; i++)
+      for (int d = 0; d < D; d++) {
+        x[D * (c0 + i) + d] = xbuf[D * i + d] - dxi[D * i + d];
+        ix[D * (c0 + i) + d] = ix[D * (c0 + i) + d] + dxi[D * i + d];
+      }
+  }
+  free(xbuf);
+  free(dxi);
+  free(rgamma);
+  free(vpb);
+  return energy;
+}
+
+/* Splitting alone, for tests: returns VP count and writes VP records
+ * (x0,y0,[z0],x1,y1,[z1],q,[vz]) and cells. */
+int orc_split(int dims, int np, const int *dxi, const real *xnew, const int *ixold,
+              const real *xold, const real *q, const real *rgamma, const real *u, real *vpf,
+              int *vpi) {
+  if (dims == 2) {
+    vp2d *vp = malloc(sizeof(vp2d) * 3 * (size_t)(np > 0 ? np : 1));
+    int ns = split_2d(dxi, xnew, ixold, xold, q, rgamma, u, np, vp);
+    for (int k = 0; k < ns; k++) {
+      real *f = vpf + 6 * k;
+      f[0] = vp[k].x0; f[1] = vp[k].y0; f[2] = vp[k].x1; f[3] = vp[k].y1;
+      f[4] = vp[k].q; f[5] = vp[k].vz;
+      vpi[2 * k] = vp[k].i; vpi[2 * k + 1] = vp[k].j;
+    }
+    free(vp);
+    return ns;
+  } else {
+    vp3d *vp = malloc(sizeof(vp3d) * 4 * (size_t)(np > 0 ? np : 1));
+    int ns = split_3d(dxi, xnew, ixold, xold, q, np, vp);
+    for (int k = 0; k < ns; k++) {
+      real *f = vpf + 7 * k;
+      f[0] = vp[k].x0; f[1] = vp[k].y0; f[2] = vp[k].z0; f[3] = vp[k].x1;
+      f[4] = vp[k].y1; f[5] = vp[k].z1; f[6] = vp[k].q;
+      vpi[3 * k] = vp[k].i; vpi[3 * k + 1] = vp[k].j; vpi[3 * k + 2] = vp[k].k;
+    }
+    free(vp);
+    return ns;
+  }
+}
+
+/* ------------------------------------------------------------------------------------
+ * Periodic boundaries, single node.
+ * Grids: vdf/os-vdf-comm.f90:142-330 update_periodic_vdf, dims in order 1..x_dim
+ * (:685-692).  mode 0 = p_vdf_replace (E,B), 1 = p_vdf_add (current fold).
+ * Particles: spec/os-spec-boundary.f03:1280-1290 (ix += nx below 1, ix -= nx above nx).
+ * ------------------------------------------------------------------------------------ */
+void orc_update_periodic(const orc_grid *g, real *f, int nc, int mode) {
+  const int D = g->dims;
+  for (int dim = 0; dim < D; dim++) {
+    int shift[3] = {0, 0, 0};
+    shift[dim] = g->nx[dim];
+    int llo[3], lhi[3], ulo[3], uhi[3];
+    for (int d = 0; d < 3; d++) {
+      if (d >= D) {
+        llo[d] = lhi[d] = ulo[d] = uhi[d] = 1;
+      } else if (d == dim) {
+        llo[d] = 1 - g->gcl[d];
+        lhi[d] = 0;
+        ulo[d] = g->nx[d] + 1;
+        uhi[d] = g->nx[d] + g->gcu[d];
+      } else {
+        llo[d] = ulo[d] = 1 - g->gcl[d];
+        lhi[d] = uhi[d] = g->nx[d] + g->gcu[d];
+      }
+    }
+#define IDX(i1, i2, i3) (nc * cidx(g, (i1), (i2), (i3)))
+    for (int i3 = llo[2]; i3 <= lhi[2]; i3++)
+      for (int i2 = llo[1]; i2 <= lhi[1]; i2++)
+        for (int i1 = llo[0]; i1 <= lhi[0]; i1++)
+          for (int c = 0; c < nc; c++) {
+            real *a = f + IDX(i1, i2, i3) + c;
+            real *s = f + IDX(i1 + shift[0], i2 + shift[1], i3 + shift[2]) + c;
+            if (mode == 1) {
+              real val = *a + *s;
+              *a = val;
+              *s = val;
+            } else
+              *a = *s;
+          }
+    for (int i3 = ulo[2]; i3 <= uhi[2]; i3++)
+      for (int i2 = ulo[1]; i2 <= uhi[1]; i2++)
+        for (int i1 = ulo[0]; i1 <= uhi[0]; i1++)
+          for (int c = 0; c < nc; c++) {
+            real *a = f + IDX(i1, i2, i3) + c;
+            real *s = f + IDX(i1 - shift[0], i2 - shift[1], i3 - shift[2]) + c;
+            if (mode == 1) {
+              real val = *a + *s;
+              *s = val;
+              *a = val;
+            } else
+              *a = *s;
+          }
+#undef IDX
+  }
+}
+
+void orc_particles_periodic(const orc_grid *g, int *ix, int np) {
+  for (int i = 0; i < np; i++)
+    for (int d = 0; d < g->dims; d++) {
+      int *v = &ix[g->dims * i + d];
+      if (*v < 1)
+        *v += g->nx[d];
+      else if (*v > g->nx[d])
+        *v -= g->nx[d];
+    }
+}
+
+/* ------------------------------------------------------------------------------------
+ * Yee solver.  emf/solver/os-emf-solver-yee.f03: dbdt_2d :102-138, dbdt_3d :211-251,
+ * dedt_2d :287-326, dedt_3d :385-428, advance_yee :542-568 (B dt/2, E dt, B dt/2 with a
+ * periodic refresh after each sub-step).
+ * ------------------------------------------------------------------------------------ */
+static void dbdt(const orc_grid *g, const double *dx, double dt, real *b, const real *e) {
+  real d1 = (real)(dt / dx[0]), d2 = (real)(dt / dx[1]);
+  if (g->dims == 2) {
+    for (int i2 = 0; i2 <= g->nx[1] + 1; i2++)
+      for (int i1 = 0; i1 <= g->nx[0] + 1; i1++) {
+        real *bb = &FLD(b, g, 0, i1, i2, 1);
+        bb[0] = bb[0] - d2 * (FLD(e, g, 2, i1, i2 + 1, 1) - FLD(e, g, 2, i1, i2, 1));
+        bb[1] = bb[1] + d1 * (FLD(e, g, 2, i1 + 1, i2, 1) - FLD(e, g, 2, i1, i2, 1));
+        bb[2] = bb[2] - d1 * (FLD(e, g, 1, i1 + 1, i2, 1) - FLD(e, g, 1, i1, i2, 1)) +
+                d2 * (FLD(e, g, 0, i1, i2 + 1, 1) - FLD(e, g, 0, i1, i2, 1));
+      }
+  } else {
+    real d3 = (real)(dt / dx[2]);
+    for (int i3 = 0; i3 <= g->nx[2] + 1; i3++)
+      for (int i2 = 0; i2 <= g->nx[1] + 1; i2++)
+        for (int i1 = 0; i1 <= g->nx[0] + 1; i1++) {
+          real *bb = &FLD(b, g, 0, i1, i2, i3);
+          bb[0] = bb[0] - d2 * (FLD(e, g, 2, i1, i2 + 1, i3) - FLD(e, g, 2, i1, i2, i3)) +
+                  d3 * (FLD(e, g, 1, i1, i2, i3 + 1) - FLD(e, g, 1, i1, i2, i3));
+          bb[1] = bb[1] + d1 * (FLD(e, g, 2, i1 + 1, i2, i3) - FLD(e, g, 2, i1, i2, i3)) -
+                  d3 * (FLD(e, g, 0, i1, i2, i3 + 1) - FLD(e, g, 0, i1, i2, i3));
+          bb[2] = bb[2] - d1 * (FLD(e, g, 1, i1 + 1, i2, i3) - FLD(e, g, 1, i1, i2, i3)) +
+                  d2 * (FLD(e, g, 0, i1, i2 + 1, i3) - FLD(e, g, 0, i1, i2, i3));
+        }
+  }
+}
+
+static void dedt(const orc_grid *g, const double *dx, double dt, real *e, const real *b,
+                 const real *j) {
+  real d1 = (real)(dt / dx[0]), d2 = (real)(dt / dx[1]), dtif = (real)dt;
+  if (g->dims == 2) {
+    for (int i2 = 1; i2 <= g->nx[1] + 1; i2++)
+      for (int i1 = 1; i1 <= g->nx[0] + 1; i1++) {
+        real *ee = &FLD(e, g, 0, i1, i2, 1);
+        const real *jj = &FLD(j, g, 0, i1, i2, 1);
+        ee[0] = ee[0] - dtif * jj[0] + d2 * (FLD(b, g, 2, i1, i2, 1) - FLD(b, g, 2, i1, i2 - 1, 1));
+        ee[1] = ee[1] - dtif * jj[1] - d1 * (FLD(b, g, 2, i1, i2, 1) - FLD(b, g, 2, i1 - 1, i2, 1));
+        ee[2] = ee[2] - dtif * jj[2] + d1 * (FLD(b, g, 1, i1, i2, 1) - FLD(b, g, 1, i1 - 1, i2, 1)) -
+                d2 * (FLD(b, g, 0, i1, i2, 1) - FLD(b, g, 0, i1, i2 - 1, 1));
+      }
+  } else {
+    real d3 = (real)(dt / dx[2]);
+    for (int i3 = 1; i3 <= g->nx[2] + 1; i3++)
+      for (int i2 = 1; i2 <= g->nx[1] + 1; i2++)
+        for (int i1 = 1; i1 <= g->nx[0] + 1; i1++) {
+          real *ee = &FLD(e, g, 0, i1, i2, i3);
+          const real *jj = &FLD(j, g, 0, i1, i2, i3);
+          ee[0] = ee[0] - dtif * jj[0] +
+                  d2 * (FLD(b, g, 2, i1, i2, i3) - FLD(b, g, 2, i1, i2 - 1, i3)) -
+                  d3 * (FLD(b, g, 1, i1, i2, i3) - FLD(b, g, 1, i1, i2, i3 - 1));
+          ee[1] = ee[1] - dtif * jj[1] +
+                  d3 * (FLD(b, g, 0, i1, i2, i3) - FLD(b, g, 0, i1, i2, i3 - 1)) -
+                  d1 * (FLD(b, g, 2, i1, i2, i3) - FLD(b, g, 2, i1 - 1, i2, i3));
+          ee[2] = ee[2] - dtif * jj[2] +
+                  d1 * (FLD(b, g, 1, i1, i2, i3) - FLD(b, g, 1, i1 - 1, i2, i3)) -
+                  d2 * (FLD(b, g, 0, i1, i2, i3) - FLD(b, g, 0, i1, i2 - 1, i3));
+        }
+  }
+}
+
+void orc_yee_advance(const orc_grid *g, const double *dx, double dt, real *e, real *b,
+                     const real *j) {
+  dbdt(g, dx, dt / 2.0, b, e);
+  orc_update_periodic(g, b, 3, 0);
+  dedt(g, dx, dt, e, b, j);
+  orc_update_periodic(g, e, 3, 0);
+  dbdt(g, dx, dt / 2.0, b, e);
+  orc_update_periodic(g, b, 3, 0);
+}
+
+/* ------------------------------------------------------------------------------------
+ * Diagnostics.
+ * total_vdf(pow=2) over interior cells, vdf/os-vdf-math.f90:625-660, in double;
+ * report_energy_emf: emf/diagnostics/os-emf-diagnostics.f03:44-55 (x 0.5*dvol),
+ * out = (B1,B2,B3,E1,E2,E3).
+ * ------------------------------------------------------------------------------------ */
+void orc_field_energy(const orc_grid *g, const real *e, const real *b, double dvol,
+                      double *out) {
+  double t[6] = {0, 0, 0, 0, 0, 0};
+  int n3 = g->dims > 2 ? g->nx[2] : 1;
+  for (int i3 = 1; i3 <= n3; i3++)
+    for (int i2 = 1; i2 <= g->nx[1]; i2++)
+      for (int i1 = 1; i1 <= g->nx[0]; i1++)
+        for (int c = 0; c < 3; c++) {
+          real vb = FLD(b, g, c, i1, i2, i3), ve = FLD(e, g, c, i1, i2, i3);
+          t[c] = t[c] + (double)(vb * vb);
+          t[3 + c] = t[3 + c] + (double)(ve * ve);
+        }
+  for (int c = 0; c < 6; c++) out[c] = t[c] * dvol * 0.5;
+}
+
+/* get_energy recalculation branch, spec/os-spec-define.f03:1041-1052 (un-normalised) */
+double orc_kinetic_energy(const real *u, const real *q, int np) {
+  double energy = 0;
+  for (int i = 0; i < np; i++) {
+    double u2 = (double)u[3 * i] * u[3 * i] + (double)u[3 * i + 1] * u[3 * i + 1] +
+                (double)u[3 * i + 2] * u[3 * i + 2];
+    double gamma = sqrt(u2 + 1);
+    energy = energy + (double)q[i] * (u2 / (gamma + 1));
+  }
+  return energy;
+}
+
+/* DEPOSIT_RHO_2D/3D, spec/os-spec-charge.f90:500-580: rho on the lower cell corner with
+ * spline weights of stencil LP..UP.  rho is a one-component grid (nc = 1). */
+void orc_deposit_rho(const orc_grid *g, int order, const int *ix, const real *x,
+                     const real *q, int np, real *rho) {
+  const int n = order + 1, lp = lp_of(order), D = g->dims;
+  real w1[4], w2[4], w3[4];
+  for (int l = 0; l < np; l++) {
+    orc_spline(order, x[D * l], w1);
+    orc_spline(order, x[D * l + 1], w2);
+    if (D == 3) orc_spline(order, x[D * l + 2], w3);
+    int n3 = D == 3 ? n : 1;
+    for (int k3 = 0; k3 < n3; k3++)
+      for (int k2 = 0; k2 < n; k2++)
+        for (int k1 = 0; k1 < n; k1++) {
+          long c = cidx(g, ix[D * l] + lp + k1, ix[D * l + 1] + lp + k2,
+                        D == 3 ? ix[D * l + 2] + lp + k3 : 1);
+          if (D == 2)
+            rho[c] = rho[c] + q[l] * w1[k1] * w2[k2];
+          else
+            rho[c] = rho[c] + q[l] * w1[k1] * w2[k2] * w3[k3];
+        }
+  }
+}
+
+/* div_vdf with offset 0, vdf/os-vdf-math.f90:1013-1073; div is a one-component grid */
+void orc_div(const orc_grid *g, const double *dx, const real *f, real *div) {
+  real r1 = (real)(1.0 / dx[0]), r2 = (real)(1.0 / dx[1]);
+  real r3 = g->dims > 2 ? (real)(1.0 / dx[2]) : 0;
+  int n3 = g->dims > 2 ? g->nx[2] : 1;
+  for (int i3 = 1; i3 <= n3; i3++)
+    for (int i2 = 1; i2 <= g->nx[1]; i2++)
+      for (int i1 = 1; i1 <= g->nx[0]; i1++) {
+        real v = (FLD(f, g, 0, i1, i2, i3) - FLD(f, g, 0, i1 - 1, i2, i3)) * r1 +
+                 (FLD(f, g, 1, i1, i2, i3) - FLD(f, g, 1, i1, i2 - 1, i3)) * r2;
+        if (g->dims > 2) v = v + (FLD(f, g, 2, i1, i2, i3) - FLD(f, g, 2, i1, i2, i3 - 1)) * r3;
+        div[cidx(g, i1, i2, i3)] = v;
+      }
+}
+
+/* ------------------------------------------------------------------------------------
+ * One full time step of the per-step order of os-simulation.f03:498-666 for periodic
+ * boundaries, no smoothing, no external fields: E/B guard refresh (:561), push+deposit
+ * into a zeroed J (:588, os-particles.f03:1453-1466), particle wrap (:593), energy report
+ * of E^n/B^n and the time-centred kinetic energy (:596-597), J fold (:652), Yee (:661).
+ *
+ * Species are passed as arrays of pointers.  fld_ene[6] and par_ene[nspec] receive the
+ * step's HIST rows (normalised as report_energy_* does).
+ * ------------------------------------------------------------------------------------ */
+void orc_step(const orc_grid *g, int order, const double *dx, double dt, real *e, real *b,
+              real *j, int nspec, const int *pusher, const real *rqm, int *const *ix,
+              real *const *x, real *const *u, real *const *q, const int *np, double *fld_ene,
+              double *par_ene) {
+  double dvol = dx[0] * dx[1] * (g->dims > 2 ? dx[2] : 1.0);
+  orc_update_periodic(g, e, 3, 0);
+  orc_update_periodic(g, b, 3, 0);
+  memset(j, 0, sizeof(real) * 3 * orc_grid_cells(g));
+  for (int s = 0; s < nspec; s++) {
+    double ene = orc_advance_deposit(g, order, pusher[s], rqm[s], dx, dt, e, b, j, ix[s], x[s],
+                                     u[s], q[s], np[s]);
+    orc_particles_periodic(g, ix[s], np[s]);
+    if (par_ene) par_ene[s] = ene * (double)rqm[s] * dvol;
+  }
+  if (fld_ene) orc_field_energy(g, e, b, dvol, fld_ene);
+  orc_update_periodic(g, j, 3, 1);
+  orc_yee_advance(g, dx, dt, e, b, j);
+}

@@ -1,0 +1,11 @@
+"""osiris_amd -- the OSIRIS per-timestep PIC hot path on MI355X (gfx950).
+
+Gather + Boris/Vay push + charge-conserving deposit (splines of order 1..3) + Yee FDTD,
+as hand-written HIP kernels behind a C ABI (include/osiris_amd.h); this package is the
+Python host side over that ABI.
+"""
+from ._lib import OSR_BORIS, OSR_VAY, OsrError, lib  # noqa: F401
+from .sim import Simulation, advance_deposit, device_count, yee_advance  # noqa: F401
+
+__all__ = ["Simulation", "advance_deposit", "yee_advance", "device_count", "OSR_BORIS",
+           "OSR_VAY", "OsrError", "lib"]

@@ -1,0 +1,88 @@
+"""ctypes binding of libosiris_amd.so (the C ABI of include/osiris_amd.h).
+
+The library is built in-tree (``osiris_amd/lib/libosiris_amd.so``).  Loading fails
+loudly when it is missing: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libosiris_amd.so")
+
+OSR_OK = 0
+OSR_BORIS = 0
+OSR_VAY = 1
+
+P = C.c_void_p
+I = C.c_int
+I64 = C.c_int64
+D = C.c_double
+F = C.c_float
+
+# name -> (restype, argtypes); every symbol of include/osiris_amd.h
+SIGNATURES = {
+    "osr_last_error": (C.c_char_p, []),
+    "osr_version": (I, []),
+    "osr_device_count": (I, []),
+    "osr_advance_deposit": (I, [I, I, I, P, P, P, P, I, I, F, P, P, P, P, P, P, P, P, D, P]),
+    "hip_vadvance_deposit_2d_s1_": (None, [P] * 17),
+    "hip_vadvance_deposit_2d_s2_": (None, [P] * 17),
+    "hip_vadvance_deposit_2d_s3_": (None, [P] * 17),
+    "hip_vadvance_deposit_3d_s1_": (None, [P] * 17),
+    "hip_vadvance_deposit_3d_s2_": (None, [P] * 17),
+    "hip_vadvance_deposit_3d_s3_": (None, [P] * 17),
+    "osr_yee_advance": (I, [I, P, P, P, P, D, P, P, P]),
+    "osr_sim_create": (I, [P, P]),
+    "osr_sim_destroy": (I, [P]),
+    "osr_sim_add_species": (I, [P, F, I, P]),
+    "osr_species_set": (I, [P, I, I64, P, P, P, P]),
+    "osr_species_count": (I, [P, I, P]),
+    "osr_species_get": (I, [P, I, P, P, P, P]),
+    "osr_species_generate": (I, [P, I, P, D, P, P, C.c_uint64, I]),
+    "osr_fields_set": (I, [P, P, P, P, P]),
+    "osr_fields_get": (I, [P, P, P, P, P]),
+    "osr_fields_fill": (I, [P, P, P]),
+    "osr_sim_run": (I, [P, I]),
+    "osr_sim_steps": (I, [P, P]),
+    "osr_sim_history": (I, [P, I64, I64, P, P]),
+    "osr_sim_gauss": (I, [P, P, P]),
+    "osr_sim_relayouts": (I, [P, P]),
+    "osr_sim_time_kernels": (I, [P, I, P]),
+    "osr_sim_footprint": (I, [P, P]),
+}
+
+
+class SimDesc(C.Structure):
+    _fields_ = [("dims", C.c_int), ("nx", C.c_int * 3), ("dx", C.c_double * 3),
+                ("dt", C.c_double), ("order", C.c_int), ("tile", C.c_int * 3),
+                ("reserved", C.c_int * 4)]
+
+
+class OsrError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load the library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OsrError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int):
+    if status != OSR_OK:
+        msg = lib().osr_last_error().decode(errors="replace")
+        raise OsrError(f"osiris_amd error {status}: {msg}")

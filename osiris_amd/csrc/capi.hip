@@ -1,0 +1,1380 @@
+// capi.hip -- host side of the C ABI declared in include/osiris_amd.h.
+//
+// One translation unit: the kernel files are included here so that every kernel
+// template is instantiated next to the code that launches it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/osiris_amd.h"
+#include "k_dropin.hip"
+#include "k_field.hip"
+#include "k_loader.hip"
+#include "k_tiles.hip"
+
+using namespace osr;
+
+// ------------------------------------------------------------------------------------
+// errors
+// ------------------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPC(call)                                                                       \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(OSR_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),   \
+                  __FILE__, __LINE__);                                                   \
+  } while (0)
+
+#define TRY(call)              \
+  do {                         \
+    int r_ = (call);           \
+    if (r_ != OSR_OK) return r_; \
+  } while (0)
+
+namespace {
+
+constexpr int kNT = 256;     // threads per workgroup of the tile kernels
+constexpr int kWaves = kNT / 64;
+
+struct DBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  int alloc(size_t bytes) {
+    release();
+    if (bytes == 0) bytes = 256;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      return fail(OSR_ERR_NOMEM, "hipMalloc(%zu bytes) failed", bytes);
+    }
+    n = bytes;
+    return OSR_OK;
+  }
+  template <class T>
+  T* as() const {
+    return (T*)p;
+  }
+};
+
+struct Species {
+  float rqm = -1.f;
+  int pusher = OSR_BORIS;
+  int64_t np = 0;
+  int64_t cap_total = 0;
+  double qabs = 0.0, qmax = 0.0;
+  DBuf cell, x, y, z, ux, uy, uz, q, pid;
+  DBuf mcell, mx, my, mz, mux, muy, muz, mq, mpid, midx;
+  DBuf off, cnt, cap, mv_cnt, mv_stride, newcnt, inc, epart;
+  std::vector<int> h_off, h_cap;
+};
+
+// host-side particle list used by uploads and re-layouts
+struct HostParts {
+  std::vector<uint32_t> cell, pid;
+  std::vector<float> x, y, z, ux, uy, uz, q;
+  size_t size() const { return cell.size(); }
+  void reserve(size_t n) {
+    cell.reserve(n); pid.reserve(n); x.reserve(n); y.reserve(n); z.reserve(n);
+    ux.reserve(n); uy.reserve(n); uz.reserve(n); q.reserve(n);
+  }
+};
+
+}  // namespace
+
+struct osr_sim {
+  osr_sim_desc desc{};
+  GridDesc g{};
+  int dims = 2, order = 2;
+  int ntiles = 0;
+  int nblk = 0;  // Yee blocks of 256 cells
+  double dx[3] = {1, 1, 1};
+  double dt = 0;
+  DBuf E, B, J, ctrl, fe_part, hist;
+  int64_t hist_cap = 0;
+  std::vector<std::unique_ptr<Species>> sp;
+  DBuf d_spec, d_work, d_fd, d_ywork, d_fwork;
+  FieldDesc fd{};
+  hipStream_t stream = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool graph_ok = false;
+  int64_t steps = 0;
+  int64_t relayouts = 0;
+  size_t lds_push = 0;
+  Ctrl* h_ctrl = nullptr;  // pinned mirror
+};
+
+namespace {
+
+int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+// LDS bytes of the push kernel for a tile shape
+size_t push_lds_bytes(int dims, int order, const int* t) {
+  const int glo = order == 1 ? 1 : 2, ghi = order == 3 ? 2 : 1;
+  const int jlo = order == 1 ? 1 : 2, jhi = order == 1 ? 2 : (order == 2 ? 2 : 3);
+  size_t gn = 1, jn = 1;
+  for (int d = 0; d < dims; d++) {
+    gn *= (size_t)(t[d] + glo + ghi);
+    jn *= (size_t)(t[d] + jlo + jhi);
+  }
+  return 3 * jn * sizeof(unsigned long long) + 6 * gn * sizeof(float);
+}
+
+// fixed-point shift of the current: the largest one for which neither the sum of all
+// contributions that can reach one node nor a single contribution can leave the
+// representable range (see the bound in the design notes of the deposit kernel).
+void update_jscale(osr_sim* s) {
+  double qabs = 0.0, qmax = 0.0;
+  for (auto& p : s->sp) {
+    qabs += p->qabs;
+    qmax = std::max(qmax, p->qmax);
+  }
+  const double wsum = s->dims == 2 ? 2.0 : 3.0;
+  double jn = 0.0;
+  for (int d = 0; d < s->dims; d++) jn += s->g.jnorm[d];
+  const double per = wsum * jn + 1.0;
+  const double total = std::max(qabs * per * 8.0, 1e-300);
+  const double single = std::max(qmax * per * 2.0, 1e-300);
+  int sh = std::min(62 - (int)std::ceil(std::log2(total)), 50 - (int)std::ceil(std::log2(single)));
+  sh = std::max(0, std::min(sh, 80));
+  s->g.jscale = std::ldexp(1.0, sh);
+  s->g.jinv = std::ldexp(1.0, -sh);
+}
+
+int upload_descs(osr_sim* s) {
+  std::vector<SpecDesc> v(kMaxSpecies);
+  for (size_t i = 0; i < s->sp.size(); i++) {
+    Species& p = *s->sp[i];
+    SpecDesc& d = v[i];
+    d.g = s->g;
+    d.cell = p.cell.as<uint32_t>();
+    d.x = p.x.as<float>();
+    d.y = p.y.as<float>();
+    d.z = p.z.as<float>();
+    d.ux = p.ux.as<float>();
+    d.uy = p.uy.as<float>();
+    d.uz = p.uz.as<float>();
+    d.q = p.q.as<float>();
+    d.pid = p.pid.as<uint32_t>();
+    d.mcell = p.mcell.as<uint32_t>();
+    d.mx = p.mx.as<float>();
+    d.my = p.my.as<float>();
+    d.mz = p.mz.as<float>();
+    d.mux = p.mux.as<float>();
+    d.muy = p.muy.as<float>();
+    d.muz = p.muz.as<float>();
+    d.mq = p.mq.as<float>();
+    d.mpid = p.mpid.as<uint32_t>();
+    d.midx = p.midx.as<int>();
+    d.off = p.off.as<int>();
+    d.cnt = p.cnt.as<int>();
+    d.cap = p.cap.as<int>();
+    d.mv_cnt = p.mv_cnt.as<int>();
+    d.mv_stride = p.mv_stride.as<int>();
+    d.newcnt = p.newcnt.as<int>();
+    d.inc = p.inc.as<int>();
+    d.epart = p.epart.as<double>();
+    d.tem = (float)(0.5 * s->dt / (double)p.rqm);
+    d.waves = kWaves;
+  }
+  HIPC(hipMemcpy(s->d_spec.p, v.data(), sizeof(SpecDesc) * kMaxSpecies, hipMemcpyHostToDevice));
+  FieldDesc& f = s->fd;
+  f.g = s->g;
+  f.fe_part = s->fe_part.as<double>();
+  f.nblk = s->nblk;
+  f.nspec = (int)s->sp.size();
+  double dvol = 1.0;
+  for (int d = 0; d < s->dims; d++) dvol *= s->dx[d];
+  f.fe_scale = 0.5 * dvol;
+  for (size_t i = 0; i < s->sp.size(); i++) {
+    f.ep[i] = s->sp[i]->epart.as<double>();
+    f.ep_n[i] = s->ntiles;
+    f.ep_scale[i] = (double)s->sp[i]->rqm * dvol;
+  }
+  f.hist = s->hist.as<double>();
+  f.hist_cap = s->hist_cap;
+  for (int d = 0; d < 3; d++) {
+    f.bdt[d] = d < s->dims ? (float)((s->dt / 2.0) / s->dx[d]) : 0.f;
+    f.edt[d] = d < s->dims ? (float)(s->dt / s->dx[d]) : 0.f;
+  }
+  f.dtif = (float)s->dt;
+  f.Jf = nullptr;
+  HIPC(hipMemcpy(s->d_fd.p, &f, sizeof(FieldDesc), hipMemcpyHostToDevice));
+  return OSR_OK;
+}
+
+// Tile index and local cell of a packed cell.
+inline void cell_coords(const osr_sim* s, uint32_t c, int* ic) {
+  if (s->dims == 2) {
+    unpack2(c, ic[0], ic[1]);
+    ic[2] = 0;
+  } else
+    unpack3(c, ic[0], ic[1], ic[2]);
+}
+
+// Allocate a species' arrays for the given per-tile counts and capacities.
+int alloc_species(osr_sim* s, Species& p, const std::vector<int>& cnt) {
+  const int nt = s->ntiles;
+  std::vector<int> cap(nt), off(nt);
+  int64_t total = 0;
+  for (int t = 0; t < nt; t++) {
+    const int c = cnt[t];
+    int64_t cp = (int64_t)c + std::max(c / 4, 64) + kNT;
+    cp = (cp + 63) / 64 * 64;
+    if (total + cp > INT32_MAX) return fail(OSR_ERR_ARG, "species too large for int32 slots");
+    cap[t] = (int)cp;
+    off[t] = (int)total;
+    total += cp;
+  }
+  p.cap_total = total;
+  const size_t n = (size_t)total;
+  TRY(p.cell.alloc(4 * n));
+  TRY(p.x.alloc(4 * n));
+  TRY(p.y.alloc(4 * n));
+  TRY(p.z.alloc(s->dims == 3 ? 4 * n : 4));
+  TRY(p.ux.alloc(4 * n));
+  TRY(p.uy.alloc(4 * n));
+  TRY(p.uz.alloc(4 * n));
+  TRY(p.q.alloc(4 * n));
+  TRY(p.pid.alloc(4 * n));
+  TRY(p.mcell.alloc(4 * n));
+  TRY(p.mx.alloc(4 * n));
+  TRY(p.my.alloc(4 * n));
+  TRY(p.mz.alloc(s->dims == 3 ? 4 * n : 4));
+  TRY(p.mux.alloc(4 * n));
+  TRY(p.muy.alloc(4 * n));
+  TRY(p.muz.alloc(4 * n));
+  TRY(p.mq.alloc(4 * n));
+  TRY(p.mpid.alloc(4 * n));
+  TRY(p.midx.alloc(4 * n));
+  TRY(p.off.alloc(4 * (size_t)nt));
+  TRY(p.cnt.alloc(4 * (size_t)nt));
+  TRY(p.cap.alloc(4 * (size_t)nt));
+  TRY(p.mv_cnt.alloc(4 * (size_t)nt * kWaves));
+  TRY(p.mv_stride.alloc(4 * (size_t)nt));
+  TRY(p.newcnt.alloc(4 * (size_t)nt));
+  TRY(p.inc.alloc(4 * (size_t)nt));
+  TRY(p.epart.alloc(8 * (size_t)nt));
+  HIPC(hipMemcpy(p.off.p, off.data(), 4 * nt, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(p.cnt.p, cnt.data(), 4 * nt, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(p.cap.p, cap.data(), 4 * nt, hipMemcpyHostToDevice));
+  HIPC(hipMemset(p.mv_cnt.p, 0, 4 * (size_t)nt * kWaves));
+  HIPC(hipMemset(p.mv_stride.p, 0, 4 * (size_t)nt));
+  HIPC(hipMemset(p.epart.p, 0, 8 * (size_t)nt));
+  // the memsets above run on the null stream; the simulation stream must see them
+  HIPC(hipDeviceSynchronize());
+  p.h_off = off;
+  p.h_cap = cap;
+  int64_t np = 0;
+  for (int c : cnt) np += c;
+  p.np = np;
+  return OSR_OK;
+}
+
+template <class T>
+int upload(DBuf& b, const std::vector<T>& v) {
+  if (v.empty()) return OSR_OK;
+  HIPC(hipMemcpy(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  return OSR_OK;
+}
+
+// Lay a host particle list out in tiles (order inside a tile: round robin over the
+// tile's cells, so that neighbouring lanes work on different cells) and upload it.
+int layout_upload(osr_sim* s, Species& p, const HostParts& hp) {
+  const int nt = s->ntiles;
+  const size_t n = hp.size();
+  std::vector<int> tile(n), cnt(nt, 0);
+  for (size_t i = 0; i < n; i++) {
+    tile[i] = tile_of(s->g, hp.cell[i]);
+    cnt[tile[i]]++;
+  }
+  TRY(alloc_species(s, p, cnt));
+  // bucket by tile (stable), then by local cell, round robin
+  std::vector<int64_t> tstart(nt + 1, 0);
+  for (int t = 0; t < nt; t++) tstart[t + 1] = tstart[t] + cnt[t];
+  std::vector<int64_t> byt(n);
+  {
+    std::vector<int64_t> fillp(tstart.begin(), tstart.end() - 1);
+    for (size_t i = 0; i < n; i++) byt[fillp[tile[i]]++] = (int64_t)i;
+  }
+  const size_t C = (size_t)p.cap_total;
+  std::vector<uint32_t> cell(C, 0), pid(C, 0);
+  std::vector<float> x(C, 0.f), y(C, 0.f), z(s->dims == 3 ? C : 0, 0.f), ux(C, 0.f), uy(C, 0.f),
+      uz(C, 0.f), q(C, 0.f);
+  const int tcell = s->g.tx * s->g.ty * (s->dims == 3 ? s->g.tz : 1);
+  std::vector<int> ccount(tcell), crank;
+  std::vector<int64_t> order;
+  for (int t = 0; t < nt; t++) {
+    const int64_t b0 = tstart[t], b1 = tstart[t + 1];
+    if (b1 == b0) continue;
+    std::fill(ccount.begin(), ccount.end(), 0);
+    crank.assign((size_t)(b1 - b0), 0);
+    int maxc = 0;
+    for (int64_t k = b0; k < b1; k++) {
+      int ic[3];
+      cell_coords(s, hp.cell[byt[k]], ic);
+      const int lc = (ic[0] % s->g.tx) + s->g.tx * ((ic[1] % s->g.ty) +
+                                                     s->g.ty * (s->dims == 3 ? ic[2] % s->g.tz : 0));
+      crank[k - b0] = ccount[lc]++;
+      maxc = std::max(maxc, ccount[lc]);
+    }
+    // order by (rank within cell, local cell, input order)
+    order.resize((size_t)(b1 - b0));
+    std::vector<int> bucket(maxc + 1, 0);
+    for (int64_t k = b0; k < b1; k++) bucket[crank[k - b0] + 1]++;
+    for (int r = 0; r < maxc; r++) bucket[r + 1] += bucket[r];
+    // inside one rank, keep local-cell order: counting sort by cell first
+    std::vector<int64_t> bycell((size_t)(b1 - b0));
+    {
+      std::vector<int> cstart(tcell + 1, 0);
+      std::vector<int> lcv((size_t)(b1 - b0));
+      for (int64_t k = b0; k < b1; k++) {
+        int ic[3];
+        cell_coords(s, hp.cell[byt[k]], ic);
+        lcv[k - b0] = (ic[0] % s->g.tx) + s->g.tx * ((ic[1] % s->g.ty) +
+                                                      s->g.ty * (s->dims == 3 ? ic[2] % s->g.tz : 0));
+        cstart[lcv[k - b0] + 1]++;
+      }
+      for (int c = 0; c < tcell; c++) cstart[c + 1] += cstart[c];
+      for (int64_t k = b0; k < b1; k++) bycell[cstart[lcv[k - b0]]++] = k;
+    }
+    for (int64_t kk : bycell) order[bucket[crank[kk - b0]]++] = byt[kk];
+    const int64_t base = p.h_off[t];
+    for (int64_t k = 0; k < b1 - b0; k++) {
+      const int64_t i = order[k], o = base + k;
+      cell[o] = hp.cell[i];
+      pid[o] = hp.pid[i];
+      x[o] = hp.x[i];
+      y[o] = hp.y[i];
+      if (s->dims == 3) z[o] = hp.z[i];
+      ux[o] = hp.ux[i];
+      uy[o] = hp.uy[i];
+      uz[o] = hp.uz[i];
+      q[o] = hp.q[i];
+    }
+  }
+  TRY(upload(p.cell, cell));
+  TRY(upload(p.pid, pid));
+  TRY(upload(p.x, x));
+  TRY(upload(p.y, y));
+  if (s->dims == 3) TRY(upload(p.z, z));
+  TRY(upload(p.ux, ux));
+  TRY(upload(p.uy, uy));
+  TRY(upload(p.uz, uz));
+  TRY(upload(p.q, q));
+  double qa = 0.0, qm = 0.0;
+  for (size_t i = 0; i < n; i++) {
+    qa += std::fabs((double)hp.q[i]);
+    qm = std::max(qm, std::fabs((double)hp.q[i]));
+  }
+  p.qabs = qa;
+  p.qmax = qm;
+  return OSR_OK;
+}
+
+template <class T>
+int download(std::vector<T>& v, const DBuf& b, size_t n) {
+  v.resize(n);
+  if (n) HIPC(hipMemcpy(v.data(), b.p, sizeof(T) * n, hipMemcpyDeviceToHost));
+  return OSR_OK;
+}
+
+// Collect the valid particles of a species.  `pending`: the last push moved leavers
+// into the mover buffer but the fill did not run (halted step).
+int collect(osr_sim* s, Species& p, bool pending, HostParts& hp) {
+  const int nt = s->ntiles;
+  const size_t C = (size_t)p.cap_total;
+  std::vector<int> cnt, mvc, mvs, midx;
+  TRY(download(cnt, p.cnt, nt));
+  std::vector<uint32_t> cell, pid;
+  std::vector<float> x, y, z, ux, uy, uz, q;
+  TRY(download(cell, p.cell, C));
+  TRY(download(pid, p.pid, C));
+  TRY(download(x, p.x, C));
+  TRY(download(y, p.y, C));
+  if (s->dims == 3) TRY(download(z, p.z, C));
+  TRY(download(ux, p.ux, C));
+  TRY(download(uy, p.uy, C));
+  TRY(download(uz, p.uz, C));
+  TRY(download(q, p.q, C));
+  std::vector<uint32_t> mcell, mpid;
+  std::vector<float> mx, my, mz, mux, muy, muz, mq;
+  if (pending) {
+    TRY(download(mvc, p.mv_cnt, (size_t)nt * kWaves));
+    TRY(download(mvs, p.mv_stride, nt));
+    TRY(download(midx, p.midx, C));
+    TRY(download(mcell, p.mcell, C));
+    TRY(download(mpid, p.mpid, C));
+    TRY(download(mx, p.mx, C));
+    TRY(download(my, p.my, C));
+    if (s->dims == 3) TRY(download(mz, p.mz, C));
+    TRY(download(mux, p.mux, C));
+    TRY(download(muy, p.muy, C));
+    TRY(download(muz, p.muz, C));
+    TRY(download(mq, p.mq, C));
+  }
+  hp = HostParts();
+  hp.reserve((size_t)p.np);
+  auto push = [&](const std::vector<uint32_t>& c_, const std::vector<uint32_t>& id_,
+                  const std::vector<float>& x_, const std::vector<float>& y_,
+                  const std::vector<float>& z_, const std::vector<float>& ux_,
+                  const std::vector<float>& uy_, const std::vector<float>& uz_,
+                  const std::vector<float>& q_, size_t i) {
+    hp.cell.push_back(c_[i]);
+    hp.pid.push_back(id_[i]);
+    hp.x.push_back(x_[i]);
+    hp.y.push_back(y_[i]);
+    hp.z.push_back(s->dims == 3 ? z_[i] : 0.f);
+    hp.ux.push_back(ux_[i]);
+    hp.uy.push_back(uy_[i]);
+    hp.uz.push_back(uz_[i]);
+    hp.q.push_back(q_[i]);
+  };
+  std::vector<char> hole;
+  for (int t = 0; t < nt; t++) {
+    const int base = p.h_off[t];
+    hole.assign((size_t)cnt[t], 0);
+    if (pending)
+      for (int w = 0; w < kWaves; w++)
+        for (int k = 0; k < mvc[t * kWaves + w]; k++) {
+          const int h = midx[base + w * mvs[t] + k];
+          if (h >= 0 && h < cnt[t]) hole[h] = 1;
+        }
+    for (int k = 0; k < cnt[t]; k++)
+      if (!hole[k]) push(cell, pid, x, y, z, ux, uy, uz, q, (size_t)base + k);
+  }
+  if (pending)
+    for (int t = 0; t < nt; t++)
+      for (int w = 0; w < kWaves; w++)
+        for (int k = 0; k < mvc[t * kWaves + w]; k++)
+          push(mcell, mpid, mx, my, mz, mux, muy, muz, mq, (size_t)p.h_off[t] + w * mvs[t] + k);
+  return OSR_OK;
+}
+
+// Stable order by pid, used to return particles in the order they were given.
+void sort_by_pid(HostParts& hp) {
+  const size_t n = hp.size();
+  std::vector<size_t> idx(n);
+  for (size_t i = 0; i < n; i++) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return hp.pid[a] < hp.pid[b]; });
+  HostParts o;
+  o.reserve(n);
+  for (size_t i : idx) {
+    o.cell.push_back(hp.cell[i]);
+    o.pid.push_back(hp.pid[i]);
+    o.x.push_back(hp.x[i]);
+    o.y.push_back(hp.y[i]);
+    o.z.push_back(hp.z[i]);
+    o.ux.push_back(hp.ux[i]);
+    o.uy.push_back(hp.uy[i]);
+    o.uz.push_back(hp.uz[i]);
+    o.q.push_back(hp.q[i]);
+  }
+  hp = std::move(o);
+}
+
+int relayout(osr_sim* s, bool pending) {
+  for (auto& p : s->sp) {
+    HostParts hp;
+    TRY(collect(s, *p, pending, hp));
+    TRY(layout_upload(s, *p, hp));
+  }
+  TRY(upload_descs(s));
+  s->relayouts++;
+  return OSR_OK;
+}
+
+// ---- kernel dispatch -------------------------------------------------------------
+typedef void (*push_fn)(const SpecDesc*, const int2*);
+
+push_fn pick_push(int dims, int order, int pusher) {
+#define OSR_P(D, O, P) \
+  if (dims == D && order == O && pusher == P) return push_deposit_##D##d<O, P, kNT>;
+  OSR_P(2, 1, 0) OSR_P(2, 2, 0) OSR_P(2, 3, 0) OSR_P(2, 1, 1) OSR_P(2, 2, 1) OSR_P(2, 3, 1)
+  OSR_P(3, 1, 0) OSR_P(3, 2, 0) OSR_P(3, 3, 0) OSR_P(3, 1, 1) OSR_P(3, 2, 1) OSR_P(3, 3, 1)
+#undef OSR_P
+  return nullptr;
+}
+
+int launch_push(osr_sim* s, int sid) {
+  push_fn f = pick_push(s->dims, s->order, s->sp[sid]->pusher);
+  if (!f) return fail(OSR_ERR_ARG, "no push kernel for dims=%d order=%d", s->dims, s->order);
+  hipLaunchKernelGGL(f, dim3(s->ntiles), dim3(kNT), s->lds_push, s->stream,
+                     s->d_spec.as<SpecDesc>(), s->d_work.as<int2>() + (size_t)sid * s->ntiles);
+  HIPC(hipGetLastError());
+  return OSR_OK;
+}
+
+int launch_tiles(osr_sim* s) {
+  const int nw = (int)s->sp.size() * s->ntiles;
+  hipLaunchKernelGGL(tile_count<kNT>, dim3(nw), dim3(kNT), 0, s->stream, s->d_spec.as<SpecDesc>(),
+                     s->d_work.as<int2>());
+  HIPC(hipGetLastError());
+  return OSR_OK;
+}
+
+int launch_fill(osr_sim* s) {
+  const int nw = (int)s->sp.size() * s->ntiles;
+  hipLaunchKernelGGL(tile_fill<kNT>, dim3(nw), dim3(kNT), 0, s->stream, s->d_spec.as<SpecDesc>(),
+                     s->d_work.as<int2>());
+  HIPC(hipGetLastError());
+  return OSR_OK;
+}
+
+int launch_yee(osr_sim* s) {
+  const FieldDesc* fd = s->d_fd.as<FieldDesc>();
+  const int2* w = s->d_ywork.as<int2>();
+  if (s->dims == 2) {
+    hipLaunchKernelGGL((yee_dbdt<2, true>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
+    hipLaunchKernelGGL((yee_dedt<2>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
+    hipLaunchKernelGGL((yee_dbdt<2, false>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
+  } else {
+    hipLaunchKernelGGL((yee_dbdt<3, true>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
+    hipLaunchKernelGGL((yee_dedt<3>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
+    hipLaunchKernelGGL((yee_dbdt<3, false>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
+  }
+  HIPC(hipGetLastError());
+  return OSR_OK;
+}
+
+int launch_finalize(osr_sim* s) {
+  hipLaunchKernelGGL(finalize_step, dim3(1), dim3(256), 0, s->stream, s->d_fd.as<FieldDesc>(),
+                     s->d_fwork.as<int>());
+  HIPC(hipGetLastError());
+  return OSR_OK;
+}
+
+int enqueue_step(osr_sim* s) {
+  for (size_t i = 0; i < s->sp.size(); i++) TRY(launch_push(s, (int)i));
+  if (!s->sp.empty()) {
+    TRY(launch_tiles(s));
+    TRY(launch_fill(s));
+  }
+  TRY(launch_yee(s));
+  TRY(launch_finalize(s));
+  return OSR_OK;
+}
+
+int build_graph(osr_sim* s) {
+  if (s->gexec) {
+    (void)hipGraphExecDestroy(s->gexec);
+    s->gexec = nullptr;
+  }
+  s->graph_ok = false;
+  hipGraph_t graph = nullptr;
+  HIPC(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  int r = enqueue_step(s);
+  hipError_t e = hipStreamEndCapture(s->stream, &graph);
+  if (r != OSR_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return r;
+  }
+  if (e != hipSuccess) return fail(OSR_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&s->gexec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(OSR_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  s->graph_ok = true;
+  return OSR_OK;
+}
+
+int ensure_hist(osr_sim* s, int64_t rows) {
+  if (rows <= s->hist_cap) return OSR_OK;
+  int64_t cap = std::max<int64_t>(rows, std::max<int64_t>(2 * s->hist_cap, 1024));
+  const int ncol = 6 + (int)s->sp.size();
+  DBuf nb;
+  TRY(nb.alloc(sizeof(double) * (size_t)cap * ncol));
+  if (s->hist_cap > 0 && s->hist.p)
+    HIPC(hipMemcpy(nb.p, s->hist.p, sizeof(double) * (size_t)s->hist_cap * ncol,
+                   hipMemcpyDeviceToDevice));
+  std::swap(s->hist.p, nb.p);
+  std::swap(s->hist.n, nb.n);
+  s->hist_cap = cap;
+  return upload_descs(s);
+}
+
+int read_ctrl(osr_sim* s, Ctrl& c) {
+  HIPC(hipMemcpyAsync(s->h_ctrl, s->ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, s->stream));
+  HIPC(hipStreamSynchronize(s->stream));
+  c = *s->h_ctrl;
+  return OSR_OK;
+}
+
+int check_sim(osr_sim* s) {
+  if (!s) return fail(OSR_ERR_ARG, "null simulation handle");
+  return OSR_OK;
+}
+
+}  // namespace
+
+// ====================================================================================
+// C ABI
+// ====================================================================================
+extern "C" {
+
+const char* osr_last_error(void) { return g_err.c_str(); }
+int osr_version(void) { return 100; }
+
+int osr_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
+  if (!desc || !out) return fail(OSR_ERR_ARG, "null argument");
+  *out = nullptr;
+  const int dims = desc->dims;
+  if (dims != 2 && dims != 3) return fail(OSR_ERR_ARG, "dims must be 2 or 3");
+  if (desc->order < 1 || desc->order > 3) return fail(OSR_ERR_ARG, "order must be 1..3");
+  for (int d = 0; d < dims; d++)
+    if (desc->nx[d] < 1 || !(desc->dx[d] > 0)) return fail(OSR_ERR_ARG, "bad nx/dx");
+  if (dims == 2 && (desc->nx[0] > 65535 || desc->nx[1] > 65535))
+    return fail(OSR_ERR_ARG, "2D grids are limited to 65535 cells per dimension");
+  if (dims == 3 && (desc->nx[0] > 2047 || desc->nx[1] > 2047 || desc->nx[2] > 1023))
+    return fail(OSR_ERR_ARG, "3D grids are limited to 2047x2047x1023 cells");
+  if (!(desc->dt > 0)) return fail(OSR_ERR_ARG, "dt must be positive");
+  {  // test_stability_yee, os-emf-solver-yee.f03:430-456
+    double cour = 0;
+    for (int d = 0; d < dims; d++) cour += 1.0 / (desc->dx[d] * desc->dx[d]);
+    cour = std::sqrt(1.0 / cour);
+    if (desc->dt > cour)
+      return fail(OSR_ERR_ARG, "Yee stability condition violated: dt=%g > %g", desc->dt, cour);
+  }
+  if (osr_device_count() < 1) return fail(OSR_ERR_HIP, "no GPU visible");
+  std::unique_ptr<osr_sim> s(new osr_sim());
+  s->desc = *desc;
+  s->dims = dims;
+  s->order = desc->order;
+  s->dt = desc->dt;
+  int tile[3] = {1, 1, 1};
+  for (int d = 0; d < dims; d++) {
+    s->dx[d] = desc->dx[d];
+    tile[d] = desc->tile[d] > 0 ? desc->tile[d] : (dims == 2 ? 16 : (d < 2 ? 8 : 4));
+    tile[d] = std::min(tile[d], desc->nx[d]);
+  }
+  GridDesc& g = s->g;
+  g.dims = dims;
+  g.nx = desc->nx[0];
+  g.ny = desc->nx[1];
+  g.nz = dims == 3 ? desc->nx[2] : 1;
+  g.ncell = (long long)g.nx * g.ny * g.nz;
+  g.tx = tile[0];
+  g.ty = tile[1];
+  g.tz = tile[2];
+  g.ntx = ceil_div(g.nx, g.tx);
+  g.nty = ceil_div(g.ny, g.ty);
+  g.ntz = ceil_div(g.nz, g.tz);
+  g.ntiles = g.ntx * g.nty * g.ntz;
+  s->ntiles = g.ntiles;
+  for (int d = 0; d < 3; d++) {
+    g.dt_dx[d] = d < dims ? (float)(desc->dt / desc->dx[d]) : 0.f;
+    g.jnorm[d] = d < dims ? (float)(desc->dx[d] / desc->dt / (dims == 2 ? 2.0 : 3.0)) : 0.f;
+  }
+  s->lds_push = push_lds_bytes(dims, s->order, tile);
+  if (s->lds_push > 160 * 1024) return fail(OSR_ERR_ARG, "tile too large for LDS");
+  HIPC(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  const size_t nc = (size_t)g.ncell;
+  TRY(s->E.alloc(3 * nc * 4));
+  TRY(s->B.alloc(3 * nc * 4));
+  TRY(s->J.alloc(3 * nc * 8));
+  TRY(s->ctrl.alloc(sizeof(Ctrl)));
+  HIPC(hipMemset(s->E.p, 0, 3 * nc * 4));
+  HIPC(hipMemset(s->B.p, 0, 3 * nc * 4));
+  HIPC(hipMemset(s->J.p, 0, 3 * nc * 8));
+  HIPC(hipMemset(s->ctrl.p, 0, sizeof(Ctrl)));
+  HIPC(hipHostMalloc((void**)&s->h_ctrl, sizeof(Ctrl), hipHostMallocDefault));
+  HIPC(hipDeviceSynchronize());
+  g.E = s->E.as<float>();
+  g.B = s->B.as<float>();
+  g.J = s->J.as<unsigned long long>();
+  g.ctrl = s->ctrl.as<Ctrl>();
+  s->nblk = (int)((nc + 255) / 256);
+  TRY(s->fe_part.alloc(sizeof(double) * 6 * (size_t)s->nblk));
+  HIPC(hipMemset(s->fe_part.p, 0, sizeof(double) * 6 * (size_t)s->nblk));
+  TRY(s->d_spec.alloc(sizeof(SpecDesc) * kMaxSpecies));
+  TRY(s->d_fd.alloc(sizeof(FieldDesc)));
+  TRY(s->d_ywork.alloc(sizeof(int2) * (size_t)s->nblk));
+  TRY(s->d_fwork.alloc(sizeof(int)));
+  {
+    std::vector<int2> yw((size_t)s->nblk);
+    for (int b = 0; b < s->nblk; b++) yw[b] = make_int2(0, b);
+    HIPC(hipMemcpy(s->d_ywork.p, yw.data(), sizeof(int2) * yw.size(), hipMemcpyHostToDevice));
+    int z = 0;
+    HIPC(hipMemcpy(s->d_fwork.p, &z, sizeof(int), hipMemcpyHostToDevice));
+  }
+  update_jscale(s.get());
+  TRY(ensure_hist(s.get(), 1024));
+  if (s->lds_push > 64 * 1024) {
+    for (int o = 1; o <= 3; o++)
+      for (int p = 0; p <= 1; p++) {
+        push_fn f = pick_push(dims, o, p);
+        HIPC(hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)s->lds_push));
+      }
+  }
+  *out = s.release();
+  return OSR_OK;
+}
+
+int osr_sim_destroy(osr_sim* s) {
+  if (!s) return OSR_OK;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->h_ctrl) (void)hipHostFree(s->h_ctrl);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return OSR_OK;
+}
+
+int osr_sim_add_species(osr_sim* s, float rqm, int pusher, int* sid) {
+  TRY(check_sim(s));
+  if ((int)s->sp.size() >= kMaxSpecies) return fail(OSR_ERR_ARG, "too many species");
+  if (pusher != OSR_BORIS && pusher != OSR_VAY) return fail(OSR_ERR_ARG, "unknown pusher");
+  if (!(rqm != 0.f)) return fail(OSR_ERR_ARG, "rqm must be non-zero");
+  std::unique_ptr<Species> p(new Species());
+  p->rqm = rqm;
+  p->pusher = pusher;
+  std::vector<int> cnt(s->ntiles, 0);
+  TRY(alloc_species(s, *p, cnt));
+  s->sp.push_back(std::move(p));
+  // the work list covers (species, tile) pairs
+  const int ns = (int)s->sp.size();
+  TRY(s->d_work.alloc(sizeof(int2) * (size_t)ns * s->ntiles));
+  std::vector<int2> w((size_t)ns * s->ntiles);
+  for (int i = 0; i < ns; i++)
+    for (int t = 0; t < s->ntiles; t++) w[(size_t)i * s->ntiles + t] = make_int2(i, t);
+  HIPC(hipMemcpy(s->d_work.p, w.data(), sizeof(int2) * w.size(), hipMemcpyHostToDevice));
+  // history rows gain a column: restart the history buffer
+  s->hist.release();
+  s->hist_cap = 0;
+  TRY(ensure_hist(s, std::max<int64_t>(1024, s->steps + 1)));
+  s->graph_ok = false;
+  if (sid) *sid = ns - 1;
+  return OSR_OK;
+}
+
+int osr_species_set(osr_sim* s, int sid, int64_t np, const int* ix, const float* x,
+                    const float* u, const float* q) {
+  TRY(check_sim(s));
+  if (sid < 0 || sid >= (int)s->sp.size()) return fail(OSR_ERR_ARG, "bad species id");
+  if (np < 0 || np >= (int64_t)UINT32_MAX) return fail(OSR_ERR_ARG, "bad particle count");
+  if (np > 0 && (!ix || !x || !u || !q)) return fail(OSR_ERR_ARG, "null particle array");
+  const int D = s->dims;
+  const int nxs[3] = {s->g.nx, s->g.ny, s->g.nz};
+  HostParts hp;
+  hp.reserve((size_t)np);
+  for (int64_t i = 0; i < np; i++) {
+    int c[3] = {0, 0, 0};
+    for (int d = 0; d < D; d++) {
+      c[d] = ix[D * i + d] - 1;
+      const float xv = x[D * i + d];
+      if (c[d] < 0 || c[d] >= nxs[d] || !(xv >= -0.5f && xv < 0.5f))
+        return fail(OSR_ERR_ARG, "particle %lld outside the grid or x not in [-0.5,0.5)",
+                    (long long)i);
+    }
+    hp.cell.push_back(D == 2 ? pack2(c[0], c[1]) : pack3(c[0], c[1], c[2]));
+    hp.pid.push_back((uint32_t)i);
+    hp.x.push_back(x[D * i]);
+    hp.y.push_back(x[D * i + 1]);
+    hp.z.push_back(D == 3 ? x[D * i + 2] : 0.f);
+    hp.ux.push_back(u[3 * i]);
+    hp.uy.push_back(u[3 * i + 1]);
+    hp.uz.push_back(u[3 * i + 2]);
+    hp.q.push_back(q[i]);
+  }
+  HIPC(hipStreamSynchronize(s->stream));
+  TRY(layout_upload(s, *s->sp[sid], hp));
+  update_jscale(s);
+  TRY(upload_descs(s));
+  s->graph_ok = false;
+  return OSR_OK;
+}
+
+int osr_species_count(osr_sim* s, int sid, int64_t* np) {
+  TRY(check_sim(s));
+  if (sid < 0 || sid >= (int)s->sp.size() || !np) return fail(OSR_ERR_ARG, "bad argument");
+  *np = s->sp[sid]->np;
+  return OSR_OK;
+}
+
+int osr_species_get(osr_sim* s, int sid, int* ix, float* x, float* u, float* q) {
+  TRY(check_sim(s));
+  if (sid < 0 || sid >= (int)s->sp.size()) return fail(OSR_ERR_ARG, "bad species id");
+  HIPC(hipStreamSynchronize(s->stream));
+  HostParts hp;
+  TRY(collect(s, *s->sp[sid], false, hp));
+  sort_by_pid(hp);
+  const int D = s->dims;
+  for (size_t i = 0; i < hp.size(); i++) {
+    int c[3];
+    cell_coords(s, hp.cell[i], c);
+    if (ix)
+      for (int d = 0; d < D; d++) ix[D * i + d] = c[d] + 1;
+    if (x) {
+      x[D * i] = hp.x[i];
+      x[D * i + 1] = hp.y[i];
+      if (D == 3) x[D * i + 2] = hp.z[i];
+    }
+    if (u) {
+      u[3 * i] = hp.ux[i];
+      u[3 * i + 1] = hp.uy[i];
+      u[3 * i + 2] = hp.uz[i];
+    }
+    if (q) q[i] = hp.q[i];
+  }
+  return OSR_OK;
+}
+
+int osr_species_generate(osr_sim* s, int sid, const int* ppc, double density,
+                         const double* ufl, const double* uth, uint64_t seed, int random_pos) {
+  TRY(check_sim(s));
+  if (sid < 0 || sid >= (int)s->sp.size() || !ppc || !ufl || !uth)
+    return fail(OSR_ERR_ARG, "bad argument");
+  int tot = 1;
+  for (int d = 0; d < s->dims; d++) {
+    if (ppc[d] < 1) return fail(OSR_ERR_ARG, "ppc must be >= 1");
+    tot *= ppc[d];
+  }
+  if ((double)tot * (double)s->g.ncell >= 4294967295.0)
+    return fail(OSR_ERR_ARG, "too many particles for 32-bit particle ids");
+  Species& p = *s->sp[sid];
+  std::vector<int> cnt(s->ntiles);
+  for (int t = 0; t < s->ntiles; t++) {
+    const int tx = t % s->g.ntx, ty = (t / s->g.ntx) % s->g.nty, tz = t / (s->g.ntx * s->g.nty);
+    const int wx = std::min(s->g.tx, s->g.nx - tx * s->g.tx);
+    const int wy = std::min(s->g.ty, s->g.ny - ty * s->g.ty);
+    const int wz = s->dims == 3 ? std::min(s->g.tz, s->g.nz - tz * s->g.tz) : 1;
+    cnt[t] = wx * wy * wz * tot;
+  }
+  HIPC(hipStreamSynchronize(s->stream));
+  TRY(alloc_species(s, p, cnt));
+  const float q = (float)((p.rqm < 0 ? -1.0 : 1.0) * density / tot);
+  p.qabs = std::fabs((double)q) * (double)p.np;
+  p.qmax = std::fabs((double)q);
+  update_jscale(s);
+  TRY(upload_descs(s));
+  GenArgs a{};
+  for (int d = 0; d < 3; d++) {
+    a.ppc[d] = d < s->dims ? ppc[d] : 1;
+    a.ufl[d] = ufl[d];
+    a.uth[d] = uth[d];
+  }
+  a.ppc_tot = tot;
+  a.q = q;
+  a.seed = seed;
+  a.random_pos = random_pos;
+  hipLaunchKernelGGL(generate_particles, dim3(s->ntiles), dim3(256), 0, s->stream,
+                     s->d_spec.as<SpecDesc>(), sid, a);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(s->stream));
+  s->graph_ok = false;
+  return OSR_OK;
+}
+
+static int fields_xfer(osr_sim* s, float* e, float* b, const int* ld, const int* gcl, bool set) {
+  const int D = s->dims;
+  const int nxs[3] = {s->g.nx, s->g.ny, s->g.nz};
+  for (int d = 0; d < D; d++)
+    if (ld[d] < nxs[d] + gcl[d] || gcl[d] < 0) return fail(OSR_ERR_ARG, "bad field extents");
+  const size_t nc = (size_t)s->g.ncell;
+  std::vector<float> pe(3 * nc), pb(3 * nc);
+  const long ld0 = ld[0], ld1 = ld[1];
+  auto ref = [&](int i, int j, int k) -> size_t {
+    size_t o = (size_t)(i + gcl[0]) + (size_t)ld0 * (size_t)(j + gcl[1]);
+    if (D == 3) o += (size_t)ld0 * ld1 * (size_t)(k + gcl[2]);
+    return 3 * o;
+  };
+  if (set) {
+    for (int k = 0; k < nxs[2]; k++)
+      for (int j = 0; j < nxs[1]; j++)
+        for (int i = 0; i < nxs[0]; i++) {
+          const size_t c = (size_t)i + (size_t)nxs[0] * (j + (size_t)nxs[1] * k);
+          const size_t r = ref(i, j, k);
+          for (int m = 0; m < 3; m++) {
+            pe[m * nc + c] = e ? e[r + m] : 0.f;
+            pb[m * nc + c] = b ? b[r + m] : 0.f;
+          }
+        }
+    HIPC(hipStreamSynchronize(s->stream));
+    if (e) HIPC(hipMemcpy(s->E.p, pe.data(), 12 * nc, hipMemcpyHostToDevice));
+    if (b) HIPC(hipMemcpy(s->B.p, pb.data(), 12 * nc, hipMemcpyHostToDevice));
+    return OSR_OK;
+  }
+  HIPC(hipStreamSynchronize(s->stream));
+  HIPC(hipMemcpy(pe.data(), s->E.p, 12 * nc, hipMemcpyDeviceToHost));
+  HIPC(hipMemcpy(pb.data(), s->B.p, 12 * nc, hipMemcpyDeviceToHost));
+  // every array cell (guards included) gets its periodic image
+  const int ldk = D == 3 ? ld[2] : 1;
+  for (int k = 0; k < ldk; k++)
+    for (int j = 0; j < ld[1]; j++)
+      for (int i = 0; i < ld[0]; i++) {
+        const int ii = ((i - gcl[0]) % nxs[0] + nxs[0]) % nxs[0];
+        const int jj = ((j - gcl[1]) % nxs[1] + nxs[1]) % nxs[1];
+        const int kk = D == 3 ? ((k - gcl[2]) % nxs[2] + nxs[2]) % nxs[2] : 0;
+        const size_t c = (size_t)ii + (size_t)nxs[0] * (jj + (size_t)nxs[1] * kk);
+        const size_t r = 3 * ((size_t)i + (size_t)ld0 * (j + (size_t)ld1 * k));
+        for (int m = 0; m < 3; m++) {
+          if (e) e[r + m] = pe[m * nc + c];
+          if (b) b[r + m] = pb[m * nc + c];
+        }
+      }
+  return OSR_OK;
+}
+
+int osr_fields_set(osr_sim* s, const float* e, const float* b, const int* ld, const int* gcl) {
+  TRY(check_sim(s));
+  if (!ld || !gcl) return fail(OSR_ERR_ARG, "null extents");
+  return fields_xfer(s, (float*)e, (float*)b, ld, gcl, true);
+}
+
+int osr_fields_get(osr_sim* s, float* e, float* b, const int* ld, const int* gcl) {
+  TRY(check_sim(s));
+  if (!ld || !gcl) return fail(OSR_ERR_ARG, "null extents");
+  return fields_xfer(s, e, b, ld, gcl, false);
+}
+
+int osr_fields_fill(osr_sim* s, const double* e0, const double* b0) {
+  TRY(check_sim(s));
+  const size_t nc = (size_t)s->g.ncell;
+  std::vector<float> v(3 * nc);
+  HIPC(hipStreamSynchronize(s->stream));
+  for (int w = 0; w < 2; w++) {
+    const double* f = w == 0 ? e0 : b0;
+    if (!f) continue;
+    for (int m = 0; m < 3; m++) std::fill(v.begin() + m * nc, v.begin() + (m + 1) * nc, (float)f[m]);
+    HIPC(hipMemcpy(w == 0 ? s->E.p : s->B.p, v.data(), 12 * nc, hipMemcpyHostToDevice));
+  }
+  return OSR_OK;
+}
+
+int osr_sim_run(osr_sim* s, int nsteps) {
+  TRY(check_sim(s));
+  if (nsteps < 0) return fail(OSR_ERR_ARG, "negative step count");
+  if (nsteps == 0) return OSR_OK;
+  TRY(ensure_hist(s, s->steps + nsteps));
+  if (!s->graph_ok) TRY(build_graph(s));
+  int64_t left = nsteps;
+  int batch = 8;
+  while (left > 0) {
+    const int nb = (int)std::min<int64_t>(left, batch);
+    for (int i = 0; i < nb; i++) HIPC(hipGraphLaunch(s->gexec, s->stream));
+    Ctrl c;
+    TRY(read_ctrl(s, c));
+    if (c.error) return fail(OSR_ERR_LAYOUT, "particle found outside its tile");
+    const int64_t done = c.steps - s->steps;
+    s->steps = c.steps;
+    left -= done;
+    if (c.halt == 2) {
+      TRY(relayout(s, true));
+      Ctrl z = c;
+      z.halt = 0;
+      HIPC(hipMemcpy(s->ctrl.p, &z, sizeof(Ctrl), hipMemcpyHostToDevice));
+      batch = 8;
+    } else if (done != nb) {
+      return fail(OSR_ERR_STATE, "step counter advanced by %lld instead of %d",
+                  (long long)done, nb);
+    } else {
+      batch = std::min(batch * 2, 256);
+    }
+  }
+  return OSR_OK;
+}
+
+int osr_sim_steps(osr_sim* s, int64_t* steps) {
+  TRY(check_sim(s));
+  if (!steps) return fail(OSR_ERR_ARG, "null argument");
+  *steps = s->steps;
+  return OSR_OK;
+}
+
+int osr_sim_history(osr_sim* s, int64_t first, int64_t count, double* fld, double* par) {
+  TRY(check_sim(s));
+  if (first < 0 || count < 0 || first + count > s->steps)
+    return fail(OSR_ERR_ARG, "history rows out of range");
+  const int ns = (int)s->sp.size(), ncol = 6 + ns;
+  std::vector<double> rows((size_t)count * ncol);
+  HIPC(hipStreamSynchronize(s->stream));
+  if (count)
+    HIPC(hipMemcpy(rows.data(), s->hist.as<double>() + first * ncol,
+                   sizeof(double) * rows.size(), hipMemcpyDeviceToHost));
+  for (int64_t r = 0; r < count; r++) {
+    if (fld)
+      for (int c = 0; c < 6; c++) fld[6 * r + c] = rows[r * ncol + c];
+    if (par)
+      for (int c = 0; c < ns; c++) par[ns * r + c] = rows[r * ncol + 6 + c];
+  }
+  return OSR_OK;
+}
+
+int osr_sim_gauss(osr_sim* s, double* res, double* maxabs) {
+  TRY(check_sim(s));
+  const size_t nc = (size_t)s->g.ncell;
+  DBuf rho, dres;
+  TRY(rho.alloc(8 * nc));
+  TRY(dres.alloc(8 * nc));
+  HIPC(hipStreamSynchronize(s->stream));
+  HIPC(hipMemset(rho.p, 0, 8 * nc));
+  HIPC(hipDeviceSynchronize());
+  double qabs = 0;
+  for (auto& p : s->sp) qabs += p->qabs;
+  const int sh = 62 - (int)std::ceil(std::log2(std::max(qabs * 1.01, 1e-300)));
+  const double scale = std::ldexp(1.0, std::max(0, std::min(sh, 100)));
+  for (size_t i = 0; i < s->sp.size(); i++) {
+    typedef void (*rf)(const SpecDesc*, const int2*, unsigned long long*, double);
+    rf f = s->order == 1 ? rho_deposit<1> : (s->order == 2 ? rho_deposit<2> : rho_deposit<3>);
+    hipLaunchKernelGGL(f, dim3(s->ntiles), dim3(256), 0, s->stream, s->d_spec.as<SpecDesc>(),
+                       s->d_work.as<int2>() + i * s->ntiles, rho.as<unsigned long long>(), scale);
+    HIPC(hipGetLastError());
+  }
+  hipLaunchKernelGGL(gauss_residual, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s->stream,
+                     s->g, rho.as<unsigned long long>(), 1.0 / scale, 1.0 / s->dx[0],
+                     1.0 / s->dx[1], s->dims == 3 ? 1.0 / s->dx[2] : 0.0, dres.as<double>());
+  HIPC(hipGetLastError());
+  std::vector<double> h(nc);
+  HIPC(hipStreamSynchronize(s->stream));
+  HIPC(hipMemcpy(h.data(), dres.p, 8 * nc, hipMemcpyDeviceToHost));
+  double m = 0;
+  for (double v : h) m = std::max(m, std::fabs(v));
+  if (maxabs) *maxabs = m;
+  if (res) std::memcpy(res, h.data(), 8 * nc);
+  return OSR_OK;
+}
+
+int osr_sim_relayouts(osr_sim* s, int64_t* count) {
+  TRY(check_sim(s));
+  if (!count) return fail(OSR_ERR_ARG, "null argument");
+  *count = s->relayouts;
+  return OSR_OK;
+}
+
+int osr_sim_footprint(osr_sim* s, int64_t* bytes) {
+  TRY(check_sim(s));
+  if (!bytes) return fail(OSR_ERR_ARG, "null argument");
+  size_t b = s->E.n + s->B.n + s->J.n + s->fe_part.n + s->hist.n;
+  for (auto& p : s->sp) {
+    const DBuf* all[] = {&p->cell, &p->x, &p->y, &p->z, &p->ux, &p->uy, &p->uz, &p->q, &p->pid,
+                         &p->mcell, &p->mx, &p->my, &p->mz, &p->mux, &p->muy, &p->muz, &p->mq,
+                         &p->mpid, &p->midx, &p->off, &p->cnt, &p->cap, &p->mv_cnt,
+                         &p->mv_stride, &p->newcnt, &p->inc, &p->epart};
+    for (const DBuf* d : all) b += d->n;
+  }
+  *bytes = (int64_t)b;
+  return OSR_OK;
+}
+
+int osr_sim_time_kernels(osr_sim* s, int nsteps, double* ms) {
+  TRY(check_sim(s));
+  if (nsteps < 1 || !ms) return fail(OSR_ERR_ARG, "bad argument");
+  TRY(ensure_hist(s, s->steps + nsteps));
+  const int nev = 6;
+  hipEvent_t ev[nev];
+  for (int i = 0; i < nev; i++) HIPC(hipEventCreate(&ev[i]));
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  int done = 0;
+  for (int k = 0; k < nsteps; k++) {
+    HIPC(hipEventRecord(ev[0], s->stream));
+    for (size_t i = 0; i < s->sp.size(); i++) TRY(launch_push(s, (int)i));
+    HIPC(hipEventRecord(ev[1], s->stream));
+    if (!s->sp.empty()) TRY(launch_tiles(s));
+    HIPC(hipEventRecord(ev[2], s->stream));
+    if (!s->sp.empty()) TRY(launch_fill(s));
+    HIPC(hipEventRecord(ev[3], s->stream));
+    TRY(launch_yee(s));
+    HIPC(hipEventRecord(ev[4], s->stream));
+    TRY(launch_finalize(s));
+    HIPC(hipEventRecord(ev[5], s->stream));
+    HIPC(hipEventSynchronize(ev[5]));
+    Ctrl c;
+    TRY(read_ctrl(s, c));
+    if (c.error) return fail(OSR_ERR_LAYOUT, "particle found outside its tile");
+    s->steps = c.steps;
+    if (c.halt == 2) {
+      TRY(relayout(s, true));
+      Ctrl z = c;
+      z.halt = 0;
+      HIPC(hipMemcpy(s->ctrl.p, &z, sizeof(Ctrl), hipMemcpyHostToDevice));
+      continue;  // a re-laid-out step is not representative
+    }
+    float t;
+    for (int i = 0; i < 5; i++) {
+      HIPC(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
+      acc[i] += t;
+    }
+    HIPC(hipEventElapsedTime(&t, ev[0], ev[5]));
+    acc[5] += t;
+    done++;
+  }
+  for (int i = 0; i < nev; i++) (void)hipEventDestroy(ev[i]);
+  for (int i = 0; i < 6; i++) ms[i] = done ? acc[i] / done : 0.0;
+  return OSR_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// stateless drop-ins
+// ------------------------------------------------------------------------------------
+namespace {
+struct Pool {
+  DBuf b[8];
+  int get(int i, size_t bytes) {
+    if (b[i].n >= bytes && b[i].p) return OSR_OK;
+    return b[i].alloc(bytes);
+  }
+};
+thread_local Pool g_pool;
+}  // namespace
+
+int osr_advance_deposit(int dims, int order, int pusher, int* ix, float* x, float* u,
+                        const float* q, int i0, int i1, float rqm, const float* efield,
+                        const float* bfield, const int* emf_size, const int* emf_offset,
+                        float* j, const int* j_size, const int* j_offset, const double* dx,
+                        double dt, double* ene) {
+  if (dims != 2 && dims != 3) return fail(OSR_ERR_ARG, "dims must be 2 or 3");
+  if (order < 1 || order > 3) return fail(OSR_ERR_ARG, "order must be 1..3");
+  if (pusher != OSR_BORIS && pusher != OSR_VAY) return fail(OSR_ERR_ARG, "unknown pusher");
+  if (!ix || !x || !u || !q || !efield || !bfield || !emf_size || !emf_offset || !j || !j_size ||
+      !j_offset || !dx || !ene)
+    return fail(OSR_ERR_ARG, "null argument");
+  if (i0 < 1 || i1 < i0 - 1) return fail(OSR_ERR_ARG, "bad particle range");
+  const int np = i1 - i0 + 1;
+  const int lp = order == 1 ? 0 : -1;
+  size_t ne = 3, nj = 3;
+  for (int d = 0; d < dims; d++) {
+    ne *= (size_t)emf_size[d];
+    nj *= (size_t)j_size[d];
+  }
+  // range check of cell indices on the host: gather reaches ix-1+lp .. ix+lp+order,
+  // the deposit ix-1+lp .. ix+1+lp+order
+  const int* pix = ix + (size_t)dims * (i0 - 1);
+  for (int i = 0; i < np; i++)
+    for (int d = 0; d < dims; d++) {
+      const int c = pix[(size_t)dims * i + d];
+      const int glo = c - 1 + lp, ghi = c + lp + order, jlo = c - 1 + lp, jhi = c + 1 + lp + order;
+      if (glo + emf_offset[d] < 1 || ghi + emf_offset[d] > emf_size[d] || jlo + j_offset[d] < 1 ||
+          jhi + j_offset[d] > j_size[d])
+        return fail(OSR_ERR_ARG, "particle %d stencil leaves the guarded arrays", i0 + i);
+    }
+  Pool& P = g_pool;
+  TRY(P.get(0, 4 * (size_t)dims * np + 4));
+  TRY(P.get(1, 4 * (size_t)dims * np + 4));
+  TRY(P.get(2, 12 * (size_t)np + 4));
+  TRY(P.get(3, 4 * (size_t)np + 4));
+  TRY(P.get(4, 4 * ne));
+  TRY(P.get(5, 4 * ne));
+  TRY(P.get(6, 8 * nj));
+  const int nblk = (np + 255) / 256;
+  TRY(P.get(7, 8 * (size_t)(nblk > 0 ? nblk : 1) + 4 * nj));
+  float* dj = (float*)((char*)P.b[7].p + 8 * (size_t)(nblk > 0 ? nblk : 1));
+  if (np > 0) {
+    HIPC(hipMemcpy(P.b[0].p, pix, 4 * (size_t)dims * np, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(P.b[1].p, x + (size_t)dims * (i0 - 1), 4 * (size_t)dims * np, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(P.b[2].p, u + (size_t)3 * (i0 - 1), 12 * (size_t)np, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(P.b[3].p, q + (i0 - 1), 4 * (size_t)np, hipMemcpyHostToDevice));
+  }
+  HIPC(hipMemcpy(P.b[4].p, efield, 4 * ne, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(P.b[5].p, bfield, 4 * ne, hipMemcpyHostToDevice));
+  HIPC(hipMemset(P.b[6].p, 0, 8 * nj));
+  HIPC(hipMemcpy(dj, j, 4 * nj, hipMemcpyHostToDevice));
+  DropinArgs a{};
+  a.ix = P.b[0].as<int>();
+  a.x = P.b[1].as<float>();
+  a.u = P.b[2].as<float>();
+  a.q = P.b[3].as<float>();
+  a.np = np;
+  a.e = P.b[4].as<float>();
+  a.b = P.b[5].as<float>();
+  for (int d = 0; d < 3; d++) {
+    a.emf_ld[d] = d < dims ? emf_size[d] : 1;
+    a.emf_off[d] = d < dims ? emf_offset[d] : 0;
+    a.j_ld[d] = d < dims ? j_size[d] : 1;
+    a.j_off[d] = d < dims ? j_offset[d] : 0;
+    a.dt_dx[d] = d < dims ? (float)(dt / dx[d]) : 0.f;
+    a.jnorm[d] = d < dims ? (float)(dx[d] / dt / (dims == 2 ? 2.0 : 3.0)) : 0.f;
+  }
+  a.jfix = P.b[6].as<unsigned long long>();
+  a.tem = (float)(0.5 * dt / (double)rqm);
+  // fixed-point scale from this call's charges (same bound as the resident path)
+  double qabs = 0, qmax = 0;
+  for (int i = 0; i < np; i++) {
+    qabs += std::fabs((double)q[i0 - 1 + i]);
+    qmax = std::max(qmax, std::fabs((double)q[i0 - 1 + i]));
+  }
+  double jn = 0;
+  for (int d = 0; d < dims; d++) jn += a.jnorm[d];
+  const double per = (dims == 2 ? 2.0 : 3.0) * jn + 1.0;
+  int sh = std::min(62 - (int)std::ceil(std::log2(std::max(qabs * per * 8.0, 1e-300))),
+                    50 - (int)std::ceil(std::log2(std::max(qmax * per * 2.0, 1e-300))));
+  sh = std::max(0, std::min(sh, 80));
+  a.jscale = std::ldexp(1.0, sh);
+  a.epart = P.b[7].as<double>();
+  if (np > 0) {
+    typedef void (*df)(DropinArgs);
+    df f = nullptr;
+#define OSR_D(D, O, PP) \
+  if (dims == D && order == O && pusher == PP) f = dropin_push<O, PP, D>;
+    OSR_D(2, 1, 0) OSR_D(2, 2, 0) OSR_D(2, 3, 0) OSR_D(2, 1, 1) OSR_D(2, 2, 1) OSR_D(2, 3, 1)
+    OSR_D(3, 1, 0) OSR_D(3, 2, 0) OSR_D(3, 3, 0) OSR_D(3, 1, 1) OSR_D(3, 2, 1) OSR_D(3, 3, 1)
+#undef OSR_D
+    hipLaunchKernelGGL(f, dim3(nblk), dim3(256), 0, 0, a);
+    HIPC(hipGetLastError());
+  }
+  hipLaunchKernelGGL(dropin_jadd, dim3((unsigned)((nj + 255) / 256)), dim3(256), 0, 0, dj,
+                     a.jfix, (long long)nj, std::ldexp(1.0, -sh));
+  HIPC(hipGetLastError());
+  HIPC(hipDeviceSynchronize());
+  if (np > 0) {
+    HIPC(hipMemcpy(ix + (size_t)dims * (i0 - 1), P.b[0].p, 4 * (size_t)dims * np, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(x + (size_t)dims * (i0 - 1), P.b[1].p, 4 * (size_t)dims * np, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(u + (size_t)3 * (i0 - 1), P.b[2].p, 12 * (size_t)np, hipMemcpyDeviceToHost));
+    std::vector<double> ep((size_t)nblk);
+    HIPC(hipMemcpy(ep.data(), P.b[7].p, 8 * (size_t)nblk, hipMemcpyDeviceToHost));
+    double e = 0;
+    for (double v : ep) e += v;
+    *ene += e;
+  }
+  HIPC(hipMemcpy(j, dj, 4 * nj, hipMemcpyDeviceToHost));
+  return OSR_OK;
+}
+
+#define OSR_FORTRAN_DROPIN(D, O)                                                            \
+  void hip_vadvance_deposit_##D##d_s##O##_(int* ix, float* x, float* u, float* q, int* i0, \
+                                           int* i1, float* rqm, float* efield, float* bfield, \
+                                           int* emf_size, int* emf_offset, float* j,          \
+                                           int* j_size, int* j_offset, double* dx, double* dt, \
+                                           double* ene) {                                     \
+    int r = osr_advance_deposit(D, O, OSR_BORIS, ix, x, u, q, *i0, *i1, *rqm, efield, bfield, \
+                                emf_size, emf_offset, j, j_size, j_offset, dx, *dt, ene);     \
+    if (r != OSR_OK) {                                                                      \
+      printf("(*error*) hip_vadvance_deposit_%dd_s%d: %s\n", D, O, osr_last_error());        \
+      exit(1);                                                                              \
+    }                                                                                       \
+  }
+OSR_FORTRAN_DROPIN(2, 1)
+OSR_FORTRAN_DROPIN(2, 2)
+OSR_FORTRAN_DROPIN(2, 3)
+OSR_FORTRAN_DROPIN(3, 1)
+OSR_FORTRAN_DROPIN(3, 2)
+OSR_FORTRAN_DROPIN(3, 3)
+
+int osr_yee_advance(int dims, const int* nx, const int* ld, const int* gcl, const double* dx,
+                    double dt, float* e, float* b, const float* jay) {
+  if ((dims != 2 && dims != 3) || !nx || !ld || !gcl || !dx || !e || !b || !jay)
+    return fail(OSR_ERR_ARG, "bad argument");
+  const int nxs[3] = {nx[0], nx[1], dims == 3 ? nx[2] : 1};
+  for (int d = 0; d < dims; d++)
+    if (nx[d] < 1 || gcl[d] < 1 || ld[d] < nx[d] + gcl[d] + 2)
+      return fail(OSR_ERR_ARG, "Yee needs 1 lower and 2 upper guard cells");
+  const size_t nc = (size_t)nxs[0] * nxs[1] * nxs[2];
+  std::vector<float> pe(3 * nc), pb(3 * nc), pj(3 * nc);
+  const size_t ld0 = ld[0], ld1 = ld[1];
+  auto ref = [&](int i, int j, int k) {
+    return 3 * ((size_t)(i + gcl[0]) + ld0 * ((size_t)(j + gcl[1]) + (dims == 3 ? ld1 * (size_t)(k + gcl[2]) : 0)));
+  };
+  for (int k = 0; k < nxs[2]; k++)
+    for (int j = 0; j < nxs[1]; j++)
+      for (int i = 0; i < nxs[0]; i++) {
+        const size_t c = (size_t)i + (size_t)nxs[0] * (j + (size_t)nxs[1] * k), r = ref(i, j, k);
+        for (int m = 0; m < 3; m++) {
+          pe[m * nc + c] = e[r + m];
+          pb[m * nc + c] = b[r + m];
+          pj[m * nc + c] = jay[r + m];
+        }
+      }
+  DBuf dE, dB, dJ, dctrl, dfd, dw, dfe;
+  TRY(dE.alloc(12 * nc));
+  TRY(dB.alloc(12 * nc));
+  TRY(dJ.alloc(12 * nc));
+  TRY(dctrl.alloc(sizeof(Ctrl)));
+  TRY(dfd.alloc(sizeof(FieldDesc)));
+  const int nblk = (int)((nc + 255) / 256);
+  TRY(dw.alloc(sizeof(int2) * (size_t)nblk));
+  TRY(dfe.alloc(48 * (size_t)nblk));
+  HIPC(hipMemcpy(dE.p, pe.data(), 12 * nc, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(dB.p, pb.data(), 12 * nc, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(dJ.p, pj.data(), 12 * nc, hipMemcpyHostToDevice));
+  HIPC(hipMemset(dctrl.p, 0, sizeof(Ctrl)));
+  FieldDesc f{};
+  f.g.dims = dims;
+  f.g.nx = nxs[0];
+  f.g.ny = nxs[1];
+  f.g.nz = nxs[2];
+  f.g.ncell = (long long)nc;
+  f.g.E = dE.as<float>();
+  f.g.B = dB.as<float>();
+  f.g.ctrl = dctrl.as<Ctrl>();
+  f.Jf = dJ.as<float>();
+  f.fe_part = dfe.as<double>();
+  f.nblk = nblk;
+  for (int d = 0; d < 3; d++) {
+    f.bdt[d] = d < dims ? (float)((dt / 2.0) / dx[d]) : 0.f;
+    f.edt[d] = d < dims ? (float)(dt / dx[d]) : 0.f;
+  }
+  f.dtif = (float)dt;
+  HIPC(hipMemcpy(dfd.p, &f, sizeof(FieldDesc), hipMemcpyHostToDevice));
+  std::vector<int2> w((size_t)nblk);
+  for (int i = 0; i < nblk; i++) w[i] = make_int2(0, i);
+  HIPC(hipMemcpy(dw.p, w.data(), sizeof(int2) * w.size(), hipMemcpyHostToDevice));
+  const FieldDesc* fd = dfd.as<FieldDesc>();
+  const int2* wk = dw.as<int2>();
+  if (dims == 2) {
+    hipLaunchKernelGGL((yee_dbdt<2, false>), dim3(nblk), dim3(256), 0, 0, fd, wk);
+    hipLaunchKernelGGL((yee_dedt<2>), dim3(nblk), dim3(256), 0, 0, fd, wk);
+    hipLaunchKernelGGL((yee_dbdt<2, false>), dim3(nblk), dim3(256), 0, 0, fd, wk);
+  } else {
+    hipLaunchKernelGGL((yee_dbdt<3, false>), dim3(nblk), dim3(256), 0, 0, fd, wk);
+    hipLaunchKernelGGL((yee_dedt<3>), dim3(nblk), dim3(256), 0, 0, fd, wk);
+    hipLaunchKernelGGL((yee_dbdt<3, false>), dim3(nblk), dim3(256), 0, 0, fd, wk);
+  }
+  HIPC(hipGetLastError());
+  HIPC(hipDeviceSynchronize());
+  HIPC(hipMemcpy(pe.data(), dE.p, 12 * nc, hipMemcpyDeviceToHost));
+  HIPC(hipMemcpy(pb.data(), dB.p, 12 * nc, hipMemcpyDeviceToHost));
+  // write back interior and periodic images into every guard cell
+  const int ldk = dims == 3 ? ld[2] : 1;
+  for (int k = 0; k < ldk; k++)
+    for (int j = 0; j < ld[1]; j++)
+      for (int i = 0; i < ld[0]; i++) {
+        const int ii = ((i - gcl[0]) % nxs[0] + nxs[0]) % nxs[0];
+        const int jj = ((j - gcl[1]) % nxs[1] + nxs[1]) % nxs[1];
+        const int kk = dims == 3 ? ((k - gcl[2]) % nxs[2] + nxs[2]) % nxs[2] : 0;
+        const size_t c = (size_t)ii + (size_t)nxs[0] * (jj + (size_t)nxs[1] * kk);
+        const size_t r = 3 * ((size_t)i + ld0 * (j + ld1 * (size_t)k));
+        for (int m = 0; m < 3; m++) {
+          e[r + m] = pe[m * nc + c];
+          b[r + m] = pb[m * nc + c];
+        }
+      }
+  return OSR_OK;
+}
+
+}  // extern "C"

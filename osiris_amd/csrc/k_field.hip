@@ -1,0 +1,312 @@
+// k_field.hip -- Yee field advance, current conversion, per-step diagnostics.
+//
+// Yee: emf/solver/os-emf-solver-yee.f03 dbdt_2d :102-138, dbdt_3d :211-251,
+// dedt_2d :287-326, dedt_3d :385-428, advance_yee :542-568.  Periodic boundaries are
+// index wraps on the interior arrays, which reproduces the reference's guard-cell
+// refresh (update_periodic_vdf, vdf/os-vdf-comm.f90:142-330) on every interior cell.
+// The arithmetic keeps the reference's operation order in FP32 (no contraction).
+//
+// The three sub-steps are memory-bound stencils over planar component arrays: one thread
+// per cell, consecutive threads on consecutive x (coalesced).  The E step also converts
+// the fixed-point current to float and clears it for the next deposit, and the first B
+// step accumulates the field energy of E^n, B^n (the fields the particles were pushed
+// with) into per-block FP64 partial sums.
+#include "osr_device.hpp"
+#include "osr_types.hpp"
+
+namespace osr {
+
+struct FieldDesc {
+  GridDesc g;
+  double* fe_part;   // [nblk][6] B1,B2,B3,E1,E2,E3 partial sums of squares
+  int nblk;
+  // finalisation
+  int nspec;
+  double* ep[kMaxSpecies];  // per-species tile energy partials
+  int ep_n[kMaxSpecies];
+  double ep_scale[kMaxSpecies];  // rqm * dvol
+  double fe_scale;               // 0.5 * dvol
+  double* hist;                  // rows of (6 + nspec) doubles
+  long long hist_cap;
+  // Yee coefficients: real(dt/dx) for the half B steps and the full E step, real(dt)
+  float bdt[3];
+  float edt[3];
+  float dtif;
+  // when set, the E step reads a float current from here instead of the fixed-point J
+  const float* Jf;
+};
+
+OSR_DEV int wrp(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+template <int DIMS, bool ENERGY>
+__global__ __launch_bounds__(256) void yee_dbdt(const FieldDesc* __restrict__ fds,
+                                                const int2* __restrict__ work) {
+  __shared__ double s_red[4][6];
+  const int2 wk = work[blockIdx.x];
+  const FieldDesc& fd = fds[wk.x];
+  const GridDesc& g = fd.g;
+  if (g.ctrl->halt == 2) return;
+  const long long nc = g.ncell;
+  const long long idx = (long long)wk.y * 256 + threadIdx.x;
+  const float d1 = fd.bdt[0], d2 = fd.bdt[1], d3 = fd.bdt[2];
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  if (idx < nc) {
+    const int nx = g.nx, ny = g.ny;
+    const int i = (int)(idx % nx);
+    const int j = (int)((idx / nx) % ny);
+    const long long ip = idx - i + wrp(i + 1, nx);
+    const long long jp = idx + (long long)(wrp(j + 1, ny) - j) * nx;
+    const float* E1 = g.E;
+    const float* E2 = g.E + nc;
+    const float* E3 = g.E + 2 * nc;
+    float* B1 = g.B;
+    float* B2 = g.B + nc;
+    float* B3 = g.B + 2 * nc;
+    const float e1 = E1[idx], e2 = E2[idx], e3 = E3[idx];
+    float b1 = B1[idx], b2 = B2[idx], b3 = B3[idx];
+    if (ENERGY) {
+      acc[0] = (double)b1 * b1;
+      acc[1] = (double)b2 * b2;
+      acc[2] = (double)b3 * b3;
+      acc[3] = (double)e1 * e1;
+      acc[4] = (double)e2 * e2;
+      acc[5] = (double)e3 * e3;
+    }
+    if (DIMS == 2) {
+      b1 = b1 - d2 * (E3[jp] - e3);
+      b2 = b2 + d1 * (E3[ip] - e3);
+      b3 = b3 - d1 * (E2[ip] - e2) + d2 * (E1[jp] - e1);
+    } else {
+      const int nz = g.nz;
+      const int k = (int)(idx / ((long long)nx * ny));
+      const long long kp = idx + (long long)(wrp(k + 1, nz) - k) * nx * ny;
+      b1 = b1 - d2 * (E3[jp] - e3) + d3 * (E2[kp] - e2);
+      b2 = b2 + d1 * (E3[ip] - e3) - d3 * (E1[kp] - e1);
+      b3 = b3 - d1 * (E2[ip] - e2) + d2 * (E1[jp] - e1);
+    }
+    B1[idx] = b1;
+    B2[idx] = b2;
+    B3[idx] = b3;
+  }
+  if (ENERGY) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      double v = wave_sum(acc[c]);
+      if (lane == 0) s_red[wave][c] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+      const int c = threadIdx.x;
+      fd.fe_part[(long long)wk.y * 6 + c] =
+          ((s_red[0][c] + s_red[1][c]) + s_red[2][c]) + s_red[3][c];
+    }
+  }
+}
+
+template <int DIMS>
+__global__ __launch_bounds__(256) void yee_dedt(const FieldDesc* __restrict__ fds,
+                                                const int2* __restrict__ work) {
+  const int2 wk = work[blockIdx.x];
+  const FieldDesc& fd = fds[wk.x];
+  const GridDesc& g = fd.g;
+  if (g.ctrl->halt == 2) return;
+  const long long nc = g.ncell;
+  const long long idx = (long long)wk.y * 256 + threadIdx.x;
+  if (idx >= nc) return;
+  const float d1 = fd.edt[0], d2 = fd.edt[1], d3 = fd.edt[2], dtif = fd.dtif;
+  const int nx = g.nx, ny = g.ny;
+  const int i = (int)(idx % nx);
+  const int j = (int)((idx / nx) % ny);
+  const long long im = idx - i + wrp(i - 1, nx);
+  const long long jm = idx + (long long)(wrp(j - 1, ny) - j) * nx;
+  float* E1 = g.E;
+  float* E2 = g.E + nc;
+  float* E3 = g.E + 2 * nc;
+  const float* B1 = g.B;
+  const float* B2 = g.B + nc;
+  const float* B3 = g.B + 2 * nc;
+  unsigned long long* J = g.J;
+  float j1, j2, j3;
+  if (fd.Jf) {
+    j1 = fd.Jf[idx];
+    j2 = fd.Jf[nc + idx];
+    j3 = fd.Jf[2 * nc + idx];
+  } else {
+    const double jinv = g.jinv;
+    j1 = (float)((double)(long long)J[idx] * jinv);
+    j2 = (float)((double)(long long)J[nc + idx] * jinv);
+    j3 = (float)((double)(long long)J[2 * nc + idx] * jinv);
+    J[idx] = 0ull;
+    J[nc + idx] = 0ull;
+    J[2 * nc + idx] = 0ull;
+  }
+  const float b1 = B1[idx], b2 = B2[idx], b3 = B3[idx];
+  float e1 = E1[idx], e2 = E2[idx], e3 = E3[idx];
+  if (DIMS == 2) {
+    e1 = e1 - dtif * j1 + d2 * (b3 - B3[jm]);
+    e2 = e2 - dtif * j2 - d1 * (b3 - B3[im]);
+    e3 = e3 - dtif * j3 + d1 * (b2 - B2[im]) - d2 * (b1 - B1[jm]);
+  } else {
+    const int nz = g.nz;
+    const int k = (int)(idx / ((long long)nx * ny));
+    const long long km = idx + (long long)(wrp(k - 1, nz) - k) * nx * ny;
+    e1 = e1 - dtif * j1 + d2 * (b3 - B3[jm]) - d3 * (b2 - B2[km]);
+    e2 = e2 - dtif * j2 + d3 * (b1 - B1[km]) - d1 * (b3 - B3[im]);
+    e3 = e3 - dtif * j3 + d1 * (b2 - B2[im]) - d2 * (b1 - B1[jm]);
+  }
+  E1[idx] = e1;
+  E2[idx] = e2;
+  E3[idx] = e3;
+}
+
+// Neumaier compensated accumulation
+OSR_DEV void kadd(double& s, double& c, double v) {
+  const double t = s + v;
+  if (fabs(s) >= fabs(v))
+    c += (s - t) + v;
+  else
+    c += (v - t) + s;
+  s = t;
+}
+
+// One block per simulation: fixed-order compensated reduction of the step's partial
+// sums into one history row, then the step counter and the halt protocol.
+__global__ __launch_bounds__(256) void finalize_step(const FieldDesc* __restrict__ fds,
+                                                     const int* __restrict__ work) {
+  __shared__ double s_s[256], s_c[256];
+  const FieldDesc& fd = fds[work[blockIdx.x]];
+  const GridDesc& g = fd.g;
+  Ctrl* ctrl = g.ctrl;
+  if (ctrl->halt == 2) return;
+  const long long step = ctrl->steps;
+  const int ncol = 6 + fd.nspec;
+  for (int col = 0; col < ncol; col++) {
+    double s = 0.0, c = 0.0;
+    if (col < 6) {
+      for (int b = threadIdx.x; b < fd.nblk; b += 256) kadd(s, c, fd.fe_part[(long long)b * 6 + col]);
+    } else {
+      const int sp = col - 6;
+      const double* ep = fd.ep[sp];
+      for (int t = threadIdx.x; t < fd.ep_n[sp]; t += 256) kadd(s, c, ep[t]);
+    }
+    s_s[threadIdx.x] = s;
+    s_c[threadIdx.x] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double ts = 0.0, tc = 0.0;
+      for (int t = 0; t < 256; t++) {
+        kadd(ts, tc, s_s[t]);
+        tc += s_c[t];
+      }
+      const double v = ts + tc;
+      if (step < fd.hist_cap)
+        fd.hist[step * ncol + col] = col < 6 ? v * fd.fe_scale : v * fd.ep_scale[col - 6];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    ctrl->steps = step + 1;
+    if (ctrl->halt == 1) ctrl->halt = 2;
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Gauss-law diagnostic (FP64): rho on cell corners with the same splines as
+// DEPOSIT_RHO_2D/3D (spec/os-spec-charge.f90:500-580), accumulated in int64 fixed point,
+// and div E (vdf/os-vdf-math.f90:1013-1073, offset 0).
+// ------------------------------------------------------------------------------------
+template <int ORDER>
+__global__ __launch_bounds__(256) void rho_deposit(const SpecDesc* __restrict__ descs,
+                                                   const int2* __restrict__ work,
+                                                   unsigned long long* rho, double scale) {
+  using O = Ord<ORDER>;
+  constexpr int NP = O::NP;
+  const int2 wk = work[blockIdx.x];
+  const SpecDesc& d = descs[wk.x];
+  const GridDesc& g = d.g;
+  const int tile = wk.y;
+  const int base = d.off[tile], n = d.cnt[tile];
+  for (int p = threadIdx.x; p < n; p += 256) {
+    const int s = base + p;
+    int ix, iy, iz = 0;
+    double w1[NP], w2[NP], w3[NP];
+    float fw[NP];
+    if (g.dims == 2)
+      unpack2(d.cell[s], ix, iy);
+    else
+      unpack3(d.cell[s], ix, iy, iz);
+    // splines in double precision of the (float) positions
+    {
+      const double xs[3] = {(double)d.x[s], (double)d.y[s], g.dims == 3 ? (double)d.z[s] : 0.0};
+      for (int dd = 0; dd < g.dims; dd++) {
+        const double x = xs[dd];
+        double* w = dd == 0 ? w1 : (dd == 1 ? w2 : w3);
+        if (ORDER == 1) {
+          w[0] = 0.5 - x;
+          w[1] = 0.5 + x;
+        } else if (ORDER == 2) {
+          const double t0 = 0.5 - x, t1 = 0.5 + x;
+          w[0] = 0.5 * t0 * t0;
+          w[1] = 0.5 + t0 * t1;
+          w[2] = 0.5 * t1 * t1;
+        } else {
+          const double t0 = 0.5 - x, t1 = 0.5 + x;
+          w[0] = t0 * t0 * t0 / 6.0;
+          w[1] = (2.0 / 3.0 - t1 * t1) + 0.5 * t1 * t1 * t1;
+          w[2] = (2.0 / 3.0 - t0 * t0) + 0.5 * t0 * t0 * t0;
+          w[NP - 1] = t1 * t1 * t1 / 6.0;
+        }
+      }
+    }
+    (void)fw;
+    const double q = d.q[s];
+    const int nz = g.dims == 3 ? NP : 1;
+    for (int k3 = 0; k3 < nz; k3++)
+      for (int k2 = 0; k2 < NP; k2++)
+        for (int k1 = 0; k1 < NP; k1++) {
+          const int gx = wrp(ix + O::LP + k1, g.nx), gy = wrp(iy + O::LP + k2, g.ny);
+          long long gi = gx + (long long)g.nx * gy;
+          double w = q * w1[k1] * w2[k2];
+          if (g.dims == 3) {
+            gi += (long long)g.nx * g.ny * wrp(iz + O::LP + k3, g.nz);
+            w *= w3[k3];
+          }
+          const long long v = __double2ll_rn(w * scale);
+          atomicAdd(&rho[gi], (unsigned long long)v);
+        }
+  }
+}
+
+__global__ __launch_bounds__(256) void gauss_residual(GridDesc g, const unsigned long long* rho,
+                                                      double rscale, double r1, double r2,
+                                                      double r3, double* res) {
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= g.ncell) return;
+  const long long nc = g.ncell;
+  const int nx = g.nx, ny = g.ny;
+  const int i = (int)(idx % nx);
+  const int j = (int)((idx / nx) % ny);
+  const long long im = idx - i + wrp(i - 1, nx);
+  const long long jm = idx + (long long)(wrp(j - 1, ny) - j) * nx;
+  double dv = ((double)g.E[idx] - (double)g.E[im]) * r1 +
+              ((double)g.E[nc + idx] - (double)g.E[nc + jm]) * r2;
+  if (g.dims == 3) {
+    const int k = (int)(idx / ((long long)nx * ny));
+    const long long km = idx + (long long)(wrp(k - 1, g.nz) - k) * nx * ny;
+    dv += ((double)g.E[2 * nc + idx] - (double)g.E[2 * nc + km]) * r3;
+  }
+  res[idx] = dv - (double)(long long)rho[idx] * rscale;
+}
+
+template __global__ void yee_dbdt<2, true>(const FieldDesc*, const int2*);
+template __global__ void yee_dbdt<2, false>(const FieldDesc*, const int2*);
+template __global__ void yee_dbdt<3, true>(const FieldDesc*, const int2*);
+template __global__ void yee_dbdt<3, false>(const FieldDesc*, const int2*);
+template __global__ void yee_dedt<2>(const FieldDesc*, const int2*);
+template __global__ void yee_dedt<3>(const FieldDesc*, const int2*);
+template __global__ void rho_deposit<1>(const SpecDesc*, const int2*, unsigned long long*, double);
+template __global__ void rho_deposit<2>(const SpecDesc*, const int2*, unsigned long long*, double);
+template __global__ void rho_deposit<3>(const SpecDesc*, const int2*, unsigned long long*, double);
+
+}  // namespace osr

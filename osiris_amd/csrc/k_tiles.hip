@@ -1,0 +1,724 @@
+// k_tiles.hip -- the fused tile kernels of the resident fast path.
+//
+//   push_deposit_{2d,3d}<ORDER,PUSHER>: one workgroup per (species, tile).  E and B of
+//     the tile plus the stencil halo are staged in LDS; every wave walks a contiguous
+//     slice of the tile's particles (coalesced SoA loads), gathers from LDS, pushes
+//     (Boris or Vay), advances the position, splits the trajectory into virtual
+//     particles and deposits them into an LDS current tile with int64 fixed-point
+//     atomics.  Integer addition is associative, so the deposited current is bit-for-bit
+//     independent of the order in which lanes, waves and workgroups add their share:
+//     the reduction is reproducible run to run.  The LDS tile is then added to the global
+//     fixed-point J with the periodic wrap (the guard-cell fold of the reference).
+//     Particles that stay in the tile are written back in place; leavers go to the
+//     wave's region of the mover buffer (ballot-ranked, so the order is deterministic).
+//   tile_count / tile_fill: move the leavers into their destination tiles, filling the
+//     holes they left; a deterministic stable compaction.  If a tile would overflow its
+//     capacity the step is flagged (Ctrl::halt) and the host re-lays the particles out.
+//
+// Reference: spec/os-spec-push.f03:1438-1605 (advance_deposit_2d/3d),
+// emf/os-emf-interpolate.f03:554-779 (GET_EMF_2D/3D), spec/os-spec-current.f03:659-905,
+// 2691-2905 (split + DEP_CURRENT), simd/os-spec-push-avx2.c:471-760 (fused SIMD kernel),
+// particles/os-particles.f03:1416-1443 (private J + reduce), spec/os-species.f03:269-441
+// (counting sort of particles by cell).
+#include "osr_device.hpp"
+#include "osr_types.hpp"
+
+namespace osr {
+
+OSR_DEV int wrapi(int i, int n) {
+  while (i < 0) i += n;
+  while (i >= n) i -= n;
+  return i;
+}
+
+// Exclusive rank of `flag` among the threads of the block in thread order, and the total.
+// Uses wave ballots and one LDS exchange; all threads must call it.
+template <int NT>
+OSR_DEV int block_rank(bool flag, int* s_wtot, int& total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned long long m = __ballot(flag);
+  int r = __popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) s_wtot[wave] = __popcll(m);
+  __syncthreads();
+  int before = 0;
+  total = 0;
+  for (int w = 0; w < NT / 64; w++) {
+    int c = s_wtot[w];
+    if (w < wave) before += c;
+    total += c;
+  }
+  __syncthreads();
+  return before + r;
+}
+
+// ------------------------------------------------------------------------------------
+// 2D push + deposit
+// ------------------------------------------------------------------------------------
+template <int ORDER, int PUSHER, int NT>
+__global__ __launch_bounds__(NT) void push_deposit_2d(const SpecDesc* __restrict__ descs,
+                                                      const int2* __restrict__ work) {
+  using O = Ord<ORDER>;
+  constexpr int NP = O::NP;
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double s_ene[NT / 64];
+
+  const int2 wk = work[blockIdx.x];
+  const SpecDesc& d = descs[wk.x];
+  const GridDesc& g = d.g;
+  if (g.ctrl->halt != 0) return;
+  const int tile = wk.y;
+  const int TX = g.tx, TY = g.ty;
+  const int GX = TX + O::GLO + O::GHI, GY = TY + O::GLO + O::GHI;
+  const int JX = TX + O::JLO + O::JHI, JY = TY + O::JLO + O::JHI;
+  const int GN = GX * GY, JN = JX * JY;
+  unsigned long long* sJ = (unsigned long long*)smem;      // [3][JN]
+  float* sF = (float*)(smem + sizeof(unsigned long long) * 3 * JN);  // E[3][GN], B[3][GN]
+  const int cx0 = (tile % g.ntx) * TX, cy0 = (tile / g.ntx) * TY;
+  const int nx = g.nx, ny = g.ny;
+  const long long nc = g.ncell;
+
+  for (int i = threadIdx.x; i < GN; i += NT) {
+    int gx = wrapi(cx0 + (i % GX) - O::GLO, nx), gy = wrapi(cy0 + (i / GX) - O::GLO, ny);
+    long long gi = gx + (long long)nx * gy;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      sF[c * GN + i] = g.E[c * nc + gi];
+      sF[(3 + c) * GN + i] = g.B[c * nc + gi];
+    }
+  }
+  for (int i = threadIdx.x; i < 3 * JN; i += NT) sJ[i] = 0ull;
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int NW = NT / 64;
+  const int base = d.off[tile];
+  const int n = d.cnt[tile];
+  const int per_wave = ((n + NT - 1) / NT) * 64;
+  const int w0 = wave * per_wave;
+  const int w1 = min(w0 + per_wave, n);
+  const float tem = d.tem;
+  const float dtdx1 = g.dt_dx[0], dtdx2 = g.dt_dx[1];
+  const float jn1 = g.jnorm[0], jn2 = g.jnorm[1];
+  const double jscale = g.jscale;
+  const float* sE1 = sF;
+  const float* sE2 = sF + GN;
+  const float* sE3 = sF + 2 * GN;
+  const float* sB1 = sF + 3 * GN;
+  const float* sB2 = sF + 4 * GN;
+  const float* sB3 = sF + 5 * GN;
+  unsigned long long* sJ1 = sJ;
+  unsigned long long* sJ2 = sJ + JN;
+  unsigned long long* sJ3 = sJ + 2 * JN;
+
+  double ene = 0.0;
+  int mcount = 0;
+  bool bad = false;
+  for (int p0 = w0; p0 < w1; p0 += 64) {
+    const int p = p0 + lane;
+    bool act = p < w1;
+    uint32_t cell = 0;
+    float x = 0.f, y = 0.f, u1 = 0.f, u2 = 0.f, u3 = 0.f, q = 0.f;
+    int lx = 0, ly = 0;
+    if (act) {
+      const int s = base + p;
+      cell = d.cell[s];
+      x = d.x[s];
+      y = d.y[s];
+      u1 = d.ux[s];
+      u2 = d.uy[s];
+      u3 = d.uz[s];
+      q = d.q[s];
+      int ix, iy;
+      unpack2(cell, ix, iy);
+      lx = ix - cx0;
+      ly = iy - cy0;
+      if (lx < 0 || lx >= TX || ly < 0 || ly >= TY) {
+        bad = true;
+        act = false;
+        lx = ly = 0;
+      }
+    }
+    // ---- gather (GET_EMF_2D, same FMA order as the AVX2 kernel)
+    float wx[NP], wy[NP], wxh[NP], wyh[NP];
+    spline<ORDER>(x, wx);
+    spline<ORDER>(y, wy);
+    const float hx = x < 0.f ? 1.f : 0.f, hy = y < 0.f ? 1.f : 0.f;
+    splineh<ORDER>(x, hx, wxh);
+    splineh<ORDER>(y, hy, wyh);
+    const int gx = lx + O::GLO + O::LP, gy = ly + O::GLO + O::LP;
+    const int gxh = gx - (x < 0.f), gyh = gy - (y < 0.f);
+    float e1 = 0.f, e2 = 0.f, e3 = 0.f, b1 = 0.f, b2 = 0.f, b3 = 0.f;
+#pragma unroll
+    for (int k2 = 0; k2 < NP; k2++) {
+      float l1 = 0.f, l2 = 0.f, l3 = 0.f;
+#pragma unroll
+      for (int k1 = 0; k1 < NP; k1++) {
+        l1 = fma_(sE1[(gy + k2) * GX + gxh + k1], wxh[k1], l1);
+        l2 = fma_(sE2[(gyh + k2) * GX + gx + k1], wx[k1], l2);
+        l3 = fma_(sE3[(gy + k2) * GX + gx + k1], wx[k1], l3);
+      }
+      e1 = fma_(l1, wy[k2], e1);
+      e2 = fma_(l2, wyh[k2], e2);
+      e3 = fma_(l3, wy[k2], e3);
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < NP; k2++) {
+      float l1 = 0.f, l2 = 0.f, l3 = 0.f;
+#pragma unroll
+      for (int k1 = 0; k1 < NP; k1++) {
+        l1 = fma_(sB1[(gyh + k2) * GX + gx + k1], wx[k1], l1);
+        l2 = fma_(sB2[(gy + k2) * GX + gxh + k1], wxh[k1], l2);
+        l3 = fma_(sB3[(gyh + k2) * GX + gxh + k1], wxh[k1], l3);
+      }
+      b1 = fma_(l1, wyh[k2], b1);
+      b2 = fma_(l2, wy[k2], b2);
+      b3 = fma_(l3, wyh[k2], b3);
+    }
+    // ---- push
+    const float ke = push<PUSHER>(tem, e1, e2, e3, b1, b2, b3, u1, u2, u3);
+    if (act) ene += (double)q * (double)ke;
+    // ---- position
+    const float rg = rgamma(u1, u2, u3);
+    const float v1 = u1 * rg, v2 = u2 * rg, v3 = u3 * rg;
+    const float x1 = fma_(v1, dtdx1, x), y1 = fma_(v2, dtdx2, y);
+    const float tx = trim(x1), ty = trim(y1);
+    const int dix = (int)tx, diy = (int)ty;
+    // ---- split + deposit
+    VP2 vp[3];
+    const int nvp = act ? split2d(x, y, x1, y1, dix, diy, q, v3, vp) : 0;
+    for (int v = 0; v < nvp; v++) {
+      const VP2 V = pick(vp, v);
+      float S0x[NP], S1x[NP], S0y[NP], S1y[NP], wl1[NP], wl2[NP];
+      spline<ORDER>(V.x0, S0x);
+      spline<ORDER>(V.x1, S1x);
+      spline<ORDER>(V.y0, S0y);
+      spline<ORDER>(V.y1, S1y);
+      wl<ORDER>(V.q * jn1, V.x0, V.x1, wl1);
+      wl<ORDER>(V.q * jn2, V.y0, V.y1, wl2);
+      const float qvz = (V.q * V.vz) * (1.0f / 3.0f);
+      const int jx0 = lx + V.di + O::LP + O::JLO, jy0 = ly + V.dj + O::LP + O::JLO;
+#pragma unroll
+      for (int k2 = 0; k2 < NP; k2++) {
+        const float wp1 = S0y[k2] + S1y[k2];
+#pragma unroll
+        for (int k1 = 0; k1 < NP; k1++) {
+          const int idx = (jy0 + k2) * JX + jx0 + k1;
+          const float wp2 = S0x[k1] + S1x[k1];
+          float t1 = fma_(S1y[k2], S1x[k1], S0x[k1] * S0y[k2]);
+          const float t2 = fma_(S0y[k2], S1x[k1], S0x[k1] * S1y[k2]);
+          t1 = fma_(t2, 0.5f, t1);
+          if (k1 < NP - 1) atomicAdd(&sJ1[idx], (unsigned long long)to_fix(wl1[k1] * wp1, jscale));
+          if (k2 < NP - 1) atomicAdd(&sJ2[idx], (unsigned long long)to_fix(wl2[k2] * wp2, jscale));
+          atomicAdd(&sJ3[idx], (unsigned long long)to_fix(qvz * t1, jscale));
+        }
+      }
+    }
+    // ---- store: stay in place, or leave through the mover buffer
+    int nix = wrapi(lx + cx0 + dix, nx), niy = wrapi(ly + cy0 + diy, ny);
+    const uint32_t ncell = pack2(nix, niy);
+    const bool stay = (nix / TX + g.ntx * (niy / TY)) == tile;
+    const float xs = x1 - tx, ys = y1 - ty;
+    if (act && stay) {
+      const int s = base + p;
+      d.cell[s] = ncell;
+      d.x[s] = xs;
+      d.y[s] = ys;
+      d.ux[s] = u1;
+      d.uy[s] = u2;
+      d.uz[s] = u3;
+    }
+    const bool mv = act && !stay;
+    const unsigned long long mm = __ballot(mv);
+    if (mv) {
+      const int slot = base + w0 + mcount + __popcll(mm & ((1ull << lane) - 1ull));
+      const int s = base + p;
+      d.mcell[slot] = ncell;
+      d.mx[slot] = xs;
+      d.my[slot] = ys;
+      d.mux[slot] = u1;
+      d.muy[slot] = u2;
+      d.muz[slot] = u3;
+      d.mq[slot] = q;
+      d.mpid[slot] = d.pid[s];
+      d.midx[slot] = p;
+    }
+    mcount += __popcll(mm);
+  }
+  if (bad) atomicOr(&g.ctrl->error, 1);
+  if (lane == 0) d.mv_cnt[tile * NW + wave] = mcount;
+  if (threadIdx.x == 0) d.mv_stride[tile] = per_wave;
+  ene = wave_sum(ene);
+  if (lane == 0) s_ene[wave] = ene;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < NW; w++) t += s_ene[w];
+    d.epart[tile] = t;
+  }
+  // ---- fold the LDS current tile into global J with the periodic wrap
+  for (int i = threadIdx.x; i < JN; i += NT) {
+    const int gx = wrapi(cx0 + (i % JX) - O::JLO, nx), gy = wrapi(cy0 + (i / JX) - O::JLO, ny);
+    const long long gi = gx + (long long)nx * gy;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const unsigned long long v = sJ[c * JN + i];
+      if (v) atomicAdd(&g.J[c * nc + gi], v);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// 3D push + deposit
+// ------------------------------------------------------------------------------------
+template <int ORDER, int PUSHER, int NT>
+__global__ __launch_bounds__(NT) void push_deposit_3d(const SpecDesc* __restrict__ descs,
+                                                      const int2* __restrict__ work) {
+  using O = Ord<ORDER>;
+  constexpr int NP = O::NP;
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double s_ene[NT / 64];
+
+  const int2 wk = work[blockIdx.x];
+  const SpecDesc& d = descs[wk.x];
+  const GridDesc& g = d.g;
+  if (g.ctrl->halt != 0) return;
+  const int tile = wk.y;
+  const int TX = g.tx, TY = g.ty, TZ = g.tz;
+  const int GX = TX + O::GLO + O::GHI, GY = TY + O::GLO + O::GHI, GZ = TZ + O::GLO + O::GHI;
+  const int JX = TX + O::JLO + O::JHI, JY = TY + O::JLO + O::JHI, JZ = TZ + O::JLO + O::JHI;
+  const int GN = GX * GY * GZ, JN = JX * JY * JZ;
+  unsigned long long* sJ = (unsigned long long*)smem;
+  float* sF = (float*)(smem + sizeof(unsigned long long) * 3 * JN);
+  const int ttx = tile % g.ntx, tty = (tile / g.ntx) % g.nty, ttz = tile / (g.ntx * g.nty);
+  const int cx0 = ttx * TX, cy0 = tty * TY, cz0 = ttz * TZ;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const long long nc = g.ncell;
+
+  for (int i = threadIdx.x; i < GN; i += NT) {
+    const int lx = i % GX, ly = (i / GX) % GY, lz = i / (GX * GY);
+    const int gx = wrapi(cx0 + lx - O::GLO, nx), gy = wrapi(cy0 + ly - O::GLO, ny),
+              gz = wrapi(cz0 + lz - O::GLO, nz);
+    const long long gi = gx + (long long)nx * (gy + (long long)ny * gz);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      sF[c * GN + i] = g.E[c * nc + gi];
+      sF[(3 + c) * GN + i] = g.B[c * nc + gi];
+    }
+  }
+  for (int i = threadIdx.x; i < 3 * JN; i += NT) sJ[i] = 0ull;
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int NW = NT / 64;
+  const int base = d.off[tile];
+  const int n = d.cnt[tile];
+  const int per_wave = ((n + NT - 1) / NT) * 64;
+  const int w0 = wave * per_wave;
+  const int w1 = min(w0 + per_wave, n);
+  const float tem = d.tem;
+  const float dtdx1 = g.dt_dx[0], dtdx2 = g.dt_dx[1], dtdx3 = g.dt_dx[2];
+  const float jn1 = g.jnorm[0], jn2 = g.jnorm[1], jn3 = g.jnorm[2];
+  const double jscale = g.jscale;
+  const int GXY = GX * GY, JXY = JX * JY;
+
+  double ene = 0.0;
+  int mcount = 0;
+  bool bad = false;
+  for (int p0 = w0; p0 < w1; p0 += 64) {
+    const int p = p0 + lane;
+    bool act = p < w1;
+    uint32_t cell = 0;
+    float x = 0.f, y = 0.f, z = 0.f, u1 = 0.f, u2 = 0.f, u3 = 0.f, q = 0.f;
+    int lx = 0, ly = 0, lz = 0;
+    if (act) {
+      const int s = base + p;
+      cell = d.cell[s];
+      x = d.x[s];
+      y = d.y[s];
+      z = d.z[s];
+      u1 = d.ux[s];
+      u2 = d.uy[s];
+      u3 = d.uz[s];
+      q = d.q[s];
+      int ix, iy, iz;
+      unpack3(cell, ix, iy, iz);
+      lx = ix - cx0;
+      ly = iy - cy0;
+      lz = iz - cz0;
+      if (lx < 0 || lx >= TX || ly < 0 || ly >= TY || lz < 0 || lz >= TZ) {
+        bad = true;
+        act = false;
+        lx = ly = lz = 0;
+      }
+    }
+    float wx[NP], wy[NP], wz[NP], wxh[NP], wyh[NP], wzh[NP];
+    spline<ORDER>(x, wx);
+    spline<ORDER>(y, wy);
+    spline<ORDER>(z, wz);
+    splineh<ORDER>(x, x < 0.f ? 1.f : 0.f, wxh);
+    splineh<ORDER>(y, y < 0.f ? 1.f : 0.f, wyh);
+    splineh<ORDER>(z, z < 0.f ? 1.f : 0.f, wzh);
+    const int gx = lx + O::GLO + O::LP, gy = ly + O::GLO + O::LP, gz = lz + O::GLO + O::LP;
+    const int gxh = gx - (x < 0.f), gyh = gy - (y < 0.f), gzh = gz - (z < 0.f);
+    const float* sE1 = sF;
+    const float* sE2 = sF + GN;
+    const float* sE3 = sF + 2 * GN;
+    const float* sB1 = sF + 3 * GN;
+    const float* sB2 = sF + 4 * GN;
+    const float* sB3 = sF + 5 * GN;
+    // GET_EMF_3D line/plane order (os-emf-interpolate.f03:697-773); the AVX2 3D kernel
+    // uses the same fma order along k1, then k2, then k3.
+    float e1 = 0.f, e2 = 0.f, e3 = 0.f;
+#pragma unroll
+    for (int k3 = 0; k3 < NP; k3++) {
+      float p1 = 0.f, p2 = 0.f, p3 = 0.f;
+#pragma unroll
+      for (int k2 = 0; k2 < NP; k2++) {
+        float l1 = 0.f, l2 = 0.f, l3 = 0.f;
+#pragma unroll
+        for (int k1 = 0; k1 < NP; k1++) {
+          l1 = fma_(sE1[(gz + k3) * GXY + (gy + k2) * GX + gxh + k1], wxh[k1], l1);
+          l2 = fma_(sE2[(gz + k3) * GXY + (gyh + k2) * GX + gx + k1], wx[k1], l2);
+          l3 = fma_(sE3[(gzh + k3) * GXY + (gy + k2) * GX + gx + k1], wx[k1], l3);
+        }
+        p1 = fma_(l1, wy[k2], p1);
+        p2 = fma_(l2, wyh[k2], p2);
+        p3 = fma_(l3, wy[k2], p3);
+      }
+      e1 = fma_(p1, wz[k3], e1);
+      e2 = fma_(p2, wz[k3], e2);
+      e3 = fma_(p3, wzh[k3], e3);
+    }
+    float b1 = 0.f, b2 = 0.f, b3 = 0.f;
+#pragma unroll
+    for (int k3 = 0; k3 < NP; k3++) {
+      float p1 = 0.f, p2 = 0.f, p3 = 0.f;
+#pragma unroll
+      for (int k2 = 0; k2 < NP; k2++) {
+        float l1 = 0.f, l2 = 0.f, l3 = 0.f;
+#pragma unroll
+        for (int k1 = 0; k1 < NP; k1++) {
+          l1 = fma_(sB1[(gzh + k3) * GXY + (gyh + k2) * GX + gx + k1], wx[k1], l1);
+          l2 = fma_(sB2[(gzh + k3) * GXY + (gy + k2) * GX + gxh + k1], wxh[k1], l2);
+          l3 = fma_(sB3[(gz + k3) * GXY + (gyh + k2) * GX + gxh + k1], wxh[k1], l3);
+        }
+        p1 = fma_(l1, wyh[k2], p1);
+        p2 = fma_(l2, wy[k2], p2);
+        p3 = fma_(l3, wyh[k2], p3);
+      }
+      b1 = fma_(p1, wzh[k3], b1);
+      b2 = fma_(p2, wzh[k3], b2);
+      b3 = fma_(p3, wz[k3], b3);
+    }
+    const float ke = push<PUSHER>(tem, e1, e2, e3, b1, b2, b3, u1, u2, u3);
+    if (act) ene += (double)q * (double)ke;
+    const float rg = rgamma(u1, u2, u3);
+    const float x1 = fma_(u1 * rg, dtdx1, x), y1 = fma_(u2 * rg, dtdx2, y),
+                z1 = fma_(u3 * rg, dtdx3, z);
+    const float tx = trim(x1), ty = trim(y1), tz = trim(z1);
+    const int dix = (int)tx, diy = (int)ty, diz = (int)tz;
+    VP3 vp[4];
+    const int nvp = act ? split3d(x, y, z, x1, y1, z1, dix, diy, diz, q, vp) : 0;
+    for (int v = 0; v < nvp; v++) {
+      const VP3 V = pick(vp, v);
+      float S0x[NP], S1x[NP], S0y[NP], S1y[NP], S0z[NP], S1z[NP], wl1[NP], wl2[NP], wl3[NP];
+      spline<ORDER>(V.x0, S0x);
+      spline<ORDER>(V.x1, S1x);
+      spline<ORDER>(V.y0, S0y);
+      spline<ORDER>(V.y1, S1y);
+      spline<ORDER>(V.z0, S0z);
+      spline<ORDER>(V.z1, S1z);
+      wl<ORDER>(V.q * jn1, V.x0, V.x1, wl1);
+      wl<ORDER>(V.q * jn2, V.y0, V.y1, wl2);
+      wl<ORDER>(V.q * jn3, V.z0, V.z1, wl3);
+      const int jx0 = lx + V.di + O::LP + O::JLO, jy0 = ly + V.dj + O::LP + O::JLO,
+                jz0 = lz + V.dk + O::LP + O::JLO;
+#pragma unroll
+      for (int k3 = 0; k3 < NP; k3++) {
+#pragma unroll
+        for (int k2 = 0; k2 < NP; k2++) {
+          // wp1(k2,k3), os-spec-current.f03:2876-2882
+          const float wp1 = fma_(0.5f, fma_(S1y[k2], S0z[k3], S0y[k2] * S1z[k3]),
+                                 fma_(S1y[k2], S1z[k3], S0y[k2] * S0z[k3]));
+#pragma unroll
+          for (int k1 = 0; k1 < NP; k1++) {
+            const int idx = (jz0 + k3) * JXY + (jy0 + k2) * JX + jx0 + k1;
+            const float wp2 = fma_(0.5f, fma_(S1x[k1], S0z[k3], S0x[k1] * S1z[k3]),
+                                   fma_(S1x[k1], S1z[k3], S0x[k1] * S0z[k3]));
+            const float wp3 = fma_(0.5f, fma_(S1x[k1], S0y[k2], S0x[k1] * S1y[k2]),
+                                   fma_(S1x[k1], S1y[k2], S0x[k1] * S0y[k2]));
+            if (k1 < NP - 1)
+              atomicAdd(&sJ[idx], (unsigned long long)to_fix(wl1[k1] * wp1, jscale));
+            if (k2 < NP - 1)
+              atomicAdd(&sJ[JN + idx], (unsigned long long)to_fix(wl2[k2] * wp2, jscale));
+            if (k3 < NP - 1)
+              atomicAdd(&sJ[2 * JN + idx], (unsigned long long)to_fix(wl3[k3] * wp3, jscale));
+          }
+        }
+      }
+    }
+    const int nix = wrapi(lx + cx0 + dix, nx), niy = wrapi(ly + cy0 + diy, ny),
+              niz = wrapi(lz + cz0 + diz, nz);
+    const uint32_t ncell = pack3(nix, niy, niz);
+    const bool stay = (nix / TX + g.ntx * (niy / TY + g.nty * (niz / TZ))) == tile;
+    const float xs = x1 - tx, ys = y1 - ty, zs = z1 - tz;
+    if (act && stay) {
+      const int s = base + p;
+      d.cell[s] = ncell;
+      d.x[s] = xs;
+      d.y[s] = ys;
+      d.z[s] = zs;
+      d.ux[s] = u1;
+      d.uy[s] = u2;
+      d.uz[s] = u3;
+    }
+    const bool mv = act && !stay;
+    const unsigned long long mm = __ballot(mv);
+    if (mv) {
+      const int slot = base + w0 + mcount + __popcll(mm & ((1ull << lane) - 1ull));
+      const int s = base + p;
+      d.mcell[slot] = ncell;
+      d.mx[slot] = xs;
+      d.my[slot] = ys;
+      d.mz[slot] = zs;
+      d.mux[slot] = u1;
+      d.muy[slot] = u2;
+      d.muz[slot] = u3;
+      d.mq[slot] = q;
+      d.mpid[slot] = d.pid[s];
+      d.midx[slot] = p;
+    }
+    mcount += __popcll(mm);
+  }
+  if (bad) atomicOr(&g.ctrl->error, 1);
+  if (lane == 0) d.mv_cnt[tile * NW + wave] = mcount;
+  if (threadIdx.x == 0) d.mv_stride[tile] = per_wave;
+  ene = wave_sum(ene);
+  if (lane == 0) s_ene[wave] = ene;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < NW; w++) t += s_ene[w];
+    d.epart[tile] = t;
+  }
+  for (int i = threadIdx.x; i < JN; i += NT) {
+    const int lx = i % JX, ly = (i / JX) % JY, lz = i / JXY;
+    const int gx = wrapi(cx0 + lx - O::JLO, nx), gy = wrapi(cy0 + ly - O::JLO, ny),
+              gz = wrapi(cz0 + lz - O::JLO, nz);
+    const long long gi = gx + (long long)nx * (gy + (long long)ny * gz);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const unsigned long long v = sJ[c * JN + i];
+      if (v) atomicAdd(&g.J[c * nc + gi], v);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Tile maintenance
+// ------------------------------------------------------------------------------------
+// Distinct neighbour tiles of `tile` (excluding itself) in a fixed order.
+OSR_DEV int neighbours(const GridDesc& g, int tile, int* nb) {
+  int tx, ty, tz;
+  if (g.dims == 2) {
+    tx = tile % g.ntx;
+    ty = tile / g.ntx;
+    tz = 0;
+  } else {
+    tx = tile % g.ntx;
+    ty = (tile / g.ntx) % g.nty;
+    tz = tile / (g.ntx * g.nty);
+  }
+  const int zr = g.dims == 2 ? 0 : 1;
+  int k = 0;
+  for (int dz = -zr; dz <= zr; dz++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        if (dx == 0 && dy == 0 && dz == 0) continue;
+        const int nxt = wrapi(tx + dx, g.ntx), nyt = wrapi(ty + dy, g.nty);
+        const int nzt = g.dims == 2 ? 0 : wrapi(tz + dz, g.ntz);
+        const int t = nxt + g.ntx * (nyt + g.nty * nzt);
+        if (t == tile) continue;
+        bool dup = false;
+        for (int j = 0; j < k; j++) dup |= nb[j] == t;
+        if (!dup) nb[k++] = t;
+      }
+  return k;
+}
+
+// Count incoming movers and check the capacity of every tile.
+template <int NT>
+__global__ __launch_bounds__(NT) void tile_count(const SpecDesc* __restrict__ descs,
+                                                 const int2* __restrict__ work) {
+  __shared__ int s_nb[26];
+  __shared__ int s_nnb;
+  __shared__ int s_red[NT / 64];
+  const int2 wk = work[blockIdx.x];
+  const SpecDesc& d = descs[wk.x];
+  const GridDesc& g = d.g;
+  if (g.ctrl->halt != 0) return;
+  const int tile = wk.y;
+  const int NW = d.waves;
+  if (threadIdx.x == 0) s_nnb = neighbours(g, tile, s_nb);
+  __syncthreads();
+  int mine = 0;
+  for (int k = 0; k < s_nnb; k++) {
+    const int nt = s_nb[k];
+    const int stride = d.mv_stride[nt];
+    for (int w = 0; w < NW; w++) {
+      const int c = d.mv_cnt[nt * NW + w];
+      const int r0 = d.off[nt] + w * stride;
+      for (int i = threadIdx.x; i < c; i += NT) mine += tile_of(g, d.mcell[r0 + i]) == tile;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) mine += __shfl_down(mine, off, 64);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = mine;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int inc = 0;
+    for (int w = 0; w < NT / 64; w++) inc += s_red[w];
+    int holes = 0;
+    for (int w = 0; w < NW; w++) holes += d.mv_cnt[tile * NW + w];
+    const int n = d.cnt[tile];
+    const int nn = n - holes + inc;
+    d.inc[tile] = inc;
+    d.newcnt[tile] = nn;
+    // keep one wave-region rounding of slack so the next push's regions fit
+    if (nn > d.cap[tile] - 64 * NW || holes - inc > kTailBitmapBits) atomicOr(&g.ctrl->halt, 1);
+  }
+}
+
+OSR_DEV void copy_from_mover(const SpecDesc& d, int src, int dst, int dims) {
+  d.cell[dst] = d.mcell[src];
+  d.x[dst] = d.mx[src];
+  d.y[dst] = d.my[src];
+  if (dims == 3) d.z[dst] = d.mz[src];
+  d.ux[dst] = d.mux[src];
+  d.uy[dst] = d.muy[src];
+  d.uz[dst] = d.muz[src];
+  d.q[dst] = d.mq[src];
+  d.pid[dst] = d.mpid[src];
+}
+
+OSR_DEV void copy_particle(const SpecDesc& d, int src, int dst, int dims) {
+  d.cell[dst] = d.cell[src];
+  d.x[dst] = d.x[src];
+  d.y[dst] = d.y[src];
+  if (dims == 3) d.z[dst] = d.z[src];
+  d.ux[dst] = d.ux[src];
+  d.uy[dst] = d.uy[src];
+  d.uz[dst] = d.uz[src];
+  d.q[dst] = d.q[src];
+  d.pid[dst] = d.pid[src];
+}
+
+// Move incoming movers and tail particles into the holes (deterministic order:
+// sources = incoming movers by (neighbour, wave, slot), then the non-hole particles of
+// the tail [newcnt, cnt) ascending; targets = holes below newcnt ascending, then the
+// appended slots [cnt, newcnt)).
+template <int NT>
+__global__ __launch_bounds__(NT) void tile_fill(const SpecDesc* __restrict__ descs,
+                                                const int2* __restrict__ work) {
+  __shared__ int s_nb[26];
+  __shared__ int s_nnb;
+  __shared__ int s_wtot[NT / 64];
+  __shared__ int s_hpre[17];  // prefix of this tile's per-wave hole counts
+  __shared__ unsigned int s_bits[kTailBitmapBits / 32];
+  const int2 wk = work[blockIdx.x];
+  const SpecDesc& d = descs[wk.x];
+  const GridDesc& g = d.g;
+  if (g.ctrl->halt != 0) return;
+  const int tile = wk.y;
+  const int NW = d.waves;
+  const int dims = g.dims;
+  const int base = d.off[tile];
+  const int n = d.cnt[tile];
+  const int nn = d.newcnt[tile];
+  const int I = d.inc[tile];
+  const int stride = d.mv_stride[tile];
+  if (threadIdx.x == 0) {
+    s_nnb = neighbours(g, tile, s_nb);
+    s_hpre[0] = 0;
+    for (int w = 0; w < NW; w++) s_hpre[w + 1] = s_hpre[w] + d.mv_cnt[tile * NW + w];
+  }
+  __syncthreads();
+  const int H = s_hpre[NW];
+  // hole k (ascending) -> slot relative to base
+  auto hole = [&](int k) -> int {
+    int w = 0;
+    while (k >= s_hpre[w + 1]) w++;
+    return d.midx[base + w * stride + (k - s_hpre[w])];
+  };
+  // number of holes below nn
+  int hlo = H;
+  if (I < H) {
+    // holes are ascending: count those < nn by scanning (H is small)
+    int c = 0;
+    for (int k = threadIdx.x; k < H; k += NT) c += hole(k) < nn;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+    if ((threadIdx.x & 63) == 0) s_wtot[threadIdx.x >> 6] = c;
+    __syncthreads();
+    hlo = 0;
+    for (int w = 0; w < NT / 64; w++) hlo += s_wtot[w];
+    __syncthreads();
+  }
+  auto target = [&](int k) -> int { return k < hlo ? hole(k) : n + (k - H); };
+
+  // pass A: incoming movers
+  int rank0 = 0;
+  for (int k = 0; k < s_nnb; k++) {
+    const int nt = s_nb[k];
+    const int nstride = d.mv_stride[nt];
+    for (int w = 0; w < NW; w++) {
+      const int c = d.mv_cnt[nt * NW + w];
+      const int r0 = d.off[nt] + w * nstride;
+      for (int i0 = 0; i0 < c; i0 += NT) {
+        const int i = i0 + threadIdx.x;
+        const bool f = i < c && tile_of(g, d.mcell[r0 + i]) == tile;
+        int tot;
+        const int r = block_rank<NT>(f, s_wtot, tot);
+        if (f) copy_from_mover(d, r0 + i, base + target(rank0 + r), dims);
+        rank0 += tot;
+      }
+    }
+  }
+  // pass B: tail particles into the remaining holes
+  if (I < H) {
+    const int tail = n - nn;  // = H - I
+    for (int i = threadIdx.x; i < (tail + 31) / 32; i += NT) s_bits[i] = 0u;
+    __syncthreads();
+    for (int k = hlo + threadIdx.x; k < H; k += NT) {
+      const int t = hole(k) - nn;
+      atomicOr(&s_bits[t >> 5], 1u << (t & 31));
+    }
+    __syncthreads();
+    int r1 = 0;
+    for (int i0 = 0; i0 < tail; i0 += NT) {
+      const int i = i0 + threadIdx.x;
+      const bool f = i < tail && !((s_bits[i >> 5] >> (i & 31)) & 1u);
+      int tot;
+      const int r = block_rank<NT>(f, s_wtot, tot);
+      if (f) copy_particle(d, base + nn + i, base + target(I + r1 + r), dims);
+      r1 += tot;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) d.cnt[tile] = nn;
+}
+
+// ------------------------------------------------------------------------------------
+// explicit instantiations
+// ------------------------------------------------------------------------------------
+#define OSR_INST(O, P)                                                                    \
+  template __global__ void push_deposit_2d<O, P, 256>(const SpecDesc*, const int2*);      \
+  template __global__ void push_deposit_3d<O, P, 256>(const SpecDesc*, const int2*);
+OSR_INST(1, 0)
+OSR_INST(2, 0)
+OSR_INST(3, 0)
+OSR_INST(1, 1)
+OSR_INST(2, 1)
+OSR_INST(3, 1)
+template __global__ void tile_count<256>(const SpecDesc*, const int2*);
+template __global__ void tile_fill<256>(const SpecDesc*, const int2*);
+
+}  // namespace osr

@@ -1,0 +1,467 @@
+// osr_device.hpp -- device-side building blocks of the PIC hot path for gfx950.
+//
+// Arithmetic follows the reference's AVX2 single precision kernels operation by
+// operation (the same fused multiply-adds in the same places, IEEE division and square
+// root), so that gather, push and position advance give bit-identical results to
+// FNAME(vadvance_deposit_*) of simd/os-spec-push-avx2.c on the same inputs.  The file is
+// compiled with -ffp-contract=off: every FMA below is explicit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define OSR_DEV __device__ __forceinline__
+
+namespace osr {
+
+OSR_DEV float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// Stencil parameters per interpolation order (emf/os-emf-interpolate.f03:385-461,
+// spec/os-spec-current.f03:2495-2533).  LP: lower stencil point of spline/wl.
+template <int ORDER> struct Ord {
+  static constexpr int NP = ORDER + 1;
+  static constexpr int LP = ORDER == 1 ? 0 : -1;
+  // field halo needed by the gather (half-shifted stencil reaches one lower)
+  static constexpr int GLO = 1 - LP;                // 1, 2, 2
+  static constexpr int GHI = LP + ORDER;            // 1, 1, 2
+  // current halo touched by the deposit (virtual particle cells are ix-1..ix+1)
+  static constexpr int JLO = 1 - LP;                // 1, 2, 2
+  static constexpr int JHI = 1 + LP + ORDER;        // 2, 2, 3
+};
+
+// ------------------------------------------------------------------------------------
+// Splines: simd/splines-avx2.h:26-123 (vspline_s1..s3, vsplineh_s1..s3).
+// ------------------------------------------------------------------------------------
+template <int ORDER> OSR_DEV void spline(float x, float* w);
+template <> OSR_DEV void spline<1>(float x, float* w) {
+  w[0] = 0.5f - x;
+  w[1] = 0.5f + x;
+}
+template <> OSR_DEV void spline<2>(float x, float* w) {
+  float t0 = 0.5f - x, t1 = 0.5f + x;
+  w[0] = (t0 * t0) * 0.5f;
+  w[1] = fma_(t0, t1, 0.5f);
+  w[2] = (t1 * t1) * 0.5f;
+}
+template <> OSR_DEV void spline<3>(float x, float* w) {
+  const float c1_6 = 1.0f / 6.0f, c2_3 = 2.0f / 3.0f;
+  float t0 = 0.5f - x, t1 = 0.5f + x;
+  float t2 = t0 * t0, t3 = t1 * t1;
+  t0 = t0 * t2;
+  t1 = t1 * t3;
+  w[0] = c1_6 * t0;
+  w[1] = fma_(0.5f, t1, c2_3 - t3);
+  w[2] = fma_(0.5f, t0, c2_3 - t2);
+  w[3] = c1_6 * t1;
+}
+// h is 1.0f when x < 0, else 0.0f
+template <int ORDER> OSR_DEV void splineh(float x, float h, float* w);
+template <> OSR_DEV void splineh<1>(float x, float h, float* w) {
+  w[0] = (1.0f - h) - x;
+  w[1] = h + x;
+}
+template <> OSR_DEV void splineh<2>(float x, float h, float* w) {
+  float t0 = (1.0f - h) - x, t1 = h + x;
+  w[0] = (t0 * t0) * 0.5f;
+  w[1] = fma_(t0, t1, 0.5f);
+  w[2] = (t1 * t1) * 0.5f;
+}
+template <> OSR_DEV void splineh<3>(float x, float h, float* w) {
+  const float c1_6 = 1.0f / 6.0f, c2_3 = 2.0f / 3.0f;
+  float t0 = (1.0f - h) - x, t1 = h + x;
+  float t2 = t0 * t0, t3 = t1 * t1;
+  t0 = t0 * t2;
+  t1 = t1 * t3;
+  w[0] = c1_6 * t0;
+  w[1] = fma_(0.5f, t1, c2_3 - t3);
+  w[2] = fma_(0.5f, t0, c2_3 - t2);
+  w[3] = c1_6 * t1;
+}
+
+// Longitudinal weights wl[0..NP-1] with wl[NP-1] = 0: simd/os-spec-current-avx2.c:20-64
+template <int ORDER> OSR_DEV void wl(float qn, float x0, float x1, float* w);
+template <> OSR_DEV void wl<1>(float qn, float x0, float x1, float* w) {
+  w[0] = qn * (x1 - x0);
+  w[1] = 0.0f;
+}
+template <> OSR_DEV void wl<2>(float qn, float x0, float x1, float* w) {
+  float d = x1 - x0, s1_2 = 0.5f - x0, p1_2 = 0.5f + x0;
+  float n = qn * d;
+  w[0] = n * fma_(-0.5f, d, s1_2);
+  w[1] = n * fma_(0.5f, d, p1_2);
+  w[2] = 0.0f;
+}
+template <> OSR_DEV void wl<3>(float qn, float x0, float x1, float* w) {
+  const float c1_3 = 1.0f / 3.0f;
+  float d = x1 - x0, s = 0.5f - x0, p = 0.5f + x0, d_3 = c1_3 * d;
+  float a = 0.5f * fma_(s, s, -(d * (s - d_3)));
+  float b = fma_(-d, x0 + d_3, fma_(-x0, x0, 0.75f));
+  float c = 0.5f * fma_(p, p, d * (p + d_3));
+  float n = qn * d;
+  w[0] = n * a;
+  w[1] = n * b;
+  w[2] = n * c;
+  w[3] = 0.0f;
+}
+
+// ------------------------------------------------------------------------------------
+// Boris push: simd/os-spec-push-avx2.c:90-146 (VDUDT_BORIS).  e/b are the gathered
+// fields; returns the time centred u^2/(gamma+1) (per particle, float, as vene).
+// ------------------------------------------------------------------------------------
+OSR_DEV float boris(float tem, float e1, float e2, float e3, float b1, float b2, float b3,
+                    float& u1, float& u2, float& u3) {
+  e1 = e1 * tem;
+  e2 = e2 * tem;
+  e3 = e3 * tem;
+  float ut1 = u1 + e1, ut2 = u2 + e2, ut3 = u3 + e3;
+  float utsq = fma_(ut3, ut3, fma_(ut2, ut2, ut1 * ut1));
+  float gamma = __builtin_sqrtf(utsq + 1.0f);
+  float ene = utsq / (gamma + 1.0f);
+  float tg = tem / gamma;
+  b1 = b1 * tg;
+  b2 = b2 * tg;
+  b3 = b3 * tg;
+  float v1 = fma_(b3, ut2, ut1);
+  float v2 = fma_(b1, ut3, ut2);
+  float v3 = fma_(b2, ut1, ut3);
+  v1 = fma_(-b2, ut3, v1);
+  v2 = fma_(-b3, ut1, v2);
+  v3 = fma_(-b1, ut2, v3);
+  float otsq = fma_(b1, b1, 1.0f);
+  otsq = fma_(b2, b2, otsq);
+  otsq = fma_(b3, b3, otsq);
+  otsq = 2.0f / otsq;
+  b1 = b1 * otsq;
+  b2 = b2 * otsq;
+  b3 = b3 * otsq;
+  ut1 = fma_(b3, v2, ut1);
+  ut2 = fma_(b1, v3, ut2);
+  ut3 = fma_(b2, v1, ut3);
+  ut1 = fma_(-b2, v3, ut1);
+  ut2 = fma_(-b3, v1, ut2);
+  ut3 = fma_(-b1, v2, ut3);
+  u1 = ut1 + e1;
+  u2 = ut2 + e2;
+  u3 = ut3 + e3;
+  return ene;
+}
+
+// ------------------------------------------------------------------------------------
+// Vay push: spec/os-spec-push.f03:636-796, in the Fortran operation order (no FMA).
+// Returns uc^2/(gamma_c+1) of the time centred momentum (:772-792).
+// ------------------------------------------------------------------------------------
+OSR_DEV float vay(float tem, float e1, float e2, float e3, float b1, float b2, float b3,
+                  float& u1, float& u2, float& u3) {
+  e1 *= tem; e2 *= tem; e3 *= tem;
+  b1 *= tem; b2 *= tem; b3 *= tem;
+  float rgam = 1.0f / __builtin_sqrtf(1.0f + u1 * u1 + u2 * u2 + u3 * u3);
+  float v1 = u1 * rgam, v2 = u2 * rgam, v3 = u3 * rgam;
+  float bpsq = b1 * b1 + b2 * b2 + b3 * b3;
+  float ut1 = u1 + e1 + v2 * b3 - v3 * b2;
+  float ut2 = u2 + e2 + v3 * b1 - v1 * b3;
+  float ut3 = u3 + e3 + v1 * b2 - v2 * b1;
+  float w1 = ut1 + e1, w2 = ut2 + e2, w3 = ut3 + e3;
+  float ustar = w1 * b1 + w2 * b2 + w3 * b3;
+  float sigma = 1.0f + w1 * w1 + w2 * w2 + w3 * w3 - bpsq;
+  rgam = 1.0f / __builtin_sqrtf(0.5f * (sigma + __builtin_sqrtf(sigma * sigma +
+                                                                 4.0f * (bpsq + ustar * ustar))));
+  b1 *= rgam; b2 *= rgam; b3 *= rgam;
+  float spar = 1.0f / (1.0f + b1 * b1 + b2 * b2 + b3 * b3);
+  float uptp = w1 * b1 + w2 * b2 + w3 * b3;
+  float o1 = u1, o2 = u2, o3 = u3;
+  u1 = spar * (w1 + uptp * b1 + w2 * b3 - w3 * b2);
+  u2 = spar * (w2 + uptp * b2 + w3 * b1 - w1 * b3);
+  u3 = spar * (w3 + uptp * b3 + w1 * b2 - w2 * b1);
+  float uc1 = 0.5f * (o1 + u1), uc2 = 0.5f * (o2 + u2), uc3 = 0.5f * (o3 + u3);
+  float uc2s = uc1 * uc1 + uc2 * uc2 + uc3 * uc3;
+  float gc = __builtin_sqrtf(uc2s + 1.0f);
+  return uc2s / (gc + 1.0f);
+}
+
+template <int PUSHER>
+OSR_DEV float push(float tem, float e1, float e2, float e3, float b1, float b2, float b3,
+                   float& u1, float& u2, float& u3) {
+  if (PUSHER == 0) return boris(tem, e1, e2, e3, b1, b2, b3, u1, u2, u3);
+  return vay(tem, e1, e2, e3, b1, b2, b3, u1, u2, u3);
+}
+
+// 1/gamma: simd/os-spec-push-avx2.c:156-161 (VRGAMMA)
+OSR_DEV float rgamma(float u1, float u2, float u3) {
+  return 1.0f / __builtin_sqrtf(fma_(u3, u3, fma_(u2, u2, fma_(u1, u1, 1.0f))));
+}
+
+// Cell shift keeping x in [-0.5, 0.5): spec/os-spec-push.f03:1332-1359, vmntrim
+OSR_DEV float trim(float x) { return (x >= 0.5f ? 1.0f : 0.0f) - (x < -0.5f ? 1.0f : 0.0f); }
+
+// ------------------------------------------------------------------------------------
+// Fixed point conversion for the deterministic current accumulation.  For |v*scale|
+// < 2^51, fma(v, scale, 1.5*2^52) rounds v*scale to the nearest integer in the low
+// mantissa bits; subtracting the bit pattern of 1.5*2^52 yields it as int64.
+// ------------------------------------------------------------------------------------
+OSR_DEV long long to_fix(float v, double scale) {
+  double t = __builtin_fma((double)v, scale, 6755399441055744.0);
+  return __double_as_longlong(t) - 0x4338000000000000LL;
+}
+
+// ------------------------------------------------------------------------------------
+// Virtual particles.  split_2d: spec/os-spec-current.f03:659-888, vsplit2D
+// simd/split-vec.h:105-222 (scalar IEEE arithmetic, as compiled by the reference).
+// ------------------------------------------------------------------------------------
+struct VP2 {
+  float x0, y0, x1, y1, q, vz;
+  int di, dj;  // cell offset relative to the particle's old cell
+};
+
+OSR_DEV int split2d(float xo, float yo, float xn, float yn, int d1, int d2, float q, float vz,
+                    VP2* vp) {
+  // All three records are written unconditionally at the end from scalar variables;
+  // conditional stores into different array elements would be merged by the compiler
+  // into stores through a selected pointer, which forces the array into scratch memory.
+  const int cross = (d1 != 0) + 2 * (d2 != 0);
+  const float f1 = (float)d1, f2 = (float)d2;
+  VP2 a = {xo, yo, xn, yn, q, vz, 0, 0};
+  VP2 b = a, c = a;
+  int n = 1;
+  if (cross == 1) {
+    const float xint = 0.5f * f1;
+    const float delta = (xint - xo) / (xn - xo);
+    const float yint = yo + (yn - yo) * delta;
+    a = {xo, yo, xint, yint, q, vz * delta, 0, 0};
+    b = {-xint, yint, xn - f1, yn, q, vz * (1.0f - delta), d1, 0};
+    n = 2;
+  } else if (cross == 2) {
+    const float yint = 0.5f * f2;
+    const float delta = (yint - yo) / (yn - yo);
+    const float xint = xo + (xn - xo) * delta;
+    a = {xo, yo, xint, yint, q, vz * delta, 0, 0};
+    b = {xint, -yint, xn, yn - f2, q, vz * (1.0f - delta), 0, d2};
+    n = 2;
+  } else if (cross == 3) {
+    const float xint = 0.5f * f1;
+    float delta = (xint - xo) / (xn - xo);
+    const float yint = yo + (yn - yo) * delta;
+    if (yint >= -0.5f && yint < 0.5f) {
+      const float vzint = vz * delta;
+      const float vzint2 = vz * (1.0f - delta);
+      const float yint2 = 0.5f * f2;
+      delta = (yint2 - yint) / (yn - yint);
+      const float xint2 = -xint + (xn - xint) * delta;
+      a = {xo, yo, xint, yint, q, vzint, 0, 0};
+      b = {-xint, yint, xint2, yint2, q, vzint2 * delta, d1, 0};
+      c = {xint2, -yint2, xn - f1, yn - f2, q, vzint2 * (1.0f - delta), d1, d2};
+    } else {
+      const float vzint = vz * delta;
+      const float yint2 = 0.5f * f2;
+      delta = (yint2 - yo) / (yint - yo);
+      const float xint2 = xo + (xint - xo) * delta;
+      a = {xo, yo, xint2, yint2, q, vzint * delta, 0, 0};
+      b = {xint2, -yint2, xint, yint - f2, q, vzint * (1.0f - delta), 0, d2};
+      c = {-xint, yint - f2, xn - f1, yn - f2, q, vz - vzint, d1, d2};
+    }
+    n = 3;
+  }
+  vp[0] = a;
+  vp[1] = b;
+  vp[2] = c;
+  return n;
+}
+
+struct VP3 {
+  float x0, y0, z0, x1, y1, z1, q;
+  int di, dj, dk;
+};
+
+// Select virtual particle v of a register array field by field, so that the array
+// stays in registers (no dynamic indexing of private memory).
+// (values are passed by value: a conditional over lvalues would select addresses)
+template <class T>
+OSR_DEV T sel3(int v, T a, T b, T c) { return v == 0 ? a : (v == 1 ? b : c); }
+template <class T>
+OSR_DEV T sel4(int v, T a, T b, T c, T d) { return v == 0 ? a : (v == 1 ? b : (v == 2 ? c : d)); }
+#define OSR_SEL3(F) r.F = sel3(v, vp[0].F, vp[1].F, vp[2].F)
+#define OSR_SEL4(F) r.F = sel4(v, vp[0].F, vp[1].F, vp[2].F, vp[3].F)
+OSR_DEV VP2 pick(const VP2* vp, int v) {
+  VP2 r;
+  OSR_SEL3(x0); OSR_SEL3(y0); OSR_SEL3(x1); OSR_SEL3(y1);
+  OSR_SEL3(q); OSR_SEL3(vz); OSR_SEL3(di); OSR_SEL3(dj);
+  return r;
+}
+OSR_DEV VP3 pick(const VP3* vp, int v) {
+  VP3 r;
+  OSR_SEL4(x0); OSR_SEL4(y0); OSR_SEL4(z0); OSR_SEL4(x1); OSR_SEL4(y1); OSR_SEL4(z1);
+  OSR_SEL4(q); OSR_SEL4(di); OSR_SEL4(dj); OSR_SEL4(dk);
+  return r;
+}
+#undef OSR_SEL3
+#undef OSR_SEL4
+
+// split_3d: spec/os-spec-current.f03:895-1601 (vsplit3D, simd/split-vec.h:231-).
+OSR_DEV VP3 mk3(float x0, float y0, float z0, float x1, float y1, float z1, float q, int i, int j,
+                int k) {
+  VP3 v;
+  v.x0 = x0; v.y0 = y0; v.z0 = z0; v.x1 = x1; v.y1 = y1; v.z1 = z1; v.q = q;
+  v.di = i; v.dj = j; v.dk = k;
+  return v;
+}
+
+OSR_DEV int split3d(float xo, float yo, float zo, float xn, float yn, float zn, int d1, int d2,
+                    int d3, float q, VP3* vp) {
+  const int cross = (d1 != 0) + 2 * (d2 != 0) + 4 * (d3 != 0);
+  float xint, yint, zint, xint2, yint2, zint2, delta;
+  const float f1 = (float)d1, f2 = (float)d2, f3 = (float)d3;
+  switch (cross) {
+  case 0:
+    vp[0] = mk3(xo, yo, zo, xn, yn, zn, q, 0, 0, 0);
+    return 1;
+  case 1:
+    xint = 0.5f * f1;
+    delta = (xint - xo) / (xn - xo);
+    yint = yo + (yn - yo) * delta;
+    zint = zo + (zn - zo) * delta;
+    vp[0] = mk3(xo, yo, zo, xint, yint, zint, q, 0, 0, 0);
+    vp[1] = mk3(-xint, yint, zint, xn - f1, yn, zn, q, d1, 0, 0);
+    return 2;
+  case 2:
+    yint = 0.5f * f2;
+    delta = (yint - yo) / (yn - yo);
+    xint = xo + (xn - xo) * delta;
+    zint = zo + (zn - zo) * delta;
+    vp[0] = mk3(xo, yo, zo, xint, yint, zint, q, 0, 0, 0);
+    vp[1] = mk3(xint, -yint, zint, xn, yn - f2, zn, q, 0, d2, 0);
+    return 2;
+  case 3:
+    xint = 0.5f * f1;
+    delta = (xint - xo) / (xn - xo);
+    yint = yo + (yn - yo) * delta;
+    zint = zo + (zn - zo) * delta;
+    if (yint >= -0.5f && yint < 0.5f) {
+      yint2 = 0.5f * f2;
+      delta = (yint2 - yint) / (yn - yint);
+      xint2 = -xint + (xn - xint) * delta;
+      zint2 = zint + (zn - zint) * delta;
+      vp[0] = mk3(xo, yo, zo, xint, yint, zint, q, 0, 0, 0);
+      vp[1] = mk3(-xint, yint, zint, xint2, yint2, zint2, q, d1, 0, 0);
+      vp[2] = mk3(xint2, -yint2, zint2, xn - f1, yn - f2, zn, q, d1, d2, 0);
+    } else {
+      yint2 = 0.5f * f2;
+      delta = (yint2 - yo) / (yint - yo);
+      xint2 = xo + (xint - xo) * delta;
+      zint2 = zo + (zint - zo) * delta;
+      vp[0] = mk3(xo, yo, zo, xint2, yint2, zint2, q, 0, 0, 0);
+      vp[1] = mk3(xint2, -yint2, zint2, xint, yint - f2, zint, q, 0, d2, 0);
+      vp[2] = mk3(-xint, yint - f2, zint, xn - f1, yn - f2, zn, q, d1, d2, 0);
+    }
+    return 3;
+  case 4:
+    zint = 0.5f * f3;
+    delta = (zint - zo) / (zn - zo);
+    xint = xo + (xn - xo) * delta;
+    yint = yo + (yn - yo) * delta;
+    vp[0] = mk3(xo, yo, zo, xint, yint, zint, q, 0, 0, 0);
+    vp[1] = mk3(xint, yint, -zint, xn, yn, zn - f3, q, 0, 0, d3);
+    return 2;
+  case 5:
+    xint = 0.5f * f1;
+    delta = (xint - xo) / (xn - xo);
+    yint = yo + (yn - yo) * delta;
+    zint = zo + (zn - zo) * delta;
+    zint2 = 0.5f * f3;
+    if (zint >= -0.5f && zint < 0.5f) {
+      delta = (zint2 - zint) / (zn - zint);
+      xint2 = -xint + (xn - xint) * delta;
+      yint2 = yint + (yn - yint) * delta;
+      vp[0] = mk3(xo, yo, zo, xint, yint, zint, q, 0, 0, 0);
+      vp[1] = mk3(-xint, yint, zint, xint2, yint2, zint2, q, d1, 0, 0);
+      vp[2] = mk3(xint2, yint2, -zint2, xn - f1, yn, zn - f3, q, d1, 0, d3);
+    } else {
+      delta = (zint2 - zo) / (zint - zo);
+      xint2 = xo + (xint - xo) * delta;
+      yint2 = yo + (yint - yo) * delta;
+      vp[0] = mk3(xo, yo, zo, xint2, yint2, zint2, q, 0, 0, 0);
+      vp[1] = mk3(xint2, yint2, -zint2, xint, yint, zint - f3, q, 0, 0, d3);
+      vp[2] = mk3(-xint, yint, zint - f3, xn - f1, yn, zn - f3, q, d1, 0, d3);
+    }
+    return 3;
+  case 6:
+    yint = 0.5f * f2;
+    delta = (yint - yo) / (yn - yo);
+    xint = xo + (xn - xo) * delta;
+    zint = zo + (zn - zo) * delta;
+    zint2 = 0.5f * f3;
+    if (zint >= -0.5f && zint < 0.5f) {
+      delta = (zint2 - zint) / (zn - zint);
+      xint2 = xint + (xn - xint) * delta;
+      yint2 = -yint + (yn - yint) * delta;
+      vp[0] = mk3(xo, yo, zo, xint, yint, zint, q, 0, 0, 0);
+      vp[1] = mk3(xint, -yint, zint, xint2, yint2, zint2, q, 0, d2, 0);
+      vp[2] = mk3(xint2, yint2, -zint2, xn, yn - f2, zn - f3, q, 0, d2, d3);
+    } else {
+      delta = (zint2 - zo) / (zint - zo);
+      xint2 = xo + (xint - xo) * delta;
+      yint2 = yo + (yint - yo) * delta;
+      vp[0] = mk3(xo, yo, zo, xint2, yint2, zint2, q, 0, 0, 0);
+      vp[1] = mk3(xint2, yint2, -zint2, xint, yint, zint - f3, q, 0, 0, d3);
+      vp[2] = mk3(xint, -yint, zint - f3, xn, yn - f2, zn - f3, q, 0, d2, d3);
+    }
+    return 3;
+  default: {  // 7: x, y and z crossings
+    xint = 0.5f * f1;
+    delta = (xint - xo) / (xn - xo);
+    yint = yo + (yn - yo) * delta;
+    zint = zo + (zn - zo) * delta;
+    if (yint >= -0.5f && yint < 0.5f) {
+      yint2 = 0.5f * f2;
+      delta = (yint2 - yint) / (yn - yint);
+      xint2 = -xint + (xn - xint) * delta;
+      zint2 = zint + (zn - zint) * delta;
+      vp[0] = mk3(xo, yo, zo, xint, yint, zint, q, 0, 0, 0);
+      vp[1] = mk3(-xint, yint, zint, xint2, yint2, zint2, q, d1, 0, 0);
+      vp[2] = mk3(xint2, -yint2, zint2, xn - f1, yn - f2, zn, q, d1, d2, 0);
+    } else {
+      yint2 = 0.5f * f2;
+      delta = (yint2 - yo) / (yint - yo);
+      xint2 = xo + (xint - xo) * delta;
+      zint2 = zo + (zint - zo) * delta;
+      vp[0] = mk3(xo, yo, zo, xint2, yint2, zint2, q, 0, 0, 0);
+      vp[1] = mk3(xint2, -yint2, zint2, xint, yint - f2, zint, q, 0, d2, 0);
+      vp[2] = mk3(-xint, yint - f2, zint, xn - f1, yn - f2, zn, q, d1, d2, 0);
+    }
+    // one of the three needs an extra z split (os-spec-current.f03:1561-1590); the new
+    // virtual particle is appended last and the ones after the split one move one cell
+    // in z.  Static indices only (no dynamic register-array indexing).
+    zint = 0.5f * f3;
+    const bool c0 = vp[0].z1 < -0.5f || vp[0].z1 >= 0.5f;
+    const bool c1 = vp[1].z1 < -0.5f || vp[1].z1 >= 0.5f;
+    const bool c2 = vp[2].z1 < -0.5f || vp[2].z1 >= 0.5f;
+    if (!(c0 || c1 || c2)) return 3;
+    const int s = c0 ? 0 : (c1 ? 1 : 2);
+    const VP3 o = pick(vp, s);
+    delta = (zint - o.z0) / (o.z1 - o.z0);
+    xint = o.x0 + (o.x1 - o.x0) * delta;
+    yint = o.y0 + (o.y1 - o.y0) * delta;
+    vp[3] = mk3(xint, yint, -zint, o.x1, o.y1, o.z1 - f3, o.q, o.di, o.dj, o.dk + d3);
+#pragma unroll
+    for (int jj = 0; jj < 3; jj++) {
+      if (jj == s) {
+        vp[jj].x1 = xint;
+        vp[jj].y1 = yint;
+        vp[jj].z1 = zint;
+      } else if (jj > s) {
+        vp[jj].z0 = vp[jj].z0 - f3;
+        vp[jj].z1 = vp[jj].z1 - f3;
+        vp[jj].dk = vp[jj].dk + d3;
+      }
+    }
+    return 4;
+  }
+  }
+}
+
+// Wave-level helpers (64-wide wavefronts)
+OSR_DEV int lane_id() { return __lane_id(); }
+
+OSR_DEV double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+}  // namespace osr

@@ -1,0 +1,111 @@
+// osr_types.hpp -- descriptors shared by host code and kernels.
+//
+// Device data layout of one simulation (all resident in HBM):
+//   fields   E, B : float, planar [3][nz][ny][nx] (interior cells only; periodic
+//                   boundaries are index wraps, so no guard cells are stored);
+//   current  J    : int64 fixed point, planar [3][nz][ny][nx]; the value is J * 2^jshift;
+//   particles     : structure of arrays, grouped by tile.  Tile t owns the slots
+//                   [off[t], off[t]+cap[t]) of every array and its first cnt[t] slots hold
+//                   particles whose cell lies inside the tile.  A particle is
+//                   cell (packed uint32), x, y[, z] in [-0.5, 0.5) relative to the cell
+//                   centre, ux, uy, uz (proper velocity), q, pid (upload order).
+//   movers        : one buffer per species with the same tile layout; the push kernel
+//                   writes particles that leave their tile there, one contiguous region
+//                   per (tile, wave), and the fill kernel moves them into their new tile.
+#pragma once
+#include <stdint.h>
+
+namespace osr {
+
+constexpr int kMaxSpecies = 8;
+constexpr int kTailBitmapBits = 32768;  // fill kernel: max holes left in the tail
+
+// Control block, one per simulation (device memory).
+struct Ctrl {
+  int halt;          // 0 run, 1 a tile overflowed this step, 2 stopped at a step end
+  int error;         // sticky: 1 = particle found outside its tile (layout invariant broken)
+  long long steps;   // completed steps
+};
+
+struct GridDesc {
+  int dims;
+  int nx, ny, nz;
+  long long ncell;
+  int tx, ty, tz;     // tile size in cells
+  int ntx, nty, ntz;  // tiles per dimension
+  int ntiles;
+  float* E;
+  float* B;
+  unsigned long long* J;
+  double jscale;     // 2^jshift
+  double jinv;       // 2^-jshift
+  float dt_dx[3];    // (float)(dt/dx)
+  float jnorm[3];    // (float)(dx/dt/2) in 2D, (float)(dx/dt/3) in 3D
+  Ctrl* ctrl;
+};
+
+struct SpecDesc {
+  GridDesc g;
+  // particles
+  uint32_t* cell;
+  float* x;
+  float* y;
+  float* z;
+  float* ux;
+  float* uy;
+  float* uz;
+  float* q;
+  uint32_t* pid;
+  // movers
+  uint32_t* mcell;
+  float* mx;
+  float* my;
+  float* mz;
+  float* mux;
+  float* muy;
+  float* muz;
+  float* mq;
+  uint32_t* mpid;
+  int* midx;       // slot the mover left (relative to the tile offset)
+  // tiles
+  int* off;
+  int* cnt;
+  int* cap;
+  int* mv_cnt;     // [ntiles][waves]
+  int* mv_stride;  // [ntiles] per-wave region size used by the last push
+  int* newcnt;     // [ntiles]
+  int* inc;        // [ntiles] incoming count
+  double* epart;   // [ntiles] time-centred energy partial sums (un-normalised)
+  float tem;       // (float)(0.5*dt/rqm)
+  int waves;       // waves per workgroup in the push kernel
+};
+
+// Cell packing.  2D: ix | iy<<16.  3D: ix | iy<<11 | iz<<22.
+__host__ __device__ inline uint32_t pack2(int ix, int iy) {
+  return (uint32_t)ix | ((uint32_t)iy << 16);
+}
+__host__ __device__ inline uint32_t pack3(int ix, int iy, int iz) {
+  return (uint32_t)ix | ((uint32_t)iy << 11) | ((uint32_t)iz << 22);
+}
+__host__ __device__ inline void unpack2(uint32_t c, int& ix, int& iy) {
+  ix = (int)(c & 0xFFFFu);
+  iy = (int)(c >> 16);
+}
+__host__ __device__ inline void unpack3(uint32_t c, int& ix, int& iy, int& iz) {
+  ix = (int)(c & 0x7FFu);
+  iy = (int)((c >> 11) & 0x7FFu);
+  iz = (int)(c >> 22);
+}
+
+__host__ __device__ inline int tile_of(const GridDesc& g, uint32_t c) {
+  if (g.dims == 2) {
+    int ix, iy;
+    unpack2(c, ix, iy);
+    return ix / g.tx + g.ntx * (iy / g.ty);
+  }
+  int ix, iy, iz;
+  unpack3(c, ix, iy, iz);
+  return ix / g.tx + g.ntx * (iy / g.ty + g.nty * (iz / g.tz));
+}
+
+}  // namespace osr

@@ -1,0 +1,205 @@
+"""Host-side mirror of the reference's species-push / EMF-solver seams over the C ABI.
+
+``advance_deposit`` mirrors the reference's native fused kernel entry point
+(FNAME(vadvance_deposit_{2d,3d}_sN), simd/os-spec-push-avx2.c:471-475) with the same
+argument meaning: particles in the reference layout are pushed and their current is
+added to ``j``.  ``yee_advance`` mirrors t_emf_solver%advance for periodic boundaries
+(emf/solver/os-emf-solver-yee.f03:542-568).  ``Simulation`` is the resident path: the
+particles and fields stay in HBM and ``run`` advances whole time steps in the order of
+os-simulation.f03:498-666 (push + deposit, energy report, current fold, Yee).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import OSR_BORIS, OSR_VAY, SimDesc, check, lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def device_count() -> int:
+    return lib().osr_device_count()
+
+
+def advance_deposit(dims, order, ix, x, u, q, rqm, e, b, emf_lo, j, j_lo, dx, dt, pusher=OSR_BORIS):
+    """Push all particles of the arrays (in place) and ADD their current to ``j``.
+
+    ``e``, ``b``, ``j`` are grids in the reference layout (numpy arrays of shape
+    ``(ld_z, ld_y, ld_x, 3)`` or ``(ld_y, ld_x, 3)``) with ``emf_lo`` / ``j_lo`` lower guard
+    cells.  Returns the time-centred energy sum (not normalised)."""
+    ix = np.ascontiguousarray(ix, np.int32)
+    x = np.ascontiguousarray(x, np.float32)
+    u = np.ascontiguousarray(u, np.float32)
+    q = np.ascontiguousarray(q, np.float32)
+    e = np.ascontiguousarray(e, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    assert j.dtype == np.float32 and j.flags["C_CONTIGUOUS"]
+    emf_size = np.array(list(reversed(e.shape[:-1])), np.int32)
+    j_size = np.array(list(reversed(j.shape[:-1])), np.int32)
+    emf_off = np.array([emf_lo] * dims, np.int32)
+    j_off = np.array([j_lo] * dims, np.int32)
+    dxa = np.array(list(dx), np.float64)
+    ene = np.zeros(1, np.float64)
+    n = len(q)
+    check(lib().osr_advance_deposit(dims, order, pusher, _p(ix), _p(x), _p(u), _p(q), 1, n,
+                                    float(rqm), _p(e), _p(b), _p(emf_size), _p(emf_off), _p(j),
+                                    _p(j_size), _p(j_off), _p(dxa), float(dt), _p(ene)))
+    return ix, x, u, float(ene[0])
+
+
+def yee_advance(dims, nx, gcl, dx, dt, e, b, jay):
+    """Periodic Yee step on guarded reference-layout arrays (in place)."""
+    assert e.dtype == np.float32 and b.dtype == np.float32
+    jay = np.ascontiguousarray(jay, np.float32)
+    ld = np.array(list(reversed(e.shape[:-1])), np.int32)
+    check(lib().osr_yee_advance(dims, _p(np.array(nx, np.int32)), _p(ld),
+                                _p(np.array(gcl, np.int32)), _p(np.array(dx, np.float64)),
+                                float(dt), _p(e), _p(b), _p(jay)))
+
+
+class Simulation:
+    """A resident simulation on the GPU (one per grid; species share the grid)."""
+
+    def __init__(self, dims, nx, dx, dt, order, tile=None):
+        d = SimDesc()
+        d.dims = dims
+        for i in range(3):
+            d.nx[i] = nx[i] if i < dims else 1
+            d.dx[i] = dx[i] if i < dims else 1.0
+            d.tile[i] = (tile[i] if tile is not None and i < dims else 0)
+        d.dt = dt
+        d.order = order
+        self.dims, self.nx, self.dx, self.dt, self.order = dims, tuple(nx), tuple(dx), dt, order
+        h = C.c_void_p()
+        check(lib().osr_sim_create(C.byref(d), C.byref(h)))
+        self._h = h
+        self.nspecies = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().osr_sim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- species
+    def add_species(self, rqm, pusher=OSR_BORIS):
+        sid = C.c_int()
+        check(lib().osr_sim_add_species(self._h, float(rqm), int(pusher), C.byref(sid)))
+        self.nspecies += 1
+        return sid.value
+
+    def set_particles(self, sid, ix, x, u, q):
+        ix = np.ascontiguousarray(ix, np.int32)
+        x = np.ascontiguousarray(x, np.float32)
+        u = np.ascontiguousarray(u, np.float32)
+        q = np.ascontiguousarray(q, np.float32)
+        check(lib().osr_species_set(self._h, sid, len(q), _p(ix), _p(x), _p(u), _p(q)))
+
+    def generate(self, sid, ppc, density=1.0, ufl=(0, 0, 0), uth=(0, 0, 0), seed=0, random_pos=True):
+        ppc = np.array(list(ppc) + [1] * (3 - len(ppc)), np.int32)
+        check(lib().osr_species_generate(self._h, sid, _p(ppc), float(density),
+                                         _p(np.array(ufl, np.float64)),
+                                         _p(np.array(uth, np.float64)), int(seed), int(random_pos)))
+
+    def count(self, sid):
+        n = C.c_int64()
+        check(lib().osr_species_count(self._h, sid, C.byref(n)))
+        return n.value
+
+    def get_particles(self, sid):
+        n = self.count(sid)
+        ix = np.zeros((n, self.dims), np.int32)
+        x = np.zeros((n, self.dims), np.float32)
+        u = np.zeros((n, 3), np.float32)
+        q = np.zeros(n, np.float32)
+        check(lib().osr_species_get(self._h, sid, _p(ix), _p(x), _p(u), _p(q)))
+        return ix, x, u, q
+
+    # ---- fields (reference layout)
+    def set_fields(self, e=None, b=None, gcl=None):
+        ref = e if e is not None else b
+        ld = np.array(list(reversed(ref.shape[:-1])), np.int32)
+        gcl = np.array(gcl if gcl is not None else [0] * self.dims, np.int32)
+        e = None if e is None else np.ascontiguousarray(e, np.float32)
+        b = None if b is None else np.ascontiguousarray(b, np.float32)
+        check(lib().osr_fields_set(self._h, _p(e), _p(b), _p(ld), _p(gcl)))
+
+    def get_fields(self, gcl=None, gcu=None):
+        gcl = list(gcl) if gcl is not None else [0] * self.dims
+        gcu = list(gcu) if gcu is not None else [0] * self.dims
+        ld = [self.nx[d] + gcl[d] + gcu[d] for d in range(self.dims)]
+        e = np.zeros(tuple(reversed(ld)) + (3,), np.float32)
+        b = np.zeros_like(e)
+        check(lib().osr_fields_get(self._h, _p(e), _p(b), _p(np.array(ld, np.int32)),
+                                   _p(np.array(gcl, np.int32))))
+        return e, b
+
+    def fill_fields(self, e0=None, b0=None):
+        check(lib().osr_fields_fill(self._h, _p(None if e0 is None else np.array(e0, np.float64)),
+                                    _p(None if b0 is None else np.array(b0, np.float64))))
+
+    # ---- stepping and diagnostics
+    def run(self, nsteps):
+        check(lib().osr_sim_run(self._h, int(nsteps)))
+
+    @property
+    def steps(self):
+        s = C.c_int64()
+        check(lib().osr_sim_steps(self._h, C.byref(s)))
+        return s.value
+
+    def history(self, first=0, count=None):
+        if count is None:
+            count = self.steps - first
+        fld = np.zeros((count, 6))
+        par = np.zeros((count, max(self.nspecies, 1)))
+        check(lib().osr_sim_history(self._h, first, count, _p(fld), _p(par)))
+        return fld, par[:, :self.nspecies]
+
+    def gauss(self, want_field=False):
+        n = int(np.prod(self.nx[:self.dims]))
+        res = np.zeros(n) if want_field else None
+        m = np.zeros(1)
+        check(lib().osr_sim_gauss(self._h, _p(res), _p(m)))
+        if want_field:
+            return float(m[0]), res.reshape(tuple(reversed(self.nx[:self.dims])))
+        return float(m[0])
+
+    @property
+    def relayouts(self):
+        n = C.c_int64()
+        check(lib().osr_sim_relayouts(self._h, C.byref(n)))
+        return n.value
+
+    def time_kernels(self, nsteps):
+        ms = np.zeros(6)
+        check(lib().osr_sim_time_kernels(self._h, int(nsteps), _p(ms)))
+        return {"push_deposit": ms[0], "tile_count": ms[1], "tile_fill": ms[2], "yee": ms[3],
+                "finalize": ms[4], "step": ms[5]}
+
+    @property
+    def footprint(self):
+        n = C.c_int64()
+        check(lib().osr_sim_footprint(self._h, C.byref(n)))
+        return n.value
+
+
+__all__ = ["Simulation", "advance_deposit", "yee_advance", "device_count", "OSR_BORIS", "OSR_VAY"]

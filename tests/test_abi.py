@@ -1,0 +1,68 @@
+"""CPU checks of the C ABI: the library loads without a GPU, exports every function the
+header declares, the ctypes signatures cover them, and calls fail cleanly (status, no
+crash) when no GPU is present or arguments are invalid."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+import osiris_amd as oa
+from osiris_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "osiris_amd.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"\b(?:int|void|const char\s*\*)\s*\*?\s*(\w+)\s*\(", txt)
+    return sorted(set(names))
+
+
+def test_header_declares_expected_entry_points():
+    names = header_functions()
+    for n in ("osr_advance_deposit", "osr_sim_create", "osr_sim_run", "osr_yee_advance",
+              "hip_vadvance_deposit_2d_s2_", "hip_vadvance_deposit_3d_s3_"):
+        assert n in names
+    assert len(names) >= 25
+
+
+def test_library_exports_every_declared_symbol():
+    L = oa.lib()
+    for n in header_functions():
+        assert hasattr(L, n), n
+    assert set(header_functions()) == set(_lib.SIGNATURES)
+
+
+def test_errors_without_gpu_are_status_codes():
+    L = oa.lib()
+    if L.osr_device_count() > 0:
+        pytest.skip("GPU present")
+    d = _lib.SimDesc()
+    d.dims = 2
+    d.nx[0] = d.nx[1] = 16
+    d.dx[0] = d.dx[1] = 0.1
+    d.dt = 0.05
+    d.order = 2
+    h = C.c_void_p()
+    st = L.osr_sim_create(C.byref(d), C.byref(h))
+    assert st == 2  # OSR_ERR_HIP
+    assert "GPU" in L.osr_last_error().decode()
+
+
+def test_argument_validation():
+    L = oa.lib()
+    d = _lib.SimDesc()
+    d.dims = 4
+    h = C.c_void_p()
+    assert L.osr_sim_create(C.byref(d), C.byref(h)) == 1
+    d.dims = 2
+    d.nx[0] = d.nx[1] = 16
+    d.dx[0] = d.dx[1] = 0.1
+    d.order = 2
+    d.dt = 0.2  # violates the Courant condition of the Yee scheme
+    assert L.osr_sim_create(C.byref(d), C.byref(h)) == 1
+    assert "stability" in L.osr_last_error().decode()
+    assert L.osr_advance_deposit(5, 2, 0, *([None] * 4), 1, 0, -1.0, *([None] * 8), 0.1,
+                                 None) == 1
