@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Headline benchmark: particle-updates/s of the full PIC time step on one MI355X.
+
+Workload (the reference's headline configuration): 2D periodic counter-streaming
+electron beams, 1024 x 1024 cells, dx = dy = 0.05, dt = 0.035, 64 particles per cell
+split equally into two beams drifting at u_x = +/-0.2 with u_th = 0.01 (uniform random
+in-cell positions, seeded), quadratic splines, Boris pusher, FP32.  A step is one full
+time step: field gather + push + charge-conserving deposit + current fold + Yee +
+energy diagnostics, all resident in HBM.
+
+Prints ONE JSON line with the metric, the roofline of the dominant kernel (push +
+deposit, measured with HIP events on the simulation stream) and a CPU baseline: the
+reference's own AVX2 FP32 kernel (simd/os-spec-push-avx2.c) on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-updates/s (push+deposit+Yee) 2D quad 64ppc; ensemble p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes (FP32 SoA): per particle read ix2+x2+u3+q = 32 B, write
+# ix2+x2+u3 = 28 B; per cell the gather reads E,B (24 B) and the deposit writes J (12 B).
+BYTES_PER_PARTICLE = 60.0
+BYTES_PER_CELL_PUSH = 36.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(nx, ppc_total, n_total, order, dx, dt, seconds, threads):
+    """The reference's AVX2 FP32 fused kernel on a strip of the same workload, host
+    threads with private currents (the reference's OpenMP scheme) + FP32 Yee on the full
+    grid; extrapolated to the full particle count."""
+    from oracle import oracle as O
+
+    if not O.ref_available("s"):
+        return None
+    rng = np.random.default_rng(12345)
+    rows = max(1, min(nx[1], 4_194_304 // (nx[0] * ppc_total)))
+    g = O.make_grid(2, nx, order)
+    glo = g.gcl[0]
+    e = g.zeros(3, np.float32)
+    b = g.zeros(3, np.float32)
+    e[...] = rng.normal(0, 1e-3, e.shape)
+    b[...] = rng.normal(0, 1e-3, b.shape)
+    cells = np.stack(np.meshgrid(np.arange(1, nx[0] + 1), np.arange(1, rows + 1), indexing="ij"),
+                     -1).reshape(-1, 2)
+    ix = np.repeat(cells, ppc_total, axis=0).astype(np.int32)
+    n = len(ix)
+    x = rng.uniform(-0.5, 0.5, (n, 2)).astype(np.float32)
+    u = rng.normal(0, 0.01, (n, 3)).astype(np.float32)
+    u[: n // 2, 0] += 0.2
+    u[n // 2:, 0] -= 0.2
+    q = np.full(n, -1.0 / ppc_total, np.float32)
+    timer = O.RefSimdTimer(order, ix, x, u, q, -1.0, e, b, glo, dx, dt, threads)
+    timer.step()  # warm
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        timer.step()
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    t_push = el / reps  # push + deposit + reduce for n particles
+    # FP32 Yee of the reference restatement on the full grid
+    j = g.zeros(3, np.float32)
+    t1 = time.perf_counter()
+    ny = 3
+    for _ in range(ny):
+        O.yee_advance(g, dx, dt, e, b, j, prec="f32")
+    t_yee = (time.perf_counter() - t1) / ny
+    t_step = t_push * (n_total / n) + t_yee
+    return {
+        "value": n_total / t_step,
+        "unit": "particle-updates/s",
+        "cores": threads,
+        "kind": "reference",
+        "sample": (f"reference FNAME(vadvance_deposit_2d_s{order}) AVX2 FP32 "
+                   f"(simd/os-spec-push-avx2.c, gcc -O2 -mavx2 -mfma) on {n} particles "
+                   f"({nx[0]}x{rows} cells x {ppc_total} ppc, two beams), {threads} threads "
+                   f"with private J + reduction, {reps} steps in {el:.1f} s; plus FP32 Yee "
+                   f"(oracle) on the full {nx[0]}x{nx[1]} grid ({t_yee * 1e3:.1f} ms); "
+                   f"extrapolated to {n_total} particles"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--nx", type=int, nargs=2, default=[1024, 1024])
+    ap.add_argument("--ppc", type=int, nargs=2, default=[8, 4],
+                    help="particles per cell per beam along x and y (2 beams)")
+    ap.add_argument("--order", type=int, default=2)
+    ap.add_argument("--tile", type=int, nargs=2, default=[16, 16])
+    ap.add_argument("--timing-steps", type=int, default=10,
+                    help="steps measured per kernel with HIP events (roofline)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles",
+                                                      "traffic_push_deposit_2d_c2.json"))
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU path "
+                                   "(spatial domain decomposition) is design-only",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import osiris_amd as oa
+
+    if oa.device_count() < 1:
+        print(json.dumps({"error": "no GPU visible"}))
+        sys.exit(1)
+
+    nx = tuple(args.nx)
+    dx, dt = (0.05, 0.05), 0.035
+    ppc_beam = int(np.prod(args.ppc))
+    sim = oa.Simulation(2, nx, dx, dt, args.order, tile=tuple(args.tile))
+    for k, ufl in enumerate((0.2, -0.2)):
+        sid = sim.add_species(-1.0, oa.OSR_BORIS)
+        sim.generate(sid, args.ppc, density=0.5, ufl=(ufl, 0.0, 0.0), uth=(0.01, 0.01, 0.01),
+                     seed=1000 + k, random_pos=True)
+    n_total = sum(sim.count(s) for s in range(2))
+    cells = nx[0] * nx[1]
+    log(f"workload: {nx[0]}x{nx[1]} cells, {n_total} particles, footprint "
+        f"{sim.footprint / 2**30:.2f} GiB")
+
+    sim.run(args.warmup)
+    t0 = time.perf_counter()
+    sim.run(args.steps)  # returns after the device finished (status read back)
+    el = time.perf_counter() - t0
+    ms_step = el * 1e3 / args.steps
+    value = n_total * args.steps / el
+    log(f"timed: {args.steps} steps in {el:.3f} s -> {ms_step:.3f} ms/step, {value:.4e} upd/s")
+
+    # per-kernel timing with HIP events on the simulation stream
+    kt = sim.time_kernels(args.timing_steps)
+    nsp = 2
+    t_launch = kt["push_deposit"] / nsp * 1e-3  # s per push launch (one species)
+    bytes_launch = (n_total / nsp) * BYTES_PER_PARTICLE + cells * BYTES_PER_CELL_PUSH
+    achieved = bytes_launch / t_launch / 1e9
+    log("kernel ms/step: " + ", ".join(f"{k} {v:.4f}" for k, v in kt.items()))
+    traffic = None
+    if os.path.exists(args.traffic):
+        try:
+            with open(args.traffic) as f:
+                traffic = json.load(f).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+    fld, par = sim.history()
+    relayouts = sim.relayouts
+    sim.close()
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded counter-based RNG on the device)",
+        "config": {
+            "workload": (f"2D periodic counter-streaming electron beams {nx[0]}x{nx[1]}, "
+                         f"dx=0.05, dt=0.035, {2 * ppc_beam} ppc (2 beams, u_x=+/-0.2, "
+                         f"u_th=0.01), order {args.order} splines, Boris"),
+            "particles": n_total,
+            "cells": cells,
+            "tile": list(args.tile),
+            "relayouts": relayouts,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "push_deposit_2d<2,Boris> (one launch per species)",
+            "kernel_ms": t_launch * 1e3,
+            "algorithmic_bytes": bytes_launch,
+        },
+        "kernels_ms_per_step": kt,
+    }
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cb = cpu_baseline(nx, 2 * ppc_beam, n_total, args.order, dx, dt, args.cpu_seconds,
+                          threads)
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -183,6 +183,14 @@ int osr_sim_time_kernels(osr_sim *sim, int nsteps, double *ms);
 /* Bytes of device memory held by a simulation. */
 int osr_sim_footprint(osr_sim *sim, int64_t *bytes);
 
+/* ---------------------------------------------------------------------------------
+ * Measurement helper (no counterpart in the reference): streams `bytes` bytes through a
+ * plain device copy kernel `reps` times and returns the mean kernel time in *ms; used
+ * to calibrate hardware traffic counters (FETCH_SIZE / WRITE_SIZE) and as a bandwidth
+ * probe.  Allocates and frees its own buffers.
+ * ------------------------------------------------------------------------------- */
+int osr_copy_probe(int64_t bytes, int reps, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

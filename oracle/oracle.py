@@ -403,3 +403,65 @@ class Sim:
         dv = div(self.grid, self.dx, e, self.prec)
         return self.grid.interior(dv)[..., 0].astype(np.float64) - \
             self.grid.interior(rho)[..., 0].astype(np.float64)
+
+
+# --------------------------------------------------------------------------------------
+# CPU baseline: the reference's AVX2 kernel run the way the reference runs it with
+# OpenMP (particles/os-particles.f03:1416-1443): T threads, disjoint particle chunks,
+# private current arrays, then a fixed-order reduction.
+# --------------------------------------------------------------------------------------
+class RefSimdTimer:
+    """Times FNAME(vadvance_deposit_2d_s<order>) (FP32 AVX2) on host threads."""
+
+    def __init__(self, order, ix, x, u, q, rqm, e, b, glo, dx, dt, threads):
+        import threading  # noqa: F401
+
+        self.L = _ref("s")
+        self.fn = getattr(self.L, f"vadvance_deposit_2d_s{order}_")
+        self.threads = threads
+        n = len(q)
+        chunks = np.array_split(np.arange(n), threads)
+        self.e = _copy_aligned(e, np.float32)
+        self.b = _copy_aligned(b, np.float32)
+        self.emf_size = np.array(list(reversed(e.shape[:-1])), np.int32)
+        self.off = np.array([glo, glo], np.int32)
+        self.dx = np.array(list(dx), np.float64)
+        self.dt = C.c_double(dt)
+        self.rqm = C.c_float(rqm)
+        self.parts = []
+        for c in chunks:
+            m = len(c)
+            pad = m + 16
+            ixp = _aligned((pad, 2), np.int32, 1)
+            ixp[:m] = ix[c]
+            xp = _aligned((pad, 2), np.float32)
+            xp[:m] = x[c]
+            up = _aligned((pad, 3), np.float32)
+            up[:m] = u[c]
+            qp = _aligned((pad,), np.float32)
+            qp[:m] = q[c]
+            jj = _aligned(e.shape, np.float32)
+            self.parts.append((m, ixp, xp, up, qp, jj))
+
+    def _run(self, k):
+        m, ixp, xp, up, qp, jj = self.parts[k]
+        C.memset(_p(jj), 0, jj.nbytes)  # each thread zeroes its private current
+        i0, i1 = C.c_int(1), C.c_int(m)
+        ene = C.c_double(0)
+        self.fn(_p(ixp), _p(xp), _p(up), _p(qp), C.byref(i0), C.byref(i1), C.byref(self.rqm),
+                _p(self.e), _p(self.b), _p(self.emf_size), _p(self.off), _p(jj),
+                _p(self.emf_size), _p(self.off), _p(self.dx), C.byref(self.dt), C.byref(ene))
+
+    def step(self):
+        """One push+deposit of every particle on all threads, then the J reduction."""
+        import threading
+
+        ts = [threading.Thread(target=self._run, args=(k,)) for k in range(self.threads)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        j = self.parts[0][5]
+        for p in self.parts[1:]:
+            j += p[5]
+        return j

@@ -51,6 +51,7 @@ SIGNATURES = {
     "osr_sim_relayouts": (I, [P, P]),
     "osr_sim_time_kernels": (I, [P, I, P]),
     "osr_sim_footprint": (I, [P, P]),
+    "osr_copy_probe": (I, [I64, I, P]),
 }
 
 

@@ -1140,6 +1140,39 @@ int osr_sim_time_kernels(osr_sim* s, int nsteps, double* ms) {
   return OSR_OK;
 }
 
+__global__ __launch_bounds__(256) void copy_kernel(const float4* __restrict__ src,
+                                                   float4* __restrict__ dst, long long n) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (long long)gridDim.x * 256)
+    dst[i] = src[i];
+}
+
+int osr_copy_probe(int64_t bytes, int reps, double* ms) {
+  if (bytes < 16 || reps < 1 || !ms) return fail(OSR_ERR_ARG, "bad argument");
+  const long long n = bytes / 16;
+  DBuf a, b;
+  TRY(a.alloc((size_t)n * 16));
+  TRY(b.alloc((size_t)n * 16));
+  HIPC(hipMemset(a.p, 0, (size_t)n * 16));
+  HIPC(hipDeviceSynchronize());
+  hipEvent_t e0, e1;
+  HIPC(hipEventCreate(&e0));
+  HIPC(hipEventCreate(&e1));
+  const int grid = 256 * 64;
+  hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, 0, a.as<float4>(), b.as<float4>(), n);
+  HIPC(hipEventRecord(e0, 0));
+  for (int r = 0; r < reps; r++)
+    hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, 0, a.as<float4>(), b.as<float4>(), n);
+  HIPC(hipEventRecord(e1, 0));
+  HIPC(hipEventSynchronize(e1));
+  float t = 0;
+  HIPC(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = t / reps;
+  return OSR_OK;
+}
+
 // ------------------------------------------------------------------------------------
 // stateless drop-ins
 // ------------------------------------------------------------------------------------
