@@ -104,6 +104,10 @@ def main():
                     help="particles per cell per beam along x and y (2 beams)")
     ap.add_argument("--order", type=int, default=2)
     ap.add_argument("--tile", type=int, nargs=2, default=[16, 16])
+    ap.add_argument("--autotune", action="store_true", default=True)
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false")
+    ap.add_argument("--plain-steps", type=int, default=20,
+                    help="steps of the one-kernel-per-phase GPU path (0 = skip)")
     ap.add_argument("--timing-steps", type=int, default=10,
                     help="steps measured per kernel with HIP events (roofline)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -128,11 +132,25 @@ def main():
     nx = tuple(args.nx)
     dx, dt = (0.05, 0.05), 0.035
     ppc_beam = int(np.prod(args.ppc))
-    sim = oa.Simulation(2, nx, dx, dt, args.order, tile=tuple(args.tile))
-    for k, ufl in enumerate((0.2, -0.2)):
-        sid = sim.add_species(-1.0, oa.OSR_BORIS)
-        sim.generate(sid, args.ppc, density=0.5, ufl=(ufl, 0.0, 0.0), uth=(0.01, 0.01, 0.01),
-                     seed=1000 + k, random_pos=True)
+
+    def make(tile, plain=False):
+        s = oa.Simulation(2, nx, dx, dt, args.order, tile=tuple(tile), plain=plain)
+        for k, ufl in enumerate((0.2, -0.2)):
+            sid = s.add_species(-1.0, oa.OSR_BORIS)
+            s.generate(sid, args.ppc, density=0.5, ufl=(ufl, 0.0, 0.0), uth=(0.01, 0.01, 0.01),
+                       seed=1000 + k, random_pos=True)
+        return s
+
+    tile = tuple(args.tile)
+    tune = None
+    if args.autotune:
+        from osiris_amd.autotune import tune_tile
+
+        key = f"2d-o{args.order}-boris-{nx[0]}x{nx[1]}-ppc{2 * ppc_beam}-beams"
+        tile, tune = tune_tile(make, key, dims=2, use_cache=False)
+        log("autotune (ms/step): " + ", ".join(f"{'x'.join(map(str, t))}: {v:.3f}"
+                                                for t, v in tune.items()) + f" -> {tile}")
+    sim = make(tile)
     n_total = sum(sim.count(s) for s in range(2))
     cells = nx[0] * nx[1]
     log(f"workload: {nx[0]}x{nx[1]} cells, {n_total} particles, footprint "
@@ -164,6 +182,23 @@ def main():
     relayouts = sim.relayouts
     sim.close()
 
+    plain = None
+    if args.plain_steps > 0:
+        ps = make(tile, plain=True)
+        ps.run(2)
+        t0 = time.perf_counter()
+        ps.run(args.plain_steps)
+        pel = time.perf_counter() - t0
+        pkt = ps.time_kernels(3)
+        ps.close()
+        plain = {"value": n_total * args.plain_steps / pel,
+                 "ms_per_step": pel * 1e3 / args.plain_steps,
+                 "kernels_ms_per_step": pkt,
+                 "speedup_fused_over_plain": (n_total * args.plain_steps / pel) and
+                 value / (n_total * args.plain_steps / pel)}
+        log(f"plain one-kernel-per-phase path: {plain['ms_per_step']:.3f} ms/step, "
+            f"fused/plain = {plain['speedup_fused_over_plain']:.2f}x")
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -183,7 +218,9 @@ def main():
                          f"u_th=0.01), order {args.order} splines, Boris"),
             "particles": n_total,
             "cells": cells,
-            "tile": list(args.tile),
+            "tile": list(tile),
+            "autotune_ms_per_step": ({"x".join(map(str, t)): v for t, v in tune.items()}
+                                     if tune else None),
             "relayouts": relayouts,
         },
         "roofline": {
@@ -198,6 +235,7 @@ def main():
             "algorithmic_bytes": bytes_launch,
         },
         "kernels_ms_per_step": kt,
+        "plain_path": plain,
     }
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

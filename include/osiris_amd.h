@@ -126,8 +126,18 @@ typedef struct {
   double dt;      /* time step (Courant condition of the Yee scheme is checked) */
   int order;      /* interpolation / deposit order 1..3 */
   int tile[3];    /* tile size in cells; 0 = default (2D 16x16, 3D 8x8x4) */
-  int reserved[4];
+  int path;       /* OSR_PATH_FUSED (default) or OSR_PATH_PLAIN (2D only) */
+  int kernel;     /* fused push kernel: 0 auto, 1 generic (runtime tile shape, int64 LDS
+                     current), 2 2D compile-time tile with VP queue and per-wave FP32 LDS
+                     current (tiles 8x8, 16x16, 32x16) */
+  int reserved[2];
 } osr_sim_desc;
+
+/* OSR_PATH_FUSED: tiled fused push+deposit, LDS-staged fields and current, particles
+ * sorted into tiles every step, reproducible current.  OSR_PATH_PLAIN: one kernel per
+ * phase (gather, push, move/split, deposit with global FP32 atomics), intermediate
+ * arrays in HBM, no sorting -- the comparison baseline; not reproducible run to run. */
+enum { OSR_PATH_FUSED = 0, OSR_PATH_PLAIN = 1 };
 
 int osr_sim_create(const osr_sim_desc *desc, osr_sim **out);
 int osr_sim_destroy(osr_sim *sim);
@@ -169,16 +179,18 @@ int osr_sim_steps(osr_sim *sim, int64_t *steps);
  * time-centred kinetic energy, normalised as HIST/fld_ene and HIST/par_ene
  * (os-emf-diagnostics.f03:44-55, os-spec-define.f03:1030-1086). */
 int osr_sim_history(osr_sim *sim, int64_t first, int64_t count, double *fld, double *par);
-/* Gauss-law residual div E - rho (FP64) on the interior cell corners; res may be NULL.
+/* Gauss-law residual div E - rho (FP64) on the interior cell corners, with rho deposited
+ * by DEPOSIT_RHO (spec/os-spec-charge.f90:500-580) and div E by div_vdf
+ * (vdf/os-vdf-math.f90:1013-1073).  res and rho (ncell doubles each) may be NULL;
  * *maxabs receives max |res|. */
-int osr_sim_gauss(osr_sim *sim, double *res, double *maxabs);
+int osr_sim_gauss(osr_sim *sim, double *res, double *rho, double *maxabs);
 /* Particle re-layouts done because a tile ran out of capacity. */
 int osr_sim_relayouts(osr_sim *sim, int64_t *count);
 
 /* Per-kernel timing of nsteps steps, measured with HIP events on the simulation's
  * stream (the steps are really advanced).  ms[k] = mean duration per step of kernel
- * class k: 0 push+deposit, 1 tile count, 2 tile fill, 3 Yee (3 sub-steps),
- * 4 energy finalisation; ms[5] = mean wall time per step between events. */
+ * class k: 0 push+deposit (all species), 1 tile fill, 2 Yee (3 sub-steps), 3 energy
+ * finalisation; ms[4] = mean time per step from the first to the last event; ms[5] = 0. */
 int osr_sim_time_kernels(osr_sim *sim, int nsteps, double *ms);
 /* Bytes of device memory held by a simulation. */
 int osr_sim_footprint(osr_sim *sim, int64_t *bytes);

@@ -360,10 +360,10 @@ class Sim:
         return _lib(self.prec)._real
 
     def add_species(self, rqm, pusher, ix, x, u, q):
+        """Copies the arrays (the simulation advances its own copies in place)."""
         r = self.real
-        self.species.append(Species(rqm, pusher, np.ascontiguousarray(ix, np.int32),
-                                    np.ascontiguousarray(x, r), np.ascontiguousarray(u, r),
-                                    np.ascontiguousarray(q, r)))
+        self.species.append(Species(rqm, pusher, np.array(ix, np.int32), np.array(x, r),
+                                    np.array(u, r), np.array(q, r)))
 
     def set_interior(self, which, arr):
         """Set interior E or B from an array of shape (nz, ny, nx, 3) (C order)."""

@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(_HERE, "lib", "libosiris_amd.so")
 OSR_OK = 0
 OSR_BORIS = 0
 OSR_VAY = 1
+OSR_PATH_FUSED = 0
+OSR_PATH_PLAIN = 1
 
 P = C.c_void_p
 I = C.c_int
@@ -47,7 +49,7 @@ SIGNATURES = {
     "osr_sim_run": (I, [P, I]),
     "osr_sim_steps": (I, [P, P]),
     "osr_sim_history": (I, [P, I64, I64, P, P]),
-    "osr_sim_gauss": (I, [P, P, P]),
+    "osr_sim_gauss": (I, [P, P, P, P]),
     "osr_sim_relayouts": (I, [P, P]),
     "osr_sim_time_kernels": (I, [P, I, P]),
     "osr_sim_footprint": (I, [P, P]),
@@ -58,7 +60,7 @@ SIGNATURES = {
 class SimDesc(C.Structure):
     _fields_ = [("dims", C.c_int), ("nx", C.c_int * 3), ("dx", C.c_double * 3),
                 ("dt", C.c_double), ("order", C.c_int), ("tile", C.c_int * 3),
-                ("reserved", C.c_int * 4)]
+                ("path", C.c_int), ("kernel", C.c_int), ("reserved", C.c_int * 2)]
 
 
 class OsrError(RuntimeError):

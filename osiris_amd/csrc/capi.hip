@@ -19,6 +19,8 @@
 #include "k_field.hip"
 #include "k_loader.hip"
 #include "k_tiles.hip"
+#include "k_push2d.hip"
+#include "k_plain.hip"
 
 using namespace osr;
 
@@ -95,6 +97,9 @@ struct Species {
   DBuf mcell, mx, my, mz, mux, muy, muz, mq, mpid, midx;
   DBuf off, cnt, cap, mv_cnt, mv_stride, newcnt, inc, epart;
   std::vector<int> h_off, h_cap;
+  // plain path intermediates
+  DBuf pl_ep, pl_bp, pl_vp, pl_nvp;
+  int epart_n = 0;
 };
 
 // host-side particle list used by uploads and re-layouts
@@ -128,6 +133,7 @@ struct osr_sim {
   bool graph_ok = false;
   int64_t steps = 0;
   int64_t relayouts = 0;
+  bool plain = false;
   size_t lds_push = 0;
   Ctrl* h_ctrl = nullptr;  // pinned mirror
 };
@@ -216,7 +222,7 @@ int upload_descs(osr_sim* s) {
   f.fe_scale = 0.5 * dvol;
   for (size_t i = 0; i < s->sp.size(); i++) {
     f.ep[i] = s->sp[i]->epart.as<double>();
-    f.ep_n[i] = s->ntiles;
+    f.ep_n[i] = s->sp[i]->epart_n;
     f.ep_scale[i] = (double)s->sp[i]->rqm * dvol;
   }
   f.hist = s->hist.as<double>();
@@ -226,7 +232,8 @@ int upload_descs(osr_sim* s) {
     f.edt[d] = d < s->dims ? (float)(s->dt / s->dx[d]) : 0.f;
   }
   f.dtif = (float)s->dt;
-  f.Jf = nullptr;
+  f.Jf = s->plain ? s->J.as<float>() : nullptr;
+  f.jf_zero = s->plain ? 1 : 0;
   HIPC(hipMemcpy(s->d_fd.p, &f, sizeof(FieldDesc), hipMemcpyHostToDevice));
   return OSR_OK;
 }
@@ -282,13 +289,21 @@ int alloc_species(osr_sim* s, Species& p, const std::vector<int>& cnt) {
   TRY(p.mv_stride.alloc(4 * (size_t)nt));
   TRY(p.newcnt.alloc(4 * (size_t)nt));
   TRY(p.inc.alloc(4 * (size_t)nt));
-  TRY(p.epart.alloc(8 * (size_t)nt));
+  p.epart_n = s->plain ? (int)((total + 255) / 256) : nt;
+  TRY(p.epart.alloc(8 * (size_t)p.epart_n));
+  if (s->plain) {
+    TRY(p.pl_ep.alloc(12 * n));
+    TRY(p.pl_bp.alloc(12 * n));
+    TRY(p.pl_vp.alloc(7 * 3 * 4 * n));
+    TRY(p.pl_nvp.alloc(4 * n));
+  }
   HIPC(hipMemcpy(p.off.p, off.data(), 4 * nt, hipMemcpyHostToDevice));
   HIPC(hipMemcpy(p.cnt.p, cnt.data(), 4 * nt, hipMemcpyHostToDevice));
   HIPC(hipMemcpy(p.cap.p, cap.data(), 4 * nt, hipMemcpyHostToDevice));
   HIPC(hipMemset(p.mv_cnt.p, 0, 4 * (size_t)nt * kWaves));
   HIPC(hipMemset(p.mv_stride.p, 0, 4 * (size_t)nt));
-  HIPC(hipMemset(p.epart.p, 0, 8 * (size_t)nt));
+  HIPC(hipMemset(p.epart.p, 0, 8 * (size_t)p.epart_n));
+  HIPC(hipMemset(p.inc.p, 0, 4 * (size_t)nt));
   // the memsets above run on the null stream; the simulation stream must see them
   HIPC(hipDeviceSynchronize());
   p.h_off = off;
@@ -458,11 +473,15 @@ int collect(osr_sim* s, Species& p, bool pending, HostParts& hp) {
     hp.uz.push_back(uz_[i]);
     hp.q.push_back(q_[i]);
   };
+  // after a halted step, tiles whose fill ran have inc >= 0 and already hold their
+  // incoming movers; the others still have their holes and their movers are pending
+  std::vector<int> inc;
+  if (pending) TRY(download(inc, p.inc, nt));
   std::vector<char> hole;
   for (int t = 0; t < nt; t++) {
     const int base = p.h_off[t];
     hole.assign((size_t)cnt[t], 0);
-    if (pending)
+    if (pending && inc[t] < 0)
       for (int w = 0; w < kWaves; w++)
         for (int k = 0; k < mvc[t * kWaves + w]; k++) {
           const int h = midx[base + w * mvs[t] + k];
@@ -474,8 +493,11 @@ int collect(osr_sim* s, Species& p, bool pending, HostParts& hp) {
   if (pending)
     for (int t = 0; t < nt; t++)
       for (int w = 0; w < kWaves; w++)
-        for (int k = 0; k < mvc[t * kWaves + w]; k++)
-          push(mcell, mpid, mx, my, mz, mux, muy, muz, mq, (size_t)p.h_off[t] + w * mvs[t] + k);
+        for (int k = 0; k < mvc[t * kWaves + w]; k++) {
+          const size_t m = (size_t)p.h_off[t] + w * mvs[t] + k;
+          if (inc[tile_of(s->g, mcell[m])] < 0)
+            push(mcell, mpid, mx, my, mz, mux, muy, muz, mq, m);
+        }
   return OSR_OK;
 }
 
@@ -524,19 +546,29 @@ push_fn pick_push(int dims, int order, int pusher) {
   return nullptr;
 }
 
-int launch_push(osr_sim* s, int sid) {
-  push_fn f = pick_push(s->dims, s->order, s->sp[sid]->pusher);
-  if (!f) return fail(OSR_ERR_ARG, "no push kernel for dims=%d order=%d", s->dims, s->order);
-  hipLaunchKernelGGL(f, dim3(s->ntiles), dim3(kNT), s->lds_push, s->stream,
-                     s->d_spec.as<SpecDesc>(), s->d_work.as<int2>() + (size_t)sid * s->ntiles);
-  HIPC(hipGetLastError());
-  return OSR_OK;
+// 2D kernels with compile-time tile shape (static LDS); nullptr if not instantiated
+push_fn pick_push2d_t(int order, int pusher, int tx, int ty) {
+#define OSR_T(O, P, X, Y) \
+  if (order == O && pusher == P && tx == X && ty == Y) return push_deposit_2d_t<O, P, X, Y>;
+#define OSR_TT(X, Y) \
+  OSR_T(1, 0, X, Y) OSR_T(2, 0, X, Y) OSR_T(3, 0, X, Y) OSR_T(1, 1, X, Y) OSR_T(2, 1, X, Y) OSR_T(3, 1, X, Y)
+  OSR_TT(8, 8) OSR_TT(16, 16) OSR_TT(32, 16)
+#undef OSR_TT
+#undef OSR_T
+  return nullptr;
 }
 
-int launch_tiles(osr_sim* s) {
-  const int nw = (int)s->sp.size() * s->ntiles;
-  hipLaunchKernelGGL(tile_count<kNT>, dim3(nw), dim3(kNT), 0, s->stream, s->d_spec.as<SpecDesc>(),
-                     s->d_work.as<int2>());
+int launch_push(osr_sim* s, int sid) {
+  push_fn f = nullptr;
+  size_t lds = s->lds_push;
+  if (s->dims == 2 && s->desc.kernel != 1) {
+    f = pick_push2d_t(s->order, s->sp[sid]->pusher, s->g.tx, s->g.ty);
+    if (f) lds = 0;
+  }
+  if (!f) f = pick_push(s->dims, s->order, s->sp[sid]->pusher);
+  if (!f) return fail(OSR_ERR_ARG, "no push kernel for dims=%d order=%d", s->dims, s->order);
+  hipLaunchKernelGGL(f, dim3(s->ntiles), dim3(kNT), lds, s->stream,
+                     s->d_spec.as<SpecDesc>(), s->d_work.as<int2>() + (size_t)sid * s->ntiles);
   HIPC(hipGetLastError());
   return OSR_OK;
 }
@@ -572,12 +604,39 @@ int launch_finalize(osr_sim* s) {
   return OSR_OK;
 }
 
+int launch_plain(osr_sim* s, int sid) {
+  Species& p = *s->sp[sid];
+  if (p.np == 0) return OSR_OK;
+  PlainBufs b;
+  b.ep = p.pl_ep.as<float>();
+  b.bp = p.pl_bp.as<float>();
+  b.vp = p.pl_vp.as<float>();
+  b.nvp = p.pl_nvp.as<int>();
+  b.Jf = s->J.as<float>();
+  b.cap = p.cap_total;
+  const SpecDesc* d = s->d_spec.as<SpecDesc>();
+  const unsigned nb = (unsigned)((p.np + 255) / 256), nb3 = (unsigned)((3 * p.np + 255) / 256);
+  typedef void (*pk)(const SpecDesc*, int, PlainBufs);
+  pk gth = s->order == 1 ? plain_gather<1> : (s->order == 2 ? plain_gather<2> : plain_gather<3>);
+  pk psh = p.pusher == OSR_BORIS ? plain_push<0> : plain_push<1>;
+  pk dep = s->order == 1 ? plain_deposit<1> : (s->order == 2 ? plain_deposit<2> : plain_deposit<3>);
+  hipLaunchKernelGGL(gth, dim3(nb), dim3(256), 0, s->stream, d, sid, b);
+  hipLaunchKernelGGL(psh, dim3(nb), dim3(256), 0, s->stream, d, sid, b);
+  hipLaunchKernelGGL(plain_move, dim3(nb), dim3(256), 0, s->stream, d, sid, b);
+  hipLaunchKernelGGL(dep, dim3(nb3), dim3(256), 0, s->stream, d, sid, b);
+  HIPC(hipGetLastError());
+  return OSR_OK;
+}
+
 int enqueue_step(osr_sim* s) {
-  for (size_t i = 0; i < s->sp.size(); i++) TRY(launch_push(s, (int)i));
-  if (!s->sp.empty()) {
-    TRY(launch_tiles(s));
-    TRY(launch_fill(s));
+  if (s->plain) {
+    for (size_t i = 0; i < s->sp.size(); i++) TRY(launch_plain(s, (int)i));
+    TRY(launch_yee(s));
+    TRY(launch_finalize(s));
+    return OSR_OK;
   }
+  for (size_t i = 0; i < s->sp.size(); i++) TRY(launch_push(s, (int)i));
+  if (!s->sp.empty()) TRY(launch_fill(s));
   TRY(launch_yee(s));
   TRY(launch_finalize(s));
   return OSR_OK;
@@ -672,8 +731,14 @@ int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
       return fail(OSR_ERR_ARG, "Yee stability condition violated: dt=%g > %g", desc->dt, cour);
   }
   if (osr_device_count() < 1) return fail(OSR_ERR_HIP, "no GPU visible");
+  if (desc->path != OSR_PATH_FUSED && desc->path != OSR_PATH_PLAIN)
+    return fail(OSR_ERR_ARG, "unknown path");
+  if (desc->path == OSR_PATH_PLAIN && dims != 2)
+    return fail(OSR_ERR_ARG, "the plain path is 2D only");
+  if (desc->kernel < 0 || desc->kernel > 2) return fail(OSR_ERR_ARG, "unknown kernel variant");
   std::unique_ptr<osr_sim> s(new osr_sim());
   s->desc = *desc;
+  s->plain = desc->path == OSR_PATH_PLAIN;
   s->dims = dims;
   s->order = desc->order;
   s->dt = desc->dt;
@@ -682,6 +747,7 @@ int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
     s->dx[d] = desc->dx[d];
     tile[d] = desc->tile[d] > 0 ? desc->tile[d] : (dims == 2 ? 16 : (d < 2 ? 8 : 4));
     tile[d] = std::min(tile[d], desc->nx[d]);
+    if (desc->path == OSR_PATH_PLAIN) tile[d] = desc->nx[d];  // one "tile": no sorting
   }
   GridDesc& g = s->g;
   g.dims = dims;
@@ -701,7 +767,7 @@ int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
     g.dt_dx[d] = d < dims ? (float)(desc->dt / desc->dx[d]) : 0.f;
     g.jnorm[d] = d < dims ? (float)(desc->dx[d] / desc->dt / (dims == 2 ? 2.0 : 3.0)) : 0.f;
   }
-  s->lds_push = push_lds_bytes(dims, s->order, tile);
+  s->lds_push = s->plain ? 0 : push_lds_bytes(dims, s->order, tile);
   if (s->lds_push > 160 * 1024) return fail(OSR_ERR_ARG, "tile too large for LDS");
   HIPC(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   const size_t nc = (size_t)g.ncell;
@@ -894,7 +960,10 @@ int osr_species_generate(osr_sim* s, int sid, const int* ppc, double density,
   a.q = q;
   a.seed = seed;
   a.random_pos = random_pos;
-  hipLaunchKernelGGL(generate_particles, dim3(s->ntiles), dim3(256), 0, s->stream,
+  int maxc = 0;
+  for (int v : cnt) maxc = std::max(maxc, v);
+  const unsigned ych = (unsigned)std::max(1, std::min(65535, maxc / 4096));
+  hipLaunchKernelGGL(generate_particles, dim3(s->ntiles, ych), dim3(256), 0, s->stream,
                      s->d_spec.as<SpecDesc>(), sid, a);
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(s->stream));
@@ -1037,7 +1106,7 @@ int osr_sim_history(osr_sim* s, int64_t first, int64_t count, double* fld, doubl
   return OSR_OK;
 }
 
-int osr_sim_gauss(osr_sim* s, double* res, double* maxabs) {
+int osr_sim_gauss(osr_sim* s, double* res, double* rho_out, double* maxabs) {
   TRY(check_sim(s));
   const size_t nc = (size_t)s->g.ncell;
   DBuf rho, dres;
@@ -1068,6 +1137,11 @@ int osr_sim_gauss(osr_sim* s, double* res, double* maxabs) {
   for (double v : h) m = std::max(m, std::fabs(v));
   if (maxabs) *maxabs = m;
   if (res) std::memcpy(res, h.data(), 8 * nc);
+  if (rho_out) {
+    std::vector<long long> r(nc);
+    HIPC(hipMemcpy(r.data(), rho.p, 8 * nc, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < nc; i++) rho_out[i] = (double)r[i] / scale;
+  }
   return OSR_OK;
 }
 
@@ -1104,17 +1178,15 @@ int osr_sim_time_kernels(osr_sim* s, int nsteps, double* ms) {
   int done = 0;
   for (int k = 0; k < nsteps; k++) {
     HIPC(hipEventRecord(ev[0], s->stream));
-    for (size_t i = 0; i < s->sp.size(); i++) TRY(launch_push(s, (int)i));
+    for (size_t i = 0; i < s->sp.size(); i++) TRY(s->plain ? launch_plain(s, (int)i) : launch_push(s, (int)i));
     HIPC(hipEventRecord(ev[1], s->stream));
-    if (!s->sp.empty()) TRY(launch_tiles(s));
+    if (!s->sp.empty() && !s->plain) TRY(launch_fill(s));
     HIPC(hipEventRecord(ev[2], s->stream));
-    if (!s->sp.empty()) TRY(launch_fill(s));
-    HIPC(hipEventRecord(ev[3], s->stream));
     TRY(launch_yee(s));
-    HIPC(hipEventRecord(ev[4], s->stream));
+    HIPC(hipEventRecord(ev[3], s->stream));
     TRY(launch_finalize(s));
-    HIPC(hipEventRecord(ev[5], s->stream));
-    HIPC(hipEventSynchronize(ev[5]));
+    HIPC(hipEventRecord(ev[4], s->stream));
+    HIPC(hipEventSynchronize(ev[4]));
     Ctrl c;
     TRY(read_ctrl(s, c));
     if (c.error) return fail(OSR_ERR_LAYOUT, "particle found outside its tile");
@@ -1127,12 +1199,12 @@ int osr_sim_time_kernels(osr_sim* s, int nsteps, double* ms) {
       continue;  // a re-laid-out step is not representative
     }
     float t;
-    for (int i = 0; i < 5; i++) {
+    for (int i = 0; i < 4; i++) {
       HIPC(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
       acc[i] += t;
     }
-    HIPC(hipEventElapsedTime(&t, ev[0], ev[5]));
-    acc[5] += t;
+    HIPC(hipEventElapsedTime(&t, ev[0], ev[4]));
+    acc[4] += t;
     done++;
   }
   for (int i = 0; i < nev; i++) (void)hipEventDestroy(ev[i]);

@@ -33,7 +33,8 @@ struct FieldDesc {
   float edt[3];
   float dtif;
   // when set, the E step reads a float current from here instead of the fixed-point J
-  const float* Jf;
+  float* Jf;
+  int jf_zero;  // clear Jf after reading it (the plain path accumulates into it)
 };
 
 OSR_DEV int wrp(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
@@ -132,6 +133,11 @@ __global__ __launch_bounds__(256) void yee_dedt(const FieldDesc* __restrict__ fd
     j1 = fd.Jf[idx];
     j2 = fd.Jf[nc + idx];
     j3 = fd.Jf[2 * nc + idx];
+    if (fd.jf_zero) {
+      fd.Jf[idx] = 0.f;
+      fd.Jf[nc + idx] = 0.f;
+      fd.Jf[2 * nc + idx] = 0.f;
+    }
   } else {
     const double jinv = g.jinv;
     j1 = (float)((double)(long long)J[idx] * jinv);
@@ -216,6 +222,27 @@ __global__ __launch_bounds__(256) void finalize_step(const FieldDesc* __restrict
 // DEPOSIT_RHO_2D/3D (spec/os-spec-charge.f90:500-580), accumulated in int64 fixed point,
 // and div E (vdf/os-vdf-math.f90:1013-1073, offset 0).
 // ------------------------------------------------------------------------------------
+// charge-density splines in FP64 (spec/os-spec-charge.f90 uses the interpolation
+// splines, emf/os-emf-interpolate.f03:171-293)
+template <int ORDER>
+OSR_DEV void dspline(double x, double* w) {
+  const double t0 = 0.5 - x, t1 = 0.5 + x;
+  if (ORDER == 1) {
+    w[0] = t0;
+    w[1] = t1;
+  } else if (ORDER == 2) {
+    w[0] = 0.5 * (t0 * t0);
+    w[1] = 0.5 + t0 * t1;
+    w[2] = 0.5 * (t1 * t1);
+  } else {
+    const double t2 = t0 * t0, t3 = t1 * t1;
+    w[0] = (t0 * t2) / 6.0;
+    w[1] = (2.0 / 3.0 - t3) + 0.5 * (t1 * t3);
+    w[2] = (2.0 / 3.0 - t2) + 0.5 * (t0 * t2);
+    w[3] = (t1 * t3) / 6.0;
+  }
+}
+
 template <int ORDER>
 __global__ __launch_bounds__(256) void rho_deposit(const SpecDesc* __restrict__ descs,
                                                    const int2* __restrict__ work,
@@ -227,54 +254,39 @@ __global__ __launch_bounds__(256) void rho_deposit(const SpecDesc* __restrict__ 
   const GridDesc& g = d.g;
   const int tile = wk.y;
   const int base = d.off[tile], n = d.cnt[tile];
+  const int dims = g.dims;
   for (int p = threadIdx.x; p < n; p += 256) {
     const int s = base + p;
     int ix, iy, iz = 0;
-    double w1[NP], w2[NP], w3[NP];
-    float fw[NP];
-    if (g.dims == 2)
+    if (dims == 2)
       unpack2(d.cell[s], ix, iy);
     else
       unpack3(d.cell[s], ix, iy, iz);
-    // splines in double precision of the (float) positions
-    {
-      const double xs[3] = {(double)d.x[s], (double)d.y[s], g.dims == 3 ? (double)d.z[s] : 0.0};
-      for (int dd = 0; dd < g.dims; dd++) {
-        const double x = xs[dd];
-        double* w = dd == 0 ? w1 : (dd == 1 ? w2 : w3);
-        if (ORDER == 1) {
-          w[0] = 0.5 - x;
-          w[1] = 0.5 + x;
-        } else if (ORDER == 2) {
-          const double t0 = 0.5 - x, t1 = 0.5 + x;
-          w[0] = 0.5 * t0 * t0;
-          w[1] = 0.5 + t0 * t1;
-          w[2] = 0.5 * t1 * t1;
-        } else {
-          const double t0 = 0.5 - x, t1 = 0.5 + x;
-          w[0] = t0 * t0 * t0 / 6.0;
-          w[1] = (2.0 / 3.0 - t1 * t1) + 0.5 * t1 * t1 * t1;
-          w[2] = (2.0 / 3.0 - t0 * t0) + 0.5 * t0 * t0 * t0;
-          w[NP - 1] = t1 * t1 * t1 / 6.0;
+    double w1[NP], w2[NP], w3[NP];
+    dspline<ORDER>((double)d.x[s], w1);
+    dspline<ORDER>((double)d.y[s], w2);
+    if (dims == 3)
+      dspline<ORDER>((double)d.z[s], w3);
+    else
+#pragma unroll
+      for (int k = 0; k < NP; k++) w3[k] = k == 0 ? 1.0 : 0.0;
+    const double q = d.q[s];
+#pragma unroll
+    for (int k3 = 0; k3 < NP; k3++) {
+      if (dims == 2 && k3 > 0) break;
+      const long long zoff =
+          dims == 3 ? (long long)g.nx * g.ny * wrp(iz + O::LP + k3, g.nz) : 0;
+#pragma unroll
+      for (int k2 = 0; k2 < NP; k2++) {
+        const long long row = zoff + (long long)g.nx * wrp(iy + O::LP + k2, g.ny);
+#pragma unroll
+        for (int k1 = 0; k1 < NP; k1++) {
+          const double w = ((q * w1[k1]) * w2[k2]) * w3[k3];
+          const long long v = __double2ll_rn(w * scale);
+          atomicAdd(&rho[row + wrp(ix + O::LP + k1, g.nx)], (unsigned long long)v);
         }
       }
     }
-    (void)fw;
-    const double q = d.q[s];
-    const int nz = g.dims == 3 ? NP : 1;
-    for (int k3 = 0; k3 < nz; k3++)
-      for (int k2 = 0; k2 < NP; k2++)
-        for (int k1 = 0; k1 < NP; k1++) {
-          const int gx = wrp(ix + O::LP + k1, g.nx), gy = wrp(iy + O::LP + k2, g.ny);
-          long long gi = gx + (long long)g.nx * gy;
-          double w = q * w1[k1] * w2[k2];
-          if (g.dims == 3) {
-            gi += (long long)g.nx * g.ny * wrp(iz + O::LP + k3, g.nz);
-            w *= w3[k3];
-          }
-          const long long v = __double2ll_rn(w * scale);
-          atomicAdd(&rho[gi], (unsigned long long)v);
-        }
   }
 }
 

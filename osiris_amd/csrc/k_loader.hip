@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void generate_particles(const SpecDesc* __rest
   const int nct = wx * wy * wz;
   const int n = d.cnt[tile];
   const int base = d.off[tile];
-  for (int p = threadIdx.x; p < n; p += 256) {
+  for (int p = blockIdx.y * 256 + threadIdx.x; p < n; p += 256 * gridDim.y) {
     const int c = p % nct, k = p / nct;
     const int lx = c % wx, ly = (c / wx) % wy, lz = c / (wx * wy);
     const int ix = tx0 + lx, iy = ty0 + ly, iz = tz0 + lz;

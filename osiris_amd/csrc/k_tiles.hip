@@ -11,9 +11,10 @@
 //     fixed-point J with the periodic wrap (the guard-cell fold of the reference).
 //     Particles that stay in the tile are written back in place; leavers go to the
 //     wave's region of the mover buffer (ballot-ranked, so the order is deterministic).
-//   tile_count / tile_fill: move the leavers into their destination tiles, filling the
-//     holes they left; a deterministic stable compaction.  If a tile would overflow its
-//     capacity the step is flagged (Ctrl::halt) and the host re-lays the particles out.
+//   tile_fill: moves the leavers into their destination tiles, filling the holes they
+//     left; a deterministic stable compaction.  A tile that would overflow its capacity
+//     is left unchanged and flags the step (Ctrl::halt); the host re-lays the particles
+//     out before the next push.
 //
 // Reference: spec/os-spec-push.f03:1438-1605 (advance_deposit_2d/3d),
 // emf/os-emf-interpolate.f03:554-779 (GET_EMF_2D/3D), spec/os-spec-current.f03:659-905,
@@ -546,48 +547,6 @@ OSR_DEV int neighbours(const GridDesc& g, int tile, int* nb) {
   return k;
 }
 
-// Count incoming movers and check the capacity of every tile.
-template <int NT>
-__global__ __launch_bounds__(NT) void tile_count(const SpecDesc* __restrict__ descs,
-                                                 const int2* __restrict__ work) {
-  __shared__ int s_nb[26];
-  __shared__ int s_nnb;
-  __shared__ int s_red[NT / 64];
-  const int2 wk = work[blockIdx.x];
-  const SpecDesc& d = descs[wk.x];
-  const GridDesc& g = d.g;
-  if (g.ctrl->halt != 0) return;
-  const int tile = wk.y;
-  const int NW = d.waves;
-  if (threadIdx.x == 0) s_nnb = neighbours(g, tile, s_nb);
-  __syncthreads();
-  int mine = 0;
-  for (int k = 0; k < s_nnb; k++) {
-    const int nt = s_nb[k];
-    const int stride = d.mv_stride[nt];
-    for (int w = 0; w < NW; w++) {
-      const int c = d.mv_cnt[nt * NW + w];
-      const int r0 = d.off[nt] + w * stride;
-      for (int i = threadIdx.x; i < c; i += NT) mine += tile_of(g, d.mcell[r0 + i]) == tile;
-    }
-  }
-  for (int off = 32; off > 0; off >>= 1) mine += __shfl_down(mine, off, 64);
-  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = mine;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int inc = 0;
-    for (int w = 0; w < NT / 64; w++) inc += s_red[w];
-    int holes = 0;
-    for (int w = 0; w < NW; w++) holes += d.mv_cnt[tile * NW + w];
-    const int n = d.cnt[tile];
-    const int nn = n - holes + inc;
-    d.inc[tile] = inc;
-    d.newcnt[tile] = nn;
-    // keep one wave-region rounding of slack so the next push's regions fit
-    if (nn > d.cap[tile] - 64 * NW || holes - inc > kTailBitmapBits) atomicOr(&g.ctrl->halt, 1);
-  }
-}
-
 OSR_DEV void copy_from_mover(const SpecDesc& d, int src, int dst, int dims) {
   d.cell[dst] = d.mcell[src];
   d.x[dst] = d.mx[src];
@@ -612,29 +571,36 @@ OSR_DEV void copy_particle(const SpecDesc& d, int src, int dst, int dims) {
   d.pid[dst] = d.pid[src];
 }
 
-// Move incoming movers and tail particles into the holes (deterministic order:
+constexpr int kIncMax = 4096;  // incoming movers one fill can place
+
+// Move the movers destined to this tile into it and close the holes (deterministic:
 // sources = incoming movers by (neighbour, wave, slot), then the non-hole particles of
 // the tail [newcnt, cnt) ascending; targets = holes below newcnt ascending, then the
-// appended slots [cnt, newcnt)).
+// appended slots [cnt, newcnt)).  A tile that cannot take its incoming particles (its
+// capacity, or the LDS lists of this kernel, would overflow) changes nothing, marks
+// itself with inc = -1 and raises Ctrl::halt; the host then re-lays the species out from
+// the tiles that did fill plus the mover buffers.
 template <int NT>
 __global__ __launch_bounds__(NT) void tile_fill(const SpecDesc* __restrict__ descs,
                                                 const int2* __restrict__ work) {
+  constexpr int MAXL = 26 * 4;  // candidate lists: neighbours x waves
   __shared__ int s_nb[26];
   __shared__ int s_nnb;
+  __shared__ int s_lofs[MAXL + 1];
+  __shared__ int s_lbase[MAXL];
   __shared__ int s_wtot[NT / 64];
-  __shared__ int s_hpre[17];  // prefix of this tile's per-wave hole counts
+  __shared__ int s_hpre[17];
+  __shared__ int s_inc[kIncMax];
   __shared__ unsigned int s_bits[kTailBitmapBits / 32];
   const int2 wk = work[blockIdx.x];
   const SpecDesc& d = descs[wk.x];
   const GridDesc& g = d.g;
-  if (g.ctrl->halt != 0) return;
+  if (g.ctrl->halt == 2) return;
   const int tile = wk.y;
   const int NW = d.waves;
   const int dims = g.dims;
   const int base = d.off[tile];
   const int n = d.cnt[tile];
-  const int nn = d.newcnt[tile];
-  const int I = d.inc[tile];
   const int stride = d.mv_stride[tile];
   if (threadIdx.x == 0) {
     s_nnb = neighbours(g, tile, s_nb);
@@ -642,17 +608,53 @@ __global__ __launch_bounds__(NT) void tile_fill(const SpecDesc* __restrict__ des
     for (int w = 0; w < NW; w++) s_hpre[w + 1] = s_hpre[w] + d.mv_cnt[tile * NW + w];
   }
   __syncthreads();
+  const int L = s_nnb * NW;
+  if (threadIdx.x < L) {
+    const int nt = s_nb[threadIdx.x / NW], w = threadIdx.x % NW;
+    s_lofs[threadIdx.x + 1] = d.mv_cnt[nt * NW + w];
+    s_lbase[threadIdx.x] = d.off[nt] + w * d.mv_stride[nt];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    s_lofs[0] = 0;
+    for (int l = 0; l < L; l++) s_lofs[l + 1] += s_lofs[l];
+  }
+  __syncthreads();
+  const int T = s_lofs[L];
   const int H = s_hpre[NW];
-  // hole k (ascending) -> slot relative to base
+  // pass 1: collect the incoming movers
+  int I = 0;
+  for (int c0 = 0; c0 < T; c0 += NT) {
+    const int i = c0 + threadIdx.x;
+    int src = -1;
+    bool f = false;
+    if (i < T) {
+      int l = 0;
+      while (i >= s_lofs[l + 1]) l++;
+      src = s_lbase[l] + (i - s_lofs[l]);
+      f = tile_of(g, d.mcell[src]) == tile;
+    }
+    int tot;
+    const int r = block_rank<NT>(f, s_wtot, tot);
+    if (f && I + r < kIncMax) s_inc[I + r] = src;
+    I += tot;
+  }
+  const int nn = n - H + I;
+  if (I > kIncMax || nn > d.cap[tile] - 64 * NW || H - I > kTailBitmapBits) {
+    if (threadIdx.x == 0) {
+      d.inc[tile] = -1;
+      atomicOr(&g.ctrl->halt, 1);
+    }
+    return;
+  }
   auto hole = [&](int k) -> int {
     int w = 0;
     while (k >= s_hpre[w + 1]) w++;
     return d.midx[base + w * stride + (k - s_hpre[w])];
   };
-  // number of holes below nn
+  // number of holes below nn (holes are ascending)
   int hlo = H;
   if (I < H) {
-    // holes are ascending: count those < nn by scanning (H is small)
     int c = 0;
     for (int k = threadIdx.x; k < H; k += NT) c += hole(k) < nn;
     for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
@@ -663,26 +665,9 @@ __global__ __launch_bounds__(NT) void tile_fill(const SpecDesc* __restrict__ des
     __syncthreads();
   }
   auto target = [&](int k) -> int { return k < hlo ? hole(k) : n + (k - H); };
-
-  // pass A: incoming movers
-  int rank0 = 0;
-  for (int k = 0; k < s_nnb; k++) {
-    const int nt = s_nb[k];
-    const int nstride = d.mv_stride[nt];
-    for (int w = 0; w < NW; w++) {
-      const int c = d.mv_cnt[nt * NW + w];
-      const int r0 = d.off[nt] + w * nstride;
-      for (int i0 = 0; i0 < c; i0 += NT) {
-        const int i = i0 + threadIdx.x;
-        const bool f = i < c && tile_of(g, d.mcell[r0 + i]) == tile;
-        int tot;
-        const int r = block_rank<NT>(f, s_wtot, tot);
-        if (f) copy_from_mover(d, r0 + i, base + target(rank0 + r), dims);
-        rank0 += tot;
-      }
-    }
-  }
-  // pass B: tail particles into the remaining holes
+  // pass 2: incoming movers into the holes / appended slots
+  for (int r = threadIdx.x; r < I; r += NT) copy_from_mover(d, s_inc[r], base + target(r), dims);
+  // pass 3: tail particles into the remaining holes
   if (I < H) {
     const int tail = n - nn;  // = H - I
     for (int i = threadIdx.x; i < (tail + 31) / 32; i += NT) s_bits[i] = 0u;
@@ -702,8 +687,10 @@ __global__ __launch_bounds__(NT) void tile_fill(const SpecDesc* __restrict__ des
       r1 += tot;
     }
   }
-  __syncthreads();
-  if (threadIdx.x == 0) d.cnt[tile] = nn;
+  if (threadIdx.x == 0) {
+    d.cnt[tile] = nn;
+    d.inc[tile] = I;
+  }
 }
 
 // ------------------------------------------------------------------------------------
@@ -718,7 +705,6 @@ OSR_INST(3, 0)
 OSR_INST(1, 1)
 OSR_INST(2, 1)
 OSR_INST(3, 1)
-template __global__ void tile_count<256>(const SpecDesc*, const int2*);
 template __global__ void tile_fill<256>(const SpecDesc*, const int2*);
 
 }  // namespace osr

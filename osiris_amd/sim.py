@@ -67,8 +67,10 @@ def yee_advance(dims, nx, gcl, dx, dt, e, b, jay):
 class Simulation:
     """A resident simulation on the GPU (one per grid; species share the grid)."""
 
-    def __init__(self, dims, nx, dx, dt, order, tile=None):
+    def __init__(self, dims, nx, dx, dt, order, tile=None, plain=False, kernel=0):
         d = SimDesc()
+        d.path = 1 if plain else 0
+        d.kernel = kernel
         d.dims = dims
         for i in range(3):
             d.nx[i] = nx[i] if i < dims else 1
@@ -175,13 +177,22 @@ class Simulation:
         return fld, par[:, :self.nspecies]
 
     def gauss(self, want_field=False):
+        """max |div E - rho|, and with want_field the residual field."""
         n = int(np.prod(self.nx[:self.dims]))
         res = np.zeros(n) if want_field else None
         m = np.zeros(1)
-        check(lib().osr_sim_gauss(self._h, _p(res), _p(m)))
+        check(lib().osr_sim_gauss(self._h, _p(res), None, _p(m)))
         if want_field:
             return float(m[0]), res.reshape(tuple(reversed(self.nx[:self.dims])))
         return float(m[0])
+
+    def charge_density(self):
+        """rho on the interior cell corners (FP64), shape (nz, ny, nx) / (ny, nx)."""
+        n = int(np.prod(self.nx[:self.dims]))
+        rho = np.zeros(n)
+        m = np.zeros(1)
+        check(lib().osr_sim_gauss(self._h, None, _p(rho), _p(m)))
+        return rho.reshape(tuple(reversed(self.nx[:self.dims])))
 
     @property
     def relayouts(self):
@@ -192,8 +203,8 @@ class Simulation:
     def time_kernels(self, nsteps):
         ms = np.zeros(6)
         check(lib().osr_sim_time_kernels(self._h, int(nsteps), _p(ms)))
-        return {"push_deposit": ms[0], "tile_count": ms[1], "tile_fill": ms[2], "yee": ms[3],
-                "finalize": ms[4], "step": ms[5]}
+        return {"push_deposit": ms[0], "tile_fill": ms[1], "yee": ms[2], "finalize": ms[3],
+                "step": ms[4]}
 
     @property
     def footprint(self):

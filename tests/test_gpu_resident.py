@@ -188,3 +188,67 @@ def test_generate_statistics():
     assert np.all(counts == 16)
     assert abs(u[:, 0].mean() - 0.2) < 1e-3 and abs(u[:, 0].std() - 0.05) < 1e-3
     assert abs(u[:, 2].mean()) < 1e-3 and abs(u[:, 1].std() - 0.05) < 1e-3
+
+
+def test_plain_path_matches_fused_first_step_and_oracle():
+    """The one-kernel-per-phase path: same particle arithmetic (bit-identical first step),
+    current via global FP32 atomics (order not reproducible), energies vs the oracle."""
+    rng = np.random.default_rng(48)
+    nx, dx, dt = (48, 32), (0.1, 0.1), 0.07
+    ix, x, u, q = thermal_plasma(rng, 2, nx, 8, 0.1)
+    res = []
+    for plain in (False, True):
+        with oa.Simulation(2, nx, dx, dt, 2, plain=plain) as sim:
+            sid = sim.add_species(-1.0)
+            sim.set_particles(sid, ix, x, u, q)
+            sim.run(1)
+            res.append(sim.get_particles(sid))
+    for a, b in zip(res[0], res[1]):
+        assert np.array_equal(a, b)
+    with oa.Simulation(2, nx, dx, dt, 2, plain=True) as sim:
+        sid = sim.add_species(-1.0)
+        sim.set_particles(sid, ix, x, u, q)
+        sim.run(20)
+        fld, par = sim.history()
+    ref = O.Sim(2, nx, dx, dt, 2)
+    ref.add_species(-1.0, 0, ix, x, u, q)
+    ref.step(20)
+    rpar = np.array(ref.par_hist)
+    assert np.max(np.abs(par - rpar) / np.abs(rpar)) <= 1e-4
+
+
+def test_determinism_large():
+    """Bit-reproducibility of the fused kernel (per-wave FP32 LDS tiles + fixed-order sum
+    + integer global current) on a larger two-stream run with many tile crossings."""
+    rng = np.random.default_rng(49)
+    nx, dx, dt = (256, 256), (0.05, 0.05), 0.035
+    out = []
+    for _ in range(2):
+        with oa.Simulation(2, nx, dx, dt, 2, tile=(16, 16)) as sim:
+            for k, ufl in enumerate((0.2, -0.2)):
+                sid = sim.add_species(-1.0)
+                sim.generate(sid, (4, 4), density=0.5, ufl=(ufl, 0, 0), uth=(0.05,) * 3,
+                             seed=77 + k)
+            sim.run(40)
+            out.append((sim.get_fields(), sim.history(), sim.get_particles(0)))
+    (f0, h0, p0), (f1, h1, p1) = out
+    for a, b in zip(f0 + h0 + p0, f1 + h1 + p1):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("dims,order", [(2, 1), (2, 2), (2, 3), (3, 2), (3, 3)])
+def test_charge_density_vs_oracle(dims, order):
+    """The FP64 rho of the Gauss-law diagnostic equals DEPOSIT_RHO of the oracle."""
+    rng = np.random.default_rng(50 + 3 * dims + order)
+    nx = (32, 32) if dims == 2 else (16, 16, 8)
+    n = 4000
+    ix, x, u, q = random_particles(rng, dims, nx, n, uth=0.1, q=-0.02)
+    g = O.make_grid(dims, nx, order)
+    rho = O.deposit_rho(g, order, ix, x.astype(np.float64), q.astype(np.float64))
+    O.update_periodic(g, rho, 1)
+    ro = g.interior(rho)[..., 0]
+    with oa.Simulation(dims, nx, (0.1,) * dims, 0.04, order) as sim:
+        sid = sim.add_species(-1.0)
+        sim.set_particles(sid, ix, x, u, q)
+        rg = sim.charge_density()
+    assert np.abs(rg - ro).max() <= 1e-12 * np.abs(ro).max()
