@@ -115,6 +115,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles",
                                                       "traffic_push_deposit_2d_c2.json"))
+    ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
+                    help="only run the plain copy kernel over GIB GiB (traffic-counter "
+                         "calibration and bandwidth probe) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -128,6 +131,13 @@ def main():
     if oa.device_count() < 1:
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(1)
+
+    if args.copy_probe > 0:
+        nbytes = int(args.copy_probe * 2**30)
+        ms = oa.copy_probe(nbytes, 5)
+        print(json.dumps({"copy_probe_bytes": nbytes, "ms": ms,
+                          "GB/s (read+write)": 2 * nbytes / ms / 1e6}), flush=True)
+        return
 
     nx = tuple(args.nx)
     dx, dt = (0.05, 0.05), 0.035
@@ -171,11 +181,15 @@ def main():
     bytes_launch = (n_total / nsp) * BYTES_PER_PARTICLE + cells * BYTES_PER_CELL_PUSH
     achieved = bytes_launch / t_launch / 1e9
     log("kernel ms/step: " + ", ".join(f"{k} {v:.4f}" for k, v in kt.items()))
+    kname = f"push_deposit_2d_t<{args.order},0,{tile[0]},{tile[1]}>"
     traffic = None
     if os.path.exists(args.traffic):
         try:
             with open(args.traffic) as f:
-                traffic = json.load(f).get("hbm_bytes_per_launch")
+                tj = json.load(f)
+            # counters measured on the same kernel instantiation only
+            if tuple(tj.get("tile", ())) == tuple(tile):
+                traffic = tj.get("hbm_bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
     fld, par = sim.history()
@@ -230,7 +244,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "push_deposit_2d<2,Boris> (one launch per species)",
+            "kernel": kname + " (Boris; one launch per species)",
             "kernel_ms": t_launch * 1e3,
             "algorithmic_bytes": bytes_launch,
         },

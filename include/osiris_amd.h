@@ -127,9 +127,10 @@ typedef struct {
   int order;      /* interpolation / deposit order 1..3 */
   int tile[3];    /* tile size in cells; 0 = default (2D 16x16, 3D 8x8x4) */
   int path;       /* OSR_PATH_FUSED (default) or OSR_PATH_PLAIN (2D only) */
-  int kernel;     /* fused push kernel: 0 auto, 1 generic (runtime tile shape, int64 LDS
-                     current), 2 2D compile-time tile with VP queue and per-wave FP32 LDS
-                     current (tiles 8x8, 16x16, 32x16) */
+  int kernel;     /* fused push kernel: 0 auto, 1 generic (runtime tile shape), 2 2D
+                     compile-time tile shape with a virtual-particle queue per wave
+                     (tiles 8x8, 16x16, 32x16); both accumulate the current in int64
+                     fixed point in LDS */
   int reserved[2];
 } osr_sim_desc;
 
@@ -194,6 +195,78 @@ int osr_sim_relayouts(osr_sim *sim, int64_t *count);
 int osr_sim_time_kernels(osr_sim *sim, int nsteps, double *ms);
 /* Bytes of device memory held by a simulation. */
 int osr_sim_footprint(osr_sim *sim, int64_t *bytes);
+
+/* ---------------------------------------------------------------------------------
+ * 4. Ensemble serving (no counterpart in the reference, which runs one simulation per
+ * process).  Independent jobs -- each one simulation as osr_sim_create would run it, set
+ * up by osr_species_generate -- are submitted at any time; jobs are admitted in
+ * submission order while their device footprint fits the budget, and all admitted jobs
+ * that share dims, order and tile shape are advanced together: one launch per kernel
+ * and time step covers the tiles of every job (descriptor arrays + work lists), and
+ * each step is a captured hipGraph.  A job parks itself on the device when it reaches
+ * its step count; finished jobs free their memory for waiting ones.  A job's results
+ * are bit-identical to running it alone.
+ * ------------------------------------------------------------------------------- */
+typedef struct osr_ensemble osr_ensemble;
+
+typedef struct {
+  int64_t hbm_budget;  /* device bytes the admitted jobs may hold (0: 80% of free memory) */
+  int max_active;      /* max admitted jobs (0: 256) */
+  int chunk_steps;     /* graph replays between scheduling points (0: 16) */
+  int reserved[5];
+} osr_ens_desc;
+
+typedef struct {
+  float rqm;          /* charge/mass ratio */
+  int pusher;         /* OSR_BORIS / OSR_VAY */
+  int ppc[3];         /* particles per cell per dimension */
+  int random_pos;     /* 1: uniform random in the cell, 0: lattice */
+  double density;
+  double ufl[3];      /* fluid proper velocity */
+  double uth[3];      /* thermal spread */
+  uint64_t seed;
+} osr_job_species;
+
+typedef struct {
+  osr_sim_desc sim;   /* grid, order, tile (path must be OSR_PATH_FUSED) */
+  int64_t nsteps;     /* time steps to run */
+  double e0[3], b0[3];/* uniform initial fields */
+  int nspecies;       /* 1..8 */
+  int reserved;
+  osr_job_species species[8];
+} osr_job_desc;
+
+enum { OSR_JOB_PENDING = 0, OSR_JOB_ACTIVE = 1, OSR_JOB_DONE = 2, OSR_JOB_FAILED = 3 };
+
+typedef struct {
+  int state;          /* OSR_JOB_* */
+  int nspecies;
+  int64_t steps;      /* completed steps */
+  int64_t particles;  /* total particles (0 before admission) */
+  int64_t footprint;  /* device bytes held while active */
+  double t_submit, t_admit, t_done;  /* seconds since osr_ens_create */
+} osr_job_info;
+
+int osr_ens_create(const osr_ens_desc *desc, osr_ensemble **out);
+/* Frees every job, finished or not. */
+int osr_ens_destroy(osr_ensemble *ens);
+/* Queues a job; *job_id identifies it from now on.  Fails if the job alone can never
+ * fit the budget. */
+int osr_ens_submit(osr_ensemble *ens, const osr_job_desc *job, int64_t *job_id);
+/* One scheduling pass, never blocks on the GPU: collects finished step chunks (re-laying
+ * out overflowed jobs, retiring finished ones), admits waiting jobs, launches the next
+ * chunk of every idle batch.  *active / *pending (may be NULL) receive the job counts. */
+int osr_ens_progress(osr_ensemble *ens, int *active, int *pending);
+/* Runs the scheduler until every submitted job has finished. */
+int osr_ens_wait(osr_ensemble *ens);
+int osr_ens_job_info(osr_ensemble *ens, int64_t job_id, osr_job_info *info);
+/* Energy history of a finished job, as osr_sim_history. */
+int osr_ens_job_history(osr_ensemble *ens, int64_t job_id, int64_t first, int64_t count,
+                        double *fld, double *par);
+/* Forgets a finished job's results. */
+int osr_ens_job_release(osr_ensemble *ens, int64_t job_id);
+/* Device bytes currently held by admitted jobs. */
+int osr_ens_usage(osr_ensemble *ens, int64_t *bytes);
 
 /* ---------------------------------------------------------------------------------
  * Measurement helper (no counterpart in the reference): streams `bytes` bytes through a

@@ -53,6 +53,15 @@ SIGNATURES = {
     "osr_sim_relayouts": (I, [P, P]),
     "osr_sim_time_kernels": (I, [P, I, P]),
     "osr_sim_footprint": (I, [P, P]),
+    "osr_ens_create": (I, [P, P]),
+    "osr_ens_destroy": (I, [P]),
+    "osr_ens_submit": (I, [P, P, P]),
+    "osr_ens_progress": (I, [P, P, P]),
+    "osr_ens_wait": (I, [P]),
+    "osr_ens_job_info": (I, [P, I64, P]),
+    "osr_ens_job_history": (I, [P, I64, I64, I64, P, P]),
+    "osr_ens_job_release": (I, [P, I64]),
+    "osr_ens_usage": (I, [P, P]),
     "osr_copy_probe": (I, [I64, I, P]),
 }
 
@@ -61,6 +70,32 @@ class SimDesc(C.Structure):
     _fields_ = [("dims", C.c_int), ("nx", C.c_int * 3), ("dx", C.c_double * 3),
                 ("dt", C.c_double), ("order", C.c_int), ("tile", C.c_int * 3),
                 ("path", C.c_int), ("kernel", C.c_int), ("reserved", C.c_int * 2)]
+
+
+class EnsDesc(C.Structure):
+    _fields_ = [("hbm_budget", C.c_int64), ("max_active", C.c_int), ("chunk_steps", C.c_int),
+                ("reserved", C.c_int * 5)]
+
+
+class JobSpecies(C.Structure):
+    _fields_ = [("rqm", C.c_float), ("pusher", C.c_int), ("ppc", C.c_int * 3),
+                ("random_pos", C.c_int), ("density", C.c_double), ("ufl", C.c_double * 3),
+                ("uth", C.c_double * 3), ("seed", C.c_uint64)]
+
+
+class JobDesc(C.Structure):
+    _fields_ = [("sim", SimDesc), ("nsteps", C.c_int64), ("e0", C.c_double * 3),
+                ("b0", C.c_double * 3), ("nspecies", C.c_int), ("reserved", C.c_int),
+                ("species", JobSpecies * 8)]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [("state", C.c_int), ("nspecies", C.c_int), ("steps", C.c_int64),
+                ("particles", C.c_int64), ("footprint", C.c_int64), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
+OSR_JOB_PENDING, OSR_JOB_ACTIVE, OSR_JOB_DONE, OSR_JOB_FAILED = 0, 1, 2, 3
 
 
 class OsrError(RuntimeError):

@@ -58,25 +58,50 @@ namespace {
 constexpr int kNT = 256;     // threads per workgroup of the tile kernels
 constexpr int kWaves = kNT / 64;
 
+// Device allocations made while g_alloc_stream is set (ensemble jobs) come from the
+// stream-ordered pool: retiring a job then frees its memory without the implicit device
+// synchronisation of hipFree, so the other jobs keep the GPU busy.
+thread_local hipStream_t g_alloc_stream = nullptr;
+thread_local bool g_alloc_async = false;
+
 struct DBuf {
   void* p = nullptr;
   size_t n = 0;
+  bool async = false;
+  hipStream_t st = nullptr;
   DBuf() = default;
   DBuf(const DBuf&) = delete;
   DBuf& operator=(const DBuf&) = delete;
   ~DBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (async)
+        (void)hipFreeAsync(p, st);
+      else
+        (void)hipFree(p);
+    }
     p = nullptr;
     n = 0;
+    async = false;
   }
   int alloc(size_t bytes) {
     release();
     if (bytes == 0) bytes = 256;
-    if (hipMalloc(&p, bytes) != hipSuccess) {
+    hipError_t e;
+    if (g_alloc_async) {
+      e = hipMallocAsync(&p, bytes, g_alloc_stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(g_alloc_stream);
+      async = true;
+      st = g_alloc_stream;
+    } else {
+      e = hipMalloc(&p, bytes);
+    }
+    if (e != hipSuccess) {
+      if (p && async) (void)hipFreeAsync(p, st);
       p = nullptr;
+      async = false;
       (void)hipGetLastError();
-      return fail(OSR_ERR_NOMEM, "hipMalloc(%zu bytes) failed", bytes);
+      return fail(OSR_ERR_NOMEM, "device allocation of %zu bytes failed", bytes);
     }
     n = bytes;
     return OSR_OK;
@@ -136,6 +161,7 @@ struct osr_sim {
   bool plain = false;
   size_t lds_push = 0;
   Ctrl* h_ctrl = nullptr;  // pinned mirror
+  int64_t desc_version = 0;  // bumped whenever the device descriptors change
 };
 
 namespace {
@@ -175,11 +201,12 @@ void update_jscale(osr_sim* s) {
   s->g.jinv = std::ldexp(1.0, -sh);
 }
 
-int upload_descs(osr_sim* s) {
-  std::vector<SpecDesc> v(kMaxSpecies);
+// device descriptors of the species of s (v[0 .. nspecies)) and of its fields
+void make_spec_descs(const osr_sim* s, SpecDesc* v) {
   for (size_t i = 0; i < s->sp.size(); i++) {
-    Species& p = *s->sp[i];
+    const Species& p = *s->sp[i];
     SpecDesc& d = v[i];
+    d = SpecDesc{};
     d.g = s->g;
     d.cell = p.cell.as<uint32_t>();
     d.x = p.x.as<float>();
@@ -211,8 +238,10 @@ int upload_descs(osr_sim* s) {
     d.tem = (float)(0.5 * s->dt / (double)p.rqm);
     d.waves = kWaves;
   }
-  HIPC(hipMemcpy(s->d_spec.p, v.data(), sizeof(SpecDesc) * kMaxSpecies, hipMemcpyHostToDevice));
-  FieldDesc& f = s->fd;
+}
+
+void make_field_desc(const osr_sim* s, FieldDesc& f) {
+  f = FieldDesc{};
   f.g = s->g;
   f.fe_part = s->fe_part.as<double>();
   f.nblk = s->nblk;
@@ -234,11 +263,19 @@ int upload_descs(osr_sim* s) {
   f.dtif = (float)s->dt;
   f.Jf = s->plain ? s->J.as<float>() : nullptr;
   f.jf_zero = s->plain ? 1 : 0;
-  HIPC(hipMemcpy(s->d_fd.p, &f, sizeof(FieldDesc), hipMemcpyHostToDevice));
+}
+
+int upload_descs(osr_sim* s) {
+  std::vector<SpecDesc> v(kMaxSpecies);
+  make_spec_descs(s, v.data());
+  HIPC(hipMemcpy(s->d_spec.p, v.data(), sizeof(SpecDesc) * kMaxSpecies, hipMemcpyHostToDevice));
+  make_field_desc(s, s->fd);
+  HIPC(hipMemcpy(s->d_fd.p, &s->fd, sizeof(FieldDesc), hipMemcpyHostToDevice));
+  s->desc_version++;
   return OSR_OK;
 }
 
-// Tile index and local cell of a packed cell.
+// Cell coordinates (0-based) of a packed cell.
 inline void cell_coords(const osr_sim* s, uint32_t c, int* ic) {
   if (s->dims == 2) {
     unpack2(c, ic[0], ic[1]);
@@ -300,12 +337,11 @@ int alloc_species(osr_sim* s, Species& p, const std::vector<int>& cnt) {
   HIPC(hipMemcpy(p.off.p, off.data(), 4 * nt, hipMemcpyHostToDevice));
   HIPC(hipMemcpy(p.cnt.p, cnt.data(), 4 * nt, hipMemcpyHostToDevice));
   HIPC(hipMemcpy(p.cap.p, cap.data(), 4 * nt, hipMemcpyHostToDevice));
-  HIPC(hipMemset(p.mv_cnt.p, 0, 4 * (size_t)nt * kWaves));
-  HIPC(hipMemset(p.mv_stride.p, 0, 4 * (size_t)nt));
-  HIPC(hipMemset(p.epart.p, 0, 8 * (size_t)p.epart_n));
-  HIPC(hipMemset(p.inc.p, 0, 4 * (size_t)nt));
-  // the memsets above run on the null stream; the simulation stream must see them
-  HIPC(hipDeviceSynchronize());
+  HIPC(hipMemsetAsync(p.mv_cnt.p, 0, 4 * (size_t)nt * kWaves, s->stream));
+  HIPC(hipMemsetAsync(p.mv_stride.p, 0, 4 * (size_t)nt, s->stream));
+  HIPC(hipMemsetAsync(p.epart.p, 0, 8 * (size_t)p.epart_n, s->stream));
+  HIPC(hipMemsetAsync(p.inc.p, 0, 4 * (size_t)nt, s->stream));
+  HIPC(hipStreamSynchronize(s->stream));
   p.h_off = off;
   p.h_cap = cap;
   int64_t np = 0;
@@ -680,6 +716,7 @@ int ensure_hist(osr_sim* s, int64_t rows) {
 }
 
 int read_ctrl(osr_sim* s, Ctrl& c) {
+  if (!s->h_ctrl) HIPC(hipHostMalloc((void**)&s->h_ctrl, sizeof(Ctrl), hipHostMallocDefault));
   HIPC(hipMemcpyAsync(s->h_ctrl, s->ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, s->stream));
   HIPC(hipStreamSynchronize(s->stream));
   c = *s->h_ctrl;
@@ -775,19 +812,19 @@ int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
   TRY(s->B.alloc(3 * nc * 4));
   TRY(s->J.alloc(3 * nc * 8));
   TRY(s->ctrl.alloc(sizeof(Ctrl)));
-  HIPC(hipMemset(s->E.p, 0, 3 * nc * 4));
-  HIPC(hipMemset(s->B.p, 0, 3 * nc * 4));
-  HIPC(hipMemset(s->J.p, 0, 3 * nc * 8));
-  HIPC(hipMemset(s->ctrl.p, 0, sizeof(Ctrl)));
-  HIPC(hipHostMalloc((void**)&s->h_ctrl, sizeof(Ctrl), hipHostMallocDefault));
-  HIPC(hipDeviceSynchronize());
+  HIPC(hipMemsetAsync(s->E.p, 0, 3 * nc * 4, s->stream));
+  HIPC(hipMemsetAsync(s->B.p, 0, 3 * nc * 4, s->stream));
+  HIPC(hipMemsetAsync(s->J.p, 0, 3 * nc * 8, s->stream));
+  HIPC(hipMemsetAsync(s->ctrl.p, 0, sizeof(Ctrl), s->stream));
+  HIPC(hipStreamSynchronize(s->stream));
   g.E = s->E.as<float>();
   g.B = s->B.as<float>();
   g.J = s->J.as<unsigned long long>();
   g.ctrl = s->ctrl.as<Ctrl>();
   s->nblk = (int)((nc + 255) / 256);
   TRY(s->fe_part.alloc(sizeof(double) * 6 * (size_t)s->nblk));
-  HIPC(hipMemset(s->fe_part.p, 0, sizeof(double) * 6 * (size_t)s->nblk));
+  HIPC(hipMemsetAsync(s->fe_part.p, 0, sizeof(double) * 6 * (size_t)s->nblk, s->stream));
+  HIPC(hipStreamSynchronize(s->stream));
   TRY(s->d_spec.alloc(sizeof(SpecDesc) * kMaxSpecies));
   TRY(s->d_fd.alloc(sizeof(FieldDesc)));
   TRY(s->d_ywork.alloc(sizeof(int2) * (size_t)s->nblk));
@@ -1113,8 +1150,7 @@ int osr_sim_gauss(osr_sim* s, double* res, double* rho_out, double* maxabs) {
   TRY(rho.alloc(8 * nc));
   TRY(dres.alloc(8 * nc));
   HIPC(hipStreamSynchronize(s->stream));
-  HIPC(hipMemset(rho.p, 0, 8 * nc));
-  HIPC(hipDeviceSynchronize());
+  HIPC(hipMemsetAsync(rho.p, 0, 8 * nc, s->stream));
   double qabs = 0;
   for (auto& p : s->sp) qabs += p->qabs;
   const int sh = 62 - (int)std::ceil(std::log2(std::max(qabs * 1.01, 1e-300)));
@@ -1483,3 +1519,5 @@ int osr_yee_advance(int dims, const int* nx, const int* ld, const int* gcl, cons
 }
 
 }  // extern "C"
+
+#include "ensemble.hip"

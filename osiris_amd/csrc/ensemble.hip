@@ -1,0 +1,559 @@
+// ensemble.hip -- continuous-batching server of independent simulations (one GPU).
+//
+// Included at the end of capi.hip (it uses the simulation internals defined there).
+//
+// Jobs are admitted in submission order while their device footprint fits the budget.
+// Admitted jobs that share dims, order, tile shape and push-kernel variant form a batch.
+// A batch advances all its jobs together: its descriptor arrays concatenate the jobs'
+// species and field descriptors, and its work lists enumerate (species, tile) and
+// (job, cell block) items over every job, so each kernel of the time step is ONE launch
+// for the whole batch.  One step is captured as a hipGraph and replayed chunk_steps times
+// between scheduling points.  The kernels are the ones of a single simulation; every job
+// keeps its own control block, so a job whose tile overflowed, or that reached its step
+// count, stops on the device while the others go on (halt protocol: 2 = wants a
+// re-layout, 3 = parked at its target).  Nothing in a job's arithmetic depends on the
+// other jobs, so its results are bit-identical alone and in any mix.
+//
+// Job memory comes from the stream-ordered pool (hipMallocAsync / hipFreeAsync), so
+// retiring a job frees its memory without synchronising the device.
+#include <chrono>
+#include <deque>
+#include <map>
+
+namespace {
+
+struct EnsJob {
+  int64_t id = 0;
+  osr_job_desc desc{};
+  int state = OSR_JOB_PENDING;
+  double t_submit = 0, t_admit = 0, t_done = 0;
+  osr_sim* sim = nullptr;
+  int64_t est = 0, bytes = 0, particles = 0, steps = 0;
+  std::vector<double> hist;  // nsteps rows of (6 + nspecies) after completion
+  int ncol = 0;
+};
+
+struct EnsBatch {
+  int dims = 2, order = 2, tile[3] = {1, 1, 1}, kernel = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev = nullptr;
+  std::vector<EnsJob*> jobs;     // jobs stepped by the graph
+  std::vector<EnsJob*> joining;  // admitted while a chunk was in flight
+  DBuf spec, fds, wpush[2], wfill, wyee, wfin;
+  int npush[2] = {0, 0}, nfill = 0, nyee = 0, nfin = 0;
+  hipGraphExec_t gexec = nullptr;
+  bool dirty = true, in_flight = false;
+  Ctrl* h_ctrl = nullptr;
+  int h_ctrl_cap = 0;
+  size_t lds = 0;
+};
+
+// RAII switch of DBuf allocations to the ensemble's stream-ordered pool
+struct PoolScope {
+  hipStream_t prev_s;
+  bool prev_a;
+  explicit PoolScope(hipStream_t s) : prev_s(g_alloc_stream), prev_a(g_alloc_async) {
+    g_alloc_stream = s;
+    g_alloc_async = true;
+  }
+  ~PoolScope() {
+    g_alloc_stream = prev_s;
+    g_alloc_async = prev_a;
+  }
+};
+
+int resolved_tile(const osr_sim_desc& d, int k) {
+  if (k >= d.dims) return 1;
+  int t = d.tile[k] > 0 ? d.tile[k] : (d.dims == 2 ? 16 : (k < 2 ? 8 : 4));
+  return std::min(t, d.nx[k]);
+}
+
+// Device bytes a job will hold once admitted (mirrors osr_sim_create, alloc_species and
+// ensure_hist), plus a margin for allocation granularity.
+int64_t job_footprint(const osr_job_desc& j) {
+  const osr_sim_desc& d = j.sim;
+  const int D = d.dims;
+  int64_t nc = 1;
+  int tl[3] = {1, 1, 1}, nt[3] = {1, 1, 1};
+  for (int k = 0; k < D; k++) {
+    nc *= d.nx[k];
+    tl[k] = resolved_tile(d, k);
+    nt[k] = ceil_div(d.nx[k], tl[k]);
+  }
+  const int64_t ntiles = (int64_t)nt[0] * nt[1] * nt[2];
+  const int64_t nblk = (nc + 255) / 256;
+  int64_t b = nc * (12 + 12 + 24) + nblk * (48 + 8);
+  const int64_t rows = std::max<int64_t>(j.nsteps, 1024);
+  b += rows * (6 + j.nspecies) * 8;
+  const int64_t slot = D == 2 ? 68 : 76;
+  for (int s = 0; s < j.nspecies; s++) {
+    int ppc = 1;
+    for (int k = 0; k < D; k++) ppc *= std::max(1, j.species[s].ppc[k]);
+    int64_t slots = 0;
+    for (int64_t t = 0; t < ntiles; t++) {
+      const int tx = (int)(t % nt[0]), ty = (int)((t / nt[0]) % nt[1]), tz = (int)(t / ((int64_t)nt[0] * nt[1]));
+      const int64_t wx = std::min(tl[0], d.nx[0] - tx * tl[0]);
+      const int64_t wy = std::min(tl[1], d.nx[1] - ty * tl[1]);
+      const int64_t wz = D == 3 ? std::min(tl[2], d.nx[2] - tz * tl[2]) : 1;
+      const int64_t c = wx * wy * wz * ppc;
+      int64_t cp = c + std::max<int64_t>(c / 4, 64) + kNT;
+      slots += (cp + 63) / 64 * 64;
+    }
+    b += slots * slot + ntiles * (48 + 8);
+  }
+  return b + (1 << 20);
+}
+
+}  // namespace
+
+struct osr_ensemble {
+  osr_ens_desc desc{};
+  int64_t budget = 0, used = 0, next_id = 1;
+  std::chrono::steady_clock::time_point t0;
+  hipStream_t alloc_stream = nullptr;
+  std::map<int64_t, std::unique_ptr<EnsJob>> jobs;
+  std::deque<EnsJob*> pending;
+  std::vector<std::unique_ptr<EnsBatch>> batches;
+  int nactive = 0;
+
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+namespace {
+
+void destroy_batch(EnsBatch* b) {
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  b->gexec = nullptr;
+  if (b->ev) (void)hipEventDestroy(b->ev);
+  b->ev = nullptr;
+  if (b->h_ctrl) (void)hipHostFree(b->h_ctrl);
+  b->h_ctrl = nullptr;
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  b->stream = nullptr;
+}
+
+int ens_fail_job(osr_ensemble* e, EnsJob* j) {
+  if (j->sim) {
+    PoolScope ps(e->alloc_stream);
+    (void)osr_sim_destroy(j->sim);
+    j->sim = nullptr;
+    e->used -= j->bytes;
+    e->nactive--;
+  }
+  j->state = OSR_JOB_FAILED;
+  j->t_done = e->now();
+  return OSR_OK;
+}
+
+// Creates the job's simulation (stream-ordered pool), sets a step target and places it
+// in the batch of its kind.
+int ens_admit(osr_ensemble* e, EnsJob* j) {
+  PoolScope ps(e->alloc_stream);
+  const osr_job_desc& jd = j->desc;
+  osr_sim* s = nullptr;
+  TRY(osr_sim_create(&jd.sim, &s));
+  std::unique_ptr<osr_sim, int (*)(osr_sim*)> guard(s, osr_sim_destroy);
+  for (int i = 0; i < jd.nspecies; i++) {
+    const osr_job_species& sp = jd.species[i];
+    int sid = 0;
+    TRY(osr_sim_add_species(s, sp.rqm, sp.pusher, &sid));
+    TRY(osr_species_generate(s, sid, sp.ppc, sp.density, sp.ufl, sp.uth, sp.seed, sp.random_pos));
+  }
+  if (jd.e0[0] != 0 || jd.e0[1] != 0 || jd.e0[2] != 0 || jd.b0[0] != 0 || jd.b0[1] != 0 ||
+      jd.b0[2] != 0)
+    TRY(osr_fields_fill(s, jd.e0, jd.b0));
+  TRY(ensure_hist(s, jd.nsteps));
+  Ctrl c{};
+  c.target = jd.nsteps;
+  HIPC(hipStreamSynchronize(s->stream));
+  HIPC(hipMemcpy(s->ctrl.p, &c, sizeof(Ctrl), hipMemcpyHostToDevice));
+  int64_t bytes = 0;
+  TRY(osr_sim_footprint(s, &bytes));
+  int64_t np = 0;
+  for (auto& p : s->sp) np += p->np;
+  j->sim = guard.release();
+  j->bytes = bytes;
+  j->particles = np;
+  j->state = OSR_JOB_ACTIVE;
+  j->t_admit = e->now();
+  j->ncol = 6 + jd.nspecies;
+  e->used += bytes;
+  e->nactive++;
+  // batch of the same kind, or a new one
+  EnsBatch* b = nullptr;
+  for (auto& bb : e->batches)
+    if (bb->dims == s->dims && bb->order == s->order && bb->tile[0] == s->g.tx &&
+        bb->tile[1] == s->g.ty && bb->tile[2] == s->g.tz && bb->kernel == jd.sim.kernel) {
+      b = bb.get();
+      break;
+    }
+  if (!b) {
+    std::unique_ptr<EnsBatch> nb(new EnsBatch());
+    nb->dims = s->dims;
+    nb->order = s->order;
+    nb->tile[0] = s->g.tx;
+    nb->tile[1] = s->g.ty;
+    nb->tile[2] = s->g.tz;
+    nb->kernel = jd.sim.kernel;
+    nb->lds = s->lds_push;
+    HIPC(hipStreamCreateWithFlags(&nb->stream, hipStreamNonBlocking));
+    HIPC(hipEventCreateWithFlags(&nb->ev, hipEventDisableTiming));
+    b = nb.get();
+    e->batches.push_back(std::move(nb));
+  }
+  if (b->in_flight)
+    b->joining.push_back(j);
+  else
+    b->jobs.push_back(j);
+  b->dirty = true;
+  return OSR_OK;
+}
+
+// Descriptor arrays, work lists and the captured step of a batch.
+int ens_build(osr_ensemble* e, EnsBatch* b) {
+  PoolScope ps(e->alloc_stream);
+  std::vector<SpecDesc> specs;
+  std::vector<FieldDesc> fds(b->jobs.size());
+  std::vector<int2> wp[2], wf, wy;
+  std::vector<int> wn;
+  for (size_t k = 0; k < b->jobs.size(); k++) {
+    osr_sim* s = b->jobs[k]->sim;
+    const int base = (int)specs.size();
+    specs.resize(base + s->sp.size());
+    make_spec_descs(s, specs.data() + base);
+    make_field_desc(s, fds[k]);
+    for (size_t i = 0; i < s->sp.size(); i++)
+      for (int t = 0; t < s->ntiles; t++) {
+        wp[s->sp[i]->pusher == OSR_VAY ? 1 : 0].push_back(make_int2(base + (int)i, t));
+        wf.push_back(make_int2(base + (int)i, t));
+      }
+    for (int blk = 0; blk < s->nblk; blk++) wy.push_back(make_int2((int)k, blk));
+    wn.push_back((int)k);
+  }
+  auto put = [&](DBuf& d, const void* src, size_t bytes) -> int {
+    if (bytes == 0) return OSR_OK;
+    if (d.n < bytes) TRY(d.alloc(bytes + bytes / 2));
+    HIPC(hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice));
+    return OSR_OK;
+  };
+  TRY(put(b->spec, specs.data(), sizeof(SpecDesc) * specs.size()));
+  TRY(put(b->fds, fds.data(), sizeof(FieldDesc) * fds.size()));
+  for (int p = 0; p < 2; p++) {
+    TRY(put(b->wpush[p], wp[p].data(), sizeof(int2) * wp[p].size()));
+    b->npush[p] = (int)wp[p].size();
+  }
+  TRY(put(b->wfill, wf.data(), sizeof(int2) * wf.size()));
+  TRY(put(b->wyee, wy.data(), sizeof(int2) * wy.size()));
+  TRY(put(b->wfin, wn.data(), sizeof(int) * wn.size()));
+  b->nfill = (int)wf.size();
+  b->nyee = (int)wy.size();
+  b->nfin = (int)wn.size();
+  if ((int)b->jobs.size() > b->h_ctrl_cap) {
+    if (b->h_ctrl) HIPC(hipHostFree(b->h_ctrl));
+    b->h_ctrl = nullptr;
+    b->h_ctrl_cap = std::max<int>((int)b->jobs.size(), 2 * b->h_ctrl_cap);
+    HIPC(hipHostMalloc((void**)&b->h_ctrl, sizeof(Ctrl) * b->h_ctrl_cap, hipHostMallocDefault));
+  }
+  if (b->gexec) {
+    (void)hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+  }
+  b->dirty = false;
+  if (b->jobs.empty()) return OSR_OK;
+  // one time step of every job of the batch
+  hipGraph_t graph = nullptr;
+  HIPC(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int r = OSR_OK;
+  for (int p = 0; p < 2 && r == OSR_OK; p++) {
+    if (!b->npush[p]) continue;
+    push_fn f = nullptr;
+    size_t lds = b->lds;
+    if (b->dims == 2 && b->kernel != 1) {
+      f = pick_push2d_t(b->order, p, b->tile[0], b->tile[1]);
+      if (f) lds = 0;
+    }
+    if (!f) f = pick_push(b->dims, b->order, p);
+    if (!f) {
+      r = fail(OSR_ERR_ARG, "no push kernel for dims=%d order=%d", b->dims, b->order);
+      break;
+    }
+    hipLaunchKernelGGL(f, dim3(b->npush[p]), dim3(kNT), lds, b->stream, b->spec.as<SpecDesc>(),
+                       b->wpush[p].as<int2>());
+  }
+  if (r == OSR_OK) {
+    hipLaunchKernelGGL(tile_fill<kNT>, dim3(b->nfill), dim3(kNT), 0, b->stream,
+                       b->spec.as<SpecDesc>(), b->wfill.as<int2>());
+    const FieldDesc* fd = b->fds.as<FieldDesc>();
+    const int2* w = b->wyee.as<int2>();
+    if (b->dims == 2) {
+      hipLaunchKernelGGL((yee_dbdt<2, true>), dim3(b->nyee), dim3(256), 0, b->stream, fd, w);
+      hipLaunchKernelGGL((yee_dedt<2>), dim3(b->nyee), dim3(256), 0, b->stream, fd, w);
+      hipLaunchKernelGGL((yee_dbdt<2, false>), dim3(b->nyee), dim3(256), 0, b->stream, fd, w);
+    } else {
+      hipLaunchKernelGGL((yee_dbdt<3, true>), dim3(b->nyee), dim3(256), 0, b->stream, fd, w);
+      hipLaunchKernelGGL((yee_dedt<3>), dim3(b->nyee), dim3(256), 0, b->stream, fd, w);
+      hipLaunchKernelGGL((yee_dbdt<3, false>), dim3(b->nyee), dim3(256), 0, b->stream, fd, w);
+    }
+    hipLaunchKernelGGL(finalize_step, dim3(b->nfin), dim3(256), 0, b->stream, fd,
+                       b->wfin.as<int>());
+  }
+  hipError_t ce = hipStreamEndCapture(b->stream, &graph);
+  if (r != OSR_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return r;
+  }
+  if (ce != hipSuccess) return fail(OSR_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ce));
+  ce = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ce != hipSuccess) return fail(OSR_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ce));
+  return OSR_OK;
+}
+
+int ens_launch(osr_ensemble* e, EnsBatch* b) {
+  if (b->dirty) TRY(ens_build(e, b));
+  if (b->jobs.empty()) return OSR_OK;
+  const int chunk = e->desc.chunk_steps;
+  for (int k = 0; k < chunk; k++) HIPC(hipGraphLaunch(b->gexec, b->stream));
+  for (size_t k = 0; k < b->jobs.size(); k++)
+    HIPC(hipMemcpyAsync(&b->h_ctrl[k], b->jobs[k]->sim->ctrl.p, sizeof(Ctrl),
+                        hipMemcpyDeviceToHost, b->stream));
+  HIPC(hipEventRecord(b->ev, b->stream));
+  b->in_flight = true;
+  return OSR_OK;
+}
+
+// Results of a finished chunk: re-layouts, retirements, joins.
+int ens_collect(osr_ensemble* e, EnsBatch* b) {
+  PoolScope ps(e->alloc_stream);
+  b->in_flight = false;
+  std::vector<EnsJob*> keep;
+  for (size_t k = 0; k < b->jobs.size(); k++) {
+    EnsJob* j = b->jobs[k];
+    const Ctrl c = b->h_ctrl[k];
+    osr_sim* s = j->sim;
+    j->steps = c.steps;
+    s->steps = c.steps;
+    if (c.error) {
+      ens_fail_job(e, j);
+      b->dirty = true;
+      continue;
+    }
+    bool done = c.halt == 3 || c.steps >= j->desc.nsteps;
+    if (c.halt == 2) {  // a tile overflowed: re-lay out this job only
+      const int64_t v = s->desc_version;
+      TRY(relayout(s, true));
+      Ctrl z = c;
+      z.halt = done ? 3 : 0;
+      HIPC(hipMemcpy(s->ctrl.p, &z, sizeof(Ctrl), hipMemcpyHostToDevice));
+      if (s->desc_version != v) b->dirty = true;
+    }
+    if (done) {
+      j->hist.resize((size_t)j->desc.nsteps * j->ncol);
+      HIPC(hipMemcpy(j->hist.data(), s->hist.p, sizeof(double) * j->hist.size(),
+                     hipMemcpyDeviceToHost));
+      (void)osr_sim_destroy(s);
+      j->sim = nullptr;
+      e->used -= j->bytes;
+      e->nactive--;
+      j->state = OSR_JOB_DONE;
+      j->t_done = e->now();
+      b->dirty = true;
+      continue;
+    }
+    keep.push_back(j);
+  }
+  b->jobs.swap(keep);
+  if (!b->joining.empty()) {
+    b->jobs.insert(b->jobs.end(), b->joining.begin(), b->joining.end());
+    b->joining.clear();
+    b->dirty = true;
+  }
+  return OSR_OK;
+}
+
+int ens_progress(osr_ensemble* e) {
+  // 1. finished chunks
+  for (auto& b : e->batches) {
+    if (!b->in_flight) continue;
+    hipError_t q = hipEventQuery(b->ev);
+    if (q == hipErrorNotReady) continue;
+    if (q != hipSuccess) return fail(OSR_ERR_HIP, "chunk failed: %s", hipGetErrorString(q));
+    TRY(ens_collect(e, b.get()));
+  }
+  // 2. admissions, in submission order
+  while (!e->pending.empty()) {
+    EnsJob* j = e->pending.front();
+    if (e->nactive >= e->desc.max_active || e->used + j->est > e->budget) break;
+    e->pending.pop_front();
+    if (ens_admit(e, j) != OSR_OK) ens_fail_job(e, j);  // the job fails, the server goes on
+  }
+  // 3. next chunk of every idle batch
+  for (auto& b : e->batches)
+    if (!b->in_flight && (!b->jobs.empty() || b->dirty)) TRY(ens_launch(e, b.get()));
+  return OSR_OK;
+}
+
+int check_ens(osr_ensemble* e) {
+  if (!e) return fail(OSR_ERR_ARG, "null ensemble handle");
+  return OSR_OK;
+}
+
+EnsJob* find_job(osr_ensemble* e, int64_t id) {
+  auto it = e->jobs.find(id);
+  return it == e->jobs.end() ? nullptr : it->second.get();
+}
+
+}  // namespace
+
+extern "C" {
+
+int osr_ens_create(const osr_ens_desc* desc, osr_ensemble** out) {
+  if (!out) return fail(OSR_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (osr_device_count() < 1) return fail(OSR_ERR_HIP, "no GPU visible");
+  std::unique_ptr<osr_ensemble> e(new osr_ensemble());
+  if (desc) e->desc = *desc;
+  if (e->desc.max_active <= 0) e->desc.max_active = 256;
+  if (e->desc.chunk_steps <= 0) e->desc.chunk_steps = 16;
+  e->budget = e->desc.hbm_budget;
+  if (e->budget <= 0) {
+    size_t fr = 0, tot = 0;
+    HIPC(hipMemGetInfo(&fr, &tot));
+    e->budget = (int64_t)(0.8 * (double)fr);
+  }
+  int dev = 0;
+  HIPC(hipGetDevice(&dev));
+  hipMemPool_t pool = nullptr;
+  HIPC(hipDeviceGetDefaultMemPool(&pool, dev));
+  uint64_t keep = UINT64_MAX;  // keep freed job memory in the pool for the next jobs
+  HIPC(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+  HIPC(hipStreamCreateWithFlags(&e->alloc_stream, hipStreamNonBlocking));
+  e->t0 = std::chrono::steady_clock::now();
+  *out = e.release();
+  return OSR_OK;
+}
+
+int osr_ens_destroy(osr_ensemble* e) {
+  if (!e) return OSR_OK;
+  for (auto& b : e->batches) destroy_batch(b.get());
+  {
+    PoolScope ps(e->alloc_stream);
+    for (auto& kv : e->jobs)
+      if (kv.second->sim) (void)osr_sim_destroy(kv.second->sim);
+    e->batches.clear();  // batch buffers go back to the pool
+  }
+  (void)hipStreamSynchronize(e->alloc_stream);
+  int dev = 0;
+  hipMemPool_t pool = nullptr;
+  if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess)
+    (void)hipMemPoolTrimTo(pool, 0);
+  (void)hipStreamDestroy(e->alloc_stream);
+  delete e;
+  return OSR_OK;
+}
+
+int osr_ens_submit(osr_ensemble* e, const osr_job_desc* job, int64_t* job_id) {
+  TRY(check_ens(e));
+  if (!job || !job_id) return fail(OSR_ERR_ARG, "null argument");
+  if (job->sim.path != OSR_PATH_FUSED) return fail(OSR_ERR_ARG, "ensemble jobs use the fused path");
+  if (job->nspecies < 1 || job->nspecies > kMaxSpecies) return fail(OSR_ERR_ARG, "1..8 species");
+  if (job->nsteps < 1) return fail(OSR_ERR_ARG, "nsteps must be >= 1");
+  if (job->sim.dims != 2 && job->sim.dims != 3) return fail(OSR_ERR_ARG, "dims must be 2 or 3");
+  for (int k = 0; k < job->sim.dims; k++)
+    if (job->sim.nx[k] < 1) return fail(OSR_ERR_ARG, "bad grid size");
+  std::unique_ptr<EnsJob> j(new EnsJob());
+  j->id = e->next_id++;
+  j->desc = *job;
+  j->est = job_footprint(*job);
+  if (j->est > e->budget)
+    return fail(OSR_ERR_NOMEM, "job needs %lld bytes, more than the ensemble budget %lld",
+                (long long)j->est, (long long)e->budget);
+  j->t_submit = e->now();
+  *job_id = j->id;
+  e->pending.push_back(j.get());
+  e->jobs[j->id] = std::move(j);
+  return OSR_OK;
+}
+
+int osr_ens_progress(osr_ensemble* e, int* active, int* pending) {
+  TRY(check_ens(e));
+  TRY(ens_progress(e));
+  if (active) *active = e->nactive;
+  if (pending) *pending = (int)e->pending.size();
+  return OSR_OK;
+}
+
+int osr_ens_wait(osr_ensemble* e) {
+  TRY(check_ens(e));
+  for (;;) {
+    TRY(ens_progress(e));
+    if (e->nactive == 0 && e->pending.empty()) return OSR_OK;
+    EnsBatch* busy = nullptr;
+    for (auto& b : e->batches)
+      if (b->in_flight) {
+        busy = b.get();
+        break;
+      }
+    if (busy) {
+      HIPC(hipEventSynchronize(busy->ev));
+    } else if (e->nactive == 0) {
+      return fail(OSR_ERR_STATE, "waiting jobs cannot be admitted");
+    }
+  }
+}
+
+int osr_ens_job_info(osr_ensemble* e, int64_t id, osr_job_info* info) {
+  TRY(check_ens(e));
+  EnsJob* j = find_job(e, id);
+  if (!j || !info) return fail(OSR_ERR_ARG, "unknown job %lld", (long long)id);
+  info->state = j->state;
+  info->nspecies = j->desc.nspecies;
+  info->steps = j->steps;
+  info->particles = j->particles;
+  info->footprint = j->bytes;
+  info->t_submit = j->t_submit;
+  info->t_admit = j->t_admit;
+  info->t_done = j->t_done;
+  return OSR_OK;
+}
+
+int osr_ens_job_history(osr_ensemble* e, int64_t id, int64_t first, int64_t count, double* fld,
+                        double* par) {
+  TRY(check_ens(e));
+  EnsJob* j = find_job(e, id);
+  if (!j) return fail(OSR_ERR_ARG, "unknown job %lld", (long long)id);
+  if (j->state != OSR_JOB_DONE) return fail(OSR_ERR_STATE, "job %lld has not finished", (long long)id);
+  if (first < 0 || count < 0 || first + count > j->desc.nsteps)
+    return fail(OSR_ERR_ARG, "history rows out of range");
+  const int ns = j->desc.nspecies, ncol = j->ncol;
+  for (int64_t r = 0; r < count; r++) {
+    const double* row = j->hist.data() + (size_t)(first + r) * ncol;
+    if (fld)
+      for (int c = 0; c < 6; c++) fld[6 * r + c] = row[c];
+    if (par)
+      for (int c = 0; c < ns; c++) par[ns * r + c] = row[6 + c];
+  }
+  return OSR_OK;
+}
+
+int osr_ens_job_release(osr_ensemble* e, int64_t id) {
+  TRY(check_ens(e));
+  EnsJob* j = find_job(e, id);
+  if (!j) return fail(OSR_ERR_ARG, "unknown job %lld", (long long)id);
+  if (j->state != OSR_JOB_DONE && j->state != OSR_JOB_FAILED)
+    return fail(OSR_ERR_STATE, "job %lld is still running", (long long)id);
+  e->jobs.erase(id);
+  return OSR_OK;
+}
+
+int osr_ens_usage(osr_ensemble* e, int64_t* bytes) {
+  TRY(check_ens(e));
+  if (!bytes) return fail(OSR_ERR_ARG, "null argument");
+  *bytes = e->used;
+  return OSR_OK;
+}
+
+}  // extern "C"

@@ -46,7 +46,7 @@ __global__ __launch_bounds__(256) void yee_dbdt(const FieldDesc* __restrict__ fd
   const int2 wk = work[blockIdx.x];
   const FieldDesc& fd = fds[wk.x];
   const GridDesc& g = fd.g;
-  if (g.ctrl->halt == 2) return;
+  if (g.ctrl->halt >= 2) return;
   const long long nc = g.ncell;
   const long long idx = (long long)wk.y * 256 + threadIdx.x;
   const float d1 = fd.bdt[0], d2 = fd.bdt[1], d3 = fd.bdt[2];
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(256) void yee_dedt(const FieldDesc* __restrict__ fd
   const int2 wk = work[blockIdx.x];
   const FieldDesc& fd = fds[wk.x];
   const GridDesc& g = fd.g;
-  if (g.ctrl->halt == 2) return;
+  if (g.ctrl->halt >= 2) return;
   const long long nc = g.ncell;
   const long long idx = (long long)wk.y * 256 + threadIdx.x;
   if (idx >= nc) return;
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(256) void finalize_step(const FieldDesc* __restrict
   const FieldDesc& fd = fds[work[blockIdx.x]];
   const GridDesc& g = fd.g;
   Ctrl* ctrl = g.ctrl;
-  if (ctrl->halt == 2) return;
+  if (ctrl->halt >= 2) return;
   const long long step = ctrl->steps;
   const int ncol = 6 + fd.nspec;
   for (int col = 0; col < ncol; col++) {
@@ -213,7 +213,10 @@ __global__ __launch_bounds__(256) void finalize_step(const FieldDesc* __restrict
   }
   if (threadIdx.x == 0) {
     ctrl->steps = step + 1;
-    if (ctrl->halt == 1) ctrl->halt = 2;
+    if (ctrl->halt == 1)
+      ctrl->halt = 2;
+    else if (ctrl->target > 0 && step + 1 >= ctrl->target)
+      ctrl->halt = 3;
   }
 }
 

@@ -6,11 +6,9 @@
 //  * each wave appends the virtual particles (VPs) of its particles to a private LDS
 //    queue and deposits them 64 at a time, so the deposit runs on full waves instead of
 //    looping max(#VP) times over partly idle lanes (a trajectory splits into 1..3 VPs);
-//  * each wave accumulates its current into a private FP32 LDS tile.  A wave's LDS
-//    atomics are applied in program order, so its partial sums are reproducible; the four
-//    wave tiles are added in a fixed order and flushed to the global int64 fixed-point J,
-//    whose integer adds are order independent.  The whole deposit is therefore
-//    bit-reproducible run to run.
+//  * the current of all waves is accumulated into one int64 fixed-point LDS tile; integer
+//    adds are exact and order independent, so the deposit is bit-reproducible run to run
+//    (FP32 LDS atomics were measured to be far slower here: a 40x longer LDS wait).
 // The arithmetic of gather, push, position advance and split is the one of the generic
 // kernel (the reference's AVX2 kernel, simd/os-spec-push-avx2.c:471-760).
 #include "osr_device.hpp"
@@ -28,7 +26,7 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
   constexpr int JX = TX + O::JLO + O::JHI, JY = TY + O::JLO + O::JHI, JN = JX * JY;
   constexpr int QC = 128;  // VP queue capacity per wave (< 64 pending + 64 appended)
   __shared__ float sF[6 * GN];
-  __shared__ float sJ[NW][3 * JN];
+  __shared__ unsigned long long sJ[3 * JN];
   __shared__ float sQ[NW][6][QC];  // x0, y0, x1, y1, q, vz
   __shared__ int sQn[NW][QC];      // LDS node index of the VP's first stencil point
   __shared__ double s_ene[NW];
@@ -51,7 +49,7 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
       sF[(3 + c) * GN + i] = g.B[c * nc + gi];
     }
   }
-  for (int i = threadIdx.x; i < NW * 3 * JN; i += NT) (&sJ[0][0])[i] = 0.f;
+  for (int i = threadIdx.x; i < 3 * JN; i += NT) sJ[i] = 0ull;
   __syncthreads();
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -63,9 +61,10 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
   const float tem = d.tem;
   const float dtdx1 = g.dt_dx[0], dtdx2 = g.dt_dx[1];
   const float jn1 = g.jnorm[0], jn2 = g.jnorm[1];
-  float* const J1 = &sJ[wave][0];
-  float* const J2 = &sJ[wave][JN];
-  float* const J3 = &sJ[wave][2 * JN];
+  const double jscale = g.jscale;
+  unsigned long long* const J1 = sJ;
+  unsigned long long* const J2 = sJ + JN;
+  unsigned long long* const J3 = sJ + 2 * JN;
   float(*const Q)[QC] = sQ[wave];
   int* const QN = sQn[wave];
 
@@ -94,9 +93,9 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
           float t1 = fma_(S1y[k2], S1x[k1], S0x[k1] * S0y[k2]);
           const float t2 = fma_(S0y[k2], S1x[k1], S0x[k1] * S1y[k2]);
           t1 = fma_(t2, 0.5f, t1);
-          if (k1 < NP - 1) atomicAdd(&J1[idx], wl1[k1] * wp1);
-          if (k2 < NP - 1) atomicAdd(&J2[idx], wl2[k2] * wp2);
-          atomicAdd(&J3[idx], qvz * t1);
+          if (k1 < NP - 1) atomicAdd(&J1[idx], (unsigned long long)to_fix(wl1[k1] * wp1, jscale));
+          if (k2 < NP - 1) atomicAdd(&J2[idx], (unsigned long long)to_fix(wl2[k2] * wp2, jscale));
+          atomicAdd(&J3[idx], (unsigned long long)to_fix(qvz * t1, jscale));
         }
       }
     }
@@ -243,19 +242,14 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
   if (lane == 0) s_ene[wave] = ene;
   __syncthreads();
   if (threadIdx.x == 0) d.epart[tile] = ((s_ene[0] + s_ene[1]) + s_ene[2]) + s_ene[3];
-  // ---- fixed-order sum of the wave tiles, fold into the global fixed-point J
-  const double jscale = g.jscale;
+  // ---- fold the LDS current tile into the global fixed-point J (periodic wrap)
   for (int i = threadIdx.x; i < JN; i += NT) {
     const int gx = wrapi(cx0 + (i % JX) - O::JLO, nx), gy = wrapi(cy0 + (i / JX) - O::JLO, ny);
     const long long gi = gx + (long long)nx * gy;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      const float v = ((sJ[0][c * JN + i] + sJ[1][c * JN + i]) + sJ[2][c * JN + i]) +
-                      sJ[3][c * JN + i];
-      // a node sum can be as large as the bound the shift was chosen for (|v*2^s| < 2^62),
-      // beyond the exact range of to_fix: use the full int64 conversion here
-      if (v != 0.f)
-        atomicAdd(&g.J[c * nc + gi], (unsigned long long)__double2ll_rn((double)v * jscale));
+      const unsigned long long v = sJ[c * JN + i];
+      if (v) atomicAdd(&g.J[c * nc + gi], v);
     }
   }
 }

@@ -595,7 +595,7 @@ __global__ __launch_bounds__(NT) void tile_fill(const SpecDesc* __restrict__ des
   const int2 wk = work[blockIdx.x];
   const SpecDesc& d = descs[wk.x];
   const GridDesc& g = d.g;
-  if (g.ctrl->halt == 2) return;
+  if (g.ctrl->halt >= 2) return;
   const int tile = wk.y;
   const int NW = d.waves;
   const int dims = g.dims;

@@ -22,9 +22,11 @@ constexpr int kTailBitmapBits = 32768;  // fill kernel: max holes left in the ta
 
 // Control block, one per simulation (device memory).
 struct Ctrl {
-  int halt;          // 0 run, 1 a tile overflowed this step, 2 stopped at a step end
+  int halt;          // 0 run, 1 a tile overflowed this step, 2 stopped at a step end,
+                     // 3 parked: the step target was reached
   int error;         // sticky: 1 = particle found outside its tile (layout invariant broken)
   long long steps;   // completed steps
+  long long target;  // park (halt = 3) once steps reaches target; 0 = no target
 };
 
 struct GridDesc {

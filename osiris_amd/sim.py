@@ -28,6 +28,14 @@ def device_count() -> int:
     return lib().osr_device_count()
 
 
+def copy_probe(nbytes: int, reps: int = 5) -> float:
+    """Mean time in ms of a plain device copy of ``nbytes`` bytes (bandwidth probe and
+    traffic-counter calibration)."""
+    ms = C.c_double()
+    check(lib().osr_copy_probe(int(nbytes), int(reps), C.byref(ms)))
+    return ms.value
+
+
 def advance_deposit(dims, order, ix, x, u, q, rqm, e, b, emf_lo, j, j_lo, dx, dt, pusher=OSR_BORIS):
     """Push all particles of the arrays (in place) and ADD their current to ``j``.
 

@@ -66,3 +66,37 @@ def test_argument_validation():
     assert "stability" in L.osr_last_error().decode()
     assert L.osr_advance_deposit(5, 2, 0, *([None] * 4), 1, 0, -1.0, *([None] * 8), 0.1,
                                  None) == 1
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """The ctypes mirrors of the header's structs have the C sizes and field offsets."""
+    import shutil
+    import subprocess
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = {"osr_sim_desc": _lib.SimDesc, "osr_ens_desc": _lib.EnsDesc,
+               "osr_job_species": _lib.JobSpecies, "osr_job_desc": _lib.JobDesc,
+               "osr_job_info": _lib.JobInfo}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "osiris_amd.h"',
+             'int main(void) {']
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    got = {}
+    for ln in out.split("\n"):
+        if ln:
+            s, f, v = ln.split()
+            got[(s, f)] = int(v)
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
