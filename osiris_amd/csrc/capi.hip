@@ -106,6 +106,13 @@ struct DBuf {
     n = bytes;
     return OSR_OK;
   }
+  // exchange ownership, including how the memory must be released
+  void swap(DBuf& o) {
+    std::swap(p, o.p);
+    std::swap(n, o.n);
+    std::swap(async, o.async);
+    std::swap(st, o.st);
+  }
   template <class T>
   T* as() const {
     return (T*)p;
@@ -709,8 +716,7 @@ int ensure_hist(osr_sim* s, int64_t rows) {
   if (s->hist_cap > 0 && s->hist.p)
     HIPC(hipMemcpy(nb.p, s->hist.p, sizeof(double) * (size_t)s->hist_cap * ncol,
                    hipMemcpyDeviceToDevice));
-  std::swap(s->hist.p, nb.p);
-  std::swap(s->hist.n, nb.n);
+  s->hist.swap(nb);
   s->hist_cap = cap;
   return upload_descs(s);
 }

@@ -176,17 +176,30 @@ OSR_DEV void kadd(double& s, double& c, double v) {
   s = t;
 }
 
-// One block per simulation: fixed-order compensated reduction of the step's partial
-// sums into one history row, then the step counter and the halt protocol.
+// Merge of two compensated sums (s, c) and (s2, c2): error-free TwoSum of the leading
+// parts, the rounding error joins the compensation.
+OSR_DEV void kmerge(double& s, double& c, double s2, double c2) {
+  const double t = s + s2;
+  const double bp = t - s;
+  const double err = (s - (t - bp)) + (s2 - bp);
+  c = (c + c2) + err;
+  s = t;
+}
+
+// One block per simulation: compensated reduction of the step's partial sums into one
+// history row, then the step counter and the halt protocol.  The order of every addition
+// is fixed (strided per-thread sums, then a fixed shuffle tree per wave and the four
+// waves in order), so the row is reproducible bit for bit.
 __global__ __launch_bounds__(256) void finalize_step(const FieldDesc* __restrict__ fds,
                                                      const int* __restrict__ work) {
-  __shared__ double s_s[256], s_c[256];
+  __shared__ double w_s[4], w_c[4];
   const FieldDesc& fd = fds[work[blockIdx.x]];
   const GridDesc& g = fd.g;
   Ctrl* ctrl = g.ctrl;
   if (ctrl->halt >= 2) return;
   const long long step = ctrl->steps;
   const int ncol = 6 + fd.nspec;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int col = 0; col < ncol; col++) {
     double s = 0.0, c = 0.0;
     if (col < 6) {
@@ -196,15 +209,18 @@ __global__ __launch_bounds__(256) void finalize_step(const FieldDesc* __restrict
       const double* ep = fd.ep[sp];
       for (int t = threadIdx.x; t < fd.ep_n[sp]; t += 256) kadd(s, c, ep[t]);
     }
-    s_s[threadIdx.x] = s;
-    s_c[threadIdx.x] = c;
+    for (int off = 32; off > 0; off >>= 1) {
+      const double s2 = __shfl_down(s, off), c2 = __shfl_down(c, off);
+      kmerge(s, c, s2, c2);
+    }
+    if (lane == 0) {
+      w_s[wave] = s;
+      w_c[wave] = c;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-      double ts = 0.0, tc = 0.0;
-      for (int t = 0; t < 256; t++) {
-        kadd(ts, tc, s_s[t]);
-        tc += s_c[t];
-      }
+      double ts = w_s[0], tc = w_c[0];
+      for (int w = 1; w < 4; w++) kmerge(ts, tc, w_s[w], w_c[w]);
       const double v = ts + tc;
       if (step < fd.hist_cap)
         fd.hist[step * ncol + col] = col < 6 ? v * fd.fe_scale : v * fd.ep_scale[col - 6];
