@@ -413,6 +413,14 @@ extern "C" {
 int osr_ens_create(const osr_ens_desc* desc, osr_ensemble** out) {
   if (!out) return fail(OSR_ERR_ARG, "null argument");
   *out = nullptr;
+#ifndef OSR_ENSEMBLE_HOST_CHECK
+  // Switched off: a long mixed serving run (256 jobs, grids up to 512^2, up to 2000
+  // steps) ended in a GPU memory fault whose cause is not identified yet.  The scheduler
+  // itself is exercised on the CPU (host-logic harness over a validating mock runtime).
+  return fail(OSR_ERR_STATE,
+              "ensemble execution is switched off: a GPU memory fault in long mixed runs is "
+              "under investigation");
+#endif
   if (osr_device_count() < 1) return fail(OSR_ERR_HIP, "no GPU visible");
   std::unique_ptr<osr_ensemble> e(new osr_ensemble());
   if (desc) e->desc = *desc;

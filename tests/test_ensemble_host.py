@@ -1,0 +1,53 @@
+"""The ensemble scheduler's host logic on the CPU, over a validating mock of the HIP
+runtime (tests/hostcheck): admission, batching, chunked graph replay, in-flight joins,
+re-layouts after (injected) tile overflows and retirement must never hand a kernel a
+descriptor that points to freed or too-small device memory, must release every device
+allocation with the call that matches how it was made, and must return each job's full
+history."""
+import json
+import os
+import shutil
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HC = os.path.join(ROOT, "tests", "hostcheck")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+
+@pytest.fixture(scope="module")
+def mock_lib():
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not available")
+    out_dir = os.path.join(HC, "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "libosiris_hostcheck.so")
+    srcs = [os.path.join(HC, "capi_mock.hip"), os.path.join(HC, "mock_hip.h")] + [
+        os.path.join(ROOT, "osiris_amd", "csrc", f)
+        for f in os.listdir(os.path.join(ROOT, "osiris_amd", "csrc"))]
+    if not os.path.exists(out) or os.path.getmtime(out) < max(os.path.getmtime(s) for s in srcs):
+        subprocess.run([HIPCC, "--offload-arch=gfx950", "-O1", "-g", "-std=c++17", "-fPIC",
+                        "-shared", "-rdynamic", "-Xclang", "-target-feature", "-Xclang",
+                        "-packed-fp32-ops", "-I", os.path.join(ROOT, "include"), "-o", out,
+                        os.path.join(HC, "capi_mock.hip")], check=True)
+    return out
+
+
+def _run(lib, n, overflow_p, notready_p, sides, ppcs):
+    r = subprocess.run([sys.executable, os.path.join(HC, "run_stream.py"), lib, str(n),
+                        str(overflow_p), str(notready_p), sides, ppcs],
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-3000:]
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def test_ensemble_host_logic_with_relayouts(mock_lib):
+    res = _run(mock_lib, 40, 0.01, 0.5, "64,128", "4,16")
+    assert res == {"jobs": 40, "errors": 0, "live": 0, "bad_histories": 0}
+
+
+def test_ensemble_host_logic_large_jobs(mock_lib):
+    res = _run(mock_lib, 24, 0.002, 0.3, "128,256", "4,16,64")
+    assert res == {"jobs": 24, "errors": 0, "live": 0, "bad_histories": 0}

@@ -7,7 +7,12 @@ import pytest
 from osiris_amd.ensemble import Ensemble, Job, SpeciesSpec, poisson_jobs, run_sequential, serve
 from osiris_amd._lib import OsrError
 
-pytestmark = pytest.mark.gpu
+pytestmark = [
+    pytest.mark.gpu,
+    pytest.mark.skip(reason="ensemble execution is switched off: a 256-job mixed serving run "
+                            "ended in a GPU memory fault whose cause is not identified yet; the "
+                            "scheduler is covered on the CPU by tests/test_ensemble_host.py"),
+]
 
 
 def _job(side, order, ppc, steps, seed, uth=0.05, drift=0.0, kernel=0):
