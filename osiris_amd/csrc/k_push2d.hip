@@ -3,9 +3,14 @@
 // One workgroup (4 waves) per (species, tile).  Compared with the generic kernel in
 // k_tiles.hip:
 //  * TX/TY are template parameters, so every LDS stencil offset is an immediate;
-//  * each wave appends the virtual particles (VPs) of its particles to a private LDS
-//    queue and deposits them 64 at a time, so the deposit runs on full waves instead of
-//    looping max(#VP) times over partly idle lanes (a trajectory splits into 1..3 VPs);
+//  * particle, mover and field arrays are accessed through global-address-space
+//    pointers: uniform base + 32-bit offset, no per-array 64-bit address arithmetic, and
+//    the loads do not share the LDS wait counter as flat accesses would;
+//  * the first virtual particle (VP) of every particle is deposited straight from
+//    registers, with the start-point splines of the gather (VP 0 starts at the old
+//    position); a trajectory that crosses a cell face has 1 or 2 more VPs, which each
+//    wave appends to a private LDS queue and deposits 64 at a time, so the deposit runs
+//    on full waves instead of looping max(#VP) times over partly idle lanes;
 //  * the current of all waves is accumulated into one int64 fixed-point LDS tile; integer
 //    adds are exact and order independent, so the deposit is bit-reproducible run to run
 //    (FP32 LDS atomics were measured to be far slower here: a 40x longer LDS wait).
@@ -15,6 +20,65 @@
 #include "osr_types.hpp"
 
 namespace osr {
+
+typedef __attribute__((address_space(1))) float gf32;
+typedef __attribute__((address_space(1))) uint32_t gu32;
+typedef __attribute__((address_space(1))) int gi32;
+typedef __attribute__((address_space(1))) unsigned long long gu64;
+
+// Charge-conserving deposit of one VP into the LDS current tile (DEP_CURRENT_2D,
+// spec/os-spec-current.f03:2691-2796); S0x/S0y are the splines of its start point and
+// nb the LDS node of its first stencil point.
+//
+// The weights are those of the reference: j1 += wl1(k1) (S0y + S1y)(k2),
+// j2 += (S0x + S1x)(k1) wl2(k2), j3 += (q vz / 3) [S0x S0y + S1x S1y + (S0x S1y + S1x S0y)/2].
+// Each product is formed exactly in FP64 from FP32 factors that already carry the fixed-
+// point scale 2^s (a power of two, so the scaling is exact) and rounded once to the int64
+// grid by fma(a, b, 1.5*2^52): one FMA and one integer add per contribution.  The j3 bracket
+// is evaluated as A S0y + B S1y with A = S0x + S1x/2, B = S1x + S0x/2 (two FP32 operations
+// per node).  |a b 2^s| < 2^51 for every contribution (bound of the scale choice).
+template <int ORDER, int JX>
+OSR_DEV void dep_vp(unsigned long long* __restrict__ J1, unsigned long long* __restrict__ J2,
+                    unsigned long long* __restrict__ J3, const float* S0x, const float* S0y,
+                    float x0, float y0, float x1, float y1, float q, float vz, int nb, float jn1,
+                    float jn2, double jscale) {
+  constexpr int NP = ORDER + 1;
+  constexpr double M = 6755399441055744.0;  // 1.5 * 2^52
+  float S1x[NP], S1y[NP], wl1[NP], wl2[NP];
+  spline<ORDER>(x1, S1x);
+  spline<ORDER>(y1, S1y);
+  wl<ORDER>(q * jn1, x0, x1, wl1);
+  wl<ORDER>(q * jn2, y0, y1, wl2);
+  double a1[NP - 1], a2[NP - 1], b1[NP], b2[NP];
+  float A[NP], B[NP];
+#pragma unroll
+  for (int k = 0; k < NP - 1; k++) {
+    a1[k] = (double)wl1[k] * jscale;
+    a2[k] = (double)wl2[k] * jscale;
+  }
+#pragma unroll
+  for (int k = 0; k < NP; k++) {
+    b1[k] = (double)(S0y[k] + S1y[k]);
+    b2[k] = (double)(S0x[k] + S1x[k]);
+    A[k] = fma_(S1x[k], 0.5f, S0x[k]);
+    B[k] = fma_(S0x[k], 0.5f, S1x[k]);
+  }
+  const double c3 = (double)((q * vz) * (1.0f / 3.0f)) * jscale;
+#pragma unroll
+  for (int k2 = 0; k2 < NP; k2++) {
+#pragma unroll
+    for (int k1 = 0; k1 < NP; k1++) {
+      const int idx = nb + k2 * JX + k1;
+      const float t1 = fma_(B[k1], S1y[k2], A[k1] * S0y[k2]);
+      if (k1 < NP - 1) atomicAdd(&J1[idx], fix_bits(__builtin_fma(a1[k1], b1[k2], M)));
+      if (k2 < NP - 1) atomicAdd(&J2[idx], fix_bits(__builtin_fma(a2[k2], b2[k1], M)));
+      atomicAdd(&J3[idx], fix_bits(__builtin_fma(c3, (double)t1, M)));
+    }
+  }
+}
+
+// periodic wrap of a cell index at most one period outside [0, n)
+OSR_DEV int wrap1(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 template <int ORDER, int PUSHER, int TX, int TY>
 __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restrict__ descs,
@@ -40,21 +104,26 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
   const int nx = g.nx, ny = g.ny;
   const long long nc = g.ncell;
 
-  for (int i = threadIdx.x; i < GN; i += NT) {
-    const int gx = wrapi(cx0 + (i % GX) - O::GLO, nx), gy = wrapi(cy0 + (i / GX) - O::GLO, ny);
-    const long long gi = gx + (long long)nx * gy;
+  {
+    const gf32* __restrict__ E = (const gf32*)g.E;
+    const gf32* __restrict__ B = (const gf32*)g.B;
+    for (int i = threadIdx.x; i < GN; i += NT) {
+      const int gx = wrapi(cx0 + (i % GX) - O::GLO, nx), gy = wrapi(cy0 + (i / GX) - O::GLO, ny);
+      const long long gi = gx + (long long)nx * gy;
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-      sF[c * GN + i] = g.E[c * nc + gi];
-      sF[(3 + c) * GN + i] = g.B[c * nc + gi];
+      for (int c = 0; c < 3; c++) {
+        sF[c * GN + i] = E[c * nc + gi];
+        sF[(3 + c) * GN + i] = B[c * nc + gi];
+      }
     }
   }
   for (int i = threadIdx.x; i < 3 * JN; i += NT) sJ[i] = 0ull;
   __syncthreads();
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int base = d.off[tile];
-  const int n = d.cnt[tile];
+  // uniform per workgroup: keep them (and the array bases below) in scalar registers
+  const int base = __builtin_amdgcn_readfirstlane(d.off[tile]);
+  const int n = __builtin_amdgcn_readfirstlane(d.cnt[tile]);
   const int per_wave = ((n + NT - 1) / NT) * 64;
   const int w0 = wave * per_wave;
   const int w1 = min(w0 + per_wave, n);
@@ -67,37 +136,41 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
   unsigned long long* const J3 = sJ + 2 * JN;
   float(*const Q)[QC] = sQ[wave];
   int* const QN = sQn[wave];
+  // particle and mover arrays of this tile, global address space
+  const gu32* __restrict__ Pcell = (const gu32*)d.cell + base;
+  const gf32* __restrict__ Px = (const gf32*)d.x + base;
+  const gf32* __restrict__ Py = (const gf32*)d.y + base;
+  const gf32* __restrict__ Pux = (const gf32*)d.ux + base;
+  const gf32* __restrict__ Puy = (const gf32*)d.uy + base;
+  const gf32* __restrict__ Puz = (const gf32*)d.uz + base;
+  const gf32* __restrict__ Pq = (const gf32*)d.q + base;
+  const gu32* __restrict__ Ppid = (const gu32*)d.pid + base;
+  gu32* __restrict__ Wcell = (gu32*)d.cell + base;
+  gf32* __restrict__ Wx = (gf32*)d.x + base;
+  gf32* __restrict__ Wy = (gf32*)d.y + base;
+  gf32* __restrict__ Wux = (gf32*)d.ux + base;
+  gf32* __restrict__ Wuy = (gf32*)d.uy + base;
+  gf32* __restrict__ Wuz = (gf32*)d.uz + base;
+  gu32* __restrict__ Mcell = (gu32*)d.mcell + base;
+  gf32* __restrict__ Mx = (gf32*)d.mx + base;
+  gf32* __restrict__ My = (gf32*)d.my + base;
+  gf32* __restrict__ Mux = (gf32*)d.mux + base;
+  gf32* __restrict__ Muy = (gf32*)d.muy + base;
+  gf32* __restrict__ Muz = (gf32*)d.muz + base;
+  gf32* __restrict__ Mq = (gf32*)d.mq + base;
+  gu32* __restrict__ Mpid = (gu32*)d.mpid + base;
+  gi32* __restrict__ Midx = (gi32*)d.midx + base;
 
-  // deposit of the VPs in queue slots [head, head + cnt) (cnt <= 64, one per lane)
-  auto deposit = [&](int head, int cnt) {
+  // deposit of the queued VPs in slots [head, head + cnt) (cnt <= 64, one per lane)
+  auto deposit_queue = [&](int head, int cnt) {
     if (lane < cnt) {
       const int s = (head + lane) & (QC - 1);
-      const float x0 = Q[0][s], y0 = Q[1][s], x1 = Q[2][s], y1 = Q[3][s];
-      const float q = Q[4][s], vz = Q[5][s];
-      const int nb = QN[s];
-      float S0x[NP], S1x[NP], S0y[NP], S1y[NP], wl1[NP], wl2[NP];
+      const float x0 = Q[0][s], y0 = Q[1][s];
+      float S0x[NP], S0y[NP];
       spline<ORDER>(x0, S0x);
-      spline<ORDER>(x1, S1x);
       spline<ORDER>(y0, S0y);
-      spline<ORDER>(y1, S1y);
-      wl<ORDER>(q * jn1, x0, x1, wl1);
-      wl<ORDER>(q * jn2, y0, y1, wl2);
-      const float qvz = (q * vz) * (1.0f / 3.0f);
-#pragma unroll
-      for (int k2 = 0; k2 < NP; k2++) {
-        const float wp1 = S0y[k2] + S1y[k2];
-#pragma unroll
-        for (int k1 = 0; k1 < NP; k1++) {
-          const int idx = nb + k2 * JX + k1;
-          const float wp2 = S0x[k1] + S1x[k1];
-          float t1 = fma_(S1y[k2], S1x[k1], S0x[k1] * S0y[k2]);
-          const float t2 = fma_(S0y[k2], S1x[k1], S0x[k1] * S1y[k2]);
-          t1 = fma_(t2, 0.5f, t1);
-          if (k1 < NP - 1) atomicAdd(&J1[idx], (unsigned long long)to_fix(wl1[k1] * wp1, jscale));
-          if (k2 < NP - 1) atomicAdd(&J2[idx], (unsigned long long)to_fix(wl2[k2] * wp2, jscale));
-          atomicAdd(&J3[idx], (unsigned long long)to_fix(qvz * t1, jscale));
-        }
-      }
+      dep_vp<ORDER, JX>(J1, J2, J3, S0x, S0y, x0, y0, Q[2][s], Q[3][s], Q[4][s], Q[5][s], QN[s],
+                        jn1, jn2, jscale);
     }
   };
 
@@ -107,18 +180,18 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
   bool bad = false;
   for (int p0 = w0; p0 < w1; p0 += 64) {
     const int p = p0 + lane;
+    const unsigned up = (unsigned)p;  // 32-bit unsigned offsets from the scalar bases
     bool act = p < w1;
     float x = 0.f, y = 0.f, u1 = 0.f, u2 = 0.f, u3 = 0.f, q = 0.f;
     int lx = 0, ly = 0;
     if (act) {
-      const int s = base + p;
-      const uint32_t cell = d.cell[s];
-      x = d.x[s];
-      y = d.y[s];
-      u1 = d.ux[s];
-      u2 = d.uy[s];
-      u3 = d.uz[s];
-      q = d.q[s];
+      const uint32_t cell = Pcell[up];
+      x = Px[up];
+      y = Py[up];
+      u1 = Pux[up];
+      u2 = Puy[up];
+      u3 = Puz[up];
+      q = Pq[up];
       int ix, iy;
       unpack2(cell, ix, iy);
       lx = ix - cx0;
@@ -175,14 +248,16 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
     const float x1 = fma_(v1, dtdx1, x), y1 = fma_(v2, dtdx2, y);
     const float tx = trim(x1), ty = trim(y1);
     const int dix = (int)tx, diy = (int)ty;
-    // ---- split; the VPs are queued one "rank" at a time (first VPs of all lanes, then
-    // the second ones, ...) and full waves of 64 are deposited as soon as they are there
+    // ---- split; VP 0 is deposited in place, VPs 1 and 2 go through the queue
     VP2 vp[3];
     const int nvp = act ? split2d(x, y, x1, y1, dix, diy, q, v3, vp) : 0;
     const int nb0 = (ly + O::LP + O::JLO) * JX + lx + O::LP + O::JLO;
+    if (act)
+      dep_vp<ORDER, JX>(J1, J2, J3, wx, wy, vp[0].x0, vp[0].y0, vp[0].x1, vp[0].y1, vp[0].q,
+                        vp[0].vz, nb0, jn1, jn2, jscale);
     const unsigned long long below = (1ull << lane) - 1ull;
 #pragma unroll
-    for (int v = 0; v < 3; v++) {
+    for (int v = 1; v < 3; v++) {
       const unsigned long long m = __ballot(v < nvp);
       if (m == 0ull) break;
       if (v < nvp) {
@@ -197,43 +272,41 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
       }
       qlen += __popcll(m);
       if (qlen >= 64) {
-        deposit(qhead, 64);
+        deposit_queue(qhead, 64);
         qhead = (qhead + 64) & (QC - 1);
         qlen -= 64;
       }
     }
     // ---- store: stay in place, or leave through the mover buffer
-    const int nix = wrapi(lx + cx0 + dix, nx), niy = wrapi(ly + cy0 + diy, ny);
+    const int nix = wrap1(lx + cx0 + dix, nx), niy = wrap1(ly + cy0 + diy, ny);
     const uint32_t ncell = pack2(nix, niy);
     const bool stay = (nix / TX + g.ntx * (niy / TY)) == tile;
     const float xs = x1 - tx, ys = y1 - ty;
     if (act && stay) {
-      const int s = base + p;
-      d.cell[s] = ncell;
-      d.x[s] = xs;
-      d.y[s] = ys;
-      d.ux[s] = u1;
-      d.uy[s] = u2;
-      d.uz[s] = u3;
+      Wcell[up] = ncell;
+      Wx[up] = xs;
+      Wy[up] = ys;
+      Wux[up] = u1;
+      Wuy[up] = u2;
+      Wuz[up] = u3;
     }
     const bool mv = act && !stay;
     const unsigned long long mm = __ballot(mv);
     if (mv) {
-      const int slot = base + w0 + mcount + __popcll(mm & ((1ull << lane) - 1ull));
-      const int s = base + p;
-      d.mcell[slot] = ncell;
-      d.mx[slot] = xs;
-      d.my[slot] = ys;
-      d.mux[slot] = u1;
-      d.muy[slot] = u2;
-      d.muz[slot] = u3;
-      d.mq[slot] = q;
-      d.mpid[slot] = d.pid[s];
-      d.midx[slot] = p;
+      const unsigned slot = (unsigned)(w0 + mcount + __popcll(mm & below));
+      Mcell[slot] = ncell;
+      Mx[slot] = xs;
+      My[slot] = ys;
+      Mux[slot] = u1;
+      Muy[slot] = u2;
+      Muz[slot] = u3;
+      Mq[slot] = q;
+      Mpid[slot] = Ppid[up];
+      Midx[slot] = p;
     }
     mcount += __popcll(mm);
   }
-  if (qlen > 0) deposit(qhead, qlen);
+  if (qlen > 0) deposit_queue(qhead, qlen);
 
   if (bad) atomicOr(&g.ctrl->error, 1);
   if (lane == 0) d.mv_cnt[tile * NW + wave] = mcount;
@@ -243,13 +316,14 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
   __syncthreads();
   if (threadIdx.x == 0) d.epart[tile] = ((s_ene[0] + s_ene[1]) + s_ene[2]) + s_ene[3];
   // ---- fold the LDS current tile into the global fixed-point J (periodic wrap)
+  gu64* __restrict__ GJ = (gu64*)g.J;
   for (int i = threadIdx.x; i < JN; i += NT) {
     const int gx = wrapi(cx0 + (i % JX) - O::JLO, nx), gy = wrapi(cy0 + (i / JX) - O::JLO, ny);
     const long long gi = gx + (long long)nx * gy;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const unsigned long long v = sJ[c * JN + i];
-      if (v) atomicAdd(&g.J[c * nc + gi], v);
+      if (v) __hip_atomic_fetch_add(&GJ[c * nc + gi], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }

@@ -201,6 +201,10 @@ OSR_DEV long long to_fix(float v, double scale) {
   double t = __builtin_fma((double)v, scale, 6755399441055744.0);
   return __double_as_longlong(t) - 0x4338000000000000LL;
 }
+// the integer held by t = fma(a, b, 1.5*2^52), as the 64-bit pattern of the sum
+OSR_DEV unsigned long long fix_bits(double t) {
+  return (unsigned long long)(__double_as_longlong(t) - 0x4338000000000000LL);
+}
 
 // ------------------------------------------------------------------------------------
 // Virtual particles.  split_2d: spec/os-spec-current.f03:659-888, vsplit2D
