@@ -55,7 +55,7 @@ static int fail(int code, const char* fmt, ...) {
 
 namespace {
 
-constexpr int kNT = 256;     // threads per workgroup of the tile kernels
+constexpr int kNT = 256;  // threads per workgroup of the push kernels
 constexpr int kWaves = kNT / 64;
 
 // Device allocations made while g_alloc_stream is set (ensemble jobs) come from the
@@ -616,9 +616,19 @@ int launch_push(osr_sim* s, int sid) {
   return OSR_OK;
 }
 
+// Fill kernel and its launch shape (4 waves per tile; LDS lists for up to 4096 incoming
+// particles and 32768 holes left in a tile's tail).
+typedef void (*fill_fn)(const SpecDesc*, const int2*);
+struct FillKernel {
+  fill_fn f;
+  int nt;
+};
+FillKernel pick_fill() { return {tile_fill<256, 4096, 32768>, 256}; }
+
 int launch_fill(osr_sim* s) {
   const int nw = (int)s->sp.size() * s->ntiles;
-  hipLaunchKernelGGL(tile_fill<kNT>, dim3(nw), dim3(kNT), 0, s->stream, s->d_spec.as<SpecDesc>(),
+  const FillKernel k = pick_fill();
+  hipLaunchKernelGGL(k.f, dim3(nw), dim3(k.nt), 0, s->stream, s->d_spec.as<SpecDesc>(),
                      s->d_work.as<int2>());
   HIPC(hipGetLastError());
   return OSR_OK;
@@ -1112,6 +1122,7 @@ int osr_sim_run(osr_sim* s, int nsteps) {
       Ctrl z = c;
       z.halt = 0;
       HIPC(hipMemcpy(s->ctrl.p, &z, sizeof(Ctrl), hipMemcpyHostToDevice));
+      TRY(build_graph(s));
       batch = 8;
     } else if (done != nb) {
       return fail(OSR_ERR_STATE, "step counter advanced by %lld instead of %d",

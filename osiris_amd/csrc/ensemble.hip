@@ -284,8 +284,9 @@ int ens_build(osr_ensemble* e, EnsBatch* b) {
                        b->wpush[p].as<int2>());
   }
   if (r == OSR_OK) {
-    hipLaunchKernelGGL(tile_fill<kNT>, dim3(b->nfill), dim3(kNT), 0, b->stream,
-                       b->spec.as<SpecDesc>(), b->wfill.as<int2>());
+    const FillKernel fk = pick_fill();
+    hipLaunchKernelGGL(fk.f, dim3(b->nfill), dim3(fk.nt), 0, b->stream, b->spec.as<SpecDesc>(),
+                       b->wfill.as<int2>());
     const FieldDesc* fd = b->fds.as<FieldDesc>();
     const int2* w = b->wyee.as<int2>();
     if (b->dims == 2) {

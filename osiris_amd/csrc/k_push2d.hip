@@ -250,7 +250,7 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
     const int dix = (int)tx, diy = (int)ty;
     // ---- split; VP 0 is deposited in place, VPs 1 and 2 go through the queue
     VP2 vp[3];
-    const int nvp = act ? split2d(x, y, x1, y1, dix, diy, q, v3, vp) : 0;
+    const int nvp = act ? split2d_fast(x, y, x1, y1, dix, diy, q, v3, vp) : 0;
     const int nb0 = (ly + O::LP + O::JLO) * JX + lx + O::LP + O::JLO;
     if (act)
       dep_vp<ORDER, JX>(J1, J2, J3, wx, wy, vp[0].x0, vp[0].y0, vp[0].x1, vp[0].y1, vp[0].q,

@@ -571,7 +571,6 @@ OSR_DEV void copy_particle(const SpecDesc& d, int src, int dst, int dims) {
   d.pid[dst] = d.pid[src];
 }
 
-constexpr int kIncMax = 4096;  // incoming movers one fill can place
 
 // Move the movers destined to this tile into it and close the holes (deterministic:
 // sources = incoming movers by (neighbour, wave, slot), then the non-hole particles of
@@ -580,7 +579,9 @@ constexpr int kIncMax = 4096;  // incoming movers one fill can place
 // capacity, or the LDS lists of this kernel, would overflow) changes nothing, marks
 // itself with inc = -1 and raises Ctrl::halt; the host then re-lays the species out from
 // the tiles that did fill plus the mover buffers.
-template <int NT>
+// NT threads per tile; INC_MAX incoming movers and TAIL_BITS holes left in the tail are the
+// most one fill can handle (LDS lists); beyond them the tile flags an overflow.
+template <int NT, int INC_MAX, int TAIL_BITS>
 __global__ __launch_bounds__(NT) void tile_fill(const SpecDesc* __restrict__ descs,
                                                 const int2* __restrict__ work) {
   constexpr int MAXL = 26 * 4;  // candidate lists: neighbours x waves
@@ -590,8 +591,8 @@ __global__ __launch_bounds__(NT) void tile_fill(const SpecDesc* __restrict__ des
   __shared__ int s_lbase[MAXL];
   __shared__ int s_wtot[NT / 64];
   __shared__ int s_hpre[17];
-  __shared__ int s_inc[kIncMax];
-  __shared__ unsigned int s_bits[kTailBitmapBits / 32];
+  __shared__ int s_inc[INC_MAX];
+  __shared__ unsigned int s_bits[TAIL_BITS / 32];
   const int2 wk = work[blockIdx.x];
   const SpecDesc& d = descs[wk.x];
   const GridDesc& g = d.g;
@@ -609,10 +610,10 @@ __global__ __launch_bounds__(NT) void tile_fill(const SpecDesc* __restrict__ des
   }
   __syncthreads();
   const int L = s_nnb * NW;
-  if (threadIdx.x < L) {
-    const int nt = s_nb[threadIdx.x / NW], w = threadIdx.x % NW;
-    s_lofs[threadIdx.x + 1] = d.mv_cnt[nt * NW + w];
-    s_lbase[threadIdx.x] = d.off[nt] + w * d.mv_stride[nt];
+  for (int l = threadIdx.x; l < L; l += NT) {
+    const int nt = s_nb[l / NW], w = l % NW;
+    s_lofs[l + 1] = d.mv_cnt[nt * NW + w];
+    s_lbase[l] = d.off[nt] + w * d.mv_stride[nt];
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -636,11 +637,11 @@ __global__ __launch_bounds__(NT) void tile_fill(const SpecDesc* __restrict__ des
     }
     int tot;
     const int r = block_rank<NT>(f, s_wtot, tot);
-    if (f && I + r < kIncMax) s_inc[I + r] = src;
+    if (f && I + r < INC_MAX) s_inc[I + r] = src;
     I += tot;
   }
   const int nn = n - H + I;
-  if (I > kIncMax || nn > d.cap[tile] - 64 * NW || H - I > kTailBitmapBits) {
+  if (I > INC_MAX || nn > d.cap[tile] - 64 * NW || H - I > TAIL_BITS) {
     if (threadIdx.x == 0) {
       d.inc[tile] = -1;
       atomicOr(&g.ctrl->halt, 1);
@@ -705,6 +706,6 @@ OSR_INST(3, 0)
 OSR_INST(1, 1)
 OSR_INST(2, 1)
 OSR_INST(3, 1)
-template __global__ void tile_fill<256>(const SpecDesc*, const int2*);
+template __global__ void tile_fill<256, 4096, 32768>(const SpecDesc*, const int2*);
 
 }  // namespace osr

@@ -269,6 +269,61 @@ OSR_DEV int split2d(float xo, float yo, float xn, float yn, int d1, int d2, floa
   return n;
 }
 
+// split2d with one division shared by the single-crossing cases (x face or y face)
+// instead of one divergent branch per case.  Every VP field is computed by the same
+// expression as in split2d, so the results are bit-identical; the double crossing
+// (a rare case) keeps its own branch.
+OSR_DEV int split2d_fast(float xo, float yo, float xn, float yn, int d1, int d2, float q,
+                         float vz, VP2* vp) {
+  const bool cx = d1 != 0, cy = d2 != 0;
+  const float f1 = (float)d1, f2 = (float)d2;
+  VP2 a = {xo, yo, xn, yn, q, vz, 0, 0};
+  VP2 b = a, c = a;
+  if (cx || cy) {
+    // first crossing: the x face when the x cell changes, else the y face
+    const float xint = 0.5f * f1, yface = 0.5f * f2;
+    const float num = cx ? xint - xo : yface - yo;
+    const float den = cx ? xn - xo : yn - yo;
+    const float delta = num / den;
+    const float o0 = cx ? yo : xo, o1 = cx ? yn : xn;
+    const float oint = o0 + (o1 - o0) * delta;  // yint (x face) or xint (y face)
+    a.x1 = cx ? xint : oint;
+    a.y1 = cx ? oint : yface;
+    a.vz = vz * delta;
+    b.x0 = cx ? -xint : oint;
+    b.y0 = cx ? oint : -yface;
+    b.x1 = cx ? xn - f1 : xn;
+    b.y1 = cx ? yn : yn - f2;
+    b.vz = vz * (1.0f - delta);
+    b.di = d1;
+    b.dj = d2;
+    if (cx && cy) {
+      const float yint = oint;
+      const float yint2 = 0.5f * f2;
+      if (yint >= -0.5f && yint < 0.5f) {
+        const float vzint = vz * delta;
+        const float vzint2 = vz * (1.0f - delta);
+        const float delta2 = (yint2 - yint) / (yn - yint);
+        const float xint2 = -xint + (xn - xint) * delta2;
+        a = {xo, yo, xint, yint, q, vzint, 0, 0};
+        b = {-xint, yint, xint2, yint2, q, vzint2 * delta2, d1, 0};
+        c = {xint2, -yint2, xn - f1, yn - f2, q, vzint2 * (1.0f - delta2), d1, d2};
+      } else {
+        const float vzint = vz * delta;
+        const float delta2 = (yint2 - yo) / (yint - yo);
+        const float xint2 = xo + (xint - xo) * delta2;
+        a = {xo, yo, xint2, yint2, q, vzint * delta2, 0, 0};
+        b = {xint2, -yint2, xint, yint - f2, q, vzint * (1.0f - delta2), 0, d2};
+        c = {-xint, yint - f2, xn - f1, yn - f2, q, vz - vzint, d1, d2};
+      }
+    }
+  }
+  vp[0] = a;
+  vp[1] = b;
+  vp[2] = c;
+  return 1 + (cx || cy) + (cx && cy);
+}
+
 struct VP3 {
   float x0, y0, z0, x1, y1, z1, q;
   int di, dj, dk;

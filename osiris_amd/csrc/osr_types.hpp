@@ -18,7 +18,6 @@
 namespace osr {
 
 constexpr int kMaxSpecies = 8;
-constexpr int kTailBitmapBits = 32768;  // fill kernel: max holes left in the tail
 
 // Control block, one per simulation (device memory).
 struct Ctrl {

@@ -68,7 +68,7 @@ void osr_mock_exec(const void* k, dim3 g, const SpecDesc* d, const int2* w) {
     if (wk.y < 0 || wk.y >= d[wk.x].g.ntiles) mock::err("tile index", d, (size_t)wk.y);
     if (seen.insert(wk.x).second) check_spec(d[wk.x]);
   }
-  if (k == (const void*)tile_fill<kNT>) {
+  if (k == (const void*)tile_fill<256, 4096, 32768>) {
     std::uniform_real_distribution<double> U(0, 1);
     for (int i : seen) {
       Ctrl* c = d[i].g.ctrl;

@@ -123,6 +123,21 @@ def test_two_stream_2d_vs_oracle():
     _compare_with_oracle(2, nx, dx, dt, 2, sp, 40, tile=(16, 16), check_ix_steps=3)
 
 
+def test_c1_thermal_128_full_run_vs_fp64_oracle():
+    """The reference's CPU-runnable configuration at full size: 2D periodic thermal
+    plasma, 128x128 cells, dx = 0.1, dt = 0.07, 16 ppc at uniform random positions,
+    u_th = 0.05, linear splines, Boris, 500 steps.  FP32 on the GPU against the FP64
+    oracle: energy histories within 1e-4 (the FP32 restatement of the oracle stays within
+    1.2e-7 of FP64 here), cell indices identical over the first steps but for particles
+    within FP32 rounding of a cell edge, Gauss law conserved."""
+    rng = np.random.default_rng(47)
+    nx, dx, dt = (128, 128), (0.1, 0.1), 0.07
+    ix, x, u, q = thermal_plasma(rng, 2, nx, 16, 0.05)
+    relayouts, g0, g1 = _compare_with_oracle(2, nx, dx, dt, 1, [(-1.0, 0, ix, x, u, q)], 500,
+                                             check_ix_steps=3)
+    assert abs(g1 - g0) < 1e-3 * max(1.0, abs(g0))
+
+
 def test_magnetized_3d_vay_vs_oracle():
     """Config 3 physics at 16^3: cubic, Vay, uniform B0 = 0.5 z."""
     rng = np.random.default_rng(45)
