@@ -178,20 +178,42 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
   int mcount = 0;
   int qhead = 0, qlen = 0;
   bool bad = false;
+  // The particle loads of the next iteration are issued before the current one is
+  // processed (software pipelining): a wave otherwise waits a full memory latency per
+  // iteration, and the bytes in flight per CU would cap the bandwidth well below peak.
+  uint32_t f_cell = 0, f_pid = 0;
+  float f_x = 0.f, f_y = 0.f, f_u1 = 0.f, f_u2 = 0.f, f_u3 = 0.f, f_q = 0.f;
+  auto prefetch = [&](int pf) {
+    if (pf < w1) {
+      const unsigned upf = (unsigned)pf;
+      f_cell = Pcell[upf];
+      f_x = Px[upf];
+      f_y = Py[upf];
+      f_u1 = Pux[upf];
+      f_u2 = Puy[upf];
+      f_u3 = Puz[upf];
+      f_q = Pq[upf];
+      f_pid = Ppid[upf];
+    }
+  };
+  prefetch(w0 + lane);
   for (int p0 = w0; p0 < w1; p0 += 64) {
     const int p = p0 + lane;
     const unsigned up = (unsigned)p;  // 32-bit unsigned offsets from the scalar bases
     bool act = p < w1;
     float x = 0.f, y = 0.f, u1 = 0.f, u2 = 0.f, u3 = 0.f, q = 0.f;
     int lx = 0, ly = 0;
+    const uint32_t cell = f_cell, pid = f_pid;
     if (act) {
-      const uint32_t cell = Pcell[up];
-      x = Px[up];
-      y = Py[up];
-      u1 = Pux[up];
-      u2 = Puy[up];
-      u3 = Puz[up];
-      q = Pq[up];
+      x = f_x;
+      y = f_y;
+      u1 = f_u1;
+      u2 = f_u2;
+      u3 = f_u3;
+      q = f_q;
+    }
+    prefetch(p + 64);
+    if (act) {
       int ix, iy;
       unpack2(cell, ix, iy);
       lx = ix - cx0;
@@ -301,7 +323,7 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
       Muy[slot] = u2;
       Muz[slot] = u3;
       Mq[slot] = q;
-      Mpid[slot] = Ppid[up];
+      Mpid[slot] = pid;
       Midx[slot] = p;
     }
     mcount += __popcll(mm);
