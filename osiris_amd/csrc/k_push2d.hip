@@ -37,7 +37,7 @@ typedef __attribute__((address_space(1))) unsigned long long gu64;
 // grid by fma(a, b, 1.5*2^52): one FMA and one integer add per contribution.  The j3 bracket
 // is evaluated as A S0y + B S1y with A = S0x + S1x/2, B = S1x + S0x/2 (two FP32 operations
 // per node).  |a b 2^s| < 2^51 for every contribution (bound of the scale choice).
-template <int ORDER, int JX>
+template <int ORDER, int JS>
 OSR_DEV void dep_vp(unsigned long long* __restrict__ J1, unsigned long long* __restrict__ J2,
                     unsigned long long* __restrict__ J3, const float* S0x, const float* S0y,
                     float x0, float y0, float x1, float y1, float q, float vz, int nb, float jn1,
@@ -68,7 +68,7 @@ OSR_DEV void dep_vp(unsigned long long* __restrict__ J1, unsigned long long* __r
   for (int k2 = 0; k2 < NP; k2++) {
 #pragma unroll
     for (int k1 = 0; k1 < NP; k1++) {
-      const int idx = nb + k2 * JX + k1;
+      const int idx = nb + k2 * JS + k1;
       const float t1 = fma_(B[k1], S1y[k2], A[k1] * S0y[k2]);
       if (k1 < NP - 1) atomicAdd(&J1[idx], fix_bits(__builtin_fma(a1[k1], b1[k2], M)));
       if (k2 < NP - 1) atomicAdd(&J2[idx], fix_bits(__builtin_fma(a2[k2], b2[k1], M)));
@@ -76,6 +76,16 @@ OSR_DEV void dep_vp(unsigned long long* __restrict__ J1, unsigned long long* __r
     }
   }
 }
+
+// extra row pitch of the LDS field and current tiles (bank-conflict tuning)
+#ifndef OSR_FIELD_PAD
+#define OSR_FIELD_PAD 0
+#endif
+#ifndef OSR_CUR_PAD
+#define OSR_CUR_PAD 0
+#endif
+constexpr int kFieldPad = OSR_FIELD_PAD;
+constexpr int kCurPad = OSR_CUR_PAD;
 
 // periodic wrap of a cell index at most one period outside [0, n)
 OSR_DEV int wrap1(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
@@ -86,10 +96,17 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
   using O = Ord<ORDER>;
   constexpr int NP = O::NP;
   constexpr int NT = 256, NW = 4;
-  constexpr int GX = TX + O::GLO + O::GHI, GY = TY + O::GLO + O::GHI, GN = GX * GY;
-  constexpr int JX = TX + O::JLO + O::JHI, JY = TY + O::JLO + O::JHI, JN = JX * JY;
+  // staged field tile GX x GY and current tile JX x JY, stored with row pitches GS / JS
+  constexpr int GX = TX + O::GLO + O::GHI, GY = TY + O::GLO + O::GHI;
+  constexpr int JX = TX + O::JLO + O::JHI, JY = TY + O::JLO + O::JHI;
+  constexpr int GS = GX + kFieldPad, GN = GS * GY;
+  constexpr int JS = JX + kCurPad, JN = JS * JY;
   constexpr int QC = 128;  // VP queue capacity per wave (< 64 pending + 64 appended)
-  __shared__ float sF[6 * GN];
+  // staged fields: components that share a stencil are paired so that one 64-bit LDS
+  // read serves both: (E1, B2) at (half x, full y), (E2, B1) at (full x, half y);
+  // E3 (full, full) and B3 (half, half) are single
+  __shared__ float2 sEB12[GN], sEB21[GN];
+  __shared__ float sE3[GN], sB3[GN];
   __shared__ unsigned long long sJ[3 * JN];
   __shared__ float sQ[NW][6][QC];  // x0, y0, x1, y1, q, vz
   __shared__ int sQn[NW][QC];      // LDS node index of the VP's first stencil point
@@ -107,14 +124,14 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
   {
     const gf32* __restrict__ E = (const gf32*)g.E;
     const gf32* __restrict__ B = (const gf32*)g.B;
-    for (int i = threadIdx.x; i < GN; i += NT) {
-      const int gx = wrapi(cx0 + (i % GX) - O::GLO, nx), gy = wrapi(cy0 + (i / GX) - O::GLO, ny);
+    for (int i = threadIdx.x; i < GX * GY; i += NT) {
+      const int col = i % GX, row = i / GX, li = row * GS + col;
+      const int gx = wrapi(cx0 + col - O::GLO, nx), gy = wrapi(cy0 + row - O::GLO, ny);
       const long long gi = gx + (long long)nx * gy;
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        sF[c * GN + i] = E[c * nc + gi];
-        sF[(3 + c) * GN + i] = B[c * nc + gi];
-      }
+      sEB12[li] = make_float2(E[gi], B[nc + gi]);
+      sEB21[li] = make_float2(E[nc + gi], B[gi]);
+      sE3[li] = E[2 * nc + gi];
+      sB3[li] = B[2 * nc + gi];
     }
   }
   for (int i = threadIdx.x; i < 3 * JN; i += NT) sJ[i] = 0ull;
@@ -169,7 +186,7 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
       float S0x[NP], S0y[NP];
       spline<ORDER>(x0, S0x);
       spline<ORDER>(y0, S0y);
-      dep_vp<ORDER, JX>(J1, J2, J3, S0x, S0y, x0, y0, Q[2][s], Q[3][s], Q[4][s], Q[5][s], QN[s],
+      dep_vp<ORDER, JS>(J1, J2, J3, S0x, S0y, x0, y0, Q[2][s], Q[3][s], Q[4][s], Q[5][s], QN[s],
                         jn1, jn2, jscale);
     }
   };
@@ -230,36 +247,31 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
     spline<ORDER>(y, wy);
     splineh<ORDER>(x, x < 0.f ? 1.f : 0.f, wxh);
     splineh<ORDER>(y, y < 0.f ? 1.f : 0.f, wyh);
-    const int gb = (ly + O::GLO + O::LP) * GX + lx + O::GLO + O::LP;
+    const int gb = (ly + O::GLO + O::LP) * GS + lx + O::GLO + O::LP;
     const int gbx = gb - (x < 0.f ? 1 : 0);
-    const int gby = gb - (y < 0.f ? GX : 0);
-    const int gbxy = gbx - (y < 0.f ? GX : 0);
+    const int gby = gb - (y < 0.f ? GS : 0);
+    const int gbxy = gbx - (y < 0.f ? GS : 0);
     float e1 = 0.f, e2 = 0.f, e3 = 0.f, b1 = 0.f, b2 = 0.f, b3 = 0.f;
 #pragma unroll
     for (int k2 = 0; k2 < NP; k2++) {
-      float l1 = 0.f, l2 = 0.f, l3 = 0.f;
+      float le1 = 0.f, le2 = 0.f, le3 = 0.f, lb1 = 0.f, lb2 = 0.f, lb3 = 0.f;
 #pragma unroll
       for (int k1 = 0; k1 < NP; k1++) {
-        l1 = fma_(sF[0 * GN + gbx + k2 * GX + k1], wxh[k1], l1);
-        l2 = fma_(sF[1 * GN + gby + k2 * GX + k1], wx[k1], l2);
-        l3 = fma_(sF[2 * GN + gb + k2 * GX + k1], wx[k1], l3);
+        const float2 p12 = sEB12[gbx + k2 * GS + k1];
+        const float2 p21 = sEB21[gby + k2 * GS + k1];
+        le1 = fma_(p12.x, wxh[k1], le1);
+        le2 = fma_(p21.x, wx[k1], le2);
+        le3 = fma_(sE3[gb + k2 * GS + k1], wx[k1], le3);
+        lb1 = fma_(p21.y, wx[k1], lb1);
+        lb2 = fma_(p12.y, wxh[k1], lb2);
+        lb3 = fma_(sB3[gbxy + k2 * GS + k1], wxh[k1], lb3);
       }
-      e1 = fma_(l1, wy[k2], e1);
-      e2 = fma_(l2, wyh[k2], e2);
-      e3 = fma_(l3, wy[k2], e3);
-    }
-#pragma unroll
-    for (int k2 = 0; k2 < NP; k2++) {
-      float l1 = 0.f, l2 = 0.f, l3 = 0.f;
-#pragma unroll
-      for (int k1 = 0; k1 < NP; k1++) {
-        l1 = fma_(sF[3 * GN + gby + k2 * GX + k1], wx[k1], l1);
-        l2 = fma_(sF[4 * GN + gbx + k2 * GX + k1], wxh[k1], l2);
-        l3 = fma_(sF[5 * GN + gbxy + k2 * GX + k1], wxh[k1], l3);
-      }
-      b1 = fma_(l1, wyh[k2], b1);
-      b2 = fma_(l2, wy[k2], b2);
-      b3 = fma_(l3, wyh[k2], b3);
+      e1 = fma_(le1, wy[k2], e1);
+      e2 = fma_(le2, wyh[k2], e2);
+      e3 = fma_(le3, wy[k2], e3);
+      b1 = fma_(lb1, wyh[k2], b1);
+      b2 = fma_(lb2, wy[k2], b2);
+      b3 = fma_(lb3, wyh[k2], b3);
     }
     // ---- push
     const float ke = push<PUSHER>(tem, e1, e2, e3, b1, b2, b3, u1, u2, u3);
@@ -273,9 +285,9 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
     // ---- split; VP 0 is deposited in place, VPs 1 and 2 go through the queue
     VP2 vp[3];
     const int nvp = act ? split2d_fast(x, y, x1, y1, dix, diy, q, v3, vp) : 0;
-    const int nb0 = (ly + O::LP + O::JLO) * JX + lx + O::LP + O::JLO;
+    const int nb0 = (ly + O::LP + O::JLO) * JS + lx + O::LP + O::JLO;
     if (act)
-      dep_vp<ORDER, JX>(J1, J2, J3, wx, wy, vp[0].x0, vp[0].y0, vp[0].x1, vp[0].y1, vp[0].q,
+      dep_vp<ORDER, JS>(J1, J2, J3, wx, wy, vp[0].x0, vp[0].y0, vp[0].x1, vp[0].y1, vp[0].q,
                         vp[0].vz, nb0, jn1, jn2, jscale);
     const unsigned long long below = (1ull << lane) - 1ull;
 #pragma unroll
@@ -290,7 +302,7 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
         Q[3][s] = vp[v].y1;
         Q[4][s] = vp[v].q;
         Q[5][s] = vp[v].vz;
-        QN[s] = nb0 + vp[v].dj * JX + vp[v].di;
+        QN[s] = nb0 + vp[v].dj * JS + vp[v].di;
       }
       qlen += __popcll(m);
       if (qlen >= 64) {
@@ -339,12 +351,13 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
   if (threadIdx.x == 0) d.epart[tile] = ((s_ene[0] + s_ene[1]) + s_ene[2]) + s_ene[3];
   // ---- fold the LDS current tile into the global fixed-point J (periodic wrap)
   gu64* __restrict__ GJ = (gu64*)g.J;
-  for (int i = threadIdx.x; i < JN; i += NT) {
-    const int gx = wrapi(cx0 + (i % JX) - O::JLO, nx), gy = wrapi(cy0 + (i / JX) - O::JLO, ny);
+  for (int i = threadIdx.x; i < JX * JY; i += NT) {
+    const int col = i % JX, row = i / JX, li = row * JS + col;
+    const int gx = wrapi(cx0 + col - O::JLO, nx), gy = wrapi(cy0 + row - O::JLO, ny);
     const long long gi = gx + (long long)nx * gy;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      const unsigned long long v = sJ[c * JN + i];
+      const unsigned long long v = sJ[c * JN + li];
       if (v) __hip_atomic_fetch_add(&GJ[c * nc + gi], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
