@@ -94,6 +94,89 @@ def cpu_baseline(nx, ppc_total, n_total, order, dx, dt, seconds, threads):
     }
 
 
+BYTES_PER_PARTICLE_3D = 76.0  # read ix3+x3+u3+q = 40 B, write ix3+x3+u3 = 36 B
+
+
+def cpu_baseline_c3(seconds):
+    """The FP32 oracle (a single-threaded C restatement; the reference's own SIMD kernels
+    have no Vay pusher) on a 32^3 sample of the c3 workload: push + deposit of every
+    particle, repeated for about `seconds` seconds."""
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(7)
+    nx = (32, 32, 32)
+    g = O.make_grid(3, nx, 3)
+    e = g.zeros(3, np.float32)
+    b = g.zeros(3, np.float32)
+    b[..., 2] = 0.5
+    cells = np.stack(np.meshgrid(*[np.arange(1, 33)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    ix = np.repeat(cells, 8, axis=0).astype(np.int32)
+    n = len(ix)
+    x = rng.uniform(-0.5, 0.5, (n, 3)).astype(np.float32)
+    u = rng.normal(0, 0.1, (n, 3)).astype(np.float32)
+    q = np.full(n, -1.0 / 8, np.float32)
+    j = g.zeros(3, np.float32)
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        ixc = ix.copy()  # every repetition starts from the same cells (no boundary pass)
+        O.advance_deposit(g, 3, 1, -1.0, (0.1,) * 3, 0.05, e, b, j, ixc, x, u, q, prec="f32")
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    return {"value": n * reps / el, "unit": "particle-updates/s", "cores": 1, "kind": "port",
+            "sample": (f"FP32 oracle (oracle/pic_oracle.c, one thread) push + deposit of {n} "
+                       f"particles (32^3 cells x 8 ppc, cubic, Vay, B0 = 0.5 z), {reps} "
+                       f"repetitions in {el:.1f} s")}
+
+
+def run_c3(args, oa):
+    """3D periodic magnetized thermal plasma: 256^3 cells, dx = 0.1, dt = 0.05, 8 ppc
+    (2x2x2, uniform random), u_th = 0.1, uniform B0 = 0.5 z, cubic splines, Vay pusher."""
+    nx, dx, dt = (256, 256, 256), (0.1, 0.1, 0.1), 0.05
+    tile = tuple(args.tile) if len(args.tile) == 3 else (8, 8, 4)
+    steps = min(args.steps, 100)
+    sim = oa.Simulation(3, nx, dx, dt, 3, tile=tile)
+    sid = sim.add_species(-1.0, oa.OSR_VAY)
+    sim.generate(sid, (2, 2, 2), density=1.0, uth=(0.1, 0.1, 0.1), seed=3, random_pos=True)
+    sim.fill_fields((0.0, 0.0, 0.0), (0.0, 0.0, 0.5))
+    n_total = sim.count(sid)
+    cells = int(np.prod(nx))
+    log(f"workload c3: {n_total} particles, footprint {sim.footprint / 2**30:.2f} GiB")
+    sim.run(args.warmup)
+    t0 = time.perf_counter()
+    sim.run(steps)
+    el = time.perf_counter() - t0
+    value = n_total * steps / el
+    kt = sim.time_kernels(args.timing_steps)
+    log(f"timed: {steps} steps in {el:.3f} s -> {el * 1e3 / steps:.3f} ms/step, {value:.4e} upd/s; "
+        "kernel ms/step: " + ", ".join(f"{k} {v:.4f}" for k, v in kt.items()))
+    t_launch = kt["push_deposit"] * 1e-3
+    bytes_launch = n_total * BYTES_PER_PARTICLE_3D + cells * BYTES_PER_CELL_PUSH
+    achieved = bytes_launch / t_launch / 1e9
+    relayouts = sim.relayouts
+    sim.close()
+    out = {
+        "metric": "particle-updates/s (push+deposit+Yee) 3D cubic Vay 8ppc",
+        "value": value, "unit": "particle-updates/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": el * 1e3 / steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded counter-based RNG on the device)",
+        "config": {"workload": ("3D periodic magnetized thermal plasma 256^3, dx=0.1, "
+                                "dt=0.05, 8 ppc, u_th=0.1, B0=0.5 z, cubic splines, Vay"),
+                   "particles": n_total, "cells": cells, "tile": list(tile),
+                   "relayouts": relayouts},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "push_deposit_3d<3,1,256>", "kernel_ms": t_launch * 1e3,
+                     "algorithmic_bytes": bytes_launch},
+        "kernels_ms_per_step": kt,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_c3(args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,7 +186,10 @@ def main():
     ap.add_argument("--ppc", type=int, nargs=2, default=[8, 4],
                     help="particles per cell per beam along x and y (2 beams)")
     ap.add_argument("--order", type=int, default=2)
-    ap.add_argument("--tile", type=int, nargs=2, default=[16, 16])
+    ap.add_argument("--tile", type=int, nargs="+", default=[16, 16])
+    ap.add_argument("--workload", choices=("c2", "c3"), default="c2",
+                    help="c2: the headline 2D two-stream step (default); c3: 3D periodic "
+                         "magnetized thermal plasma 256^3, 8 ppc, cubic, Vay")
     ap.add_argument("--autotune", action="store_true", default=True)
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
     ap.add_argument("--plain-steps", type=int, default=20,
@@ -139,6 +225,10 @@ def main():
                           "GB/s (read+write)": 2 * nbytes / ms / 1e6}), flush=True)
         return
 
+    if args.workload == "c3":
+        run_c3(args, oa)
+        return
+
     nx = tuple(args.nx)
     dx, dt = (0.05, 0.05), 0.035
     ppc_beam = int(np.prod(args.ppc))
@@ -151,7 +241,7 @@ def main():
                        seed=1000 + k, random_pos=True)
         return s
 
-    tile = tuple(args.tile)
+    tile = tuple(args.tile[:2])
     tune = None
     if args.autotune:
         from osiris_amd.autotune import tune_tile
