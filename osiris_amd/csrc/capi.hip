@@ -167,6 +167,7 @@ struct osr_sim {
   int64_t relayouts = 0;
   bool plain = false;
   size_t lds_push = 0;
+  int push_waves = 4;  // waves per push workgroup (SpecDesc::waves)
   Ctrl* h_ctrl = nullptr;  // pinned mirror
   int64_t desc_version = 0;  // bumped whenever the device descriptors change
 };
@@ -243,7 +244,7 @@ void make_spec_descs(const osr_sim* s, SpecDesc* v) {
     d.inc = p.inc.as<int>();
     d.epart = p.epart.as<double>();
     d.tem = (float)(0.5 * s->dt / (double)p.rqm);
-    d.waves = kWaves;
+    d.waves = s->push_waves;
   }
 }
 
@@ -298,7 +299,7 @@ int alloc_species(osr_sim* s, Species& p, const std::vector<int>& cnt) {
   int64_t total = 0;
   for (int t = 0; t < nt; t++) {
     const int c = cnt[t];
-    int64_t cp = (int64_t)c + std::max(c / 4, 64) + kNT;
+    int64_t cp = (int64_t)c + std::max(c / 4, 64) + 64 * kMaxPushWaves;
     cp = (cp + 63) / 64 * 64;
     if (total + cp > INT32_MAX) return fail(OSR_ERR_ARG, "species too large for int32 slots");
     cap[t] = (int)cp;
@@ -329,7 +330,7 @@ int alloc_species(osr_sim* s, Species& p, const std::vector<int>& cnt) {
   TRY(p.off.alloc(4 * (size_t)nt));
   TRY(p.cnt.alloc(4 * (size_t)nt));
   TRY(p.cap.alloc(4 * (size_t)nt));
-  TRY(p.mv_cnt.alloc(4 * (size_t)nt * kWaves));
+  TRY(p.mv_cnt.alloc(4 * (size_t)nt * kMaxPushWaves));
   TRY(p.mv_stride.alloc(4 * (size_t)nt));
   TRY(p.newcnt.alloc(4 * (size_t)nt));
   TRY(p.inc.alloc(4 * (size_t)nt));
@@ -344,7 +345,7 @@ int alloc_species(osr_sim* s, Species& p, const std::vector<int>& cnt) {
   HIPC(hipMemcpy(p.off.p, off.data(), 4 * nt, hipMemcpyHostToDevice));
   HIPC(hipMemcpy(p.cnt.p, cnt.data(), 4 * nt, hipMemcpyHostToDevice));
   HIPC(hipMemcpy(p.cap.p, cap.data(), 4 * nt, hipMemcpyHostToDevice));
-  HIPC(hipMemsetAsync(p.mv_cnt.p, 0, 4 * (size_t)nt * kWaves, s->stream));
+  HIPC(hipMemsetAsync(p.mv_cnt.p, 0, 4 * (size_t)nt * kMaxPushWaves, s->stream));
   HIPC(hipMemsetAsync(p.mv_stride.p, 0, 4 * (size_t)nt, s->stream));
   HIPC(hipMemsetAsync(p.epart.p, 0, 8 * (size_t)p.epart_n, s->stream));
   HIPC(hipMemsetAsync(p.inc.p, 0, 4 * (size_t)nt, s->stream));
@@ -469,6 +470,7 @@ int download(std::vector<T>& v, const DBuf& b, size_t n) {
 // into the mover buffer but the fill did not run (halted step).
 int collect(osr_sim* s, Species& p, bool pending, HostParts& hp) {
   const int nt = s->ntiles;
+  const int W = s->push_waves;  // wave count of the push that filled the mover buffers
   const size_t C = (size_t)p.cap_total;
   std::vector<int> cnt, mvc, mvs, midx;
   TRY(download(cnt, p.cnt, nt));
@@ -486,7 +488,7 @@ int collect(osr_sim* s, Species& p, bool pending, HostParts& hp) {
   std::vector<uint32_t> mcell, mpid;
   std::vector<float> mx, my, mz, mux, muy, muz, mq;
   if (pending) {
-    TRY(download(mvc, p.mv_cnt, (size_t)nt * kWaves));
+    TRY(download(mvc, p.mv_cnt, (size_t)nt * W));
     TRY(download(mvs, p.mv_stride, nt));
     TRY(download(midx, p.midx, C));
     TRY(download(mcell, p.mcell, C));
@@ -525,8 +527,8 @@ int collect(osr_sim* s, Species& p, bool pending, HostParts& hp) {
     const int base = p.h_off[t];
     hole.assign((size_t)cnt[t], 0);
     if (pending && inc[t] < 0)
-      for (int w = 0; w < kWaves; w++)
-        for (int k = 0; k < mvc[t * kWaves + w]; k++) {
+      for (int w = 0; w < W; w++)
+        for (int k = 0; k < mvc[t * W + w]; k++) {
           const int h = midx[base + w * mvs[t] + k];
           if (h >= 0 && h < cnt[t]) hole[h] = 1;
         }
@@ -535,8 +537,8 @@ int collect(osr_sim* s, Species& p, bool pending, HostParts& hp) {
   }
   if (pending)
     for (int t = 0; t < nt; t++)
-      for (int w = 0; w < kWaves; w++)
-        for (int k = 0; k < mvc[t * kWaves + w]; k++) {
+      for (int w = 0; w < W; w++)
+        for (int k = 0; k < mvc[t * W + w]; k++) {
           const size_t m = (size_t)p.h_off[t] + w * mvs[t] + k;
           if (inc[tile_of(s->g, mcell[m])] < 0)
             push(mcell, mpid, mx, my, mz, mux, muy, muz, mq, m);
@@ -604,13 +606,18 @@ push_fn pick_push2d_t(int order, int pusher, int tx, int ty) {
 int launch_push(osr_sim* s, int sid) {
   push_fn f = nullptr;
   size_t lds = s->lds_push;
+  int nt = kNT;
   if (s->dims == 2 && s->desc.kernel != 1) {
     f = pick_push2d_t(s->order, s->sp[sid]->pusher, s->g.tx, s->g.ty);
-    if (f) lds = 0;
+    if (f) {
+      lds = 0;
+      nt = push2d_t_threads(s->g.tx, s->g.ty);
+    }
   }
   if (!f) f = pick_push(s->dims, s->order, s->sp[sid]->pusher);
   if (!f) return fail(OSR_ERR_ARG, "no push kernel for dims=%d order=%d", s->dims, s->order);
-  hipLaunchKernelGGL(f, dim3(s->ntiles), dim3(kNT), lds, s->stream,
+  if (nt != 64 * s->push_waves) return fail(OSR_ERR_STATE, "push wave count mismatch");
+  hipLaunchKernelGGL(f, dim3(s->ntiles), dim3(nt), lds, s->stream,
                      s->d_spec.as<SpecDesc>(), s->d_work.as<int2>() + (size_t)sid * s->ntiles);
   HIPC(hipGetLastError());
   return OSR_OK;
@@ -822,6 +829,9 @@ int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
   }
   s->lds_push = s->plain ? 0 : push_lds_bytes(dims, s->order, tile);
   if (s->lds_push > 160 * 1024) return fail(OSR_ERR_ARG, "tile too large for LDS");
+  s->push_waves = kWaves;
+  if (!s->plain && dims == 2 && desc->kernel != 1 && pick_push2d_t(s->order, 0, g.tx, g.ty))
+    s->push_waves = push2d_t_threads(g.tx, g.ty) / 64;
   HIPC(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   const size_t nc = (size_t)g.ncell;
   TRY(s->E.alloc(3 * nc * 4));
@@ -1113,7 +1123,8 @@ int osr_sim_run(osr_sim* s, int nsteps) {
     for (int i = 0; i < nb; i++) HIPC(hipGraphLaunch(s->gexec, s->stream));
     Ctrl c;
     TRY(read_ctrl(s, c));
-    if (c.error) return fail(OSR_ERR_LAYOUT, "particle found outside its tile");
+    if (c.error & 1) return fail(OSR_ERR_LAYOUT, "particle found outside its tile");
+    if (c.error) return fail(OSR_ERR_STATE, "mover lists exceed the tile fill tables");
     const int64_t done = c.steps - s->steps;
     s->steps = c.steps;
     left -= done;
@@ -1242,7 +1253,8 @@ int osr_sim_time_kernels(osr_sim* s, int nsteps, double* ms) {
     HIPC(hipEventSynchronize(ev[4]));
     Ctrl c;
     TRY(read_ctrl(s, c));
-    if (c.error) return fail(OSR_ERR_LAYOUT, "particle found outside its tile");
+    if (c.error & 1) return fail(OSR_ERR_LAYOUT, "particle found outside its tile");
+    if (c.error) return fail(OSR_ERR_STATE, "mover lists exceed the tile fill tables");
     s->steps = c.steps;
     if (c.halt == 2) {
       TRY(relayout(s, true));

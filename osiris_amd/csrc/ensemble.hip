@@ -96,7 +96,7 @@ int64_t job_footprint(const osr_job_desc& j) {
       const int64_t wy = std::min(tl[1], d.nx[1] - ty * tl[1]);
       const int64_t wz = D == 3 ? std::min(tl[2], d.nx[2] - tz * tl[2]) : 1;
       const int64_t c = wx * wy * wz * ppc;
-      int64_t cp = c + std::max<int64_t>(c / 4, 64) + kNT;
+      int64_t cp = c + std::max<int64_t>(c / 4, 64) + 64 * kMaxPushWaves;
       slots += (cp + 63) / 64 * 64;
     }
     b += slots * slot + ntiles * (48 + 8);
@@ -271,16 +271,20 @@ int ens_build(osr_ensemble* e, EnsBatch* b) {
     if (!b->npush[p]) continue;
     push_fn f = nullptr;
     size_t lds = b->lds;
+    int nt = kNT;
     if (b->dims == 2 && b->kernel != 1) {
       f = pick_push2d_t(b->order, p, b->tile[0], b->tile[1]);
-      if (f) lds = 0;
+      if (f) {
+        lds = 0;
+        nt = push2d_t_threads(b->tile[0], b->tile[1]);
+      }
     }
     if (!f) f = pick_push(b->dims, b->order, p);
     if (!f) {
       r = fail(OSR_ERR_ARG, "no push kernel for dims=%d order=%d", b->dims, b->order);
       break;
     }
-    hipLaunchKernelGGL(f, dim3(b->npush[p]), dim3(kNT), lds, b->stream, b->spec.as<SpecDesc>(),
+    hipLaunchKernelGGL(f, dim3(b->npush[p]), dim3(nt), lds, b->stream, b->spec.as<SpecDesc>(),
                        b->wpush[p].as<int2>());
   }
   if (r == OSR_OK) {

@@ -1,6 +1,6 @@
 // k_push2d.hip -- fused 2D push + deposit with compile-time tile shape.
 //
-// One workgroup (4 waves) per (species, tile).  Compared with the generic kernel in
+// One workgroup (4 waves, 8 for tiles of 512 cells or more) per (species, tile).  Compared with the generic kernel in
 // k_tiles.hip:
 //  * TX/TY are template parameters, so every LDS stencil offset is an immediate;
 //  * particle, mover and field arrays are accessed through global-address-space
@@ -25,56 +25,14 @@ typedef __attribute__((address_space(1))) float gf32;
 typedef __attribute__((address_space(1))) uint32_t gu32;
 typedef __attribute__((address_space(1))) int gi32;
 typedef __attribute__((address_space(1))) unsigned long long gu64;
+typedef __attribute__((address_space(1))) char gch;
 
-// Charge-conserving deposit of one VP into the LDS current tile (DEP_CURRENT_2D,
-// spec/os-spec-current.f03:2691-2796); S0x/S0y are the splines of its start point and
-// nb the LDS node of its first stencil point.
-//
-// The weights are those of the reference: j1 += wl1(k1) (S0y + S1y)(k2),
-// j2 += (S0x + S1x)(k1) wl2(k2), j3 += (q vz / 3) [S0x S0y + S1x S1y + (S0x S1y + S1x S0y)/2].
-// Each product is formed exactly in FP64 from FP32 factors that already carry the fixed-
-// point scale 2^s (a power of two, so the scaling is exact) and rounded once to the int64
-// grid by fma(a, b, 1.5*2^52): one FMA and one integer add per contribution.  The j3 bracket
-// is evaluated as A S0y + B S1y with A = S0x + S1x/2, B = S1x + S0x/2 (two FP32 operations
-// per node).  |a b 2^s| < 2^51 for every contribution (bound of the scale choice).
-template <int ORDER, int JS>
-OSR_DEV void dep_vp(unsigned long long* __restrict__ J1, unsigned long long* __restrict__ J2,
-                    unsigned long long* __restrict__ J3, const float* S0x, const float* S0y,
-                    float x0, float y0, float x1, float y1, float q, float vz, int nb, float jn1,
-                    float jn2, double jscale) {
-  constexpr int NP = ORDER + 1;
-  constexpr double M = 6755399441055744.0;  // 1.5 * 2^52
-  float S1x[NP], S1y[NP], wl1[NP], wl2[NP];
-  spline<ORDER>(x1, S1x);
-  spline<ORDER>(y1, S1y);
-  wl<ORDER>(q * jn1, x0, x1, wl1);
-  wl<ORDER>(q * jn2, y0, y1, wl2);
-  double a1[NP - 1], a2[NP - 1], b1[NP], b2[NP];
-  float A[NP], B[NP];
-#pragma unroll
-  for (int k = 0; k < NP - 1; k++) {
-    a1[k] = (double)wl1[k] * jscale;
-    a2[k] = (double)wl2[k] * jscale;
-  }
-#pragma unroll
-  for (int k = 0; k < NP; k++) {
-    b1[k] = (double)(S0y[k] + S1y[k]);
-    b2[k] = (double)(S0x[k] + S1x[k]);
-    A[k] = fma_(S1x[k], 0.5f, S0x[k]);
-    B[k] = fma_(S0x[k], 0.5f, S1x[k]);
-  }
-  const double c3 = (double)((q * vz) * (1.0f / 3.0f)) * jscale;
-#pragma unroll
-  for (int k2 = 0; k2 < NP; k2++) {
-#pragma unroll
-    for (int k1 = 0; k1 < NP; k1++) {
-      const int idx = nb + k2 * JS + k1;
-      const float t1 = fma_(B[k1], S1y[k2], A[k1] * S0y[k2]);
-      if (k1 < NP - 1) atomicAdd(&J1[idx], fix_bits(__builtin_fma(a1[k1], b1[k2], M)));
-      if (k2 < NP - 1) atomicAdd(&J2[idx], fix_bits(__builtin_fma(a2[k2], b2[k1], M)));
-      atomicAdd(&J3[idx], fix_bits(__builtin_fma(c3, (double)t1, M)));
-    }
-  }
+// &b[i] through a 32-bit byte offset: base in scalar registers + one 32-bit VGPR, the
+// form the saddr global loads and stores take directly (no 64-bit address arithmetic per
+// array).  i * sizeof(*b) < 2^32 holds for every per-tile particle index.
+template <class P>
+OSR_DEV P el(P b, uint32_t i) {
+  return (P)((gch*)b + i * (uint32_t)sizeof(*b));
 }
 
 // extra row pitch of the LDS field and current tiles (bank-conflict tuning)
@@ -91,11 +49,11 @@ constexpr int kCurPad = OSR_CUR_PAD;
 OSR_DEV int wrap1(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 template <int ORDER, int PUSHER, int TX, int TY>
-__global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restrict__ descs,
+__global__ __launch_bounds__(push2d_t_threads(TX, TY), 4) void push_deposit_2d_t(const SpecDesc* __restrict__ descs,
                                                          const int2* __restrict__ work) {
   using O = Ord<ORDER>;
   constexpr int NP = O::NP;
-  constexpr int NT = 256, NW = 4;
+  constexpr int NT = push2d_t_threads(TX, TY), NW = NT / 64;
   // staged field tile GX x GY and current tile JX x JY, stored with row pitches GS / JS
   constexpr int GX = TX + O::GLO + O::GHI, GY = TY + O::GLO + O::GHI;
   constexpr int JX = TX + O::JLO + O::JHI, JY = TY + O::JLO + O::JHI;
@@ -186,8 +144,8 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
       float S0x[NP], S0y[NP];
       spline<ORDER>(x0, S0x);
       spline<ORDER>(y0, S0y);
-      dep_vp<ORDER, JS>(J1, J2, J3, S0x, S0y, x0, y0, Q[2][s], Q[3][s], Q[4][s], Q[5][s], QN[s],
-                        jn1, jn2, jscale);
+      dep_vp<ORDER>(J1, J2, J3, S0x, S0y, x0, y0, Q[2][s], Q[3][s], Q[4][s], Q[5][s], QN[s], JS,
+                     jn1, jn2, jscale);
     }
   };
 
@@ -203,14 +161,14 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
   auto prefetch = [&](int pf) {
     if (pf < w1) {
       const unsigned upf = (unsigned)pf;
-      f_cell = Pcell[upf];
-      f_x = Px[upf];
-      f_y = Py[upf];
-      f_u1 = Pux[upf];
-      f_u2 = Puy[upf];
-      f_u3 = Puz[upf];
-      f_q = Pq[upf];
-      f_pid = Ppid[upf];
+      f_cell = *el(Pcell, upf);
+      f_x = *el(Px, upf);
+      f_y = *el(Py, upf);
+      f_u1 = *el(Pux, upf);
+      f_u2 = *el(Puy, upf);
+      f_u3 = *el(Puz, upf);
+      f_q = *el(Pq, upf);
+      f_pid = *el(Ppid, upf);
     }
   };
   prefetch(w0 + lane);
@@ -287,8 +245,8 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
     const int nvp = act ? split2d_fast(x, y, x1, y1, dix, diy, q, v3, vp) : 0;
     const int nb0 = (ly + O::LP + O::JLO) * JS + lx + O::LP + O::JLO;
     if (act)
-      dep_vp<ORDER, JS>(J1, J2, J3, wx, wy, vp[0].x0, vp[0].y0, vp[0].x1, vp[0].y1, vp[0].q,
-                        vp[0].vz, nb0, jn1, jn2, jscale);
+      dep_vp<ORDER>(J1, J2, J3, wx, wy, vp[0].x0, vp[0].y0, vp[0].x1, vp[0].y1, vp[0].q,
+                    vp[0].vz, nb0, JS, jn1, jn2, jscale);
     const unsigned long long below = (1ull << lane) - 1ull;
 #pragma unroll
     for (int v = 1; v < 3; v++) {
@@ -317,26 +275,26 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
     const bool stay = (nix / TX + g.ntx * (niy / TY)) == tile;
     const float xs = x1 - tx, ys = y1 - ty;
     if (act && stay) {
-      Wcell[up] = ncell;
-      Wx[up] = xs;
-      Wy[up] = ys;
-      Wux[up] = u1;
-      Wuy[up] = u2;
-      Wuz[up] = u3;
+      *el(Wcell, up) = ncell;
+      *el(Wx, up) = xs;
+      *el(Wy, up) = ys;
+      *el(Wux, up) = u1;
+      *el(Wuy, up) = u2;
+      *el(Wuz, up) = u3;
     }
     const bool mv = act && !stay;
     const unsigned long long mm = __ballot(mv);
     if (mv) {
       const unsigned slot = (unsigned)(w0 + mcount + __popcll(mm & below));
-      Mcell[slot] = ncell;
-      Mx[slot] = xs;
-      My[slot] = ys;
-      Mux[slot] = u1;
-      Muy[slot] = u2;
-      Muz[slot] = u3;
-      Mq[slot] = q;
-      Mpid[slot] = pid;
-      Midx[slot] = p;
+      *el(Mcell, slot) = ncell;
+      *el(Mx, slot) = xs;
+      *el(My, slot) = ys;
+      *el(Mux, slot) = u1;
+      *el(Muy, slot) = u2;
+      *el(Muz, slot) = u3;
+      *el(Mq, slot) = q;
+      *el(Mpid, slot) = pid;
+      *el(Midx, slot) = p;
     }
     mcount += __popcll(mm);
   }
@@ -348,7 +306,12 @@ __global__ __launch_bounds__(256) void push_deposit_2d_t(const SpecDesc* __restr
   ene = wave_sum(ene);
   if (lane == 0) s_ene[wave] = ene;
   __syncthreads();
-  if (threadIdx.x == 0) d.epart[tile] = ((s_ene[0] + s_ene[1]) + s_ene[2]) + s_ene[3];
+  if (threadIdx.x == 0) {
+    double t = s_ene[0];
+#pragma unroll
+    for (int w = 1; w < NW; w++) t += s_ene[w];
+    d.epart[tile] = t;
+  }
   // ---- fold the LDS current tile into the global fixed-point J (periodic wrap)
   gu64* __restrict__ GJ = (gu64*)g.J;
   for (int i = threadIdx.x; i < JX * JY; i += NT) {

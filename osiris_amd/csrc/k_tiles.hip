@@ -189,30 +189,12 @@ __global__ __launch_bounds__(NT) void push_deposit_2d(const SpecDesc* __restrict
     const int nvp = act ? split2d(x, y, x1, y1, dix, diy, q, v3, vp) : 0;
     for (int v = 0; v < nvp; v++) {
       const VP2 V = pick(vp, v);
-      float S0x[NP], S1x[NP], S0y[NP], S1y[NP], wl1[NP], wl2[NP];
+      float S0x[NP], S0y[NP];
       spline<ORDER>(V.x0, S0x);
-      spline<ORDER>(V.x1, S1x);
       spline<ORDER>(V.y0, S0y);
-      spline<ORDER>(V.y1, S1y);
-      wl<ORDER>(V.q * jn1, V.x0, V.x1, wl1);
-      wl<ORDER>(V.q * jn2, V.y0, V.y1, wl2);
-      const float qvz = (V.q * V.vz) * (1.0f / 3.0f);
       const int jx0 = lx + V.di + O::LP + O::JLO, jy0 = ly + V.dj + O::LP + O::JLO;
-#pragma unroll
-      for (int k2 = 0; k2 < NP; k2++) {
-        const float wp1 = S0y[k2] + S1y[k2];
-#pragma unroll
-        for (int k1 = 0; k1 < NP; k1++) {
-          const int idx = (jy0 + k2) * JX + jx0 + k1;
-          const float wp2 = S0x[k1] + S1x[k1];
-          float t1 = fma_(S1y[k2], S1x[k1], S0x[k1] * S0y[k2]);
-          const float t2 = fma_(S0y[k2], S1x[k1], S0x[k1] * S1y[k2]);
-          t1 = fma_(t2, 0.5f, t1);
-          if (k1 < NP - 1) atomicAdd(&sJ1[idx], (unsigned long long)to_fix(wl1[k1] * wp1, jscale));
-          if (k2 < NP - 1) atomicAdd(&sJ2[idx], (unsigned long long)to_fix(wl2[k2] * wp2, jscale));
-          atomicAdd(&sJ3[idx], (unsigned long long)to_fix(qvz * t1, jscale));
-        }
-      }
+      dep_vp<ORDER>(sJ1, sJ2, sJ3, S0x, S0y, V.x0, V.y0, V.x1, V.y1, V.q, V.vz, jy0 * JX + jx0,
+                    JX, jn1, jn2, jscale);
     }
     // ---- store: stay in place, or leave through the mover buffer
     int nix = wrapi(lx + cx0 + dix, nx), niy = wrapi(ly + cy0 + diy, ny);
@@ -581,6 +563,21 @@ OSR_DEV void copy_particle(const SpecDesc& d, int src, int dst, int dims) {
 // the tiles that did fill plus the mover buffers.
 // NT threads per tile; INC_MAX incoming movers and TAIL_BITS holes left in the tail are the
 // most one fill can handle (LDS lists); beyond them the tile flags an overflow.
+// Segment of item i in a concatenation of n lists with start offsets ofs[0..n]
+// (ofs[0] = 0, non-decreasing, 0 <= i < ofs[n]): the last l with ofs[l] <= i, which skips
+// empty lists.  Binary search: the fill walks up to 8 x 8 (2D) or 26 x 4 (3D) lists.
+OSR_DEV int seg_find(const int* ofs, int n, int i) {
+  int lo = 0, hi = n;  // ofs[lo] <= i < ofs[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (ofs[mid] <= i)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
 template <int NT, int INC_MAX, int TAIL_BITS>
 __global__ __launch_bounds__(NT) void tile_fill(const SpecDesc* __restrict__ descs,
                                                 const int2* __restrict__ work) {
@@ -610,15 +607,29 @@ __global__ __launch_bounds__(NT) void tile_fill(const SpecDesc* __restrict__ des
   }
   __syncthreads();
   const int L = s_nnb * NW;
-  for (int l = threadIdx.x; l < L; l += NT) {
-    const int nt = s_nb[l / NW], w = l % NW;
-    s_lofs[l + 1] = d.mv_cnt[nt * NW + w];
-    s_lbase[l] = d.off[nt] + w * d.mv_stride[nt];
+  if (L > MAXL) {  // more mover lists than the LDS tables hold: never with 26 x 4 or 8 x 8
+    if (threadIdx.x == 0) atomicOr(&g.ctrl->error, 2);
+    return;
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    s_lofs[0] = 0;
-    for (int l = 0; l < L; l++) s_lofs[l + 1] += s_lofs[l];
+  if (threadIdx.x < 64) {  // wave 0: list bases and the prefix of the list lengths
+    const int lane = threadIdx.x;
+    int carry = 0;
+    for (int c0 = 0; c0 < L; c0 += 64) {
+      const int l = c0 + lane;
+      int v = 0;
+      if (l < L) {
+        const int nt = s_nb[l / NW], w = l % NW;
+        v = d.mv_cnt[nt * NW + w];
+        s_lbase[l] = d.off[nt] + w * d.mv_stride[nt];
+      }
+      for (int o = 1; o < 64; o <<= 1) {  // inclusive scan over the wave
+        const int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+      }
+      if (l < L) s_lofs[l + 1] = carry + v;
+      carry += __shfl(v, 63, 64);
+    }
+    if (lane == 0) s_lofs[0] = 0;
   }
   __syncthreads();
   const int T = s_lofs[L];
@@ -630,8 +641,7 @@ __global__ __launch_bounds__(NT) void tile_fill(const SpecDesc* __restrict__ des
     int src = -1;
     bool f = false;
     if (i < T) {
-      int l = 0;
-      while (i >= s_lofs[l + 1]) l++;
+      const int l = seg_find(s_lofs, L, i);
       src = s_lbase[l] + (i - s_lofs[l]);
       f = tile_of(g, d.mcell[src]) == tile;
     }
@@ -649,8 +659,7 @@ __global__ __launch_bounds__(NT) void tile_fill(const SpecDesc* __restrict__ des
     return;
   }
   auto hole = [&](int k) -> int {
-    int w = 0;
-    while (k >= s_hpre[w + 1]) w++;
+    const int w = seg_find(s_hpre, NW, k);
     return d.midx[base + w * stride + (k - s_hpre[w])];
   };
   // number of holes below nn (holes are ascending)

@@ -206,6 +206,58 @@ OSR_DEV unsigned long long fix_bits(double t) {
   return (unsigned long long)(__double_as_longlong(t) - 0x4338000000000000LL);
 }
 
+// Charge-conserving deposit of one VP into the LDS current tile (DEP_CURRENT_2D,
+// spec/os-spec-current.f03:2691-2796); S0x/S0y are the splines of its start point and
+// nb the LDS node of its first stencil point, js the row pitch of the tile (a compile-time
+// constant after inlining in the fixed-tile kernels).
+//
+// The weights are those of the reference: j1 += wl1(k1) (S0y + S1y)(k2),
+// j2 += (S0x + S1x)(k1) wl2(k2), j3 += (q vz / 3) [S0x S0y + S1x S1y + (S0x S1y + S1x S0y)/2].
+// Each product is formed exactly in FP64 from FP32 factors that already carry the fixed-
+// point scale 2^s (a power of two, so the scaling is exact) and rounded once to the int64
+// grid by fma(a, b, 1.5*2^52): one FMA and one integer add per contribution.  The j3 bracket
+// is evaluated as A S0y + B S1y with A = S0x + S1x/2, B = S1x + S0x/2 (two FP32 operations
+// per node).  |a b 2^s| < 2^51 for every contribution (bound of the scale choice).
+template <int ORDER>
+OSR_DEV void dep_vp(unsigned long long* __restrict__ J1, unsigned long long* __restrict__ J2,
+                    unsigned long long* __restrict__ J3, const float* S0x, const float* S0y,
+                    float x0, float y0, float x1, float y1, float q, float vz, int nb, int js,
+                    float jn1, float jn2, double jscale) {
+  constexpr int NP = ORDER + 1;
+  constexpr double M = 6755399441055744.0;  // 1.5 * 2^52
+  float S1x[NP], S1y[NP], wl1[NP], wl2[NP];
+  spline<ORDER>(x1, S1x);
+  spline<ORDER>(y1, S1y);
+  wl<ORDER>(q * jn1, x0, x1, wl1);
+  wl<ORDER>(q * jn2, y0, y1, wl2);
+  double a1[NP - 1], a2[NP - 1], b1[NP], b2[NP];
+  float A[NP], B[NP];
+#pragma unroll
+  for (int k = 0; k < NP - 1; k++) {
+    a1[k] = (double)wl1[k] * jscale;
+    a2[k] = (double)wl2[k] * jscale;
+  }
+#pragma unroll
+  for (int k = 0; k < NP; k++) {
+    b1[k] = (double)(S0y[k] + S1y[k]);
+    b2[k] = (double)(S0x[k] + S1x[k]);
+    A[k] = fma_(S1x[k], 0.5f, S0x[k]);
+    B[k] = fma_(S0x[k], 0.5f, S1x[k]);
+  }
+  const double c3 = (double)((q * vz) * (1.0f / 3.0f)) * jscale;
+#pragma unroll
+  for (int k2 = 0; k2 < NP; k2++) {
+#pragma unroll
+    for (int k1 = 0; k1 < NP; k1++) {
+      const int idx = nb + k2 * js + k1;
+      const float t1 = fma_(B[k1], S1y[k2], A[k1] * S0y[k2]);
+      if (k1 < NP - 1) atomicAdd(&J1[idx], fix_bits(__builtin_fma(a1[k1], b1[k2], M)));
+      if (k2 < NP - 1) atomicAdd(&J2[idx], fix_bits(__builtin_fma(a2[k2], b2[k1], M)));
+      atomicAdd(&J3[idx], fix_bits(__builtin_fma(c3, (double)t1, M)));
+    }
+  }
+}
+
 // ------------------------------------------------------------------------------------
 // Virtual particles.  split_2d: spec/os-spec-current.f03:659-888, vsplit2D
 // simd/split-vec.h:105-222 (scalar IEEE arithmetic, as compiled by the reference).

@@ -18,6 +18,13 @@
 namespace osr {
 
 constexpr int kMaxSpecies = 8;
+// Push workgroups have 4 waves, or 8 for the 2D kernels with compile-time tiles of at least
+// 512 cells: their field and current tiles in LDS are then shared by twice as many waves,
+// so two workgroups (16 waves) fit in a CU's LDS where three 4-wave ones (12 waves) did.
+constexpr int kMaxPushWaves = 8;
+__host__ __device__ constexpr int push2d_t_threads(int tx, int ty) {
+  return tx * ty >= 512 ? 512 : 256;
+}
 
 // Control block, one per simulation (device memory).
 struct Ctrl {

@@ -267,3 +267,28 @@ def test_charge_density_vs_oracle(dims, order):
         sim.set_particles(sid, ix, x, u, q)
         rg = sim.charge_density()
     assert np.abs(rg - ro).max() <= 1e-12 * np.abs(ro).max()
+
+
+def test_tile_shape_and_kernel_independence():
+    """Particles and fields do not depend on the tile shape, the workgroup size (4 waves
+    for 8x8 and 16x16 tiles, 8 waves for 32x16) or the kernel variant (compile-time tile vs
+    generic): the current is summed in integers, so every path gives the same bits.  The
+    FP64 energy sums are per tile and per wave, so they agree to round-off only."""
+    nx, dx, dt = (128, 96), (0.05, 0.05), 0.035
+    out = []
+    for tile, kernel in (((32, 16), 0), ((16, 16), 0), ((8, 8), 0), ((32, 16), 1)):
+        print(f"case tile={tile} kernel={kernel}", flush=True)
+        with oa.Simulation(2, nx, dx, dt, 2, tile=tile, kernel=kernel) as sim:
+            for k, ufl in enumerate((0.2, -0.2)):
+                sid = sim.add_species(-1.0)
+                sim.generate(sid, (4, 4), density=0.5, ufl=(ufl, 0, 0), uth=(0.05,) * 3,
+                             seed=91 + k)
+            sim.run(30)
+            out.append((sim.get_fields(), sim.get_particles(0), sim.get_particles(1),
+                        sim.history()))
+    f0, a0, b0, h0 = out[0]
+    for f, a, b, h in out[1:]:
+        for x, y in zip(f0 + a0 + b0, f + a + b):
+            assert np.array_equal(x, y)
+        assert np.array_equal(h0[0], h[0])  # field energies: FP64 sums over the same grid
+        assert np.allclose(h0[1], h[1], rtol=1e-12, atol=0)
