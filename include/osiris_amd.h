@@ -269,6 +269,54 @@ int osr_ens_job_release(osr_ensemble *ens, int64_t job_id);
 int osr_ens_usage(osr_ensemble *ens, int64_t *bytes);
 
 /* ---------------------------------------------------------------------------------
+ * 5. Current smoothing (vdf/os-vdf-smooth.f90), the per-step option of the current's
+ * nl_smooth section (current/os-current-define.f03:298-312, applied after the current's
+ * boundary update, os-simulation.f03:650-657).  Filter types use the reference's codes
+ * (os-vdf-smooth.f90:22-31).
+ * ------------------------------------------------------------------------------- */
+enum {
+  OSR_SMOOTH_NONE = 0,
+  OSR_SMOOTH_CUSTOM = 1,
+  OSR_SMOOTH_BINOMIAL = 2,
+  OSR_SMOOTH_COMPENSATED = 3,
+  OSR_SMOOTH_5PASS = 4,
+  OSR_SMOOTH_TRI6 = 5,
+  OSR_SMOOTH_TRI8 = 6,
+  OSR_SMOOTH_LASERKDX = 10,
+  OSR_SMOOTH_DIGITAL = 11
+};
+#define OSR_SMOOTH_ORDER_MAX 15 /* p_order_max */
+#define OSR_SMOOTH_NCOEF 31     /* 2 * OSR_SMOOTH_ORDER_MAX + 1 */
+
+/* The nl_smooth namelist (os-vdf-smooth.f90:214-216), one entry per direction. */
+typedef struct {
+  int type[3];                 /* type(:) as an OSR_SMOOTH_* code */
+  int order[3];                /* order(:) */
+  double swfj[3][15][3];       /* swfj(1:3, pass, dir) of the custom filter */
+  double laserkdx_omega0;      /* laserkdx_omega0 */
+  int laserkdx_dir;            /* laserkdx_dir (1-based) */
+  int reserved;
+  double digital_A[3];         /* digital_A(:) */
+  double digital_fc[3];        /* digital_fc(:) */
+} osr_smooth_params;
+
+/* read_nml_smooth + setup_smooth (os-vdf-smooth.f90:183-470) for the single-precision
+ * build: order[3] receives the filter order per direction (0 = none), coef[3 * 31] the
+ * normalised FP32 coefficients, coef[31 d + 15 + k] = s(k) for k = -order[d]..order[d].
+ * gamma is the simulation's boost factor (1 when not boosted), used by laserkdx only.
+ * Host only; returns OSR_ERR_ARG where the reference aborts. */
+int osr_smooth_setup(int dims, const osr_smooth_params *p, const double *dx, double gamma,
+                     int *order, float *coef);
+/* smooth_vdf (os-vdf-smooth.f90:515-540) on a host array f(fdim, ld[0], ld[1][, ld[2]])
+ * with guard cells: direction by direction, every line of the array, cells at least
+ * order[d] away from both ends replaced by the filtered value; in place. */
+int osr_smooth_vdf(int dims, const int *ld, int fdim, const int *order, const float *coef,
+                   float *f);
+/* Resident simulation: smooth the current every step before the field advance
+ * (t_current%smooth); all orders 0 switches it off. */
+int osr_sim_set_smooth(osr_sim *sim, const int *order, const float *coef);
+
+/* ---------------------------------------------------------------------------------
  * Measurement helper (no counterpart in the reference): streams `bytes` bytes through a
  * plain device copy kernel `reps` times and returns the mean kernel time in *ms; used
  * to calibrate hardware traffic counters (FETCH_SIZE / WRITE_SIZE) and as a bandwidth

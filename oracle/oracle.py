@@ -62,7 +62,8 @@ def _lib(prec: str):
         L.orc_deposit_rho.argtypes = [P, C.c_int, P, P, P, C.c_int, P]
         L.orc_div.argtypes = [P, P, P, P]
         L.orc_step.argtypes = [P, C.c_int, P, C.c_double, P, P, P, C.c_int, P, P, P, P, P, P,
-                               P, P, P]
+                               P, P, P, P, P]
+        L.orc_smooth.argtypes = [P, C.c_int, P, P, P]
         L._real = np.float64 if prec == "f64" else np.float32
         _LIBS[prec] = L
     return _LIBS[prec]
@@ -106,6 +107,8 @@ def _aligned(shape, dtype, fill=0, align=64):
 
 
 def _p(a: np.ndarray):
+    if a is None:
+        return None
     assert a.flags["C_CONTIGUOUS"], "arrays must be C-contiguous"
     return a.ctypes.data_as(C.c_void_p)
 
@@ -237,6 +240,16 @@ def particles_periodic(grid: Grid, ix, prec="f64"):
     L.orc_particles_periodic(C.byref(g), _p(ix), len(ix))
 
 
+def smooth(grid: Grid, f, order, coef, prec="f64"):
+    """smooth_vdf (vdf/os-vdf-smooth.f90:515-540) in place on a grid array, guard cells
+    included; order per direction, coef (3, 31) centred at 15."""
+    L = _lib(prec)
+    g = grid.c()
+    n = np.array(list(order) + [0] * (3 - len(order)), np.int32)
+    c = np.ascontiguousarray(np.asarray(coef, np.float64).reshape(3, 31), L._real)
+    L.orc_smooth(C.byref(g), f.shape[-1], _p(f), _p(n), _p(c))
+
+
 def yee_advance(grid: Grid, dx, dt, e, b, j, prec="f64"):
     L = _lib(prec)
     g = grid.c()
@@ -345,10 +358,23 @@ class Sim:
     order: int
     prec: str = "f64"
     species: list = field(default_factory=list)
+    # current smoothing: per-direction filter orders and coefficients (3, 31), centre 15,
+    # from oracle.smooth.setup; None = no smoothing
+    smooth_order: tuple = None
+    smooth_coef: np.ndarray = None
 
     def __post_init__(self):
         self.grid = make_grid(self.dims, self.nx, self.order)
         r = _lib(self.prec)._real
+        self.sm_order = self.sm_coef = None
+        if self.smooth_order is not None:
+            # current/os-current-define.f03:196-205: at least n lower and n+1 upper guards
+            n = list(self.smooth_order) + [0] * (3 - len(self.smooth_order))
+            g = self.grid
+            self.grid = Grid(self.dims, g.nx, tuple(max(g.gcl[d], n[d]) for d in range(self.dims)),
+                             tuple(max(g.gcu[d], n[d] + 1) for d in range(self.dims)))
+            self.sm_order = np.array(n[:3], np.int32)
+            self.sm_coef = np.ascontiguousarray(np.asarray(self.smooth_coef, np.float64).reshape(3, 31), r)
         self.e = self.grid.zeros(3, r)
         self.b = self.grid.zeros(3, r)
         self.j = self.grid.zeros(3, r)
@@ -388,7 +414,7 @@ class Sim:
             pe = np.zeros(ns)
             L.orc_step(C.byref(g), self.order, _p(dxa), self.dt, _p(self.e), _p(self.b),
                        _p(self.j), ns, _p(pus), _p(rqm), ixs, xs, us, qs, _p(npa), _p(fe),
-                       _p(pe))
+                       _p(pe), _p(self.sm_order), _p(self.sm_coef))
             self.fld_hist.append(fe)
             self.par_hist.append(pe)
 

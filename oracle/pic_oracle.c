@@ -1016,10 +1016,48 @@ void orc_div(const orc_grid *g, const double *dx, const real *f, real *div) {
  * Species are passed as arrays of pointers.  fld_ene[6] and par_ene[nspec] receive the
  * step's HIST rows (normalised as report_energy_* does).
  * ------------------------------------------------------------------------------------ */
+/* Current smoothing, one call = smooth_vdf (vdf/os-vdf-smooth.f90:515-540) on a grid array
+ * with nc components: filter_1d/2d/3d (:545-1100).  Each direction d whose order
+ * n = order[d] is positive is filtered in turn (x, then y, then z) along every line of the
+ * whole array, guard cells included (ext_lb..ext_ub in the other directions).  Along the
+ * line, cells int_lb..int_ub = ext_lb+n .. ext_ub-n are replaced by
+ *   s(-n) o(i-n) + s(-n+1) o(i-n+1) + ... + s(n) o(i+n)
+ * summed left to right in the working precision, o being the values of the line before
+ * the pass (the reference's rolling buffer o(:,-n:n) holds exactly those); the other cells
+ * keep their values.  coef[31 d + 15 + k] = s(k) of direction d (scoef(:, d) centred). */
+void orc_smooth(const orc_grid *g, int nc, real *f, const int *order, const real *coef) {
+  const int D = g->dims;
+  long ld[3] = {1, 1, 1};
+  for (int d = 0; d < D; d++) ld[d] = ld_(g, d);
+  const long stride[3] = {nc, nc * ld[0], nc * ld[0] * ld[1]};
+  long lmax = ld[0];
+  if (ld[1] > lmax) lmax = ld[1];
+  if (ld[2] > lmax) lmax = ld[2];
+  real *line = (real *)malloc(sizeof(real) * (size_t)lmax);
+  for (int d = 0; d < D; d++) {
+    const int n = order[d];
+    if (n <= 0) continue;
+    const real *sc = coef + 31 * d + 15;
+    const int a = d == 0 ? 1 : 0, b = d == 2 ? 1 : 2;  /* the two other directions */
+    for (long jb = 0; jb < ld[b]; jb++)
+      for (long ja = 0; ja < ld[a]; ja++)
+        for (int c = 0; c < nc; c++) {
+          real *base = f + c + ja * stride[a] + jb * stride[b];  /* ld[b] = 1 beyond D */
+          for (long i = 0; i < ld[d]; i++) line[i] = base[i * stride[d]];
+          for (long i = n; i <= ld[d] - 1 - n; i++) {
+            real acc = line[i - n] * sc[-n];
+            for (int k = -n + 1; k <= n; k++) acc = acc + line[i + k] * sc[k];
+            base[i * stride[d]] = acc;
+          }
+        }
+  }
+  free(line);
+}
+
 void orc_step(const orc_grid *g, int order, const double *dx, double dt, real *e, real *b,
               real *j, int nspec, const int *pusher, const real *rqm, int *const *ix,
               real *const *x, real *const *u, real *const *q, const int *np, double *fld_ene,
-              double *par_ene) {
+              double *par_ene, const int *sm_order, const real *sm_coef) {
   double dvol = dx[0] * dx[1] * (g->dims > 2 ? dx[2] : 1.0);
   orc_update_periodic(g, e, 3, 0);
   orc_update_periodic(g, b, 3, 0);
@@ -1032,5 +1070,7 @@ void orc_step(const orc_grid *g, int order, const double *dx, double dt, real *e
   }
   if (fld_ene) orc_field_energy(g, e, b, dvol, fld_ene);
   orc_update_periodic(g, j, 3, 1);
+  /* os-simulation.f03:650-657: the folded current is smoothed before the field advance */
+  if (sm_order) orc_smooth(g, 3, j, sm_order, sm_coef);
   orc_yee_advance(g, dx, dt, e, b, j);
 }

@@ -62,6 +62,9 @@ SIGNATURES = {
     "osr_ens_job_history": (I, [P, I64, I64, I64, P, P]),
     "osr_ens_job_release": (I, [P, I64]),
     "osr_ens_usage": (I, [P, P]),
+    "osr_smooth_setup": (I, [I, P, P, D, P, P]),
+    "osr_smooth_vdf": (I, [I, P, I, P, P, P]),
+    "osr_sim_set_smooth": (I, [P, P, P]),
     "osr_copy_probe": (I, [I64, I, P]),
 }
 
@@ -70,6 +73,18 @@ class SimDesc(C.Structure):
     _fields_ = [("dims", C.c_int), ("nx", C.c_int * 3), ("dx", C.c_double * 3),
                 ("dt", C.c_double), ("order", C.c_int), ("tile", C.c_int * 3),
                 ("path", C.c_int), ("kernel", C.c_int), ("reserved", C.c_int * 2)]
+
+
+class SmoothParams(C.Structure):
+    """osr_smooth_params: the nl_smooth namelist, one entry per direction."""
+    _fields_ = [("type", C.c_int * 3), ("order", C.c_int * 3),
+                ("swfj", ((C.c_double * 3) * 15) * 3), ("laserkdx_omega0", C.c_double),
+                ("laserkdx_dir", C.c_int), ("reserved", C.c_int),
+                ("digital_A", C.c_double * 3), ("digital_fc", C.c_double * 3)]
+
+
+OSR_SMOOTH = {"none": 0, "custom": 1, "binomial": 2, "compensated": 3, "5pass": 4, "tri6": 5,
+              "tri8": 6, "laserkdx": 10, "digital": 11}
 
 
 class EnsDesc(C.Structure):

@@ -17,6 +17,7 @@
 #include "../../include/osiris_amd.h"
 #include "k_dropin.hip"
 #include "k_field.hip"
+#include "k_smooth.hip"
 #include "k_loader.hip"
 #include "k_tiles.hip"
 #include "k_push2d.hip"
@@ -168,6 +169,9 @@ struct osr_sim {
   bool plain = false;
   size_t lds_push = 0;
   int push_waves = 4;  // waves per push workgroup (SpecDesc::waves)
+  int sm_order[3] = {0, 0, 0};  // current smoothing per direction (0: none)
+  float sm_coef[3][31] = {};
+  DBuf sm_buf[2];
   Ctrl* h_ctrl = nullptr;  // pinned mirror
   int64_t desc_version = 0;  // bumped whenever the device descriptors change
 };
@@ -271,6 +275,23 @@ void make_field_desc(const osr_sim* s, FieldDesc& f) {
   f.dtif = (float)s->dt;
   f.Jf = s->plain ? s->J.as<float>() : nullptr;
   f.jf_zero = s->plain ? 1 : 0;
+  f.sm_np = 0;
+  f.jsrc_clear = 0;
+  for (int d = 0; d < s->dims; d++) {
+    if (s->sm_order[d] <= 0) continue;
+    const int k = f.sm_np++;
+    f.sm_dir[k] = d;
+    f.sm_n[k] = s->sm_order[d];
+    for (int c = 0; c < 31; c++) f.sm_coef[k][c] = s->sm_coef[d][c];
+  }
+  if (f.sm_np > 0) {
+    f.sm_src = s->plain ? 1 : 0;
+    f.sm_buf[0] = s->sm_buf[0].as<float>();
+    f.sm_buf[1] = s->sm_buf[1].as<float>();
+    f.Jf = f.sm_buf[(f.sm_np - 1) & 1];
+    f.jf_zero = 0;
+    f.jsrc_clear = s->plain ? 2 : 1;
+  }
 }
 
 int upload_descs(osr_sim* s) {
@@ -688,15 +709,28 @@ int launch_plain(osr_sim* s, int sid) {
   return OSR_OK;
 }
 
+// current smoothing passes between the deposit and the field advance
+int launch_smooth(osr_sim* s) {
+  int np = 0;
+  for (int d = 0; d < s->dims; d++) np += s->sm_order[d] > 0;
+  for (int p = 0; p < np; p++)
+    hipLaunchKernelGGL(smooth_pass, dim3(s->nblk), dim3(256), 0, s->stream,
+                       s->d_fd.as<FieldDesc>(), s->d_ywork.as<int2>(), p);
+  HIPC(hipGetLastError());
+  return OSR_OK;
+}
+
 int enqueue_step(osr_sim* s) {
   if (s->plain) {
     for (size_t i = 0; i < s->sp.size(); i++) TRY(launch_plain(s, (int)i));
+    TRY(launch_smooth(s));
     TRY(launch_yee(s));
     TRY(launch_finalize(s));
     return OSR_OK;
   }
   for (size_t i = 0; i < s->sp.size(); i++) TRY(launch_push(s, (int)i));
   if (!s->sp.empty()) TRY(launch_fill(s));
+  TRY(launch_smooth(s));
   TRY(launch_yee(s));
   TRY(launch_finalize(s));
   return OSR_OK;
@@ -1544,6 +1578,272 @@ int osr_yee_advance(int dims, const int* nx, const int* ld, const int* gcl, cons
           b[r + m] = pb[m * nc + c];
         }
       }
+  return OSR_OK;
+}
+
+}  // extern "C"
+
+// ---- current smoothing -----------------------------------------------------------------
+namespace {
+
+// besseli0_rdoub, os-math.f90:450-472 (Abramowitz & Stegun 9.8.1/9.8.2).  Literals without
+// a kind suffix in the reference are default (single precision) reals: (double)<float>.
+double besseli0(double x) {
+  const double ax = std::fabs(x);
+  if (ax < (double)3.75f) {
+    const double y = (x / (double)3.75f) * (x / (double)3.75f);
+    return (double)1.0f +
+           y * ((double)3.5156229f +
+                y * ((double)3.0899424f +
+                     y * ((double)1.2067492f + y * ((double)0.2659732f + y * (0.360768e-1 + y * 0.45813e-2)))));
+  }
+  const double y = (double)3.75f / ax;
+  return (std::exp(ax) / std::sqrt(ax)) *
+         ((double)0.39894228f +
+          y * (0.1328592e-1 +
+               y * (0.225319e-2 +
+                    y * (-0.157565e-2 +
+                         y * (0.916281e-2 +
+                              y * (-0.2057706e-1 +
+                                   y * (0.2635537e-1 + y * (-0.1647633e-1 + y * 0.392377e-2))))))));
+}
+
+// combine_kernel3, os-vdf-smooth.f90:142-172: n 3-point kernels k3[j] = kern3(1:3, j+1) into
+// one kernel of 2 nmax + 1 points (double); the first entry of a 3-point kernel weighs the
+// upper neighbour, the third the lower one.
+void combine_kernel3(const double (*k3)[3], int n, int nmax, double* out) {
+  for (int i = 0; i < 2 * nmax + 1; i++) out[i] = 0.0;
+  out[nmax] = 1.0;
+  for (int k = 1; k <= n; k++) {
+    const double a = k3[k - 1][0], b = k3[k - 1][1], c = k3[k - 1][2];
+    double f1 = 0.0;
+    for (int i = nmax - (k - 1); i <= nmax + k; i++) {
+      out[i - 1] = c * out[i] + out[i - 1];
+      const double f2 = a * out[i];
+      out[i] = b * out[i] + f1;
+      f1 = f2;
+    }
+  }
+}
+
+constexpr double kPi = 3.14159265358979323846264;  // os-math.f90:13
+
+// digital_filter, os-vdf-smooth.f90:110-137: Kaiser-windowed sinc of 2n+1 points
+void digital_filter(double fc, double A, int n, double* out) {
+  double alpha;
+  if (A <= 21.0)
+    alpha = 0.0;
+  else if (A >= 50.0)
+    alpha = 0.1102 * (A - 8.7);
+  else
+    alpha = 0.5842 * std::pow(A - 21.0, 0.4) + 0.07886 * (A - 21.0);
+  for (int i = 1; i <= n; i++) {
+    const float r = (float)i / (float)n;
+    out[n + i] = besseli0(alpha * std::sqrt(1.0 - (double)(r * r))) / besseli0(alpha) *
+                 std::sin(i * kPi * fc) / (i * kPi);
+  }
+  out[n] = fc;
+  for (int i = 1; i <= n; i++) out[n - i] = out[n + i];
+}
+
+
+int check_smooth_args(int dims, const int* order, const float* coef) {
+  if ((dims != 2 && dims != 3) || !order || !coef) return fail(OSR_ERR_ARG, "bad argument");
+  for (int d = 0; d < dims; d++)
+    if (order[d] < 0 || order[d] > OSR_SMOOTH_ORDER_MAX)
+      return fail(OSR_ERR_ARG, "smoothing order must be between 0 and %d", OSR_SMOOTH_ORDER_MAX);
+  return OSR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// read_nml_smooth (os-vdf-smooth.f90:183-361) + setup_smooth (:366-470) in the reference's
+// single-precision build: scoef is an FP32 array, the normalisation sums those values in
+// double and rounds each quotient to FP32.
+int osr_smooth_setup(int dims, const osr_smooth_params* p, const double* dx, double gamma,
+                     int* order_out, float* coef_out) {
+  if ((dims < 1 || dims > 3) || !p || !dx || !order_out || !coef_out)
+    return fail(OSR_ERR_ARG, "bad argument");
+  constexpr int OM = OSR_SMOOTH_ORDER_MAX;
+  int type[3] = {OSR_SMOOTH_NONE, OSR_SMOOTH_NONE, OSR_SMOOTH_NONE};
+  int order[3] = {0, 0, 0};
+  const bool lkdx = p->type[0] == OSR_SMOOTH_LASERKDX;
+  if (lkdx) {
+    if (p->laserkdx_dir < 1 || p->laserkdx_dir > dims)
+      return fail(OSR_ERR_ARG, "laserkdx_dir must be between 1 and %d", dims);
+    for (int i = 0; i < dims; i++) {
+      type[i] = OSR_SMOOTH_LASERKDX;
+      order[i] = i + 1 == p->laserkdx_dir ? 2 : 0;
+    }
+    if (p->laserkdx_omega0 <= 0.0) return fail(OSR_ERR_ARG, "laserkdx_omega0 must be > 0");
+  }
+  for (int i = 0; i < dims; i++) {
+    switch (p->type[i]) {
+      case OSR_SMOOTH_NONE:
+        type[i] = OSR_SMOOTH_NONE;
+        order[i] = 0;
+        break;
+      case OSR_SMOOTH_CUSTOM:
+      case OSR_SMOOTH_BINOMIAL:
+        type[i] = p->type[i];
+        if (p->order[i] < 1 || p->order[i] > OM)
+          return fail(OSR_ERR_ARG, "smoothing order must be between 1 and %d", OM);
+        order[i] = p->order[i];
+        break;
+      case OSR_SMOOTH_COMPENSATED:
+        type[i] = p->type[i];
+        order[i] = p->order[i] < 2 ? 2 : p->order[i];
+        if (order[i] > OM) return fail(OSR_ERR_ARG, "smoothing order above %d", OM);
+        break;
+      case OSR_SMOOTH_5PASS:
+        type[i] = p->type[i];
+        order[i] = 5;
+        break;
+      case OSR_SMOOTH_TRI6:
+        type[i] = p->type[i];
+        order[i] = 3;
+        break;
+      case OSR_SMOOTH_TRI8:
+        type[i] = p->type[i];
+        order[i] = 4;
+        break;
+      case OSR_SMOOTH_LASERKDX:
+        break;
+      case OSR_SMOOTH_DIGITAL:
+        type[i] = p->type[i];
+        if (p->order[i] < 1 || p->order[i] > OM)
+          return fail(OSR_ERR_ARG, "digital smoothing order must be between 1 and %d", OM);
+        order[i] = p->order[i];
+        break;
+      default:
+        return fail(OSR_ERR_ARG, "invalid smoothing type %d", p->type[i]);
+    }
+  }
+  int mo = 0;
+  for (int i = 0; i < dims; i++) mo = std::max(mo, order[i]);
+  float sc[3][2 * OM + 1] = {};
+  double k[2 * OM + 1];
+  double k3[OM][3];
+  if (mo > 0) {
+    for (int i = 0; i < dims; i++)
+      if (type[i] == OSR_SMOOTH_CUSTOM) {
+        combine_kernel3(p->swfj[i], order[i], mo, k);
+        for (int j = 0; j < 2 * mo + 1; j++) sc[i][j] = (float)k[j];
+      }
+    if (type[0] == OSR_SMOOTH_LASERKDX) {
+      const int d = p->laserkdx_dir - 1;
+      const double lk = p->laserkdx_omega0 * dx[d] / (gamma + std::sqrt(gamma * gamma - 1));
+      const double kl[2][3] = {{1.0, 2.0, 1.0}, {-1.0, 4 + 2 * std::cos(lk), -1.0}};
+      combine_kernel3(kl, 2, mo, k);
+      for (int j = 0; j < 2 * mo + 1; j++) sc[d][j] = (float)k[j];
+    } else {
+      for (int i = 0; i < dims; i++) {
+        const int n = order[i];
+        const int c = mo;
+        switch (type[i]) {
+          case OSR_SMOOTH_BINOMIAL:
+            for (int j = 0; j < n; j++) k3[j][0] = 1.0, k3[j][1] = 2.0, k3[j][2] = 1.0;
+            combine_kernel3(k3, n, mo, k);
+            for (int j = 0; j < 2 * mo + 1; j++) sc[i][j] = (float)k[j];
+            break;
+          case OSR_SMOOTH_COMPENSATED: {
+            for (int j = 0; j < n - 1; j++) k3[j][0] = 1.0, k3[j][1] = 2.0, k3[j][2] = 1.0;
+            // (4.0 + 2.0 * (order - 1)) / (order - 1) in default (single) precision
+            const double comp = (double)((4.0f + 2.0f * (float)(n - 1)) / (float)(n - 1));
+            k3[n - 1][0] = -1.0, k3[n - 1][1] = comp, k3[n - 1][2] = -1.0;
+            combine_kernel3(k3, n, mo, k);
+            for (int j = 0; j < 2 * mo + 1; j++) sc[i][j] = (float)k[j];
+            break;
+          }
+          case OSR_SMOOTH_5PASS:
+            for (int j = 0; j < 4; j++) k3[j][0] = 1.0, k3[j][1] = 2.0, k3[j][2] = 1.0;
+            k3[4][0] = -5.0, k3[4][1] = 15.0, k3[4][2] = -5.0;
+            combine_kernel3(k3, 5, mo, k);
+            for (int j = 0; j < 2 * mo + 1; j++) sc[i][j] = (float)k[j];
+            break;
+          case OSR_SMOOTH_TRI6: {
+            const float t[7] = {0.015625f, -0.09375f, 0.234375f, 0.6875f, 0.234375f, -0.09375f, 0.015625f};
+            for (int j = 0; j < 7; j++) sc[i][c - 3 + j] = t[j];
+            break;
+          }
+          case OSR_SMOOTH_TRI8: {
+            const float t[9] = {-0.00390625f, 0.03125f, -0.109375f, 0.21875f, 0.7265625f,
+                                0.21875f, -0.109375f, 0.03125f, -0.00390625f};
+            for (int j = 0; j < 9; j++) sc[i][c - 4 + j] = t[j];
+            break;
+          }
+          case OSR_SMOOTH_DIGITAL:
+            digital_filter(p->digital_fc[i], p->digital_A[i], n, k);
+            for (int j = 0; j < 2 * n + 1; j++) sc[i][c - n + j] = (float)k[j];
+            break;
+          default:
+            break;
+        }
+      }
+    }
+    for (int i = 0; i < dims; i++) {
+      const int n = order[i];
+      if (n <= 0) continue;
+      double total = 0.0;
+      for (int j = mo - n; j <= mo + n; j++) total = total + (double)sc[i][j];
+      for (int j = mo - n; j <= mo + n; j++) sc[i][j] = (float)((double)sc[i][j] / total);
+    }
+  }
+  for (int i = 0; i < 3; i++) {
+    order_out[i] = i < dims ? order[i] : 0;
+    for (int j = 0; j < OSR_SMOOTH_NCOEF; j++) coef_out[i * OSR_SMOOTH_NCOEF + j] = 0.f;
+    if (i < dims && order[i] > 0)
+      for (int j = -order[i]; j <= order[i]; j++) coef_out[i * OSR_SMOOTH_NCOEF + OM + j] = sc[i][mo + j];
+  }
+  return OSR_OK;
+}
+
+int osr_smooth_vdf(int dims, const int* ld, int fdim, const int* order, const float* coef,
+                   float* f) {
+  TRY(check_smooth_args(dims, order, coef));
+  if (!ld || !f || fdim < 1) return fail(OSR_ERR_ARG, "bad argument");
+  const int l[3] = {ld[0], ld[1], dims == 3 ? ld[2] : 1};
+  for (int d = 0; d < 3; d++)
+    if (l[d] < 1) return fail(OSR_ERR_ARG, "bad extents");
+  const size_t n = (size_t)fdim * l[0] * l[1] * l[2];
+  DBuf a, b;
+  TRY(a.alloc(4 * n));
+  TRY(b.alloc(4 * n));
+  HIPC(hipMemcpy(a.p, f, 4 * n, hipMemcpyHostToDevice));
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  for (int d = 0; d < dims; d++) {
+    if (order[d] <= 0) continue;
+    SmoothCoef c;
+    for (int j = 0; j < 31; j++) c.s[j] = coef[d * OSR_SMOOTH_NCOEF + j];
+    hipLaunchKernelGGL(smooth_vdf_pass, dim3(nb), dim3(256), 0, 0, a.as<float>(), b.as<float>(),
+                       fdim, l[0], l[1], l[2], d, order[d], c);
+    HIPC(hipGetLastError());
+    a.swap(b);
+  }
+  HIPC(hipDeviceSynchronize());
+  HIPC(hipMemcpy(f, a.p, 4 * n, hipMemcpyDeviceToHost));
+  return OSR_OK;
+}
+
+int osr_sim_set_smooth(osr_sim* s, const int* order, const float* coef) {
+  TRY(check_sim(s));
+  TRY(check_smooth_args(s->dims, order, coef));
+  HIPC(hipStreamSynchronize(s->stream));
+  bool any = false;
+  for (int d = 0; d < 3; d++) {
+    s->sm_order[d] = d < s->dims ? order[d] : 0;
+    any = any || s->sm_order[d] > 0;
+    for (int j = 0; j < 31; j++) s->sm_coef[d][j] = d < s->dims ? coef[d * OSR_SMOOTH_NCOEF + j] : 0.f;
+  }
+  const size_t bytes = 12 * (size_t)s->g.ncell;
+  if (any && !s->sm_buf[0].p) {
+    TRY(s->sm_buf[0].alloc(bytes));
+    TRY(s->sm_buf[1].alloc(bytes));
+  }
+  TRY(upload_descs(s));
+  s->graph_ok = false;
   return OSR_OK;
 }
 

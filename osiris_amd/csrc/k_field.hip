@@ -35,6 +35,16 @@ struct FieldDesc {
   // when set, the E step reads a float current from here instead of the fixed-point J
   float* Jf;
   int jf_zero;  // clear Jf after reading it (the plain path accumulates into it)
+  // current smoothing: passes (one per filtered direction, x before y before z) from the
+  // source current into the ping-pong buffers; the last output is Jf, and the E step then
+  // clears the source (1: fixed-point J, 2: the plain path's float J)
+  int sm_np;
+  int sm_dir[3];
+  int sm_n[3];
+  int sm_src;  // 0 fixed-point J, 1 float J (plain path)
+  int jsrc_clear;
+  float sm_coef[3][31];  // per pass, s(k) at [15 + k]
+  float* sm_buf[2];
 };
 
 OSR_DEV int wrp(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
@@ -137,6 +147,16 @@ __global__ __launch_bounds__(256) void yee_dedt(const FieldDesc* __restrict__ fd
       fd.Jf[idx] = 0.f;
       fd.Jf[nc + idx] = 0.f;
       fd.Jf[2 * nc + idx] = 0.f;
+    }
+    if (fd.jsrc_clear == 1) {
+      J[idx] = 0ull;
+      J[nc + idx] = 0ull;
+      J[2 * nc + idx] = 0ull;
+    } else if (fd.jsrc_clear == 2) {
+      float* Jp = (float*)J;
+      Jp[idx] = 0.f;
+      Jp[nc + idx] = 0.f;
+      Jp[2 * nc + idx] = 0.f;
     }
   } else {
     const double jinv = g.jinv;

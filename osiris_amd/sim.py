@@ -14,7 +14,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import OSR_BORIS, OSR_VAY, SimDesc, check, lib
+from ._lib import OSR_BORIS, OSR_SMOOTH, OSR_VAY, SimDesc, SmoothParams, check, lib
 
 
 def _p(a):
@@ -70,6 +70,43 @@ def yee_advance(dims, nx, gcl, dx, dt, e, b, jay):
     check(lib().osr_yee_advance(dims, _p(np.array(nx, np.int32)), _p(ld),
                                 _p(np.array(gcl, np.int32)), _p(np.array(dx, np.float64)),
                                 float(dt), _p(e), _p(b), _p(jay)))
+
+
+def smooth_setup(dims, type, order=(0, 0, 0), dx=(1.0, 1.0, 1.0), gamma=1.0, swfj=None,
+                 laserkdx_omega0=-1.0, laserkdx_dir=1, digital_A=None, digital_fc=None):
+    """The nl_smooth section of the current (vdf/os-vdf-smooth.f90 read_nml_smooth +
+    setup_smooth): ``type`` per direction as names ("binomial", "compensated", ...) or
+    OSR_SMOOTH codes.  Returns (order[3], coef float32 (3, 31), s(k) at [:, 15 + k])."""
+    p = SmoothParams()
+    for d in range(3):
+        t = type[d] if d < len(type) else "none"
+        p.type[d] = OSR_SMOOTH[t] if isinstance(t, str) else int(t)
+        p.order[d] = int(order[d]) if d < len(order) else 0
+        for k in range(15):
+            for m in range(3):
+                p.swfj[d][k][m] = (float(swfj[d][k][m]) if swfj is not None and d < len(swfj)
+                                   else 1.0)
+        huge = -1.7976931348623157e308  # the namelist default -huge(1.0d0)
+        p.digital_A[d] = (float(digital_A[d]) if digital_A is not None and d < len(digital_A)
+                          else huge)
+        p.digital_fc[d] = (float(digital_fc[d]) if digital_fc is not None and d < len(digital_fc)
+                           else huge)
+    p.laserkdx_omega0 = float(laserkdx_omega0)
+    p.laserkdx_dir = int(laserkdx_dir)
+    n = np.zeros(3, np.int32)
+    coef = np.zeros((3, 31), np.float32)
+    dxa = np.array(list(dx) + [1.0] * (3 - len(dx)), np.float64)
+    check(lib().osr_smooth_setup(dims, C.byref(p), _p(dxa), float(gamma), _p(n), _p(coef)))
+    return [int(v) for v in n], coef
+
+
+def smooth_vdf(dims, f, order, coef):
+    """smooth_vdf on a guarded reference-layout float32 array (in place)."""
+    assert f.dtype == np.float32 and f.flags["C_CONTIGUOUS"]
+    ld = np.array(list(reversed(f.shape[:-1])), np.int32)
+    n = np.array(list(order) + [0] * (3 - len(order)), np.int32)
+    check(lib().osr_smooth_vdf(dims, _p(ld), f.shape[-1], _p(n),
+                               _p(np.ascontiguousarray(coef, np.float32)), _p(f)))
 
 
 class Simulation:
@@ -165,6 +202,11 @@ class Simulation:
     def fill_fields(self, e0=None, b0=None):
         check(lib().osr_fields_fill(self._h, _p(None if e0 is None else np.array(e0, np.float64)),
                                     _p(None if b0 is None else np.array(b0, np.float64))))
+
+    def set_smooth(self, order, coef):
+        """Smooth the current every step (order per direction, coef from smooth_setup)."""
+        n = np.array(list(order) + [0] * (3 - len(order)), np.int32)
+        check(lib().osr_sim_set_smooth(self._h, _p(n), _p(np.ascontiguousarray(coef, np.float32))))
 
     # ---- stepping and diagnostics
     def run(self, nsteps):
