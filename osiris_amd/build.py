@@ -55,9 +55,11 @@ def build_fortran(verbose: bool = False) -> str | None:
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    drv = os.path.join(fdir, "push_example.f03")
-    if os.path.exists(drv):
-        exe = os.path.join(out_dir, "push_example")
+    for name in ("push_example", "smooth_setup_example"):
+        drv = os.path.join(fdir, name + ".f03")
+        if not os.path.exists(drv):
+            continue
+        exe = os.path.join(out_dir, name)
         cmd = [fc, "-O2", "-I", mod_dir, "-o", exe, drv, "-L" + out_dir, "-losiris_amd_f",
                "-losiris_amd", "-Wl,-rpath,$ORIGIN"]
         if verbose:

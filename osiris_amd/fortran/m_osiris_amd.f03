@@ -34,6 +34,25 @@ module m_osiris_amd
     integer(c_int) :: reserved(2) = 0
   end type osr_sim_desc
 
+  ! current smoothing (vdf/os-vdf-smooth.f90): the reference's filter type codes and the
+  ! nl_smooth namelist; swfj has the reference's shape swfj(3, p_order_max, p_x_dim)
+  integer(c_int), parameter, public :: OSR_SMOOTH_NONE = 0, OSR_SMOOTH_CUSTOM = 1, &
+                                       OSR_SMOOTH_BINOMIAL = 2, OSR_SMOOTH_COMPENSATED = 3, &
+                                       OSR_SMOOTH_5PASS = 4, OSR_SMOOTH_TRI6 = 5, &
+                                       OSR_SMOOTH_TRI8 = 6, OSR_SMOOTH_LASERKDX = 10, &
+                                       OSR_SMOOTH_DIGITAL = 11
+  integer(c_int), parameter, public :: OSR_SMOOTH_ORDER_MAX = 15, OSR_SMOOTH_NCOEF = 31
+  type, bind(c), public :: osr_smooth_params
+    integer(c_int) :: type(3) = OSR_SMOOTH_NONE
+    integer(c_int) :: order(3) = 0
+    real(c_double) :: swfj(3, 15, 3) = 1.0_c_double
+    real(c_double) :: laserkdx_omega0 = -1.0_c_double
+    integer(c_int) :: laserkdx_dir = 1
+    integer(c_int) :: reserved = 0
+    real(c_double) :: digital_A(3) = -huge(1.0_c_double)
+    real(c_double) :: digital_fc(3) = -huge(1.0_c_double)
+  end type osr_smooth_params
+
   public :: osr_version, osr_device_count, osr_error_message
   public :: osr_advance_deposit, osr_yee_advance
   public :: hip_vadvance_deposit_2d_s1, hip_vadvance_deposit_2d_s2, hip_vadvance_deposit_2d_s3
@@ -42,6 +61,7 @@ module m_osiris_amd
             osr_species_count, osr_species_get, osr_species_generate, osr_fields_set, &
             osr_fields_get, osr_fields_fill, osr_sim_run, osr_sim_steps, osr_sim_history, &
             osr_sim_gauss, osr_sim_relayouts, osr_sim_time_kernels, osr_sim_footprint
+  public :: osr_smooth_setup, osr_smooth_vdf, osr_sim_set_smooth
 
   interface
     function osr_last_error() bind(c, name="osr_last_error")
@@ -256,6 +276,33 @@ module m_osiris_amd
       type(c_ptr), value :: sim
       integer(c_int64_t), intent(out) :: bytes
       integer(c_int) :: osr_sim_footprint
+    end function
+
+    ! ---- 5. current smoothing
+    function osr_smooth_setup(dims, p, dx, gamma, order, coef) bind(c, name="osr_smooth_setup")
+      import :: c_int, c_float, c_double, osr_smooth_params
+      integer(c_int), value :: dims
+      type(osr_smooth_params), intent(in) :: p
+      real(c_double), intent(in) :: dx(*)
+      real(c_double), value :: gamma
+      integer(c_int), intent(out) :: order(3)
+      real(c_float), intent(out) :: coef(31, 3)
+      integer(c_int) :: osr_smooth_setup
+    end function
+    function osr_smooth_vdf(dims, ld, fdim, order, coef, f) bind(c, name="osr_smooth_vdf")
+      import :: c_int, c_float
+      integer(c_int), value :: dims, fdim
+      integer(c_int), intent(in) :: ld(*), order(*)
+      real(c_float), intent(in) :: coef(31, 3)
+      real(c_float), intent(inout) :: f(*)
+      integer(c_int) :: osr_smooth_vdf
+    end function
+    function osr_sim_set_smooth(sim, order, coef) bind(c, name="osr_sim_set_smooth")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: sim
+      integer(c_int), intent(in) :: order(3)
+      real(c_float), intent(in) :: coef(31, 3)
+      integer(c_int) :: osr_sim_set_smooth
     end function
   end interface
 

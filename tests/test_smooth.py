@@ -177,3 +177,21 @@ def test_setup_rejects_what_the_reference_aborts_on():
         oa.smooth_setup(2, (7, "none"), (1, 0))
     with pytest.raises(ValueError):
         S.coefficients(2, (S.BINOMIAL, S.NONE), (0, 0), (1.0, 1.0))
+
+
+def test_fortran_binding_setup():
+    """The bind(c) osr_smooth_params of m_osiris_amd matches the C layout: the Fortran
+    example's coefficients equal the closed forms (host code only, no GPU)."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "osiris_amd", "lib", "smooth_setup_example")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran example not built (no Fortran compiler)")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.split()
+    vals = [float(v) for v in lines if "E" in v]
+    assert lines[:4] == ["dir", "1", "order", "2"]
+    assert np.allclose(vals[:5], np.array([-1, 4, 10, 4, -1]) / 16.0, rtol=0, atol=0)
+    assert np.allclose(vals[5:], np.array([1, 6, 15, 20, 15, 6, 1]) / 64.0, rtol=0, atol=0)
