@@ -189,7 +189,9 @@ size_t push_lds_bytes(int dims, int order, const int* t) {
     gn *= (size_t)(t[d] + glo + ghi);
     jn *= (size_t)(t[d] + jlo + jhi);
   }
-  return 3 * jn * sizeof(unsigned long long) + 6 * gn * sizeof(float);
+  size_t q = 0;  // 3D: per-wave VP queues of kVpQueue3 entries of 8 words
+  if (dims == 3) q = (size_t)kWaves * kVpQueue3 * 8 * sizeof(float);
+  return 3 * jn * sizeof(unsigned long long) + 6 * gn * sizeof(float) + q;
 }
 
 // fixed-point shift of the current: the largest one for which neither the sum of all

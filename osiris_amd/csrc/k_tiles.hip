@@ -272,6 +272,10 @@ __global__ __launch_bounds__(NT) void push_deposit_3d(const SpecDesc* __restrict
   const int GN = GX * GY * GZ, JN = JX * JY * JZ;
   unsigned long long* sJ = (unsigned long long*)smem;
   float* sF = (float*)(smem + sizeof(unsigned long long) * 3 * JN);
+  // per-wave queues of the virtual particles 1..3 (x0, y0, z0, x1, y1, z1, q; node)
+  constexpr int QC = kVpQueue3;
+  float* sQall = sF + 6 * GN;
+  int* sQnall = (int*)(sQall + (NT / 64) * 7 * QC);
   const int ttx = tile % g.ntx, tty = (tile / g.ntx) % g.nty, ttz = tile / (g.ntx * g.nty);
   const int cx0 = ttx * TX, cy0 = tty * TY, cz0 = ttz * TZ;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -303,6 +307,23 @@ __global__ __launch_bounds__(NT) void push_deposit_3d(const SpecDesc* __restrict
   const float jn1 = g.jnorm[0], jn2 = g.jnorm[1], jn3 = g.jnorm[2];
   const double jscale = g.jscale;
   const int GXY = GX * GY, JXY = JX * JY;
+
+  float* const Q = sQall + wave * 7 * QC;
+  int* const QN = sQnall + wave * QC;
+  int qhead = 0, qlen = 0;
+  // deposit of the queued VPs in slots [head, head + cnt) (cnt <= 64, one per lane)
+  auto deposit_queue = [&](int head, int cnt) {
+    if (lane < cnt) {
+      const int sl = (head + lane) & (QC - 1);
+      const float x0 = Q[sl], y0 = Q[QC + sl], z0 = Q[2 * QC + sl];
+      float S0x[NP], S0y[NP], S0z[NP];
+      spline<ORDER>(x0, S0x);
+      spline<ORDER>(y0, S0y);
+      spline<ORDER>(z0, S0z);
+      dep_vp3<ORDER>(sJ, JN, S0x, S0y, S0z, x0, y0, z0, Q[3 * QC + sl], Q[4 * QC + sl],
+                     Q[5 * QC + sl], Q[6 * QC + sl], QN[sl], JX, JXY, jn1, jn2, jn3, jscale);
+    }
+  };
 
   double ene = 0.0;
   int mcount = 0;
@@ -400,44 +421,36 @@ __global__ __launch_bounds__(NT) void push_deposit_3d(const SpecDesc* __restrict
                 z1 = fma_(u3 * rg, dtdx3, z);
     const float tx = trim(x1), ty = trim(y1), tz = trim(z1);
     const int dix = (int)tx, diy = (int)ty, diz = (int)tz;
+    // split; VP 0 is deposited in place with the gather's splines (it starts at the old
+    // position), VPs 1..3 of the lanes whose trajectory crosses faces go through the queue
     VP3 vp[4];
     const int nvp = act ? split3d(x, y, z, x1, y1, z1, dix, diy, diz, q, vp) : 0;
-    for (int v = 0; v < nvp; v++) {
-      const VP3 V = pick(vp, v);
-      float S0x[NP], S1x[NP], S0y[NP], S1y[NP], S0z[NP], S1z[NP], wl1[NP], wl2[NP], wl3[NP];
-      spline<ORDER>(V.x0, S0x);
-      spline<ORDER>(V.x1, S1x);
-      spline<ORDER>(V.y0, S0y);
-      spline<ORDER>(V.y1, S1y);
-      spline<ORDER>(V.z0, S0z);
-      spline<ORDER>(V.z1, S1z);
-      wl<ORDER>(V.q * jn1, V.x0, V.x1, wl1);
-      wl<ORDER>(V.q * jn2, V.y0, V.y1, wl2);
-      wl<ORDER>(V.q * jn3, V.z0, V.z1, wl3);
-      const int jx0 = lx + V.di + O::LP + O::JLO, jy0 = ly + V.dj + O::LP + O::JLO,
-                jz0 = lz + V.dk + O::LP + O::JLO;
+    const int nb0 = (lz + O::LP + O::JLO) * JXY + (ly + O::LP + O::JLO) * JX + lx + O::LP + O::JLO;
+    if (act)
+      dep_vp3<ORDER>(sJ, JN, wx, wy, wz, vp[0].x0, vp[0].y0, vp[0].z0, vp[0].x1, vp[0].y1,
+                     vp[0].z1, vp[0].q, nb0, JX, JXY, jn1, jn2, jn3, jscale);
+    const unsigned long long below = (1ull << lane) - 1ull;
 #pragma unroll
-      for (int k3 = 0; k3 < NP; k3++) {
-#pragma unroll
-        for (int k2 = 0; k2 < NP; k2++) {
-          // wp1(k2,k3), os-spec-current.f03:2876-2882
-          const float wp1 = fma_(0.5f, fma_(S1y[k2], S0z[k3], S0y[k2] * S1z[k3]),
-                                 fma_(S1y[k2], S1z[k3], S0y[k2] * S0z[k3]));
-#pragma unroll
-          for (int k1 = 0; k1 < NP; k1++) {
-            const int idx = (jz0 + k3) * JXY + (jy0 + k2) * JX + jx0 + k1;
-            const float wp2 = fma_(0.5f, fma_(S1x[k1], S0z[k3], S0x[k1] * S1z[k3]),
-                                   fma_(S1x[k1], S1z[k3], S0x[k1] * S0z[k3]));
-            const float wp3 = fma_(0.5f, fma_(S1x[k1], S0y[k2], S0x[k1] * S1y[k2]),
-                                   fma_(S1x[k1], S1y[k2], S0x[k1] * S0y[k2]));
-            if (k1 < NP - 1)
-              atomicAdd(&sJ[idx], (unsigned long long)to_fix(wl1[k1] * wp1, jscale));
-            if (k2 < NP - 1)
-              atomicAdd(&sJ[JN + idx], (unsigned long long)to_fix(wl2[k2] * wp2, jscale));
-            if (k3 < NP - 1)
-              atomicAdd(&sJ[2 * JN + idx], (unsigned long long)to_fix(wl3[k3] * wp3, jscale));
-          }
-        }
+    for (int v = 1; v < 4; v++) {
+      const unsigned long long m = __ballot(v < nvp);
+      if (m == 0ull) break;
+      if (v < nvp) {
+        const VP3 V = pick(vp, v);
+        const int sl = (qhead + qlen + __popcll(m & below)) & (QC - 1);
+        Q[0 * QC + sl] = V.x0;
+        Q[1 * QC + sl] = V.y0;
+        Q[2 * QC + sl] = V.z0;
+        Q[3 * QC + sl] = V.x1;
+        Q[4 * QC + sl] = V.y1;
+        Q[5 * QC + sl] = V.z1;
+        Q[6 * QC + sl] = V.q;
+        QN[sl] = nb0 + V.dk * JXY + V.dj * JX + V.di;
+      }
+      qlen += __popcll(m);
+      if (qlen >= 64) {
+        deposit_queue(qhead, 64);
+        qhead = (qhead + 64) & (QC - 1);
+        qlen -= 64;
       }
     }
     const int nix = wrapi(lx + cx0 + dix, nx), niy = wrapi(ly + cy0 + diy, ny),
@@ -473,6 +486,7 @@ __global__ __launch_bounds__(NT) void push_deposit_3d(const SpecDesc* __restrict
     }
     mcount += __popcll(mm);
   }
+  if (qlen > 0) deposit_queue(qhead, qlen);
   if (bad) atomicOr(&g.ctrl->error, 1);
   if (lane == 0) d.mv_cnt[tile * NW + wave] = mcount;
   if (threadIdx.x == 0) d.mv_stride[tile] = per_wave;

@@ -405,6 +405,61 @@ OSR_DEV VP3 pick(const VP3* vp, int v) {
 #undef OSR_SEL3
 #undef OSR_SEL4
 
+// Charge-conserving deposit of one 3D VP into the LDS current tile (DEP_CURRENT_3D,
+// spec/os-spec-current.f03:2800-2905): j1 += wl1(k1) wp1(k2,k3), j2 += wl2(k2) wp2(k1,k3),
+// j3 += wl3(k3) wp3(k1,k2), each wp = S0 S0 + S1 S1 + (S0 S1 + S1 S0)/2 over the two other
+// directions, evaluated in FP32 in the reference's operation order (:2876-2890).  As in
+// 2D, every contribution is the exact FP64 product of wl 2^s and wp, rounded once to the
+// int64 grid: one FMA and one integer add per node.  S0x/S0y/S0z are the splines of the
+// VP's start point, nb its first stencil node, jy/jz the row and plane pitches of the
+// tile, JN the size of one component plane.
+template <int ORDER>
+OSR_DEV void dep_vp3(unsigned long long* __restrict__ J, int JN, const float* S0x,
+                     const float* S0y, const float* S0z, float x0, float y0, float z0, float x1,
+                     float y1, float z1, float q, int nb, int jy, int jz, float jn1, float jn2,
+                     float jn3, double jscale) {
+  constexpr int NP = ORDER + 1;
+  constexpr double M = 6755399441055744.0;  // 1.5 * 2^52
+  float S1x[NP], S1y[NP], S1z[NP], wl1[NP], wl2[NP], wl3[NP];
+  spline<ORDER>(x1, S1x);
+  spline<ORDER>(y1, S1y);
+  spline<ORDER>(z1, S1z);
+  wl<ORDER>(q * jn1, x0, x1, wl1);
+  wl<ORDER>(q * jn2, y0, y1, wl2);
+  wl<ORDER>(q * jn3, z0, z1, wl3);
+  double a1[NP - 1], a2[NP - 1], a3[NP - 1];
+#pragma unroll
+  for (int k = 0; k < NP - 1; k++) {
+    a1[k] = (double)wl1[k] * jscale;
+    a2[k] = (double)wl2[k] * jscale;
+    a3[k] = (double)wl3[k] * jscale;
+  }
+  // wp(a, b) over the pair of directions other than the current's
+  auto wp = [](float s0a, float s1a, float s0b, float s1b) {
+    return fma_(0.5f, fma_(s1a, s0b, s0a * s1b), fma_(s1a, s1b, s0a * s0b));
+  };
+#pragma unroll
+  for (int k3 = 0; k3 < NP; k3++) {
+#pragma unroll
+    for (int k2 = 0; k2 < NP; k2++) {
+      const double b1 = (double)wp(S0y[k2], S1y[k2], S0z[k3], S1z[k3]);
+#pragma unroll
+      for (int k1 = 0; k1 < NP; k1++) {
+        const int idx = nb + k3 * jz + k2 * jy + k1;
+        if (k1 < NP - 1) atomicAdd(&J[idx], fix_bits(__builtin_fma(a1[k1], b1, M)));
+        if (k2 < NP - 1) {
+          const double b2 = (double)wp(S0x[k1], S1x[k1], S0z[k3], S1z[k3]);
+          atomicAdd(&J[JN + idx], fix_bits(__builtin_fma(a2[k2], b2, M)));
+        }
+        if (k3 < NP - 1) {
+          const double b3 = (double)wp(S0x[k1], S1x[k1], S0y[k2], S1y[k2]);
+          atomicAdd(&J[2 * JN + idx], fix_bits(__builtin_fma(a3[k3], b3, M)));
+        }
+      }
+    }
+  }
+}
+
 // split_3d: spec/os-spec-current.f03:895-1601 (vsplit3D, simd/split-vec.h:231-).
 OSR_DEV VP3 mk3(float x0, float y0, float z0, float x1, float y1, float z1, float q, int i, int j,
                 int k) {

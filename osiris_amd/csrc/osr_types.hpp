@@ -22,6 +22,9 @@ constexpr int kMaxSpecies = 8;
 // 512 cells: their field and current tiles in LDS are then shared by twice as many waves,
 // so two workgroups (16 waves) fit in a CU's LDS where three 4-wave ones (12 waves) did.
 constexpr int kMaxPushWaves = 8;
+// capacity of the per-wave virtual-particle queue of the 3D push (pending < 64 plus up to
+// 64 appended per VP level)
+constexpr int kVpQueue3 = 128;
 __host__ __device__ constexpr int push2d_t_threads(int tx, int ty) {
   return tx * ty >= 512 ? 512 : 256;
 }

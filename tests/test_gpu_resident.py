@@ -292,3 +292,25 @@ def test_tile_shape_and_kernel_independence():
             assert np.array_equal(x, y)
         assert np.array_equal(h0[0], h[0])  # field energies: FP64 sums over the same grid
         assert np.allclose(h0[1], h[1], rtol=1e-12, atol=0)
+
+
+def test_tile_shape_independence_3d():
+    """3D (cubic, Vay, magnetized): particles and fields do not depend on the tile shape;
+    the current is summed in integers whatever order the queued virtual particles are
+    deposited in."""
+    nx, dx, dt = (24, 16, 16), (0.1, 0.1, 0.1), 0.05
+    out = []
+    for tile in ((8, 8, 4), (4, 4, 4), (8, 4, 8)):
+        print(f"case tile={tile}", flush=True)
+        with oa.Simulation(3, nx, dx, dt, 3, tile=tile) as sim:
+            sid = sim.add_species(-1.0, oa.OSR_VAY)
+            sim.generate(sid, (2, 2, 2), density=1.0, uth=(0.1,) * 3, seed=5)
+            sim.fill_fields(b0=(0.0, 0.0, 0.5))
+            sim.run(12)
+            out.append((sim.get_fields(), sim.get_particles(0), sim.history()))
+    f0, p0, h0 = out[0]
+    for f, p, h in out[1:]:
+        for x, y in zip(f0 + p0, f + p):
+            assert np.array_equal(x, y)
+        assert np.array_equal(h0[0], h[0])
+        assert np.allclose(h0[1], h[1], rtol=1e-12, atol=0)
