@@ -317,6 +317,72 @@ int osr_smooth_vdf(int dims, const int *ld, int fdim, const int *order, const fl
 int osr_sim_set_smooth(osr_sim *sim, const int *order, const float *coef);
 
 /* ---------------------------------------------------------------------------------
+ * 6. Output in the reference's formats (host code; the resident forms download first)
+ * ------------------------------------------------------------------------------- */
+/* HIST energy files: appends nrows rows to <dir>/fld_ene (report_energy_emf,
+ * os-emf-diagnostics.f03:44-85), <dir>/par_ene (report_energy_particles,
+ * os-particles.f03:1710-1780) and <dir>/parNN_ene, NN = 01.. (report_energy_species,
+ * os-spec-diagnostics.f03:1840-1885), creating each file with its header when a row's time
+ * is 0, with the reference's Fortran formats ('(I6,1X,G15.8,6(1X,ES23.16))' etc.).
+ * fld[6 r + c]: B1 B2 B3 E1 E2 E3 of row r; par[nspec r + s]; npar[s] particle counts;
+ * names[s] species names (NULL: "species <s+1>").  Creates dir if needed. */
+int osr_hist_append(const char *dir, int nspec, const char *const *names, int64_t nrows,
+                    const int64_t *iter, const double *time, const double *fld,
+                    const double *par, const int64_t *npar);
+/* The same for history rows first .. first+count-1 of a resident simulation (iteration n,
+ * time n dt). */
+int osr_sim_write_hist(osr_sim *sim, const char *dir, const char *const *names, int64_t first,
+                       int64_t count);
+
+/* ZDF (zdf/zdf.h): data types and metadata as t_zdf_grid_info / t_zdf_grid_axis /
+ * t_zdf_iteration / t_zdf_part_info. */
+enum { OSR_ZDF_INT8 = 1, OSR_ZDF_UINT8, OSR_ZDF_INT16, OSR_ZDF_UINT16, OSR_ZDF_INT32,
+       OSR_ZDF_UINT32, OSR_ZDF_INT64, OSR_ZDF_UINT64, OSR_ZDF_FLOAT32, OSR_ZDF_FLOAT64 };
+typedef struct {
+  const char *name;
+  int type; /* 0 linear, 1 log10, 2 log2 */
+  double min, max;
+  const char *label, *units;
+} osr_zdf_grid_axis;
+typedef struct {
+  const char *name;
+  int ndims;
+  uint64_t count[3];
+  const char *label, *units;
+  const osr_zdf_grid_axis *axis; /* ndims entries or NULL */
+} osr_zdf_grid_info;
+typedef struct {
+  const char *name; /* "ITERATION" in OSIRIS */
+  int32_t n;
+  double t;
+  const char *time_units;
+} osr_zdf_iteration;
+typedef struct {
+  const char *name, *label;
+  uint64_t np;
+  int nquants;
+  const char *const *quants, *const *qlabels, *const *qunits;
+} osr_zdf_part_info;
+
+/* zdf_save_grid (zdf.c:1585-1610): <path>/<label>-<n %06u>.zdf with one dataset. */
+int osr_zdf_save_grid(const void *data, int data_type, const osr_zdf_grid_info *info,
+                      const osr_zdf_iteration *iter, const char *path);
+/* A grid file as OSIRIS's diagnostic files write it from one node (open_file_zdf +
+ * start_cdset / write_cdset / end_cdset, os-diagfile-zdf.f03:297-347, 481-595): the
+ * dataset as a chunked dataset of one chunk (start 0, stride 1); file is the full name. */
+int osr_zdf_write_grid_file(const char *file, const void *data, int data_type,
+                            const osr_zdf_grid_info *info, const osr_zdf_iteration *iter);
+/* zdf_open_part_file + zdf_add_quant_part_file (zdf.c:1612-1660): <path>/particles-<name>-
+ * <n %06u>.zdf, one float32 dataset of np values per quantity. */
+int osr_zdf_save_particles(const osr_zdf_part_info *info, const osr_zdf_iteration *iter,
+                           const char *path, const float *const *quants);
+/* Field dump of a resident simulation as OSIRIS writes MS/FLD/<q>/<q>-<n %06u>.zdf
+ * (labels and units of os-emf-diag-define.f03:164-190): quant 0..5 = e1 e2 e3 b1 b2 b3,
+ * interior cells, axes x1..x3 over [xmin, xmin + nx dx), iteration = steps taken,
+ * time = steps * dt. */
+int osr_sim_save_field(osr_sim *sim, int quant, const char *ms_dir, const double *xmin);
+
+/* ---------------------------------------------------------------------------------
  * Measurement helper (no counterpart in the reference): streams `bytes` bytes through a
  * plain device copy kernel `reps` times and returns the mean kernel time in *ms; used
  * to calibrate hardware traffic counters (FETCH_SIZE / WRITE_SIZE) and as a bandwidth

@@ -65,6 +65,12 @@ SIGNATURES = {
     "osr_smooth_setup": (I, [I, P, P, D, P, P]),
     "osr_smooth_vdf": (I, [I, P, I, P, P, P]),
     "osr_sim_set_smooth": (I, [P, P, P]),
+    "osr_hist_append": (I, [P, I, P, I64, P, P, P, P, P]),
+    "osr_sim_write_hist": (I, [P, P, P, I64, I64]),
+    "osr_zdf_save_grid": (I, [P, I, P, P, P]),
+    "osr_zdf_write_grid_file": (I, [P, P, I, P, P]),
+    "osr_zdf_save_particles": (I, [P, P, P, P]),
+    "osr_sim_save_field": (I, [P, I, P, P]),
     "osr_copy_probe": (I, [I64, I, P]),
 }
 

@@ -54,6 +54,8 @@ static int fail(int code, const char* fmt, ...) {
     if (r_ != OSR_OK) return r_; \
   } while (0)
 
+#include "output.hip"
+
 namespace {
 
 constexpr int kNT = 256;  // threads per workgroup of the push kernels
@@ -1205,6 +1207,58 @@ int osr_sim_history(osr_sim* s, int64_t first, int64_t count, double* fld, doubl
       for (int c = 0; c < ns; c++) par[ns * r + c] = rows[r * ncol + 6 + c];
   }
   return OSR_OK;
+}
+
+int osr_sim_write_hist(osr_sim* s, const char* dir, const char* const* names, int64_t first,
+                       int64_t count) {
+  TRY(check_sim(s));
+  const int ns = (int)s->sp.size();
+  std::vector<double> fld((size_t)(count > 0 ? count : 0) * 6), par((size_t)(count > 0 ? count : 0) * ns);
+  TRY(osr_sim_history(s, first, count, fld.data(), par.data()));
+  std::vector<int64_t> iter((size_t)count), npar((size_t)ns);
+  std::vector<double> t((size_t)count);
+  for (int64_t r = 0; r < count; r++) {
+    iter[r] = first + r;
+    t[r] = (double)(first + r) * s->dt;
+  }
+  for (int k = 0; k < ns; k++) npar[k] = s->sp[k]->np;
+  return osr_hist_append(dir, ns, names, count, iter.data(), t.data(), fld.data(), par.data(),
+                         npar.data());
+}
+
+int osr_sim_save_field(osr_sim* s, int quant, const char* ms_dir, const double* xmin) {
+  TRY(check_sim(s));
+  if (quant < 0 || quant > 5 || !ms_dir) return fail(OSR_ERR_ARG, "bad argument");
+  const size_t nc = (size_t)s->g.ncell;
+  std::vector<float> v(nc);
+  HIPC(hipStreamSynchronize(s->stream));
+  const DBuf& src = quant < 3 ? s->E : s->B;
+  HIPC(hipMemcpy(v.data(), src.as<float>() + (size_t)(quant % 3) * nc, 4 * nc,
+                 hipMemcpyDeviceToHost));
+  // os-emf-diag-define.f03:164-190: names, labels and units of the field reports
+  static const char* qname[6] = {"e1", "e2", "e3", "b1", "b2", "b3"};
+  static const char* qlabel[6] = {"E_1", "E_2", "E_3", "B_1", "B_2", "B_3"};
+  static const char* xname[3] = {"x1", "x2", "x3"};
+  static const char* xlabel[3] = {"x_1", "x_2", "x_3"};
+  osr_zdf_grid_axis ax[3];
+  osr_zdf_grid_info gi{};
+  gi.name = qname[quant];
+  gi.ndims = s->dims;
+  gi.label = qlabel[quant];
+  gi.units = "m_e c \\omega_p e^{-1}";
+  const int nxs[3] = {s->g.nx, s->g.ny, s->g.nz};
+  for (int d = 0; d < s->dims; d++) {
+    gi.count[d] = (uint64_t)nxs[d];
+    const double x0 = xmin ? xmin[d] : 0.0;
+    ax[d] = osr_zdf_grid_axis{xname[d], 0, x0, x0 + nxs[d] * s->dx[d], xlabel[d], "c / \\omega_p"};
+  }
+  gi.axis = ax;
+  osr_zdf_iteration it{"ITERATION", (int32_t)s->steps, (double)s->steps * s->dt, "1 / \\omega_p"};
+  std::string dir = std::string(ms_dir) + "/FLD/" + qname[quant];
+  if (osr_out::make_dirs(dir)) return fail(OSR_ERR_ARG, "cannot create directory %s", dir.c_str());
+  char fn[64];
+  snprintf(fn, sizeof fn, "/%s-%06u.zdf", qname[quant], (unsigned)s->steps);
+  return osr_zdf_write_grid_file((dir + fn).c_str(), v.data(), OSR_ZDF_FLOAT32, &gi, &it);
 }
 
 int osr_sim_gauss(osr_sim* s, double* res, double* rho_out, double* maxabs) {

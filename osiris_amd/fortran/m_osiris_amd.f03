@@ -62,6 +62,7 @@ module m_osiris_amd
             osr_fields_get, osr_fields_fill, osr_sim_run, osr_sim_steps, osr_sim_history, &
             osr_sim_gauss, osr_sim_relayouts, osr_sim_time_kernels, osr_sim_footprint
   public :: osr_smooth_setup, osr_smooth_vdf, osr_sim_set_smooth
+  public :: osr_sim_write_hist, osr_sim_save_field
 
   interface
     function osr_last_error() bind(c, name="osr_last_error")
@@ -303,6 +304,24 @@ module m_osiris_amd
       integer(c_int), intent(in) :: order(3)
       real(c_float), intent(in) :: coef(31, 3)
       integer(c_int) :: osr_sim_set_smooth
+    end function
+
+    ! ---- 6. output in the reference's formats (strings NUL-terminated: 'HIST'//c_null_char)
+    function osr_sim_write_hist(sim, dir, names, first, count) bind(c, name="osr_sim_write_hist")
+      import :: c_int, c_ptr, c_char, c_int64_t
+      type(c_ptr), value :: sim
+      character(kind=c_char), intent(in) :: dir(*)
+      type(c_ptr), value :: names        ! char*[nspecies] or c_null_ptr
+      integer(c_int64_t), value :: first, count
+      integer(c_int) :: osr_sim_write_hist
+    end function
+    function osr_sim_save_field(sim, quant, ms_dir, xmin) bind(c, name="osr_sim_save_field")
+      import :: c_int, c_ptr, c_char, c_double
+      type(c_ptr), value :: sim
+      integer(c_int), value :: quant
+      character(kind=c_char), intent(in) :: ms_dir(*)
+      real(c_double), intent(in) :: xmin(*)
+      integer(c_int) :: osr_sim_save_field
     end function
   end interface
 

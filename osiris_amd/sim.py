@@ -208,6 +208,20 @@ class Simulation:
         n = np.array(list(order) + [0] * (3 - len(order)), np.int32)
         check(lib().osr_sim_set_smooth(self._h, _p(n), _p(np.ascontiguousarray(coef, np.float32))))
 
+    def write_hist(self, dir, names=None, first=0, count=None):
+        """HIST/fld_ene, par_ene, parNN_ene rows of the history (reference text formats)."""
+        if count is None:
+            count = self.steps - first
+        arr = None
+        if names is not None:
+            arr = (C.c_char_p * len(names))(*[str(n).encode() for n in names])
+        check(lib().osr_sim_write_hist(self._h, str(dir).encode(), arr, int(first), int(count)))
+
+    def save_field(self, quant, ms_dir, xmin=None):
+        """OSIRIS-style MS/FLD/<q>/<q>-<n>.zdf dump of e1..e3 (0..2) or b1..b3 (3..5)."""
+        xm = None if xmin is None else np.array(list(xmin) + [0.0] * (3 - len(xmin)), np.float64)
+        check(lib().osr_sim_save_field(self._h, int(quant), str(ms_dir).encode(), _p(xm)))
+
     # ---- stepping and diagnostics
     def run(self, nsteps):
         check(lib().osr_sim_run(self._h, int(nsteps)))
