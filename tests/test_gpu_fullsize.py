@@ -1,0 +1,67 @@
+"""The headline workloads at their full sizes, checked through properties that do not
+depend on size (the CPU oracle cannot run 67 M or 134 M particles in test time):
+
+* charge conservation: the Gauss-law residual div E - rho changes only at FP32 round-off
+  of E over the run (the deposit is charge conserving);
+* bit reproducibility: a second run gives identical fields and energy histories;
+* tile-shape independence: a run with another tile shape (other workgroup size, other
+  mover traffic) gives identical fields and field energies, and particle energies to
+  FP64 round-off of the per-tile sums.
+
+Sizes: 2D counter-streaming beams 1024 x 1024, 64 ppc (67 M particles), quadratic, Boris;
+3D magnetized thermal plasma 256^3, 8 ppc (134 M particles), cubic, Vay, B0 = 0.5 z.
+"""
+import numpy as np
+import pytest
+
+import osiris_amd as oa
+
+pytestmark = pytest.mark.gpu
+
+
+def _two_stream(tile):
+    s = oa.Simulation(2, (1024, 1024), (0.05, 0.05), 0.035, 2, tile=tile)
+    for k, ufl in enumerate((0.2, -0.2)):
+        sid = s.add_species(-1.0, oa.OSR_BORIS)
+        s.generate(sid, (8, 4), density=0.5, ufl=(ufl, 0.0, 0.0), uth=(0.01,) * 3,
+                   seed=1000 + k, random_pos=True)
+    return s
+
+
+def _magnetized(tile):
+    s = oa.Simulation(3, (256, 256, 256), (0.1, 0.1, 0.1), 0.05, 3, tile=tile)
+    sid = s.add_species(-1.0, oa.OSR_VAY)
+    s.generate(sid, (2, 2, 2), density=1.0, uth=(0.1,) * 3, seed=3, random_pos=True)
+    s.fill_fields(b0=(0.0, 0.0, 0.5))
+    return s
+
+
+@pytest.mark.parametrize("name,make,tiles,steps", [
+    ("2d-two-stream-1024^2-64ppc", _two_stream, ((32, 16), (16, 16)), 20),
+    ("3d-magnetized-256^3-8ppc", _magnetized, ((8, 8, 4), (8, 4, 4)), 8),
+])
+def test_full_size_invariants(name, make, tiles, steps):
+    out = []
+    for rep, tile in enumerate((tiles[0], tiles[0], tiles[1])):
+        print(f"case {name} tile={tile} run={rep}", flush=True)
+        with make(tile) as sim:
+            g0, r0 = sim.gauss(True)
+            sim.run(steps)
+            g1, r1 = sim.gauss(True)
+            e, b = sim.get_fields()
+            fld, par = sim.history()
+            emax = float(np.abs(e).max())
+            out.append((e, b, fld, par))
+            # Gauss law: the residual is constant up to the FP32 round-off that E collects
+            # over the run (each step rounds every E component once)
+            dx = 0.05 if sim.dims == 2 else 0.1
+            roundoff = 2 * sim.dims * steps * np.finfo(np.float32).eps * emax / dx
+            assert np.abs(r1 - r0).max() <= max(roundoff, 1e-5 * np.abs(r0).max())
+            assert np.all(np.isfinite(fld)) and np.all(par > 0)
+    (e0, b0, f0, p0), (e1, b1, f1, p1), (e2, b2, f2, p2) = out
+    # reproducible run to run
+    for x, y in zip((e0, b0, f0, p0), (e1, b1, f1, p1)):
+        assert np.array_equal(x, y)
+    # independent of the tile shape
+    assert np.array_equal(e0, e2) and np.array_equal(b0, b2) and np.array_equal(f0, f2)
+    assert np.allclose(p0, p2, rtol=1e-12, atol=0)
