@@ -683,7 +683,7 @@ int launch_yee(osr_sim* s) {
 }
 
 int launch_finalize(osr_sim* s) {
-  hipLaunchKernelGGL(finalize_step, dim3(1), dim3(256), 0, s->stream, s->d_fd.as<FieldDesc>(),
+  hipLaunchKernelGGL(finalize_step, dim3(1), dim3(kFinNT), 0, s->stream, s->d_fd.as<FieldDesc>(),
                      s->d_fwork.as<int>());
   HIPC(hipGetLastError());
   return OSR_OK;

@@ -302,7 +302,7 @@ int ens_build(osr_ensemble* e, EnsBatch* b) {
       hipLaunchKernelGGL((yee_dedt<3>), dim3(b->nyee), dim3(256), 0, b->stream, fd, w);
       hipLaunchKernelGGL((yee_dbdt<3, false>), dim3(b->nyee), dim3(256), 0, b->stream, fd, w);
     }
-    hipLaunchKernelGGL(finalize_step, dim3(b->nfin), dim3(256), 0, b->stream, fd,
+    hipLaunchKernelGGL(finalize_step, dim3(b->nfin), dim3(kFinNT), 0, b->stream, fd,
                        b->wfin.as<int>());
   }
   hipError_t ce = hipStreamEndCapture(b->stream, &graph);

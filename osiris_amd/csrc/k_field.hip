@@ -208,44 +208,77 @@ OSR_DEV void kmerge(double& s, double& c, double s2, double c2) {
 
 // One block per simulation: compensated reduction of the step's partial sums into one
 // history row, then the step counter and the halt protocol.  The order of every addition
-// is fixed (strided per-thread sums, then a fixed shuffle tree per wave and the four
-// waves in order), so the row is reproducible bit for bit.
-__global__ __launch_bounds__(256) void finalize_step(const FieldDesc* __restrict__ fds,
-                                                     const int* __restrict__ work) {
-  __shared__ double w_s[4], w_c[4];
+// is fixed (strided per-thread sums, then a fixed shuffle tree per wave and the waves in
+// order), so the row is reproducible bit for bit.  The six field columns are reduced in
+// one pass (each thread reads the six contiguous partials of its blocks), and the loads
+// of several blocks are issued before they are accumulated.
+constexpr int kFinNT = 1024;
+
+// block-wide compensated merge of (s, c) in a fixed order; the result is valid in thread 0
+OSR_DEV void block_kmerge(double& s, double& c, double* w_s, double* w_c) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int off = 32; off > 0; off >>= 1) {
+    const double s2 = __shfl_down(s, off), c2 = __shfl_down(c, off);
+    kmerge(s, c, s2, c2);
+  }
+  if (lane == 0) {
+    w_s[wave] = s;
+    w_c[wave] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int w = 1; w < kFinNT / 64; w++) kmerge(s, c, w_s[w], w_c[w]);
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kFinNT) void finalize_step(const FieldDesc* __restrict__ fds,
+                                                        const int* __restrict__ work) {
+  __shared__ double w_s[kFinNT / 64], w_c[kFinNT / 64];
   const FieldDesc& fd = fds[work[blockIdx.x]];
   const GridDesc& g = fd.g;
   Ctrl* ctrl = g.ctrl;
   if (ctrl->halt >= 2) return;
   const long long step = ctrl->steps;
   const int ncol = 6 + fd.nspec;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int col = 0; col < ncol; col++) {
+  constexpr int U = 4;  // blocks loaded ahead per thread
+  {
+    double s[6] = {0, 0, 0, 0, 0, 0}, c[6] = {0, 0, 0, 0, 0, 0};
+    for (int b0 = threadIdx.x; b0 < fd.nblk; b0 += kFinNT * U) {
+      double v[U][6];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int b = b0 + u * kFinNT;
+#pragma unroll
+        for (int col = 0; col < 6; col++)
+          v[u][col] = b < fd.nblk ? fd.fe_part[(long long)b * 6 + col] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (b0 + u * kFinNT < fd.nblk)
+#pragma unroll
+          for (int col = 0; col < 6; col++) kadd(s[col], c[col], v[u][col]);
+    }
+    for (int col = 0; col < 6; col++) {
+      block_kmerge(s[col], c[col], w_s, w_c);
+      if (threadIdx.x == 0 && step < fd.hist_cap)
+        fd.hist[step * ncol + col] = (s[col] + c[col]) * fd.fe_scale;
+    }
+  }
+  for (int sp = 0; sp < fd.nspec; sp++) {
+    const double* ep = fd.ep[sp];
+    const int n = fd.ep_n[sp];
     double s = 0.0, c = 0.0;
-    if (col < 6) {
-      for (int b = threadIdx.x; b < fd.nblk; b += 256) kadd(s, c, fd.fe_part[(long long)b * 6 + col]);
-    } else {
-      const int sp = col - 6;
-      const double* ep = fd.ep[sp];
-      for (int t = threadIdx.x; t < fd.ep_n[sp]; t += 256) kadd(s, c, ep[t]);
+    for (int t0 = threadIdx.x; t0 < n; t0 += kFinNT * U) {
+      double v[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) v[u] = t0 + u * kFinNT < n ? ep[t0 + u * kFinNT] : 0.0;
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (t0 + u * kFinNT < n) kadd(s, c, v[u]);
     }
-    for (int off = 32; off > 0; off >>= 1) {
-      const double s2 = __shfl_down(s, off), c2 = __shfl_down(c, off);
-      kmerge(s, c, s2, c2);
-    }
-    if (lane == 0) {
-      w_s[wave] = s;
-      w_c[wave] = c;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double ts = w_s[0], tc = w_c[0];
-      for (int w = 1; w < 4; w++) kmerge(ts, tc, w_s[w], w_c[w]);
-      const double v = ts + tc;
-      if (step < fd.hist_cap)
-        fd.hist[step * ncol + col] = col < 6 ? v * fd.fe_scale : v * fd.ep_scale[col - 6];
-    }
-    __syncthreads();
+    block_kmerge(s, c, w_s, w_c);
+    if (threadIdx.x == 0 && step < fd.hist_cap)
+      fd.hist[step * ncol + 6 + sp] = (s + c) * fd.ep_scale[sp];
   }
   if (threadIdx.x == 0) {
     ctrl->steps = step + 1;
