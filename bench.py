@@ -31,10 +31,46 @@ HBM_PEAK_GBS = 8000.0
 # ix2+x2+u3 = 28 B; per cell the gather reads E,B (24 B) and the deposit writes J (12 B).
 BYTES_PER_PARTICLE = 60.0
 BYTES_PER_CELL_PUSH = 36.0
+# Yee per cell: B half step 36 B (read E, B; write B), E step 48 B (read E, B; write E;
+# J counted with the deposit), B half step 36 B
+BYTES_PER_CELL_YEE = 120.0
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def oracle_check(oa, steps=30):
+    """Error of the GPU path against the FP64 oracle, reported beside the CPU baseline:
+    the reference's CPU-runnable configuration (2D periodic thermal plasma 128 x 128,
+    dx = 0.1, dt = 0.07, 16 ppc, u_th = 0.05, linear splines, Boris) on the same seeded
+    particles, energy histories compared row by row."""
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(2024)
+    nx, dx, dt, ppc = (128, 128), (0.1, 0.1), 0.07, 16
+    cells = np.stack(np.meshgrid(np.arange(1, nx[0] + 1), np.arange(1, nx[1] + 1),
+                                 indexing="ij"), -1).reshape(-1, 2)
+    ix = np.repeat(cells, ppc, axis=0).astype(np.int32)
+    n = len(ix)
+    x = rng.uniform(-0.5, 0.5, (n, 2)).astype(np.float32)
+    u = rng.normal(0.0, 0.05, (n, 3)).astype(np.float32)
+    q = np.full(n, -1.0 / ppc, np.float32)
+    with oa.Simulation(2, nx, dx, dt, 1) as sim:
+        sid = sim.add_species(-1.0, oa.OSR_BORIS)
+        sim.set_particles(sid, ix, x, u, q)
+        sim.run(steps)
+        fld, par = sim.history()
+    ref = O.Sim(2, nx, dx, dt, 1)
+    ref.add_species(-1.0, 0, ix, x, u, q)
+    ref.step(steps)
+    rfld, rpar = np.array(ref.fld_hist), np.array(ref.par_hist)
+    fscale = np.maximum(rfld.sum(1, keepdims=True), 1e-300)
+    return {"workload": "2D periodic thermal plasma 128x128, 16 ppc, u_th=0.05, linear, Boris",
+            "steps": steps, "reference": "FP64 CPU oracle (restatement of the reference)",
+            "max_rel_err_particle_energy": float(np.max(np.abs(par - rpar) / np.abs(rpar))),
+            "max_rel_err_field_energy": float(np.max(np.abs(fld - rfld) / fscale)),
+            "tolerance": 1e-4}
 
 
 def cpu_baseline(nx, ppc_total, n_total, order, dx, dt, seconds, threads):
@@ -339,12 +375,25 @@ def main():
             "algorithmic_bytes": bytes_launch,
         },
         "kernels_ms_per_step": kt,
+        "kernels_roofline": {
+            # algorithmic bytes per step (HBM-bound kernels); measured traffic of every
+            # kernel is in profiles/kernels_c2.md
+            "push_deposit": {"bytes": nsp * bytes_launch, "ms": kt["push_deposit"],
+                             "GB/s": nsp * bytes_launch / kt["push_deposit"] / 1e6,
+                             "frac": nsp * bytes_launch / kt["push_deposit"] / 1e6 / HBM_PEAK_GBS},
+            "yee": {"bytes": cells * BYTES_PER_CELL_YEE, "ms": kt["yee"],
+                    "GB/s": cells * BYTES_PER_CELL_YEE / kt["yee"] / 1e6,
+                    "frac": cells * BYTES_PER_CELL_YEE / kt["yee"] / 1e6 / HBM_PEAK_GBS},
+        },
         "plain_path": plain,
     }
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         cb = cpu_baseline(nx, 2 * ppc_beam, n_total, args.order, dx, dt, args.cpu_seconds,
                           threads)
+        if cb is not None:
+            cb["oracle_check"] = oracle_check(oa)
+            log("oracle check: " + json.dumps(cb["oracle_check"]))
         out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 

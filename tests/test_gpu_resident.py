@@ -99,6 +99,13 @@ def _compare_with_oracle(dims, nx, dx, dt, order, species, steps, tile=None, fie
         # field energy components: relative to the total field energy of the row
         fscale = np.maximum(rfld.sum(1, keepdims=True), 1e-300)
         assert np.max(np.abs(fld - rfld) / fscale) <= e_tol
+        # particle moments: mean and rms momentum per component, relative to the rms
+        for s in range(len(species)):
+            gu = sim.get_particles(s)[2].astype(np.float64)
+            ru = ref.species[s].u.astype(np.float64)
+            rms = np.sqrt((ru ** 2).mean(0))
+            assert np.all(np.abs(gu.mean(0) - ru.mean(0)) <= e_tol * rms)
+            assert np.all(np.abs(np.sqrt((gu ** 2).mean(0)) - rms) <= e_tol * rms)
         g1 = sim.gauss()
         return sim.relayouts, g0, g1
 
