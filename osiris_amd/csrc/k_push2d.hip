@@ -6,11 +6,11 @@
 //  * particle, mover and field arrays are accessed through global-address-space
 //    pointers: uniform base + 32-bit offset, no per-array 64-bit address arithmetic, and
 //    the loads do not share the LDS wait counter as flat accesses would;
-//  * a trajectory that stays in its cell is one virtual particle (VP) and is deposited
-//    straight from registers with the start-point splines of the gather; a trajectory
-//    that crosses a cell face goes to a per-wave LDS queue, and 64 of them at a time are
-//    split into their 2-3 VPs and deposited, so the split and the deposit run on full
-//    waves instead of every wave paying for its few crossing lanes;
+//  * the first virtual particle (VP) of every particle is deposited straight from
+//    registers, with the start-point splines of the gather (VP 0 starts at the old
+//    position); a trajectory that crosses a cell face has 1 or 2 more VPs, which each
+//    wave appends to a private LDS queue and deposits 64 at a time, so the deposit runs
+//    on full waves instead of looping max(#VP) times over partly idle lanes;
 //  * the current of all waves is accumulated into one int64 fixed-point LDS tile; integer
 //    adds are exact and order independent, so the deposit is bit-reproducible run to run
 //    (FP32 LDS atomics were measured to be far slower here: a 40x longer LDS wait).
@@ -59,15 +59,15 @@ __global__ __launch_bounds__(push2d_t_threads(TX, TY), 4) void push_deposit_2d_t
   constexpr int JX = TX + O::JLO + O::JHI, JY = TY + O::JLO + O::JHI;
   constexpr int GS = GX + kFieldPad, GN = GS * GY;
   constexpr int JS = JX + kCurPad, JN = JS * JY;
-  constexpr int QC = 128;  // queue capacity per wave (< 64 pending + 64 appended)
+  constexpr int QC = 128;  // VP queue capacity per wave (< 64 pending + 64 appended)
   // staged fields: components that share a stencil are paired so that one 64-bit LDS
   // read serves both: (E1, B2) at (half x, full y), (E2, B1) at (full x, half y);
   // E3 (full, full) and B3 (half, half) are single
   __shared__ float2 sEB12[GN], sEB21[GN];
   __shared__ float sE3[GN], sB3[GN];
   __shared__ unsigned long long sJ[3 * JN];
-  __shared__ float sQ[NW][6][QC];  // crossing trajectories: x0, y0, x1, y1, q, vz
-  __shared__ int sQn[NW][QC];      // old-cell LDS node | cell steps << 24
+  __shared__ float sQ[NW][6][QC];  // x0, y0, x1, y1, q, vz
+  __shared__ int sQn[NW][QC];      // LDS node index of the VP's first stencil point
   __shared__ double s_ene[NW];
 
   const int2 wk = work[blockIdx.x];
@@ -137,30 +137,15 @@ __global__ __launch_bounds__(push2d_t_threads(TX, TY), 4) void push_deposit_2d_t
   gi32* __restrict__ Midx = (gi32*)d.midx + base;
 
   // deposit of the queued VPs in slots [head, head + cnt) (cnt <= 64, one per lane)
-  // The particles whose trajectory crosses a cell face (about one in seven here) are
-  // split and deposited from a per-wave LDS queue, 64 at a time, so the split and the
-  // deposit of their 2-3 virtual particles run on full waves; the others deposit their
-  // whole trajectory as one VP in the main loop and never enter the split.
-  // Queue entry: x0, y0, x1, y1, q, vz and the LDS node of the old cell with the cell
-  // steps packed above it (node | (dix + 1) << 24 | (diy + 1) << 26).
   auto deposit_queue = [&](int head, int cnt) {
     if (lane < cnt) {
       const int s = (head + lane) & (QC - 1);
-      const int code = QN[s];
-      const int nb = code & 0xFFFFFF;
-      const int d1 = ((code >> 24) & 3) - 1, d2 = ((code >> 26) & 3) - 1;
-      VP2 vp[3];
-      const int nvp = split2d_fast(Q[0][s], Q[1][s], Q[2][s], Q[3][s], d1, d2, Q[4][s],
-                                   Q[5][s], vp);
-#pragma unroll 1
-      for (int v = 0; v < nvp; v++) {
-        const VP2 V = pick(vp, v);
-        float S0x[NP], S0y[NP];
-        spline<ORDER>(V.x0, S0x);
-        spline<ORDER>(V.y0, S0y);
-        dep_vp<ORDER>(J1, J2, J3, S0x, S0y, V.x0, V.y0, V.x1, V.y1, V.q, V.vz,
-                      nb + V.dj * JS + V.di, JS, jn1, jn2, jscale);
-      }
+      const float x0 = Q[0][s], y0 = Q[1][s];
+      float S0x[NP], S0y[NP];
+      spline<ORDER>(x0, S0x);
+      spline<ORDER>(y0, S0y);
+      dep_vp<ORDER>(J1, J2, J3, S0x, S0y, x0, y0, Q[2][s], Q[3][s], Q[4][s], Q[5][s], QN[s], JS,
+                     jn1, jn2, jscale);
     }
   };
 
@@ -255,32 +240,33 @@ __global__ __launch_bounds__(push2d_t_threads(TX, TY), 4) void push_deposit_2d_t
     const float x1 = fma_(v1, dtdx1, x), y1 = fma_(v2, dtdx2, y);
     const float tx = trim(x1), ty = trim(y1);
     const int dix = (int)tx, diy = (int)ty;
-    // ---- deposit: a trajectory inside its cell is one VP (split_2d leaves it whole) and is
-    // deposited here with the gather's splines; crossing ones go through the queue
+    // ---- split; VP 0 is deposited in place, VPs 1 and 2 go through the queue
+    VP2 vp[3];
+    const int nvp = act ? split2d_fast(x, y, x1, y1, dix, diy, q, v3, vp) : 0;
     const int nb0 = (ly + O::LP + O::JLO) * JS + lx + O::LP + O::JLO;
+    if (act)
+      dep_vp<ORDER>(J1, J2, J3, wx, wy, vp[0].x0, vp[0].y0, vp[0].x1, vp[0].y1, vp[0].q,
+                    vp[0].vz, nb0, JS, jn1, jn2, jscale);
     const unsigned long long below = (1ull << lane) - 1ull;
-    const bool cross = act && (dix != 0 || diy != 0);
-    if (act && !cross)
-      dep_vp<ORDER>(J1, J2, J3, wx, wy, x, y, x1, y1, q, v3, nb0, JS, jn1, jn2, jscale);
-    {
-      const unsigned long long m = __ballot(cross);
-      if (m != 0ull) {
-        if (cross) {
-          const int s = (qhead + qlen + __popcll(m & below)) & (QC - 1);
-          Q[0][s] = x;
-          Q[1][s] = y;
-          Q[2][s] = x1;
-          Q[3][s] = y1;
-          Q[4][s] = q;
-          Q[5][s] = v3;
-          QN[s] = nb0 | ((dix + 1) << 24) | ((diy + 1) << 26);
-        }
-        qlen += __popcll(m);
-        if (qlen >= 64) {
-          deposit_queue(qhead, 64);
-          qhead = (qhead + 64) & (QC - 1);
-          qlen -= 64;
-        }
+#pragma unroll
+    for (int v = 1; v < 3; v++) {
+      const unsigned long long m = __ballot(v < nvp);
+      if (m == 0ull) break;
+      if (v < nvp) {
+        const int s = (qhead + qlen + __popcll(m & below)) & (QC - 1);
+        Q[0][s] = vp[v].x0;
+        Q[1][s] = vp[v].y0;
+        Q[2][s] = vp[v].x1;
+        Q[3][s] = vp[v].y1;
+        Q[4][s] = vp[v].q;
+        Q[5][s] = vp[v].vz;
+        QN[s] = nb0 + vp[v].dj * JS + vp[v].di;
+      }
+      qlen += __popcll(m);
+      if (qlen >= 64) {
+        deposit_queue(qhead, 64);
+        qhead = (qhead + 64) & (QC - 1);
+        qlen -= 64;
       }
     }
     // ---- store: stay in place, or leave through the mover buffer
