@@ -156,12 +156,13 @@ struct osr_sim {
   int dims = 2, order = 2;
   int ntiles = 0;
   int nblk = 0;  // Yee blocks of 256 cells
+  int nblk_e = 0;  // energy workgroups of the first B half step (kEneBlk blocks each)
   double dx[3] = {1, 1, 1};
   double dt = 0;
   DBuf E, B, J, ctrl, fe_part, hist;
   int64_t hist_cap = 0;
   std::vector<std::unique_ptr<Species>> sp;
-  DBuf d_spec, d_work, d_fd, d_ywork, d_fwork;
+  DBuf d_spec, d_work, d_fd, d_ywork, d_ework, d_fwork;
   FieldDesc fd{};
   hipStream_t stream = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -260,7 +261,7 @@ void make_field_desc(const osr_sim* s, FieldDesc& f) {
   f = FieldDesc{};
   f.g = s->g;
   f.fe_part = s->fe_part.as<double>();
-  f.nblk = s->nblk;
+  f.nblk = s->nblk_e;
   f.nspec = (int)s->sp.size();
   double dvol = 1.0;
   for (int d = 0; d < s->dims; d++) dvol *= s->dx[d];
@@ -669,12 +670,13 @@ int launch_fill(osr_sim* s) {
 int launch_yee(osr_sim* s) {
   const FieldDesc* fd = s->d_fd.as<FieldDesc>();
   const int2* w = s->d_ywork.as<int2>();
+  const int2* we = s->d_ework.as<int2>();
   if (s->dims == 2) {
-    hipLaunchKernelGGL((yee_dbdt<2, true>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
+    hipLaunchKernelGGL((yee_dbdt<2, true>), dim3(s->nblk_e), dim3(256), 0, s->stream, fd, we);
     hipLaunchKernelGGL((yee_dedt<2>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
     hipLaunchKernelGGL((yee_dbdt<2, false>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
   } else {
-    hipLaunchKernelGGL((yee_dbdt<3, true>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
+    hipLaunchKernelGGL((yee_dbdt<3, true>), dim3(s->nblk_e), dim3(256), 0, s->stream, fd, we);
     hipLaunchKernelGGL((yee_dedt<3>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
     hipLaunchKernelGGL((yee_dbdt<3, false>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
   }
@@ -886,17 +888,20 @@ int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
   g.J = s->J.as<unsigned long long>();
   g.ctrl = s->ctrl.as<Ctrl>();
   s->nblk = (int)((nc + 255) / 256);
-  TRY(s->fe_part.alloc(sizeof(double) * 6 * (size_t)s->nblk));
-  HIPC(hipMemsetAsync(s->fe_part.p, 0, sizeof(double) * 6 * (size_t)s->nblk, s->stream));
+  s->nblk_e = (s->nblk + kEneBlk - 1) / kEneBlk;
+  TRY(s->fe_part.alloc(sizeof(double) * 6 * (size_t)s->nblk_e));
+  HIPC(hipMemsetAsync(s->fe_part.p, 0, sizeof(double) * 6 * (size_t)s->nblk_e, s->stream));
   HIPC(hipStreamSynchronize(s->stream));
   TRY(s->d_spec.alloc(sizeof(SpecDesc) * kMaxSpecies));
   TRY(s->d_fd.alloc(sizeof(FieldDesc)));
   TRY(s->d_ywork.alloc(sizeof(int2) * (size_t)s->nblk));
+  TRY(s->d_ework.alloc(sizeof(int2) * (size_t)s->nblk_e));
   TRY(s->d_fwork.alloc(sizeof(int)));
   {
     std::vector<int2> yw((size_t)s->nblk);
     for (int b = 0; b < s->nblk; b++) yw[b] = make_int2(0, b);
     HIPC(hipMemcpy(s->d_ywork.p, yw.data(), sizeof(int2) * yw.size(), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(s->d_ework.p, yw.data(), sizeof(int2) * (size_t)s->nblk_e, hipMemcpyHostToDevice));
     int z = 0;
     HIPC(hipMemcpy(s->d_fwork.p, &z, sizeof(int), hipMemcpyHostToDevice));
   }

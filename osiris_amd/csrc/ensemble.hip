@@ -39,8 +39,8 @@ struct EnsBatch {
   hipEvent_t ev = nullptr;
   std::vector<EnsJob*> jobs;     // jobs stepped by the graph
   std::vector<EnsJob*> joining;  // admitted while a chunk was in flight
-  DBuf spec, fds, wpush[2], wfill, wyee, wfin;
-  int npush[2] = {0, 0}, nfill = 0, nyee = 0, nfin = 0;
+  DBuf spec, fds, wpush[2], wfill, wyee, wene, wfin;
+  int npush[2] = {0, 0}, nfill = 0, nyee = 0, nene = 0, nfin = 0;
   hipGraphExec_t gexec = nullptr;
   bool dirty = true, in_flight = false;
   Ctrl* h_ctrl = nullptr;
@@ -217,7 +217,7 @@ int ens_build(osr_ensemble* e, EnsBatch* b) {
   PoolScope ps(e->alloc_stream);
   std::vector<SpecDesc> specs;
   std::vector<FieldDesc> fds(b->jobs.size());
-  std::vector<int2> wp[2], wf, wy;
+  std::vector<int2> wp[2], wf, wy, we;
   std::vector<int> wn;
   for (size_t k = 0; k < b->jobs.size(); k++) {
     osr_sim* s = b->jobs[k]->sim;
@@ -231,6 +231,7 @@ int ens_build(osr_ensemble* e, EnsBatch* b) {
         wf.push_back(make_int2(base + (int)i, t));
       }
     for (int blk = 0; blk < s->nblk; blk++) wy.push_back(make_int2((int)k, blk));
+    for (int blk = 0; blk < s->nblk_e; blk++) we.push_back(make_int2((int)k, blk));
     wn.push_back((int)k);
   }
   auto put = [&](DBuf& d, const void* src, size_t bytes) -> int {
@@ -247,9 +248,11 @@ int ens_build(osr_ensemble* e, EnsBatch* b) {
   }
   TRY(put(b->wfill, wf.data(), sizeof(int2) * wf.size()));
   TRY(put(b->wyee, wy.data(), sizeof(int2) * wy.size()));
+  TRY(put(b->wene, we.data(), sizeof(int2) * we.size()));
   TRY(put(b->wfin, wn.data(), sizeof(int) * wn.size()));
   b->nfill = (int)wf.size();
   b->nyee = (int)wy.size();
+  b->nene = (int)we.size();
   b->nfin = (int)wn.size();
   if ((int)b->jobs.size() > b->h_ctrl_cap) {
     if (b->h_ctrl) HIPC(hipHostFree(b->h_ctrl));
@@ -293,12 +296,13 @@ int ens_build(osr_ensemble* e, EnsBatch* b) {
                        b->wfill.as<int2>());
     const FieldDesc* fd = b->fds.as<FieldDesc>();
     const int2* w = b->wyee.as<int2>();
+    const int2* we = b->wene.as<int2>();
     if (b->dims == 2) {
-      hipLaunchKernelGGL((yee_dbdt<2, true>), dim3(b->nyee), dim3(256), 0, b->stream, fd, w);
+      hipLaunchKernelGGL((yee_dbdt<2, true>), dim3(b->nene), dim3(256), 0, b->stream, fd, we);
       hipLaunchKernelGGL((yee_dedt<2>), dim3(b->nyee), dim3(256), 0, b->stream, fd, w);
       hipLaunchKernelGGL((yee_dbdt<2, false>), dim3(b->nyee), dim3(256), 0, b->stream, fd, w);
     } else {
-      hipLaunchKernelGGL((yee_dbdt<3, true>), dim3(b->nyee), dim3(256), 0, b->stream, fd, w);
+      hipLaunchKernelGGL((yee_dbdt<3, true>), dim3(b->nene), dim3(256), 0, b->stream, fd, we);
       hipLaunchKernelGGL((yee_dedt<3>), dim3(b->nyee), dim3(256), 0, b->stream, fd, w);
       hipLaunchKernelGGL((yee_dbdt<3, false>), dim3(b->nyee), dim3(256), 0, b->stream, fd, w);
     }

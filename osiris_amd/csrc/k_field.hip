@@ -19,7 +19,7 @@ namespace osr {
 struct FieldDesc {
   GridDesc g;
   double* fe_part;   // [nblk][6] B1,B2,B3,E1,E2,E3 partial sums of squares
-  int nblk;
+  int nblk;          // energy partials: one per kEneBlk blocks of 256 cells
   // finalisation
   int nspec;
   double* ep[kMaxSpecies];  // per-species tile energy partials
@@ -49,19 +49,26 @@ struct FieldDesc {
 
 OSR_DEV int wrp(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
+// 256-cell blocks per workgroup of the B half step that also sums the field energy: the
+// FP64 block reduction is paid once per 1024 cells
+constexpr int kEneBlk = 4;
+
 template <int DIMS, bool ENERGY>
 __global__ __launch_bounds__(256) void yee_dbdt(const FieldDesc* __restrict__ fds,
                                                 const int2* __restrict__ work) {
   __shared__ double s_red[4][6];
+  constexpr int NB = ENERGY ? kEneBlk : 1;
   const int2 wk = work[blockIdx.x];
   const FieldDesc& fd = fds[wk.x];
   const GridDesc& g = fd.g;
   if (g.ctrl->halt >= 2) return;
   const long long nc = g.ncell;
-  const long long idx = (long long)wk.y * 256 + threadIdx.x;
   const float d1 = fd.bdt[0], d2 = fd.bdt[1], d3 = fd.bdt[2];
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  if (idx < nc) {
+#pragma unroll
+  for (int qb = 0; qb < NB; qb++) {
+    const long long idx = ((long long)wk.y * NB + qb) * 256 + threadIdx.x;
+    if (idx >= nc) break;
     const int nx = g.nx, ny = g.ny;
     const int i = (int)(idx % nx);
     const int j = (int)((idx / nx) % ny);
@@ -76,12 +83,12 @@ __global__ __launch_bounds__(256) void yee_dbdt(const FieldDesc* __restrict__ fd
     const float e1 = E1[idx], e2 = E2[idx], e3 = E3[idx];
     float b1 = B1[idx], b2 = B2[idx], b3 = B3[idx];
     if (ENERGY) {
-      acc[0] = (double)b1 * b1;
-      acc[1] = (double)b2 * b2;
-      acc[2] = (double)b3 * b3;
-      acc[3] = (double)e1 * e1;
-      acc[4] = (double)e2 * e2;
-      acc[5] = (double)e3 * e3;
+      acc[0] += (double)b1 * b1;
+      acc[1] += (double)b2 * b2;
+      acc[2] += (double)b3 * b3;
+      acc[3] += (double)e1 * e1;
+      acc[4] += (double)e2 * e2;
+      acc[5] += (double)e3 * e3;
     }
     if (DIMS == 2) {
       b1 = b1 - d2 * (E3[jp] - e3);
