@@ -131,8 +131,17 @@ typedef struct {
                      compile-time tile shape with a virtual-particle queue per wave
                      (tiles 8x8, 16x16, 32x16); both accumulate the current in int64
                      fixed point in LDS */
-  int reserved[2];
+  int xbnd;       /* x walls: OSR_BND_PERIODIC (0) or OSR_BND_OPEN (1) */
+  int reserved;
 } osr_sim_desc;
+
+/* x boundaries of a resident simulation.  OSR_BND_OPEN: Lindman absorbing field walls at
+ * both x ends (emf_bound type "lindman", emf/boundary/os-emf-lindman.f90:190-941, applied
+ * by os-emf-boundary.f90:723-1112) and absorbing particles (spec_bound type "absorbing" /
+ * "open", spec/os-spec-boundary.f03:150-151, 476-513: particles leaving through x are
+ * removed after the push); y stays periodic.  2D fused path only, nx a multiple of the
+ * tile width, no current smoothing; osr_sim_gauss is not available. */
+enum { OSR_BND_PERIODIC = 0, OSR_BND_OPEN = 1 };
 
 /* OSR_PATH_FUSED: tiled fused push+deposit, LDS-staged fields and current, particles
  * sorted into tiles every step, reproducible current.  OSR_PATH_PLAIN: one kernel per

@@ -64,9 +64,43 @@ def _lib(prec: str):
         L.orc_step.argtypes = [P, C.c_int, P, C.c_double, P, P, P, C.c_int, P, P, P, P, P, P,
                                P, P, P, P, P]
         L.orc_smooth.argtypes = [P, C.c_int, P, P, P]
+        L.orc_step_open.argtypes = [P, C.c_int, P, C.c_double, P, P, P, P, C.c_int, P, P, P,
+                                    P, P, P, P, P, P]
+        L.orc_open_boundary.argtypes = [P, P, P, P]
+        L.orc_yee_advance_open.argtypes = [P, P, C.c_double, P, P, P, P]
+        L.orc_particles_open.argtypes = [P, P, P, P, P, C.c_int]
+        L.orc_particles_open.restype = C.c_int
         L._real = np.float64 if prec == "f64" else np.float32
         _LIBS[prec] = L
     return _LIBS[prec]
+
+
+def _lindman_struct(real):
+    """orc_lindman of pic_oracle.c for the working precision."""
+    R = C.c_double if real == np.float64 else C.c_float
+
+    class Lindman(C.Structure):
+        _fields_ = [("disp_corr", R), ("dtdxi", R * 2), ("ebl", C.c_void_p),
+                    ("bbl", C.c_void_p), ("ebu", C.c_void_p)]
+    return Lindman
+
+
+class LindmanWalls:
+    """Lindman x walls of a 2D grid (new_lindman, emf/boundary/os-emf-lindman.f90:190-330):
+    zeroed wall buffers and the coefficients in the working precision."""
+
+    def __init__(self, grid, dx, dt, real):
+        ld1 = grid.nx[1] + grid.gcl[1] + grid.gcu[1]
+        self.ebl = np.zeros((ld1, 2, 3), real)
+        self.bbl = np.zeros((ld1, 2, 3), real)
+        self.ebu = np.zeros((ld1, 3, 3), real)
+        S = _lindman_struct(real)
+        self.c = S()
+        r = real
+        self.c.disp_corr = r((1.0 - dt / dx[0]) / (1.0 + dt / dx[0]))
+        for d in range(2):
+            self.c.dtdxi[d] = r(r(dt / dx[d]) * r(0.5))
+        self.c.ebl, self.c.bbl, self.c.ebu = (a.ctypes.data for a in (self.ebl, self.bbl, self.ebu))
 
 
 def ref_available(prec: str = "s") -> bool:
@@ -362,6 +396,9 @@ class Sim:
     # from oracle.smooth.setup; None = no smoothing
     smooth_order: tuple = None
     smooth_coef: np.ndarray = None
+    # x walls: False = periodic; True = Lindman absorbing fields and absorbing particles
+    # (2D, y periodic)
+    open_x: bool = False
 
     def __post_init__(self):
         self.grid = make_grid(self.dims, self.nx, self.order)
@@ -378,6 +415,11 @@ class Sim:
         self.e = self.grid.zeros(3, r)
         self.b = self.grid.zeros(3, r)
         self.j = self.grid.zeros(3, r)
+        self.walls = None
+        if self.open_x:
+            if self.dims != 2 or self.smooth_order is not None:
+                raise ValueError("open x walls are implemented for 2D without smoothing")
+            self.walls = LindmanWalls(self.grid, self.dx, self.dt, r)
         self.fld_hist = []
         self.par_hist = []
 
@@ -396,7 +438,14 @@ class Sim:
         f = self.e if which == "e" else self.b
         self.grid.interior(f)[...] = arr
 
+    def fill_uniform(self, e0=(0.0, 0.0, 0.0), b0=(0.0, 0.0, 0.0)):
+        """Uniform fields on every cell, guard cells included (os-emf-gridval.f03)."""
+        self.e[...] = np.asarray(e0, np.float64).astype(self.real)
+        self.b[...] = np.asarray(b0, np.float64).astype(self.real)
+
     def step(self, n=1):
+        if self.open_x:
+            return self._step_open(n)
         L = _lib(self.prec)
         g = self.grid.c()
         ns = len(self.species)
@@ -417,6 +466,37 @@ class Sim:
                        _p(pe), _p(self.sm_order), _p(self.sm_coef))
             self.fld_hist.append(fe)
             self.par_hist.append(pe)
+
+    def _step_open(self, n):
+        L = _lib(self.prec)
+        g = self.grid.c()
+        ns = len(self.species)
+        dxa = np.array(list(self.dx) + [1.0], np.float64)
+        IA = C.c_void_p * ns
+        pus = np.array([s.pusher for s in self.species], np.int32)
+        rqm = np.array([s.rqm for s in self.species], self.real)
+        npa = np.array([len(s.q) for s in self.species], np.int32)
+        ixs = IA(*[s.ix.ctypes.data for s in self.species])
+        xs = IA(*[s.x.ctypes.data for s in self.species])
+        us = IA(*[s.u.ctypes.data for s in self.species])
+        qs = IA(*[s.q.ctypes.data for s in self.species])
+        for _ in range(n):
+            fe = np.zeros(6)
+            pe = np.zeros(ns)
+            L.orc_step_open(C.byref(g), self.order, _p(dxa), self.dt, _p(self.e), _p(self.b),
+                            _p(self.j), C.byref(self.walls.c), ns, _p(pus), _p(rqm), ixs, xs,
+                            us, qs, _p(npa), _p(fe), _p(pe))
+            self.fld_hist.append(fe)
+            self.par_hist.append(pe)
+        for s, k in zip(self.species, npa):  # leading slices stay contiguous
+            s.ix, s.x, s.u, s.q = s.ix[:k], s.x[:k], s.u[:k], s.q[:k]
+
+    def wall_refresh(self):
+        """The boundary update the next step starts with (Lindman restore, periodic y), so
+        that the guard cells hold what the next push reads."""
+        L = _lib(self.prec)
+        g = self.grid.c()
+        L.orc_open_boundary(C.byref(g), C.byref(self.walls.c), _p(self.e), _p(self.b))
 
     def gauss_residual(self):
         """div E - rho (interior nodes) with rho from all species (no ion background)."""

@@ -799,9 +799,10 @@ int orc_split(int dims, int np, const int *dxi, const real *xnew, const int *ixo
  * (:685-692).  mode 0 = p_vdf_replace (E,B), 1 = p_vdf_add (current fold).
  * Particles: spec/os-spec-boundary.f03:1280-1290 (ix += nx below 1, ix -= nx above nx).
  * ------------------------------------------------------------------------------------ */
-void orc_update_periodic(const orc_grid *g, real *f, int nc, int mode) {
+/* directions d0..x_dim-1 only (d0 = 1 when x is not periodic) */
+static void update_periodic_from(const orc_grid *g, real *f, int nc, int mode, int d0) {
   const int D = g->dims;
-  for (int dim = 0; dim < D; dim++) {
+  for (int dim = d0; dim < D; dim++) {
     int shift[3] = {0, 0, 0};
     shift[dim] = g->nx[dim];
     int llo[3], lhi[3], ulo[3], uhi[3];
@@ -847,6 +848,10 @@ void orc_update_periodic(const orc_grid *g, real *f, int nc, int mode) {
           }
 #undef IDX
   }
+}
+
+void orc_update_periodic(const orc_grid *g, real *f, int nc, int mode) {
+  update_periodic_from(g, f, nc, mode, 0);
 }
 
 void orc_particles_periodic(const orc_grid *g, int *ix, int np) {
@@ -1073,4 +1078,149 @@ void orc_step(const orc_grid *g, int order, const double *dx, double dt, real *e
   /* os-simulation.f03:650-657: the folded current is smoothed before the field advance */
   if (sm_order) orc_smooth(g, 3, j, sm_order, sm_coef);
   orc_yee_advance(g, dx, dt, e, b, j);
+}
+
+/* ------------------------------------------------------------------------------------
+ * Open x walls, 2D: Lindman absorbing field boundaries on both x walls
+ * (emf/boundary/os-emf-lindman.f90) and absorbing particles
+ * (spec/os-spec-boundary.f03:150-151, 476-513: particles with ix < 1 or ix > nx after
+ * the push are removed); y stays periodic.
+ *
+ * The wall buffers are t_wall objects (vdf/os-wall.f90:111-215): zero at creation,
+ * x range (0,1) at the lower wall and (nx, nx+2) at the upper one
+ * (os-emf-lindman.f90:308-324), y index 1-gcl..ny+gcu.  They are stored here as
+ * (3, ncol, ld1) with ncol = 2 (lower) or 3 (upper); the upper wall's B buffer is never
+ * updated nor read (os-emf-lindman.f90:572-573, 994-1011) and is not kept.
+ * ------------------------------------------------------------------------------------ */
+typedef struct {
+  real disp_corr;  /* (1 - dt/dx1) / (1 + dt/dx1), os-emf-lindman.f90:237-238 */
+  real dtdxi[2];   /* real(dt/dx, p_k_fld) * 0.5, os-emf-lindman.f90:236, 241 */
+  real *ebl, *bbl; /* lower wall E and B buffers, x = 0, 1 */
+  real *ebu;       /* upper wall E buffer, x = nx, nx+1, nx+2 */
+} orc_lindman;
+
+#define WB(w, ncol, c, col, j) ((w)[3 * ((col) + (ncol) * (long)((j) - 1 + g->gcl[1])) + (c)])
+
+/* update_periodic_wall with p_vdf_replace along y, vdf/os-wall-comm.f90:182-344 */
+static void wall_periodic_y(const orc_grid *g, real *w, int ncol) {
+  const int ny = g->nx[1];
+  for (int j = 1 - g->gcl[1]; j <= 0; j++)
+    for (int col = 0; col < ncol; col++)
+      for (int c = 0; c < 3; c++) WB(w, ncol, c, col, j) = WB(w, ncol, c, col, j + ny);
+  for (int j = ny + 1; j <= ny + g->gcu[1]; j++)
+    for (int col = 0; col < ncol; col++)
+      for (int c = 0; c < 3; c++) WB(w, ncol, c, col, j) = WB(w, ncol, c, col, j - ny);
+}
+
+/* update_lindman_2d_e, idir = 1, os-emf-lindman.f90:454-535 (perp = components 2, 3) */
+static void lindman_e(const orc_grid *g, orc_lindman *w, real *e) {
+  const int nx = g->nx[0];
+  for (int j = 1; j <= g->nx[1]; j++)
+    for (int p = 1; p <= 2; p++) {
+      /* lower wall, i_range = (0, 1) */
+      FLD(e, g, p, 0, j, 1) = WB(w->ebl, 2, p, 1, j) +
+                              w->disp_corr * (WB(w->ebl, 2, p, 0, j) - FLD(e, g, p, 1, j, 1));
+      WB(w->ebl, 2, p, 0, j) = FLD(e, g, p, 0, j, 1);
+      WB(w->ebl, 2, p, 1, j) = FLD(e, g, p, 1, j, 1);
+    }
+  for (int j = 1; j <= g->nx[1]; j++)
+    for (int p = 1; p <= 2; p++) {
+      /* upper wall, i_range = (nx, nx+2): buffer columns 1, 2 are x = nx+1, nx+2 */
+      FLD(e, g, p, nx + 2, j, 1) =
+          WB(w->ebu, 3, p, 1, j) + w->disp_corr * (WB(w->ebu, 3, p, 2, j) - FLD(e, g, p, nx + 1, j, 1));
+      WB(w->ebu, 3, p, 1, j) = FLD(e, g, p, nx + 1, j, 1);
+      WB(w->ebu, 3, p, 2, j) = FLD(e, g, p, nx + 2, j, 1);
+    }
+}
+
+/* update_lindman_2d_b, idir = 1, lower wall only, os-emf-lindman.f90:539-600 */
+static void lindman_b(const orc_grid *g, orc_lindman *w, const real *e) {
+  const real d1 = w->dtdxi[0], d2 = w->dtdxi[1];
+  for (int j = 1; j <= g->nx[1]; j++) {
+    real *bb = &WB(w->bbl, 2, 0, 0, j);
+    bb[0] = bb[0] - d2 * (FLD(e, g, 2, 0, j + 1, 1) - FLD(e, g, 2, 0, j, 1));
+    bb[1] = bb[1] + d1 * (FLD(e, g, 2, 1, j, 1) - FLD(e, g, 2, 0, j, 1));
+    bb[2] = bb[2] - d1 * (FLD(e, g, 1, 1, j, 1) - FLD(e, g, 1, 0, j, 1)) +
+            d2 * (FLD(e, g, 0, 0, j + 1, 1) - FLD(e, g, 0, 0, j, 1));
+  }
+}
+
+/* update_boundary_emf_bound for open x / periodic y on one node,
+ * emf/boundary/os-emf-boundary.f90:723-830: wall buffers refreshed along y (:768-769,
+ * os-emf-lindman.f90:352-368), then the x walls restore the wall values
+ * (update_emfbound_lindman, os-emf-lindman.f90:861-941: B and E at x = 0 from the lower
+ * buffers, E at x = nx+1 from the upper one, every y), then the periodic y refresh. */
+void orc_open_boundary(const orc_grid *g, orc_lindman *w, real *e, real *b) {
+  wall_periodic_y(g, w->bbl, 2);
+  wall_periodic_y(g, w->ebl, 2);
+  wall_periodic_y(g, w->ebu, 3);
+  const int nx = g->nx[0];
+  for (int j = 1 - g->gcl[1]; j <= g->nx[1] + g->gcu[1]; j++)
+    for (int c = 0; c < 3; c++) {
+      FLD(b, g, c, 0, j, 1) = WB(w->bbl, 2, c, 0, j);
+      FLD(e, g, c, 0, j, 1) = WB(w->ebl, 2, c, 0, j);
+      FLD(e, g, c, nx + 1, j, 1) = WB(w->ebu, 3, c, 1, j);
+    }
+  update_periodic_from(g, e, 3, 0, 1);
+  update_periodic_from(g, b, 3, 0, 1);
+}
+#undef WB
+
+/* advance_yee with Lindman x walls, os-emf-solver-yee.f03:542-568 with
+ * update_boundary_bfld/efld (os-emf-boundary.f90:840-1112): no guard refresh between the
+ * sub-steps. */
+void orc_yee_advance_open(const orc_grid *g, const double *dx, double dt, real *e, real *b,
+                          const real *j, orc_lindman *w) {
+  dbdt(g, dx, dt / 2.0, b, e);
+  lindman_b(g, w, e);
+  dedt(g, dx, dt, e, b, j);
+  lindman_e(g, w, e);
+  dbdt(g, dx, dt / 2.0, b, e);
+  lindman_b(g, w, e);
+}
+
+/* Particle boundaries: y periodic (as orc_particles_periodic), x absorbing: particles with
+ * ix < 1 or ix > nx are removed, the others keep their order.  Returns the new count. */
+int orc_particles_open(const orc_grid *g, int *ix, real *x, real *u, real *q, int np) {
+  const int D = g->dims;
+  int k = 0;
+  for (int i = 0; i < np; i++) {
+    int *v = &ix[D * i + 1];
+    if (*v < 1)
+      *v += g->nx[1];
+    else if (*v > g->nx[1])
+      *v -= g->nx[1];
+    if (ix[D * i] < 1 || ix[D * i] > g->nx[0]) continue;
+    if (k != i) {
+      for (int d = 0; d < D; d++) {
+        ix[D * k + d] = ix[D * i + d];
+        x[D * k + d] = x[D * i + d];
+      }
+      for (int c = 0; c < 3; c++) u[3 * k + c] = u[3 * i + c];
+      q[k] = q[i];
+    }
+    k++;
+  }
+  return k;
+}
+
+/* orc_step with open x walls (2D): boundary update of E/B (Lindman restore + periodic y),
+ * push+deposit, particle boundaries, energies, J folded along y only, Yee with the
+ * Lindman wall updates.  np[s] is updated to the particles left. */
+void orc_step_open(const orc_grid *g, int order, const double *dx, double dt, real *e, real *b,
+                   real *j, orc_lindman *w, int nspec, const int *pusher, const real *rqm,
+                   int *const *ix, real *const *x, real *const *u, real *const *q, int *np,
+                   double *fld_ene, double *par_ene) {
+  double dvol = dx[0] * dx[1];
+  orc_open_boundary(g, w, e, b);
+  memset(j, 0, sizeof(real) * 3 * orc_grid_cells(g));
+  for (int s = 0; s < nspec; s++) {
+    double ene = orc_advance_deposit(g, order, pusher[s], rqm[s], dx, dt, e, b, j, ix[s], x[s],
+                                     u[s], q[s], np[s]);
+    np[s] = orc_particles_open(g, ix[s], x[s], u[s], q[s], np[s]);
+    if (par_ene) par_ene[s] = ene * (double)rqm[s] * dvol;
+  }
+  if (fld_ene) orc_field_energy(g, e, b, dvol, fld_ene);
+  update_periodic_from(g, j, 3, 1, 1);
+  orc_yee_advance_open(g, dx, dt, e, b, j, w);
 }

@@ -16,6 +16,8 @@ OSR_BORIS = 0
 OSR_VAY = 1
 OSR_PATH_FUSED = 0
 OSR_PATH_PLAIN = 1
+OSR_BND_PERIODIC = 0
+OSR_BND_OPEN = 1
 
 P = C.c_void_p
 I = C.c_int
@@ -78,7 +80,8 @@ SIGNATURES = {
 class SimDesc(C.Structure):
     _fields_ = [("dims", C.c_int), ("nx", C.c_int * 3), ("dx", C.c_double * 3),
                 ("dt", C.c_double), ("order", C.c_int), ("tile", C.c_int * 3),
-                ("path", C.c_int), ("kernel", C.c_int), ("reserved", C.c_int * 2)]
+                ("path", C.c_int), ("kernel", C.c_int), ("xbnd", C.c_int),
+                ("reserved", C.c_int)]
 
 
 class SmoothParams(C.Structure):

@@ -175,6 +175,7 @@ struct osr_sim {
   int sm_order[3] = {0, 0, 0};  // current smoothing per direction (0: none)
   float sm_coef[3][31] = {};
   DBuf sm_buf[2];
+  DBuf wall;  // Lindman wall buffers of open x walls (FieldDesc::wall)
   Ctrl* h_ctrl = nullptr;  // pinned mirror
   int64_t desc_version = 0;  // bumped whenever the device descriptors change
 };
@@ -288,6 +289,11 @@ void make_field_desc(const osr_sim* s, FieldDesc& f) {
     f.sm_dir[k] = d;
     f.sm_n[k] = s->sm_order[d];
     for (int c = 0; c < 31; c++) f.sm_coef[k][c] = s->sm_coef[d][c];
+  }
+  if (s->g.open_nx) {  // new_lindman, emf/boundary/os-emf-lindman.f90:235-241
+    f.wall = s->wall.as<float>();
+    f.wall_dc = (float)((1.0 - s->dt / s->dx[0]) / (1.0 + s->dt / s->dx[0]));
+    for (int d = 0; d < 2; d++) f.wall_dt[d] = (float)(s->dt / s->dx[d]) * 0.5f;
   }
   if (f.sm_np > 0) {
     f.sm_src = s->plain ? 1 : 0;
@@ -667,10 +673,29 @@ int launch_fill(osr_sim* s) {
   return OSR_OK;
 }
 
+// E and B on the x walls from the Lindman buffers (the state every push starts from)
+int launch_wall_restore(osr_sim* s) {
+  const unsigned nb = (unsigned)ceil_div(s->g.ny, 256);
+  hipLaunchKernelGGL(wall_restore, dim3(nb), dim3(256), 0, s->stream, s->d_fd.as<FieldDesc>());
+  HIPC(hipGetLastError());
+  return OSR_OK;
+}
+
 int launch_yee(osr_sim* s) {
   const FieldDesc* fd = s->d_fd.as<FieldDesc>();
   const int2* w = s->d_ywork.as<int2>();
   const int2* we = s->d_ework.as<int2>();
+  if (s->g.open_nx) {  // advance_yee with Lindman x walls (os-emf-solver-yee.f03:542-568)
+    const unsigned nb = (unsigned)ceil_div(s->g.ny, 256);
+    hipLaunchKernelGGL((yee_dbdt<2, true>), dim3(s->nblk_e), dim3(256), 0, s->stream, fd, we);
+    hipLaunchKernelGGL(wall_b<1>, dim3(nb), dim3(256), 0, s->stream, fd);
+    hipLaunchKernelGGL((yee_dedt<2>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
+    hipLaunchKernelGGL(wall_e, dim3(nb), dim3(256), 0, s->stream, fd);
+    hipLaunchKernelGGL((yee_dbdt<2, false>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
+    hipLaunchKernelGGL(wall_b<2>, dim3(nb), dim3(256), 0, s->stream, fd);
+    HIPC(hipGetLastError());
+    return launch_wall_restore(s);
+  }
   if (s->dims == 2) {
     hipLaunchKernelGGL((yee_dbdt<2, true>), dim3(s->nblk_e), dim3(256), 0, s->stream, fd, we);
     hipLaunchKernelGGL((yee_dedt<2>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
@@ -836,6 +861,11 @@ int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
   if (desc->path == OSR_PATH_PLAIN && dims != 2)
     return fail(OSR_ERR_ARG, "the plain path is 2D only");
   if (desc->kernel < 0 || desc->kernel > 2) return fail(OSR_ERR_ARG, "unknown kernel variant");
+  if (desc->xbnd != OSR_BND_PERIODIC && desc->xbnd != OSR_BND_OPEN)
+    return fail(OSR_ERR_ARG, "unknown x boundary type");
+  const bool open_x = desc->xbnd == OSR_BND_OPEN;
+  if (open_x && (dims != 2 || desc->path != OSR_PATH_FUSED))
+    return fail(OSR_ERR_ARG, "open x walls need the 2D fused path");
   std::unique_ptr<osr_sim> s(new osr_sim());
   s->desc = *desc;
   s->plain = desc->path == OSR_PATH_PLAIN;
@@ -852,6 +882,16 @@ int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
   GridDesc& g = s->g;
   g.dims = dims;
   g.nx = desc->nx[0];
+  if (open_x) {
+    // guard band of whole tiles: the reference's guard cells of the order
+    // (get_min_gc_part, particles/os-particles.f03:321-360) fit with room to spare
+    const int gc = s->order == 1 ? 3 : (s->order == 2 ? 5 : 7);
+    if (desc->nx[0] % tile[0] != 0)
+      return fail(OSR_ERR_ARG, "open x walls need nx to be a multiple of the tile width");
+    g.open_nx = desc->nx[0];
+    g.nx = desc->nx[0] + ceil_div(gc, tile[0]) * tile[0];
+    if (g.nx > 65535) return fail(OSR_ERR_ARG, "2D grids are limited to 65535 cells per dimension");
+  }
   g.ny = desc->nx[1];
   g.nz = dims == 3 ? desc->nx[2] : 1;
   g.ncell = (long long)g.nx * g.ny * g.nz;
@@ -878,6 +918,11 @@ int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
   TRY(s->B.alloc(3 * nc * 4));
   TRY(s->J.alloc(3 * nc * 8));
   TRY(s->ctrl.alloc(sizeof(Ctrl)));
+  if (open_x) {
+    const size_t wb = sizeof(float) * ((size_t)kWallStale * g.ny + 1);
+    TRY(s->wall.alloc(wb));
+    HIPC(hipMemsetAsync(s->wall.p, 0, wb, s->stream));
+  }
   HIPC(hipMemsetAsync(s->E.p, 0, 3 * nc * 4, s->stream));
   HIPC(hipMemsetAsync(s->B.p, 0, 3 * nc * 4, s->stream));
   HIPC(hipMemsetAsync(s->J.p, 0, 3 * nc * 8, s->stream));
@@ -963,7 +1008,7 @@ int osr_species_set(osr_sim* s, int sid, int64_t np, const int* ix, const float*
   if (np < 0 || np >= (int64_t)UINT32_MAX) return fail(OSR_ERR_ARG, "bad particle count");
   if (np > 0 && (!ix || !x || !u || !q)) return fail(OSR_ERR_ARG, "null particle array");
   const int D = s->dims;
-  const int nxs[3] = {s->g.nx, s->g.ny, s->g.nz};
+  const int nxs[3] = {interior_nx(s->g), s->g.ny, s->g.nz};
   HostParts hp;
   hp.reserve((size_t)np);
   for (int64_t i = 0; i < np; i++) {
@@ -1044,7 +1089,7 @@ int osr_species_generate(osr_sim* s, int sid, const int* ppc, double density,
   std::vector<int> cnt(s->ntiles);
   for (int t = 0; t < s->ntiles; t++) {
     const int tx = t % s->g.ntx, ty = (t / s->g.ntx) % s->g.nty, tz = t / (s->g.ntx * s->g.nty);
-    const int wx = std::min(s->g.tx, s->g.nx - tx * s->g.tx);
+    const int wx = std::max(0, std::min(s->g.tx, interior_nx(s->g) - tx * s->g.tx));
     const int wy = std::min(s->g.ty, s->g.ny - ty * s->g.ty);
     const int wz = s->dims == 3 ? std::min(s->g.tz, s->g.nz - tz * s->g.tz) : 1;
     cnt[t] = wx * wy * wz * tot;
@@ -1077,13 +1122,30 @@ int osr_species_generate(osr_sim* s, int sid, const int* ppc, double density,
   return OSR_OK;
 }
 
+// open x walls: the wall values of the buffers, as the reference's boundary update puts
+// them before every push (also after the host changed the fields)
+static int restore_walls(osr_sim* s) {
+  if (!s->g.open_nx) return OSR_OK;
+  TRY(launch_wall_restore(s));
+  HIPC(hipStreamSynchronize(s->stream));
+  return OSR_OK;
+}
+
 static int fields_xfer(osr_sim* s, float* e, float* b, const int* ld, const int* gcl, bool set) {
   const int D = s->dims;
+  // nxs: device extents (with open x walls the x ring includes the guard band);
+  // nin: interior cells
   const int nxs[3] = {s->g.nx, s->g.ny, s->g.nz};
+  const int nin[3] = {interior_nx(s->g), s->g.ny, s->g.nz};
   for (int d = 0; d < D; d++)
-    if (ld[d] < nxs[d] + gcl[d] || gcl[d] < 0) return fail(OSR_ERR_ARG, "bad field extents");
+    if (ld[d] < nin[d] + gcl[d] || gcl[d] < 0) return fail(OSR_ERR_ARG, "bad field extents");
   const size_t nc = (size_t)s->g.ncell;
   std::vector<float> pe(3 * nc), pb(3 * nc);
+  if (set && s->g.open_nx) {  // the guard band keeps its values
+    HIPC(hipStreamSynchronize(s->stream));
+    HIPC(hipMemcpy(pe.data(), s->E.p, 12 * nc, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(pb.data(), s->B.p, 12 * nc, hipMemcpyDeviceToHost));
+  }
   const long ld0 = ld[0], ld1 = ld[1];
   auto ref = [&](int i, int j, int k) -> size_t {
     size_t o = (size_t)(i + gcl[0]) + (size_t)ld0 * (size_t)(j + gcl[1]);
@@ -1091,9 +1153,9 @@ static int fields_xfer(osr_sim* s, float* e, float* b, const int* ld, const int*
     return 3 * o;
   };
   if (set) {
-    for (int k = 0; k < nxs[2]; k++)
-      for (int j = 0; j < nxs[1]; j++)
-        for (int i = 0; i < nxs[0]; i++) {
+    for (int k = 0; k < nin[2]; k++)
+      for (int j = 0; j < nin[1]; j++)
+        for (int i = 0; i < nin[0]; i++) {
           const size_t c = (size_t)i + (size_t)nxs[0] * (j + (size_t)nxs[1] * k);
           const size_t r = ref(i, j, k);
           for (int m = 0; m < 3; m++) {
@@ -1104,12 +1166,13 @@ static int fields_xfer(osr_sim* s, float* e, float* b, const int* ld, const int*
     HIPC(hipStreamSynchronize(s->stream));
     if (e) HIPC(hipMemcpy(s->E.p, pe.data(), 12 * nc, hipMemcpyHostToDevice));
     if (b) HIPC(hipMemcpy(s->B.p, pb.data(), 12 * nc, hipMemcpyHostToDevice));
-    return OSR_OK;
+    return restore_walls(s);
   }
   HIPC(hipStreamSynchronize(s->stream));
   HIPC(hipMemcpy(pe.data(), s->E.p, 12 * nc, hipMemcpyDeviceToHost));
   HIPC(hipMemcpy(pb.data(), s->B.p, 12 * nc, hipMemcpyDeviceToHost));
-  // every array cell (guards included) gets its periodic image
+  // every array cell (guards included) gets its periodic image; with open x walls the x
+  // guard cells come from the guard band of the ring
   const int ldk = D == 3 ? ld[2] : 1;
   for (int k = 0; k < ldk; k++)
     for (int j = 0; j < ld[1]; j++)
@@ -1150,6 +1213,18 @@ int osr_fields_fill(osr_sim* s, const double* e0, const double* b0) {
     for (int m = 0; m < 3; m++) std::fill(v.begin() + m * nc, v.begin() + (m + 1) * nc, (float)f[m]);
     HIPC(hipMemcpy(w == 0 ? s->E.p : s->B.p, v.data(), 12 * nc, hipMemcpyHostToDevice));
   }
+  return restore_walls(s);
+}
+
+// particles left after absorbing walls removed some: the sum of the tile counts
+static int refresh_counts(osr_sim* s) {
+  for (auto& p : s->sp) {
+    std::vector<int> cnt((size_t)s->ntiles);
+    HIPC(hipMemcpy(cnt.data(), p->cnt.p, sizeof(int) * cnt.size(), hipMemcpyDeviceToHost));
+    int64_t n = 0;
+    for (int c : cnt) n += c;
+    p->np = n;
+  }
   return OSR_OK;
 }
 
@@ -1185,6 +1260,7 @@ int osr_sim_run(osr_sim* s, int nsteps) {
       batch = std::min(batch * 2, 256);
     }
   }
+  if (s->g.open_nx) TRY(refresh_counts(s));
   return OSR_OK;
 }
 
@@ -1240,6 +1316,11 @@ int osr_sim_save_field(osr_sim* s, int quant, const char* ms_dir, const double* 
   const DBuf& src = quant < 3 ? s->E : s->B;
   HIPC(hipMemcpy(v.data(), src.as<float>() + (size_t)(quant % 3) * nc, 4 * nc,
                  hipMemcpyDeviceToHost));
+  if (s->g.open_nx) {  // interior columns of the x ring only
+    const int nxi = s->g.open_nx, nxr = s->g.nx;
+    for (size_t r = 0; r < nc / nxr; r++)
+      std::copy(v.begin() + r * nxr, v.begin() + r * nxr + nxi, v.begin() + r * nxi);
+  }
   // os-emf-diag-define.f03:164-190: names, labels and units of the field reports
   static const char* qname[6] = {"e1", "e2", "e3", "b1", "b2", "b3"};
   static const char* qlabel[6] = {"E_1", "E_2", "E_3", "B_1", "B_2", "B_3"};
@@ -1251,7 +1332,7 @@ int osr_sim_save_field(osr_sim* s, int quant, const char* ms_dir, const double* 
   gi.ndims = s->dims;
   gi.label = qlabel[quant];
   gi.units = "m_e c \\omega_p e^{-1}";
-  const int nxs[3] = {s->g.nx, s->g.ny, s->g.nz};
+  const int nxs[3] = {interior_nx(s->g), s->g.ny, s->g.nz};
   for (int d = 0; d < s->dims; d++) {
     gi.count[d] = (uint64_t)nxs[d];
     const double x0 = xmin ? xmin[d] : 0.0;
@@ -1268,6 +1349,7 @@ int osr_sim_save_field(osr_sim* s, int quant, const char* ms_dir, const double* 
 
 int osr_sim_gauss(osr_sim* s, double* res, double* rho_out, double* maxabs) {
   TRY(check_sim(s));
+  if (s->g.open_nx) return fail(OSR_ERR_STATE, "the Gauss residual needs periodic x walls");
   const size_t nc = (size_t)s->g.ncell;
   DBuf rho, dres;
   TRY(rho.alloc(8 * nc));
@@ -1891,6 +1973,11 @@ int osr_smooth_vdf(int dims, const int* ld, int fdim, const int* order, const fl
 int osr_sim_set_smooth(osr_sim* s, const int* order, const float* coef) {
   TRY(check_sim(s));
   TRY(check_smooth_args(s->dims, order, coef));
+  if (s->g.open_nx) {
+    bool any = false;
+    for (int d = 0; d < s->dims; d++) any = any || order[d] > 0;
+    if (any) return fail(OSR_ERR_STATE, "current smoothing needs periodic x walls");
+  }
   HIPC(hipStreamSynchronize(s->stream));
   bool any = false;
   for (int d = 0; d < 3; d++) {

@@ -476,6 +476,7 @@ int osr_ens_submit(osr_ensemble* e, const osr_job_desc* job, int64_t* job_id) {
   TRY(check_ens(e));
   if (!job || !job_id) return fail(OSR_ERR_ARG, "null argument");
   if (job->sim.path != OSR_PATH_FUSED) return fail(OSR_ERR_ARG, "ensemble jobs use the fused path");
+  if (job->sim.xbnd != OSR_BND_PERIODIC) return fail(OSR_ERR_ARG, "ensemble jobs are periodic");
   if (job->nspecies < 1 || job->nspecies > kMaxSpecies) return fail(OSR_ERR_ARG, "1..8 species");
   if (job->nsteps < 1) return fail(OSR_ERR_ARG, "nsteps must be >= 1");
   if (job->sim.dims != 2 && job->sim.dims != 3) return fail(OSR_ERR_ARG, "dims must be 2 or 3");

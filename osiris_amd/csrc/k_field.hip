@@ -45,7 +45,16 @@ struct FieldDesc {
   int jsrc_clear;
   float sm_coef[3][31];  // per pass, s(k) at [15 + k]
   float* sm_buf[2];
+  // open x walls (g.open_nx > 0): Lindman buffers, planar, ny values per entry:
+  // ebl[3][2] (x = 0, 1), bbl[3] (x = 0), ebu[3][2] (x = nx+1, nx+2), then one value:
+  // E3(x = 0, y = 1) as it was when the step began
+  float* wall;
+  float wall_dc;     // real((1 - dt/dx1) / (1 + dt/dx1))
+  float wall_dt[2];  // real(dt/dx) * 0.5
 };
+
+// offsets in FieldDesc::wall (units of ny values)
+constexpr int kWallEbl = 0, kWallBbl = 6, kWallEbu = 9, kWallStale = 15;
 
 OSR_DEV int wrp(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
@@ -82,7 +91,11 @@ __global__ __launch_bounds__(256) void yee_dbdt(const FieldDesc* __restrict__ fd
     float* B3 = g.B + 2 * nc;
     const float e1 = E1[idx], e2 = E2[idx], e3 = E3[idx];
     float b1 = B1[idx], b2 = B2[idx], b3 = B3[idx];
-    if (ENERGY) {
+    // open x walls: the reference advances B on x = 0 .. nx+1 only (dbdt_2d loops), so
+    // the guard band keeps its values elsewhere; the energy sums the interior
+    const int nxi = interior_nx(g);
+    const bool live = i <= nxi || i == nx - 1;
+    if (ENERGY && i < nxi) {
       acc[0] += (double)b1 * b1;
       acc[1] += (double)b2 * b2;
       acc[2] += (double)b3 * b3;
@@ -102,9 +115,11 @@ __global__ __launch_bounds__(256) void yee_dbdt(const FieldDesc* __restrict__ fd
       b2 = b2 + d1 * (E3[ip] - e3) - d3 * (E1[kp] - e1);
       b3 = b3 - d1 * (E2[ip] - e2) + d2 * (E1[jp] - e1);
     }
-    B1[idx] = b1;
-    B2[idx] = b2;
-    B3[idx] = b3;
+    if (live) {
+      B1[idx] = b1;
+      B2[idx] = b2;
+      B3[idx] = b3;
+    }
   }
   if (ENERGY) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -188,9 +203,103 @@ __global__ __launch_bounds__(256) void yee_dedt(const FieldDesc* __restrict__ fd
     e2 = e2 - dtif * j2 + d3 * (b1 - B1[km]) - d1 * (b3 - B3[im]);
     e3 = e3 - dtif * j3 + d1 * (b2 - B2[im]) - d2 * (b1 - B1[jm]);
   }
+  // open x walls: E is advanced on x = 1 .. nx+1 only (dedt_2d loops); the current of
+  // every column is still converted and cleared above
+  if (i > interior_nx(g)) return;
   E1[idx] = e1;
   E2[idx] = e2;
   E3[idx] = e3;
+}
+
+// ------------------------------------------------------------------------------------
+// Lindman absorbing walls at x = 0 (ring column nx-1) and x = nx+1, nx+2 (columns
+// open_nx, open_nx+1), 2D, one thread per row y.  emf/boundary/os-emf-lindman.f90:
+// update_lindman_2d_e :454-535 (after the E step), update_lindman_2d_b :539-600 (after
+// each B half step), update_emfbound_lindman :861-941 (wall values restored before the
+// next push).  The wall buffers start at zero (vdf/os-wall.f90:189-205).
+// ------------------------------------------------------------------------------------
+OSR_DEV float* wall_at(const FieldDesc& fd, int k, int ny) { return fd.wall + (size_t)k * ny; }
+
+// B buffer of the lower wall.  STEP 2: the reference reads E3(0, ny+1), a guard cell that
+// nothing refreshes between the start of the step and the second B half step, so it still
+// holds E3(0, 1) of the start of the step (saved by wall_e).
+template <int STEP>
+__global__ __launch_bounds__(256) void wall_b(const FieldDesc* __restrict__ fds) {
+  const FieldDesc& fd = *fds;
+  const GridDesc& g = fd.g;
+  if (g.ctrl->halt >= 2) return;
+  const int ny = g.ny, nx = g.nx;
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= ny) return;
+  const long long nc = g.ncell;
+  const float* E1 = g.E;
+  const float* E2 = g.E + nc;
+  const float* E3 = g.E + 2 * nc;
+  const int rn = wrp(r + 1, ny);
+  const long long c0 = (long long)r * nx + nx - 1, c1 = (long long)r * nx;
+  const long long n0 = (long long)rn * nx + nx - 1;
+  const float e3n = (STEP == 2 && r == ny - 1) ? wall_at(fd, kWallStale, ny)[0] : E3[n0];
+  float* bb = wall_at(fd, kWallBbl, ny);
+  const float d1 = fd.wall_dt[0], d2 = fd.wall_dt[1];
+  bb[r] = bb[r] - d2 * (e3n - E3[c0]);
+  bb[ny + r] = bb[ny + r] + d1 * (E3[c1] - E3[c0]);
+  bb[2 * ny + r] = bb[2 * ny + r] - d1 * (E2[c1] - E2[c0]) + d2 * (E1[n0] - E1[c0]);
+}
+
+__global__ __launch_bounds__(256) void wall_e(const FieldDesc* __restrict__ fds) {
+  const FieldDesc& fd = *fds;
+  const GridDesc& g = fd.g;
+  if (g.ctrl->halt >= 2) return;
+  const int ny = g.ny, nx = g.nx, nxi = g.open_nx;
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= ny) return;
+  const long long nc = g.ncell;
+  const float dc = fd.wall_dc;
+  float* ebl = wall_at(fd, kWallEbl, ny);
+  float* ebu = wall_at(fd, kWallEbu, ny);
+  if (r == 0) wall_at(fd, kWallStale, ny)[0] = ebl[(2 * 2 + 0) * ny];
+  const long long row = (long long)r * nx;
+#pragma unroll
+  for (int p = 1; p <= 2; p++) {
+    float* E = g.E + p * nc + row;
+    // lower wall: E(0) = buf(1) + dc (buf(0) - E(1)); buffers <- E(0), E(1)
+    float* b0 = ebl + (2 * p + 0) * ny + r;
+    float* b1 = ebl + (2 * p + 1) * ny + r;
+    const float e1 = E[0];
+    const float v = *b1 + dc * (*b0 - e1);
+    E[nx - 1] = v;
+    *b0 = v;
+    *b1 = e1;
+    // upper wall: E(nx+2) = buf(nx+1) + dc (buf(nx+2) - E(nx+1)); buffers <- E(nx+1), E(nx+2)
+    float* u1 = ebu + (2 * p + 0) * ny + r;
+    float* u2 = ebu + (2 * p + 1) * ny + r;
+    const float en = E[nxi];
+    const float w = *u1 + dc * (*u2 - en);
+    E[nxi + 1] = w;
+    *u1 = en;
+    *u2 = w;
+  }
+}
+
+// the wall values the next push reads: E and B at x = 0 and E at x = nx+1 from the buffers
+// (the E1 entries are never written and stay zero)
+__global__ __launch_bounds__(256) void wall_restore(const FieldDesc* __restrict__ fds) {
+  const FieldDesc& fd = *fds;
+  const GridDesc& g = fd.g;
+  if (g.ctrl->halt >= 2) return;
+  const int ny = g.ny, nx = g.nx, nxi = g.open_nx;
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= ny) return;
+  const long long nc = g.ncell, row = (long long)r * nx;
+  const float* ebl = wall_at(fd, kWallEbl, ny);
+  const float* bbl = wall_at(fd, kWallBbl, ny);
+  const float* ebu = wall_at(fd, kWallEbu, ny);
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    g.E[c * nc + row + nx - 1] = ebl[(2 * c) * ny + r];
+    g.B[c * nc + row + nx - 1] = bbl[c * ny + r];
+    g.E[c * nc + row + nxi] = ebu[(2 * c) * ny + r];
+  }
 }
 
 // Neumaier compensated accumulation

@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void generate_particles(const SpecDesc* __rest
     tz0 = (tile / (g.ntx * g.nty)) * g.tz;
     wz = min(g.tz, g.nz - tz0);
   }
-  wx = min(g.tx, g.nx - tx0);
+  wx = min(g.tx, interior_nx(g) - tx0);
   wy = min(g.ty, g.ny - ty0);
   const int nct = wx * wy * wz;
   const int n = d.cnt[tile];
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void generate_particles(const SpecDesc* __rest
     const int c = p % nct, k = p / nct;
     const int lx = c % wx, ly = (c / wx) % wy, lz = c / (wx * wy);
     const int ix = tx0 + lx, iy = ty0 + ly, iz = tz0 + lz;
-    const long long gcell = ix + (long long)g.nx * (iy + (long long)g.ny * iz);
+    const long long gcell = ix + (long long)interior_nx(g) * (iy + (long long)g.ny * iz);
     const uint64_t id = (uint64_t)gcell * (uint64_t)a.ppc_tot + (uint64_t)k;
     float pos[3];
     if (a.random_pos) {

@@ -646,7 +646,10 @@ __global__ __launch_bounds__(NT) void tile_fill(const SpecDesc* __restrict__ des
     if (lane == 0) s_lofs[0] = 0;
   }
   __syncthreads();
-  const int T = s_lofs[L];
+  // a tile of the guard band of open x walls takes no movers: the particles that left
+  // through a wall (ix < 1 or ix > nx) are dropped (spec/os-spec-boundary.f03:476-513)
+  const bool absorb = (tile % g.ntx) * g.tx >= interior_nx(g);
+  const int T = absorb ? 0 : s_lofs[L];
   const int H = s_hpre[NW];
   // pass 1: collect the incoming movers
   int I = 0;

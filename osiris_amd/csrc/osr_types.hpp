@@ -2,7 +2,12 @@
 //
 // Device data layout of one simulation (all resident in HBM):
 //   fields   E, B : float, planar [3][nz][ny][nx] (interior cells only; periodic
-//                   boundaries are index wraps, so no guard cells are stored);
+//                   boundaries are index wraps, so no guard cells are stored).  With
+//                   open x walls the x extent is a ring of the open_nx interior columns
+//                   followed by a guard band of whole tiles: columns open_nx, open_nx+1,
+//                   ... are the upper guard cells x = nx+1, nx+2, ... and the last
+//                   columns, reached by the same index wrap from column 0, the lower
+//                   guard cells x = 0, -1, ... (1-based cell indices of the reference);
 //   current  J    : int64 fixed point, planar [3][nz][ny][nx]; the value is J * 2^jshift;
 //   particles     : structure of arrays, grouped by tile.  Tile t owns the slots
 //                   [off[t], off[t]+cap[t]) of every array and its first cnt[t] slots hold
@@ -45,6 +50,7 @@ struct GridDesc {
   int tx, ty, tz;     // tile size in cells
   int ntx, nty, ntz;  // tiles per dimension
   int ntiles;
+  int open_nx;  // 0: x periodic; else open x walls and this many interior columns (< nx)
   float* E;
   float* B;
   unsigned long long* J;
@@ -106,6 +112,11 @@ __host__ __device__ inline void unpack3(uint32_t c, int& ix, int& iy, int& iz) {
   ix = (int)(c & 0x7FFu);
   iy = (int)((c >> 11) & 0x7FFu);
   iz = (int)(c >> 22);
+}
+
+// interior cells along x (the ring without the guard band of open x walls)
+__host__ __device__ inline int interior_nx(const GridDesc& g) {
+  return g.open_nx ? g.open_nx : g.nx;
 }
 
 __host__ __device__ inline int tile_of(const GridDesc& g, uint32_t c) {

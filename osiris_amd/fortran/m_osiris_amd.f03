@@ -21,6 +21,9 @@ module m_osiris_amd
                                        OSR_ERR_NOMEM = 3, OSR_ERR_STATE = 4, OSR_ERR_LAYOUT = 5
   integer(c_int), parameter, public :: OSR_BORIS = 0, OSR_VAY = 1
   integer(c_int), parameter, public :: OSR_PATH_FUSED = 0, OSR_PATH_PLAIN = 1
+  ! x walls: periodic, or Lindman absorbing fields + absorbing particles (emf_bound
+  ! "lindman", spec_bound "absorbing")
+  integer(c_int), parameter, public :: OSR_BND_PERIODIC = 0, OSR_BND_OPEN = 1
 
   type, bind(c), public :: osr_sim_desc
     integer(c_int) :: dims = 2
@@ -31,7 +34,8 @@ module m_osiris_amd
     integer(c_int) :: tile(3) = 0
     integer(c_int) :: path = OSR_PATH_FUSED
     integer(c_int) :: kernel = 0
-    integer(c_int) :: reserved(2) = 0
+    integer(c_int) :: xbnd = OSR_BND_PERIODIC
+    integer(c_int) :: reserved = 0
   end type osr_sim_desc
 
   ! current smoothing (vdf/os-vdf-smooth.f90): the reference's filter type codes and the

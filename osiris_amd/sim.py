@@ -14,7 +14,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import OSR_BORIS, OSR_SMOOTH, OSR_VAY, SimDesc, SmoothParams, check, lib
+from ._lib import (OSR_BND_OPEN, OSR_BND_PERIODIC, OSR_BORIS, OSR_SMOOTH, OSR_VAY, SimDesc,
+                   SmoothParams, check, lib)
 
 
 def _p(a):
@@ -112,10 +113,14 @@ def smooth_vdf(dims, f, order, coef):
 class Simulation:
     """A resident simulation on the GPU (one per grid; species share the grid)."""
 
-    def __init__(self, dims, nx, dx, dt, order, tile=None, plain=False, kernel=0):
+    def __init__(self, dims, nx, dx, dt, order, tile=None, plain=False, kernel=0,
+                 xbnd="periodic"):
+        """xbnd: "periodic", or "open" for Lindman absorbing field walls and absorbing
+        particles at both x ends (2D fused path, nx a multiple of the tile width)."""
         d = SimDesc()
         d.path = 1 if plain else 0
         d.kernel = kernel
+        d.xbnd = {"periodic": OSR_BND_PERIODIC, "open": OSR_BND_OPEN}[xbnd]
         d.dims = dims
         for i in range(3):
             d.nx[i] = nx[i] if i < dims else 1

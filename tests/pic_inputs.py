@@ -42,3 +42,36 @@ def thermal_plasma(rng, dims, nx, ppc, uth, ufl=(0.0, 0.0, 0.0), density=1.0, rq
     u = (np.array(ufl) + uth * rng.normal(0, 1, (n, 3))).astype(np.float32)
     q = np.full(n, np.sign(rqm) * density / ppc, np.float32)
     return ix, x, u, q
+
+
+def vacuum_pulse(nx, ny, dx, k0=4.0, width=2.0, direction=1, dtype=np.float64):
+    """Interior E, B (shape (ny, nx, 3)) of a y-uniform light pulse travelling along
+    +x (direction 1) or -x (-1): E2 = f(x) on the Yee E2 points x_i, B3 = +-f on the B3
+    points x_{i+1/2}, f = exp(-((x - x0)/width)^2) cos(k0 (x - x0)) centred in the box."""
+    xi = np.arange(1, nx + 1) * dx
+    x0 = nx * dx / 2
+
+    def f(x):
+        return np.exp(-((x - x0) / width) ** 2) * np.cos(k0 * (x - x0))
+
+    e = np.zeros((ny, nx, 3), dtype)
+    b = np.zeros((ny, nx, 3), dtype)
+    e[..., 1] = f(xi)[None, :]
+    b[..., 2] = (1.0 if direction > 0 else -1.0) * f(xi + dx / 2)[None, :]
+    return e, b
+
+
+def mur_reflection(kdx, courant):
+    """Reflection coefficient of the first-order Mur wall E_0^{n+1} = E_1^n +
+    c (E_0^n - E_1^{n+1}), c = (1 - S)/(1 + S), on the 1D Yee lattice (waves along x on
+    the 2D lattice with k_y = 0), for a monochromatic wave of wavenumber k dx: with
+    E_i^n = z^n (e^{i k i} + R e^{-i k i}), z = e^{i w dt} and the lattice dispersion
+    sin(w dt/2) = S sin(k dx/2), the wall condition gives R = (e^{ik}(1 - cz) - (z - c)) /
+    ((z - c) - e^{-ik}(1 - cz))."""
+    k = np.asarray(kdx, np.float64)
+    S = courant
+    z = np.exp(2j * np.arcsin(S * np.sin(k / 2)))
+    c = (1 - S) / (1 + S)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        r = (np.exp(1j * k) * (1 - c * z) - (z - c)) / ((z - c) - np.exp(-1j * k) * (1 - c * z))
+    return np.nan_to_num(r)
