@@ -70,6 +70,7 @@ def _lib(prec: str):
         L.orc_yee_advance_open.argtypes = [P, P, C.c_double, P, P, P, P]
         L.orc_particles_open.argtypes = [P, P, P, P, P, C.c_int]
         L.orc_particles_open.restype = C.c_int
+        L.orc_zpulse_wall_2d.argtypes = [P, P, C.c_int, P, C.c_double, C.c_double, P]
         L._real = np.float64 if prec == "f64" else np.float32
         _LIBS[prec] = L
     return _LIBS[prec]
@@ -101,6 +102,69 @@ class LindmanWalls:
         for d in range(2):
             self.c.dtdxi[d] = r(r(dt / dx[d]) * r(0.5))
         self.c.ebl, self.c.bbl, self.c.ebu = (a.ctypes.data for a in (self.ebl, self.bbl, self.ebu))
+
+
+class ZpulseParams(C.Structure):
+    """orc_zpulse of pic_oracle.c (parameters after read_input_zpulse_wall)."""
+    _fields_ = [("a0", C.c_double), ("omega0", C.c_double), ("phase0", C.c_double),
+                ("pol", C.c_double), ("pol_type", C.c_int), ("backward", C.c_int),
+                ("tenv_type", C.c_int), ("per_type", C.c_int), ("tem_mode", C.c_int),
+                ("pad", C.c_int), ("rise", C.c_double), ("flat", C.c_double),
+                ("fall", C.c_double), ("duration", C.c_double), ("range", C.c_double),
+                ("center", C.c_double), ("w0", C.c_double), ("focus", C.c_double),
+                ("launch_time", C.c_double), ("xmin", C.c_double * 2), ("xmax", C.c_double * 2)]
+
+
+ZPULSE_DEFAULTS = dict(a0=1.0, omega0=10.0, phase=0.0, pol_type=0, pol=90.0,
+                       propagation="forward", tenv_type="polynomial", tenv_fwhm=-1.0,
+                       tenv_rise=0.0, tenv_flat=0.0, tenv_fall=0.0, tenv_duration=0.0,
+                       tenv_range=None, per_type="plane", per_center=None, per_w0=0.0,
+                       per_fwhm=0.0, per_focus=None, per_tem_mode=0, launch_time=0.0)
+
+
+def zpulse_wall_params(nx, dx, xmin=(0.0, 0.0), **kw):
+    """read_input_zpulse_wall (zpulse/os-zpulse-wall.f03:196-628) for direction 1 of a 2D
+    box: the reference's defaults (:196-239) updated by kw; returns ZpulseParams."""
+    unknown = set(kw) - set(ZPULSE_DEFAULTS)
+    if unknown:
+        raise ValueError(f"unknown zpulse_wall parameters {sorted(unknown)}")
+    a = dict(ZPULSE_DEFAULTS, **kw)
+    pi_180 = 0.0174532925199432957692369  # os-math.f90:15
+    p = ZpulseParams()
+    p.a0, p.omega0 = a["a0"], a["omega0"]
+    p.phase0 = a["phase"] * pi_180
+    if a["pol_type"] not in (-1, 0, 1):
+        raise ValueError("pol_type must be -1, 0 or 1")
+    p.pol_type = a["pol_type"]
+    p.pol = a["pol"] * pi_180
+    p.backward = {"forward": 0, "backward": 1}[a["propagation"]]
+    for d in range(2):
+        p.xmin[d] = xmin[d]
+        p.xmax[d] = xmin[d] + nx[d] * dx[d]
+    p.tenv_type = {"polynomial": 0, "sin2": 1, "gaussian": 2}[a["tenv_type"]]
+    if p.tenv_type < 2:
+        if a["tenv_fwhm"] > 0:
+            p.rise, p.flat, p.fall = a["tenv_fwhm"] / 2.0, 0.0, a["tenv_fwhm"] / 2.0
+        else:
+            p.rise, p.flat, p.fall = a["tenv_rise"], a["tenv_flat"], a["tenv_fall"]
+    else:
+        p.duration = (a["tenv_fwhm"] / np.sqrt(2.0 * np.log(2.0)) if a["tenv_fwhm"] > 0
+                      else a["tenv_duration"])
+        if a["tenv_range"] is None:
+            raise ValueError("a gaussian envelope needs tenv_range")
+        p.range = a["tenv_range"]
+    p.per_type = {"plane": 0, "gaussian": 1, "hermite": 1}[a["per_type"]]
+    if p.per_type == 1:
+        p.tem_mode = a["per_tem_mode"]
+        p.w0 = a["per_fwhm"] / np.sqrt(4.0 * np.log(2.0)) if a["per_fwhm"] > 0 else a["per_w0"]
+        if p.w0 <= 0 or a["per_focus"] is None:
+            raise ValueError("a gaussian beam needs per_w0 (or per_fwhm) and per_focus")
+        p.focus = a["per_focus"]
+    p.center = (0.5 * (p.xmin[1] + p.xmax[1]) if a["per_center"] is None else a["per_center"])
+    p.launch_time = a["launch_time"]
+    if p.backward:
+        p.focus = -p.focus
+    return p
 
 
 def ref_available(prec: str = "s") -> bool:
@@ -416,6 +480,7 @@ class Sim:
         self.b = self.grid.zeros(3, r)
         self.j = self.grid.zeros(3, r)
         self.walls = None
+        self.zpulses = []
         if self.open_x:
             if self.dims != 2 or self.smooth_order is not None:
                 raise ValueError("open x walls are implemented for 2D without smoothing")
@@ -481,6 +546,11 @@ class Sim:
         us = IA(*[s.u.ctypes.data for s in self.species])
         qs = IA(*[s.q.ctypes.data for s in self.species])
         for _ in range(n):
+            # zpulse launch at t = n dt before the step (os-simulation.f03:528-531)
+            t = len(self.fld_hist) * self.dt
+            for zp in self.zpulses:
+                L.orc_zpulse_wall_2d(C.byref(g), C.byref(zp), self.order, _p(dxa), self.dt, t,
+                                     _p(self.b))
             fe = np.zeros(6)
             pe = np.zeros(ns)
             L.orc_step_open(C.byref(g), self.order, _p(dxa), self.dt, _p(self.e), _p(self.b),
@@ -490,6 +560,13 @@ class Sim:
             self.par_hist.append(pe)
         for s, k in zip(self.species, npa):  # leading slices stay contiguous
             s.ix, s.x, s.u, s.q = s.ix[:k], s.x[:k], s.u[:k], s.q[:k]
+
+    def add_zpulse_wall(self, xmin=(0.0, 0.0), **kw):
+        """A zpulse_wall section (direction 1); needs open x walls (os-zpulse-wall.f03:
+        600-608)."""
+        if not self.open_x:
+            raise ValueError("the launch wall must be open")
+        self.zpulses.append(zpulse_wall_params(self.nx, self.dx, xmin, **kw))
 
     def wall_refresh(self):
         """The boundary update the next step starts with (Lindman restore, periodic y), so

@@ -1224,3 +1224,126 @@ void orc_step_open(const orc_grid *g, int order, const double *dx, double dt, re
   update_periodic_from(g, j, 3, 1, 1);
   orc_yee_advance_open(g, dx, dt, e, b, j, w);
 }
+
+/* ------------------------------------------------------------------------------------
+ * Laser injection from an x wall, 2D (zpulse_wall, zpulse/os-zpulse-wall.f03).
+ * orc_zpulse holds the parameters after read_input_zpulse_wall (:90-628): angles in
+ * radians, envelope lengths resolved from fwhm, per_focus negated for backward pulses.
+ * orc_zpulse_wall_2d is launch_zpulse_wall (:633-700) at time t: nothing outside
+ * launch_time .. launch_time + duration; otherwise launch_em_wall_2d (:865-960) adds
+ * real(b12), real(b3) to B2, B3 of column x = 1 (forward) or nx (backward) for
+ * ip = 0 .. ny+2, twice with the second launch's phase and polarisation for pol_type != 0.
+ * ------------------------------------------------------------------------------------ */
+typedef struct {
+  double a0, omega0, phase0, pol;
+  int pol_type, backward, tenv_type, per_type, tem_mode, pad;
+  double rise, flat, fall, duration, range;
+  double center, w0, focus, launch_time;
+  double xmin[2], xmax[2];
+} orc_zpulse;
+
+static const double ORC_PI_2 = 1.57079632679489661923132; /* os-math.f90:14 */
+
+/* fenv_poly, os-math.f90:83-128 */
+static double zp_fenv(double tt) {
+  return 10 * (tt * tt * tt) - 15 * (tt * tt * tt * tt) + 6 * (tt * tt * tt * tt * tt);
+}
+static double zp_poly(double x, double rise, double flat, double fall) {
+  double length = rise + flat + fall;
+  if (x < 0.0) return 0.0;
+  if (x < rise) return zp_fenv(x / rise);
+  if (x < rise + flat) return 1.0;
+  if (x < length) return zp_fenv((length - x) / fall);
+  return 0.0;
+}
+/* fenv_sin2, os-math.f90:132-165 */
+static double zp_sin2(double x, double rise, double flat, double fall) {
+  double length = rise + flat + fall, v;
+  if (x < 0.0) return 0.0;
+  if (x < rise) {
+    v = sin(ORC_PI_2 * x / rise);
+    return v * v;
+  }
+  if (x < rise + flat) return 1.0;
+  if (x < length) {
+    v = sin(ORC_PI_2 * (length - x) / fall);
+    return v * v;
+  }
+  return 0.0;
+}
+/* hermite_double, os-math.f90:478-509 */
+static double zp_hermite(int m, double u) {
+  double u2 = u * u;
+  switch (m) {
+    case 0: return 1.0;
+    case 1: return 2.0 * u;
+    case 2: return 4.0 * u2 - 2.0;
+    case 3: return 8.0 * (u2 * u) - 12.0 * u;
+    case 4: return 16.0 * (u2 * u * u) - 48.0 * u2 + 12.0;
+    case 5: return 32.0 * (u2 * u * u * u) - 160.0 * (u2 * u) + 120.0 * u;
+    case 6: return 64.0 * (u2 * u * u * u * u) - 480.0 * (u2 * u * u) + 720.0 * u2 - 120.0;
+    case 7:
+      return 128.0 * (u2 * u * u * u * u * u) - 1344.0 * (u2 * u * u * u) + 3360.0 * (u2 * u) -
+             1680.0 * u;
+  }
+  return 0.0;
+}
+/* t_env_zpulse_wall :705-746 (types 0 polynomial, 1 sin2, 2 gaussian) */
+static double zp_tenv(const orc_zpulse *p, double t) {
+  if (p->tenv_type == 0) return zp_poly(t, p->rise, p->flat, p->fall);
+  if (p->tenv_type == 1) return zp_sin2(t, p->rise, p->flat, p->fall);
+  double a = (t - 0.5 * p->range) / p->duration;
+  return exp(-2 * (a * a));
+}
+/* per_envelope_2d_zpulse (types 0 plane, 1 hermite_gaussian) with tin_sim absent (t = 0)
+ * and lon_center_zpulse_wall :781-805 (moves with the injection time) */
+static double zp_per(const orc_zpulse *p, double phase0, double x1, double x2, double inj) {
+  double zc = (p->tenv_type == 2 ? 0.5 * p->range : p->rise + p->flat / 2) + inj;
+  double k = p->omega0, ps = p->backward ? -1.0 : 1.0, t = 0.0;
+  if (p->per_type == 0) return cos(k * (x1 - zc - ps * t) + phase0);
+  double z0 = k * (p->w0 * p->w0) * 0.5;
+  double rho = x2 - p->center, rho2 = rho * rho, z = x1 - p->focus;
+  double rwl2 = 1, curv = 0, gouy = 0;
+  if (z != 0) {
+    rwl2 = (z0 * z0) / ((z0 * z0) + (z * z));
+    curv = 0.5 * rho2 * z / ((z * z) + (z0 * z0));
+    gouy = (p->tem_mode + 1) * atan2(z, z0);
+  }
+  double rwu = sqrt(2 * rwl2) / p->w0;
+  return sqrt(sqrt(rwl2)) * zp_hermite(p->tem_mode, rho * rwu) *
+         exp(-(rho2 * rwl2 / (p->w0 * p->w0))) * cos(k * (x1 - zc - ps * t) + k * curv - gouy + phase0);
+}
+
+static void zp_launch(const orc_grid *g, const orc_zpulse *p, int order, const double *dx,
+                      double dt, double t, double phase0, double pol, real *b) {
+  int inpos = p->backward ? g->nx[0] : 1;
+  double ps = p->backward ? -1.0 : 1.0;
+  double z = p->backward ? -p->xmax[0] : p->xmin[0];
+  double inj = t - p->launch_time;
+  double rdtdx = (double)(real)(dt / dx[0]);
+  double rmin = p->xmin[1];
+  double cos_pol = cos(pol) * ps, sin_pol = -sin(pol) * ps;
+  double amp = 2.0 * p->omega0 * p->a0, dr_2 = dx[1] / 2.0;
+  if (order == 2) {
+    z = z + dx[0] * 0.5;
+    rmin = rmin + dr_2;
+  }
+  double lenv = amp * zp_tenv(p, inj);
+  for (int ip = 0; ip <= g->nx[1] + 2; ip++) {
+    double r = rmin + (ip - 1) * dx[1], r_2 = r + dr_2;
+    double b12 = lenv * rdtdx * zp_per(p, phase0, z, r, inj) * sin_pol;
+    double b3 = lenv * rdtdx * zp_per(p, phase0, z, r_2, inj) * cos_pol;
+    FLD(b, g, 1, inpos, ip, 1) = FLD(b, g, 1, inpos, ip, 1) + (real)b12;
+    FLD(b, g, 2, inpos, ip, 1) = FLD(b, g, 2, inpos, ip, 1) + (real)b3;
+  }
+}
+
+void orc_zpulse_wall_2d(const orc_grid *g, const orc_zpulse *p, int order, const double *dx,
+                        double dt, double t, real *b) {
+  double dur = p->tenv_type == 2 ? p->range : p->rise + p->flat + p->fall;
+  if (t > p->launch_time + dur || !(t >= p->launch_time)) return;
+  zp_launch(g, p, order, dx, dt, t, p->phase0, p->pol, b);
+  if (p->pol_type != 0)
+    zp_launch(g, p, order, dx, dt, t, p->phase0 + (p->pol_type > 0 ? ORC_PI_2 : -ORC_PI_2),
+              p->pol + ORC_PI_2, b);
+}

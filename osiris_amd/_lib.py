@@ -73,8 +73,28 @@ SIGNATURES = {
     "osr_zdf_write_grid_file": (I, [P, P, I, P, P]),
     "osr_zdf_save_particles": (I, [P, P, P, P]),
     "osr_sim_save_field": (I, [P, I, P, P]),
+    "osr_zpulse_wall_defaults": (None, [P]),
+    "osr_sim_add_zpulse_wall": (I, [P, P]),
+    "osr_sim_zpulse_column": (I, [P, I, I, C.c_double, P, P, P]),
     "osr_copy_probe": (I, [I64, I, P]),
 }
+
+
+class ZpulseWall(C.Structure):
+    """osr_zpulse_wall"""
+    _fields_ = [("a0", C.c_double), ("omega0", C.c_double), ("phase", C.c_double),
+                ("pol", C.c_double), ("pol_type", C.c_int), ("propagation", C.c_int),
+                ("tenv_type", C.c_int), ("per_type", C.c_int), ("tenv_fwhm", C.c_double),
+                ("tenv_rise", C.c_double), ("tenv_flat", C.c_double), ("tenv_fall", C.c_double),
+                ("tenv_duration", C.c_double), ("tenv_range", C.c_double),
+                ("per_center", C.c_double), ("per_w0", C.c_double), ("per_fwhm", C.c_double),
+                ("per_focus", C.c_double), ("per_tem_mode", C.c_int), ("reserved", C.c_int),
+                ("launch_time", C.c_double), ("xmin", C.c_double * 2)]
+
+
+OSR_TENV = {"polynomial": 0, "sin2": 1, "gaussian": 2}
+OSR_PER = {"plane": 0, "gaussian": 1, "hermite": 1}
+OSR_PROP = {"forward": 0, "backward": 1}
 
 
 class SimDesc(C.Structure):

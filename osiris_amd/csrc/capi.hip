@@ -55,6 +55,7 @@ static int fail(int code, const char* fmt, ...) {
   } while (0)
 
 #include "output.hip"
+#include "zpulse.hip"
 
 namespace {
 
@@ -176,6 +177,8 @@ struct osr_sim {
   float sm_coef[3][31] = {};
   DBuf sm_buf[2];
   DBuf wall;  // Lindman wall buffers of open x walls (FieldDesc::wall)
+  std::vector<osr_zp::Wall> zp;  // laser walls, launch order
+  DBuf zp_tab, zp_mask, zp_first;  // FieldDesc::las_* (kZpRows steps per fill)
   Ctrl* h_ctrl = nullptr;  // pinned mirror
   int64_t desc_version = 0;  // bumped whenever the device descriptors change
 };
@@ -183,6 +186,15 @@ struct osr_sim {
 namespace {
 
 int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+constexpr int kZpRows = 256;  // steps of laser values per host evaluation (the longest batch)
+
+// launches per step of the laser walls (two for a circularly polarised pulse)
+int zp_launches(const osr_sim* s) {
+  int k = 0;
+  for (auto& w : s->zp) k += w.pol_type != 0 ? 2 : 1;
+  return k;
+}
 
 // LDS bytes of the push kernel for a tile shape
 size_t push_lds_bytes(int dims, int order, const int* t) {
@@ -294,6 +306,17 @@ void make_field_desc(const osr_sim* s, FieldDesc& f) {
     f.wall = s->wall.as<float>();
     f.wall_dc = (float)((1.0 - s->dt / s->dx[0]) / (1.0 + s->dt / s->dx[0]));
     for (int d = 0; d < 2; d++) f.wall_dt[d] = (float)(s->dt / s->dx[d]) * 0.5f;
+  }
+  if (!s->zp.empty()) {
+    f.las_k = zp_launches(s);
+    f.las_rows = kZpRows;
+    int k = 0;
+    for (auto& w : s->zp)
+      for (int r = 0; r < (w.pol_type != 0 ? 2 : 1); r++)
+        f.las_col[k++] = w.backward ? s->g.open_nx - 1 : 0;  // x = nx or x = 1
+    f.las_tab = s->zp_tab.as<float>();
+    f.las_mask = s->zp_mask.as<int>();
+    f.las_first = s->zp_first.as<long long>();
   }
   if (f.sm_np > 0) {
     f.sm_src = s->plain ? 1 : 0;
@@ -751,7 +774,44 @@ int launch_smooth(osr_sim* s) {
   return OSR_OK;
 }
 
+// laser walls of the step, before the push (os-simulation.f03:528-531)
+int launch_laser(osr_sim* s) {
+  if (s->zp.empty()) return OSR_OK;
+  hipLaunchKernelGGL(laser_wall, dim3(ceil_div(s->g.ny, 256)), dim3(256), 0, s->stream,
+                     s->d_fd.as<FieldDesc>());
+  HIPC(hipGetLastError());
+  return OSR_OK;
+}
+
+// Host evaluation of the laser columns of steps first .. first+n-1 (t = step dt) into the
+// device table the laser kernel reads.
+int zp_fill(osr_sim* s, int64_t first, int n) {
+  HIPC(hipStreamSynchronize(s->stream));  // the copies below are not ordered with the stream
+  const int K = zp_launches(s), ny = s->g.ny;
+  n = std::min(n, kZpRows);
+  std::vector<float> tab((size_t)n * K * 2 * ny);
+  std::vector<int> mask((size_t)kZpRows * K, 0);
+  for (int r = 0; r < n; r++) {
+    const double t = (double)(first + r) * s->dt;
+    int k = 0;
+    for (auto& w : s->zp)
+      for (int sec = 0; sec < (w.pol_type != 0 ? 2 : 1); sec++, k++) {
+        const size_t e = (size_t)r * K + k;
+        if (!osr_zp::active(w, t)) continue;
+        mask[e] = 1;
+        osr_zp::column(w, ny, s->dx, s->g.dt_dx[0], s->order, t, sec == 1, &tab[e * 2 * ny],
+                       &tab[e * 2 * ny + ny]);
+      }
+  }
+  const long long f = first;
+  HIPC(hipMemcpy(s->zp_tab.p, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(s->zp_mask.p, mask.data(), sizeof(int) * mask.size(), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(s->zp_first.p, &f, sizeof(f), hipMemcpyHostToDevice));
+  return OSR_OK;
+}
+
 int enqueue_step(osr_sim* s) {
+  TRY(launch_laser(s));
   if (s->plain) {
     for (size_t i = 0; i < s->sp.size(); i++) TRY(launch_plain(s, (int)i));
     TRY(launch_smooth(s));
@@ -1238,6 +1298,7 @@ int osr_sim_run(osr_sim* s, int nsteps) {
   int batch = 8;
   while (left > 0) {
     const int nb = (int)std::min<int64_t>(left, batch);
+    if (!s->zp.empty()) TRY(zp_fill(s, s->steps, nb));
     for (int i = 0; i < nb; i++) HIPC(hipGraphLaunch(s->gexec, s->stream));
     Ctrl c;
     TRY(read_ctrl(s, c));
@@ -1967,6 +2028,58 @@ int osr_smooth_vdf(int dims, const int* ld, int fdim, const int* order, const fl
   }
   HIPC(hipDeviceSynchronize());
   HIPC(hipMemcpy(f, a.p, 4 * n, hipMemcpyDeviceToHost));
+  return OSR_OK;
+}
+
+void osr_zpulse_wall_defaults(osr_zpulse_wall* zp) {
+  if (!zp) return;
+  *zp = osr_zpulse_wall{};
+  zp->a0 = 1.0;
+  zp->omega0 = 10.0;
+  zp->phase = 0.0;
+  zp->pol = 90.0;
+  zp->pol_type = 0;
+  zp->propagation = OSR_PROP_FORWARD;
+  zp->tenv_type = OSR_TENV_POLYNOMIAL;
+  zp->per_type = OSR_PER_PLANE;
+  zp->tenv_fwhm = -1.0;
+  zp->tenv_range = OSR_ZP_UNSET;
+  zp->per_center = OSR_ZP_UNSET;
+  zp->per_focus = OSR_ZP_UNSET;
+}
+
+int osr_sim_add_zpulse_wall(osr_sim* s, const osr_zpulse_wall* zp) {
+  TRY(check_sim(s));
+  if (!zp) return fail(OSR_ERR_ARG, "null argument");
+  if (!s->g.open_nx) return fail(OSR_ERR_STATE, "the launch wall must be open (xbnd = OSR_BND_OPEN)");
+  if ((int)s->zp.size() >= osr_zp::kMaxPulses) return fail(OSR_ERR_ARG, "at most 8 laser walls");
+  osr_zp::Wall w;
+  const int nx[2] = {s->g.open_nx, s->g.ny};
+  const char* err = osr_zp::resolve(*zp, s->order, nx, s->dx, w);
+  if (err) return fail(OSR_ERR_ARG, "zpulse_wall: %s", err);
+  HIPC(hipStreamSynchronize(s->stream));
+  s->zp.push_back(w);
+  const int K = zp_launches(s);
+  TRY(s->zp_tab.alloc(sizeof(float) * (size_t)kZpRows * K * 2 * s->g.ny));
+  TRY(s->zp_mask.alloc(sizeof(int) * (size_t)kZpRows * K));
+  TRY(s->zp_first.alloc(sizeof(long long)));
+  HIPC(hipMemset(s->zp_mask.p, 0, sizeof(int) * (size_t)kZpRows * K));
+  const long long none = -(1ll << 62);  // no table yet: every step falls outside it
+  HIPC(hipMemcpy(s->zp_first.p, &none, sizeof(none), hipMemcpyHostToDevice));
+  TRY(upload_descs(s));
+  s->graph_ok = false;
+  return OSR_OK;
+}
+
+int osr_sim_zpulse_column(osr_sim* s, int k, int second, double t, float* b12, float* b3,
+                          int* active) {
+  TRY(check_sim(s));
+  if (k < 0 || k >= (int)s->zp.size() || !b12 || !b3 || !active)
+    return fail(OSR_ERR_ARG, "bad argument");
+  const osr_zp::Wall& w = s->zp[k];
+  if (second && w.pol_type == 0) return fail(OSR_ERR_ARG, "a linear pulse launches once");
+  *active = osr_zp::active(w, t) ? 1 : 0;
+  osr_zp::column(w, s->g.ny, s->dx, s->g.dt_dx[0], s->order, t, second != 0, b12, b3);
   return OSR_OK;
 }
 

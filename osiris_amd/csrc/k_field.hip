@@ -51,6 +51,15 @@ struct FieldDesc {
   float* wall;
   float wall_dc;     // real((1 - dt/dx1) / (1 + dt/dx1))
   float wall_dt[2];  // real(dt/dx) * 0.5
+  // laser walls: las_k launches per step, into ring columns las_col[k]; for table row
+  // r = step - *las_first: las_mask[r][k] (0: the launch adds nothing) and the values
+  // las_tab[r][k][2][ny] (B2, B3 increments of rows y = 1..ny)
+  int las_k;
+  int las_rows;
+  int las_col[16];
+  const float* las_tab;
+  const int* las_mask;
+  const long long* las_first;
 };
 
 // offsets in FieldDesc::wall (units of ny values)
@@ -278,6 +287,29 @@ __global__ __launch_bounds__(256) void wall_e(const FieldDesc* __restrict__ fds)
     E[nxi + 1] = w;
     *u1 = en;
     *u2 = w;
+  }
+}
+
+// zpulse_wall launches (launch_em_wall_2d, zpulse/os-zpulse-wall.f03:865-960): B2, B3 of
+// one x column += the FP32 values evaluated on the host, launches in list order; before
+// the push of the step.
+__global__ __launch_bounds__(256) void laser_wall(const FieldDesc* __restrict__ fds) {
+  const FieldDesc& fd = *fds;
+  const GridDesc& g = fd.g;
+  if (g.ctrl->halt >= 2) return;
+  const int ny = g.ny, nx = g.nx;
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= ny) return;
+  const long long row = g.ctrl->steps - *fd.las_first;
+  if (row < 0 || row >= fd.las_rows) return;
+  const long long nc = g.ncell;
+  for (int k = 0; k < fd.las_k; k++) {
+    const long long e = row * fd.las_k + k;
+    if (!fd.las_mask[e]) continue;
+    const float* v = fd.las_tab + e * 2 * ny;
+    const long long c = (long long)r * nx + fd.las_col[k];
+    g.B[nc + c] = g.B[nc + c] + v[r];
+    g.B[2 * nc + c] = g.B[2 * nc + c] + v[ny + r];
   }
 }
 

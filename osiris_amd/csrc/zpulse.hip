@@ -1,0 +1,212 @@
+// zpulse.hip -- laser injection from an x wall (zpulse_wall), 2D: host-side set-up and the
+// FP64 evaluation of the column a pulse adds to B2, B3 at one time.
+//
+// Reference: zpulse/os-zpulse-wall.f03 (read_input_zpulse_wall :90-628, launch_zpulse_wall
+// :633-700, t_env_zpulse_wall :705-746, t_duration :752-771, lon_center :781-805,
+// launch_em_wall_2d :865-960), per_envelope_2d_zpulse zpulse/os-zpulse-std.f03:2599-2753,
+// fenv_poly / fenv_sin2 os-math.f90:83-165, hermite_double os-math.f90:478-509.
+//
+// The evaluation keeps the reference's operation order in double precision; the values are
+// rounded to float (real(., p_k_fld)) and the device adds them to B in FP32.  Included
+// by capi.hip.
+namespace osr_zp {
+
+constexpr double kPi2 = 1.57079632679489661923132;     // os-math.f90:14
+constexpr double kPi180 = 0.0174532925199432957692369;  // os-math.f90:15
+constexpr int kMaxPulses = 8;
+
+struct Wall {
+  double a0, omega0, phase0, pol;  // phase0 and pol in radians
+  int pol_type, backward, tenv_type, per_type, tem_mode;
+  double rise, flat, fall, duration, range;
+  double center, w0, focus;
+  double launch_time;
+  double xmin[2], xmax[2];
+};
+
+// os-math.f90:83-128
+inline double fenv_poly(double x, double rise, double flat, double fall) {
+  const double length = rise + flat + fall;
+  auto f = [](double tt) { return 10 * (tt * tt * tt) - 15 * (tt * tt * tt * tt) + 6 * (tt * tt * tt * tt * tt); };
+  if (x < 0.0) return 0.0;
+  if (x < rise) return f(x / rise);
+  if (x < rise + flat) return 1.0;
+  if (x < length) return f((length - x) / fall);
+  return 0.0;
+}
+
+// os-math.f90:132-165
+inline double fenv_sin2(double x, double rise, double flat, double fall) {
+  const double length = rise + flat + fall;
+  if (x < 0.0) return 0.0;
+  if (x < rise) {
+    const double s = std::sin(kPi2 * x / rise);
+    return s * s;
+  }
+  if (x < rise + flat) return 1.0;
+  if (x < length) {
+    const double s = std::sin(kPi2 * (length - x) / fall);
+    return s * s;
+  }
+  return 0.0;
+}
+
+// os-math.f90:478-509
+inline double hermite(int mode, double u) {
+  switch (mode) {
+    case 0: return 1.0;
+    case 1: return 2.0 * u;
+    case 2: return 4.0 * (u * u) - 2.0;
+    case 3: return 8.0 * (u * u * u) - 12.0 * u;
+    case 4: return 16.0 * (u * u * u * u) - 48.0 * (u * u) + 12.0;
+    case 5: return 32.0 * (u * u * u * u * u) - 160.0 * (u * u * u) + 120.0 * u;
+    case 6: return 64.0 * (u * u * u * u * u * u) - 480.0 * (u * u * u * u) + 720.0 * (u * u) - 120.0;
+    case 7:
+      return 128.0 * (u * u * u * u * u * u * u) - 1344.0 * (u * u * u * u * u) +
+             3360.0 * (u * u * u) - 1680.0 * u;
+    default: return 0.0;
+  }
+}
+
+inline double t_duration(const Wall& w) {
+  return w.tenv_type == OSR_TENV_GAUSSIAN ? w.range : w.rise + w.flat + w.fall;
+}
+
+inline double t_envelope(const Wall& w, double t) {
+  switch (w.tenv_type) {
+    case OSR_TENV_POLYNOMIAL: return fenv_poly(t, w.rise, w.flat, w.fall);
+    case OSR_TENV_SIN2: return fenv_sin2(t, w.rise, w.flat, w.fall);
+    default: {
+      const double a = (t - 0.5 * w.range) / w.duration;
+      return std::exp(-2 * (a * a));
+    }
+  }
+}
+
+// per_envelope_2d_zpulse at simulation time 0 (the wall calls it without a time); the
+// pulse centre moves with the injection time through lon_center
+inline double per_envelope_2d(const Wall& w, double x1, double x2, double inj_time) {
+  const double lon = w.tenv_type == OSR_TENV_GAUSSIAN ? 0.5 * w.range : w.rise + w.flat / 2;
+  const double z_center = lon + inj_time;
+  const double k = w.omega0;
+  const double prop_sign = w.backward ? -1.0 : 1.0;
+  const double t = 0.0;
+  if (w.per_type == OSR_PER_PLANE) return std::cos(k * (x1 - z_center - prop_sign * t) + w.phase0);
+  const double z0 = k * (w.w0 * w.w0) * 0.5;
+  const double rho = x2 - w.center;
+  const double rho2 = rho * rho;
+  const double z = x1 - w.focus;
+  double rWl2 = 1, curv = 0, gouy = 0;
+  if (z != 0) {
+    rWl2 = (z0 * z0) / ((z0 * z0) + (z * z));
+    curv = 0.5 * rho2 * z / ((z * z) + (z0 * z0));
+    gouy = (w.tem_mode + 1) * std::atan2(z, z0);
+  }
+  const double rWu = std::sqrt(2 * rWl2) / w.w0;
+  return std::sqrt(std::sqrt(rWl2)) * hermite(w.tem_mode, rho * rWu) *
+         std::exp(-(rho2 * rWl2 / (w.w0 * w.w0))) *
+         std::cos(k * (x1 - z_center - prop_sign * t) + k * curv - gouy + w.phase0);
+}
+
+// Whether the pulse adds anything at time t (launch_zpulse_wall: active from launch_time
+// through launch_time + duration).
+inline bool active(const Wall& w, double t) {
+  return t >= w.launch_time && !(t > w.launch_time + t_duration(w));
+}
+
+// launch_em_wall_2d, direction 1: rows y = 1..ny of the column (the guard rows it also
+// writes are replaced by the periodic refresh that follows).  second: the circular
+// polarisation's second launch (phase0 + sign(pi/2, pol_type), pol + pi/2).
+inline void column(const Wall& w0, int ny, const double* dx, float dtdx1, int order, double t,
+                   bool second, float* b12, float* b3) {
+  Wall w = w0;
+  if (second) {
+    w.phase0 = w.phase0 + (w.pol_type > 0 ? kPi2 : -kPi2);
+    w.pol = w.pol + kPi2;
+  }
+  const double prop_sign = w.backward ? -1.0 : 1.0;
+  double z = w.backward ? -w.xmax[0] : w.xmin[0];
+  const double inj_time = t - w.launch_time;
+  const double ldx = dx[0];
+  const double rdtdx = (double)dtdx1;  // real(dt/dx, p_k_fld)
+  const double ldxp = dx[1];
+  double rmin = w.xmin[1];  // one node: nx_p_min = 1
+  const double cos_pol = std::cos(w.pol) * prop_sign;
+  const double sin_pol = -std::sin(w.pol) * prop_sign;
+  const double amp = 2.0 * w.omega0 * w.a0;
+  const double dr_2 = ldxp / 2.0;
+  if (order == 2) {
+    z = z + ldx * 0.5;
+    rmin = rmin + dr_2;
+  }
+  const double lenv = amp * t_envelope(w, inj_time);
+  for (int ip = 1; ip <= ny; ip++) {
+    const double r = rmin + (ip - 1) * ldxp;
+    const double r_2 = r + dr_2;
+    b12[ip - 1] = (float)(lenv * rdtdx * per_envelope_2d(w, z, r, inj_time) * sin_pol);
+    b3[ip - 1] = (float)(lenv * rdtdx * per_envelope_2d(w, z, r_2, inj_time) * cos_pol);
+  }
+}
+
+// read_input_zpulse_wall for direction 1 (:196-628): resolved parameters, or an error text
+inline const char* resolve(const osr_zpulse_wall& p, int order, const int* nx, const double* dx,
+                           Wall& w) {
+  (void)order;
+  w = Wall{};
+  if (p.pol_type < -1 || p.pol_type > 1) return "pol_type must be -1, 0 or +1";
+  if (p.propagation != OSR_PROP_FORWARD && p.propagation != OSR_PROP_BACKWARD)
+    return "propagation must be forward or backward";
+  w.a0 = p.a0;
+  w.omega0 = p.omega0;
+  w.phase0 = p.phase * kPi180;
+  w.pol_type = p.pol_type;
+  w.pol = p.pol * kPi180;
+  w.backward = p.propagation == OSR_PROP_BACKWARD;
+  for (int d = 0; d < 2; d++) {
+    w.xmin[d] = p.xmin[d];
+    w.xmax[d] = p.xmin[d] + nx[d] * dx[d];
+  }
+  w.tenv_type = p.tenv_type;
+  switch (p.tenv_type) {
+    case OSR_TENV_POLYNOMIAL:
+    case OSR_TENV_SIN2:
+      if (p.tenv_fwhm > 0.0) {
+        w.rise = p.tenv_fwhm / 2.0;
+        w.flat = 0.0;
+        w.fall = p.tenv_fwhm / 2.0;
+      } else {
+        w.rise = p.tenv_rise;
+        w.flat = p.tenv_flat;
+        w.fall = p.tenv_fall;
+      }
+      break;
+    case OSR_TENV_GAUSSIAN:
+      w.duration = p.tenv_fwhm > 0.0 ? p.tenv_fwhm / std::sqrt(2.0 * std::log(2.0)) : p.tenv_duration;
+      w.range = p.tenv_range;
+      if (p.tenv_range == OSR_ZP_UNSET) return "a gaussian envelope needs tenv_range";
+      break;
+    default:
+      return "tenv_type must be polynomial, sin2 or gaussian";
+  }
+  w.per_type = p.per_type;
+  switch (p.per_type) {
+    case OSR_PER_PLANE:
+      break;
+    case OSR_PER_HERMITE:
+      if (p.per_tem_mode < 0 || p.per_tem_mode > 7) return "per_tem_mode must be 0..7";
+      w.tem_mode = p.per_tem_mode;
+      w.w0 = p.per_fwhm > 0.0 ? p.per_fwhm / std::sqrt(4.0 * std::log(2.0)) : p.per_w0;
+      if (w.w0 <= 0.0) return "a gaussian beam needs per_w0 or per_fwhm > 0";
+      if (p.per_focus == OSR_ZP_UNSET) return "a gaussian beam needs per_focus";
+      w.focus = p.per_focus;
+      break;
+    default:
+      return "per_type must be plane or hermite";
+  }
+  w.center = p.per_center == OSR_ZP_UNSET ? 0.5 * (w.xmin[1] + w.xmax[1]) : p.per_center;
+  w.launch_time = p.launch_time;
+  if (w.backward) w.focus = -w.focus;
+  return nullptr;
+}
+
+}  // namespace osr_zp

@@ -14,8 +14,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (OSR_BND_OPEN, OSR_BND_PERIODIC, OSR_BORIS, OSR_SMOOTH, OSR_VAY, SimDesc,
-                   SmoothParams, check, lib)
+from ._lib import (OSR_BND_OPEN, OSR_BND_PERIODIC, OSR_BORIS, OSR_PER, OSR_PROP, OSR_SMOOTH,
+                   OSR_TENV, OSR_VAY, SimDesc, SmoothParams, ZpulseWall, check, lib)
 
 
 def _p(a):
@@ -207,6 +207,30 @@ class Simulation:
     def fill_fields(self, e0=None, b0=None):
         check(lib().osr_fields_fill(self._h, _p(None if e0 is None else np.array(e0, np.float64)),
                                     _p(None if b0 is None else np.array(b0, np.float64))))
+
+    def add_zpulse_wall(self, xmin=(0.0, 0.0), **kw):
+        """Laser injection from an x wall: keyword arguments as the zpulse_wall input
+        section (a0, omega0, phase, pol, pol_type, propagation, tenv_type, tenv_fwhm,
+        tenv_rise, tenv_flat, tenv_fall, tenv_duration, tenv_range, per_type, per_center,
+        per_w0, per_fwhm, per_focus, per_tem_mode, launch_time), with its defaults."""
+        z = ZpulseWall()
+        lib().osr_zpulse_wall_defaults(C.byref(z))
+        names = {"propagation": OSR_PROP, "tenv_type": OSR_TENV, "per_type": OSR_PER}
+        for k, v in kw.items():
+            if k not in {f[0] for f in ZpulseWall._fields_} - {"reserved", "xmin"}:
+                raise ValueError(f"unknown zpulse_wall parameter {k}")
+            setattr(z, k, names[k][v] if k in names else v)
+        z.xmin[0], z.xmin[1] = xmin
+        check(lib().osr_sim_add_zpulse_wall(self._h, C.byref(z)))
+
+    def zpulse_column(self, k, t, second=False):
+        """(b12, b3, active): what pulse k adds to B2, B3 of its column at time t."""
+        b12 = np.zeros(self.nx[1], np.float32)
+        b3 = np.zeros(self.nx[1], np.float32)
+        act = C.c_int()
+        check(lib().osr_sim_zpulse_column(self._h, k, int(second), float(t), _p(b12), _p(b3),
+                                          C.byref(act)))
+        return b12, b3, bool(act.value)
 
     def set_smooth(self, order, coef):
         """Smooth the current every step (order per direction, coef from smooth_setup)."""
