@@ -55,7 +55,7 @@ def build_fortran(verbose: bool = False) -> str | None:
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    for name in ("push_example", "smooth_setup_example"):
+    for name in ("push_example", "smooth_setup_example", "zpulse_layout_example"):
         drv = os.path.join(fdir, name + ".f03")
         if not os.path.exists(drv):
             continue

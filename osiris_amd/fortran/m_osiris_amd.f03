@@ -57,6 +57,33 @@ module m_osiris_amd
     real(c_double) :: digital_fc(3) = -huge(1.0_c_double)
   end type osr_smooth_params
 
+  ! laser injection from an x wall (zpulse_wall, zpulse/os-zpulse-wall.f03): the namelist
+  ! nl_zpulse_wall for direction 1 of a 2D box; defaults are the reference's
+  integer(c_int), parameter, public :: OSR_TENV_POLYNOMIAL = 0, OSR_TENV_SIN2 = 1, &
+                                       OSR_TENV_GAUSSIAN = 2
+  integer(c_int), parameter, public :: OSR_PER_PLANE = 0, OSR_PER_HERMITE = 1
+  integer(c_int), parameter, public :: OSR_PROP_FORWARD = 0, OSR_PROP_BACKWARD = 1
+  type, bind(c), public :: osr_zpulse_wall
+    real(c_double) :: a0 = 1.0_c_double, omega0 = 10.0_c_double
+    real(c_double) :: phase = 0.0_c_double
+    real(c_double) :: pol = 90.0_c_double
+    integer(c_int) :: pol_type = 0
+    integer(c_int) :: propagation = OSR_PROP_FORWARD
+    integer(c_int) :: tenv_type = OSR_TENV_POLYNOMIAL
+    integer(c_int) :: per_type = OSR_PER_PLANE
+    real(c_double) :: tenv_fwhm = -1.0_c_double
+    real(c_double) :: tenv_rise = 0.0_c_double, tenv_flat = 0.0_c_double, tenv_fall = 0.0_c_double
+    real(c_double) :: tenv_duration = 0.0_c_double
+    real(c_double) :: tenv_range = -huge(1.0_c_double)
+    real(c_double) :: per_center = -huge(1.0_c_double)
+    real(c_double) :: per_w0 = 0.0_c_double, per_fwhm = 0.0_c_double
+    real(c_double) :: per_focus = -huge(1.0_c_double)
+    integer(c_int) :: per_tem_mode = 0
+    integer(c_int) :: reserved = 0
+    real(c_double) :: launch_time = 0.0_c_double
+    real(c_double) :: xmin(2) = 0.0_c_double
+  end type osr_zpulse_wall
+
   public :: osr_version, osr_device_count, osr_error_message
   public :: osr_advance_deposit, osr_yee_advance
   public :: hip_vadvance_deposit_2d_s1, hip_vadvance_deposit_2d_s2, hip_vadvance_deposit_2d_s3
@@ -67,6 +94,7 @@ module m_osiris_amd
             osr_sim_gauss, osr_sim_relayouts, osr_sim_time_kernels, osr_sim_footprint
   public :: osr_smooth_setup, osr_smooth_vdf, osr_sim_set_smooth
   public :: osr_sim_write_hist, osr_sim_save_field
+  public :: osr_sim_add_zpulse_wall, osr_sim_zpulse_column
 
   interface
     function osr_last_error() bind(c, name="osr_last_error")
@@ -326,6 +354,22 @@ module m_osiris_amd
       character(kind=c_char), intent(in) :: ms_dir(*)
       real(c_double), intent(in) :: xmin(*)
       integer(c_int) :: osr_sim_save_field
+    end function
+    function osr_sim_add_zpulse_wall(sim, zp) bind(c, name="osr_sim_add_zpulse_wall")
+      import :: c_int, c_ptr, osr_zpulse_wall
+      type(c_ptr), value :: sim
+      type(osr_zpulse_wall), intent(in) :: zp
+      integer(c_int) :: osr_sim_add_zpulse_wall
+    end function
+    function osr_sim_zpulse_column(sim, k, second, t, b12, b3, active) &
+        bind(c, name="osr_sim_zpulse_column")
+      import :: c_int, c_ptr, c_float, c_double
+      type(c_ptr), value :: sim
+      integer(c_int), value :: k, second
+      real(c_double), value :: t
+      real(c_float), intent(out) :: b12(*), b3(*)
+      integer(c_int), intent(out) :: active
+      integer(c_int) :: osr_sim_zpulse_column
     end function
   end interface
 

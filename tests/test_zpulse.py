@@ -93,3 +93,41 @@ def test_zpulse_needs_open_walls():
     s = O.Sim(2, (16, 8), (0.1, 0.1), 0.05, 2)
     with pytest.raises(ValueError):
         s.add_zpulse_wall(a0=1.0)
+
+
+def test_library_defaults_are_the_reference_defaults():
+    """osr_zpulse_wall_defaults (host code) = the nl_zpulse_wall defaults
+    (os-zpulse-wall.f03:196-239) the oracle resolves from."""
+    import osiris_amd._lib as L
+    z = L.ZpulseWall()
+    L.lib().osr_zpulse_wall_defaults(C.byref(z))
+    d = O.ZPULSE_DEFAULTS
+    for k in ("a0", "omega0", "phase", "pol", "pol_type", "tenv_fwhm", "tenv_rise", "tenv_flat",
+              "tenv_fall", "tenv_duration", "per_w0", "per_fwhm", "per_tem_mode", "launch_time"):
+        assert getattr(z, k) == d[k], k
+    unset = -np.finfo(np.float64).max  # the reference's -huge(1.0)
+    for k in ("tenv_range", "per_center", "per_focus"):
+        assert d[k] is None and getattr(z, k) == unset, k
+    assert (z.propagation, z.tenv_type, z.per_type) == (L.OSR_PROP[d["propagation"]],
+                                                        L.OSR_TENV[d["tenv_type"]],
+                                                        L.OSR_PER[d["per_type"]])
+
+
+def test_fortran_binding_layout():
+    """The bind(c) osr_zpulse_wall of m_osiris_amd has the C size and, with its default
+    initialisers, the values osr_zpulse_wall_defaults writes (host code, no GPU)."""
+    import os
+    import subprocess
+
+    import osiris_amd._lib as L
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "osiris_amd", "lib", "zpulse_layout_example")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran example not built (no Fortran compiler)")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.splitlines()
+    assert lines[0] == f"size {C.sizeof(L.ZpulseWall)}"
+    f = [ln.split(None, 1)[1] for ln in lines[1:] if ln.startswith("fortran")]
+    c = [ln.split(None, 1)[1] for ln in lines[1:] if ln.startswith("c ")]
+    assert len(f) == 2 and f == c
