@@ -213,6 +213,136 @@ def run_c3(args, oa):
     print(json.dumps(out), flush=True)
 
 
+# c4: laser-plasma interaction between open x walls (emf_bound "lindman", spec_bound
+# "absorbing", zpulse_wall from x-min), periodic y
+C4 = dict(dx=(0.2, 0.5), dt=0.14, density=0.01, ramp_cells=100, gap_cells=100, ppc=(4, 2),
+          laser=dict(a0=2.0, omega0=10.0, pol=90.0, tenv_type="sin2", tenv_rise=10.0,
+                     tenv_flat=10.0, tenv_fall=10.0, per_type="gaussian", per_w0=20.0,
+                     per_focus=40.0))
+
+
+def c4_particles(nx, gap, ramp, ppc, density):
+    """Cold electrons on a regular lattice (ppc[0] x ppc[1] per cell, os-psource-std.f03:
+    846-866) in cells x > gap, density rising linearly over `ramp` cells to `density`;
+    charge per particle -n(x)/ppc."""
+    xs = np.arange(gap + 1, nx[0] + 1)
+    cells = np.stack(np.meshgrid(xs, np.arange(1, nx[1] + 1), indexing="ij"), -1).reshape(-1, 2)
+    k = np.arange(ppc[0] * ppc[1])
+    lx = (k // ppc[1]).astype(np.float32)
+    ly = (k % ppc[1]).astype(np.float32)
+    off = np.stack([(2 * (lx + 1) - 1 - ppc[0]) * (0.5 / ppc[0]),
+                    (2 * (ly + 1) - 1 - ppc[1]) * (0.5 / ppc[1])], 1).astype(np.float32)
+    ix = np.repeat(cells, len(k), axis=0).astype(np.int32)
+    x = np.tile(off, (len(cells), 1))
+    u = np.zeros((len(ix), 3), np.float32)
+    dens = density * np.clip((ix[:, 0] - gap) / float(ramp), 0.0, 1.0)
+    q = (-dens / len(k)).astype(np.float32)
+    return ix, x, u, q
+
+
+def oracle_check_c4(oa, steps=160):
+    """The c4 physics (cold ramped plasma, a0 = 2 sin^2 laser with a gaussian profile from
+    x-min, Lindman x walls, absorbing particles) on a 256 x 32 grid with a 40-cell gap and
+    ramp: GPU against the FP64 oracle on the same particles."""
+    from oracle import oracle as O
+
+    nx = (256, 32)
+    dx, dt = C4["dx"], C4["dt"]
+    laser = dict(C4["laser"], tenv_rise=4.0, tenv_flat=4.0, tenv_fall=4.0, per_w0=4.0,
+                 per_focus=12.0)
+    ix, x, u, q = c4_particles(nx, 40, 40, C4["ppc"], C4["density"])
+    with oa.Simulation(2, nx, dx, dt, 2, tile=(32, 16), xbnd="open") as sim:
+        sid = sim.add_species(-1.0)
+        sim.set_particles(sid, ix, x, u, q)
+        sim.add_zpulse_wall(**laser)
+        sim.run(steps)
+        fld, par = sim.history()
+        n_gpu = sim.count(sid)
+    ref = O.Sim(2, nx, dx, dt, 2, open_x=True)
+    ref.add_species(-1.0, 0, ix, x, u, q)
+    ref.add_zpulse_wall(**laser)
+    ref.step(steps)
+    rfld, rpar = np.array(ref.fld_hist), np.array(ref.par_hist)
+    fscale = np.maximum(rfld.sum(1, keepdims=True), 1e-300)
+    # the cold plasma has no kinetic energy until the laser reaches it: rows relative to
+    # themselves once the energy exceeds 1e-3 of its peak, and every row against the peak
+    peak = np.abs(rpar[:, 0]).max()
+    live = np.abs(rpar[:, 0]) > 1e-3 * peak
+    return {"workload": ("c4 physics on 256x32 (gap and ramp 40 cells, sin^2 4/4/4, w0=4), "
+                         f"{len(q)} particles"),
+            "steps": steps, "reference": "FP64 CPU oracle (restatement of the reference)",
+            "particles_left": {"gpu": int(n_gpu), "oracle": int(len(ref.species[0].q))},
+            "max_rel_err_particle_energy": float(np.max(np.abs(par[live] - rpar[live]) /
+                                                        np.abs(rpar[live]))),
+            "particle_energy_rows_compared": int(live.sum()),
+            "max_err_particle_energy_over_peak": float(np.max(np.abs(par - rpar)) / peak),
+            "max_rel_err_field_energy": float(np.max(np.abs(fld - rfld) / fscale)),
+            "tolerance": 1e-4}
+
+
+def run_c4(args, oa):
+    """Laser-plasma interaction: 2048 x 512 cells, dx = 0.2, dy = 0.5, dt = 0.14, cold
+    electrons (8 ppc) at 0.01 n_c behind a 100-cell vacuum gap and a 100-cell linear ramp,
+    a0 = 2 laser (omega0 = 10, sin^2 10/10/10, gaussian w0 = 20 focused at x = 40) launched
+    from x-min, Lindman x walls with absorbing particles, periodic y, quadratic, Boris."""
+    nx = (2048, 512)
+    dx, dt = C4["dx"], C4["dt"]
+    tile = tuple(args.tile) if args.tile != [16, 16] else (32, 16)
+    steps = min(args.steps, 1500)
+    ix, x, u, q = c4_particles(nx, C4["gap_cells"], C4["ramp_cells"], C4["ppc"], C4["density"])
+    sim = oa.Simulation(2, nx, dx, dt, 2, tile=tile, xbnd="open")
+    sid = sim.add_species(-1.0)
+    sim.set_particles(sid, ix, x, u, q)
+    sim.add_zpulse_wall(**C4["laser"])
+    del ix, x, u, q
+    cells = nx[0] * nx[1]
+    log(f"workload c4: {sim.count(sid)} particles, footprint {sim.footprint / 2**30:.2f} GiB")
+    sim.run(args.warmup)
+    n0 = sim.count(sid)
+    t0 = time.perf_counter()
+    sim.run(steps)
+    el = time.perf_counter() - t0
+    n1 = sim.count(sid)
+    n_mean = 0.5 * (n0 + n1)
+    value = n_mean * steps / el
+    kt = sim.time_kernels(args.timing_steps)
+    log(f"timed: {steps} steps in {el:.3f} s -> {el * 1e3 / steps:.3f} ms/step, {value:.4e} upd/s "
+        f"({n0} -> {n1} particles); kernel ms/step: " +
+        ", ".join(f"{k} {v:.4f}" for k, v in kt.items()))
+    t_launch = kt["push_deposit"] * 1e-3
+    bytes_launch = n1 * BYTES_PER_PARTICLE + cells * BYTES_PER_CELL_PUSH
+    achieved = bytes_launch / t_launch / 1e9
+    fld, par = sim.history()
+    relayouts = sim.relayouts
+    sim.close()
+    out = {
+        "metric": "particle-updates/s (push+deposit+Yee+walls+laser) 2D quadratic 8ppc",
+        "value": value, "unit": "particle-updates/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": el * 1e3 / steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (lattice-loaded cold plasma, host generated)",
+        "config": {"workload": ("2D laser-plasma 2048x512, dx=0.2, dy=0.5, dt=0.14, 0.01 n_c "
+                                "with 100-cell ramp, 8 ppc cold, a0=2 zpulse_wall from x-min, "
+                                "Lindman x walls + absorbing particles, periodic y, quadratic"),
+                   "particles": [n0, n1], "cells": cells, "tile": list(tile),
+                   "relayouts": relayouts,
+                   "field_energy_last": float(fld[-1].sum()),
+                   "particle_energy_last": float(par[-1].sum())},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "push_deposit_2d_t<2,0,%d,%d>" % tile, "kernel_ms": t_launch * 1e3,
+                     "algorithmic_bytes": bytes_launch},
+        "kernels_ms_per_step": kt,
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(nx, 8, n1, 2, dx, dt, args.cpu_seconds,
+                          args.cpu_threads or min(16, os.cpu_count() or 1))
+        if cb is not None:
+            cb["oracle_check"] = oracle_check_c4(oa)
+            out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -223,9 +353,10 @@ def main():
                     help="particles per cell per beam along x and y (2 beams)")
     ap.add_argument("--order", type=int, default=2)
     ap.add_argument("--tile", type=int, nargs="+", default=[16, 16])
-    ap.add_argument("--workload", choices=("c2", "c3"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c3", "c4"), default="c2",
                     help="c2: the headline 2D two-stream step (default); c3: 3D periodic "
-                         "magnetized thermal plasma 256^3, 8 ppc, cubic, Vay")
+                         "magnetized thermal plasma 256^3, 8 ppc, cubic, Vay; c4: 2D "
+                         "laser-plasma 2048x512 between open x walls")
     ap.add_argument("--autotune", action="store_true", default=True)
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
     ap.add_argument("--plain-steps", type=int, default=20,
@@ -263,6 +394,9 @@ def main():
 
     if args.workload == "c3":
         run_c3(args, oa)
+        return
+    if args.workload == "c4":
+        run_c4(args, oa)
         return
 
     nx = tuple(args.nx)

@@ -1479,6 +1479,10 @@ int osr_sim_time_kernels(osr_sim* s, int nsteps, double* ms) {
   double acc[6] = {0, 0, 0, 0, 0, 0};
   int done = 0;
   for (int k = 0; k < nsteps; k++) {
+    if (!s->zp.empty()) {  // the laser walls of the step, not timed
+      TRY(zp_fill(s, s->steps, 1));
+      TRY(launch_laser(s));
+    }
     HIPC(hipEventRecord(ev[0], s->stream));
     for (size_t i = 0; i < s->sp.size(); i++) TRY(s->plain ? launch_plain(s, (int)i) : launch_push(s, (int)i));
     HIPC(hipEventRecord(ev[1], s->stream));
@@ -1512,6 +1516,7 @@ int osr_sim_time_kernels(osr_sim* s, int nsteps, double* ms) {
   }
   for (int i = 0; i < nev; i++) (void)hipEventDestroy(ev[i]);
   for (int i = 0; i < 6; i++) ms[i] = done ? acc[i] / done : 0.0;
+  if (s->g.open_nx) TRY(refresh_counts(s));
   return OSR_OK;
 }
 
