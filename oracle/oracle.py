@@ -71,6 +71,8 @@ def _lib(prec: str):
         L.orc_particles_open.argtypes = [P, P, P, P, P, C.c_int]
         L.orc_particles_open.restype = C.c_int
         L.orc_zpulse_wall_2d.argtypes = [P, P, C.c_int, P, C.c_double, C.c_double, P]
+        L.orc_es_solver_2d.argtypes = [P, P, P, P]
+        L.orc_update_periodic_from.argtypes = [P, P, C.c_int, C.c_int, C.c_int]
         L._real = np.float64 if prec == "f64" else np.float32
         _LIBS[prec] = L
     return _LIBS[prec]
@@ -560,6 +562,22 @@ class Sim:
             self.par_hist.append(pe)
         for s, k in zip(self.species, npa):  # leading slices stay contiguous
             s.ix, s.x, s.u, s.q = s.ix[:k], s.x[:k], s.u[:k], s.q[:k]
+
+    def init_fields(self, k):
+        """init_fields of species k (spec/os-spec-initfields.f90:38-100, ufl(1) = 0):
+        rho deposited and folded, the 2D direct-sum ES field added to E."""
+        if self.dims != 2:
+            raise ValueError("the reference's ES solver is 2D only")
+        sp = self.species[k]
+        rho = deposit_rho(self.grid, self.order, sp.ix, sp.x, sp.q, self.prec)
+        L = _lib(self.prec)
+        g = self.grid.c()
+        L.orc_update_periodic_from(C.byref(g), _p(rho), 1, 1, 1 if self.open_x else 0)
+        ef = self.grid.zeros(3, self.real)
+        dxa = np.array(list(self.dx) + [1.0], np.float64)
+        L.orc_es_solver_2d(C.byref(g), _p(dxa), _p(rho), _p(ef))
+        self.grid.interior(self.e)[...] += self.grid.interior(ef)
+        return ef
 
     def add_zpulse_wall(self, xmin=(0.0, 0.0), **kw):
         """A zpulse_wall section (direction 1); needs open x walls (os-zpulse-wall.f03:

@@ -854,6 +854,11 @@ void orc_update_periodic(const orc_grid *g, real *f, int nc, int mode) {
   update_periodic_from(g, f, nc, mode, 0);
 }
 
+/* the same for directions d0.. only (d0 = 1: x walls that are not periodic) */
+void orc_update_periodic_from(const orc_grid *g, real *f, int nc, int mode, int d0) {
+  update_periodic_from(g, f, nc, mode, d0);
+}
+
 void orc_particles_periodic(const orc_grid *g, int *ix, int np) {
   for (int i = 0; i < np; i++)
     for (int d = 0; d < g->dims; d++) {
@@ -1346,4 +1351,50 @@ void orc_zpulse_wall_2d(const orc_grid *g, const orc_zpulse *p, int order, const
   if (p->pol_type != 0)
     zp_launch(g, p, order, dx, dt, t, p->phase0 + (p->pol_type > 0 ? ORC_PI_2 : -ORC_PI_2),
               p->pol + ORC_PI_2, b);
+}
+
+/* ------------------------------------------------------------------------------------
+ * Initial electrostatic field of a species (init_fields, spec/os-spec-initfields.f90:
+ * 38-100, non-beam branch): rho deposited and folded, then es_solver_2d
+ * (emf/os-emf-es-solver.f03:125-236): the free-space direct sum over every source cell
+ * with nonzero charge, in source order (y outer, x inner), of charge * d / r^2 at the
+ * Yee points of E1 (x + 1/2, y) and E2 (x, y + 1/2), each term in FP64 and accumulated
+ * into the working-precision field; then times the cell volume, E3 = 0.  rho is a
+ * one-component grid, ef a three-component grid (guards untouched, interior written).
+ * ------------------------------------------------------------------------------------ */
+void orc_es_solver_2d(const orc_grid *g, const double *dx, const real *rho, real *ef) {
+  const int nx = g->nx[0], ny = g->nx[1];
+  for (int j = 1; j <= ny; j++)
+    for (int i = 1; i <= nx; i++)
+      for (int c = 0; c < 3; c++) FLD(ef, g, c, i, j, 1) = 0;
+  for (int ri2 = 1; ri2 <= ny; ri2++)
+    for (int ri1 = 1; ri1 <= nx; ri1++) {
+      double charge = (double)rho[cidx(g, ri1, ri2, 1)];
+      if (charge == 0) continue;
+      for (int ei2 = 1; ei2 <= ny; ei2++) {
+        double dx2_e1 = (double)(ei2 - ri2);
+        double dx2_e2 = dx2_e1 + 0.5;
+        dx2_e1 = dx2_e1 * dx[1];
+        dx2_e2 = dx2_e2 * dx[1];
+        for (int ei1 = 1; ei1 <= nx; ei1++) {
+          double dx1_e2 = (double)(ei1 - ri1);
+          double dx1_e1 = dx1_e2 + 0.5;
+          dx1_e1 = dx1_e1 * dx[0];
+          dx1_e2 = dx1_e2 * dx[0];
+          double r_2 = dx1_e1 * dx1_e1 + dx2_e1 * dx2_e1;
+          real *e = &FLD(ef, g, 0, ei1, ei2, 1);
+          e[0] = (real)((double)e[0] + charge * dx1_e1 / r_2);
+          r_2 = dx1_e2 * dx1_e2 + dx2_e2 * dx2_e2;
+          e[1] = (real)((double)e[1] + charge * dx2_e2 / r_2);
+        }
+      }
+    }
+  double cell_vol = dx[0] * dx[1];
+  for (int j = 1; j <= ny; j++)
+    for (int i = 1; i <= nx; i++) {
+      real *e = &FLD(ef, g, 0, i, j, 1);
+      e[0] = (real)((double)e[0] * cell_vol);
+      e[1] = (real)((double)e[1] * cell_vol);
+      e[2] = 0;
+    }
 }

@@ -1447,6 +1447,36 @@ int osr_sim_gauss(osr_sim* s, double* res, double* rho_out, double* maxabs) {
   return OSR_OK;
 }
 
+int osr_sim_init_fields(osr_sim* s, int sid) {
+  TRY(check_sim(s));
+  if (s->dims != 2) return fail(OSR_ERR_ARG, "the electrostatic solver is 2D only");
+  if (sid < 0 || sid >= (int)s->sp.size()) return fail(OSR_ERR_ARG, "bad species id");
+  if (s->plain) return fail(OSR_ERR_STATE, "init_fields needs the fused path");
+  const size_t nc = (size_t)s->g.ncell;
+  const long long nt = (long long)interior_nx(s->g) * s->g.ny;
+  DBuf rho, charge;
+  TRY(rho.alloc(8 * nc));
+  TRY(charge.alloc(8 * (size_t)nt));
+  HIPC(hipStreamSynchronize(s->stream));
+  HIPC(hipMemsetAsync(rho.p, 0, 8 * nc, s->stream));
+  const Species& p = *s->sp[sid];
+  const int sh = 62 - (int)std::ceil(std::log2(std::max(p.qabs * 1.01, 1e-300)));
+  const double scale = std::ldexp(1.0, std::max(0, std::min(sh, 100)));
+  typedef void (*rf)(const SpecDesc*, const int2*, unsigned long long*, double);
+  rf f = s->order == 1 ? rho_deposit<1> : (s->order == 2 ? rho_deposit<2> : rho_deposit<3>);
+  hipLaunchKernelGGL(f, dim3(s->ntiles), dim3(256), 0, s->stream, s->d_spec.as<SpecDesc>(),
+                     s->d_work.as<int2>() + (size_t)sid * s->ntiles, rho.as<unsigned long long>(),
+                     scale);
+  const unsigned nb = (unsigned)((nt + 255) / 256);
+  hipLaunchKernelGGL(es_charge, dim3(nb), dim3(256), 0, s->stream, s->g,
+                     rho.as<unsigned long long>(), 1.0 / scale, charge.as<double>());
+  hipLaunchKernelGGL(es_direct_2d, dim3(nb), dim3(256), 0, s->stream, s->g, charge.as<double>(),
+                     s->dx[0], s->dx[1], s->dx[0] * s->dx[1]);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(s->stream));
+  return OSR_OK;
+}
+
 int osr_sim_relayouts(osr_sim* s, int64_t* count) {
   TRY(check_sim(s));
   if (!count) return fail(OSR_ERR_ARG, "null argument");

@@ -531,6 +531,76 @@ __global__ __launch_bounds__(256) void gauss_residual(GridDesc g, const unsigned
   res[idx] = dv - (double)(long long)rho[idx] * rscale;
 }
 
+// ------------------------------------------------------------------------------------
+// Initial electrostatic field (init_fields, spec/os-spec-initfields.f90:38-100, and
+// es_solver_2d, emf/os-emf-es-solver.f03:125-236): the free-space direct sum of
+// charge * d / r^2 over every interior source cell with nonzero charge, per target cell
+// in source order (y outer, x inner), each term in FP64 and accumulated into an FP32
+// field as the reference's p_k_fld grid does; then times the cell volume and added to E.
+// ------------------------------------------------------------------------------------
+
+// the fixed-point charge density of the interior, rounded to FP32 (the reference's rho
+// grid) and widened to FP64, row-major
+__global__ __launch_bounds__(256) void es_charge(GridDesc g, const unsigned long long* rho,
+                                                 double inv, double* charge) {
+  const int nxi = interior_nx(g);
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long long)nxi * g.ny) return;
+  const long long c = (t / nxi) * g.nx + t % nxi;
+  charge[t] = (double)(float)((double)(long long)rho[c] * inv);
+}
+
+constexpr int kEsChunk = 1024;
+
+// one thread per target cell; source charges staged through LDS (every lane reads the same
+// source: broadcast reads, uniform branches)
+__global__ __launch_bounds__(256) void es_direct_2d(GridDesc g, const double* __restrict__ charge,
+                                                    double d0, double d1, double vol) {
+  __shared__ double s_q[kEsChunk];
+  const int nxi = interior_nx(g), ny = g.ny;
+  const long long nt = (long long)nxi * ny;
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  const bool live = t < nt;
+  const int ei1 = live ? (int)(t % nxi) : 0, ei2 = live ? (int)(t / nxi) : 0;
+  float a1 = 0.f, a2 = 0.f;
+  for (long long c0 = 0; c0 < nt; c0 += kEsChunk) {
+    const int m = (int)(nt - c0 < kEsChunk ? nt - c0 : kEsChunk);
+    __syncthreads();
+    for (int k = threadIdx.x; k < m; k += 256) s_q[k] = charge[c0 + k];
+    __syncthreads();
+    if (!live) continue;
+    int ri1 = (int)(c0 % nxi), ri2 = (int)(c0 / nxi);
+    for (int k = 0; k < m; k++) {
+      const double q = s_q[k];
+      if (q != 0) {
+        double y1 = (double)(ei2 - ri2);
+        double y2 = y1 + 0.5;
+        y1 = y1 * d1;
+        y2 = y2 * d1;
+        double x2 = (double)(ei1 - ri1);
+        double x1 = x2 + 0.5;
+        x1 = x1 * d0;
+        x2 = x2 * d0;
+        double r2 = x1 * x1 + y1 * y1;
+        a1 = (float)((double)a1 + q * x1 / r2);
+        r2 = x2 * x2 + y2 * y2;
+        a2 = (float)((double)a2 + q * y2 / r2);
+      }
+      if (++ri1 == nxi) {
+        ri1 = 0;
+        ri2++;
+      }
+    }
+  }
+  if (!live) return;
+  a1 = (float)((double)a1 * vol);
+  a2 = (float)((double)a2 * vol);
+  const long long nc = g.ncell, c = (long long)ei2 * g.nx + ei1;
+  g.E[c] = g.E[c] + a1;
+  g.E[nc + c] = g.E[nc + c] + a2;
+  g.E[2 * nc + c] = g.E[2 * nc + c] + 0.0f;  // E3 of the ES field is 0 (-0 becomes +0)
+}
+
 template __global__ void yee_dbdt<2, true>(const FieldDesc*, const int2*);
 template __global__ void yee_dbdt<2, false>(const FieldDesc*, const int2*);
 template __global__ void yee_dbdt<3, true>(const FieldDesc*, const int2*);

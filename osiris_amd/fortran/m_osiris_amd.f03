@@ -94,7 +94,7 @@ module m_osiris_amd
             osr_sim_gauss, osr_sim_relayouts, osr_sim_time_kernels, osr_sim_footprint
   public :: osr_smooth_setup, osr_smooth_vdf, osr_sim_set_smooth
   public :: osr_sim_write_hist, osr_sim_save_field
-  public :: osr_sim_add_zpulse_wall, osr_sim_zpulse_column
+  public :: osr_sim_add_zpulse_wall, osr_sim_zpulse_column, osr_sim_init_fields
 
   interface
     function osr_last_error() bind(c, name="osr_last_error")
@@ -354,6 +354,12 @@ module m_osiris_amd
       character(kind=c_char), intent(in) :: ms_dir(*)
       real(c_double), intent(in) :: xmin(*)
       integer(c_int) :: osr_sim_save_field
+    end function
+    function osr_sim_init_fields(sim, sid) bind(c, name="osr_sim_init_fields")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: sim
+      integer(c_int), value :: sid
+      integer(c_int) :: osr_sim_init_fields
     end function
     function osr_sim_add_zpulse_wall(sim, zp) bind(c, name="osr_sim_add_zpulse_wall")
       import :: c_int, c_ptr, osr_zpulse_wall
