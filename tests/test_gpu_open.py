@@ -81,6 +81,49 @@ def test_open_plasma_vs_fp64_oracle():
     assert np.all(np.abs(gu.astype(np.float64).mean(0) - ru.mean(0)) <= tol * rms)
 
 
+@pytest.mark.parametrize("order,pusher", [(1, 0), (3, 0), (3, 1)])
+def test_open_orders_vs_oracles(order, pusher):
+    """Linear and cubic interpolation reach other guard columns (x = 0 only, or
+    x = -1 .. nx+2 with the cubic stencil) and need other guard bands (8 columns here):
+    fields bit-exact against the FP32 oracle, then a leaking plasma against FP64."""
+    print(f"case order={order} pusher={pusher}", flush=True)
+    nx, ny, dx, dt = 64, 16, 0.2, 0.12
+    e, b = vacuum_pulse(nx, ny, dx, direction=-1)
+    e, b = e.astype(np.float32), b.astype(np.float32)
+    ref = O.Sim(2, (nx, ny), (dx, dx), dt, order, prec="f32", open_x=True)
+    g = ref.grid
+    with oa.Simulation(2, (nx, ny), (dx, dx), dt, order, tile=(8, 8), xbnd="open") as sim:
+        sim.set_fields(e, b)
+        ref.set_interior("e", e)
+        ref.set_interior("b", b)
+        sim.run(70)
+        ref.step(70)
+        ref.wall_refresh()
+        ge, gb = sim.get_fields(gcl=g.gcl, gcu=g.gcu)
+    assert np.array_equal(ge.view(np.uint32), ref.e.view(np.uint32))
+    assert np.array_equal(gb.view(np.uint32), ref.b.view(np.uint32))
+
+    rng = np.random.default_rng(90 + order)
+    nxp, dxp, dtp, steps = (48, 24), (0.1, 0.1), 0.06, 30
+    ix, x, u, q = thermal_plasma(rng, 2, nxp, 8, 0.2)
+    ref = O.Sim(2, nxp, dxp, dtp, order, open_x=True)
+    ref.add_species(-1.0, pusher, ix, x, u, q)
+    with oa.Simulation(2, nxp, dxp, dtp, order, tile=(8, 8), xbnd="open") as sim:
+        sid = sim.add_species(-1.0, pusher)
+        sim.set_particles(sid, ix, x, u, q)
+        sim.run(steps)
+        ref.step(steps)
+        n_gpu = sim.count(sid)
+        fld, par = sim.history()
+    n_ref = len(ref.species[0].q)
+    assert n_ref < len(q) and abs(n_gpu - n_ref) <= 2
+    tol = 1e-4 + 2.0 * abs(n_gpu - n_ref) / n_ref
+    rfld, rpar = np.array(ref.fld_hist), np.array(ref.par_hist)
+    assert np.max(np.abs(par - rpar) / np.abs(rpar)) <= tol
+    fscale = np.maximum(rfld.sum(1, keepdims=True), 1e-300)  # row 0: no field yet
+    assert np.max(np.abs(fld - rfld) / fscale) <= tol
+
+
 def test_open_tile_and_kernel_independence():
     nx, dx, dt = (128, 64), (0.05, 0.05), 0.035
     out = []

@@ -76,6 +76,24 @@ def test_laser_vacuum_bitexact_vs_fp32_oracle():
     assert fld[-1].sum() > 0
 
 
+@pytest.mark.parametrize("order", [1, 3])
+def test_laser_vacuum_other_orders(order):
+    """Linear and cubic interpolation launch at x = xmin itself (no half-cell shift)."""
+    nx, dx, dt, steps = (64, 24), (0.2, 0.3), 0.1, 80
+    ref = O.Sim(2, nx, dx, dt, order, prec="f32", open_x=True)
+    with oa.Simulation(2, nx, dx, dt, order, tile=(8, 8), xbnd="open") as sim:
+        for kw in PULSES[:2]:
+            ref.add_zpulse_wall(**kw)
+            sim.add_zpulse_wall(**kw)
+        sim.run(steps)
+        ref.step(steps)
+        ref.wall_refresh()
+        g = ref.grid
+        ge, gb = sim.get_fields(gcl=g.gcl, gcu=g.gcu)
+    assert np.array_equal(ge.view(np.uint32), ref.e.view(np.uint32))
+    assert np.array_equal(gb.view(np.uint32), ref.b.view(np.uint32))
+
+
 def test_laser_plasma_slab_vs_fp64_oracle():
     rng = np.random.default_rng(5)
     nx, dx, dt, steps = (256, 32), (0.2, 0.5), 0.14, 160
