@@ -437,8 +437,9 @@ def main():
     # per-kernel timing with HIP events on the simulation stream
     kt = sim.time_kernels(args.timing_steps)
     nsp = 2
-    t_launch = kt["push_deposit"] / nsp * 1e-3  # s per push launch (one species)
-    bytes_launch = (n_total / nsp) * BYTES_PER_PARTICLE + cells * BYTES_PER_CELL_PUSH
+    # both species (same pusher) are pushed by one launch
+    t_launch = kt["push_deposit"] * 1e-3
+    bytes_launch = n_total * BYTES_PER_PARTICLE + cells * BYTES_PER_CELL_PUSH
     achieved = bytes_launch / t_launch / 1e9
     log("kernel ms/step: " + ", ".join(f"{k} {v:.4f}" for k, v in kt.items()))
     kname = f"push_deposit_2d_t<{args.order},0,{tile[0]},{tile[1]}>"
@@ -447,8 +448,8 @@ def main():
         try:
             with open(args.traffic) as f:
                 tj = json.load(f)
-            # counters measured on the same kernel instantiation only
-            if tuple(tj.get("tile", ())) == tuple(tile):
+            # counters measured on the same kernel instantiation and launch shape only
+            if tuple(tj.get("tile", ())) == tuple(tile) and tj.get("species_per_launch") == nsp:
                 traffic = tj.get("hbm_bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
@@ -504,7 +505,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": kname + " (Boris; one launch per species)",
+            "kernel": kname + " (Boris; both species in one launch)",
             "kernel_ms": t_launch * 1e3,
             "algorithmic_bytes": bytes_launch,
         },
@@ -512,9 +513,9 @@ def main():
         "kernels_roofline": {
             # algorithmic bytes per step (HBM-bound kernels); measured traffic of every
             # kernel is in profiles/kernels_c2.md
-            "push_deposit": {"bytes": nsp * bytes_launch, "ms": kt["push_deposit"],
-                             "GB/s": nsp * bytes_launch / kt["push_deposit"] / 1e6,
-                             "frac": nsp * bytes_launch / kt["push_deposit"] / 1e6 / HBM_PEAK_GBS},
+            "push_deposit": {"bytes": bytes_launch, "ms": kt["push_deposit"],
+                             "GB/s": bytes_launch / kt["push_deposit"] / 1e6,
+                             "frac": bytes_launch / kt["push_deposit"] / 1e6 / HBM_PEAK_GBS},
             "yee": {"bytes": cells * BYTES_PER_CELL_YEE, "ms": kt["yee"],
                     "GB/s": cells * BYTES_PER_CELL_YEE / kt["yee"] / 1e6,
                     "frac": cells * BYTES_PER_CELL_YEE / kt["yee"] / 1e6 / HBM_PEAK_GBS},

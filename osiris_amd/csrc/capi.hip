@@ -658,23 +658,40 @@ push_fn pick_push2d_t(int order, int pusher, int tx, int ty) {
   return nullptr;
 }
 
-int launch_push(osr_sim* s, int sid) {
+// One push launch for species first .. first+count-1 (same pusher): the work list holds
+// their (species, tile) items back to back, so the workgroups of the later species fill
+// the tail of the earlier one's.  The species only share the read-only fields and the
+// integer current, so the result does not depend on the grouping.
+int launch_push(osr_sim* s, int first, int count = 1) {
+  const int pusher = s->sp[first]->pusher;
   push_fn f = nullptr;
   size_t lds = s->lds_push;
   int nt = kNT;
   if (s->dims == 2 && s->desc.kernel != 1) {
-    f = pick_push2d_t(s->order, s->sp[sid]->pusher, s->g.tx, s->g.ty);
+    f = pick_push2d_t(s->order, pusher, s->g.tx, s->g.ty);
     if (f) {
       lds = 0;
       nt = push2d_t_threads(s->g.tx, s->g.ty);
     }
   }
-  if (!f) f = pick_push(s->dims, s->order, s->sp[sid]->pusher);
+  if (!f) f = pick_push(s->dims, s->order, pusher);
   if (!f) return fail(OSR_ERR_ARG, "no push kernel for dims=%d order=%d", s->dims, s->order);
   if (nt != 64 * s->push_waves) return fail(OSR_ERR_STATE, "push wave count mismatch");
-  hipLaunchKernelGGL(f, dim3(s->ntiles), dim3(nt), lds, s->stream,
-                     s->d_spec.as<SpecDesc>(), s->d_work.as<int2>() + (size_t)sid * s->ntiles);
+  hipLaunchKernelGGL(f, dim3((unsigned)(count * s->ntiles)), dim3(nt), lds, s->stream,
+                     s->d_spec.as<SpecDesc>(), s->d_work.as<int2>() + (size_t)first * s->ntiles);
   HIPC(hipGetLastError());
+  return OSR_OK;
+}
+
+// every species, one launch per run of consecutive species with the same pusher
+int launch_pushes(osr_sim* s) {
+  const int ns = (int)s->sp.size();
+  for (int a = 0; a < ns;) {
+    int b = a + 1;
+    while (b < ns && s->sp[b]->pusher == s->sp[a]->pusher) b++;
+    TRY(launch_push(s, a, b - a));
+    a = b;
+  }
   return OSR_OK;
 }
 
@@ -819,7 +836,7 @@ int enqueue_step(osr_sim* s) {
     TRY(launch_finalize(s));
     return OSR_OK;
   }
-  for (size_t i = 0; i < s->sp.size(); i++) TRY(launch_push(s, (int)i));
+  TRY(launch_pushes(s));
   if (!s->sp.empty()) TRY(launch_fill(s));
   TRY(launch_smooth(s));
   TRY(launch_yee(s));
@@ -1514,7 +1531,10 @@ int osr_sim_time_kernels(osr_sim* s, int nsteps, double* ms) {
       TRY(launch_laser(s));
     }
     HIPC(hipEventRecord(ev[0], s->stream));
-    for (size_t i = 0; i < s->sp.size(); i++) TRY(s->plain ? launch_plain(s, (int)i) : launch_push(s, (int)i));
+    if (s->plain)
+      for (size_t i = 0; i < s->sp.size(); i++) TRY(launch_plain(s, (int)i));
+    else
+      TRY(launch_pushes(s));
     HIPC(hipEventRecord(ev[1], s->stream));
     if (!s->sp.empty() && !s->plain) TRY(launch_fill(s));
     HIPC(hipEventRecord(ev[2], s->stream));
