@@ -172,7 +172,7 @@ def run_c3(args, oa):
     nx, dx, dt = (256, 256, 256), (0.1, 0.1, 0.1), 0.05
     tile = tuple(args.tile) if len(args.tile) == 3 else (8, 8, 4)
     steps = min(args.steps, 100)
-    sim = oa.Simulation(3, nx, dx, dt, 3, tile=tile)
+    sim = oa.Simulation(3, nx, dx, dt, 3, tile=tile, reorder=args.reorder)
     sid = sim.add_species(-1.0, oa.OSR_VAY)
     sim.generate(sid, (2, 2, 2), density=1.0, uth=(0.1, 0.1, 0.1), seed=3, random_pos=True)
     sim.fill_fields((0.0, 0.0, 0.0), (0.0, 0.0, 0.5))
@@ -201,7 +201,7 @@ def run_c3(args, oa):
         "config": {"workload": ("3D periodic magnetized thermal plasma 256^3, dx=0.1, "
                                 "dt=0.05, 8 ppc, u_th=0.1, B0=0.5 z, cubic splines, Vay"),
                    "particles": n_total, "cells": cells, "tile": list(tile),
-                   "relayouts": relayouts},
+                   "reorder": args.reorder, "relayouts": relayouts},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "push_deposit_3d<3,1,256>", "kernel_ms": t_launch * 1e3,
@@ -357,6 +357,9 @@ def main():
                     help="c2: the headline 2D two-stream step (default); c3: 3D periodic "
                          "magnetized thermal plasma 256^3, 8 ppc, cubic, Vay; c4: 2D "
                          "laser-plasma 2048x512 between open x walls")
+    ap.add_argument("--reorder", type=int, default=0,
+                    help="in-tile particle order restored every N steps (0 = library default: "
+                         "4 in 3D, off in 2D; negative = off)")
     ap.add_argument("--autotune", action="store_true", default=True)
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
     ap.add_argument("--plain-steps", type=int, default=20,

@@ -132,7 +132,11 @@ typedef struct {
                      (tiles 8x8, 16x16, 32x16); both accumulate the current in int64
                      fixed point in LDS */
   int xbnd;       /* x walls: OSR_BND_PERIODIC (0) or OSR_BND_OPEN (1) */
-  int reserved;
+  int reorder;    /* fused path: restore the (rank in cell, cell) particle order inside the
+                     tiles every this many steps; 0 = default (4 in 3D, off in 2D),
+                     negative = off.  Changes no result beyond FP64 round-off of the
+                     energy sums (the counterpart of the reference's periodic sort,
+                     sort_species, spec/os-species.f03:269-332) */
 } osr_sim_desc;
 
 /* x boundaries of a resident simulation.  OSR_BND_OPEN: Lindman absorbing field walls at

@@ -173,6 +173,7 @@ struct osr_sim {
   bool plain = false;
   size_t lds_push = 0;
   int push_waves = 4;  // waves per push workgroup (SpecDesc::waves)
+  int reorder = 0;     // tile_reorder period in steps (0: off)
   int sm_order[3] = {0, 0, 0};  // current smoothing per direction (0: none)
   float sm_coef[3][31] = {};
   DBuf sm_buf[2];
@@ -704,6 +705,17 @@ struct FillKernel {
 };
 FillKernel pick_fill() { return {tile_fill<256, 4096, 32768>, 256}; }
 
+// the in-tile particle order restored every s->reorder steps (the kernel itself checks the
+// step and returns at once on the other steps)
+int launch_reorder(osr_sim* s) {
+  if (s->reorder <= 0 || s->plain) return OSR_OK;
+  const int nw = (int)s->sp.size() * s->ntiles;
+  hipLaunchKernelGGL((tile_reorder<256, 64, 512, 4096>), dim3(nw), dim3(256), 0, s->stream,
+                     s->d_spec.as<SpecDesc>(), s->d_work.as<int2>(), s->reorder);
+  HIPC(hipGetLastError());
+  return OSR_OK;
+}
+
 int launch_fill(osr_sim* s) {
   const int nw = (int)s->sp.size() * s->ntiles;
   const FillKernel k = pick_fill();
@@ -838,6 +850,7 @@ int enqueue_step(osr_sim* s) {
   }
   TRY(launch_pushes(s));
   if (!s->sp.empty()) TRY(launch_fill(s));
+  if (!s->sp.empty()) TRY(launch_reorder(s));
   TRY(launch_smooth(s));
   TRY(launch_yee(s));
   TRY(launch_finalize(s));
@@ -986,6 +999,7 @@ int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
   }
   s->lds_push = s->plain ? 0 : push_lds_bytes(dims, s->order, tile);
   if (s->lds_push > 160 * 1024) return fail(OSR_ERR_ARG, "tile too large for LDS");
+  s->reorder = desc->reorder > 0 ? desc->reorder : (desc->reorder == 0 && dims == 3 ? 4 : 0);
   s->push_waves = kWaves;
   if (!s->plain && dims == 2 && desc->kernel != 1 && pick_push2d_t(s->order, 0, g.tx, g.ty))
     s->push_waves = push2d_t_threads(g.tx, g.ty) / 64;
@@ -1536,7 +1550,10 @@ int osr_sim_time_kernels(osr_sim* s, int nsteps, double* ms) {
     else
       TRY(launch_pushes(s));
     HIPC(hipEventRecord(ev[1], s->stream));
-    if (!s->sp.empty() && !s->plain) TRY(launch_fill(s));
+    if (!s->sp.empty() && !s->plain) {
+      TRY(launch_fill(s));
+      TRY(launch_reorder(s));
+    }
     HIPC(hipEventRecord(ev[2], s->stream));
     TRY(launch_yee(s));
     HIPC(hipEventRecord(ev[3], s->stream));

@@ -721,6 +721,177 @@ __global__ __launch_bounds__(NT) void tile_fill(const SpecDesc* __restrict__ des
 }
 
 // ------------------------------------------------------------------------------------
+// Re-interleave the particles of each tile: slot order (rank within the cell, cell), the
+// order the loader creates, so that the 64 lanes of a wave hold 64 different, consecutive
+// cells and their LDS field reads and current updates spread over the banks.  The fill
+// perturbs that order step by step; this restores it every `period` steps (after the
+// fill of steps period-1, 2 period-1, ...).  Ranks follow the slot order, so the result
+// is deterministic; the current is an integer sum and no arithmetic depends on the order
+// (only the FP64 energy partials, at round-off).  Each particle array of the tile is
+// permuted through LDS (coalesced read, LDS scatter, coalesced write back).  A step whose
+// fill overflowed is skipped (the host re-lays those particles out), as are tiles with
+// more than MAXN particles or RMAX particles in one cell.
+// ------------------------------------------------------------------------------------
+template <int NT, int RMAX, int MAXC, int MAXN>
+__global__ __launch_bounds__(NT) void tile_reorder(const SpecDesc* __restrict__ descs,
+                                                   const int2* __restrict__ work, int period) {
+  constexpr int MAXW = RMAX * MAXC / 32;
+  __shared__ int s_cnt[MAXC];
+  __shared__ unsigned int s_bits[MAXW];
+  __shared__ int s_wpre[MAXW];
+  __shared__ int s_key[MAXN];
+  __shared__ uint32_t s_tmp[MAXN];
+  __shared__ int s_tot[NT / 64];
+  __shared__ int s_abort;
+  const int2 wk = work[blockIdx.x];
+  const SpecDesc& d = descs[wk.x];
+  const GridDesc& g = d.g;
+  if (period <= 0 || g.ctrl->halt != 0 || g.ctrl->steps % period != period - 1) return;
+  const int tile = wk.y;
+  const int base = d.off[tile], n = d.cnt[tile];
+  const int dims = g.dims;
+  const int C = g.tx * g.ty * (dims == 3 ? g.tz : 1);
+  if (n < 2 || n > MAXN || C > MAXC) return;
+  const int cx0 = (tile % g.ntx) * g.tx, cy0 = ((tile / g.ntx) % g.nty) * g.ty;
+  const int cz0 = dims == 3 ? (tile / (g.ntx * g.nty)) * g.tz : 0;
+  for (int c = threadIdx.x; c < C; c += NT) s_cnt[c] = 0;
+  if (threadIdx.x == 0) s_abort = 0;
+  // the tile's packed cells, coalesced, into s_key (each entry is replaced by its key)
+  for (int p = threadIdx.x; p < n; p += NT) s_key[p] = (int)d.cell[base + p];
+  __syncthreads();
+  int nbits = 0;  // bits of a local cell index
+  while ((1 << nbits) < C) nbits++;
+  if (threadIdx.x < 64) {  // one wave: ranks within the cell in slot order
+    const int lane = threadIdx.x;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int c0 = 0; c0 < n; c0 += 64) {
+      const int p = c0 + lane;
+      int lc = -1 - lane;  // distinct for the lanes past the end
+      if (p < n) {
+        int ix, iy, iz = 0;
+        if (dims == 2)
+          unpack2((uint32_t)s_key[p], ix, iy);
+        else
+          unpack3((uint32_t)s_key[p], ix, iy, iz);
+        const int lx = ix - cx0, ly = iy - cy0, lz = iz - cz0;
+        if (lx < 0 || lx >= g.tx || ly < 0 || ly >= g.ty || lz < 0 ||
+            lz >= (dims == 3 ? g.tz : 1))
+          s_abort = 1;
+        else
+          lc = lx + g.tx * (ly + g.ty * lz);
+      }
+      // lanes holding the same cell: one ballot per bit of the cell index
+      unsigned long long same = __ballot(lc >= 0);
+      for (int b = 0; b < nbits; b++) {
+        const unsigned long long m = __ballot(lc >= 0 && ((lc >> b) & 1));
+        same &= ((lc >> b) & 1) ? m : ~m;
+      }
+      const int before = __popcll(same & below);
+      const int after = __popcll(same & ~below) - 1;
+      if (lc >= 0) {
+        const int r = s_cnt[lc] + before;
+        if (r >= RMAX) s_abort = 1;
+        s_key[p] = r * C + lc;
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (lc >= 0 && after == 0) s_cnt[lc] += before + 1;
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  __syncthreads();
+  if (s_abort) return;
+  // occupancy of the keys (rank, cell): occupied iff the cell holds more than rank
+  const int nw = (RMAX * C + 31) / 32;
+  for (int w = threadIdx.x; w < nw; w += NT) {
+    unsigned int m = 0u;
+    int r = (32 * w) / C, c = (32 * w) % C;  // key 32 w = r C + c, walked bit by bit
+    for (int b = 0; b < 32; b++) {
+      if (r < RMAX && s_cnt[c] > r) m |= 1u << b;
+      if (++c == C) {
+        c = 0;
+        r++;
+      }
+    }
+    s_bits[w] = m;
+  }
+  __syncthreads();
+  // exclusive prefix of the word populations: each thread owns consecutive words
+  constexpr int PER = (MAXW + NT - 1) / NT;
+  const int w0 = threadIdx.x * PER;
+  int own = 0;
+  for (int k = 0; k < PER; k++)
+    if (w0 + k < nw) own += __popc(s_bits[w0 + k]);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int inc = own;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  if (lane == 63) s_tot[wave] = inc;
+  __syncthreads();
+  int run = inc - own;
+  for (int w = 0; w < wave; w++) run += s_tot[w];
+  for (int k = 0; k < PER; k++)
+    if (w0 + k < nw) {
+      s_wpre[w0 + k] = run;
+      run += __popc(s_bits[w0 + k]);
+    }
+  __syncthreads();
+  // new slot of every particle, in place of its key
+  for (int p = threadIdx.x; p < n; p += NT) {
+    const int k = s_key[p];
+    s_key[p] = s_wpre[k >> 5] + __popc(s_bits[k >> 5] & ((1u << (k & 31)) - 1u));
+  }
+  __syncthreads();
+  // permute each array through LDS: coalesced 16-byte reads of 4 consecutive slots per
+  // thread (tile bases are multiples of 64 slots), LDS scatter, coalesced writes back
+  constexpr int PT = MAXN / (4 * NT);  // 4-slot groups per thread
+  int dst[PT][4];
+#pragma unroll
+  for (int j = 0; j < PT; j++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int p = 4 * (threadIdx.x + NT * j) + e;
+      dst[j][e] = p < n ? s_key[p] : -1;
+    }
+  uint32_t* arr[9] = {d.cell, (uint32_t*)d.x, (uint32_t*)d.y, (uint32_t*)d.z, (uint32_t*)d.ux,
+                      (uint32_t*)d.uy, (uint32_t*)d.uz, (uint32_t*)d.q, d.pid};
+#pragma unroll
+  for (int a = 0; a < 9; a++) {
+    if (a == 3 && dims != 3) continue;
+    uint32_t* A = arr[a] + base;
+#pragma unroll
+    for (int j = 0; j < PT; j++) {
+      const int q = 4 * (threadIdx.x + NT * j);
+      if (q + 3 < n) {
+        const uint4 v = *(const uint4*)(A + q);
+        s_tmp[dst[j][0]] = v.x;
+        s_tmp[dst[j][1]] = v.y;
+        s_tmp[dst[j][2]] = v.z;
+        s_tmp[dst[j][3]] = v.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+          if (q + e < n) s_tmp[dst[j][e]] = A[q + e];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PT; j++) {
+      const int q = 4 * (threadIdx.x + NT * j);
+      if (q + 3 < n) {
+        *(uint4*)(A + q) = make_uint4(s_tmp[q], s_tmp[q + 1], s_tmp[q + 2], s_tmp[q + 3]);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+          if (q + e < n) A[q + e] = s_tmp[q + e];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------
 // explicit instantiations
 // ------------------------------------------------------------------------------------
 #define OSR_INST(O, P)                                                                    \
@@ -733,5 +904,6 @@ OSR_INST(1, 1)
 OSR_INST(2, 1)
 OSR_INST(3, 1)
 template __global__ void tile_fill<256, 4096, 32768>(const SpecDesc*, const int2*);
+template __global__ void tile_reorder<256, 64, 512, 4096>(const SpecDesc*, const int2*, int);
 
 }  // namespace osr

@@ -35,7 +35,7 @@ module m_osiris_amd
     integer(c_int) :: path = OSR_PATH_FUSED
     integer(c_int) :: kernel = 0
     integer(c_int) :: xbnd = OSR_BND_PERIODIC
-    integer(c_int) :: reserved = 0
+    integer(c_int) :: reorder = 0
   end type osr_sim_desc
 
   ! current smoothing (vdf/os-vdf-smooth.f90): the reference's filter type codes and the

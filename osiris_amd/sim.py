@@ -114,13 +114,14 @@ class Simulation:
     """A resident simulation on the GPU (one per grid; species share the grid)."""
 
     def __init__(self, dims, nx, dx, dt, order, tile=None, plain=False, kernel=0,
-                 xbnd="periodic"):
+                 xbnd="periodic", reorder=0):
         """xbnd: "periodic", or "open" for Lindman absorbing field walls and absorbing
         particles at both x ends (2D fused path, nx a multiple of the tile width)."""
         d = SimDesc()
         d.path = 1 if plain else 0
         d.kernel = kernel
         d.xbnd = {"periodic": OSR_BND_PERIODIC, "open": OSR_BND_OPEN}[xbnd]
+        d.reorder = reorder  # in-tile particle order restored every `reorder` steps
         d.dims = dims
         for i in range(3):
             d.nx[i] = nx[i] if i < dims else 1
