@@ -143,8 +143,8 @@ typedef struct {
  * both x ends (emf_bound type "lindman", emf/boundary/os-emf-lindman.f90:190-941, applied
  * by os-emf-boundary.f90:723-1112) and absorbing particles (spec_bound type "absorbing" /
  * "open", spec/os-spec-boundary.f03:150-151, 476-513: particles leaving through x are
- * removed after the push); y stays periodic.  2D fused path only, nx a multiple of the
- * tile width, no current smoothing; osr_sim_gauss is not available. */
+ * removed after the push); y (and z) stay periodic.  Fused path, 2D and 3D, nx a multiple
+ * of the tile width, no current smoothing; osr_sim_gauss is not available. */
 enum { OSR_BND_PERIODIC = 0, OSR_BND_OPEN = 1 };
 
 /* OSR_PATH_FUSED: tiled fused push+deposit, LDS-staged fields and current, particles

@@ -83,25 +83,26 @@ def _lindman_struct(real):
     R = C.c_double if real == np.float64 else C.c_float
 
     class Lindman(C.Structure):
-        _fields_ = [("disp_corr", R), ("dtdxi", R * 2), ("ebl", C.c_void_p),
+        _fields_ = [("disp_corr", R), ("dtdxi", R * 3), ("ebl", C.c_void_p),
                     ("bbl", C.c_void_p), ("ebu", C.c_void_p)]
     return Lindman
 
 
 class LindmanWalls:
-    """Lindman x walls of a 2D grid (new_lindman, emf/boundary/os-emf-lindman.f90:190-330):
-    zeroed wall buffers and the coefficients in the working precision."""
+    """Lindman x walls of a 2D or 3D grid (new_lindman, emf/boundary/os-emf-lindman.f90:
+    190-330): zeroed wall buffers and the coefficients in the working precision."""
 
     def __init__(self, grid, dx, dt, real):
         ld1 = grid.nx[1] + grid.gcl[1] + grid.gcu[1]
-        self.ebl = np.zeros((ld1, 2, 3), real)
-        self.bbl = np.zeros((ld1, 2, 3), real)
-        self.ebu = np.zeros((ld1, 3, 3), real)
+        ld2 = grid.nx[2] + grid.gcl[2] + grid.gcu[2] if grid.dims > 2 else 1
+        self.ebl = np.zeros((ld2, ld1, 2, 3), real)
+        self.bbl = np.zeros((ld2, ld1, 2, 3), real)
+        self.ebu = np.zeros((ld2, ld1, 3, 3), real)
         S = _lindman_struct(real)
         self.c = S()
         r = real
         self.c.disp_corr = r((1.0 - dt / dx[0]) / (1.0 + dt / dx[0]))
-        for d in range(2):
+        for d in range(len(dx)):
             self.c.dtdxi[d] = r(r(dt / dx[d]) * r(0.5))
         self.c.ebl, self.c.bbl, self.c.ebu = (a.ctypes.data for a in (self.ebl, self.bbl, self.ebu))
 
@@ -484,8 +485,8 @@ class Sim:
         self.walls = None
         self.zpulses = []
         if self.open_x:
-            if self.dims != 2 or self.smooth_order is not None:
-                raise ValueError("open x walls are implemented for 2D without smoothing")
+            if self.smooth_order is not None:
+                raise ValueError("open x walls are implemented without smoothing")
             self.walls = LindmanWalls(self.grid, self.dx, self.dt, r)
         self.fld_hist = []
         self.par_hist = []
@@ -538,7 +539,7 @@ class Sim:
         L = _lib(self.prec)
         g = self.grid.c()
         ns = len(self.species)
-        dxa = np.array(list(self.dx) + [1.0], np.float64)
+        dxa = np.array(list(self.dx) + [1.0] * (3 - self.dims), np.float64)
         IA = C.c_void_p * ns
         pus = np.array([s.pusher for s in self.species], np.int32)
         rqm = np.array([s.rqm for s in self.species], self.real)
@@ -547,6 +548,8 @@ class Sim:
         xs = IA(*[s.x.ctypes.data for s in self.species])
         us = IA(*[s.u.ctypes.data for s in self.species])
         qs = IA(*[s.q.ctypes.data for s in self.species])
+        if self.zpulses and self.dims != 2:
+            raise ValueError("zpulse_wall is implemented in 2D")
         for _ in range(n):
             # zpulse launch at t = n dt before the step (os-simulation.f03:528-531)
             t = len(self.fld_hist) * self.dt

@@ -1086,90 +1086,129 @@ void orc_step(const orc_grid *g, int order, const double *dx, double dt, real *e
 }
 
 /* ------------------------------------------------------------------------------------
- * Open x walls, 2D: Lindman absorbing field boundaries on both x walls
+ * Open x walls, 2D and 3D: Lindman absorbing field boundaries on both x walls
  * (emf/boundary/os-emf-lindman.f90) and absorbing particles
  * (spec/os-spec-boundary.f03:150-151, 476-513: particles with ix < 1 or ix > nx after
- * the push are removed); y stays periodic.
+ * the push are removed); y (and z) stay periodic.
  *
  * The wall buffers are t_wall objects (vdf/os-wall.f90:111-215): zero at creation,
  * x range (0,1) at the lower wall and (nx, nx+2) at the upper one
- * (os-emf-lindman.f90:308-324), y index 1-gcl..ny+gcu.  They are stored here as
- * (3, ncol, ld1) with ncol = 2 (lower) or 3 (upper); the upper wall's B buffer is never
- * updated nor read (os-emf-lindman.f90:572-573, 994-1011) and is not kept.
+ * (os-emf-lindman.f90:308-324), y index 1-gcl..ny+gcu (z likewise).  They are stored here
+ * as (3, ncol, ld1[, ld2]) with ncol = 2 (lower) or 3 (upper); the upper wall's B buffer
+ * is never read (os-emf-lindman.f90:572-573, 994-1011) and is not kept.
  * ------------------------------------------------------------------------------------ */
 typedef struct {
   real disp_corr;  /* (1 - dt/dx1) / (1 + dt/dx1), os-emf-lindman.f90:237-238 */
-  real dtdxi[2];   /* real(dt/dx, p_k_fld) * 0.5, os-emf-lindman.f90:236, 241 */
+  real dtdxi[3];   /* real(dt/dx, p_k_fld) * 0.5, os-emf-lindman.f90:236, 241 */
   real *ebl, *bbl; /* lower wall E and B buffers, x = 0, 1 */
   real *ebu;       /* upper wall E buffer, x = nx, nx+1, nx+2 */
 } orc_lindman;
 
-#define WB(w, ncol, c, col, j) ((w)[3 * ((col) + (ncol) * (long)((j) - 1 + g->gcl[1])) + (c)])
-
-/* update_periodic_wall with p_vdf_replace along y, vdf/os-wall-comm.f90:182-344 */
-static void wall_periodic_y(const orc_grid *g, real *w, int ncol) {
-  const int ny = g->nx[1];
-  for (int j = 1 - g->gcl[1]; j <= 0; j++)
-    for (int col = 0; col < ncol; col++)
-      for (int c = 0; c < 3; c++) WB(w, ncol, c, col, j) = WB(w, ncol, c, col, j + ny);
-  for (int j = ny + 1; j <= ny + g->gcu[1]; j++)
-    for (int col = 0; col < ncol; col++)
-      for (int c = 0; c < 3; c++) WB(w, ncol, c, col, j) = WB(w, ncol, c, col, j - ny);
+/* wall buffer entry: component c, x column col, y index j, z index k (k = 1 in 2D) */
+static inline long wb_idx(const orc_grid *g, int ncol, int c, int col, int j, int k) {
+  long ld1 = ld_(g, 1);
+  long kk = g->dims > 2 ? (long)(k - 1 + g->gcl[2]) : 0;
+  return 3 * ((long)col + ncol * ((long)(j - 1 + g->gcl[1]) + ld1 * kk)) + c;
 }
+#define WB(w, ncol, c, col, j, k) ((w)[wb_idx(g, ncol, c, col, j, k)])
+#define Z3(g) ((g)->dims > 2)
 
-/* update_lindman_2d_e, idir = 1, os-emf-lindman.f90:454-535 (perp = components 2, 3) */
-static void lindman_e(const orc_grid *g, orc_lindman *w, real *e) {
-  const int nx = g->nx[0];
-  for (int j = 1; j <= g->nx[1]; j++)
-    for (int p = 1; p <= 2; p++) {
-      /* lower wall, i_range = (0, 1) */
-      FLD(e, g, p, 0, j, 1) = WB(w->ebl, 2, p, 1, j) +
-                              w->disp_corr * (WB(w->ebl, 2, p, 0, j) - FLD(e, g, p, 1, j, 1));
-      WB(w->ebl, 2, p, 0, j) = FLD(e, g, p, 0, j, 1);
-      WB(w->ebl, 2, p, 1, j) = FLD(e, g, p, 1, j, 1);
+/* update_periodic_wall with p_vdf_replace along y, then z (vdf/os-wall-comm.f90:182-344):
+ * guard entries of the buffer take the periodic images, over the full range of the
+ * other direction */
+static void wall_periodic(const orc_grid *g, real *w, int ncol) {
+  const int ny = g->nx[1], nz = Z3(g) ? g->nx[2] : 1;
+  const int klo = Z3(g) ? 1 - g->gcl[2] : 1, khi = Z3(g) ? nz + g->gcu[2] : 1;
+  for (int k = klo; k <= khi; k++)
+    for (int j = 1 - g->gcl[1]; j <= ny + g->gcu[1]; j++) {
+      int s = j < 1 ? j + ny : (j > ny ? j - ny : j);
+      if (s == j) continue;
+      for (int col = 0; col < ncol; col++)
+        for (int c = 0; c < 3; c++) WB(w, ncol, c, col, j, k) = WB(w, ncol, c, col, s, k);
     }
-  for (int j = 1; j <= g->nx[1]; j++)
-    for (int p = 1; p <= 2; p++) {
-      /* upper wall, i_range = (nx, nx+2): buffer columns 1, 2 are x = nx+1, nx+2 */
-      FLD(e, g, p, nx + 2, j, 1) =
-          WB(w->ebu, 3, p, 1, j) + w->disp_corr * (WB(w->ebu, 3, p, 2, j) - FLD(e, g, p, nx + 1, j, 1));
-      WB(w->ebu, 3, p, 1, j) = FLD(e, g, p, nx + 1, j, 1);
-      WB(w->ebu, 3, p, 2, j) = FLD(e, g, p, nx + 2, j, 1);
-    }
-}
-
-/* update_lindman_2d_b, idir = 1, lower wall only, os-emf-lindman.f90:539-600 */
-static void lindman_b(const orc_grid *g, orc_lindman *w, const real *e) {
-  const real d1 = w->dtdxi[0], d2 = w->dtdxi[1];
-  for (int j = 1; j <= g->nx[1]; j++) {
-    real *bb = &WB(w->bbl, 2, 0, 0, j);
-    bb[0] = bb[0] - d2 * (FLD(e, g, 2, 0, j + 1, 1) - FLD(e, g, 2, 0, j, 1));
-    bb[1] = bb[1] + d1 * (FLD(e, g, 2, 1, j, 1) - FLD(e, g, 2, 0, j, 1));
-    bb[2] = bb[2] - d1 * (FLD(e, g, 1, 1, j, 1) - FLD(e, g, 1, 0, j, 1)) +
-            d2 * (FLD(e, g, 0, 0, j + 1, 1) - FLD(e, g, 0, 0, j, 1));
+  if (!Z3(g)) return;
+  for (int k = klo; k <= khi; k++) {
+    int s = k < 1 ? k + nz : (k > nz ? k - nz : k);
+    if (s == k) continue;
+    for (int j = 1 - g->gcl[1]; j <= ny + g->gcu[1]; j++)
+      for (int col = 0; col < ncol; col++)
+        for (int c = 0; c < 3; c++) WB(w, ncol, c, col, j, k) = WB(w, ncol, c, col, j, s);
   }
 }
 
-/* update_boundary_emf_bound for open x / periodic y on one node,
- * emf/boundary/os-emf-boundary.f90:723-830: wall buffers refreshed along y (:768-769,
- * os-emf-lindman.f90:352-368), then the x walls restore the wall values
- * (update_emfbound_lindman, os-emf-lindman.f90:861-941: B and E at x = 0 from the lower
- * buffers, E at x = nx+1 from the upper one, every y), then the periodic y refresh. */
-void orc_open_boundary(const orc_grid *g, orc_lindman *w, real *e, real *b) {
-  wall_periodic_y(g, w->bbl, 2);
-  wall_periodic_y(g, w->ebl, 2);
-  wall_periodic_y(g, w->ebu, 3);
-  const int nx = g->nx[0];
-  for (int j = 1 - g->gcl[1]; j <= g->nx[1] + g->gcu[1]; j++)
-    for (int c = 0; c < 3; c++) {
-      FLD(b, g, c, 0, j, 1) = WB(w->bbl, 2, c, 0, j);
-      FLD(e, g, c, 0, j, 1) = WB(w->ebl, 2, c, 0, j);
-      FLD(e, g, c, nx + 1, j, 1) = WB(w->ebu, 3, c, 1, j);
+/* update_lindman_2d_e / update_lindman_3d_e, idir = 1 (os-emf-lindman.f90:454-535,
+ * 604-750): perp = components 2, 3, over y = 1..ny (and z = 1..nz) */
+static void lindman_e(const orc_grid *g, orc_lindman *w, real *e) {
+  const int nx = g->nx[0], nz = Z3(g) ? g->nx[2] : 1;
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= g->nx[1]; j++)
+      for (int p = 1; p <= 2; p++) {
+        /* lower wall, i_range = (0, 1) */
+        FLD(e, g, p, 0, j, k) = WB(w->ebl, 2, p, 1, j, k) +
+                                w->disp_corr * (WB(w->ebl, 2, p, 0, j, k) - FLD(e, g, p, 1, j, k));
+        WB(w->ebl, 2, p, 0, j, k) = FLD(e, g, p, 0, j, k);
+        WB(w->ebl, 2, p, 1, j, k) = FLD(e, g, p, 1, j, k);
+      }
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= g->nx[1]; j++)
+      for (int p = 1; p <= 2; p++) {
+        /* upper wall, i_range = (nx, nx+2): buffer columns 1, 2 are x = nx+1, nx+2 */
+        FLD(e, g, p, nx + 2, j, k) = WB(w->ebu, 3, p, 1, j, k) +
+            w->disp_corr * (WB(w->ebu, 3, p, 2, j, k) - FLD(e, g, p, nx + 1, j, k));
+        WB(w->ebu, 3, p, 1, j, k) = FLD(e, g, p, nx + 1, j, k);
+        WB(w->ebu, 3, p, 2, j, k) = FLD(e, g, p, nx + 2, j, k);
+      }
+}
+
+/* update_lindman_2d_b (os-emf-lindman.f90:539-600) / update_lindman_3d_b (:755-857),
+ * idir = 1, the lower wall's buffer (the 3D routine also advances the upper wall's,
+ * which is never read) */
+static void lindman_b(const orc_grid *g, orc_lindman *w, const real *e) {
+  const real d1 = w->dtdxi[0], d2 = w->dtdxi[1], d3 = w->dtdxi[2];
+  const int nz = Z3(g) ? g->nx[2] : 1;
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= g->nx[1]; j++) {
+      real *bb = &WB(w->bbl, 2, 0, 0, j, k);
+      if (!Z3(g)) {
+        bb[0] = bb[0] - d2 * (FLD(e, g, 2, 0, j + 1, 1) - FLD(e, g, 2, 0, j, 1));
+        bb[1] = bb[1] + d1 * (FLD(e, g, 2, 1, j, 1) - FLD(e, g, 2, 0, j, 1));
+        bb[2] = bb[2] - d1 * (FLD(e, g, 1, 1, j, 1) - FLD(e, g, 1, 0, j, 1)) +
+                d2 * (FLD(e, g, 0, 0, j + 1, 1) - FLD(e, g, 0, 0, j, 1));
+      } else {
+        bb[0] = bb[0] - d2 * (FLD(e, g, 2, 0, j + 1, k) - FLD(e, g, 2, 0, j, k)) +
+                d3 * (FLD(e, g, 1, 0, j, k + 1) - FLD(e, g, 1, 0, j, k));
+        bb[1] = bb[1] - d3 * (FLD(e, g, 0, 0, j, k + 1) - FLD(e, g, 0, 0, j, k)) +
+                d1 * (FLD(e, g, 2, 1, j, k) - FLD(e, g, 2, 0, j, k));
+        bb[2] = bb[2] - d1 * (FLD(e, g, 1, 1, j, k) - FLD(e, g, 1, 0, j, k)) +
+                d2 * (FLD(e, g, 0, 0, j + 1, k) - FLD(e, g, 0, 0, j, k));
+      }
     }
+}
+
+/* update_boundary_emf_bound for open x, periodic y (and z), one node,
+ * emf/boundary/os-emf-boundary.f90:723-830: wall buffers refreshed along y, z
+ * (:768-769, os-emf-lindman.f90:352-368), then the x walls restore the wall values
+ * (update_emfbound_lindman, os-emf-lindman.f90:861-1011: B and E at x = 0 from the
+ * lower buffers, E at x = nx+1 from the upper one, every y, z), then the periodic
+ * refresh of y (and z). */
+void orc_open_boundary(const orc_grid *g, orc_lindman *w, real *e, real *b) {
+  wall_periodic(g, w->bbl, 2);
+  wall_periodic(g, w->ebl, 2);
+  wall_periodic(g, w->ebu, 3);
+  const int nx = g->nx[0];
+  const int klo = Z3(g) ? 1 - g->gcl[2] : 1, khi = Z3(g) ? g->nx[2] + g->gcu[2] : 1;
+  for (int k = klo; k <= khi; k++)
+    for (int j = 1 - g->gcl[1]; j <= g->nx[1] + g->gcu[1]; j++)
+      for (int c = 0; c < 3; c++) {
+        FLD(b, g, c, 0, j, k) = WB(w->bbl, 2, c, 0, j, k);
+        FLD(e, g, c, 0, j, k) = WB(w->ebl, 2, c, 0, j, k);
+        FLD(e, g, c, nx + 1, j, k) = WB(w->ebu, 3, c, 1, j, k);
+      }
   update_periodic_from(g, e, 3, 0, 1);
   update_periodic_from(g, b, 3, 0, 1);
 }
 #undef WB
+#undef Z3
 
 /* advance_yee with Lindman x walls, os-emf-solver-yee.f03:542-568 with
  * update_boundary_bfld/efld (os-emf-boundary.f90:840-1112): no guard refresh between the
@@ -1184,17 +1223,19 @@ void orc_yee_advance_open(const orc_grid *g, const double *dx, double dt, real *
   lindman_b(g, w, e);
 }
 
-/* Particle boundaries: y periodic (as orc_particles_periodic), x absorbing: particles with
+/* Particle boundaries: y, z periodic (as orc_particles_periodic), x absorbing: particles with
  * ix < 1 or ix > nx are removed, the others keep their order.  Returns the new count. */
 int orc_particles_open(const orc_grid *g, int *ix, real *x, real *u, real *q, int np) {
   const int D = g->dims;
   int k = 0;
   for (int i = 0; i < np; i++) {
-    int *v = &ix[D * i + 1];
-    if (*v < 1)
-      *v += g->nx[1];
-    else if (*v > g->nx[1])
-      *v -= g->nx[1];
+    for (int d = 1; d < D; d++) {
+      int *v = &ix[D * i + d];
+      if (*v < 1)
+        *v += g->nx[d];
+      else if (*v > g->nx[d])
+        *v -= g->nx[d];
+    }
     if (ix[D * i] < 1 || ix[D * i] > g->nx[0]) continue;
     if (k != i) {
       for (int d = 0; d < D; d++) {
@@ -1209,14 +1250,14 @@ int orc_particles_open(const orc_grid *g, int *ix, real *x, real *u, real *q, in
   return k;
 }
 
-/* orc_step with open x walls (2D): boundary update of E/B (Lindman restore + periodic y),
- * push+deposit, particle boundaries, energies, J folded along y only, Yee with the
+/* orc_step with open x walls: boundary update of E/B (Lindman restore + periodic y, z),
+ * push+deposit, particle boundaries, energies, J folded along y, z only, Yee with the
  * Lindman wall updates.  np[s] is updated to the particles left. */
 void orc_step_open(const orc_grid *g, int order, const double *dx, double dt, real *e, real *b,
                    real *j, orc_lindman *w, int nspec, const int *pusher, const real *rqm,
                    int *const *ix, real *const *x, real *const *u, real *const *q, int *np,
                    double *fld_ene, double *par_ene) {
-  double dvol = dx[0] * dx[1];
+  double dvol = dx[0] * dx[1] * (g->dims > 2 ? dx[2] : 1.0);
   orc_open_boundary(g, w, e, b);
   memset(j, 0, sizeof(real) * 3 * orc_grid_cells(g));
   for (int s = 0; s < nspec; s++) {

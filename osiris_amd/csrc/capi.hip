@@ -306,7 +306,7 @@ void make_field_desc(const osr_sim* s, FieldDesc& f) {
   if (s->g.open_nx) {  // new_lindman, emf/boundary/os-emf-lindman.f90:235-241
     f.wall = s->wall.as<float>();
     f.wall_dc = (float)((1.0 - s->dt / s->dx[0]) / (1.0 + s->dt / s->dx[0]));
-    for (int d = 0; d < 2; d++) f.wall_dt[d] = (float)(s->dt / s->dx[d]) * 0.5f;
+    for (int d = 0; d < s->dims; d++) f.wall_dt[d] = (float)(s->dt / s->dx[d]) * 0.5f;
   }
   if (!s->zp.empty()) {
     f.las_k = zp_launches(s);
@@ -727,7 +727,7 @@ int launch_fill(osr_sim* s) {
 
 // E and B on the x walls from the Lindman buffers (the state every push starts from)
 int launch_wall_restore(osr_sim* s) {
-  const unsigned nb = (unsigned)ceil_div(s->g.ny, 256);
+  const unsigned nb = (unsigned)ceil_div(s->g.ny * s->g.nz, 256);
   hipLaunchKernelGGL(wall_restore, dim3(nb), dim3(256), 0, s->stream, s->d_fd.as<FieldDesc>());
   HIPC(hipGetLastError());
   return OSR_OK;
@@ -738,13 +738,22 @@ int launch_yee(osr_sim* s) {
   const int2* w = s->d_ywork.as<int2>();
   const int2* we = s->d_ework.as<int2>();
   if (s->g.open_nx) {  // advance_yee with Lindman x walls (os-emf-solver-yee.f03:542-568)
-    const unsigned nb = (unsigned)ceil_div(s->g.ny, 256);
-    hipLaunchKernelGGL((yee_dbdt<2, true>), dim3(s->nblk_e), dim3(256), 0, s->stream, fd, we);
-    hipLaunchKernelGGL(wall_b<1>, dim3(nb), dim3(256), 0, s->stream, fd);
-    hipLaunchKernelGGL((yee_dedt<2>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
-    hipLaunchKernelGGL(wall_e, dim3(nb), dim3(256), 0, s->stream, fd);
-    hipLaunchKernelGGL((yee_dbdt<2, false>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
-    hipLaunchKernelGGL(wall_b<2>, dim3(nb), dim3(256), 0, s->stream, fd);
+    const unsigned nb = (unsigned)ceil_div(s->g.ny * s->g.nz, 256);
+    if (s->dims == 2) {
+      hipLaunchKernelGGL((yee_dbdt<2, true>), dim3(s->nblk_e), dim3(256), 0, s->stream, fd, we);
+      hipLaunchKernelGGL((wall_b<1, 2>), dim3(nb), dim3(256), 0, s->stream, fd);
+      hipLaunchKernelGGL((yee_dedt<2>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
+      hipLaunchKernelGGL(wall_e, dim3(nb), dim3(256), 0, s->stream, fd);
+      hipLaunchKernelGGL((yee_dbdt<2, false>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
+      hipLaunchKernelGGL((wall_b<2, 2>), dim3(nb), dim3(256), 0, s->stream, fd);
+    } else {
+      hipLaunchKernelGGL((yee_dbdt<3, true>), dim3(s->nblk_e), dim3(256), 0, s->stream, fd, we);
+      hipLaunchKernelGGL((wall_b<1, 3>), dim3(nb), dim3(256), 0, s->stream, fd);
+      hipLaunchKernelGGL((yee_dedt<3>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
+      hipLaunchKernelGGL(wall_e, dim3(nb), dim3(256), 0, s->stream, fd);
+      hipLaunchKernelGGL((yee_dbdt<3, false>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
+      hipLaunchKernelGGL((wall_b<2, 3>), dim3(nb), dim3(256), 0, s->stream, fd);
+    }
     HIPC(hipGetLastError());
     return launch_wall_restore(s);
   }
@@ -954,8 +963,8 @@ int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
   if (desc->xbnd != OSR_BND_PERIODIC && desc->xbnd != OSR_BND_OPEN)
     return fail(OSR_ERR_ARG, "unknown x boundary type");
   const bool open_x = desc->xbnd == OSR_BND_OPEN;
-  if (open_x && (dims != 2 || desc->path != OSR_PATH_FUSED))
-    return fail(OSR_ERR_ARG, "open x walls need the 2D fused path");
+  if (open_x && desc->path != OSR_PATH_FUSED)
+    return fail(OSR_ERR_ARG, "open x walls need the fused path");
   std::unique_ptr<osr_sim> s(new osr_sim());
   s->desc = *desc;
   s->plain = desc->path == OSR_PATH_PLAIN;
@@ -980,7 +989,8 @@ int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
       return fail(OSR_ERR_ARG, "open x walls need nx to be a multiple of the tile width");
     g.open_nx = desc->nx[0];
     g.nx = desc->nx[0] + ceil_div(gc, tile[0]) * tile[0];
-    if (g.nx > 65535) return fail(OSR_ERR_ARG, "2D grids are limited to 65535 cells per dimension");
+    if (g.nx > (dims == 2 ? 65535 : 2047))
+      return fail(OSR_ERR_ARG, "open x walls: nx plus the guard band exceeds the grid limit");
   }
   g.ny = desc->nx[1];
   g.nz = dims == 3 ? desc->nx[2] : 1;
@@ -1010,7 +1020,7 @@ int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
   TRY(s->J.alloc(3 * nc * 8));
   TRY(s->ctrl.alloc(sizeof(Ctrl)));
   if (open_x) {
-    const size_t wb = sizeof(float) * ((size_t)kWallStale * g.ny + 1);
+    const size_t wb = sizeof(float) * ((size_t)kWallEntries * g.ny * g.nz + g.nz + g.ny);
     TRY(s->wall.alloc(wb));
     HIPC(hipMemsetAsync(s->wall.p, 0, wb, s->stream));
   }
@@ -2124,6 +2134,7 @@ int osr_sim_add_zpulse_wall(osr_sim* s, const osr_zpulse_wall* zp) {
   TRY(check_sim(s));
   if (!zp) return fail(OSR_ERR_ARG, "null argument");
   if (!s->g.open_nx) return fail(OSR_ERR_STATE, "the launch wall must be open (xbnd = OSR_BND_OPEN)");
+  if (s->dims != 2) return fail(OSR_ERR_ARG, "zpulse_wall is available in 2D");
   if ((int)s->zp.size() >= osr_zp::kMaxPulses) return fail(OSR_ERR_ARG, "at most 8 laser walls");
   osr_zp::Wall w;
   const int nx[2] = {s->g.open_nx, s->g.ny};

@@ -45,12 +45,13 @@ struct FieldDesc {
   int jsrc_clear;
   float sm_coef[3][31];  // per pass, s(k) at [15 + k]
   float* sm_buf[2];
-  // open x walls (g.open_nx > 0): Lindman buffers, planar, ny values per entry:
-  // ebl[3][2] (x = 0, 1), bbl[3] (x = 0), ebu[3][2] (x = nx+1, nx+2), then one value:
-  // E3(x = 0, y = 1) as it was when the step began
+  // open x walls (g.open_nx > 0): Lindman buffers, planar, one value per row (y, z) of the
+  // wall, R = ny nz values per entry: ebl[3][2] (x = 0, 1), bbl[3] (x = 0), ebu[3][2]
+  // (x = nx+1, nx+2); then E3(x = 0, y = 1, z) [nz] and E2(x = 0, y, z = 1) [ny] as they
+  // were when the step began
   float* wall;
   float wall_dc;     // real((1 - dt/dx1) / (1 + dt/dx1))
-  float wall_dt[2];  // real(dt/dx) * 0.5
+  float wall_dt[3];  // real(dt/dx) * 0.5
   // laser walls: las_k launches per step, into ring columns las_col[k]; for table row
   // r = step - *las_first: las_mask[r][k] (0: the launch adds nothing) and the values
   // las_tab[r][k][2][ny] (B2, B3 increments of rows y = 1..ny)
@@ -62,8 +63,8 @@ struct FieldDesc {
   const long long* las_first;
 };
 
-// offsets in FieldDesc::wall (units of ny values)
-constexpr int kWallEbl = 0, kWallBbl = 6, kWallEbu = 9, kWallStale = 15;
+// entries of FieldDesc::wall before the saved values (units of ny nz values)
+constexpr int kWallEntries = 15;
 
 OSR_DEV int wrp(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
@@ -222,71 +223,113 @@ __global__ __launch_bounds__(256) void yee_dedt(const FieldDesc* __restrict__ fd
 
 // ------------------------------------------------------------------------------------
 // Lindman absorbing walls at x = 0 (ring column nx-1) and x = nx+1, nx+2 (columns
-// open_nx, open_nx+1), 2D, one thread per row y.  emf/boundary/os-emf-lindman.f90:
-// update_lindman_2d_e :454-535 (after the E step), update_lindman_2d_b :539-600 (after
-// each B half step), update_emfbound_lindman :861-941 (wall values restored before the
-// next push).  The wall buffers start at zero (vdf/os-wall.f90:189-205).
+// open_nx, open_nx+1), one thread per row r = y + ny z of the wall.
+// emf/boundary/os-emf-lindman.f90: update_lindman_2d_e / 3d_e :454-535, 604-750 (after the
+// E step), update_lindman_2d_b / 3d_b :539-600, 755-857 (after each B half step),
+// update_emfbound_lindman :861-1011 (wall values restored before the next push).  The
+// wall buffers start at zero (vdf/os-wall.f90:189-205).
 // ------------------------------------------------------------------------------------
-OSR_DEV float* wall_at(const FieldDesc& fd, int k, int ny) { return fd.wall + (size_t)k * ny; }
+OSR_DEV float* wall_at(const FieldDesc& fd, int k, long long R) { return fd.wall + (size_t)k * R; }
 
-// B buffer of the lower wall.  STEP 2: the reference reads E3(0, ny+1), a guard cell that
-// nothing refreshes between the start of the step and the second B half step, so it still
-// holds E3(0, 1) of the start of the step (saved by wall_e).
-template <int STEP>
+// B buffer of the lower wall.  STEP 2: the reference reads E3(0, ny+1, z) and E2(0, y,
+// nz+1), guard cells that nothing refreshes between the start of the step and the second
+// B half step, so they still hold E3(0, 1, z) and E2(0, y, 1) of the start of the step
+// (saved by wall_e).  E1 on the wall is always zero.
+template <int STEP, int DIMS>
 __global__ __launch_bounds__(256) void wall_b(const FieldDesc* __restrict__ fds) {
   const FieldDesc& fd = *fds;
   const GridDesc& g = fd.g;
   if (g.ctrl->halt >= 2) return;
-  const int ny = g.ny, nx = g.nx;
-  const int r = blockIdx.x * 256 + threadIdx.x;
-  if (r >= ny) return;
+  const int ny = g.ny, nz = g.nz, nx = g.nx;
+  const long long R = (long long)ny * nz;
+  const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (r >= R) return;
+  const int j = (int)(r % ny), k = (int)(r / ny);
   const long long nc = g.ncell;
   const float* E1 = g.E;
   const float* E2 = g.E + nc;
   const float* E3 = g.E + 2 * nc;
-  const int rn = wrp(r + 1, ny);
-  const long long c0 = (long long)r * nx + nx - 1, c1 = (long long)r * nx;
-  const long long n0 = (long long)rn * nx + nx - 1;
-  const float e3n = (STEP == 2 && r == ny - 1) ? wall_at(fd, kWallStale, ny)[0] : E3[n0];
-  float* bb = wall_at(fd, kWallBbl, ny);
-  const float d1 = fd.wall_dt[0], d2 = fd.wall_dt[1];
-  bb[r] = bb[r] - d2 * (e3n - E3[c0]);
-  bb[ny + r] = bb[ny + r] + d1 * (E3[c1] - E3[c0]);
-  bb[2 * ny + r] = bb[2 * ny + r] - d1 * (E2[c1] - E2[c0]) + d2 * (E1[n0] - E1[c0]);
+  const long long rj = (j + 1 == ny ? 0 : j + 1) + (long long)k * ny;  // row y+1
+  const long long c0 = r * nx + nx - 1, c1 = r * nx;
+  const long long cj = rj * nx + nx - 1;
+  const float* stale3 = wall_at(fd, 15, R);
+  const float* stale2 = stale3 + nz;
+  const float e3j = (STEP == 2 && j == ny - 1) ? stale3[k] : E3[cj];
+  float* bb = wall_at(fd, 6, R);
+  const float d1 = fd.wall_dt[0], d2 = fd.wall_dt[1], d3 = fd.wall_dt[2];
+  if (DIMS == 2) {
+    bb[r] = bb[r] - d2 * (e3j - E3[c0]);
+    bb[R + r] = bb[R + r] + d1 * (E3[c1] - E3[c0]);
+    bb[2 * R + r] = bb[2 * R + r] - d1 * (E2[c1] - E2[c0]) + d2 * (E1[cj] - E1[c0]);
+  } else {
+    const long long rk = j + (long long)(k + 1 == nz ? 0 : k + 1) * ny;  // row z+1
+    const long long ck = rk * nx + nx - 1;
+    const float e2k = (STEP == 2 && k == nz - 1) ? stale2[j] : E2[ck];
+    bb[r] = bb[r] - d2 * (e3j - E3[c0]) + d3 * (e2k - E2[c0]);
+    bb[R + r] = bb[R + r] - d3 * (E1[ck] - E1[c0]) + d1 * (E3[c1] - E3[c0]);
+    bb[2 * R + r] = bb[2 * R + r] - d1 * (E2[c1] - E2[c0]) + d2 * (E1[cj] - E1[c0]);
+  }
 }
 
 __global__ __launch_bounds__(256) void wall_e(const FieldDesc* __restrict__ fds) {
   const FieldDesc& fd = *fds;
   const GridDesc& g = fd.g;
   if (g.ctrl->halt >= 2) return;
-  const int ny = g.ny, nx = g.nx, nxi = g.open_nx;
-  const int r = blockIdx.x * 256 + threadIdx.x;
-  if (r >= ny) return;
+  const int ny = g.ny, nz = g.nz, nx = g.nx, nxi = g.open_nx;
+  const long long R = (long long)ny * nz;
+  const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (r >= R) return;
+  const int j = (int)(r % ny), k = (int)(r / ny);
   const long long nc = g.ncell;
   const float dc = fd.wall_dc;
-  float* ebl = wall_at(fd, kWallEbl, ny);
-  float* ebu = wall_at(fd, kWallEbu, ny);
-  if (r == 0) wall_at(fd, kWallStale, ny)[0] = ebl[(2 * 2 + 0) * ny];
-  const long long row = (long long)r * nx;
+  float* ebl = wall_at(fd, 0, R);
+  float* ebu = wall_at(fd, 9, R);
+  float* stale3 = wall_at(fd, 15, R);
+  float* stale2 = stale3 + nz;
+  if (j == 0) stale3[k] = ebl[(2 * 2 + 0) * R + r];  // E3(0, 1, z) before this update
+  if (k == 0) stale2[j] = ebl[(2 * 1 + 0) * R + r];  // E2(0, y, 1)
+  const long long row = r * nx;
 #pragma unroll
   for (int p = 1; p <= 2; p++) {
     float* E = g.E + p * nc + row;
     // lower wall: E(0) = buf(1) + dc (buf(0) - E(1)); buffers <- E(0), E(1)
-    float* b0 = ebl + (2 * p + 0) * ny + r;
-    float* b1 = ebl + (2 * p + 1) * ny + r;
+    float* b0 = ebl + (2 * p + 0) * R + r;
+    float* b1 = ebl + (2 * p + 1) * R + r;
     const float e1 = E[0];
     const float v = *b1 + dc * (*b0 - e1);
     E[nx - 1] = v;
     *b0 = v;
     *b1 = e1;
     // upper wall: E(nx+2) = buf(nx+1) + dc (buf(nx+2) - E(nx+1)); buffers <- E(nx+1), E(nx+2)
-    float* u1 = ebu + (2 * p + 0) * ny + r;
-    float* u2 = ebu + (2 * p + 1) * ny + r;
+    float* u1 = ebu + (2 * p + 0) * R + r;
+    float* u2 = ebu + (2 * p + 1) * R + r;
     const float en = E[nxi];
     const float w = *u1 + dc * (*u2 - en);
     E[nxi + 1] = w;
     *u1 = en;
     *u2 = w;
+  }
+}
+
+// the wall values the next push reads: E and B at x = 0 and E at x = nx+1 from the buffers
+// (the E1 entries are never written and stay zero)
+__global__ __launch_bounds__(256) void wall_restore(const FieldDesc* __restrict__ fds) {
+  const FieldDesc& fd = *fds;
+  const GridDesc& g = fd.g;
+  if (g.ctrl->halt >= 2) return;
+  const int nx = g.nx, nxi = g.open_nx;
+  const long long R = (long long)g.ny * g.nz;
+  const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (r >= R) return;
+  const long long nc = g.ncell, row = r * nx;
+  const float* ebl = wall_at(fd, 0, R);
+  const float* bbl = wall_at(fd, 6, R);
+  const float* ebu = wall_at(fd, 9, R);
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    g.E[c * nc + row + nx - 1] = ebl[(2 * c) * R + r];
+    g.B[c * nc + row + nx - 1] = bbl[c * R + r];
+    g.E[c * nc + row + nxi] = ebu[(2 * c) * R + r];
   }
 }
 
@@ -313,26 +356,6 @@ __global__ __launch_bounds__(256) void laser_wall(const FieldDesc* __restrict__ 
   }
 }
 
-// the wall values the next push reads: E and B at x = 0 and E at x = nx+1 from the buffers
-// (the E1 entries are never written and stay zero)
-__global__ __launch_bounds__(256) void wall_restore(const FieldDesc* __restrict__ fds) {
-  const FieldDesc& fd = *fds;
-  const GridDesc& g = fd.g;
-  if (g.ctrl->halt >= 2) return;
-  const int ny = g.ny, nx = g.nx, nxi = g.open_nx;
-  const int r = blockIdx.x * 256 + threadIdx.x;
-  if (r >= ny) return;
-  const long long nc = g.ncell, row = (long long)r * nx;
-  const float* ebl = wall_at(fd, kWallEbl, ny);
-  const float* bbl = wall_at(fd, kWallBbl, ny);
-  const float* ebu = wall_at(fd, kWallEbu, ny);
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    g.E[c * nc + row + nx - 1] = ebl[(2 * c) * ny + r];
-    g.B[c * nc + row + nx - 1] = bbl[c * ny + r];
-    g.E[c * nc + row + nxi] = ebu[(2 * c) * ny + r];
-  }
-}
 
 // Neumaier compensated accumulation
 OSR_DEV void kadd(double& s, double& c, double v) {

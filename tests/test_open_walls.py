@@ -48,6 +48,26 @@ def test_lindman_reflection_is_the_mur_closed_form(direction, k0):
     assert abs(got / want - 1.0) < 0.02, (got, want)
 
 
+def test_lindman_3d_reflection_is_the_mur_closed_form():
+    """A y-z uniform pulse on the 3D lattice obeys the same 1D Mur reflection."""
+    nx, dx, dt, k0 = 160, 0.2, 0.1, 4.0
+    s = O.Sim(3, (nx, 4, 4), (dx, dx, dx), dt, 2, open_x=True)
+    xi = np.arange(1, nx + 1) * dx
+    x0 = nx * dx / 2
+
+    def f(x):
+        return np.exp(-((x - x0) / 2.0) ** 2) * np.cos(k0 * (x - x0))
+
+    s.grid.interior(s.e)[..., 1] = f(xi)
+    s.grid.interior(s.b)[..., 2] = f(xi + dx / 2)
+    s.step(int(1.1 * nx * dx / dt))
+    fe = np.array(s.fld_hist).sum(1)
+    F = np.abs(np.fft.fft(f(xi))) ** 2
+    k = np.fft.fftfreq(nx) * 2 * np.pi
+    want = float((np.abs(mur_reflection(k, dt / dx)) ** 2 * F).sum() / F.sum())
+    assert abs(fe[-1] / fe[0] / want - 1.0) < 0.02, (fe[-1] / fe[0], want)
+
+
 def test_lindman_planted_sign_mistake_is_caught():
     e0, e1, _ = _pulse_run(1, plant="sign")
     assert e1 / e0 > 50 * _predicted_fraction(4.0)
@@ -70,8 +90,8 @@ def test_wall_guard_values():
     i0, j0 = g.gcl[0] - 1, g.gcl[1]  # array column of x = 0, row of y = 1
     rows = slice(j0, j0 + NY)
     assert not e[:, i0, 0].any() and not e[:, i0 + NX + 1, 0].any()
-    assert np.array_equal(s.walls.ebl[rows, 0, 1:], e[rows, i0, 1:])
-    assert np.array_equal(s.walls.bbl[rows, 0, :], b[rows, i0, :])
+    assert np.array_equal(s.walls.ebl[0, rows, 0, 1:], e[rows, i0, 1:])
+    assert np.array_equal(s.walls.bbl[0, rows, 0, :], b[rows, i0, :])
     assert np.abs(e[rows, i0, 1:]).max() > 0
 
 
