@@ -16,7 +16,7 @@ import pytest
 import osiris_amd as oa
 from oracle import oracle as O
 from osiris_amd._lib import OsrError
-from tests.pic_inputs import thermal_plasma, vacuum_pulse
+from tests.pic_inputs import random_particles, thermal_plasma, vacuum_pulse
 
 pytestmark = pytest.mark.gpu
 
@@ -121,6 +121,37 @@ def test_open_orders_vs_oracles(order, pusher):
     rfld, rpar = np.array(ref.fld_hist), np.array(ref.par_hist)
     assert np.max(np.abs(par - rpar) / np.abs(rpar)) <= tol
     fscale = np.maximum(rfld.sum(1, keepdims=True), 1e-300)  # row 0: no field yet
+    assert np.max(np.abs(fld - rfld) / fscale) <= tol
+
+
+def test_open_walls_with_relayouts():
+    """A dense beam crossing into empty tiles overflows tile capacities, so steps halt and
+    the host re-lays the particles out from the tiles and the pending movers; movers
+    pushed through a wall in those steps must stay removed.  Counts and energies against
+    the FP64 oracle, as without overflows."""
+    rng = np.random.default_rng(47)
+    nx, dx, dt, steps = (32, 32), (0.1, 0.1), 0.07, 30
+    ix, x, u, q = random_particles(rng, 2, nx, 6000, uth=0.05, ufl=(2.0, 1.5, 0.0), q=-0.02,
+                                   lo=[17, 1], hi=[24, 8])
+    ref = O.Sim(2, nx, dx, dt, 2, open_x=True)
+    ref.add_species(-1.0, 0, ix, x, u, q)
+    with oa.Simulation(2, nx, dx, dt, 2, tile=(8, 8), xbnd="open") as sim:
+        sid = sim.add_species(-1.0)
+        sim.set_particles(sid, ix, x, u, q)
+        sim.run(steps)
+        ref.step(steps)
+        n_gpu, rel = sim.count(sid), sim.relayouts
+        gix = sim.get_particles(sid)[0]
+        fld, par = sim.history()
+    n_ref = len(ref.species[0].q)
+    print(f"relayouts {rel}, particles {len(q)} -> gpu {n_gpu}, oracle {n_ref}", flush=True)
+    assert rel >= 1 and n_ref < len(q) // 2
+    assert abs(n_gpu - n_ref) <= 2 and len(gix) == n_gpu
+    assert np.all((gix[:, 0] >= 1) & (gix[:, 0] <= nx[0]))
+    tol = 1e-4 + 2.0 * abs(n_gpu - n_ref) / n_ref
+    rfld, rpar = np.array(ref.fld_hist), np.array(ref.par_hist)
+    assert np.max(np.abs(par - rpar) / np.abs(rpar)) <= tol
+    fscale = np.maximum(rfld.sum(1, keepdims=True), 1e-300)
     assert np.max(np.abs(fld - rfld) / fscale) <= tol
 
 
