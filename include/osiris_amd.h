@@ -404,18 +404,20 @@ int osr_sim_save_field(osr_sim *sim, int quant, const char *ms_dir, const double
 int osr_sim_init_fields(osr_sim *sim, int sid);
 
 /* ---------------------------------------------------------------------------------
- * 7. Laser injection from an x wall (zpulse_wall, zpulse/os-zpulse-wall.f03), 2D
+ * 7. Laser injection from an x wall (zpulse_wall, zpulse/os-zpulse-wall.f03), 2D and 3D
  *
  * Replaces the zpulse_wall section of an input deck (read_input_zpulse_wall,
  * os-zpulse-wall.f03:90-628) for direction 1.  Before every push (os-simulation.f03:
  * 528-531, t = n dt, os-time.f90:258-276) each active pulse adds a column of B2, B3 at
- * x = 1 (forward) or x = nx (backward) (launch_em_wall_2d, os-zpulse-wall.f03:865-960);
+ * x = 1 (forward) or x = nx (backward) (launch_em_wall_2d / 3d, os-zpulse-wall.f03:
+ * 865-960, 965-1082; in 3D a plane of values at (y, z+1/2) and (y+1/2, z));
  * a circularly polarised pulse (pol_type = +-1) adds a second column with the phase
  * shifted by +-90 degrees and the polarisation by 90 degrees (:633-700).  Temporal envelopes "polynomial",
  * "sin2" and "gaussian" (t_env_zpulse_wall :705-746, os-math.f90:83-165); transverse
- * profiles "plane" and "gaussian"/"hermite" TEM modes 0..7 (per_envelope_2d_zpulse,
- * zpulse/os-zpulse-std.f03:2599-2753, hermite_double os-math.f90:478-509).  Chirps,
- * tilts, math-function envelopes and other directions are not available.
+ * profiles "plane" and "gaussian"/"hermite" TEM modes 0..7 (per_envelope_2d_zpulse /
+ * 3d, zpulse/os-zpulse-std.f03:2599-2753, 2760-3049, hermite_double os-math.f90:
+ * 478-509).  Chirps, tilts, math-function envelopes, astigmatic and Laguerre beams and
+ * other directions are not available.
  * The launch wall must be open (osr_sim_desc.xbnd = OSR_BND_OPEN), as the reference
  * requires (:600-608).  The FP64 amplitudes are evaluated on the host, a block of steps
  * at a time, and added in FP32 on the device.
@@ -437,12 +439,11 @@ typedef struct {
   double tenv_fwhm;       /* > 0 overrides rise/flat/fall or duration (-1) */
   double tenv_rise, tenv_flat, tenv_fall;  /* (0, 0, 0) */
   double tenv_duration, tenv_range;        /* gaussian envelope (0, unset) */
-  double per_center;      /* transverse centre (unset: middle of the box) */
+  double per_center[2];   /* transverse centre, y (and z) (unset: middle of the box) */
   double per_w0, per_fwhm, per_focus;      /* gaussian beam waist, fwhm, focus (0,0,unset) */
-  int per_tem_mode;       /* Hermite mode 0..7 (0) */
-  int reserved;
+  int per_tem_mode[2];    /* Hermite modes 0..7 along y (and z) (0, 0) */
   double launch_time;     /* (0) */
-  double xmin[2];         /* lower corner of the box (g_space xmin) */
+  double xmin[3];         /* lower corner of the box (g_space xmin) */
 } osr_zpulse_wall;
 
 /* The reference's defaults (os-zpulse-wall.f03:196-239), box corner at the origin. */
@@ -450,8 +451,9 @@ void osr_zpulse_wall_defaults(osr_zpulse_wall *zp);
 /* Adds a pulse to the simulation (at most 8; they launch in the order added). */
 int osr_sim_add_zpulse_wall(osr_sim *sim, const osr_zpulse_wall *zp);
 /* The FP32 values pulse k adds at time t (n dt) to B2 and B3 at its column, rows
- * y = 1..ny, for its first (second, circular only) launch: b12[ny], b3[ny]; *active = 0
- * when it adds nothing at t.  Host code; used to check the injection. */
+ * y = 1..ny (y fastest, then z = 1..nz in 3D), for its first (second, circular only)
+ * launch: b12[ny nz], b3[ny nz]; *active = 0 when it adds nothing at t.  Host code; used
+ * to check the injection. */
 int osr_sim_zpulse_column(osr_sim *sim, int k, int second, double t, float *b12, float *b3,
                           int *active);
 

@@ -111,11 +111,11 @@ class ZpulseParams(C.Structure):
     """orc_zpulse of pic_oracle.c (parameters after read_input_zpulse_wall)."""
     _fields_ = [("a0", C.c_double), ("omega0", C.c_double), ("phase0", C.c_double),
                 ("pol", C.c_double), ("pol_type", C.c_int), ("backward", C.c_int),
-                ("tenv_type", C.c_int), ("per_type", C.c_int), ("tem_mode", C.c_int),
-                ("pad", C.c_int), ("rise", C.c_double), ("flat", C.c_double),
+                ("tenv_type", C.c_int), ("per_type", C.c_int), ("tem_mode", C.c_int * 2),
+                ("rise", C.c_double), ("flat", C.c_double),
                 ("fall", C.c_double), ("duration", C.c_double), ("range", C.c_double),
-                ("center", C.c_double), ("w0", C.c_double), ("focus", C.c_double),
-                ("launch_time", C.c_double), ("xmin", C.c_double * 2), ("xmax", C.c_double * 2)]
+                ("center", C.c_double * 2), ("w0", C.c_double), ("focus", C.c_double),
+                ("launch_time", C.c_double), ("xmin", C.c_double * 3), ("xmax", C.c_double * 3)]
 
 
 ZPULSE_DEFAULTS = dict(a0=1.0, omega0=10.0, phase=0.0, pol_type=0, pol=90.0,
@@ -125,9 +125,10 @@ ZPULSE_DEFAULTS = dict(a0=1.0, omega0=10.0, phase=0.0, pol_type=0, pol=90.0,
                        per_fwhm=0.0, per_focus=None, per_tem_mode=0, launch_time=0.0)
 
 
-def zpulse_wall_params(nx, dx, xmin=(0.0, 0.0), **kw):
+def zpulse_wall_params(nx, dx, xmin=(0.0, 0.0, 0.0), **kw):
     """read_input_zpulse_wall (zpulse/os-zpulse-wall.f03:196-628) for direction 1 of a 2D
-    box: the reference's defaults (:196-239) updated by kw; returns ZpulseParams."""
+    or 3D box (len(nx)): the reference's defaults (:196-239) updated by kw (per_center
+    and per_tem_mode take one value per transverse direction); returns ZpulseParams."""
     unknown = set(kw) - set(ZPULSE_DEFAULTS)
     if unknown:
         raise ValueError(f"unknown zpulse_wall parameters {sorted(unknown)}")
@@ -141,9 +142,10 @@ def zpulse_wall_params(nx, dx, xmin=(0.0, 0.0), **kw):
     p.pol_type = a["pol_type"]
     p.pol = a["pol"] * pi_180
     p.backward = {"forward": 0, "backward": 1}[a["propagation"]]
-    for d in range(2):
-        p.xmin[d] = xmin[d]
-        p.xmax[d] = xmin[d] + nx[d] * dx[d]
+    dims = len(nx)
+    for d in range(dims):
+        p.xmin[d] = xmin[d] if d < len(xmin) else 0.0
+        p.xmax[d] = p.xmin[d] + nx[d] * dx[d]
     p.tenv_type = {"polynomial": 0, "sin2": 1, "gaussian": 2}[a["tenv_type"]]
     if p.tenv_type < 2:
         if a["tenv_fwhm"] > 0:
@@ -158,12 +160,18 @@ def zpulse_wall_params(nx, dx, xmin=(0.0, 0.0), **kw):
         p.range = a["tenv_range"]
     p.per_type = {"plane": 0, "gaussian": 1, "hermite": 1}[a["per_type"]]
     if p.per_type == 1:
-        p.tem_mode = a["per_tem_mode"]
+        modes = np.atleast_1d(a["per_tem_mode"])
+        for d in range(dims - 1):
+            p.tem_mode[d] = int(modes[d]) if d < len(modes) else 0
         p.w0 = a["per_fwhm"] / np.sqrt(4.0 * np.log(2.0)) if a["per_fwhm"] > 0 else a["per_w0"]
         if p.w0 <= 0 or a["per_focus"] is None:
             raise ValueError("a gaussian beam needs per_w0 (or per_fwhm) and per_focus")
         p.focus = a["per_focus"]
-    p.center = (0.5 * (p.xmin[1] + p.xmax[1]) if a["per_center"] is None else a["per_center"])
+    cen = a["per_center"]
+    cen = [None] * (dims - 1) if cen is None else list(np.atleast_1d(cen))
+    for d in range(dims - 1):
+        c = cen[d] if d < len(cen) else None
+        p.center[d] = 0.5 * (p.xmin[d + 1] + p.xmax[d + 1]) if c is None else c
     p.launch_time = a["launch_time"]
     if p.backward:
         p.focus = -p.focus
@@ -548,8 +556,6 @@ class Sim:
         xs = IA(*[s.x.ctypes.data for s in self.species])
         us = IA(*[s.u.ctypes.data for s in self.species])
         qs = IA(*[s.q.ctypes.data for s in self.species])
-        if self.zpulses and self.dims != 2:
-            raise ValueError("zpulse_wall is implemented in 2D")
         for _ in range(n):
             # zpulse launch at t = n dt before the step (os-simulation.f03:528-531)
             t = len(self.fld_hist) * self.dt
@@ -582,7 +588,7 @@ class Sim:
         self.grid.interior(self.e)[...] += self.grid.interior(ef)
         return ef
 
-    def add_zpulse_wall(self, xmin=(0.0, 0.0), **kw):
+    def add_zpulse_wall(self, xmin=(0.0, 0.0, 0.0), **kw):
         """A zpulse_wall section (direction 1); needs open x walls (os-zpulse-wall.f03:
         600-608)."""
         if not self.open_x:

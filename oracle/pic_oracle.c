@@ -1272,7 +1272,7 @@ void orc_step_open(const orc_grid *g, int order, const double *dx, double dt, re
 }
 
 /* ------------------------------------------------------------------------------------
- * Laser injection from an x wall, 2D (zpulse_wall, zpulse/os-zpulse-wall.f03).
+ * Laser injection from an x wall, 2D and 3D (zpulse_wall, zpulse/os-zpulse-wall.f03).
  * orc_zpulse holds the parameters after read_input_zpulse_wall (:90-628): angles in
  * radians, envelope lengths resolved from fwhm, per_focus negated for backward pulses.
  * orc_zpulse_wall_2d is launch_zpulse_wall (:633-700) at time t: nothing outside
@@ -1282,10 +1282,10 @@ void orc_step_open(const orc_grid *g, int order, const double *dx, double dt, re
  * ------------------------------------------------------------------------------------ */
 typedef struct {
   double a0, omega0, phase0, pol;
-  int pol_type, backward, tenv_type, per_type, tem_mode, pad;
+  int pol_type, backward, tenv_type, per_type, tem_mode[2];
   double rise, flat, fall, duration, range;
-  double center, w0, focus, launch_time;
-  double xmin[2], xmax[2];
+  double center[2], w0, focus, launch_time;
+  double xmin[3], xmax[3];
 } orc_zpulse;
 
 static const double ORC_PI_2 = 1.57079632679489661923132; /* os-math.f90:14 */
@@ -1348,15 +1348,34 @@ static double zp_per(const orc_zpulse *p, double phase0, double x1, double x2, d
   double k = p->omega0, ps = p->backward ? -1.0 : 1.0, t = 0.0;
   if (p->per_type == 0) return cos(k * (x1 - zc - ps * t) + phase0);
   double z0 = k * (p->w0 * p->w0) * 0.5;
-  double rho = x2 - p->center, rho2 = rho * rho, z = x1 - p->focus;
+  double rho = x2 - p->center[0], rho2 = rho * rho, z = x1 - p->focus;
   double rwl2 = 1, curv = 0, gouy = 0;
   if (z != 0) {
     rwl2 = (z0 * z0) / ((z0 * z0) + (z * z));
     curv = 0.5 * rho2 * z / ((z * z) + (z0 * z0));
-    gouy = (p->tem_mode + 1) * atan2(z, z0);
+    gouy = (p->tem_mode[0] + 1) * atan2(z, z0);
   }
   double rwu = sqrt(2 * rwl2) / p->w0;
-  return sqrt(sqrt(rwl2)) * zp_hermite(p->tem_mode, rho * rwu) *
+  return sqrt(sqrt(rwl2)) * zp_hermite(p->tem_mode[0], rho * rwu) *
+         exp(-(rho2 * rwl2 / (p->w0 * p->w0))) * cos(k * (x1 - zc - ps * t) + k * curv - gouy + phase0);
+}
+
+/* per_envelope_3d_zpulse, os-zpulse-std.f03:2760-3049 (plane, hermite_gaussian) */
+static double zp_per3(const orc_zpulse *p, double phase0, double x1, double x2, double x3,
+                      double inj) {
+  double zc = (p->tenv_type == 2 ? 0.5 * p->range : p->rise + p->flat / 2) + inj;
+  double ox = x2 - p->center[0], oy = x3 - p->center[1];
+  double k = p->omega0, ps = p->backward ? -1.0 : 1.0, t = 0.0;
+  if (p->per_type == 0) return cos(k * (x1 - zc - ps * t) + phase0);
+  double rho2 = ox * ox + oy * oy, z = x1 - p->focus, z0 = k * (p->w0 * p->w0) / 2;
+  double rwl2 = 1, curv = 0, gouy = 0;
+  if (z != 0) {
+    rwl2 = (z0 * z0) / ((z0 * z0) + (z * z));
+    curv = 0.5 * rho2 * z / ((z * z) + (z0 * z0));
+    gouy = (p->tem_mode[0] + p->tem_mode[1] + 1) * atan2(z, z0);
+  }
+  double rwu = sqrt(2 * rwl2) / p->w0;
+  return sqrt(rwl2) * zp_hermite(p->tem_mode[0], ox * rwu) * zp_hermite(p->tem_mode[1], oy * rwu) *
          exp(-(rho2 * rwl2 / (p->w0 * p->w0))) * cos(k * (x1 - zc - ps * t) + k * curv - gouy + phase0);
 }
 
@@ -1375,6 +1394,24 @@ static void zp_launch(const orc_grid *g, const orc_zpulse *p, int order, const d
     rmin = rmin + dr_2;
   }
   double lenv = amp * zp_tenv(p, inj);
+  if (g->dims == 3) { /* launch_em_wall_3d :965-1082, corner rounded to p_k_fld */
+    double rmin1 = (double)(real)p->xmin[1], rmin2 = (double)(real)p->xmin[2];
+    double dr2_2 = dx[2] / 2.0;
+    if (order == 2) {
+      rmin1 = rmin1 + dr_2;
+      rmin2 = rmin2 + dr2_2;
+    }
+    for (int ip2 = 0; ip2 <= g->nx[2] + 2; ip2++)
+      for (int ip1 = 0; ip1 <= g->nx[1] + 2; ip1++) {
+        double r1 = rmin1 + (ip1 - 1) * dx[1], r2 = rmin2 + (ip2 - 1) * dx[2];
+        double r1_2 = r1 + dr_2, r2_2 = r2 + dr2_2;
+        double b0 = lenv * rdtdx * zp_per3(p, phase0, z, r1, r2_2, inj) * sin_pol;
+        double b90 = lenv * rdtdx * zp_per3(p, phase0, z, r1_2, r2, inj) * cos_pol;
+        FLD(b, g, 1, inpos, ip1, ip2) = FLD(b, g, 1, inpos, ip1, ip2) + (real)b0;
+        FLD(b, g, 2, inpos, ip1, ip2) = FLD(b, g, 2, inpos, ip1, ip2) + (real)b90;
+      }
+    return;
+  }
   for (int ip = 0; ip <= g->nx[1] + 2; ip++) {
     double r = rmin + (ip - 1) * dx[1], r_2 = r + dr_2;
     double b12 = lenv * rdtdx * zp_per(p, phase0, z, r, inj) * sin_pol;

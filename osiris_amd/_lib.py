@@ -88,9 +88,9 @@ class ZpulseWall(C.Structure):
                 ("tenv_type", C.c_int), ("per_type", C.c_int), ("tenv_fwhm", C.c_double),
                 ("tenv_rise", C.c_double), ("tenv_flat", C.c_double), ("tenv_fall", C.c_double),
                 ("tenv_duration", C.c_double), ("tenv_range", C.c_double),
-                ("per_center", C.c_double), ("per_w0", C.c_double), ("per_fwhm", C.c_double),
-                ("per_focus", C.c_double), ("per_tem_mode", C.c_int), ("reserved", C.c_int),
-                ("launch_time", C.c_double), ("xmin", C.c_double * 2)]
+                ("per_center", C.c_double * 2), ("per_w0", C.c_double), ("per_fwhm", C.c_double),
+                ("per_focus", C.c_double), ("per_tem_mode", C.c_int * 2),
+                ("launch_time", C.c_double), ("xmin", C.c_double * 3)]
 
 
 OSR_TENV = {"polynomial": 0, "sin2": 1, "gaussian": 2}

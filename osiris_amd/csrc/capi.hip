@@ -190,6 +190,13 @@ int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
 constexpr int kZpRows = 256;  // steps of laser values per host evaluation (the longest batch)
 
+// steps held by the laser table: kZpRows, fewer when a wall plane is large (the table holds
+// at most 2^26 floats)
+int zp_rows(const osr_sim* s, int K) {
+  const long long per = 2ll * s->g.ny * s->g.nz * std::max(K, 1);
+  return (int)std::max(1ll, std::min<long long>(kZpRows, (1ll << 26) / per));
+}
+
 // launches per step of the laser walls (two for a circularly polarised pulse)
 int zp_launches(const osr_sim* s) {
   int k = 0;
@@ -310,7 +317,7 @@ void make_field_desc(const osr_sim* s, FieldDesc& f) {
   }
   if (!s->zp.empty()) {
     f.las_k = zp_launches(s);
-    f.las_rows = kZpRows;
+    f.las_rows = zp_rows(s, f.las_k);
     int k = 0;
     for (auto& w : s->zp)
       for (int r = 0; r < (w.pol_type != 0 ? 2 : 1); r++)
@@ -815,7 +822,7 @@ int launch_smooth(osr_sim* s) {
 // laser walls of the step, before the push (os-simulation.f03:528-531)
 int launch_laser(osr_sim* s) {
   if (s->zp.empty()) return OSR_OK;
-  hipLaunchKernelGGL(laser_wall, dim3(ceil_div(s->g.ny, 256)), dim3(256), 0, s->stream,
+  hipLaunchKernelGGL(laser_wall, dim3(ceil_div(s->g.ny * s->g.nz, 256)), dim3(256), 0, s->stream,
                      s->d_fd.as<FieldDesc>());
   HIPC(hipGetLastError());
   return OSR_OK;
@@ -825,10 +832,11 @@ int launch_laser(osr_sim* s) {
 // device table the laser kernel reads.
 int zp_fill(osr_sim* s, int64_t first, int n) {
   HIPC(hipStreamSynchronize(s->stream));  // the copies below are not ordered with the stream
-  const int K = zp_launches(s), ny = s->g.ny;
-  n = std::min(n, kZpRows);
-  std::vector<float> tab((size_t)n * K * 2 * ny);
-  std::vector<int> mask((size_t)kZpRows * K, 0);
+  const int K = zp_launches(s), rows = zp_rows(s, K);
+  const size_t R = (size_t)s->g.ny * s->g.nz;
+  n = std::min(n, rows);
+  std::vector<float> tab((size_t)n * K * 2 * R);
+  std::vector<int> mask((size_t)rows * K, 0);
   for (int r = 0; r < n; r++) {
     const double t = (double)(first + r) * s->dt;
     int k = 0;
@@ -837,8 +845,8 @@ int zp_fill(osr_sim* s, int64_t first, int n) {
         const size_t e = (size_t)r * K + k;
         if (!osr_zp::active(w, t)) continue;
         mask[e] = 1;
-        osr_zp::column(w, ny, s->dx, s->g.dt_dx[0], s->order, t, sec == 1, &tab[e * 2 * ny],
-                       &tab[e * 2 * ny + ny]);
+        osr_zp::column(w, s->dims, s->g.ny, s->g.nz, s->dx, s->g.dt_dx[0], s->order, t, sec == 1,
+                       &tab[e * 2 * R], &tab[e * 2 * R + R]);
       }
   }
   const long long f = first;
@@ -1338,8 +1346,11 @@ int osr_sim_run(osr_sim* s, int nsteps) {
   int64_t left = nsteps;
   int batch = 8;
   while (left > 0) {
-    const int nb = (int)std::min<int64_t>(left, batch);
-    if (!s->zp.empty()) TRY(zp_fill(s, s->steps, nb));
+    int nb = (int)std::min<int64_t>(left, batch);
+    if (!s->zp.empty()) {  // a batch never outruns the laser table
+      nb = std::min(nb, zp_rows(s, zp_launches(s)));
+      TRY(zp_fill(s, s->steps, nb));
+    }
     for (int i = 0; i < nb; i++) HIPC(hipGraphLaunch(s->gexec, s->stream));
     Ctrl c;
     TRY(read_ctrl(s, c));
@@ -2126,7 +2137,7 @@ void osr_zpulse_wall_defaults(osr_zpulse_wall* zp) {
   zp->per_type = OSR_PER_PLANE;
   zp->tenv_fwhm = -1.0;
   zp->tenv_range = OSR_ZP_UNSET;
-  zp->per_center = OSR_ZP_UNSET;
+  zp->per_center[0] = zp->per_center[1] = OSR_ZP_UNSET;
   zp->per_focus = OSR_ZP_UNSET;
 }
 
@@ -2134,19 +2145,18 @@ int osr_sim_add_zpulse_wall(osr_sim* s, const osr_zpulse_wall* zp) {
   TRY(check_sim(s));
   if (!zp) return fail(OSR_ERR_ARG, "null argument");
   if (!s->g.open_nx) return fail(OSR_ERR_STATE, "the launch wall must be open (xbnd = OSR_BND_OPEN)");
-  if (s->dims != 2) return fail(OSR_ERR_ARG, "zpulse_wall is available in 2D");
   if ((int)s->zp.size() >= osr_zp::kMaxPulses) return fail(OSR_ERR_ARG, "at most 8 laser walls");
   osr_zp::Wall w;
-  const int nx[2] = {s->g.open_nx, s->g.ny};
-  const char* err = osr_zp::resolve(*zp, s->order, nx, s->dx, w);
+  const int nx[3] = {s->g.open_nx, s->g.ny, s->g.nz};
+  const char* err = osr_zp::resolve(*zp, s->dims, nx, s->dx, w);
   if (err) return fail(OSR_ERR_ARG, "zpulse_wall: %s", err);
   HIPC(hipStreamSynchronize(s->stream));
   s->zp.push_back(w);
-  const int K = zp_launches(s);
-  TRY(s->zp_tab.alloc(sizeof(float) * (size_t)kZpRows * K * 2 * s->g.ny));
-  TRY(s->zp_mask.alloc(sizeof(int) * (size_t)kZpRows * K));
+  const int K = zp_launches(s), rows = zp_rows(s, K);
+  TRY(s->zp_tab.alloc(sizeof(float) * (size_t)rows * K * 2 * s->g.ny * s->g.nz));
+  TRY(s->zp_mask.alloc(sizeof(int) * (size_t)rows * K));
   TRY(s->zp_first.alloc(sizeof(long long)));
-  HIPC(hipMemset(s->zp_mask.p, 0, sizeof(int) * (size_t)kZpRows * K));
+  HIPC(hipMemset(s->zp_mask.p, 0, sizeof(int) * (size_t)rows * K));
   const long long none = -(1ll << 62);  // no table yet: every step falls outside it
   HIPC(hipMemcpy(s->zp_first.p, &none, sizeof(none), hipMemcpyHostToDevice));
   TRY(upload_descs(s));
@@ -2162,7 +2172,8 @@ int osr_sim_zpulse_column(osr_sim* s, int k, int second, double t, float* b12, f
   const osr_zp::Wall& w = s->zp[k];
   if (second && w.pol_type == 0) return fail(OSR_ERR_ARG, "a linear pulse launches once");
   *active = osr_zp::active(w, t) ? 1 : 0;
-  osr_zp::column(w, s->g.ny, s->dx, s->g.dt_dx[0], s->order, t, second != 0, b12, b3);
+  osr_zp::column(w, s->dims, s->g.ny, s->g.nz, s->dx, s->g.dt_dx[0], s->order, t, second != 0,
+                 b12, b3);
   return OSR_OK;
 }
 

@@ -56,7 +56,7 @@ struct FieldDesc {
   // r = step - *las_first: las_mask[r][k] (0: the launch adds nothing) and the values
   // las_tab[r][k][2][ny] (B2, B3 increments of rows y = 1..ny)
   int las_k;
-  int las_rows;
+  int las_rows;      // steps held by the table
   int las_col[16];
   const float* las_tab;
   const int* las_mask;
@@ -340,19 +340,20 @@ __global__ __launch_bounds__(256) void laser_wall(const FieldDesc* __restrict__ 
   const FieldDesc& fd = *fds;
   const GridDesc& g = fd.g;
   if (g.ctrl->halt >= 2) return;
-  const int ny = g.ny, nx = g.nx;
-  const int r = blockIdx.x * 256 + threadIdx.x;
-  if (r >= ny) return;
+  const int nx = g.nx;
+  const long long R = (long long)g.ny * g.nz;  // rows (y, z) of the wall column
+  const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (r >= R) return;
   const long long row = g.ctrl->steps - *fd.las_first;
   if (row < 0 || row >= fd.las_rows) return;
   const long long nc = g.ncell;
   for (int k = 0; k < fd.las_k; k++) {
     const long long e = row * fd.las_k + k;
     if (!fd.las_mask[e]) continue;
-    const float* v = fd.las_tab + e * 2 * ny;
-    const long long c = (long long)r * nx + fd.las_col[k];
+    const float* v = fd.las_tab + e * 2 * R;
+    const long long c = r * nx + fd.las_col[k];
     g.B[nc + c] = g.B[nc + c] + v[r];
-    g.B[2 * nc + c] = g.B[2 * nc + c] + v[ny + r];
+    g.B[2 * nc + c] = g.B[2 * nc + c] + v[R + r];
   }
 }
 

@@ -1,10 +1,11 @@
-// zpulse.hip -- laser injection from an x wall (zpulse_wall), 2D: host-side set-up and the
-// FP64 evaluation of the column a pulse adds to B2, B3 at one time.
+// zpulse.hip -- laser injection from an x wall (zpulse_wall), 2D and 3D: host-side set-up
+// and the FP64 evaluation of the column (plane) a pulse adds to B2, B3 at one time.
 //
 // Reference: zpulse/os-zpulse-wall.f03 (read_input_zpulse_wall :90-628, launch_zpulse_wall
 // :633-700, t_env_zpulse_wall :705-746, t_duration :752-771, lon_center :781-805,
-// launch_em_wall_2d :865-960), per_envelope_2d_zpulse zpulse/os-zpulse-std.f03:2599-2753,
-// fenv_poly / fenv_sin2 os-math.f90:83-165, hermite_double os-math.f90:478-509.
+// launch_em_wall_2d :865-960, launch_em_wall_3d :965-1082), per_envelope_2d_zpulse /
+// per_envelope_3d_zpulse zpulse/os-zpulse-std.f03:2599-2753, 2760-3049, fenv_poly /
+// fenv_sin2 os-math.f90:83-165, hermite_double os-math.f90:478-509.
 //
 // The evaluation keeps the reference's operation order in double precision; the values are
 // rounded to float (real(., p_k_fld)) and the device adds them to B in FP32.  Included
@@ -17,11 +18,11 @@ constexpr int kMaxPulses = 8;
 
 struct Wall {
   double a0, omega0, phase0, pol;  // phase0 and pol in radians
-  int pol_type, backward, tenv_type, per_type, tem_mode;
+  int pol_type, backward, tenv_type, per_type, tem_mode[2];
   double rise, flat, fall, duration, range;
-  double center, w0, focus;
+  double center[2], w0, focus;
   double launch_time;
-  double xmin[2], xmax[2];
+  double xmin[3], xmax[3];
 };
 
 // os-math.f90:83-128
@@ -93,17 +94,41 @@ inline double per_envelope_2d(const Wall& w, double x1, double x2, double inj_ti
   const double t = 0.0;
   if (w.per_type == OSR_PER_PLANE) return std::cos(k * (x1 - z_center - prop_sign * t) + w.phase0);
   const double z0 = k * (w.w0 * w.w0) * 0.5;
-  const double rho = x2 - w.center;
+  const double rho = x2 - w.center[0];
   const double rho2 = rho * rho;
   const double z = x1 - w.focus;
   double rWl2 = 1, curv = 0, gouy = 0;
   if (z != 0) {
     rWl2 = (z0 * z0) / ((z0 * z0) + (z * z));
     curv = 0.5 * rho2 * z / ((z * z) + (z0 * z0));
-    gouy = (w.tem_mode + 1) * std::atan2(z, z0);
+    gouy = (w.tem_mode[0] + 1) * std::atan2(z, z0);
   }
   const double rWu = std::sqrt(2 * rWl2) / w.w0;
-  return std::sqrt(std::sqrt(rWl2)) * hermite(w.tem_mode, rho * rWu) *
+  return std::sqrt(std::sqrt(rWl2)) * hermite(w.tem_mode[0], rho * rWu) *
+         std::exp(-(rho2 * rWl2 / (w.w0 * w.w0))) *
+         std::cos(k * (x1 - z_center - prop_sign * t) + k * curv - gouy + w.phase0);
+}
+
+// per_envelope_3d_zpulse (plane, hermite_gaussian) at simulation time 0
+inline double per_envelope_3d(const Wall& w, double x1, double x2, double x3, double inj_time) {
+  const double lon = w.tenv_type == OSR_TENV_GAUSSIAN ? 0.5 * w.range : w.rise + w.flat / 2;
+  const double z_center = lon + inj_time;
+  const double ox = x2 - w.center[0], oy = x3 - w.center[1];
+  const double k = w.omega0;
+  const double prop_sign = w.backward ? -1.0 : 1.0;
+  const double t = 0.0;
+  if (w.per_type == OSR_PER_PLANE) return std::cos(k * (x1 - z_center - prop_sign * t) + w.phase0);
+  const double rho2 = (ox * ox) + (oy * oy);
+  const double z = x1 - w.focus;
+  const double z0 = k * (w.w0 * w.w0) / 2;
+  double rWl2 = 1, curv = 0, gouy = 0;
+  if (z != 0) {
+    rWl2 = (z0 * z0) / ((z0 * z0) + (z * z));
+    curv = 0.5 * rho2 * z / ((z * z) + (z0 * z0));
+    gouy = (w.tem_mode[0] + w.tem_mode[1] + 1) * std::atan2(z, z0);
+  }
+  const double rWu = std::sqrt(2 * rWl2) / w.w0;
+  return std::sqrt(rWl2) * hermite(w.tem_mode[0], ox * rWu) * hermite(w.tem_mode[1], oy * rWu) *
          std::exp(-(rho2 * rWl2 / (w.w0 * w.w0))) *
          std::cos(k * (x1 - z_center - prop_sign * t) + k * curv - gouy + w.phase0);
 }
@@ -114,11 +139,12 @@ inline bool active(const Wall& w, double t) {
   return t >= w.launch_time && !(t > w.launch_time + t_duration(w));
 }
 
-// launch_em_wall_2d, direction 1: rows y = 1..ny of the column (the guard rows it also
-// writes are replaced by the periodic refresh that follows).  second: the circular
-// polarisation's second launch (phase0 + sign(pi/2, pol_type), pol + pi/2).
-inline void column(const Wall& w0, int ny, const double* dx, float dtdx1, int order, double t,
-                   bool second, float* b12, float* b3) {
+// launch_em_wall_2d / 3d, direction 1: rows y = 1..ny (and z = 1..nz, y fastest) of the
+// column (the guard rows it also writes are replaced by the periodic refresh that
+// follows).  second: the circular polarisation's second launch (phase0 + sign(pi/2,
+// pol_type), pol + pi/2).
+inline void column(const Wall& w0, int dims, int ny, int nz, const double* dx, float dtdx1,
+                   int order, double t, bool second, float* b12, float* b3) {
   Wall w = w0;
   if (second) {
     w.phase0 = w.phase0 + (w.pol_type > 0 ? kPi2 : -kPi2);
@@ -140,18 +166,36 @@ inline void column(const Wall& w0, int ny, const double* dx, float dtdx1, int or
     rmin = rmin + dr_2;
   }
   const double lenv = amp * t_envelope(w, inj_time);
-  for (int ip = 1; ip <= ny; ip++) {
-    const double r = rmin + (ip - 1) * ldxp;
-    const double r_2 = r + dr_2;
-    b12[ip - 1] = (float)(lenv * rdtdx * per_envelope_2d(w, z, r, inj_time) * sin_pol);
-    b3[ip - 1] = (float)(lenv * rdtdx * per_envelope_2d(w, z, r_2, inj_time) * cos_pol);
+  if (dims == 2) {
+    for (int ip = 1; ip <= ny; ip++) {
+      const double r = rmin + (ip - 1) * ldxp;
+      const double r_2 = r + dr_2;
+      b12[ip - 1] = (float)(lenv * rdtdx * per_envelope_2d(w, z, r, inj_time) * sin_pol);
+      b3[ip - 1] = (float)(lenv * rdtdx * per_envelope_2d(w, z, r_2, inj_time) * cos_pol);
+    }
+    return;
   }
+  // 3D (:1015-1075): the lower corner rounded to p_k_fld as the reference does
+  const double ldxp2 = dx[2], dr2_2 = ldxp2 / 2.0;
+  double rmin1 = (double)(float)w.xmin[1], rmin2 = (double)(float)w.xmin[2];
+  if (order == 2) {
+    rmin1 = rmin1 + dr_2;
+    rmin2 = rmin2 + dr2_2;
+  }
+  for (int ip2 = 1; ip2 <= nz; ip2++)
+    for (int ip1 = 1; ip1 <= ny; ip1++) {
+      const double r1 = rmin1 + (ip1 - 1) * ldxp;
+      const double r2 = rmin2 + (ip2 - 1) * ldxp2;
+      const double r1_2 = r1 + dr_2, r2_2 = r2 + dr2_2;
+      const size_t o = (size_t)(ip2 - 1) * ny + (ip1 - 1);
+      b12[o] = (float)(lenv * rdtdx * per_envelope_3d(w, z, r1, r2_2, inj_time) * sin_pol);
+      b3[o] = (float)(lenv * rdtdx * per_envelope_3d(w, z, r1_2, r2, inj_time) * cos_pol);
+    }
 }
 
 // read_input_zpulse_wall for direction 1 (:196-628): resolved parameters, or an error text
-inline const char* resolve(const osr_zpulse_wall& p, int order, const int* nx, const double* dx,
+inline const char* resolve(const osr_zpulse_wall& p, int dims, const int* nx, const double* dx,
                            Wall& w) {
-  (void)order;
   w = Wall{};
   if (p.pol_type < -1 || p.pol_type > 1) return "pol_type must be -1, 0 or +1";
   if (p.propagation != OSR_PROP_FORWARD && p.propagation != OSR_PROP_BACKWARD)
@@ -162,7 +206,7 @@ inline const char* resolve(const osr_zpulse_wall& p, int order, const int* nx, c
   w.pol_type = p.pol_type;
   w.pol = p.pol * kPi180;
   w.backward = p.propagation == OSR_PROP_BACKWARD;
-  for (int d = 0; d < 2; d++) {
+  for (int d = 0; d < dims; d++) {
     w.xmin[d] = p.xmin[d];
     w.xmax[d] = p.xmin[d] + nx[d] * dx[d];
   }
@@ -193,8 +237,10 @@ inline const char* resolve(const osr_zpulse_wall& p, int order, const int* nx, c
     case OSR_PER_PLANE:
       break;
     case OSR_PER_HERMITE:
-      if (p.per_tem_mode < 0 || p.per_tem_mode > 7) return "per_tem_mode must be 0..7";
-      w.tem_mode = p.per_tem_mode;
+      for (int d = 0; d < dims - 1; d++) {
+        if (p.per_tem_mode[d] < 0 || p.per_tem_mode[d] > 7) return "per_tem_mode must be 0..7";
+        w.tem_mode[d] = p.per_tem_mode[d];
+      }
       w.w0 = p.per_fwhm > 0.0 ? p.per_fwhm / std::sqrt(4.0 * std::log(2.0)) : p.per_w0;
       if (w.w0 <= 0.0) return "a gaussian beam needs per_w0 or per_fwhm > 0";
       if (p.per_focus == OSR_ZP_UNSET) return "a gaussian beam needs per_focus";
@@ -203,7 +249,9 @@ inline const char* resolve(const osr_zpulse_wall& p, int order, const int* nx, c
     default:
       return "per_type must be plane or hermite";
   }
-  w.center = p.per_center == OSR_ZP_UNSET ? 0.5 * (w.xmin[1] + w.xmax[1]) : p.per_center;
+  for (int d = 0; d < dims - 1; d++)  // :590-598
+    w.center[d] = p.per_center[d] == OSR_ZP_UNSET ? 0.5 * (w.xmin[d + 1] + w.xmax[d + 1])
+                                                   : p.per_center[d];
   w.launch_time = p.launch_time;
   if (w.backward) w.focus = -w.focus;
   return nullptr;

@@ -75,13 +75,12 @@ module m_osiris_amd
     real(c_double) :: tenv_rise = 0.0_c_double, tenv_flat = 0.0_c_double, tenv_fall = 0.0_c_double
     real(c_double) :: tenv_duration = 0.0_c_double
     real(c_double) :: tenv_range = -huge(1.0_c_double)
-    real(c_double) :: per_center = -huge(1.0_c_double)
+    real(c_double) :: per_center(2) = -huge(1.0_c_double)
     real(c_double) :: per_w0 = 0.0_c_double, per_fwhm = 0.0_c_double
     real(c_double) :: per_focus = -huge(1.0_c_double)
-    integer(c_int) :: per_tem_mode = 0
-    integer(c_int) :: reserved = 0
+    integer(c_int) :: per_tem_mode(2) = 0
     real(c_double) :: launch_time = 0.0_c_double
-    real(c_double) :: xmin(2) = 0.0_c_double
+    real(c_double) :: xmin(3) = 0.0_c_double
   end type osr_zpulse_wall
 
   public :: osr_version, osr_device_count, osr_error_message

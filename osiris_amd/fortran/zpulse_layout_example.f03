@@ -23,7 +23,7 @@ contains
     type(osr_zpulse_wall), intent(in) :: z
     write (*, '(a,1x,*(es24.16e3,1x))') tag, z%a0, z%omega0, z%phase, z%pol, z%tenv_fwhm, &
       z%tenv_rise, z%tenv_flat, z%tenv_fall, z%tenv_duration, z%tenv_range, z%per_center, &
-      z%per_w0, z%per_fwhm, z%per_focus, z%launch_time, z%xmin(1), z%xmin(2)
+      z%per_w0, z%per_fwhm, z%per_focus, z%launch_time, z%xmin
     write (*, '(a,1x,*(i0,1x))') tag, z%pol_type, z%propagation, z%tenv_type, z%per_type, &
       z%per_tem_mode
   end subroutine

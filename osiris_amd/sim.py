@@ -223,16 +223,24 @@ class Simulation:
         lib().osr_zpulse_wall_defaults(C.byref(z))
         names = {"propagation": OSR_PROP, "tenv_type": OSR_TENV, "per_type": OSR_PER}
         for k, v in kw.items():
-            if k not in {f[0] for f in ZpulseWall._fields_} - {"reserved", "xmin"}:
+            if k not in {f[0] for f in ZpulseWall._fields_} - {"xmin"}:
                 raise ValueError(f"unknown zpulse_wall parameter {k}")
+            if k in ("per_center", "per_tem_mode"):  # per transverse direction
+                v = list(v) if isinstance(v, (tuple, list)) else [v]
+                for i, vi in enumerate(v):
+                    getattr(z, k)[i] = vi
+                continue
             setattr(z, k, names[k][v] if k in names else v)
-        z.xmin[0], z.xmin[1] = xmin
+        for i, v in enumerate(xmin):
+            z.xmin[i] = v
         check(lib().osr_sim_add_zpulse_wall(self._h, C.byref(z)))
 
     def zpulse_column(self, k, t, second=False):
-        """(b12, b3, active): what pulse k adds to B2, B3 of its column at time t."""
-        b12 = np.zeros(self.nx[1], np.float32)
-        b3 = np.zeros(self.nx[1], np.float32)
+        """(b12, b3, active): what pulse k adds to B2, B3 of its column (plane in 3D, shape
+        (nz, ny)) at time t."""
+        shape = (self.nx[1],) if self.dims == 2 else (self.nx[2], self.nx[1])
+        b12 = np.zeros(shape, np.float32)
+        b3 = np.zeros(shape, np.float32)
         act = C.c_int()
         check(lib().osr_sim_zpulse_column(self._h, k, int(second), float(t), _p(b12), _p(b3),
                                           C.byref(act)))

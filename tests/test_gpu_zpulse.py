@@ -131,6 +131,7 @@ def test_laser_plasma_slab_vs_fp64_oracle():
 
 
 def test_zpulse_rejects():
+    """A periodic launch wall and incomplete gaussian parameters are refused."""
     with oa.Simulation(2, (32, 32), (0.1, 0.1), 0.05, 2, tile=(16, 16)) as sim:
         with pytest.raises(oa._lib.OsrError):
             sim.add_zpulse_wall(a0=1.0)  # periodic x: the launch wall is not open
@@ -139,3 +140,41 @@ def test_zpulse_rejects():
             sim.add_zpulse_wall(tenv_type="gaussian", tenv_fwhm=2.0)  # no tenv_range
         with pytest.raises(oa._lib.OsrError):
             sim.add_zpulse_wall(per_type="hermite", per_w0=1.0)  # no per_focus
+
+
+PULSES_3D = [
+    dict(a0=0.8, omega0=5.0, pol=60.0, tenv_type="sin2", tenv_rise=1.5, tenv_flat=0.5,
+         tenv_fall=1.5, per_type="hermite", per_w0=1.2, per_focus=2.0, per_tem_mode=(1, 0),
+         per_center=(1.5, 1.7)),
+    dict(a0=0.5, omega0=4.0, pol=0.0, pol_type=1, tenv_type="polynomial", tenv_fwhm=3.0),
+]
+
+
+def test_laser_3d_bitexact_vs_fp32_oracle():
+    """launch_em_wall_3d: the plane the host evaluates equals the FP32 oracle's, and a 3D
+    vacuum run with two pulses between Lindman walls matches bit for bit."""
+    n, d, dt, steps = (32, 16, 12), (0.2, 0.25, 0.3), 0.08, 60
+    xmin = (0.0, -0.5, 0.25)
+    ref = O.Sim(3, n, d, dt, 2, prec="f32", open_x=True)
+    g = ref.grid
+    L = O._lib("f32")
+    with oa.Simulation(3, n, d, dt, 2, tile=(8, 8, 4), xbnd="open") as sim:
+        for kw in PULSES_3D:
+            ref.add_zpulse_wall(xmin=xmin, **kw)
+            sim.add_zpulse_wall(xmin=xmin, **kw)
+        p = O.zpulse_wall_params(n, d, xmin, **PULSES_3D[0])
+        for t in (0.0, 0.8, 1.6, 3.0):
+            b = g.zeros(3, np.float32)
+            L.orc_zpulse_wall_2d(C.byref(g.c()), C.byref(p), 2, O._p(np.array(d)), dt, t, O._p(b))
+            want = b[g.gcl[2]:g.gcl[2] + n[2], g.gcl[1]:g.gcl[1] + n[1], g.gcl[0], 1:]
+            b12, b3, act = sim.zpulse_column(0, t)
+            got = np.zeros(want.shape, np.float32) + np.stack([b12, b3], -1)
+            assert act == (t <= 3.5)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), t
+        sim.run(steps)
+        ref.step(steps)
+        ref.wall_refresh()
+        ge, gb = sim.get_fields(gcl=g.gcl, gcu=g.gcu)
+    assert np.array_equal(ge.view(np.uint32), ref.e.view(np.uint32))
+    assert np.array_equal(gb.view(np.uint32), ref.b.view(np.uint32))
+    assert np.abs(gb).max() > 0

@@ -63,6 +63,28 @@ def _column(order=2, **kw):
     return g, p, b, L, dxa
 
 
+@pytest.mark.parametrize("modes", [(0, 0), (1, 2)])
+def test_focused_beam_profile_3d_is_the_tem_mode(modes):
+    """3D (launch_em_wall_3d): B2 at (y, z + 1/2) across the wall plane of a beam focused
+    there is H_m1(sqrt(2) oy / w0) H_m2(sqrt(2) oz / w0) exp(-(oy^2 + oz^2) / w0^2)."""
+    n, d = (8, 24, 20), (0.2, 0.25, 0.3)
+    g = O.make_grid(3, n, 2)
+    w0 = 1.5
+    p = O.zpulse_wall_params(n, d, per_type="hermite", per_w0=w0, per_focus=0.1,
+                             per_tem_mode=modes, omega0=5.0, pol=90.0, **FLAT)
+    b = g.zeros(3, np.float64)
+    L = O._lib("f64")
+    L.orc_zpulse_wall_2d(C.byref(g.c()), C.byref(p), 2, O._p(np.array(d)), 0.05, 10.0, O._p(b))
+    col = b[g.gcl[2]:g.gcl[2] + n[2], g.gcl[1]:g.gcl[1] + n[1], g.gcl[0], 1]  # (z, y)
+    oy = d[1] * np.arange(n[1]) + d[1] / 2 - n[1] * d[1] / 2
+    oz = d[2] * np.arange(n[2]) + d[2] - n[2] * d[2] / 2  # z + 1/2 of the quadratic shift
+    H = [lambda u: np.ones_like(u), lambda u: 2 * u, lambda u: 4 * u ** 2 - 2]
+    want = (H[modes[0]](np.sqrt(2) * oy / w0)[None, :] * H[modes[1]](np.sqrt(2) * oz / w0)[:, None] *
+            np.exp(-(oy[None, :] ** 2 + oz[:, None] ** 2) / w0 ** 2))
+    k = np.unravel_index(np.argmax(np.abs(want)), want.shape)
+    assert np.allclose(col / col[k], want / want[k], rtol=1e-11, atol=1e-13)
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2])
 def test_focused_beam_profile_is_the_tem_mode(mode):
     w0 = 2.0
@@ -103,11 +125,13 @@ def test_library_defaults_are_the_reference_defaults():
     L.lib().osr_zpulse_wall_defaults(C.byref(z))
     d = O.ZPULSE_DEFAULTS
     for k in ("a0", "omega0", "phase", "pol", "pol_type", "tenv_fwhm", "tenv_rise", "tenv_flat",
-              "tenv_fall", "tenv_duration", "per_w0", "per_fwhm", "per_tem_mode", "launch_time"):
+              "tenv_fall", "tenv_duration", "per_w0", "per_fwhm", "launch_time"):
         assert getattr(z, k) == d[k], k
+    assert list(z.per_tem_mode) == [d["per_tem_mode"]] * 2
     unset = -np.finfo(np.float64).max  # the reference's -huge(1.0)
-    for k in ("tenv_range", "per_center", "per_focus"):
+    for k in ("tenv_range", "per_focus"):
         assert d[k] is None and getattr(z, k) == unset, k
+    assert d["per_center"] is None and list(z.per_center) == [unset] * 2
     assert (z.propagation, z.tenv_type, z.per_type) == (L.OSR_PROP[d["propagation"]],
                                                         L.OSR_TENV[d["tenv_type"]],
                                                         L.OSR_PER[d["per_type"]])
