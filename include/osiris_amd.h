@@ -395,13 +395,15 @@ int osr_zdf_save_particles(const osr_zdf_part_info *info, const osr_zdf_iteratio
  * time = steps * dt. */
 int osr_sim_save_field(osr_sim *sim, int quant, const char *ms_dir, const double *xmin);
 
-/* Initial electrostatic field of species sid (init_fields, spec/os-spec-initfields.f90:
- * 38-100, branch ufl(1) = 0; es_solver_2d, emf/os-emf-es-solver.f03:125-236), 2D: the
- * species' charge density (folded like the reference's update_boundary(rho, add)) is
- * turned into the free-space direct-sum field (the reference's kernel charge * r / r^2,
- * FP64 terms, FP32 accumulation in source order) and added to E.  Call it before the
- * first step, as the reference does at species set-up.  O(cells^2) work. */
-int osr_sim_init_fields(osr_sim *sim, int sid);
+/* Initial fields of species sid (init_fields, spec/os-spec-initfields.f90:38-100), 2D.
+ * The species' charge density (folded like the reference's update_boundary(rho, add)) is
+ * turned into a field and added.  ufl1 is the species' fluid momentum along x
+ * (udist%ufl(1)): 0 selects es_solver_2d (emf/os-emf-es-solver.f03:125-236), the
+ * free-space direct sum (charge * r / r^2, FP64 terms, FP32 accumulation in source order,
+ * O(cells^2) work) added to E; otherwise es_solver_2d_beam (:457-616), the sheet field of
+ * a beam drifting along x, added to E2 and B3.  Call it before the first step, as the
+ * reference does at species set-up. */
+int osr_sim_init_fields(osr_sim *sim, int sid, float ufl1);
 
 /* ---------------------------------------------------------------------------------
  * 7. Laser injection from an x wall (zpulse_wall, zpulse/os-zpulse-wall.f03), 2D and 3D

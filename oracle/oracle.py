@@ -73,6 +73,7 @@ def _lib(prec: str):
         L.orc_zpulse_wall_2d.argtypes = [P, P, C.c_int, P, C.c_double, C.c_double, P]
         L.orc_es_solver_2d.argtypes = [P, P, P, P]
         L.orc_update_periodic_from.argtypes = [P, P, C.c_int, C.c_int, C.c_int]
+        L.orc_es_solver_2d_beam.argtypes = [P, P, P, C.c_float, C.c_int, P, P]
         L._real = np.float64 if prec == "f64" else np.float32
         _LIBS[prec] = L
     return _LIBS[prec]
@@ -572,9 +573,11 @@ class Sim:
         for s, k in zip(self.species, npa):  # leading slices stay contiguous
             s.ix, s.x, s.u, s.q = s.ix[:k], s.x[:k], s.u[:k], s.q[:k]
 
-    def init_fields(self, k):
-        """init_fields of species k (spec/os-spec-initfields.f90:38-100, ufl(1) = 0):
-        rho deposited and folded, the 2D direct-sum ES field added to E."""
+    def init_fields(self, k, ufl1=0.0):
+        """init_fields of species k (spec/os-spec-initfields.f90:38-100): rho deposited
+        and folded; ufl1 = 0: the 2D direct-sum ES field added to E; otherwise the beam
+        field of a species drifting along x with u1 = ufl1 (es_solver_2d_beam) added to E
+        and B."""
         if self.dims != 2:
             raise ValueError("the reference's ES solver is 2D only")
         sp = self.species[k]
@@ -584,9 +587,16 @@ class Sim:
         L.orc_update_periodic_from(C.byref(g), _p(rho), 1, 1, 1 if self.open_x else 0)
         ef = self.grid.zeros(3, self.real)
         dxa = np.array(list(self.dx) + [1.0], np.float64)
-        L.orc_es_solver_2d(C.byref(g), _p(dxa), _p(rho), _p(ef))
-        self.grid.interior(self.e)[...] += self.grid.interior(ef)
-        return ef
+        if ufl1 == 0:
+            L.orc_es_solver_2d(C.byref(g), _p(dxa), _p(rho), _p(ef))
+            self.grid.interior(self.e)[...] += self.grid.interior(ef)
+            return ef
+        bf = self.grid.zeros(3, self.real)
+        L.orc_es_solver_2d_beam(C.byref(g), _p(dxa), _p(rho), float(np.float32(ufl1)),
+                                int(self.open_x), _p(ef), _p(bf))
+        self.e += ef  # add(emf%e, ef), add(emf%b, bf): every cell
+        self.b += bf
+        return ef, bf
 
     def add_zpulse_wall(self, xmin=(0.0, 0.0, 0.0), **kw):
         """A zpulse_wall section (direction 1); needs open x walls (os-zpulse-wall.f03:

@@ -1476,3 +1476,44 @@ void orc_es_solver_2d(const orc_grid *g, const double *dx, const real *rho, real
       e[2] = 0;
     }
 }
+
+/* es_solver_2d_beam (emf/os-emf-es-solver.f03:457-616), one node, the branch init_fields
+ * takes for a species drifting along x (ufl(1) != 0, spec/os-spec-initfields.f90:77-92):
+ * every source cell with nonzero charge adds rho * sign(dy / gamma / 2, 0.5 + (j - rj))
+ * to E2 of its own column, for every y of the array (guards included, source order y
+ * outer, x inner); after the E guard refresh B3(i) = 0.5 u1 (E2(i) + E2(i+1)) (the sum in
+ * the working precision), then E2 *= gamma, E1 = E3 = 0.  gamma = sqrt(u1^2 + 1) with u1
+ * squared in the particle precision (float).  open_x: E2 guards along x are not refreshed.
+ * ef, bf are three-component grids (overwritten). */
+void orc_es_solver_2d_beam(const orc_grid *g, const double *dx, const real *rho, float u1,
+                           int open_x, real *ef, real *bf) {
+  const int nx = g->nx[0], ny = g->nx[1];
+  const long nc = orc_grid_cells(g);
+  memset(ef, 0, sizeof(real) * 3 * nc);
+  memset(bf, 0, sizeof(real) * 3 * nc);
+  float u1sq = u1 * u1;
+  double gamma = sqrt((double)u1sq + 1.0);
+  double charge_norm = dx[1] / gamma / 2;
+  for (int ri2 = 1; ri2 <= ny; ri2++)
+    for (int ri1 = 1; ri1 <= nx; ri1++) {
+      real r = rho[cidx(g, ri1, ri2, 1)];
+      if (r == 0) continue;
+      for (int ei2 = 1 - g->gcl[1]; ei2 <= ny + g->gcu[1]; ei2++) {
+        double s = 0.5 + (double)(ei2 - ri2) >= 0 ? charge_norm : -charge_norm;
+        real *e = &FLD(ef, g, 1, ri1, ei2, 1);
+        *e = (real)((double)*e + (double)r * s);
+      }
+    }
+  update_periodic_from(g, ef, 3, 0, open_x ? 1 : 0);
+  for (int ei2 = 1 - g->gcl[1]; ei2 <= ny + g->gcu[1]; ei2++)
+    for (int ei1 = 1; ei1 <= nx; ei1++) {
+      real sum = FLD(ef, g, 1, ei1, ei2, 1) + FLD(ef, g, 1, ei1 + 1, ei2, 1);
+      FLD(bf, g, 2, ei1, ei2, 1) = (real)(0.5 * (double)u1 * (double)sum);
+    }
+  update_periodic_from(g, bf, 3, 0, open_x ? 1 : 0);
+  for (long c = 0; c < nc; c++) {
+    ef[3 * c] = 0;
+    ef[3 * c + 1] = (real)((double)ef[3 * c + 1] * gamma);
+    ef[3 * c + 2] = 0;
+  }
+}

@@ -74,7 +74,7 @@ SIGNATURES = {
     "osr_zdf_save_particles": (I, [P, P, P, P]),
     "osr_sim_save_field": (I, [P, I, P, P]),
     "osr_zpulse_wall_defaults": (None, [P]),
-    "osr_sim_init_fields": (I, [P, I]),
+    "osr_sim_init_fields": (I, [P, I, C.c_float]),
     "osr_sim_add_zpulse_wall": (I, [P, P]),
     "osr_sim_zpulse_column": (I, [P, I, I, C.c_double, P, P, P]),
     "osr_copy_probe": (I, [I64, I, P]),

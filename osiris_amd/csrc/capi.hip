@@ -1499,7 +1499,7 @@ int osr_sim_gauss(osr_sim* s, double* res, double* rho_out, double* maxabs) {
   return OSR_OK;
 }
 
-int osr_sim_init_fields(osr_sim* s, int sid) {
+int osr_sim_init_fields(osr_sim* s, int sid, float ufl1) {
   TRY(check_sim(s));
   if (s->dims != 2) return fail(OSR_ERR_ARG, "the electrostatic solver is 2D only");
   if (sid < 0 || sid >= (int)s->sp.size()) return fail(OSR_ERR_ARG, "bad species id");
@@ -1522,8 +1522,22 @@ int osr_sim_init_fields(osr_sim* s, int sid) {
   const unsigned nb = (unsigned)((nt + 255) / 256);
   hipLaunchKernelGGL(es_charge, dim3(nb), dim3(256), 0, s->stream, s->g,
                      rho.as<unsigned long long>(), 1.0 / scale, charge.as<double>());
-  hipLaunchKernelGGL(es_direct_2d, dim3(nb), dim3(256), 0, s->stream, s->g, charge.as<double>(),
-                     s->dx[0], s->dx[1], s->dx[0] * s->dx[1]);
+  if (ufl1 == 0.0f) {  // es_solver
+    hipLaunchKernelGGL(es_direct_2d, dim3(nb), dim3(256), 0, s->stream, s->g, charge.as<double>(),
+                       s->dx[0], s->dx[1], s->dx[0] * s->dx[1]);
+  } else {  // es_solver_beam: gamma from u1^2 in the particle precision
+    const float u1sq = ufl1 * ufl1;
+    const double gamma = std::sqrt((double)u1sq + 1.0);
+    const double cn = s->dx[1] / gamma / 2;
+    DBuf e2u;
+    TRY(e2u.alloc(4 * (size_t)nt));
+    hipLaunchKernelGGL(es_beam_e2, dim3(nb), dim3(256), 0, s->stream, s->g, charge.as<double>(),
+                       cn, e2u.as<float>());
+    hipLaunchKernelGGL(es_beam_apply, dim3(nb), dim3(256), 0, s->stream, s->g, e2u.as<float>(),
+                       ufl1, gamma);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(s->stream));
+  }
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(s->stream));
   return OSR_OK;

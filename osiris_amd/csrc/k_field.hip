@@ -625,6 +625,50 @@ __global__ __launch_bounds__(256) void es_direct_2d(GridDesc g, const double* __
   g.E[2 * nc + c] = g.E[2 * nc + c] + 0.0f;  // E3 of the ES field is 0 (-0 becomes +0)
 }
 
+// es_solver_2d_beam (emf/os-emf-es-solver.f03:457-616), a species drifting along x: E2 of
+// each cell = sum over the charge of its own column, in source order (y ascending), of
+// rho * sign(dy / gamma / 2, 0.5 + (j - rj)), accumulated in FP32 through FP64 sums
+// (before the gamma scaling); written to e2u [ny][nxi]
+__global__ __launch_bounds__(256) void es_beam_e2(GridDesc g, const double* __restrict__ charge,
+                                                  double cn, float* e2u) {
+  const int nxi = interior_nx(g), ny = g.ny;
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long long)nxi * ny) return;
+  const int i = (int)(t % nxi), j = (int)(t / nxi);
+  float a = 0.f;
+  for (int rj = 0; rj < ny; rj++) {
+    const double q = charge[(long long)rj * nxi + i];
+    if (q != 0) a = (float)((double)a + q * (j - rj >= 0 ? cn : -cn));
+  }
+  e2u[t] = a;
+}
+
+// then B3 = 0.5 u1 (E2(i) + E2(i+1)) (FP32 sum; E2 beyond an open upper wall is zero),
+// E2 *= gamma, E1 = E3 = 0, and the fields are added to E and B (add(emf%e, ef),
+// add(emf%b, bf)); zero components are added too (-0 becomes +0 as in the reference)
+__global__ __launch_bounds__(256) void es_beam_apply(GridDesc g, const float* __restrict__ e2u,
+                                                     float u1, double gamma) {
+  const int nxi = interior_nx(g), ny = g.ny;
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long long)nxi * ny) return;
+  const int i = (int)(t % nxi), j = (int)(t / nxi);
+  const float e2 = e2u[t];
+  float e2n;
+  if (i + 1 < nxi)
+    e2n = e2u[t + 1];
+  else
+    e2n = g.open_nx ? 0.f : e2u[(long long)j * nxi];  // periodic image of x = 1
+  const float sum = e2 + e2n;
+  const float b3 = (float)(0.5 * (double)u1 * (double)sum);
+  const long long nc = g.ncell, c = (long long)j * g.nx + i;
+  g.E[c] = g.E[c] + 0.0f;
+  g.E[nc + c] = g.E[nc + c] + (float)((double)e2 * gamma);
+  g.E[2 * nc + c] = g.E[2 * nc + c] + 0.0f;
+  g.B[c] = g.B[c] + 0.0f;
+  g.B[nc + c] = g.B[nc + c] + 0.0f;
+  g.B[2 * nc + c] = g.B[2 * nc + c] + b3;
+}
+
 template __global__ void yee_dbdt<2, true>(const FieldDesc*, const int2*);
 template __global__ void yee_dbdt<2, false>(const FieldDesc*, const int2*);
 template __global__ void yee_dbdt<3, true>(const FieldDesc*, const int2*);

@@ -354,10 +354,11 @@ module m_osiris_amd
       real(c_double), intent(in) :: xmin(*)
       integer(c_int) :: osr_sim_save_field
     end function
-    function osr_sim_init_fields(sim, sid) bind(c, name="osr_sim_init_fields")
-      import :: c_int, c_ptr
+    function osr_sim_init_fields(sim, sid, ufl1) bind(c, name="osr_sim_init_fields")
+      import :: c_int, c_ptr, c_float
       type(c_ptr), value :: sim
       integer(c_int), value :: sid
+      real(c_float), value :: ufl1
       integer(c_int) :: osr_sim_init_fields
     end function
     function osr_sim_add_zpulse_wall(sim, zp) bind(c, name="osr_sim_add_zpulse_wall")

@@ -209,10 +209,11 @@ class Simulation:
         check(lib().osr_fields_fill(self._h, _p(None if e0 is None else np.array(e0, np.float64)),
                                     _p(None if b0 is None else np.array(b0, np.float64))))
 
-    def init_fields(self, sid):
-        """Add the electrostatic field of species sid's charge to E (2D; before the
-        first step)."""
-        check(lib().osr_sim_init_fields(self._h, sid))
+    def init_fields(self, sid, ufl1=0.0):
+        """Add the initial field of species sid's charge (2D; before the first step):
+        the electrostatic field for ufl1 = 0, the beam field of a species drifting along x
+        with fluid momentum ufl1 otherwise."""
+        check(lib().osr_sim_init_fields(self._h, sid, float(ufl1)))
 
     def add_zpulse_wall(self, xmin=(0.0, 0.0), **kw):
         """Laser injection from an x wall: keyword arguments as the zpulse_wall input

@@ -95,3 +95,33 @@ def test_fp32_accumulation_tracks_fp64():
     a = _solve(rho, "f64")
     b = _solve(rho.astype(np.float32), "f32")
     assert np.abs(b - a).max() < 2e-5 * np.abs(a).max()
+
+
+@pytest.mark.parametrize("u1", [0.5, -2.0])
+def test_beam_field_is_gauss_and_beta_e(u1):
+    """es_solver_2d_beam: the field of charged sheets drifting along x.  After the
+    solve, (E2(j) - E2(j-1)) / dy = rho(j) up to rounding between interior rows (discrete
+    Gauss law along y), and B3 = beta * (E2(i) + E2(i+1)) / 2 with beta = u1 / gamma."""
+    nx, ny = 16, 40
+    g = O.make_grid(2, (nx, ny), 2)
+    L = O._lib("f64")
+    rng = np.random.default_rng(9)
+    r = g.zeros(1, np.float64)
+    rho = np.zeros((ny, nx))
+    rho[10:30, :] = rng.uniform(0.5, 1.5, (20, nx))
+    g.interior(r)[..., 0] = rho
+    ef = g.zeros(3, np.float64)
+    bf = g.zeros(3, np.float64)
+    L.orc_es_solver_2d_beam(C.byref(g.c()), O._p(DXA), O._p(r), float(np.float32(u1)), 0,
+                            O._p(ef), O._p(bf))
+    E = g.interior(ef)
+    B = g.interior(bf)
+    # interior rows (the guard rows hold periodic images, which a finite column of sheets
+    # does not have)
+    gauss = (E[1:, :, 1] - E[:-1, :, 1]) / DX[1]
+    assert np.allclose(gauss, rho[1:], rtol=0, atol=1e-12 * np.abs(rho).max())
+    gamma = np.sqrt(float(np.float32(u1) ** 2) + 1.0)  # u1^2 in float32, the rest in FP64
+    e2p = np.concatenate([E[:, 1:, 1], E[:, :1, 1]], axis=1)  # periodic x + 1
+    want = np.float32(u1) / gamma * 0.5 * (E[..., 1] + e2p)
+    assert np.allclose(B[..., 2], want, rtol=1e-12, atol=1e-14)
+    assert not E[..., 0].any() and not E[..., 2].any() and not B[..., :2].any()

@@ -5,7 +5,8 @@ spec/os-spec-initfields.f90, es_solver_2d, emf/os-emf-es-solver.f03:125-236).
   bit: the density the device deposits (exact fixed-point sum, read back through
   osr_sim_gauss) is fed to the oracle's solver.
 * Against the FP64 oracle end to end (its own FP64 deposit): 1e-5 of max |E|, periodic
-  and open x walls; and 30 steps after the set-up, energies to 1e-4.
+  and open x walls; and 20-30 steps after the set-up, energies to 1e-4.
+* The same for the beam branch (es_solver_2d_beam, a species drifting along x), E and B.
 """
 import ctypes as C
 
@@ -71,6 +72,59 @@ def test_init_fields_vs_fp64_oracle(xbnd):
     assert np.max(np.abs(par - rpar) / np.abs(rpar)) <= 1e-4
     fscale = rfld.sum(1, keepdims=True)
     assert np.max(np.abs(fld - rfld) / fscale) <= 1e-4
+
+
+def test_beam_field_bitexact_given_the_charge():
+    """es_solver_2d_beam: from the device's charge density the FP32 oracle's beam field
+    added to E and B equals the GPU's bit for bit."""
+    rng = np.random.default_rng(5)
+    nx, dx, u1 = (48, 40), (0.1, 0.15), 0.8
+    ix, x, u, q = _plasma(rng, nx)
+    with oa.Simulation(2, nx, dx, 0.05, 2, tile=(16, 8)) as sim:
+        sid = sim.add_species(-1.0)
+        sim.set_particles(sid, ix, x, u, q)
+        rho = sim.charge_density()
+        sim.init_fields(sid, u1)
+        ge, gb = sim.get_fields()
+    g = O.make_grid(2, nx, 2)
+    L = O._lib("f32")
+    r = g.zeros(1, np.float32)
+    g.interior(r)[..., 0] = rho.astype(np.float32)
+    ef = g.zeros(3, np.float32)
+    bf = g.zeros(3, np.float32)
+    L.orc_es_solver_2d_beam(C.byref(g.c()), O._p(np.array([dx[0], dx[1], 1.0])), O._p(r),
+                            float(np.float32(u1)), 0, O._p(ef), O._p(bf))
+    we = np.zeros_like(ge) + g.interior(ef)  # added to zero fields
+    wb = np.zeros_like(gb) + g.interior(bf)
+    assert np.array_equal(ge.view(np.uint32), we.view(np.uint32))
+    assert np.array_equal(gb.view(np.uint32), wb.view(np.uint32))
+    assert np.abs(wb[..., 2]).max() > 0
+
+
+@pytest.mark.parametrize("xbnd", ["periodic", "open"])
+def test_beam_init_fields_vs_fp64_oracle(xbnd):
+    rng = np.random.default_rng(6)
+    nx, dx, dt, steps, u1 = (64, 32), (0.1, 0.1), 0.05, 20, 0.6
+    ix, x, u, q = _plasma(rng, nx)
+    u = u.copy()
+    u[:, 0] += u1
+    ref = O.Sim(2, nx, dx, dt, 2, open_x=(xbnd == "open"))
+    ref.add_species(-1.0, 0, ix, x, u, q)
+    ref.init_fields(0, u1)
+    with oa.Simulation(2, nx, dx, dt, 2, tile=(16, 16), xbnd=xbnd) as sim:
+        sid = sim.add_species(-1.0)
+        sim.set_particles(sid, ix, x, u, q)
+        sim.init_fields(sid, u1)
+        ge, gb = sim.get_fields()
+        we, wb = ref.grid.interior(ref.e), ref.grid.interior(ref.b)
+        assert np.abs(ge - we).max() <= 1e-5 * np.abs(we).max()
+        assert np.abs(gb - wb).max() <= 1e-5 * np.abs(wb).max()
+        sim.run(steps)
+        ref.step(steps)
+        fld, par = sim.history()
+    rfld, rpar = np.array(ref.fld_hist), np.array(ref.par_hist)
+    assert np.max(np.abs(par - rpar) / np.abs(rpar)) <= 1e-4
+    assert np.max(np.abs(fld - rfld) / rfld.sum(1, keepdims=True)) <= 1e-4
 
 
 def test_init_fields_rejects_3d():
