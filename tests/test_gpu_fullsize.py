@@ -9,7 +9,8 @@ depend on size (the CPU oracle cannot run 67 M or 134 M particles in test time):
   FP64 round-off of the per-tile sums.
 
 Sizes: 2D counter-streaming beams 1024 x 1024, 64 ppc (67 M particles), quadratic, Boris;
-3D magnetized thermal plasma 256^3, 8 ppc (134 M particles), cubic, Vay, B0 = 0.5 z.
+3D magnetized thermal plasma 256^3, 8 ppc (134 M particles), cubic, Vay, B0 = 0.5 z;
+2D laser-plasma 2048 x 512 between open x walls, 8 M particles, a0 = 2 laser wall.
 """
 import numpy as np
 import pytest
@@ -65,3 +66,33 @@ def test_full_size_invariants(name, make, tiles, steps):
     # independent of the tile shape
     assert np.array_equal(e0, e2) and np.array_equal(b0, b2) and np.array_equal(f0, f2)
     assert np.allclose(p0, p2, rtol=1e-12, atol=0)
+
+
+def test_laser_plasma_2048x512_full_size():
+    """The laser-plasma workload at full size (2048 x 512 cells, 8 M cold electrons behind a
+    ramp, a0 = 2 laser from the lower x wall, Lindman walls, absorbing particles), 300
+    steps: a repeated run is bit-identical, and another tile shape (guard band and mover
+    traffic differ) gives identical fields, surviving particles and counts."""
+    from bench import C4, c4_particles
+
+    ix, x, u, q = c4_particles((2048, 512), 100, 100, C4["ppc"], C4["density"])
+    out = []
+    for tile in ((32, 16), (32, 16), (16, 16)):
+        print(f"case c4 tile={tile}", flush=True)
+        with oa.Simulation(2, (2048, 512), C4["dx"], C4["dt"], 2, tile=tile, xbnd="open") as s:
+            sid = s.add_species(-1.0)
+            s.set_particles(sid, ix, x, u, q)
+            s.add_zpulse_wall(**C4["laser"])
+            s.run(300)
+            out.append((s.get_fields(gcl=(2, 2), gcu=(3, 3)), s.get_particles(sid), s.count(sid),
+                        s.history()))
+    (f0, p0, n0, h0), (f1, p1, n1, h1), (f2, p2, n2, h2) = out
+    assert h0[0][-1].sum() > 0 and h0[1][-1, 0] > 0  # the laser is in and has done work
+    assert n0 == n1 == n2 == len(p2[3])
+    for a, b in zip(f0 + p0, f1 + p1):
+        assert np.array_equal(a, b)
+    assert np.array_equal(h0[0], h1[0]) and np.array_equal(h0[1], h1[1])
+    for a, b in zip(f0 + p0, f2 + p2):
+        assert np.array_equal(a, b)
+    assert np.allclose(h0[0], h2[0], rtol=1e-12, atol=0)
+    assert np.allclose(h0[1], h2[1], rtol=1e-12, atol=0)
