@@ -64,8 +64,11 @@ def _lib(prec: str):
         L.orc_step.argtypes = [P, C.c_int, P, C.c_double, P, P, P, C.c_int, P, P, P, P, P, P,
                                P, P, P, P, P]
         L.orc_smooth.argtypes = [P, C.c_int, P, P, P]
-        L.orc_step_open.argtypes = [P, C.c_int, P, C.c_double, P, P, P, P, C.c_int, P, P, P,
-                                    P, P, P, P, P, P]
+        L.orc_step_open.argtypes = [P, C.c_int, P, C.c_double, P, P, P, P, P, C.c_int, P, P,
+                                    P, P, P, P, P, P, P]
+        L.orc_open_boundary_vpml.argtypes = [P, P, P, P]
+        L.orc_yee_advance_vpml.argtypes = [P, P, C.c_double, P, P, P, P]
+        L.orc_vpml_coefs.argtypes = [P, C.c_double if prec == "f64" else C.c_float]
         L.orc_open_boundary.argtypes = [P, P, P, P]
         L.orc_yee_advance_open.argtypes = [P, P, C.c_double, P, P, P, P]
         L.orc_particles_open.argtypes = [P, P, P, P, P, C.c_int]
@@ -106,6 +109,51 @@ class LindmanWalls:
         for d in range(len(dx)):
             self.c.dtdxi[d] = r(r(dt / dx[d]) * r(0.5))
         self.c.ebl, self.c.bbl, self.c.ebu = (a.ctypes.data for a in (self.ebl, self.bbl, self.ebu))
+
+
+VPML_MAX = 64  # ORC_VPML_MAX of pic_oracle.c
+
+
+def _vpml_struct(real):
+    """orc_vpml of pic_oracle.c for the working precision."""
+    R = C.c_double if real == np.float64 else C.c_float
+
+    class Vpml(C.Structure):
+        _fields_ = [("n", C.c_int), ("gc1", C.c_int), ("gc2", C.c_int), ("dtdx2", R),
+                    ("coef", R * VPML_MAX * 3 * 2 * 2), ("w", C.c_void_p * 2 * 2)]
+    return Vpml
+
+
+class VpmlWalls:
+    """VPML x walls of a 2D grid (setup_vpml, emf/boundary/os-emf-vpml.f03:270-474): both
+    walls zeroed, n cells thick, y guards gc1 = max(gcl, 1), gc2 = max(gcu, 2); the
+    attenuation coefficients from setup_att_coeff (:3586-3689) in the working precision.
+    e[wl], b[wl] have shape (ny + gc1 + gc2, n, 4) (wl 0 lower, 1 upper)."""
+
+    def __init__(self, grid, dx, dt, real, n=8):
+        if grid.dims != 2:
+            raise ValueError("VPML walls are implemented in 2D")
+        if not 4 <= n <= VPML_MAX:
+            raise ValueError(f"vpml_bnd_size must be in 4..{VPML_MAX}")
+        S = _vpml_struct(real)
+        self.c = S()
+        self.c.n = n
+        self.c.gc1 = max(grid.gcl[1], 1)
+        self.c.gc2 = max(grid.gcu[1], 2)
+        self.c.dtdx2 = real(dt / dx[1])
+        nrow = grid.nx[1] + self.c.gc1 + self.c.gc2
+        self.e = [np.zeros((nrow, n, 4), real) for _ in range(2)]
+        self.b = [np.zeros((nrow, n, 4), real) for _ in range(2)]
+        for wl in range(2):
+            self.c.w[wl][0] = self.e[wl].ctypes.data
+            self.c.w[wl][1] = self.b[wl].ctypes.data
+        L = _lib("f64" if real == np.float64 else "f32")
+        L.orc_vpml_coefs(C.byref(self.c), float(real(dt / dx[0])))
+
+    def coefs(self):
+        """[wall][E, B][1..3][cell] as an array (2, 2, 3, n)."""
+        a = np.ctypeslib.as_array(self.c.coef)
+        return a[..., :self.c.n].copy()
 
 
 class ZpulseParams(C.Structure):
@@ -472,9 +520,12 @@ class Sim:
     # from oracle.smooth.setup; None = no smoothing
     smooth_order: tuple = None
     smooth_coef: np.ndarray = None
-    # x walls: False = periodic; True = Lindman absorbing fields and absorbing particles
-    # (2D, y periodic)
+    # x walls: False = periodic; True = absorbing fields and absorbing particles
+    # (y, z periodic), the field walls of type xwall: "lindman" (2D, 3D) or "vpml" (2D,
+    # vpml_size cells thick)
     open_x: bool = False
+    xwall: str = "lindman"
+    vpml_size: int = 8
 
     def __post_init__(self):
         self.grid = make_grid(self.dims, self.nx, self.order)
@@ -496,7 +547,12 @@ class Sim:
         if self.open_x:
             if self.smooth_order is not None:
                 raise ValueError("open x walls are implemented without smoothing")
-            self.walls = LindmanWalls(self.grid, self.dx, self.dt, r)
+            if self.xwall == "vpml":
+                self.walls = VpmlWalls(self.grid, self.dx, self.dt, r, self.vpml_size)
+            elif self.xwall == "lindman":
+                self.walls = LindmanWalls(self.grid, self.dx, self.dt, r)
+            else:
+                raise ValueError(f"unknown x wall type {self.xwall!r}")
         self.fld_hist = []
         self.par_hist = []
 
@@ -566,12 +622,17 @@ class Sim:
             fe = np.zeros(6)
             pe = np.zeros(ns)
             L.orc_step_open(C.byref(g), self.order, _p(dxa), self.dt, _p(self.e), _p(self.b),
-                            _p(self.j), C.byref(self.walls.c), ns, _p(pus), _p(rqm), ixs, xs,
+                            _p(self.j), *self._wall_args(), ns, _p(pus), _p(rqm), ixs, xs,
                             us, qs, _p(npa), _p(fe), _p(pe))
             self.fld_hist.append(fe)
             self.par_hist.append(pe)
         for s, k in zip(self.species, npa):  # leading slices stay contiguous
             s.ix, s.x, s.u, s.q = s.ix[:k], s.x[:k], s.u[:k], s.q[:k]
+
+    def _wall_args(self):
+        if self.xwall == "vpml":
+            return None, C.byref(self.walls.c)
+        return C.byref(self.walls.c), None
 
     def init_fields(self, k, ufl1=0.0):
         """init_fields of species k (spec/os-spec-initfields.f90:38-100): rho deposited
@@ -606,11 +667,13 @@ class Sim:
         self.zpulses.append(zpulse_wall_params(self.nx, self.dx, xmin, **kw))
 
     def wall_refresh(self):
-        """The boundary update the next step starts with (Lindman restore, periodic y), so
+        """The boundary update the next step starts with (Lindman restore or VPML wall
+        refresh, periodic y), so
         that the guard cells hold what the next push reads."""
         L = _lib(self.prec)
         g = self.grid.c()
-        L.orc_open_boundary(C.byref(g), C.byref(self.walls.c), _p(self.e), _p(self.b))
+        fn = L.orc_open_boundary_vpml if self.xwall == "vpml" else L.orc_open_boundary
+        fn(C.byref(g), C.byref(self.walls.c), _p(self.e), _p(self.b))
 
     def gauss_residual(self):
         """div E - rho (interior nodes) with rho from all species (no ion background)."""

@@ -1250,15 +1250,204 @@ int orc_particles_open(const orc_grid *g, int *ix, real *x, real *u, real *q, in
   return k;
 }
 
+/* ------------------------------------------------------------------------------------
+ * VPML absorbing x walls, 2D (emf/boundary/os-emf-vpml.f03; emf_bound type "vpml" or
+ * "open", os-emf-boundary.f90:262-263), y periodic, no corners, no diffusion.
+ *
+ * Each wall holds split E and B fields, components 1 = X, 2 = Y, 3 = ZX, 4 = ZY
+ * (os-emf-vpml.f03:440-452), over n = vpml_bnd_size x cells: x = 2-n .. 1 at the lower wall,
+ * nx+1 .. nx+n at the upper one (setup_vpml :356-366), y index 1-gc1 .. ny+gc2 with
+ * gc1 = max(field gcl, 1), gc2 = max(field gcu, 2) (t_wall new with corner = 2,
+ * vdf/os-wall.f90:189-191).  Stored as (row, l, 4) with row = y - 1 + gc1, l = x - x0.
+ * All zero at creation.  The attenuation coefficients (setup_att_coeff :3586-3689) are
+ * indexed [wall][E or B][1..3][cell], cell 0-based.
+ * ------------------------------------------------------------------------------------ */
+#define ORC_VPML_MAX 64
+typedef struct {
+  int n;                              /* vpml_bnd_size */
+  int gc1, gc2;                       /* wall y guard cells */
+  real dtdx2;                         /* real(dt/dx2) */
+  real coef[2][2][3][ORC_VPML_MAX];   /* [lower, upper][E, B][coefficient][cell] */
+  real *w[2][2];                      /* [lower, upper][E, B] wall fields */
+} orc_vpml;
+
+static real rexp(real v) { return sizeof(real) == sizeof(float) ? (real)expf((float)v) : (real)exp((double)v); }
+
+/* setup_att_coeff (os-emf-vpml.f03:3586-3689) for direction 1: p_att_nexp = 4 (:48),
+ * s1 = -(0.8 (nexp+1)) / 2, s2 = 1 / (n-3); the power is the integer power t^4 = (t t)(t t)
+ * and every operation is in the field precision.  dtdx1 = real(dt/dx1). */
+void orc_vpml_coefs(orc_vpml *v, real dtdx1) {
+  const int n = v->n;
+  const real s1 = -(R(0.8) * (R(4) + R(1))) / R(2);
+  const real s2 = R(1) / (real)(n - 3);
+  const real dxdtdif = (R(1) - dtdx1) / (R(1) + dtdx1);
+#define P4(t) (((t) * (t)) * ((t) * (t)))
+  for (int j = 0; j < n; j++) {
+    const real rj = (real)j;
+    real t, tj, tj12, tj1, b1e, b1b;
+    /* lower wall: B is the outside point */
+    t = s2 * (rj - R(0.5)); tj = rexp(s1 * P4(t));
+    t = s2 * rj;            tj12 = rexp(s1 * P4(t));
+    t = s2 * (rj + R(0.5)); tj1 = rexp(s1 * P4(t));
+    b1e = dtdx1 * (R(1) + dxdtdif * (R(1) - tj12));
+    b1b = dtdx1 * (R(1) + dxdtdif * (R(1) - tj1));
+    const int k = n - 1 - j;
+    v->coef[0][0][0][k] = R(1) - b1e + tj12 * dtdx1;
+    v->coef[0][0][1][k] = tj * b1e;
+    v->coef[0][0][2][k] = dtdx1;
+    v->coef[0][1][0][k] = R(1) - b1b + tj1 * dtdx1;
+    v->coef[0][1][1][k] = tj12 * b1b;
+    v->coef[0][1][2][k] = dtdx1;
+    /* upper wall: E is the outside point */
+    t = s2 * rj;            tj = rexp(s1 * P4(t));
+    t = s2 * (rj + R(0.5)); tj12 = rexp(s1 * P4(t));
+    t = s2 * (rj + R(1.0)); tj1 = rexp(s1 * P4(t));
+    b1e = dtdx1 * (R(1) + dxdtdif * (R(1) - tj12));
+    b1b = dtdx1 * (R(1) + dxdtdif * (R(1) - tj1));
+    v->coef[1][0][0][j] = R(1) - b1e + tj12 * dtdx1;
+    v->coef[1][0][1][j] = dtdx1;
+    v->coef[1][0][2][j] = tj * b1e;
+    v->coef[1][1][0][j] = R(1) - b1b + tj1 * dtdx1;
+    v->coef[1][1][1][j] = dtdx1;
+    v->coef[1][1][2][j] = tj12 * b1b;
+  }
+#undef P4
+}
+
+static inline int vp_x0(const orc_grid *g, const orc_vpml *v, int wl) {
+  return wl == 0 ? 2 - v->n : g->nx[0] + 1;
+}
+/* wall entry: wall wl, E (0) or B (1), component c (0..3), Fortran x, y index i2 */
+#define VW(v, g, wl, eb, c, x, i2) \
+  ((v)->w[wl][eb][(((long)(i2) - 1 + (v)->gc1) * (v)->n + ((x) - vp_x0(g, v, wl))) * 4 + (c)])
+
+/* update_periodic_wall along y with p_vdf_replace (vdf/os-wall-comm.f90:182-344) */
+static void vpml_periodic(const orc_grid *g, orc_vpml *v) {
+  const int ny = g->nx[1];
+  for (int wl = 0; wl < 2; wl++)
+    for (int eb = 0; eb < 2; eb++)
+      for (int i2 = 1 - v->gc1; i2 <= ny + v->gc2; i2++) {
+        const int s = i2 < 1 ? i2 + ny : (i2 > ny ? i2 - ny : i2);
+        if (s == i2) continue;
+        for (int l = 0; l < v->n; l++) {
+          const int x = vp_x0(g, v, wl) + l;
+          for (int c = 0; c < 4; c++) VW(v, g, wl, eb, c, x, i2) = VW(v, g, wl, eb, c, x, s);
+        }
+      }
+}
+
+/* update_interface_x1_2d (os-emf-vpml.f03:2880-2997) without corners: the box value at
+ * box_to_vpml enters the wall (ZX takes the whole Z, ZY is zeroed), the wall value at
+ * vpml_to_box enters the box (Z = ZX + ZY), for y = 0 .. ny+2 */
+static void vpml_interface(const orc_grid *g, orc_vpml *v, int eb, real *f) {
+  const int n = v->n;
+  for (int wl = 0; wl < 2; wl++) {
+    const int x0 = vp_x0(g, v, wl);
+    const int btv = wl == 0 ? x0 + n - 1 : x0, vtb = wl == 0 ? x0 + n - 2 : x0 + 1;
+    for (int i2 = 0; i2 <= g->nx[1] + 2; i2++) {
+      VW(v, g, wl, eb, 0, btv, i2) = FLD(f, g, 0, btv, i2, 1);
+      VW(v, g, wl, eb, 1, btv, i2) = FLD(f, g, 1, btv, i2, 1);
+      VW(v, g, wl, eb, 2, btv, i2) = FLD(f, g, 2, btv, i2, 1);
+      VW(v, g, wl, eb, 3, btv, i2) = R(0);
+    }
+    for (int i2 = 0; i2 <= g->nx[1] + 2; i2++) {
+      FLD(f, g, 0, vtb, i2, 1) = VW(v, g, wl, eb, 0, vtb, i2);
+      FLD(f, g, 1, vtb, i2, 1) = VW(v, g, wl, eb, 1, vtb, i2);
+      FLD(f, g, 2, vtb, i2, 1) = VW(v, g, wl, eb, 2, vtb, i2) + VW(v, g, wl, eb, 3, vtb, i2);
+    }
+  }
+}
+
+/* update_e_vpml_2d (os-emf-vpml.f03:598-722) for the two x walls, after the first B half
+ * step: interface of B, then update_e_x1_2d (:925-1048), standard (non-corner) solver,
+ * x over (x0+1 .. x0+n-2) at the lower wall and (x0+1 .. x0+n-1) at the upper one */
+static void vpml_update_e(const orc_grid *g, orc_vpml *v, real *b) {
+  vpml_interface(g, v, 1, b);
+  const int n = v->n, ny = g->nx[1];
+  const real dt2 = v->dtdx2;
+  for (int wl = 0; wl < 2; wl++) {
+    const int x0 = vp_x0(g, v, wl);
+    const int lo = x0 + 1, hi = wl == 0 ? x0 + n - 2 : x0 + n - 1;
+    const real *ce = v->coef[wl][0][0];
+    for (int i2 = 2 - v->gc1; i2 <= ny + v->gc2; i2++)
+      for (int x = lo; x <= hi; x++) {
+        /* coefficient cell: coef_i = x - coef_std, coef_std = x0 - 1 (lower), x0 (upper) */
+        const int ci = wl == 0 ? x - x0 : x - x0 - 1;
+        const real c1 = ce[ci], c2 = ce[ORC_VPML_MAX + ci], c3 = ce[2 * ORC_VPML_MAX + ci];
+#define EW(c, xx, yy) VW(v, g, wl, 0, c, xx, yy)
+#define BW(c, xx, yy) VW(v, g, wl, 1, c, xx, yy)
+        EW(0, x, i2) = EW(0, x, i2) + dt2 * (BW(2, x, i2) + BW(3, x, i2) - BW(2, x, i2 - 1) -
+                                             BW(3, x, i2 - 1));
+        EW(3, x, i2) = EW(3, x, i2) - dt2 * (BW(0, x, i2) - BW(0, x, i2 - 1));
+        EW(1, x, i2) = c1 * EW(1, x, i2) - c2 * (BW(2, x, i2) + BW(3, x, i2)) +
+                       c3 * (BW(2, x - 1, i2) + BW(3, x - 1, i2));
+        EW(2, x, i2) = c1 * EW(2, x, i2) + c2 * BW(1, x, i2) - c3 * BW(1, x - 1, i2);
+      }
+  }
+}
+
+/* update_b_vpml_2d (os-emf-vpml.f03:1742-1861) after the E step: interface of E, then
+ * update_b_x1_2d (:2066-2181), x over (x0 .. x0+n-2) at the lower wall and
+ * (x0+1 .. x0+n-2) at the upper one */
+static void vpml_update_b(const orc_grid *g, orc_vpml *v, real *e) {
+  vpml_interface(g, v, 0, e);
+  const int n = v->n, ny = g->nx[1];
+  const real dt2 = v->dtdx2;
+  for (int wl = 0; wl < 2; wl++) {
+    const int x0 = vp_x0(g, v, wl);
+    const int lo = wl == 0 ? x0 : x0 + 1, hi = x0 + n - 2;
+    const real *cb = v->coef[wl][1][0];
+    for (int i2 = 1 - v->gc1; i2 <= ny + v->gc2 - 1; i2++)
+      for (int x = lo; x <= hi; x++) {
+        /* coef_std = x0 - 2 (lower), x0 (upper) */
+        const int ci = wl == 0 ? x - x0 + 1 : x - x0 - 1;
+        const real c1 = cb[ci], c2 = cb[ORC_VPML_MAX + ci], c3 = cb[2 * ORC_VPML_MAX + ci];
+        BW(0, x, i2) = BW(0, x, i2) - dt2 * (EW(2, x, i2 + 1) + EW(3, x, i2 + 1) - EW(2, x, i2) -
+                                             EW(3, x, i2));
+        BW(3, x, i2) = BW(3, x, i2) + dt2 * (EW(0, x, i2 + 1) - EW(0, x, i2));
+        BW(1, x, i2) = c1 * BW(1, x, i2) + c2 * (EW(2, x + 1, i2) + EW(3, x + 1, i2)) -
+                       c3 * (EW(2, x, i2) + EW(3, x, i2));
+        BW(2, x, i2) = c1 * BW(2, x, i2) - c2 * EW(1, x + 1, i2) + c3 * EW(1, x, i2);
+#undef EW
+#undef BW
+      }
+  }
+}
+
+/* the boundary update every step starts with, VPML walls: wall guards refreshed along y
+ * (update_boundary_vpml, os-emf-vpml.f03:506-524), then the periodic refresh of y */
+void orc_open_boundary_vpml(const orc_grid *g, orc_vpml *v, real *e, real *b) {
+  vpml_periodic(g, v);
+  update_periodic_from(g, e, 3, 0, 1);
+  update_periodic_from(g, b, 3, 0, 1);
+}
+
+/* advance_yee with VPML x walls: the wall E after the first B half step
+ * (update_boundary_bfld step 1, os-emf-boundary.f90:930-935), the wall B after the E step
+ * (update_boundary_efld, :1036-1041) */
+void orc_yee_advance_vpml(const orc_grid *g, const double *dx, double dt, real *e, real *b,
+                          const real *j, orc_vpml *v) {
+  dbdt(g, dx, dt / 2.0, b, e);
+  vpml_update_e(g, v, b);
+  dedt(g, dx, dt, e, b, j);
+  vpml_update_b(g, v, e);
+  dbdt(g, dx, dt / 2.0, b, e);
+}
+#undef VW
+
 /* orc_step with open x walls: boundary update of E/B (Lindman restore + periodic y, z),
  * push+deposit, particle boundaries, energies, J folded along y, z only, Yee with the
- * Lindman wall updates.  np[s] is updated to the particles left. */
+ * Lindman wall updates; with v non-null the walls are VPML (2D) instead.  np[s] is
+ * updated to the particles left. */
 void orc_step_open(const orc_grid *g, int order, const double *dx, double dt, real *e, real *b,
-                   real *j, orc_lindman *w, int nspec, const int *pusher, const real *rqm,
-                   int *const *ix, real *const *x, real *const *u, real *const *q, int *np,
-                   double *fld_ene, double *par_ene) {
+                   real *j, orc_lindman *w, orc_vpml *v, int nspec, const int *pusher,
+                   const real *rqm, int *const *ix, real *const *x, real *const *u,
+                   real *const *q, int *np, double *fld_ene, double *par_ene) {
   double dvol = dx[0] * dx[1] * (g->dims > 2 ? dx[2] : 1.0);
-  orc_open_boundary(g, w, e, b);
+  if (v)
+    orc_open_boundary_vpml(g, v, e, b);
+  else
+    orc_open_boundary(g, w, e, b);
   memset(j, 0, sizeof(real) * 3 * orc_grid_cells(g));
   for (int s = 0; s < nspec; s++) {
     double ene = orc_advance_deposit(g, order, pusher[s], rqm[s], dx, dt, e, b, j, ix[s], x[s],
@@ -1268,7 +1457,10 @@ void orc_step_open(const orc_grid *g, int order, const double *dx, double dt, re
   }
   if (fld_ene) orc_field_energy(g, e, b, dvol, fld_ene);
   update_periodic_from(g, j, 3, 1, 1);
-  orc_yee_advance_open(g, dx, dt, e, b, j, w);
+  if (v)
+    orc_yee_advance_vpml(g, dx, dt, e, b, j, v);
+  else
+    orc_yee_advance_open(g, dx, dt, e, b, j, w);
 }
 
 /* ------------------------------------------------------------------------------------
