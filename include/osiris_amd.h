@@ -131,21 +131,30 @@ typedef struct {
                      compile-time tile shape with a virtual-particle queue per wave
                      (tiles 8x8, 16x16, 32x16); both accumulate the current in int64
                      fixed point in LDS */
-  int xbnd;       /* x walls: OSR_BND_PERIODIC (0) or OSR_BND_OPEN (1) */
+  int xbnd;       /* x walls: OSR_BND_PERIODIC (0), OSR_BND_LINDMAN (1, also named
+                     OSR_BND_OPEN) or OSR_BND_VPML (2) */
   int reorder;    /* fused path: restore the (rank in cell, cell) particle order inside the
                      tiles every this many steps; 0 = default (4 in 3D, off in 2D),
                      negative = off.  Changes no result beyond FP64 round-off of the
                      energy sums (the counterpart of the reference's periodic sort,
                      sort_species, spec/os-species.f03:269-332) */
+  int vpml_size;  /* OSR_BND_VPML: wall thickness in cells, 4..64; 0 = 8 (vpml_bnd_size,
+                     p_vpml_size_default, emf/boundary/os-emf-boundary.f90:30, 230) */
 } osr_sim_desc;
 
-/* x boundaries of a resident simulation.  OSR_BND_OPEN: Lindman absorbing field walls at
- * both x ends (emf_bound type "lindman", emf/boundary/os-emf-lindman.f90:190-941, applied
- * by os-emf-boundary.f90:723-1112) and absorbing particles (spec_bound type "absorbing" /
- * "open", spec/os-spec-boundary.f03:150-151, 476-513: particles leaving through x are
- * removed after the push); y (and z) stay periodic.  Fused path, 2D and 3D, nx a multiple
- * of the tile width, no current smoothing; osr_sim_gauss is not available. */
-enum { OSR_BND_PERIODIC = 0, OSR_BND_OPEN = 1 };
+/* x boundaries of a resident simulation.  Both open types have absorbing particles
+ * (spec_bound type "absorbing" / "open", spec/os-spec-boundary.f03:150-151, 476-513:
+ * particles leaving through x are removed after the push); y (and z) stay periodic.
+ * OSR_BND_LINDMAN (= OSR_BND_OPEN): Lindman absorbing field walls at both x ends
+ * (emf_bound type "lindman", emf/boundary/os-emf-lindman.f90:190-941, applied by
+ * os-emf-boundary.f90:723-1112), 2D and 3D.  OSR_BND_VPML: the split-field perfectly
+ * matched layer of vpml_size cells at both x ends (emf_bound type "vpml", which the
+ * reference's deck also names "open", os-emf-boundary.f90:262-263; emf/boundary/
+ * os-emf-vpml.f03: setup_vpml :270-474, setup_att_coeff :3586-3689, update_e_vpml_2d
+ * :598-722, update_b_vpml_2d :1742-1861), 2D, without the optional diffusion
+ * (vpml_diffuse = .false., the default).  Fused path, nx a multiple of the tile width, no
+ * current smoothing; osr_sim_gauss is not available. */
+enum { OSR_BND_PERIODIC = 0, OSR_BND_OPEN = 1, OSR_BND_LINDMAN = 1, OSR_BND_VPML = 2 };
 
 /* OSR_PATH_FUSED: tiled fused push+deposit, LDS-staged fields and current, particles
  * sorted into tiles every step, reproducible current.  OSR_PATH_PLAIN: one kernel per
@@ -420,7 +429,7 @@ int osr_sim_init_fields(osr_sim *sim, int sid, float ufl1);
  * 3d, zpulse/os-zpulse-std.f03:2599-2753, 2760-3049, hermite_double os-math.f90:
  * 478-509).  Chirps, tilts, math-function envelopes, astigmatic and Laguerre beams and
  * other directions are not available.
- * The launch wall must be open (osr_sim_desc.xbnd = OSR_BND_OPEN), as the reference
+ * The launch wall must be open (osr_sim_desc.xbnd = OSR_BND_LINDMAN or OSR_BND_VPML), as the reference
  * requires (:600-608).  The FP64 amplitudes are evaluated on the host, a block of steps
  * at a time, and added in FP32 on the device.
  * ------------------------------------------------------------------------------- */

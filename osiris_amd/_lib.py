@@ -18,6 +18,8 @@ OSR_PATH_FUSED = 0
 OSR_PATH_PLAIN = 1
 OSR_BND_PERIODIC = 0
 OSR_BND_OPEN = 1
+OSR_BND_LINDMAN = 1
+OSR_BND_VPML = 2
 
 P = C.c_void_p
 I = C.c_int
@@ -102,7 +104,7 @@ class SimDesc(C.Structure):
     _fields_ = [("dims", C.c_int), ("nx", C.c_int * 3), ("dx", C.c_double * 3),
                 ("dt", C.c_double), ("order", C.c_int), ("tile", C.c_int * 3),
                 ("path", C.c_int), ("kernel", C.c_int), ("xbnd", C.c_int),
-                ("reorder", C.c_int)]
+                ("reorder", C.c_int), ("vpml_size", C.c_int)]
 
 
 class SmoothParams(C.Structure):

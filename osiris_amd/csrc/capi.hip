@@ -177,7 +177,8 @@ struct osr_sim {
   int sm_order[3] = {0, 0, 0};  // current smoothing per direction (0: none)
   float sm_coef[3][31] = {};
   DBuf sm_buf[2];
-  DBuf wall;  // Lindman wall buffers of open x walls (FieldDesc::wall)
+  DBuf wall;  // open x walls: Lindman buffers or VPML walls (FieldDesc::wall)
+  int vp_n = 0;  // VPML wall thickness (0: Lindman walls)
   std::vector<osr_zp::Wall> zp;  // laser walls, launch order
   DBuf zp_tab, zp_mask, zp_first;  // FieldDesc::las_* (kZpRows steps per fill)
   Ctrl* h_ctrl = nullptr;  // pinned mirror
@@ -195,6 +196,50 @@ constexpr int kZpRows = 256;  // steps of laser values per host evaluation (the 
 int zp_rows(const osr_sim* s, int K) {
   const long long per = 2ll * s->g.ny * s->g.nz * std::max(K, 1);
   return (int)std::max(1ll, std::min<long long>(kZpRows, (1ll << 26) / per));
+}
+
+// setup_att_coeff (emf/boundary/os-emf-vpml.f03:3586-3689) for direction 1, every
+// operation in FP32 as the reference's p_k_fld: s1 = -(0.8 (nexp + 1)) / 2 with
+// p_att_nexp = 4, s2 = 1 / (n - 3), attenuation exp(s1 (s2 p)^4) with the integer power
+// (t t)(t t); out = [lower, upper][E, B][1..3][cell], the lower wall's cells reversed
+// (cell n-1-j holds point j)
+void vpml_coefs(int n, float dtdx1, float* out) {
+  const float s1 = -(0.8f * (4.0f + 1.0f)) / 2.0f;
+  const float s2 = 1.0f / (float)(n - 3);
+  const float r = (1.0f - dtdx1) / (1.0f + dtdx1);
+  auto att = [&](float p) {
+    const float t = s2 * p;
+    return expf(s1 * ((t * t) * (t * t)));
+  };
+  auto at = [&](int wl, int eb, int k, int cell) -> float& {
+    return out[((wl * 2 + eb) * 3 + k) * n + cell];
+  };
+  for (int j = 0; j < n; j++) {
+    const float fj = (float)j;
+    // lower wall: B is the outside point
+    float tj = att(fj - 0.5f), tj12 = att(fj), tj1 = att(fj + 0.5f);
+    float b1e = dtdx1 * (1.0f + r * (1.0f - tj12));
+    float b1b = dtdx1 * (1.0f + r * (1.0f - tj1));
+    const int k = n - 1 - j;
+    at(0, 0, 0, k) = 1.0f - b1e + tj12 * dtdx1;
+    at(0, 0, 1, k) = tj * b1e;
+    at(0, 0, 2, k) = dtdx1;
+    at(0, 1, 0, k) = 1.0f - b1b + tj1 * dtdx1;
+    at(0, 1, 1, k) = tj12 * b1b;
+    at(0, 1, 2, k) = dtdx1;
+    // upper wall: E is the outside point
+    tj = att(fj);
+    tj12 = att(fj + 0.5f);
+    tj1 = att(fj + 1.0f);
+    b1e = dtdx1 * (1.0f + r * (1.0f - tj12));
+    b1b = dtdx1 * (1.0f + r * (1.0f - tj1));
+    at(1, 0, 0, j) = 1.0f - b1e + tj12 * dtdx1;
+    at(1, 0, 1, j) = dtdx1;
+    at(1, 0, 2, j) = tj * b1e;
+    at(1, 1, 0, j) = 1.0f - b1b + tj1 * dtdx1;
+    at(1, 1, 1, j) = dtdx1;
+    at(1, 1, 2, j) = tj12 * b1b;
+  }
 }
 
 // launches per step of the laser walls (two for a circularly polarised pulse)
@@ -312,6 +357,8 @@ void make_field_desc(const osr_sim* s, FieldDesc& f) {
   }
   if (s->g.open_nx) {  // new_lindman, emf/boundary/os-emf-lindman.f90:235-241
     f.wall = s->wall.as<float>();
+    f.vp_n = s->vp_n;  // setup_vpml, emf/boundary/os-emf-vpml.f03:468-470
+    f.vp_dt2 = s->dims > 1 ? (float)(s->dt / s->dx[1]) : 0.f;
     f.wall_dc = (float)((1.0 - s->dt / s->dx[0]) / (1.0 + s->dt / s->dx[0]));
     for (int d = 0; d < s->dims; d++) f.wall_dt[d] = (float)(s->dt / s->dx[d]) * 0.5f;
   }
@@ -734,6 +781,7 @@ int launch_fill(osr_sim* s) {
 
 // E and B on the x walls from the Lindman buffers (the state every push starts from)
 int launch_wall_restore(osr_sim* s) {
+  if (s->vp_n) return OSR_OK;  // VPML walls put nothing back on the box
   const unsigned nb = (unsigned)ceil_div(s->g.ny * s->g.nz, 256);
   hipLaunchKernelGGL(wall_restore, dim3(nb), dim3(256), 0, s->stream, s->d_fd.as<FieldDesc>());
   HIPC(hipGetLastError());
@@ -744,6 +792,18 @@ int launch_yee(osr_sim* s) {
   const FieldDesc* fd = s->d_fd.as<FieldDesc>();
   const int2* w = s->d_ywork.as<int2>();
   const int2* we = s->d_ework.as<int2>();
+  if (s->vp_n) {  // advance_yee with VPML x walls: update_boundary_bfld step 1 advances
+                   // the wall E, update_boundary_efld the wall B (os-emf-boundary.f90:930-935,
+                   // 1036-1041)
+    const unsigned nb = (unsigned)ceil_div(2 * s->vp_n * s->g.ny, 256);
+    hipLaunchKernelGGL((yee_dbdt<2, true>), dim3(s->nblk_e), dim3(256), 0, s->stream, fd, we);
+    hipLaunchKernelGGL(vpml_update<0>, dim3(nb), dim3(256), 0, s->stream, fd);
+    hipLaunchKernelGGL((yee_dedt<2>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
+    hipLaunchKernelGGL(vpml_update<1>, dim3(nb), dim3(256), 0, s->stream, fd);
+    hipLaunchKernelGGL((yee_dbdt<2, false>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
+    HIPC(hipGetLastError());
+    return OSR_OK;
+  }
   if (s->g.open_nx) {  // advance_yee with Lindman x walls (os-emf-solver-yee.f03:542-568)
     const unsigned nb = (unsigned)ceil_div(s->g.ny * s->g.nz, 256);
     if (s->dims == 2) {
@@ -968,9 +1028,14 @@ int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
   if (desc->path == OSR_PATH_PLAIN && dims != 2)
     return fail(OSR_ERR_ARG, "the plain path is 2D only");
   if (desc->kernel < 0 || desc->kernel > 2) return fail(OSR_ERR_ARG, "unknown kernel variant");
-  if (desc->xbnd != OSR_BND_PERIODIC && desc->xbnd != OSR_BND_OPEN)
+  if (desc->xbnd != OSR_BND_PERIODIC && desc->xbnd != OSR_BND_LINDMAN &&
+      desc->xbnd != OSR_BND_VPML)
     return fail(OSR_ERR_ARG, "unknown x boundary type");
-  const bool open_x = desc->xbnd == OSR_BND_OPEN;
+  const bool open_x = desc->xbnd != OSR_BND_PERIODIC;
+  const int vp_n = desc->xbnd == OSR_BND_VPML ? (desc->vpml_size > 0 ? desc->vpml_size : 8) : 0;
+  if (vp_n && dims != 2) return fail(OSR_ERR_ARG, "VPML x walls are implemented in 2D");
+  if (vp_n && (vp_n < 4 || vp_n > 64))
+    return fail(OSR_ERR_ARG, "vpml_size must be in 4..64 (setup_att_coeff divides by n - 3)");
   if (open_x && desc->path != OSR_PATH_FUSED)
     return fail(OSR_ERR_ARG, "open x walls need the fused path");
   std::unique_ptr<osr_sim> s(new osr_sim());
@@ -1027,7 +1092,17 @@ int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
   TRY(s->B.alloc(3 * nc * 4));
   TRY(s->J.alloc(3 * nc * 8));
   TRY(s->ctrl.alloc(sizeof(Ctrl)));
-  if (open_x) {
+  s->vp_n = vp_n;
+  if (open_x && vp_n) {
+    // both walls zero at creation (vdf/os-wall.f90:189-205), coefficients behind them
+    const size_t nw = (size_t)16 * vp_n * g.ny;
+    std::vector<float> coef(12 * (size_t)vp_n);
+    vpml_coefs(vp_n, (float)(desc->dt / desc->dx[0]), coef.data());
+    TRY(s->wall.alloc(sizeof(float) * (nw + coef.size())));
+    HIPC(hipMemsetAsync(s->wall.p, 0, sizeof(float) * nw, s->stream));
+    HIPC(hipMemcpyAsync(s->wall.as<float>() + nw, coef.data(), sizeof(float) * coef.size(),
+                        hipMemcpyHostToDevice, s->stream));
+  } else if (open_x) {
     const size_t wb = sizeof(float) * ((size_t)kWallEntries * g.ny * g.nz + g.nz + g.ny);
     TRY(s->wall.alloc(wb));
     HIPC(hipMemsetAsync(s->wall.p, 0, wb, s->stream));
@@ -2158,7 +2233,8 @@ void osr_zpulse_wall_defaults(osr_zpulse_wall* zp) {
 int osr_sim_add_zpulse_wall(osr_sim* s, const osr_zpulse_wall* zp) {
   TRY(check_sim(s));
   if (!zp) return fail(OSR_ERR_ARG, "null argument");
-  if (!s->g.open_nx) return fail(OSR_ERR_STATE, "the launch wall must be open (xbnd = OSR_BND_OPEN)");
+  if (!s->g.open_nx)
+    return fail(OSR_ERR_STATE, "the launch wall must be open (xbnd = OSR_BND_LINDMAN or OSR_BND_VPML)");
   if ((int)s->zp.size() >= osr_zp::kMaxPulses) return fail(OSR_ERR_ARG, "at most 8 laser walls");
   osr_zp::Wall w;
   const int nx[3] = {s->g.open_nx, s->g.ny, s->g.nz};

@@ -23,7 +23,8 @@ module m_osiris_amd
   integer(c_int), parameter, public :: OSR_PATH_FUSED = 0, OSR_PATH_PLAIN = 1
   ! x walls: periodic, or Lindman absorbing fields + absorbing particles (emf_bound
   ! "lindman", spec_bound "absorbing")
-  integer(c_int), parameter, public :: OSR_BND_PERIODIC = 0, OSR_BND_OPEN = 1
+  integer(c_int), parameter, public :: OSR_BND_PERIODIC = 0, OSR_BND_OPEN = 1, &
+                                       OSR_BND_LINDMAN = 1, OSR_BND_VPML = 2
 
   type, bind(c), public :: osr_sim_desc
     integer(c_int) :: dims = 2
@@ -36,6 +37,7 @@ module m_osiris_amd
     integer(c_int) :: kernel = 0
     integer(c_int) :: xbnd = OSR_BND_PERIODIC
     integer(c_int) :: reorder = 0
+    integer(c_int) :: vpml_size = 0
   end type osr_sim_desc
 
   ! current smoothing (vdf/os-vdf-smooth.f90): the reference's filter type codes and the

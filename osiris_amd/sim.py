@@ -14,8 +14,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (OSR_BND_OPEN, OSR_BND_PERIODIC, OSR_BORIS, OSR_PER, OSR_PROP, OSR_SMOOTH,
-                   OSR_TENV, OSR_VAY, SimDesc, SmoothParams, ZpulseWall, check, lib)
+from ._lib import (OSR_BND_LINDMAN, OSR_BND_OPEN, OSR_BND_PERIODIC, OSR_BND_VPML, OSR_BORIS,
+                   OSR_PER, OSR_PROP, OSR_SMOOTH, OSR_TENV, OSR_VAY, SimDesc, SmoothParams, ZpulseWall, check, lib)
 
 
 def _p(a):
@@ -114,13 +114,16 @@ class Simulation:
     """A resident simulation on the GPU (one per grid; species share the grid)."""
 
     def __init__(self, dims, nx, dx, dt, order, tile=None, plain=False, kernel=0,
-                 xbnd="periodic", reorder=0):
-        """xbnd: "periodic", or "open" for Lindman absorbing field walls and absorbing
-        particles at both x ends (2D fused path, nx a multiple of the tile width)."""
+                 xbnd="periodic", reorder=0, vpml_size=0):
+        """xbnd: "periodic"; "open" or "lindman" for Lindman absorbing field walls, "vpml"
+        for perfectly matched layers of vpml_size cells (0: 8; 2D), both with absorbing
+        particles at both x ends (fused path, nx a multiple of the tile width)."""
         d = SimDesc()
         d.path = 1 if plain else 0
         d.kernel = kernel
-        d.xbnd = {"periodic": OSR_BND_PERIODIC, "open": OSR_BND_OPEN}[xbnd]
+        d.xbnd = {"periodic": OSR_BND_PERIODIC, "open": OSR_BND_OPEN,
+                  "lindman": OSR_BND_LINDMAN, "vpml": OSR_BND_VPML}[xbnd]
+        d.vpml_size = vpml_size
         d.reorder = reorder  # in-tile particle order restored every `reorder` steps
         d.dims = dims
         for i in range(3):

@@ -61,6 +61,32 @@ def vacuum_pulse(nx, ny, dx, k0=4.0, width=2.0, direction=1, dtype=np.float64):
     return e, b
 
 
+def oblique_packet(nx, ny, dx, k0=4.0, m=2, direction=1):
+    """Interior E, B (shape (ny, nx, 3)) of a light packet along (kx, ky), ky = 2 pi m /
+    (ny dx) periodic on the box, |k| = k0, Gaussian envelope along x centred in the box,
+    both polarisations: E3 with B = k^ x z^ f, and in-plane E = z^ x k^ f with B3 = f, each
+    component sampled at its Yee point."""
+    ky = 2 * np.pi * m / (ny * dx)
+    kx = np.sqrt(k0 ** 2 - ky ** 2) * direction
+    kk = np.hypot(kx, ky)
+    x0 = nx * dx / 2
+    X, Y = np.meshgrid(np.arange(1, nx + 1) * dx, np.arange(1, ny + 1) * dx)
+    h = dx / 2
+
+    def f(x, y):
+        return np.exp(-((x - x0) / 2.0) ** 2) * np.cos(kx * (x - x0) + ky * y)
+
+    e = np.zeros((ny, nx, 3))
+    b = np.zeros((ny, nx, 3))
+    e[..., 2] = f(X, Y)
+    b[..., 0] = (ky / kk) * f(X, Y + h)
+    b[..., 1] = -(kx / kk) * f(X + h, Y)
+    e[..., 0] = -(ky / kk) * f(X + h, Y)
+    e[..., 1] = (kx / kk) * f(X, Y + h)
+    b[..., 2] = f(X + h, Y + h)
+    return e, b
+
+
 def mur_reflection(kdx, courant):
     """Reflection coefficient of the first-order Mur wall E_0^{n+1} = E_1^n +
     c (E_0^n - E_1^{n+1}), c = (1 - S)/(1 + S), on the 1D Yee lattice (waves along x on

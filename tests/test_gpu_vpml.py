@@ -1,0 +1,108 @@
+"""GPU parity of VPML absorbing x walls (emf/boundary/os-emf-vpml.f03, 2D, y periodic)
+with absorbing particles (spec/os-spec-boundary.f03:476-513) in the resident path.
+
+* Fields only: light packets at normal and oblique incidence (every field component and
+  both wall split pairs in play) run through the walls; the FP32 GPU fields equal the
+  FP32 oracle bit for bit on every interior and guard cell, for thicknesses 8 and 16 and
+  interpolation orders 1..3 (other guard bands), and the field energies agree to the
+  FP32 rounding of the oracle's squares.
+* A laser launched from the lower wall into a VPML-walled box: bit-exact fields.
+* Plasma: against the FP64 oracle, particle counts to the few particles that leave
+  within FP32 rounding of a wall, energy histories to 1e-4 plus their share.
+"""
+import numpy as np
+import pytest
+
+import osiris_amd as oa
+from oracle import oracle as O
+from osiris_amd._lib import OsrError
+from tests.pic_inputs import oblique_packet, thermal_plasma, vacuum_pulse
+
+pytestmark = pytest.mark.gpu
+
+
+def _fields_case(e, b, nx, ny, dx, dt, steps, order=2, n=8, tile=(16, 16), zpulse=None):
+    e, b = e.astype(np.float32), b.astype(np.float32)
+    ref = O.Sim(2, (nx, ny), (dx, dx), dt, order, prec="f32", open_x=True, xwall="vpml",
+                vpml_size=n)
+    g = ref.grid
+    with oa.Simulation(2, (nx, ny), (dx, dx), dt, order, tile=tile, xbnd="vpml",
+                       vpml_size=n) as sim:
+        if zpulse:
+            sim.add_zpulse_wall(**zpulse)
+            ref.add_zpulse_wall(**zpulse)
+        sim.set_fields(e, b)
+        ref.set_interior("e", e)
+        ref.set_interior("b", b)
+        sim.run(steps)
+        ref.step(steps)
+        ref.wall_refresh()  # periodic y images of the guard rows, as the GPU returns them
+        ge, gb = sim.get_fields(gcl=g.gcl, gcu=g.gcu)
+        fld, _ = sim.history()
+    assert np.array_equal(ge.view(np.uint32), ref.e.view(np.uint32))
+    assert np.array_equal(gb.view(np.uint32), ref.b.view(np.uint32))
+    assert np.allclose(fld, np.array(ref.fld_hist), rtol=2e-7, atol=0)
+    return fld
+
+
+@pytest.mark.parametrize("kind,direction", [("normal", 1), ("normal", -1), ("oblique", 1),
+                                            ("oblique", -1)])
+def test_vpml_fields_bitexact_vs_fp32_oracle(kind, direction):
+    nx, ny, dx, dt, steps = 64, 16, 0.2, 0.12, 90
+    print(f"case kind={kind} direction={direction}", flush=True)
+    if kind == "normal":
+        e, b = vacuum_pulse(nx, ny, dx, direction=direction)
+    else:
+        e, b = oblique_packet(nx, ny, dx, m=1, direction=direction)
+    fld = _fields_case(e, b, nx, ny, dx, dt, steps)
+    assert fld[-1].sum() < 0.2 * fld[0].sum()  # most of the packet has left
+
+
+@pytest.mark.parametrize("order,n", [(1, 8), (3, 8), (2, 16)])
+def test_vpml_orders_and_thickness(order, n):
+    print(f"case order={order} n={n}", flush=True)
+    nx, ny, dx, dt = 64, 16, 0.2, 0.12
+    e, b = oblique_packet(nx, ny, dx, m=1, direction=-1)
+    _fields_case(e, b, nx, ny, dx, dt, 80, order=order, n=n, tile=(8, 8))
+
+
+def test_vpml_laser_wall_bitexact():
+    print("case laser between VPML walls", flush=True)
+    nx, ny, dx, dt = 96, 32, 0.2, 0.12
+    z = np.zeros((ny, nx, 3))
+    zp = dict(a0=1.0, omega0=5.0, tenv_type="sin2", tenv_fwhm=3.0, per_type="gaussian",
+              per_w0=2.0, per_focus=2.0, per_center=(3.2,))
+    fld = _fields_case(z, z, nx, ny, dx, dt, 120, zpulse=zp)
+    assert fld[60].sum() > 0
+
+
+def test_vpml_plasma_vs_fp64_oracle():
+    rng = np.random.default_rng(72)
+    nx, dx, dt, steps = (64, 32), (0.1, 0.1), 0.07, 40
+    ix, x, u, q = thermal_plasma(rng, 2, nx, 8, 0.2)
+    ref = O.Sim(2, nx, dx, dt, 2, open_x=True, xwall="vpml")
+    ref.add_species(-1.0, 0, ix, x, u, q)
+    with oa.Simulation(2, nx, dx, dt, 2, tile=(16, 16), xbnd="vpml") as sim:
+        sid = sim.add_species(-1.0)
+        sim.set_particles(sid, ix, x, u, q)
+        sim.run(steps)
+        ref.step(steps)
+        n_gpu = sim.count(sid)
+        fld, par = sim.history()
+    n0, n_ref = len(q), len(ref.species[0].q)
+    print(f"particles {n0} -> gpu {n_gpu}, oracle {n_ref}", flush=True)
+    assert n_ref < n0 - 200
+    dn = abs(n_gpu - n_ref)
+    assert dn <= 3
+    tol = 1e-4 + 2.0 * dn / n_ref
+    rfld, rpar = np.array(ref.fld_hist), np.array(ref.par_hist)
+    assert np.max(np.abs(par - rpar) / np.abs(rpar)) <= tol
+    fscale = np.maximum(rfld.sum(1, keepdims=True), 1e-300)
+    assert np.max(np.abs(fld - rfld) / fscale) <= tol
+
+
+def test_vpml_rejects():
+    with pytest.raises(OsrError):
+        oa.Simulation(3, (16, 8, 8), (0.2,) * 3, 0.05, 2, xbnd="vpml")
+    with pytest.raises(OsrError):
+        oa.Simulation(2, (32, 16), (0.2, 0.2), 0.1, 2, xbnd="vpml", vpml_size=3)

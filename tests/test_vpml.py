@@ -14,13 +14,11 @@ os-emf-vpml.f03 (2D, x walls, no corners) against what the mathematics fixes.
 * The coefficients of thickness 8 are those of the reference's documented special case
   s1 = -2, s2 = 0.2 (os-emf-vpml.f03:3618-3621), evaluated in FP64.
 """
-import ctypes as C
-
 import numpy as np
 import pytest
 
 from oracle import oracle as O
-from tests.pic_inputs import vacuum_pulse
+from tests.pic_inputs import oblique_packet, vacuum_pulse
 
 
 def _coef_view(s):
@@ -103,29 +101,7 @@ def _reflected(xwall, direction=1, plant=None, oblique=False, n=8):
     if not oblique:
         e, b = vacuum_pulse(nx, ny, dx, k0=4.0, direction=direction)
     else:
-        # plane-wave packet along (kx, ky), ky periodic on the box, both polarisations
-        ky = 2 * np.pi * 2 / (ny * dx)
-        kx = np.sqrt(16.0 - ky ** 2) * direction
-        kk = np.hypot(kx, ky)
-        x0 = nx * dx / 2
-        xi = np.arange(1, nx + 1) * dx
-        yj = np.arange(1, ny + 1) * dx
-
-        def f(x, y):
-            return np.exp(-((x - x0) / 2.0) ** 2) * np.cos(kx * (x - x0) + ky * y)
-
-        X, Y = np.meshgrid(xi, yj)
-        h = dx / 2
-        e = np.zeros((ny, nx, 3))
-        b = np.zeros((ny, nx, 3))
-        # Ez polarisation: B = k^ x z^ f
-        e[..., 2] = f(X, Y)
-        b[..., 0] = (ky / kk) * f(X, Y + h)
-        b[..., 1] = -(kx / kk) * f(X + h, Y)
-        # in-plane polarisation: E = z^ x k^ g, B3 = g
-        e[..., 0] = -(ky / kk) * f(X + h, Y)
-        e[..., 1] = (kx / kk) * f(X, Y + h)
-        b[..., 2] = f(X + h, Y + h)
+        e, b = oblique_packet(nx, ny, dx, direction=direction)
     s.set_interior("e", e)
     s.set_interior("b", b)
     s.step(steps)
