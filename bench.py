@@ -240,10 +240,10 @@ def c4_particles(nx, gap, ramp, ppc, density):
     return ix, x, u, q
 
 
-def oracle_check_c4(oa, steps=160):
+def oracle_check_c4(oa, steps=160, xwall="lindman"):
     """The c4 physics (cold ramped plasma, a0 = 2 sin^2 laser with a gaussian profile from
-    x-min, Lindman x walls, absorbing particles) on a 256 x 32 grid with a 40-cell gap and
-    ramp: GPU against the FP64 oracle on the same particles."""
+    x-min, Lindman or VPML x walls, absorbing particles) on a 256 x 32 grid with a 40-cell
+    gap and ramp: GPU against the FP64 oracle on the same particles."""
     from oracle import oracle as O
 
     nx = (256, 32)
@@ -251,46 +251,59 @@ def oracle_check_c4(oa, steps=160):
     laser = dict(C4["laser"], tenv_rise=4.0, tenv_flat=4.0, tenv_fall=4.0, per_w0=4.0,
                  per_focus=12.0)
     ix, x, u, q = c4_particles(nx, 40, 40, C4["ppc"], C4["density"])
-    with oa.Simulation(2, nx, dx, dt, 2, tile=(32, 16), xbnd="open") as sim:
+    with oa.Simulation(2, nx, dx, dt, 2, tile=(32, 16), xbnd=xwall) as sim:
         sid = sim.add_species(-1.0)
         sim.set_particles(sid, ix, x, u, q)
         sim.add_zpulse_wall(**laser)
         sim.run(steps)
         fld, par = sim.history()
         n_gpu = sim.count(sid)
-    ref = O.Sim(2, nx, dx, dt, 2, open_x=True)
-    ref.add_species(-1.0, 0, ix, x, u, q)
-    ref.add_zpulse_wall(**laser)
-    ref.step(steps)
-    rfld, rpar = np.array(ref.fld_hist), np.array(ref.par_hist)
+    refs = {}
+    for prec in ("f64", "f32"):
+        ref = O.Sim(2, nx, dx, dt, 2, prec=prec, open_x=True, xwall=xwall)
+        ref.add_species(-1.0, 0, ix, x, u, q)
+        ref.add_zpulse_wall(**laser)
+        ref.step(steps)
+        refs[prec] = (np.array(ref.fld_hist), np.array(ref.par_hist), len(ref.species[0].q))
+    rfld, rpar, n_ref = refs["f64"]
     fscale = np.maximum(rfld.sum(1, keepdims=True), 1e-300)
     # the cold plasma has no kinetic energy until the laser reaches it: rows relative to
-    # themselves once the energy exceeds 1e-3 of its peak, and every row against the peak
+    # themselves once the energy exceeds 1e-3 of its peak.  The cold, laser-driven
+    # particle energy is sensitive to FP32 rounding: the FP32 oracle's own deviation from
+    # the FP64 one sets the scale the GPU's deviation is held to.
     peak = np.abs(rpar[:, 0]).max()
     live = np.abs(rpar[:, 0]) > 1e-3 * peak
-    return {"workload": ("c4 physics on 256x32 (gap and ramp 40 cells, sin^2 4/4/4, w0=4), "
-                         f"{len(q)} particles"),
+
+    def rel(a):
+        return float(np.max(np.abs(a[live] - rpar[live]) / np.abs(rpar[live])))
+
+    err_gpu, err_f32 = rel(par), rel(refs["f32"][1])
+    err_fld = float(np.max(np.abs(fld - rfld) / fscale))
+    return {"workload": ("c4 physics on 256x32 (gap and ramp 40 cells, sin^2 4/4/4, w0=4, "
+                         f"{xwall} walls), {len(q)} particles"),
             "steps": steps, "reference": "FP64 CPU oracle (restatement of the reference)",
-            "particles_left": {"gpu": int(n_gpu), "oracle": int(len(ref.species[0].q))},
-            "max_rel_err_particle_energy": float(np.max(np.abs(par[live] - rpar[live]) /
-                                                        np.abs(rpar[live]))),
+            "particles_left": {"gpu": int(n_gpu), "oracle": int(n_ref)},
+            "max_rel_err_particle_energy": err_gpu,
+            "fp32_oracle_max_rel_err_particle_energy": err_f32,
             "particle_energy_rows_compared": int(live.sum()),
-            "max_err_particle_energy_over_peak": float(np.max(np.abs(par - rpar)) / peak),
-            "max_rel_err_field_energy": float(np.max(np.abs(fld - rfld) / fscale)),
-            "tolerance": 1e-4}
+            "max_rel_err_field_energy": err_fld,
+            "tolerance": ("particle energy within 2x the FP32 oracle's own deviation from "
+                          "FP64; field energy 1e-4; equal particle counts"),
+            "pass": bool(err_gpu <= 2 * err_f32 and err_fld <= 1e-4 and n_gpu == n_ref)}
 
 
 def run_c4(args, oa):
     """Laser-plasma interaction: 2048 x 512 cells, dx = 0.2, dy = 0.5, dt = 0.14, cold
     electrons (8 ppc) at 0.01 n_c behind a 100-cell vacuum gap and a 100-cell linear ramp,
     a0 = 2 laser (omega0 = 10, sin^2 10/10/10, gaussian w0 = 20 focused at x = 40) launched
-    from x-min, Lindman x walls with absorbing particles, periodic y, quadratic, Boris."""
+    from x-min, Lindman (or, with --xwall vpml, 8-cell VPML) x walls with absorbing
+    particles, periodic y, quadratic, Boris."""
     nx = (2048, 512)
     dx, dt = C4["dx"], C4["dt"]
     tile = tuple(args.tile) if args.tile != [16, 16] else (32, 16)
     steps = min(args.steps, 1500)
     ix, x, u, q = c4_particles(nx, C4["gap_cells"], C4["ramp_cells"], C4["ppc"], C4["density"])
-    sim = oa.Simulation(2, nx, dx, dt, 2, tile=tile, xbnd="open")
+    sim = oa.Simulation(2, nx, dx, dt, 2, tile=tile, xbnd=args.xwall)
     sid = sim.add_species(-1.0)
     sim.set_particles(sid, ix, x, u, q)
     sim.add_zpulse_wall(**C4["laser"])
@@ -323,7 +336,8 @@ def run_c4(args, oa):
         "data": "synthetic (lattice-loaded cold plasma, host generated)",
         "config": {"workload": ("2D laser-plasma 2048x512, dx=0.2, dy=0.5, dt=0.14, 0.01 n_c "
                                 "with 100-cell ramp, 8 ppc cold, a0=2 zpulse_wall from x-min, "
-                                "Lindman x walls + absorbing particles, periodic y, quadratic"),
+                                f"{'Lindman' if args.xwall == 'lindman' else 'VPML (8 cells)'} "
+                                "x walls + absorbing particles, periodic y, quadratic"),
                    "particles": [n0, n1], "cells": cells, "tile": list(tile),
                    "relayouts": relayouts,
                    "field_energy_last": float(fld[-1].sum()),
@@ -338,7 +352,7 @@ def run_c4(args, oa):
         cb = cpu_baseline(nx, 8, n1, 2, dx, dt, args.cpu_seconds,
                           args.cpu_threads or min(16, os.cpu_count() or 1))
         if cb is not None:
-            cb["oracle_check"] = oracle_check_c4(oa)
+            cb["oracle_check"] = oracle_check_c4(oa, xwall=args.xwall)
             out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 
@@ -357,6 +371,8 @@ def main():
                     help="c2: the headline 2D two-stream step (default); c3: 3D periodic "
                          "magnetized thermal plasma 256^3, 8 ppc, cubic, Vay; c4: 2D "
                          "laser-plasma 2048x512 between open x walls")
+    ap.add_argument("--xwall", choices=("lindman", "vpml"), default="lindman",
+                    help="c4: field walls of the open x boundaries")
     ap.add_argument("--reorder", type=int, default=0,
                     help="in-tile particle order restored every N steps (0 = library default: "
                          "4 in 3D, off in 2D; negative = off)")
