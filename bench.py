@@ -240,10 +240,13 @@ def c4_particles(nx, gap, ramp, ppc, density):
     return ix, x, u, q
 
 
-def oracle_check_c4(oa, steps=160, xwall="lindman"):
+def oracle_check_c4(oa, steps=128, xwall="lindman"):
     """The c4 physics (cold ramped plasma, a0 = 2 sin^2 laser with a gaussian profile from
     x-min, Lindman or VPML x walls, absorbing particles) on a 256 x 32 grid with a 40-cell
-    gap and ramp: GPU against the FP64 oracle on the same particles."""
+    gap and ramp: GPU against the FP64 oracle on the same particles, over the steps in
+    which the laser enters the plasma.  Later the lattice-loaded cold plasma goes unstable
+    and amplifies rounding differences: the FP32 and FP64 oracles part by 6e-6 at step 136
+    and by 3e-3 at step 152, so those steps say nothing about the arithmetic."""
     from oracle import oracle as O
 
     nx = (256, 32)
@@ -268,9 +271,8 @@ def oracle_check_c4(oa, steps=160, xwall="lindman"):
     rfld, rpar, n_ref = refs["f64"]
     fscale = np.maximum(rfld.sum(1, keepdims=True), 1e-300)
     # the cold plasma has no kinetic energy until the laser reaches it: rows relative to
-    # themselves once the energy exceeds 1e-3 of its peak.  The cold, laser-driven
-    # particle energy is sensitive to FP32 rounding: the FP32 oracle's own deviation from
-    # the FP64 one sets the scale the GPU's deviation is held to.
+    # themselves once the energy exceeds 1e-3 of its peak; the FP32 oracle's own
+    # deviation from the FP64 one is reported beside the GPU's
     peak = np.abs(rpar[:, 0]).max()
     live = np.abs(rpar[:, 0]) > 1e-3 * peak
 
@@ -287,9 +289,8 @@ def oracle_check_c4(oa, steps=160, xwall="lindman"):
             "fp32_oracle_max_rel_err_particle_energy": err_f32,
             "particle_energy_rows_compared": int(live.sum()),
             "max_rel_err_field_energy": err_fld,
-            "tolerance": ("particle energy within 2x the FP32 oracle's own deviation from "
-                          "FP64; field energy 1e-4; equal particle counts"),
-            "pass": bool(err_gpu <= 2 * err_f32 and err_fld <= 1e-4 and n_gpu == n_ref)}
+            "tolerance": 1e-4,
+            "pass": bool(err_gpu <= 1e-4 and err_fld <= 1e-4 and n_gpu == n_ref)}
 
 
 def run_c4(args, oa):
