@@ -119,20 +119,21 @@ def _vpml_struct(real):
     R = C.c_double if real == np.float64 else C.c_float
 
     class Vpml(C.Structure):
-        _fields_ = [("n", C.c_int), ("gc1", C.c_int), ("gc2", C.c_int), ("dtdx2", R),
+        _fields_ = [("n", C.c_int), ("gc1", C.c_int), ("gc2", C.c_int), ("gc3", C.c_int),
+                    ("gc4", C.c_int), ("dtdx2", R), ("dtdx3", R),
                     ("coef", R * VPML_MAX * 3 * 2 * 2), ("w", C.c_void_p * 2 * 2)]
     return Vpml
 
 
 class VpmlWalls:
-    """VPML x walls of a 2D grid (setup_vpml, emf/boundary/os-emf-vpml.f03:270-474): both
-    walls zeroed, n cells thick, y guards gc1 = max(gcl, 1), gc2 = max(gcu, 2); the
-    attenuation coefficients from setup_att_coeff (:3586-3689) in the working precision.
-    e[wl], b[wl] have shape (ny + gc1 + gc2, n, 4) (wl 0 lower, 1 upper)."""
+    """VPML x walls of a 2D or 3D grid (setup_vpml, emf/boundary/os-emf-vpml.f03:270-474):
+    both walls zeroed, n cells thick, lower guards max(gcl, 1) and upper guards
+    max(gcu, 2) in y (and z); the attenuation coefficients from setup_att_coeff
+    (:3586-3689) in the working precision.  e[wl], b[wl] have shape (nz rows, ny rows, n,
+    components) (wl 0 lower, 1 upper; 4 split components in 2D, 6 in 3D; one z row in
+    2D)."""
 
     def __init__(self, grid, dx, dt, real, n=8):
-        if grid.dims != 2:
-            raise ValueError("VPML walls are implemented in 2D")
         if not 4 <= n <= VPML_MAX:
             raise ValueError(f"vpml_bnd_size must be in 4..{VPML_MAX}")
         S = _vpml_struct(real)
@@ -141,9 +142,15 @@ class VpmlWalls:
         self.c.gc1 = max(grid.gcl[1], 1)
         self.c.gc2 = max(grid.gcu[1], 2)
         self.c.dtdx2 = real(dt / dx[1])
-        nrow = grid.nx[1] + self.c.gc1 + self.c.gc2
-        self.e = [np.zeros((nrow, n, 4), real) for _ in range(2)]
-        self.b = [np.zeros((nrow, n, 4), real) for _ in range(2)]
+        nrow2 = grid.nx[1] + self.c.gc1 + self.c.gc2
+        nrow3, nc = 1, 4
+        if grid.dims > 2:
+            self.c.gc3 = max(grid.gcl[2], 1)
+            self.c.gc4 = max(grid.gcu[2], 2)
+            self.c.dtdx3 = real(dt / dx[2])
+            nrow3, nc = grid.nx[2] + self.c.gc3 + self.c.gc4, 6
+        self.e = [np.zeros((nrow3, nrow2, n, nc), real) for _ in range(2)]
+        self.b = [np.zeros((nrow3, nrow2, n, nc), real) for _ in range(2)]
         for wl in range(2):
             self.c.w[wl][0] = self.e[wl].ctypes.data
             self.c.w[wl][1] = self.b[wl].ctypes.data
@@ -521,8 +528,8 @@ class Sim:
     smooth_order: tuple = None
     smooth_coef: np.ndarray = None
     # x walls: False = periodic; True = absorbing fields and absorbing particles
-    # (y, z periodic), the field walls of type xwall: "lindman" (2D, 3D) or "vpml" (2D,
-    # vpml_size cells thick)
+    # (y, z periodic), the field walls of type xwall: "lindman" or "vpml" (vpml_size
+    # cells thick)
     open_x: bool = False
     xwall: str = "lindman"
     vpml_size: int = 8

@@ -1251,22 +1251,23 @@ int orc_particles_open(const orc_grid *g, int *ix, real *x, real *u, real *q, in
 }
 
 /* ------------------------------------------------------------------------------------
- * VPML absorbing x walls, 2D (emf/boundary/os-emf-vpml.f03; emf_bound type "vpml" or
- * "open", os-emf-boundary.f90:262-263), y periodic, no corners, no diffusion.
+ * VPML absorbing x walls, 2D and 3D (emf/boundary/os-emf-vpml.f03; emf_bound type "vpml"
+ * or "open", os-emf-boundary.f90:262-263), y (and z) periodic, no corners, no diffusion.
  *
- * Each wall holds split E and B fields, components 1 = X, 2 = Y, 3 = ZX, 4 = ZY
- * (os-emf-vpml.f03:440-452), over n = vpml_bnd_size x cells: x = 2-n .. 1 at the lower wall,
- * nx+1 .. nx+n at the upper one (setup_vpml :356-366), y index 1-gc1 .. ny+gc2 with
- * gc1 = max(field gcl, 1), gc2 = max(field gcu, 2) (t_wall new with corner = 2,
- * vdf/os-wall.f90:189-191).  Stored as (row, l, 4) with row = y - 1 + gc1, l = x - x0.
- * All zero at creation.  The attenuation coefficients (setup_att_coeff :3586-3689) are
- * indexed [wall][E or B][1..3][cell], cell 0-based.
+ * Each wall holds split E and B fields (os-emf-vpml.f03:440-452): in 2D the components
+ * 1 = X, 2 = Y, 3 = ZX, 4 = ZY; in 3D 1 = XY, 2 = XZ, 3 = YX, 4 = YZ, 5 = ZX, 6 = ZY.  The
+ * walls cover n = vpml_bnd_size x cells: x = 2-n .. 1 at the lower wall, nx+1 .. nx+n at
+ * the upper one (setup_vpml :356-366); y index 1-gc1 .. ny+gc2 and z index 1-gc3 ..
+ * nz+gc4 with lower guards max(field gcl, 1) and upper guards max(field gcu, 2) (t_wall
+ * new with corner = 2, vdf/os-wall.f90:189-191).  Stored as (z row, y row, l, component)
+ * with l = x - x0.  All zero at creation.  The attenuation coefficients
+ * (setup_att_coeff :3586-3689) are indexed [wall][E or B][1..3][cell], cell 0-based.
  * ------------------------------------------------------------------------------------ */
 #define ORC_VPML_MAX 64
 typedef struct {
   int n;                              /* vpml_bnd_size */
-  int gc1, gc2;                       /* wall y guard cells */
-  real dtdx2;                         /* real(dt/dx2) */
+  int gc1, gc2, gc3, gc4;             /* wall guard rows: y lower, upper, z lower, upper */
+  real dtdx2, dtdx3;                  /* real(dt/dx2), real(dt/dx3) */
   real coef[2][2][3][ORC_VPML_MAX];   /* [lower, upper][E, B][coefficient][cell] */
   real *w[2][2];                      /* [lower, upper][E, B] wall fields */
 } orc_vpml;
@@ -1317,105 +1318,180 @@ void orc_vpml_coefs(orc_vpml *v, real dtdx1) {
 static inline int vp_x0(const orc_grid *g, const orc_vpml *v, int wl) {
   return wl == 0 ? 2 - v->n : g->nx[0] + 1;
 }
-/* wall entry: wall wl, E (0) or B (1), component c (0..3), Fortran x, y index i2 */
-#define VW(v, g, wl, eb, c, x, i2) \
-  ((v)->w[wl][eb][(((long)(i2) - 1 + (v)->gc1) * (v)->n + ((x) - vp_x0(g, v, wl))) * 4 + (c)])
+static inline long vp_idx(const orc_grid *g, const orc_vpml *v, int wl, int c, int x, int i2,
+                          int i3) {
+  const int NC = g->dims > 2 ? 6 : 4;
+  const long ld2 = (long)g->nx[1] + v->gc1 + v->gc2;
+  const long r3 = g->dims > 2 ? (long)(i3 - 1 + v->gc3) : 0;
+  return ((r3 * ld2 + (i2 - 1 + v->gc1)) * v->n + (x - vp_x0(g, v, wl))) * NC + c;
+}
+/* wall entry: wall wl, E (0) or B (1), component c (0-based), Fortran x, y, z indices */
+#define VW(v, g, wl, eb, c, x, i2, i3) ((v)->w[wl][eb][vp_idx(g, v, wl, c, x, i2, i3)])
 
-/* update_periodic_wall along y with p_vdf_replace (vdf/os-wall-comm.f90:182-344) */
+/* update_periodic_wall along y, then z, with p_vdf_replace (vdf/os-wall-comm.f90:182-344) */
 static void vpml_periodic(const orc_grid *g, orc_vpml *v) {
-  const int ny = g->nx[1];
+  const int ny = g->nx[1], NC = g->dims > 2 ? 6 : 4;
+  const int nz = g->dims > 2 ? g->nx[2] : 1;
+  const int klo = g->dims > 2 ? 1 - v->gc3 : 1, khi = g->dims > 2 ? nz + v->gc4 : 1;
   for (int wl = 0; wl < 2; wl++)
-    for (int eb = 0; eb < 2; eb++)
-      for (int i2 = 1 - v->gc1; i2 <= ny + v->gc2; i2++) {
-        const int s = i2 < 1 ? i2 + ny : (i2 > ny ? i2 - ny : i2);
-        if (s == i2) continue;
-        for (int l = 0; l < v->n; l++) {
-          const int x = vp_x0(g, v, wl) + l;
-          for (int c = 0; c < 4; c++) VW(v, g, wl, eb, c, x, i2) = VW(v, g, wl, eb, c, x, s);
+    for (int eb = 0; eb < 2; eb++) {
+      for (int i3 = klo; i3 <= khi; i3++)
+        for (int i2 = 1 - v->gc1; i2 <= ny + v->gc2; i2++) {
+          const int s = i2 < 1 ? i2 + ny : (i2 > ny ? i2 - ny : i2);
+          if (s == i2) continue;
+          for (int l = 0; l < v->n; l++) {
+            const int x = vp_x0(g, v, wl) + l;
+            for (int c = 0; c < NC; c++)
+              VW(v, g, wl, eb, c, x, i2, i3) = VW(v, g, wl, eb, c, x, s, i3);
+          }
         }
+      if (g->dims < 3) continue;
+      for (int i3 = klo; i3 <= khi; i3++) {
+        const int s = i3 < 1 ? i3 + nz : (i3 > nz ? i3 - nz : i3);
+        if (s == i3) continue;
+        for (int i2 = 1 - v->gc1; i2 <= ny + v->gc2; i2++)
+          for (int l = 0; l < v->n; l++) {
+            const int x = vp_x0(g, v, wl) + l;
+            for (int c = 0; c < NC; c++)
+              VW(v, g, wl, eb, c, x, i2, i3) = VW(v, g, wl, eb, c, x, i2, s);
+          }
       }
+    }
 }
 
-/* update_interface_x1_2d (os-emf-vpml.f03:2880-2997) without corners: the box value at
- * box_to_vpml enters the wall (ZX takes the whole Z, ZY is zeroed), the wall value at
- * vpml_to_box enters the box (Z = ZX + ZY), for y = 0 .. ny+2 */
+/* update_interface_x1_2d (os-emf-vpml.f03:2880-2997) / update_interface_x1_3d
+ * (:2997-3186) without corners.  The box value at box_to_vpml enters the wall: in 2D over
+ * y = 0 .. ny+2 with ZX taking the whole Z and ZY zeroed; in 3D over y = 1 .. ny+1,
+ * z = 1 .. nz+1 with XY, YX, ZX taking the whole components and XZ, YZ, ZY zeroed.  The
+ * wall value at vpml_to_box enters the box as the sum of its split pair, over y (and z)
+ * = 0 .. n+2. */
 static void vpml_interface(const orc_grid *g, orc_vpml *v, int eb, real *f) {
-  const int n = v->n;
+  const int n = v->n, ny = g->nx[1], D3 = g->dims > 2;
+  const int nz = D3 ? g->nx[2] : 1;
   for (int wl = 0; wl < 2; wl++) {
     const int x0 = vp_x0(g, v, wl);
     const int btv = wl == 0 ? x0 + n - 1 : x0, vtb = wl == 0 ? x0 + n - 2 : x0 + 1;
-    for (int i2 = 0; i2 <= g->nx[1] + 2; i2++) {
-      VW(v, g, wl, eb, 0, btv, i2) = FLD(f, g, 0, btv, i2, 1);
-      VW(v, g, wl, eb, 1, btv, i2) = FLD(f, g, 1, btv, i2, 1);
-      VW(v, g, wl, eb, 2, btv, i2) = FLD(f, g, 2, btv, i2, 1);
-      VW(v, g, wl, eb, 3, btv, i2) = R(0);
+    if (!D3) {
+      for (int i2 = 0; i2 <= ny + 2; i2++) {
+        VW(v, g, wl, eb, 0, btv, i2, 1) = FLD(f, g, 0, btv, i2, 1);
+        VW(v, g, wl, eb, 1, btv, i2, 1) = FLD(f, g, 1, btv, i2, 1);
+        VW(v, g, wl, eb, 2, btv, i2, 1) = FLD(f, g, 2, btv, i2, 1);
+        VW(v, g, wl, eb, 3, btv, i2, 1) = R(0);
+      }
+      for (int i2 = 0; i2 <= ny + 2; i2++) {
+        FLD(f, g, 0, vtb, i2, 1) = VW(v, g, wl, eb, 0, vtb, i2, 1);
+        FLD(f, g, 1, vtb, i2, 1) = VW(v, g, wl, eb, 1, vtb, i2, 1);
+        FLD(f, g, 2, vtb, i2, 1) = VW(v, g, wl, eb, 2, vtb, i2, 1) + VW(v, g, wl, eb, 3, vtb, i2, 1);
+      }
+      continue;
     }
-    for (int i2 = 0; i2 <= g->nx[1] + 2; i2++) {
-      FLD(f, g, 0, vtb, i2, 1) = VW(v, g, wl, eb, 0, vtb, i2);
-      FLD(f, g, 1, vtb, i2, 1) = VW(v, g, wl, eb, 1, vtb, i2);
-      FLD(f, g, 2, vtb, i2, 1) = VW(v, g, wl, eb, 2, vtb, i2) + VW(v, g, wl, eb, 3, vtb, i2);
-    }
+    for (int i3 = 1; i3 <= nz + 1; i3++)
+      for (int i2 = 1; i2 <= ny + 1; i2++)
+        for (int c = 0; c < 3; c++) {
+          VW(v, g, wl, eb, 2 * c, btv, i2, i3) = FLD(f, g, c, btv, i2, i3);
+          VW(v, g, wl, eb, 2 * c + 1, btv, i2, i3) = R(0);
+        }
+    for (int i3 = 0; i3 <= nz + 2; i3++)
+      for (int i2 = 0; i2 <= ny + 2; i2++)
+        for (int c = 0; c < 3; c++)
+          FLD(f, g, c, vtb, i2, i3) =
+              VW(v, g, wl, eb, 2 * c, vtb, i2, i3) + VW(v, g, wl, eb, 2 * c + 1, vtb, i2, i3);
   }
 }
 
-/* update_e_vpml_2d (os-emf-vpml.f03:598-722) for the two x walls, after the first B half
- * step: interface of B, then update_e_x1_2d (:925-1048), standard (non-corner) solver,
- * x over (x0+1 .. x0+n-2) at the lower wall and (x0+1 .. x0+n-1) at the upper one */
+#define EW(c, xx, yy, zz) VW(v, g, wl, 0, c, xx, yy, zz)
+#define BW(c, xx, yy, zz) VW(v, g, wl, 1, c, xx, yy, zz)
+/* update_e_vpml_2d / 3d (os-emf-vpml.f03:598-722, 726-856) for the two x walls, after the
+ * first B half step: interface of B, then update_e_x1_2d (:925-1048) / update_e_x1_3d
+ * (:1172-1365), standard (non-corner) solver, x over (x0+1 .. x0+n-2) at the lower wall
+ * and (x0+1 .. x0+n-1) at the upper one */
 static void vpml_update_e(const orc_grid *g, orc_vpml *v, real *b) {
   vpml_interface(g, v, 1, b);
-  const int n = v->n, ny = g->nx[1];
-  const real dt2 = v->dtdx2;
+  const int n = v->n, ny = g->nx[1], D3 = g->dims > 2;
+  const int klo = D3 ? 2 - v->gc3 : 1, khi = D3 ? g->nx[2] + v->gc4 : 1;
+  const real d2 = v->dtdx2, d3 = v->dtdx3;
   for (int wl = 0; wl < 2; wl++) {
     const int x0 = vp_x0(g, v, wl);
     const int lo = x0 + 1, hi = wl == 0 ? x0 + n - 2 : x0 + n - 1;
     const real *ce = v->coef[wl][0][0];
-    for (int i2 = 2 - v->gc1; i2 <= ny + v->gc2; i2++)
-      for (int x = lo; x <= hi; x++) {
-        /* coefficient cell: coef_i = x - coef_std, coef_std = x0 - 1 (lower), x0 (upper) */
-        const int ci = wl == 0 ? x - x0 : x - x0 - 1;
-        const real c1 = ce[ci], c2 = ce[ORC_VPML_MAX + ci], c3 = ce[2 * ORC_VPML_MAX + ci];
-#define EW(c, xx, yy) VW(v, g, wl, 0, c, xx, yy)
-#define BW(c, xx, yy) VW(v, g, wl, 1, c, xx, yy)
-        EW(0, x, i2) = EW(0, x, i2) + dt2 * (BW(2, x, i2) + BW(3, x, i2) - BW(2, x, i2 - 1) -
-                                             BW(3, x, i2 - 1));
-        EW(3, x, i2) = EW(3, x, i2) - dt2 * (BW(0, x, i2) - BW(0, x, i2 - 1));
-        EW(1, x, i2) = c1 * EW(1, x, i2) - c2 * (BW(2, x, i2) + BW(3, x, i2)) +
-                       c3 * (BW(2, x - 1, i2) + BW(3, x - 1, i2));
-        EW(2, x, i2) = c1 * EW(2, x, i2) + c2 * BW(1, x, i2) - c3 * BW(1, x - 1, i2);
-      }
+    for (int k = klo; k <= khi; k++)
+      for (int j = 2 - v->gc1; j <= ny + v->gc2; j++)
+        for (int x = lo; x <= hi; x++) {
+          /* coefficient cell: coef_i = x - coef_std, coef_std = x0 - 1 (lower), x0 (upper) */
+          const int ci = wl == 0 ? x - x0 : x - x0 - 1;
+          const real c1 = ce[ci], c2 = ce[ORC_VPML_MAX + ci], c3 = ce[2 * ORC_VPML_MAX + ci];
+          if (!D3) {
+            EW(0, x, j, 1) = EW(0, x, j, 1) + d2 * (BW(2, x, j, 1) + BW(3, x, j, 1) -
+                                                    BW(2, x, j - 1, 1) - BW(3, x, j - 1, 1));
+            EW(3, x, j, 1) = EW(3, x, j, 1) - d2 * (BW(0, x, j, 1) - BW(0, x, j - 1, 1));
+            EW(1, x, j, 1) = c1 * EW(1, x, j, 1) - c2 * (BW(2, x, j, 1) + BW(3, x, j, 1)) +
+                             c3 * (BW(2, x - 1, j, 1) + BW(3, x - 1, j, 1));
+            EW(2, x, j, 1) = c1 * EW(2, x, j, 1) + c2 * BW(1, x, j, 1) - c3 * BW(1, x - 1, j, 1);
+            continue;
+          }
+          EW(0, x, j, k) = EW(0, x, j, k) + d2 * (BW(4, x, j, k) + BW(5, x, j, k) -
+                                                  BW(4, x, j - 1, k) - BW(5, x, j - 1, k));
+          EW(5, x, j, k) = EW(5, x, j, k) - d2 * (BW(0, x, j, k) + BW(1, x, j, k) -
+                                                  BW(0, x, j - 1, k) - BW(1, x, j - 1, k));
+          EW(1, x, j, k) = EW(1, x, j, k) - d3 * (BW(2, x, j, k) + BW(3, x, j, k) -
+                                                  BW(2, x, j, k - 1) - BW(3, x, j, k - 1));
+          EW(3, x, j, k) = EW(3, x, j, k) + d3 * (BW(0, x, j, k) + BW(1, x, j, k) -
+                                                  BW(0, x, j, k - 1) - BW(1, x, j, k - 1));
+          EW(2, x, j, k) = c1 * EW(2, x, j, k) - c2 * (BW(4, x, j, k) + BW(5, x, j, k)) +
+                           c3 * (BW(4, x - 1, j, k) + BW(5, x - 1, j, k));
+          EW(4, x, j, k) = c1 * EW(4, x, j, k) + c2 * (BW(2, x, j, k) + BW(3, x, j, k)) -
+                           c3 * (BW(2, x - 1, j, k) + BW(3, x - 1, j, k));
+        }
   }
 }
 
-/* update_b_vpml_2d (os-emf-vpml.f03:1742-1861) after the E step: interface of E, then
- * update_b_x1_2d (:2066-2181), x over (x0 .. x0+n-2) at the lower wall and
- * (x0+1 .. x0+n-2) at the upper one */
+/* update_b_vpml_2d / 3d (os-emf-vpml.f03:1742-1861, 1865-...) after the E step: interface
+ * of E, then update_b_x1_2d (:2066-2181) / update_b_x1_3d (:2311-2508), x over
+ * (x0 .. x0+n-2) at the lower wall and (x0+1 .. x0+n-2) at the upper one */
 static void vpml_update_b(const orc_grid *g, orc_vpml *v, real *e) {
   vpml_interface(g, v, 0, e);
-  const int n = v->n, ny = g->nx[1];
-  const real dt2 = v->dtdx2;
+  const int n = v->n, ny = g->nx[1], D3 = g->dims > 2;
+  const int klo = D3 ? 1 - v->gc3 : 1, khi = D3 ? g->nx[2] + v->gc4 - 1 : 1;
+  const real d2 = v->dtdx2, d3 = v->dtdx3;
   for (int wl = 0; wl < 2; wl++) {
     const int x0 = vp_x0(g, v, wl);
     const int lo = wl == 0 ? x0 : x0 + 1, hi = x0 + n - 2;
     const real *cb = v->coef[wl][1][0];
-    for (int i2 = 1 - v->gc1; i2 <= ny + v->gc2 - 1; i2++)
-      for (int x = lo; x <= hi; x++) {
-        /* coef_std = x0 - 2 (lower), x0 (upper) */
-        const int ci = wl == 0 ? x - x0 + 1 : x - x0 - 1;
-        const real c1 = cb[ci], c2 = cb[ORC_VPML_MAX + ci], c3 = cb[2 * ORC_VPML_MAX + ci];
-        BW(0, x, i2) = BW(0, x, i2) - dt2 * (EW(2, x, i2 + 1) + EW(3, x, i2 + 1) - EW(2, x, i2) -
-                                             EW(3, x, i2));
-        BW(3, x, i2) = BW(3, x, i2) + dt2 * (EW(0, x, i2 + 1) - EW(0, x, i2));
-        BW(1, x, i2) = c1 * BW(1, x, i2) + c2 * (EW(2, x + 1, i2) + EW(3, x + 1, i2)) -
-                       c3 * (EW(2, x, i2) + EW(3, x, i2));
-        BW(2, x, i2) = c1 * BW(2, x, i2) - c2 * EW(1, x + 1, i2) + c3 * EW(1, x, i2);
-#undef EW
-#undef BW
-      }
+    for (int k = klo; k <= khi; k++)
+      for (int j = 1 - v->gc1; j <= ny + v->gc2 - 1; j++)
+        for (int x = lo; x <= hi; x++) {
+          /* coef_std = x0 - 2 (lower), x0 (upper) */
+          const int ci = wl == 0 ? x - x0 + 1 : x - x0 - 1;
+          const real c1 = cb[ci], c2 = cb[ORC_VPML_MAX + ci], c3 = cb[2 * ORC_VPML_MAX + ci];
+          if (!D3) {
+            BW(0, x, j, 1) = BW(0, x, j, 1) - d2 * (EW(2, x, j + 1, 1) + EW(3, x, j + 1, 1) -
+                                                    EW(2, x, j, 1) - EW(3, x, j, 1));
+            BW(3, x, j, 1) = BW(3, x, j, 1) + d2 * (EW(0, x, j + 1, 1) - EW(0, x, j, 1));
+            BW(1, x, j, 1) = c1 * BW(1, x, j, 1) + c2 * (EW(2, x + 1, j, 1) + EW(3, x + 1, j, 1)) -
+                             c3 * (EW(2, x, j, 1) + EW(3, x, j, 1));
+            BW(2, x, j, 1) = c1 * BW(2, x, j, 1) - c2 * EW(1, x + 1, j, 1) + c3 * EW(1, x, j, 1);
+            continue;
+          }
+          BW(0, x, j, k) = BW(0, x, j, k) - d2 * (EW(4, x, j + 1, k) + EW(5, x, j + 1, k) -
+                                                  EW(4, x, j, k) - EW(5, x, j, k));
+          BW(5, x, j, k) = BW(5, x, j, k) + d2 * (EW(0, x, j + 1, k) + EW(1, x, j + 1, k) -
+                                                  EW(0, x, j, k) - EW(1, x, j, k));
+          BW(1, x, j, k) = BW(1, x, j, k) + d3 * (EW(2, x, j, k + 1) + EW(3, x, j, k + 1) -
+                                                  EW(2, x, j, k) - EW(3, x, j, k));
+          BW(3, x, j, k) = BW(3, x, j, k) - d3 * (EW(0, x, j, k + 1) + EW(1, x, j, k + 1) -
+                                                  EW(0, x, j, k) - EW(1, x, j, k));
+          BW(2, x, j, k) = c1 * BW(2, x, j, k) + c2 * (EW(4, x + 1, j, k) + EW(5, x + 1, j, k)) -
+                           c3 * (EW(4, x, j, k) + EW(5, x, j, k));
+          BW(4, x, j, k) = c1 * BW(4, x, j, k) - c2 * (EW(2, x + 1, j, k) + EW(3, x + 1, j, k)) +
+                           c3 * (EW(2, x, j, k) + EW(3, x, j, k));
+        }
   }
 }
+#undef EW
+#undef BW
 
-/* the boundary update every step starts with, VPML walls: wall guards refreshed along y
- * (update_boundary_vpml, os-emf-vpml.f03:506-524), then the periodic refresh of y */
+/* the boundary update every step starts with, VPML walls: wall guards refreshed along y, z
+ * (update_boundary_vpml, os-emf-vpml.f03:506-524), then the periodic refresh of y (and z) */
 void orc_open_boundary_vpml(const orc_grid *g, orc_vpml *v, real *e, real *b) {
   vpml_periodic(g, v);
   update_periodic_from(g, e, 3, 0, 1);
@@ -1423,8 +1499,8 @@ void orc_open_boundary_vpml(const orc_grid *g, orc_vpml *v, real *e, real *b) {
 }
 
 /* advance_yee with VPML x walls: the wall E after the first B half step
- * (update_boundary_bfld step 1, os-emf-boundary.f90:930-935), the wall B after the E step
- * (update_boundary_efld, :1036-1041) */
+ * (update_boundary_bfld step 1, os-emf-boundary.f90:930-935, 961-965), the wall B after the
+ * E step (update_boundary_efld, :1036-1041) */
 void orc_yee_advance_vpml(const orc_grid *g, const double *dx, double dt, real *e, real *b,
                           const real *j, orc_vpml *v) {
   dbdt(g, dx, dt / 2.0, b, e);

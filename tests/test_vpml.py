@@ -74,7 +74,7 @@ def test_unattenuated_wall_is_the_continued_lattice(prec, plant):
     r = slice(w.c.gc1, w.c.gc1 + ny)
     errs = []
     for wl, ls, X0 in ((0, range(1, n - 1), G.gcl[0]), (1, range(1, n), G.gcl[0] + nx + n - 1)):
-        we = w.e[wl][r]
+        we = w.e[wl][0, r]
         for l in ls:
             got = np.stack([we[:, l, 0], we[:, l, 1], we[:, l, 2] + we[:, l, 3]], -1)
             errs.append(np.abs(got - big.e[y, X0 + l]).max() / scale)
@@ -166,6 +166,121 @@ def test_coefficients_of_the_documented_case():
 
 def test_vpml_rejects():
     with pytest.raises(ValueError):
-        O.Sim(3, (8, 8, 8), (0.2,) * 3, 0.05, 2, open_x=True, xwall="vpml")
-    with pytest.raises(ValueError):
         O.Sim(2, (16, 8), (0.2, 0.2), 0.1, 2, open_x=True, xwall="vpml", vpml_size=3)
+
+
+@pytest.mark.parametrize("plant", [None, "dt3_sign"])
+def test_unattenuated_wall_is_the_continued_lattice_3d(plant):
+    """3D: six split components; random fields with y and z structure."""
+    nx, ny, nz, n, steps = 20, 6, 5, 24, 20
+    dx, dt = (0.2, 0.25, 0.3), 0.08
+    rng = np.random.default_rng(5)
+    e = rng.normal(0, 1, (nz, ny, nx, 3))
+    b = rng.normal(0, 1, (nz, ny, nx, 3))
+    for f in (e, b):
+        f[:, :, :3] = 0
+        f[:, :, -3:] = 0
+    s = O.Sim(3, (nx, ny, nz), dx, dt, 2, open_x=True, xwall="vpml", vpml_size=n)
+    c = _coef_view(s)
+    c[:, :, 0, :] = 1.0
+    c[:, :, 1:, :] = dt / dx[0]
+    if plant == "dt3_sign":
+        s.walls.c.dtdx3 = -s.walls.c.dtdx3
+    s.set_interior("e", e)
+    s.set_interior("b", b)
+    W = nx + 2 * n - 1
+    big = O.Sim(3, (W, ny, nz), dx, dt, 2)
+    eb = np.zeros((nz, ny, W, 3))
+    bb = np.zeros_like(eb)
+    eb[:, :, n - 1:n - 1 + nx] = e
+    bb[:, :, n - 1:n - 1 + nx] = b
+    big.set_interior("e", eb)
+    big.set_interior("b", bb)
+    s.step(steps)
+    big.step(steps)
+    g, G = s.grid, big.grid
+    yz = (slice(g.gcl[2], g.gcl[2] + nz), slice(g.gcl[1], g.gcl[1] + ny))
+    sx = slice(g.gcl[0] - 1, g.gcl[0] + nx + 1)
+    X = slice(G.gcl[0] + n - 2, G.gcl[0] + n + nx)
+    scale = max(np.abs(big.e).max(), np.abs(big.b).max())
+    err_e = np.abs(s.e[yz + (sx,)] - big.e[yz + (X,)]).max() / scale
+    err_b = np.abs(s.b[yz + (sx,)] - big.b[yz + (X,)]).max() / scale
+    w = s.walls
+    r = (slice(w.c.gc3, w.c.gc3 + nz), slice(w.c.gc1, w.c.gc1 + ny))
+    errs = []
+    for wl, ls, X0 in ((0, range(1, n - 1), G.gcl[0]), (1, range(1, n), G.gcl[0] + nx + n - 1)):
+        we = w.e[wl][r]
+        for l in ls:
+            got = we[:, :, l, 0::2] + we[:, :, l, 1::2]
+            errs.append(np.abs(got - big.e[yz + (X0 + l,)]).max() / scale)
+    err_w = max(errs)
+    assert np.abs(big.e[yz + (slice(G.gcl[0], G.gcl[0] + n - 1),)]).max() > 1e-3 * scale
+    if plant:
+        assert max(err_e, err_b, err_w) > 1e-6
+        return
+    assert max(err_e, err_b, err_w) < 1e-12, (err_e, err_b, err_w)
+
+
+@pytest.mark.parametrize("direction", [1, -1])
+def test_vpml_3d_absorbs_oblique_incidence(direction):
+    """A packet along (kx, ky, kz) in 3D, both polarisations (E along z^ x k^ and along
+    k^ x (z^ x k^)); the reflection is measured as in 2D."""
+    nx, ny, nz, dx, dt, k0 = 80, 16, 16, 0.2, 0.1, 4.0
+    steps = int(1.1 * nx * dx / dt)
+
+    def run(xwall, plant=None):
+        s = O.Sim(3, (nx, ny, nz), (dx,) * 3, dt, 2, open_x=True, xwall=xwall)
+        if plant:
+            c = _coef_view(s)
+            c[..., :8] = c[..., :8][..., ::-1].copy()
+        ky, kz = 2 * np.pi / (ny * dx), 2 * np.pi / (nz * dx)
+        kx = np.sqrt(k0 ** 2 - ky ** 2 - kz ** 2) * direction
+        k = np.array([kx, ky, kz]) / k0
+        a1 = np.cross([0, 0, 1.0], k)
+        a1 /= np.linalg.norm(a1)
+        a2 = np.cross(k, a1)
+        x0 = nx * dx / 2
+        idx = [np.arange(1, m + 1) * dx for m in (nx, ny, nz)]
+        h = dx / 2
+
+        def f(sh):
+            Z, Y, Xc = np.meshgrid(idx[2] + sh[2], idx[1] + sh[1], idx[0] + sh[0], indexing="ij")
+            return np.exp(-((Xc - x0) / 2.0) ** 2) * np.cos(kx * (Xc - x0) + ky * Y + kz * Z)
+
+        # Yee points: E_c shifted by h along c, B_c by h along the other two
+        e = np.zeros((nz, ny, nx, 3))
+        b = np.zeros_like(e)
+        for cc in range(3):
+            se = [0, 0, 0]
+            se[cc] = h
+            sb = [h, h, h]
+            sb[cc] = 0
+            e[..., cc] = (a1[cc] + a2[cc]) * f(se)
+            bvec = np.cross(k, a1 + a2)
+            b[..., cc] = bvec[cc] * f(sb)
+        s.set_interior("e", e)
+        s.set_interior("b", b)
+        s.step(steps)
+        return s, e, b
+
+    s, e, b = run("vpml")
+    m = steps + 8
+    big = O.Sim(3, (nx + 2 * m, ny, nz), (dx,) * 3, dt, 2)
+    eb = np.zeros((nz, ny, nx + 2 * m, 3))
+    bb = np.zeros_like(eb)
+    eb[:, :, m:m + nx] = e
+    bb[:, :, m:m + nx] = b
+    big.set_interior("e", eb)
+    big.set_interior("b", bb)
+    big.step(steps)
+    e0 = (e ** 2).sum() + (b ** 2).sum()
+
+    def refl(sim):
+        de = O.Grid.interior(sim.grid, sim.e) - O.Grid.interior(big.grid, big.e)[:, :, m:m + nx]
+        db = O.Grid.interior(sim.grid, sim.b) - O.Grid.interior(big.grid, big.b)[:, :, m:m + nx]
+        return float(((de ** 2).sum() + (db ** 2).sum()) / e0)
+
+    got = refl(s)
+    lind = refl(run("lindman")[0])
+    assert got < 1e-5 and got < 1e-2 * lind, (got, lind)
+    assert refl(run("vpml", plant=True)[0]) > 100 * got
