@@ -150,8 +150,8 @@ typedef struct {
  * os-emf-boundary.f90:723-1112), 2D and 3D.  OSR_BND_VPML: the split-field perfectly
  * matched layer of vpml_size cells at both x ends (emf_bound type "vpml", which the
  * reference's deck also names "open", os-emf-boundary.f90:262-263; emf/boundary/
- * os-emf-vpml.f03: setup_vpml :270-474, setup_att_coeff :3586-3689, update_e_vpml_2d
- * :598-722, update_b_vpml_2d :1742-1861), 2D, without the optional diffusion
+ * os-emf-vpml.f03: setup_vpml :270-474, setup_att_coeff :3586-3689, update_e_vpml_2d/3d
+ * :598-856, update_b_vpml_2d/3d :1742-...), 2D and 3D, without the optional 2D diffusion
  * (vpml_diffuse = .false., the default).  Fused path, nx a multiple of the tile width, no
  * current smoothing; osr_sim_gauss is not available. */
 enum { OSR_BND_PERIODIC = 0, OSR_BND_OPEN = 1, OSR_BND_LINDMAN = 1, OSR_BND_VPML = 2 };

@@ -358,7 +358,8 @@ void make_field_desc(const osr_sim* s, FieldDesc& f) {
   if (s->g.open_nx) {  // new_lindman, emf/boundary/os-emf-lindman.f90:235-241
     f.wall = s->wall.as<float>();
     f.vp_n = s->vp_n;  // setup_vpml, emf/boundary/os-emf-vpml.f03:468-470
-    f.vp_dt2 = s->dims > 1 ? (float)(s->dt / s->dx[1]) : 0.f;
+    f.vp_dt2 = (float)(s->dt / s->dx[1]);
+    f.vp_dt3 = s->dims > 2 ? (float)(s->dt / s->dx[2]) : 0.f;
     f.wall_dc = (float)((1.0 - s->dt / s->dx[0]) / (1.0 + s->dt / s->dx[0]));
     for (int d = 0; d < s->dims; d++) f.wall_dt[d] = (float)(s->dt / s->dx[d]) * 0.5f;
   }
@@ -795,12 +796,20 @@ int launch_yee(osr_sim* s) {
   if (s->vp_n) {  // advance_yee with VPML x walls: update_boundary_bfld step 1 advances
                    // the wall E, update_boundary_efld the wall B (os-emf-boundary.f90:930-935,
                    // 1036-1041)
-    const unsigned nb = (unsigned)ceil_div(2 * s->vp_n * s->g.ny, 256);
-    hipLaunchKernelGGL((yee_dbdt<2, true>), dim3(s->nblk_e), dim3(256), 0, s->stream, fd, we);
-    hipLaunchKernelGGL(vpml_update<0>, dim3(nb), dim3(256), 0, s->stream, fd);
-    hipLaunchKernelGGL((yee_dedt<2>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
-    hipLaunchKernelGGL(vpml_update<1>, dim3(nb), dim3(256), 0, s->stream, fd);
-    hipLaunchKernelGGL((yee_dbdt<2, false>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
+    const unsigned nb = (unsigned)((2ll * s->vp_n * s->g.ny * s->g.nz + 255) / 256);
+    if (s->dims == 2) {
+      hipLaunchKernelGGL((yee_dbdt<2, true>), dim3(s->nblk_e), dim3(256), 0, s->stream, fd, we);
+      hipLaunchKernelGGL((vpml_update<0, 2>), dim3(nb), dim3(256), 0, s->stream, fd);
+      hipLaunchKernelGGL((yee_dedt<2>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
+      hipLaunchKernelGGL((vpml_update<1, 2>), dim3(nb), dim3(256), 0, s->stream, fd);
+      hipLaunchKernelGGL((yee_dbdt<2, false>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
+    } else {
+      hipLaunchKernelGGL((yee_dbdt<3, true>), dim3(s->nblk_e), dim3(256), 0, s->stream, fd, we);
+      hipLaunchKernelGGL((vpml_update<0, 3>), dim3(nb), dim3(256), 0, s->stream, fd);
+      hipLaunchKernelGGL((yee_dedt<3>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
+      hipLaunchKernelGGL((vpml_update<1, 3>), dim3(nb), dim3(256), 0, s->stream, fd);
+      hipLaunchKernelGGL((yee_dbdt<3, false>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
+    }
     HIPC(hipGetLastError());
     return OSR_OK;
   }
@@ -1033,7 +1042,6 @@ int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
     return fail(OSR_ERR_ARG, "unknown x boundary type");
   const bool open_x = desc->xbnd != OSR_BND_PERIODIC;
   const int vp_n = desc->xbnd == OSR_BND_VPML ? (desc->vpml_size > 0 ? desc->vpml_size : 8) : 0;
-  if (vp_n && dims != 2) return fail(OSR_ERR_ARG, "VPML x walls are implemented in 2D");
   if (vp_n && (vp_n < 4 || vp_n > 64))
     return fail(OSR_ERR_ARG, "vpml_size must be in 4..64 (setup_att_coeff divides by n - 3)");
   if (open_x && desc->path != OSR_PATH_FUSED)
@@ -1095,7 +1103,7 @@ int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
   s->vp_n = vp_n;
   if (open_x && vp_n) {
     // both walls zero at creation (vdf/os-wall.f90:189-205), coefficients behind them
-    const size_t nw = (size_t)16 * vp_n * g.ny;
+    const size_t nw = (size_t)4 * (dims == 3 ? 6 : 4) * vp_n * g.ny * g.nz;
     std::vector<float> coef(12 * (size_t)vp_n);
     vpml_coefs(vp_n, (float)(desc->dt / desc->dx[0]), coef.data());
     TRY(s->wall.alloc(sizeof(float) * (nw + coef.size())));

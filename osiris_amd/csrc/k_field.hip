@@ -52,12 +52,13 @@ struct FieldDesc {
   float* wall;
   float wall_dc;     // real((1 - dt/dx1) / (1 + dt/dx1))
   float wall_dt[3];  // real(dt/dx) * 0.5
-  // VPML x walls instead (vp_n > 0, 2D): split-field walls vp_n cells thick; wall holds
-  // [lower, upper][E, B][X, Y, ZX, ZY][cell l][row y] (cell l is x = 2 - vp_n + l at the lower
-  // wall, x = nx + 1 + l at the upper one), then the attenuation coefficients
+  // VPML x walls instead (vp_n > 0): split-field walls vp_n cells thick; wall holds
+  // [lower, upper][E, B][component][cell l][row y + ny z] (components X, Y, ZX, ZY in 2D,
+  // XY, XZ, YX, YZ, ZX, ZY in 3D; cell l is x = 2 - vp_n + l at the lower wall,
+  // x = nx + 1 + l at the upper one), then the attenuation coefficients
   // [lower, upper][E, B][1..3][cell]
   int vp_n;
-  float vp_dt2;      // real(dt/dx2)
+  float vp_dt2, vp_dt3;  // real(dt/dx2), real(dt/dx3)
   // laser walls: las_k launches per step, into ring columns las_col[k]; for table row
   // r = step - *las_first: las_mask[r][k] (0: the launch adds nothing) and the values
   // las_tab[r][k][2][ny] (B2, B3 increments of rows y = 1..ny)
@@ -340,97 +341,135 @@ __global__ __launch_bounds__(256) void wall_restore(const FieldDesc* __restrict_
 }
 
 // ------------------------------------------------------------------------------------
-// VPML absorbing x walls, 2D (emf/boundary/os-emf-vpml.f03), one thread per (wall, cell,
-// row).  EB = 0 runs after the first B half step (update_e_vpml_2d :598-722): the B
-// interface (update_interface_x1_2d :2880-2997: box B at box_to_vpml into the wall with
-// ZY = 0, wall B at vpml_to_box into the box with B3 = ZX + ZY), then the wall E update
-// (update_e_x1_2d :925-1048).  EB = 1 runs after the E step (update_b_vpml_2d
-// :1742-1861): the E interface, then the wall B update (update_b_x1_2d :2066-2181).
+// VPML absorbing x walls, 2D and 3D (emf/boundary/os-emf-vpml.f03), one thread per (wall,
+// cell, row).  EB = 0 runs after the first B half step (update_e_vpml_2d/3d :598-856):
+// the B interface (update_interface_x1_2d/3d :2880-3186: box B at box_to_vpml into the
+// wall, the whole component in the first of its split pair and zero in the second; wall
+// B at vpml_to_box into the box as the sum of each pair), then the wall E update
+// (update_e_x1_2d :925-1048, update_e_x1_3d :1172-1365).  EB = 1 runs after the E step
+// (update_b_vpml_2d/3d :1742-...): the E interface, then the wall B update
+// (update_b_x1_2d :2066-2181, update_b_x1_3d :2311-2508).
 // The rows are periodic: the reference's guard rows of the wall are refreshed from the
 // interior rows before every step, so its interior rows evolve as on a periodic wall.
 // The one wall cell a thread would read while the interface writes it (box_to_vpml,
 // the upper wall's first cell for E, the lower wall's last cell for B) is read from the
 // box field it is copied from.
 // ------------------------------------------------------------------------------------
+// split components per wall field: X, Y, ZX, ZY in 2D; XY, XZ, YX, YZ, ZX, ZY in 3D
+template <int DIMS>
 OSR_DEV float* vp_arr(const FieldDesc& fd, int wl, int eb, int c) {
-  return fd.wall + (size_t)((wl * 2 + eb) * 4 + c) * fd.vp_n * fd.g.ny;
+  constexpr int NC = DIMS == 3 ? 6 : 4;
+  return fd.wall + (size_t)((wl * 2 + eb) * NC + c) * fd.vp_n * fd.g.ny * fd.g.nz;
 }
 
-template <int EB>
+template <int EB, int DIMS>
 __global__ __launch_bounds__(256) void vpml_update(const FieldDesc* __restrict__ fds) {
+  constexpr int NC = DIMS == 3 ? 6 : 4;
   const FieldDesc& fd = *fds;
   const GridDesc& g = fd.g;
   if (g.ctrl->halt >= 2) return;
-  const int n = fd.vp_n, ny = g.ny, nx = g.nx;
+  const int n = fd.vp_n, ny = g.ny, nz = g.nz, nx = g.nx;
+  const long long R = (long long)ny * nz;
   const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (t >= 2LL * n * ny) return;
-  const int j = (int)(t % ny);
-  const int l = (int)((t / ny) % n);
-  const int wl = (int)(t / ((long long)ny * n));
+  if (t >= 2 * n * R) return;
+  const long long r = t % R;
+  const int l = (int)((t / R) % n);
+  const int wl = (int)(t / (R * n));
+  const int j = (int)(r % ny), k = (int)(r / ny);
   const long long nc = g.ncell;
   // ring columns of the wall's interface cells: x = 1 and 0 (lower), nx+1 and nx+2 (upper)
   const int btv = wl == 0 ? n - 1 : 0, vtb = wl == 0 ? n - 2 : 1;
-  const long long cb = (long long)j * nx + (wl == 0 ? 0 : g.open_nx);
-  const long long cv = (long long)j * nx + (wl == 0 ? nx - 1 : g.open_nx + 1);
-  const long long o = (long long)l * ny + j;
-  float* const F = EB == 0 ? g.B : g.E;     // the box field of the interface
-  float* W[4];                              // its wall counterpart
-  float* U[4];                              // the wall field updated
+  const long long cb = r * nx + (wl == 0 ? 0 : g.open_nx);
+  const long long cv = r * nx + (wl == 0 ? nx - 1 : g.open_nx + 1);
+  const long long o = (long long)l * R + r;
+  float* const F = EB == 0 ? g.B : g.E;  // the box field of the interface
+  float* W[NC];                          // its wall counterpart
+  float* U[NC];                          // the wall field updated
 #pragma unroll
-  for (int c = 0; c < 4; c++) {
-    W[c] = vp_arr(fd, wl, EB == 0 ? 1 : 0, c);
-    U[c] = vp_arr(fd, wl, EB == 0 ? 0 : 1, c);
+  for (int c = 0; c < NC; c++) {
+    W[c] = vp_arr<DIMS>(fd, wl, EB == 0 ? 1 : 0, c);
+    U[c] = vp_arr<DIMS>(fd, wl, EB == 0 ? 0 : 1, c);
   }
   if (l == btv) {
-    W[0][o] = F[cb];
-    W[1][o] = F[nc + cb];
-    W[2][o] = F[2 * nc + cb];
-    W[3][o] = 0.0f;
+    if (DIMS == 2) {
+      W[0][o] = F[cb];
+      W[1][o] = F[nc + cb];
+      W[2][o] = F[2 * nc + cb];
+      W[3][o] = 0.0f;
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        W[2 * c][o] = F[c * nc + cb];
+        W[2 * c + 1][o] = 0.0f;
+      }
+    }
   } else if (l == vtb) {
-    F[cv] = W[0][o];
-    F[nc + cv] = W[1][o];
-    F[2 * nc + cv] = W[2][o] + W[3][o];
+    if (DIMS == 2) {
+      F[cv] = W[0][o];
+      F[nc + cv] = W[1][o];
+      F[2 * nc + cv] = W[2][o] + W[3][o];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; c++) F[c * nc + cv] = W[2 * c][o] + W[2 * c + 1][o];
+    }
   }
-  const float dt2 = fd.vp_dt2;
-  const float* cf = fd.wall + (size_t)16 * n * ny + (size_t)(wl * 2 + EB) * 3 * n;
+  const float d2 = fd.vp_dt2, d3 = fd.vp_dt3;
+  const float* cf = fd.wall + (size_t)4 * NC * n * R + (size_t)(wl * 2 + EB) * 3 * n;
+  // rows y-1, y+1 (and z-1, z+1) of the same cell
+  const long long ojm = o + (j == 0 ? ny - 1 : -1), ojp = o + (j + 1 == ny ? 1 - ny : 1);
+  const long long okm = o + (k == 0 ? (long long)(nz - 1) * ny : -(long long)ny);
+  const long long okp = o + (k + 1 == nz ? -(long long)(nz - 1) * ny : (long long)ny);
   if (EB == 0) {
     if (l < 1 || l > (wl == 0 ? n - 2 : n - 1)) return;
     const int ci = wl == 0 ? l : l - 1;
     const float c1 = cf[ci], c2 = cf[n + ci], c3 = cf[2 * n + ci];
-    const long long om = (long long)l * ny + (j == 0 ? ny - 1 : j - 1);  // row y-1
-    const long long ol = o - ny;                                        // cell x-1
-    float bl1, bl23;
-    if (l - 1 == btv) {  // upper wall, first updated cell: x-1 = nx+1 from the box
-      bl1 = F[nc + cb];
-      bl23 = F[2 * nc + cb] + 0.0f;
+    const long long ol = o - R;  // cell x-1
+    const bool from_box = l - 1 == btv;  // upper wall, first updated cell: x-1 = nx+1
+    if (DIMS == 2) {
+      const float bl1 = from_box ? F[nc + cb] : W[1][ol];
+      const float bl23 = from_box ? F[2 * nc + cb] + 0.0f : W[2][ol] + W[3][ol];
+      const float b0 = W[0][o], b1 = W[1][o], b2 = W[2][o], b3 = W[3][o];
+      U[0][o] = U[0][o] + d2 * (b2 + b3 - W[2][ojm] - W[3][ojm]);
+      U[3][o] = U[3][o] - d2 * (b0 - W[0][ojm]);
+      U[1][o] = c1 * U[1][o] - c2 * (b2 + b3) + c3 * bl23;
+      U[2][o] = c1 * U[2][o] + c2 * b1 - c3 * bl1;
     } else {
-      bl1 = W[1][ol];
-      bl23 = W[2][ol] + W[3][ol];
+      const float blY = from_box ? F[nc + cb] + 0.0f : W[2][ol] + W[3][ol];
+      const float blZ = from_box ? F[2 * nc + cb] + 0.0f : W[4][ol] + W[5][ol];
+      const float bX = W[0][o] + W[1][o], bY = W[2][o] + W[3][o], bZ = W[4][o] + W[5][o];
+      U[0][o] = U[0][o] + d2 * (bZ - W[4][ojm] - W[5][ojm]);
+      U[5][o] = U[5][o] - d2 * (bX - W[0][ojm] - W[1][ojm]);
+      U[1][o] = U[1][o] - d3 * (bY - W[2][okm] - W[3][okm]);
+      U[3][o] = U[3][o] + d3 * (bX - W[0][okm] - W[1][okm]);
+      U[2][o] = c1 * U[2][o] - c2 * bZ + c3 * blZ;
+      U[4][o] = c1 * U[4][o] + c2 * bY - c3 * blY;
     }
-    const float b0 = W[0][o], b1 = W[1][o], b2 = W[2][o], b3 = W[3][o];
-    U[0][o] = U[0][o] + dt2 * (b2 + b3 - W[2][om] - W[3][om]);
-    U[3][o] = U[3][o] - dt2 * (b0 - W[0][om]);
-    U[1][o] = c1 * U[1][o] - c2 * (b2 + b3) + c3 * bl23;
-    U[2][o] = c1 * U[2][o] + c2 * b1 - c3 * bl1;
   } else {
     if (l < (wl == 0 ? 0 : 1) || l > n - 2) return;
     const int ci = wl == 0 ? l + 1 : l - 1;
     const float c1 = cf[ci], c2 = cf[n + ci], c3 = cf[2 * n + ci];
-    const long long op = (long long)l * ny + (j + 1 == ny ? 0 : j + 1);  // row y+1
-    const long long or1 = o + ny;                                       // cell x+1
-    float el1, el23;
-    if (l + 1 == btv) {  // lower wall, last updated cell: x+1 = 1 from the box
-      el1 = F[nc + cb];
-      el23 = F[2 * nc + cb] + 0.0f;
+    const long long or1 = o + R;  // cell x+1
+    const bool from_box = l + 1 == btv;  // lower wall, last updated cell: x+1 = 1
+    if (DIMS == 2) {
+      const float el1 = from_box ? F[nc + cb] : W[1][or1];
+      const float el23 = from_box ? F[2 * nc + cb] + 0.0f : W[2][or1] + W[3][or1];
+      const float e0 = W[0][o], e1 = W[1][o], e2 = W[2][o], e3 = W[3][o];
+      U[0][o] = U[0][o] - d2 * (W[2][ojp] + W[3][ojp] - e2 - e3);
+      U[3][o] = U[3][o] + d2 * (W[0][ojp] - e0);
+      U[1][o] = c1 * U[1][o] + c2 * el23 - c3 * (e2 + e3);
+      U[2][o] = c1 * U[2][o] - c2 * el1 + c3 * e1;
     } else {
-      el1 = W[1][or1];
-      el23 = W[2][or1] + W[3][or1];
+      const float elY = from_box ? F[nc + cb] + 0.0f : W[2][or1] + W[3][or1];
+      const float elZ = from_box ? F[2 * nc + cb] + 0.0f : W[4][or1] + W[5][or1];
+      const float e0 = W[0][o], e1 = W[1][o], e2 = W[2][o], e3 = W[3][o];
+      const float e4 = W[4][o], e5 = W[5][o];
+      U[0][o] = U[0][o] - d2 * (W[4][ojp] + W[5][ojp] - e4 - e5);
+      U[5][o] = U[5][o] + d2 * (W[0][ojp] + W[1][ojp] - e0 - e1);
+      U[1][o] = U[1][o] + d3 * (W[2][okp] + W[3][okp] - e2 - e3);
+      U[3][o] = U[3][o] - d3 * (W[0][okp] + W[1][okp] - e0 - e1);
+      U[2][o] = c1 * U[2][o] + c2 * elZ - c3 * (e4 + e5);
+      U[4][o] = c1 * U[4][o] - c2 * elY + c3 * (e2 + e3);
     }
-    const float e0 = W[0][o], e1 = W[1][o], e2 = W[2][o], e3 = W[3][o];
-    U[0][o] = U[0][o] - dt2 * (W[2][op] + W[3][op] - e2 - e3);
-    U[3][o] = U[3][o] + dt2 * (W[0][op] - e0);
-    U[1][o] = c1 * U[1][o] + c2 * el23 - c3 * (e2 + e3);
-    U[2][o] = c1 * U[2][o] - c2 * el1 + c3 * e1;
   }
 }
 
@@ -775,8 +814,10 @@ template __global__ void yee_dbdt<2, false>(const FieldDesc*, const int2*);
 template __global__ void yee_dbdt<3, true>(const FieldDesc*, const int2*);
 template __global__ void yee_dbdt<3, false>(const FieldDesc*, const int2*);
 template __global__ void yee_dedt<2>(const FieldDesc*, const int2*);
-template __global__ void vpml_update<0>(const FieldDesc*);
-template __global__ void vpml_update<1>(const FieldDesc*);
+template __global__ void vpml_update<0, 2>(const FieldDesc*);
+template __global__ void vpml_update<1, 2>(const FieldDesc*);
+template __global__ void vpml_update<0, 3>(const FieldDesc*);
+template __global__ void vpml_update<1, 3>(const FieldDesc*);
 template __global__ void yee_dedt<3>(const FieldDesc*, const int2*);
 template __global__ void rho_deposit<1>(const SpecDesc*, const int2*, unsigned long long*, double);
 template __global__ void rho_deposit<2>(const SpecDesc*, const int2*, unsigned long long*, double);

@@ -116,7 +116,7 @@ class Simulation:
     def __init__(self, dims, nx, dx, dt, order, tile=None, plain=False, kernel=0,
                  xbnd="periodic", reorder=0, vpml_size=0):
         """xbnd: "periodic"; "open" or "lindman" for Lindman absorbing field walls, "vpml"
-        for perfectly matched layers of vpml_size cells (0: 8; 2D), both with absorbing
+        for perfectly matched layers of vpml_size cells (0: 8), both with absorbing
         particles at both x ends (fused path, nx a multiple of the tile width)."""
         d = SimDesc()
         d.path = 1 if plain else 0

@@ -87,6 +87,38 @@ def oblique_packet(nx, ny, dx, k0=4.0, m=2, direction=1):
     return e, b
 
 
+def oblique_packet_3d(nx, ny, nz, dx, k0=4.0, direction=1):
+    """Interior E, B (shape (nz, ny, nx, 3)) of a light packet along (kx, ky, kz) with
+    ky = 2 pi / (ny dx), kz = 2 pi / (nz dx), |k| = k0, Gaussian envelope along x centred in
+    the box, E along a1 + a2 (a1 = z^ x k^ normalised, a2 = k^ x a1) and B = k^ x E, each
+    component sampled at its Yee point."""
+    ky, kz = 2 * np.pi / (ny * dx), 2 * np.pi / (nz * dx)
+    kx = np.sqrt(k0 ** 2 - ky ** 2 - kz ** 2) * direction
+    k = np.array([kx, ky, kz]) / k0
+    a1 = np.cross([0, 0, 1.0], k)
+    a1 /= np.linalg.norm(a1)
+    a2 = np.cross(k, a1)
+    x0 = nx * dx / 2
+    idx = [np.arange(1, m + 1) * dx for m in (nx, ny, nz)]
+    h = dx / 2
+
+    def f(sh):
+        Z, Y, X = np.meshgrid(idx[2] + sh[2], idx[1] + sh[1], idx[0] + sh[0], indexing="ij")
+        return np.exp(-((X - x0) / 2.0) ** 2) * np.cos(kx * (X - x0) + ky * Y + kz * Z)
+
+    e = np.zeros((nz, ny, nx, 3))
+    b = np.zeros_like(e)
+    bvec = np.cross(k, a1 + a2)
+    for c in range(3):
+        se = [0.0, 0.0, 0.0]
+        se[c] = h
+        sb = [h, h, h]
+        sb[c] = 0.0
+        e[..., c] = (a1[c] + a2[c]) * f(se)
+        b[..., c] = bvec[c] * f(sb)
+    return e, b
+
+
 def mur_reflection(kdx, courant):
     """Reflection coefficient of the first-order Mur wall E_0^{n+1} = E_1^n +
     c (E_0^n - E_1^{n+1}), c = (1 - S)/(1 + S), on the 1D Yee lattice (waves along x on

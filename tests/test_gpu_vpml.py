@@ -1,4 +1,5 @@
-"""GPU parity of VPML absorbing x walls (emf/boundary/os-emf-vpml.f03, 2D, y periodic)
+"""GPU parity of VPML absorbing x walls (emf/boundary/os-emf-vpml.f03, 2D and 3D, y and z
+periodic)
 with absorbing particles (spec/os-spec-boundary.f03:476-513) in the resident path.
 
 * Fields only: light packets at normal and oblique incidence (every field component and
@@ -6,6 +7,8 @@ with absorbing particles (spec/os-spec-boundary.f03:476-513) in the resident pat
   FP32 oracle bit for bit on every interior and guard cell, for thicknesses 8 and 16 and
   interpolation orders 1..3 (other guard bands), and the field energies agree to the
   FP32 rounding of the oracle's squares.
+* 3D: an oblique packet (all six split components in play), orders 2 and 3, thickness 8
+  and 12: bit-exact fields.
 * A laser launched from the lower wall into a VPML-walled box: bit-exact fields.
 * Plasma: against the FP64 oracle, particle counts to the few particles that leave
   within FP32 rounding of a wall, energy histories to 1e-4 plus their share.
@@ -16,7 +19,7 @@ import pytest
 import osiris_amd as oa
 from oracle import oracle as O
 from osiris_amd._lib import OsrError
-from tests.pic_inputs import oblique_packet, thermal_plasma, vacuum_pulse
+from tests.pic_inputs import oblique_packet, oblique_packet_3d, thermal_plasma, vacuum_pulse
 
 pytestmark = pytest.mark.gpu
 
@@ -101,8 +104,56 @@ def test_vpml_plasma_vs_fp64_oracle():
     assert np.max(np.abs(fld - rfld) / fscale) <= tol
 
 
+@pytest.mark.parametrize("direction,order,n", [(1, 2, 8), (-1, 3, 12)])
+def test_vpml_3d_fields_bitexact_vs_fp32_oracle(direction, order, n):
+    print(f"case 3D direction={direction} order={order} n={n}", flush=True)
+    nx, ny, nz, dx, dt, steps = 48, 16, 16, 0.2, 0.1, 120
+    e, b = oblique_packet_3d(nx, ny, nz, dx, direction=direction)
+    e, b = e.astype(np.float32), b.astype(np.float32)
+    ref = O.Sim(3, (nx, ny, nz), (dx,) * 3, dt, order, prec="f32", open_x=True, xwall="vpml",
+                vpml_size=n)
+    g = ref.grid
+    with oa.Simulation(3, (nx, ny, nz), (dx,) * 3, dt, order, tile=(8, 8, 4), xbnd="vpml",
+                       vpml_size=n) as sim:
+        sim.set_fields(e, b)
+        ref.set_interior("e", e)
+        ref.set_interior("b", b)
+        sim.run(steps)
+        ref.step(steps)
+        ref.wall_refresh()
+        ge, gb = sim.get_fields(gcl=g.gcl, gcu=g.gcu)
+        fld, _ = sim.history()
+    assert np.array_equal(ge.view(np.uint32), ref.e.view(np.uint32))
+    assert np.array_equal(gb.view(np.uint32), ref.b.view(np.uint32))
+    assert np.allclose(fld, np.array(ref.fld_hist), rtol=2e-7, atol=0)
+    assert fld[-1].sum() < 0.2 * fld[0].sum()
+
+
+def test_vpml_3d_plasma_vs_fp64_oracle():
+    rng = np.random.default_rng(73)
+    nx, dx, dt, steps = (32, 16, 8), (0.1, 0.1, 0.1), 0.05, 30
+    ix, x, u, q = thermal_plasma(rng, 3, nx, 4, 0.2)
+    ref = O.Sim(3, nx, dx, dt, 2, open_x=True, xwall="vpml")
+    ref.add_species(-1.0, 0, ix, x, u, q)
+    with oa.Simulation(3, nx, dx, dt, 2, tile=(8, 8, 4), xbnd="vpml") as sim:
+        sid = sim.add_species(-1.0)
+        sim.set_particles(sid, ix, x, u, q)
+        sim.run(steps)
+        ref.step(steps)
+        n_gpu = sim.count(sid)
+        fld, par = sim.history()
+    n0, n_ref = len(q), len(ref.species[0].q)
+    print(f"particles {n0} -> gpu {n_gpu}, oracle {n_ref}", flush=True)
+    assert n_ref < n0 - 100
+    dn = abs(n_gpu - n_ref)
+    assert dn <= 3
+    tol = 1e-4 + 2.0 * dn / n_ref
+    rfld, rpar = np.array(ref.fld_hist), np.array(ref.par_hist)
+    assert np.max(np.abs(par - rpar) / np.abs(rpar)) <= tol
+    fscale = np.maximum(rfld.sum(1, keepdims=True), 1e-300)
+    assert np.max(np.abs(fld - rfld) / fscale) <= tol
+
+
 def test_vpml_rejects():
-    with pytest.raises(OsrError):
-        oa.Simulation(3, (16, 8, 8), (0.2,) * 3, 0.05, 2, xbnd="vpml")
     with pytest.raises(OsrError):
         oa.Simulation(2, (32, 16), (0.2, 0.2), 0.1, 2, xbnd="vpml", vpml_size=3)

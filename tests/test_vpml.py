@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from tests.pic_inputs import oblique_packet, vacuum_pulse
+from tests.pic_inputs import oblique_packet, oblique_packet_3d, vacuum_pulse
 
 
 def _coef_view(s):
@@ -233,31 +233,7 @@ def test_vpml_3d_absorbs_oblique_incidence(direction):
         if plant:
             c = _coef_view(s)
             c[..., :8] = c[..., :8][..., ::-1].copy()
-        ky, kz = 2 * np.pi / (ny * dx), 2 * np.pi / (nz * dx)
-        kx = np.sqrt(k0 ** 2 - ky ** 2 - kz ** 2) * direction
-        k = np.array([kx, ky, kz]) / k0
-        a1 = np.cross([0, 0, 1.0], k)
-        a1 /= np.linalg.norm(a1)
-        a2 = np.cross(k, a1)
-        x0 = nx * dx / 2
-        idx = [np.arange(1, m + 1) * dx for m in (nx, ny, nz)]
-        h = dx / 2
-
-        def f(sh):
-            Z, Y, Xc = np.meshgrid(idx[2] + sh[2], idx[1] + sh[1], idx[0] + sh[0], indexing="ij")
-            return np.exp(-((Xc - x0) / 2.0) ** 2) * np.cos(kx * (Xc - x0) + ky * Y + kz * Z)
-
-        # Yee points: E_c shifted by h along c, B_c by h along the other two
-        e = np.zeros((nz, ny, nx, 3))
-        b = np.zeros_like(e)
-        for cc in range(3):
-            se = [0, 0, 0]
-            se[cc] = h
-            sb = [h, h, h]
-            sb[cc] = 0
-            e[..., cc] = (a1[cc] + a2[cc]) * f(se)
-            bvec = np.cross(k, a1 + a2)
-            b[..., cc] = bvec[cc] * f(sb)
+        e, b = oblique_packet_3d(nx, ny, nz, dx, k0, direction)
         s.set_interior("e", e)
         s.set_interior("b", b)
         s.step(steps)
