@@ -5,7 +5,8 @@ spec/os-spec-initfields.f90, es_solver_2d, emf/os-emf-es-solver.f03:125-236).
   bit: the density the device deposits (exact fixed-point sum, read back through
   osr_sim_gauss) is fed to the oracle's solver.
 * Against the FP64 oracle end to end (its own FP64 deposit): 1e-5 of max |E|, periodic
-  and open x walls; and 20-30 steps after the set-up, energies to 1e-4.
+  and open x walls (Lindman or VPML; the reference's update_beam leaves x walls as they
+  are, os-emf-vpml.f03:3693-3750); and 20-30 steps after the set-up, energies to 1e-4.
 * The same for the beam branch (es_solver_2d_beam, a species drifting along x), E and B.
 """
 import ctypes as C
@@ -49,12 +50,13 @@ def test_es_field_bitexact_given_the_charge():
     assert np.abs(want[..., :2]).max() > 0
 
 
-@pytest.mark.parametrize("xbnd", ["periodic", "open"])
+@pytest.mark.parametrize("xbnd", ["periodic", "open", "vpml"])
 def test_init_fields_vs_fp64_oracle(xbnd):
     rng = np.random.default_rng(4)
     nx, dx, dt, steps = (64, 32), (0.1, 0.1), 0.05, 30
     ix, x, u, q = _plasma(rng, nx)
-    ref = O.Sim(2, nx, dx, dt, 2, open_x=(xbnd == "open"))
+    ref = O.Sim(2, nx, dx, dt, 2, open_x=(xbnd != "periodic"),
+                xwall="vpml" if xbnd == "vpml" else "lindman")
     ref.add_species(-1.0, 0, ix, x, u, q)
     ref.init_fields(0)
     with oa.Simulation(2, nx, dx, dt, 2, tile=(16, 16), xbnd=xbnd) as sim:
@@ -101,14 +103,15 @@ def test_beam_field_bitexact_given_the_charge():
     assert np.abs(wb[..., 2]).max() > 0
 
 
-@pytest.mark.parametrize("xbnd", ["periodic", "open"])
+@pytest.mark.parametrize("xbnd", ["periodic", "open", "vpml"])
 def test_beam_init_fields_vs_fp64_oracle(xbnd):
     rng = np.random.default_rng(6)
     nx, dx, dt, steps, u1 = (64, 32), (0.1, 0.1), 0.05, 20, 0.6
     ix, x, u, q = _plasma(rng, nx)
     u = u.copy()
     u[:, 0] += u1
-    ref = O.Sim(2, nx, dx, dt, 2, open_x=(xbnd == "open"))
+    ref = O.Sim(2, nx, dx, dt, 2, open_x=(xbnd != "periodic"),
+                xwall="vpml" if xbnd == "vpml" else "lindman")
     ref.add_species(-1.0, 0, ix, x, u, q)
     ref.init_fields(0, u1)
     with oa.Simulation(2, nx, dx, dt, 2, tile=(16, 16), xbnd=xbnd) as sim:

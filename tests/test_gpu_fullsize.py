@@ -68,18 +68,19 @@ def test_full_size_invariants(name, make, tiles, steps):
     assert np.allclose(p0, p2, rtol=1e-12, atol=0)
 
 
-def test_laser_plasma_2048x512_full_size():
+@pytest.mark.parametrize("xbnd", ["lindman", "vpml"])
+def test_laser_plasma_2048x512_full_size(xbnd):
     """The laser-plasma workload at full size (2048 x 512 cells, 8 M cold electrons behind a
-    ramp, a0 = 2 laser from the lower x wall, Lindman walls, absorbing particles), 300
-    steps: a repeated run is bit-identical, and another tile shape (guard band and mover
-    traffic differ) gives identical fields, surviving particles and counts."""
+    ramp, a0 = 2 laser from the lower x wall, Lindman or VPML walls, absorbing particles),
+    300 steps: a repeated run is bit-identical, and another tile shape (guard band and
+    mover traffic differ) gives identical fields, surviving particles and counts."""
     from bench import C4, c4_particles
 
     ix, x, u, q = c4_particles((2048, 512), 100, 100, C4["ppc"], C4["density"])
     out = []
     for tile in ((32, 16), (32, 16), (16, 16)):
-        print(f"case c4 tile={tile}", flush=True)
-        with oa.Simulation(2, (2048, 512), C4["dx"], C4["dt"], 2, tile=tile, xbnd="open") as s:
+        print(f"case c4 xbnd={xbnd} tile={tile}", flush=True)
+        with oa.Simulation(2, (2048, 512), C4["dx"], C4["dt"], 2, tile=tile, xbnd=xbnd) as s:
             sid = s.add_species(-1.0)
             s.set_particles(sid, ix, x, u, q)
             s.add_zpulse_wall(**C4["laser"])
