@@ -140,6 +140,10 @@ typedef struct {
                      sort_species, spec/os-species.f03:269-332) */
   int vpml_size;  /* OSR_BND_VPML: wall thickness in cells, 4..64; 0 = 8 (vpml_bnd_size,
                      p_vpml_size_default, emf/boundary/os-emf-boundary.f90:30, 230) */
+  int vpml_diffuse[2]; /* OSR_BND_VPML in 2D, orders 2 and 3: nonzero mixes the wall's
+                     split B3 pair before each wall E update, lower / upper x wall
+                     (vpml_diffuse(:, 1), default .false., os-emf-boundary.f90:231;
+                     os-emf-vpml.f03:977-1006) */
 } osr_sim_desc;
 
 /* x boundaries of a resident simulation.  Both open types have absorbing particles
@@ -151,8 +155,8 @@ typedef struct {
  * matched layer of vpml_size cells at both x ends (emf_bound type "vpml", which the
  * reference's deck also names "open", os-emf-boundary.f90:262-263; emf/boundary/
  * os-emf-vpml.f03: setup_vpml :270-474, setup_att_coeff :3586-3689, update_e_vpml_2d/3d
- * :598-856, update_b_vpml_2d/3d :1742-...), 2D and 3D, without the optional 2D diffusion
- * (vpml_diffuse = .false., the default).  Fused path, nx a multiple of the tile width, no
+ * :598-856, update_b_vpml_2d/3d :1742-...), 2D and 3D, with the optional 2D diffusion
+ * (osr_sim_desc.vpml_diffuse).  Fused path, nx a multiple of the tile width, no
  * current smoothing; osr_sim_gauss is not available. */
 enum { OSR_BND_PERIODIC = 0, OSR_BND_OPEN = 1, OSR_BND_LINDMAN = 1, OSR_BND_VPML = 2 };
 

@@ -120,7 +120,7 @@ def _vpml_struct(real):
 
     class Vpml(C.Structure):
         _fields_ = [("n", C.c_int), ("gc1", C.c_int), ("gc2", C.c_int), ("gc3", C.c_int),
-                    ("gc4", C.c_int), ("dtdx2", R), ("dtdx3", R),
+                    ("gc4", C.c_int), ("diffuse", C.c_int * 2), ("dtdx2", R), ("dtdx3", R),
                     ("coef", R * VPML_MAX * 3 * 2 * 2), ("w", C.c_void_p * 2 * 2)]
     return Vpml
 
@@ -133,12 +133,15 @@ class VpmlWalls:
     components) (wl 0 lower, 1 upper; 4 split components in 2D, 6 in 3D; one z row in
     2D)."""
 
-    def __init__(self, grid, dx, dt, real, n=8):
+    def __init__(self, grid, dx, dt, real, n=8, diffuse=(False, False)):
         if not 4 <= n <= VPML_MAX:
             raise ValueError(f"vpml_bnd_size must be in 4..{VPML_MAX}")
         S = _vpml_struct(real)
         self.c = S()
         self.c.n = n
+        if any(diffuse) and grid.dims != 2:
+            raise ValueError("vpml_diffuse is a 2D option")
+        self.c.diffuse[0], self.c.diffuse[1] = int(diffuse[0]), int(diffuse[1])
         self.c.gc1 = max(grid.gcl[1], 1)
         self.c.gc2 = max(grid.gcu[1], 2)
         self.c.dtdx2 = real(dt / dx[1])
@@ -533,6 +536,7 @@ class Sim:
     open_x: bool = False
     xwall: str = "lindman"
     vpml_size: int = 8
+    vpml_diffuse: tuple = (False, False)  # lower, upper x wall
 
     def __post_init__(self):
         self.grid = make_grid(self.dims, self.nx, self.order)
@@ -555,7 +559,8 @@ class Sim:
             if self.smooth_order is not None:
                 raise ValueError("open x walls are implemented without smoothing")
             if self.xwall == "vpml":
-                self.walls = VpmlWalls(self.grid, self.dx, self.dt, r, self.vpml_size)
+                self.walls = VpmlWalls(self.grid, self.dx, self.dt, r, self.vpml_size,
+                                       self.vpml_diffuse)
             elif self.xwall == "lindman":
                 self.walls = LindmanWalls(self.grid, self.dx, self.dt, r)
             else:

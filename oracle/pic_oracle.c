@@ -1252,7 +1252,8 @@ int orc_particles_open(const orc_grid *g, int *ix, real *x, real *u, real *q, in
 
 /* ------------------------------------------------------------------------------------
  * VPML absorbing x walls, 2D and 3D (emf/boundary/os-emf-vpml.f03; emf_bound type "vpml"
- * or "open", os-emf-boundary.f90:262-263), y (and z) periodic, no corners, no diffusion.
+ * or "open", os-emf-boundary.f90:262-263), y (and z) periodic, no corners; in 2D with the
+ * optional diffusion of the split B3 (vpml_diffuse, p_difrate = 0.1, :51, 977-1006).
  *
  * Each wall holds split E and B fields (os-emf-vpml.f03:440-452): in 2D the components
  * 1 = X, 2 = Y, 3 = ZX, 4 = ZY; in 3D 1 = XY, 2 = XZ, 3 = YX, 4 = YZ, 5 = ZX, 6 = ZY.  The
@@ -1267,6 +1268,7 @@ int orc_particles_open(const orc_grid *g, int *ix, real *x, real *u, real *q, in
 typedef struct {
   int n;                              /* vpml_bnd_size */
   int gc1, gc2, gc3, gc4;             /* wall guard rows: y lower, upper, z lower, upper */
+  int diffuse[2];                     /* vpml_diffuse of the lower, upper wall (2D only) */
   real dtdx2, dtdx3;                  /* real(dt/dx2), real(dt/dx3) */
   real coef[2][2][3][ORC_VPML_MAX];   /* [lower, upper][E, B][coefficient][cell] */
   real *w[2][2];                      /* [lower, upper][E, B] wall fields */
@@ -1421,6 +1423,11 @@ static void vpml_update_e(const orc_grid *g, orc_vpml *v, real *b) {
           const int ci = wl == 0 ? x - x0 : x - x0 - 1;
           const real c1 = ce[ci], c2 = ce[ORC_VPML_MAX + ci], c3 = ce[2 * ORC_VPML_MAX + ci];
           if (!D3) {
+            if (v->diffuse[wl]) {  /* before the cell's update, in loop order (:1001-1006) */
+              const real rate = R(0.1), keep = R(1) - rate, zx = BW(2, x, j, 1);
+              BW(2, x, j, 1) = keep * BW(2, x, j, 1) + rate * BW(3, x, j, 1);
+              BW(3, x, j, 1) = keep * BW(3, x, j, 1) + rate * zx;
+            }
             EW(0, x, j, 1) = EW(0, x, j, 1) + d2 * (BW(2, x, j, 1) + BW(3, x, j, 1) -
                                                     BW(2, x, j - 1, 1) - BW(3, x, j - 1, 1));
             EW(3, x, j, 1) = EW(3, x, j, 1) - d2 * (BW(0, x, j, 1) - BW(0, x, j - 1, 1));

@@ -104,7 +104,8 @@ class SimDesc(C.Structure):
     _fields_ = [("dims", C.c_int), ("nx", C.c_int * 3), ("dx", C.c_double * 3),
                 ("dt", C.c_double), ("order", C.c_int), ("tile", C.c_int * 3),
                 ("path", C.c_int), ("kernel", C.c_int), ("xbnd", C.c_int),
-                ("reorder", C.c_int), ("vpml_size", C.c_int)]
+                ("reorder", C.c_int), ("vpml_size", C.c_int),
+                ("vpml_diffuse", C.c_int * 2)]
 
 
 class SmoothParams(C.Structure):

@@ -360,6 +360,8 @@ void make_field_desc(const osr_sim* s, FieldDesc& f) {
     f.vp_n = s->vp_n;  // setup_vpml, emf/boundary/os-emf-vpml.f03:468-470
     f.vp_dt2 = (float)(s->dt / s->dx[1]);
     f.vp_dt3 = s->dims > 2 ? (float)(s->dt / s->dx[2]) : 0.f;
+    f.vp_diff[0] = s->desc.vpml_diffuse[0];
+    f.vp_diff[1] = s->desc.vpml_diffuse[1];
     f.wall_dc = (float)((1.0 - s->dt / s->dx[0]) / (1.0 + s->dt / s->dx[0]));
     for (int d = 0; d < s->dims; d++) f.wall_dt[d] = (float)(s->dt / s->dx[d]) * 0.5f;
   }
@@ -799,6 +801,8 @@ int launch_yee(osr_sim* s) {
     const unsigned nb = (unsigned)((2ll * s->vp_n * s->g.ny * s->g.nz + 255) / 256);
     if (s->dims == 2) {
       hipLaunchKernelGGL((yee_dbdt<2, true>), dim3(s->nblk_e), dim3(256), 0, s->stream, fd, we);
+      if (s->desc.vpml_diffuse[0] || s->desc.vpml_diffuse[1])
+        hipLaunchKernelGGL(vpml_diffuse, dim3(nb), dim3(256), 0, s->stream, fd);
       hipLaunchKernelGGL((vpml_update<0, 2>), dim3(nb), dim3(256), 0, s->stream, fd);
       hipLaunchKernelGGL((yee_dedt<2>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
       hipLaunchKernelGGL((vpml_update<1, 2>), dim3(nb), dim3(256), 0, s->stream, fd);
@@ -1042,6 +1046,14 @@ int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
     return fail(OSR_ERR_ARG, "unknown x boundary type");
   const bool open_x = desc->xbnd != OSR_BND_PERIODIC;
   const int vp_n = desc->xbnd == OSR_BND_VPML ? (desc->vpml_size > 0 ? desc->vpml_size : 8) : 0;
+  if ((desc->vpml_diffuse[0] || desc->vpml_diffuse[1]) && (!vp_n || dims != 2))
+    return fail(OSR_ERR_ARG, "vpml_diffuse needs 2D VPML x walls");
+  // the wall's lower guard rows are max(field gc, 1) (vdf/os-wall.f90:190): one with linear
+  // interpolation (get_min_gc_part, particles/os-particles.f03:321-360), and then the
+  // reference's mixing skips row 0, so that its wall rows 1 and ny+1 part and the box
+  // reads the guard one
+  if ((desc->vpml_diffuse[0] || desc->vpml_diffuse[1]) && desc->order == 1)
+    return fail(OSR_ERR_ARG, "vpml_diffuse is built for interpolation orders 2 and 3");
   if (vp_n && (vp_n < 4 || vp_n > 64))
     return fail(OSR_ERR_ARG, "vpml_size must be in 4..64 (setup_att_coeff divides by n - 3)");
   if (open_x && desc->path != OSR_PATH_FUSED)

@@ -59,6 +59,7 @@ struct FieldDesc {
   // [lower, upper][E, B][1..3][cell]
   int vp_n;
   float vp_dt2, vp_dt3;  // real(dt/dx2), real(dt/dx3)
+  int vp_diff[2];  // 2D vpml_diffuse per wall (lower, upper)
   // laser walls: las_k launches per step, into ring columns las_col[k]; for table row
   // r = step - *las_first: las_mask[r][k] (0: the launch adds nothing) and the values
   // las_tab[r][k][2][ny] (B2, B3 increments of rows y = 1..ny)
@@ -362,6 +363,41 @@ OSR_DEV float* vp_arr(const FieldDesc& fd, int wl, int eb, int c) {
   return fd.wall + (size_t)((wl * 2 + eb) * NC + c) * fd.vp_n * fd.g.ny * fd.g.nz;
 }
 
+// vpml_diffuse (2D, update_e_x1_2d :977-1006): before the wall E update each cell of its
+// range mixes the split B3 pair, ZX' = 0.9 ZX + 0.1 ZY, ZY' = 0.9 ZY + 0.1 ZX; a separate
+// pass, as every read of the E update sees mixed values (cells earlier in the reference's
+// loop order, or the cell itself).  This holds for interpolation orders 2 and 3, whose
+// walls have at least two lower guard rows; with one (order 1) the reference leaves row 0
+// unmixed and its rows 1 and ny+1 part, which the library refuses.  The B interface runs
+// before the mixing in the reference, so this pass also copies the vpml_to_box cell
+// (inside the mixed range) into the box, from the values before mixing.
+__global__ __launch_bounds__(256) void vpml_diffuse(const FieldDesc* __restrict__ fds) {
+  const FieldDesc& fd = *fds;
+  const GridDesc& g = fd.g;
+  if (g.ctrl->halt >= 2) return;
+  const int n = fd.vp_n, ny = g.ny;
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= 2LL * n * ny) return;
+  const int j = (int)(t % ny);
+  const int l = (int)((t / ny) % n);
+  const int wl = (int)(t / ((long long)ny * n));
+  if (!fd.vp_diff[wl] || l < 1 || l > (wl == 0 ? n - 2 : n - 1)) return;
+  const long long o = (long long)l * ny + j;
+  float* zx = vp_arr<2>(fd, wl, 1, 2);
+  float* zy = vp_arr<2>(fd, wl, 1, 3);
+  const float a = zx[o], b = zy[o];
+  if (l == (wl == 0 ? n - 2 : 1)) {  // vpml_to_box: x = 0 or nx+2
+    const long long nc = g.ncell;
+    const long long cv = (long long)j * g.nx + (wl == 0 ? g.nx - 1 : g.open_nx + 1);
+    g.B[cv] = vp_arr<2>(fd, wl, 1, 0)[o];
+    g.B[nc + cv] = vp_arr<2>(fd, wl, 1, 1)[o];
+    g.B[2 * nc + cv] = a + b;
+  }
+  const float rate = 0.1f, keep = 1.0f - rate;
+  zx[o] = keep * a + rate * b;
+  zy[o] = keep * b + rate * a;
+}
+
 template <int EB, int DIMS>
 __global__ __launch_bounds__(256) void vpml_update(const FieldDesc* __restrict__ fds) {
   constexpr int NC = DIMS == 3 ? 6 : 4;
@@ -403,7 +439,7 @@ __global__ __launch_bounds__(256) void vpml_update(const FieldDesc* __restrict__
         W[2 * c + 1][o] = 0.0f;
       }
     }
-  } else if (l == vtb) {
+  } else if (l == vtb && !(DIMS == 2 && EB == 0 && fd.vp_diff[wl])) {  // else vpml_diffuse
     if (DIMS == 2) {
       F[cv] = W[0][o];
       F[nc + cv] = W[1][o];

@@ -38,6 +38,7 @@ module m_osiris_amd
     integer(c_int) :: xbnd = OSR_BND_PERIODIC
     integer(c_int) :: reorder = 0
     integer(c_int) :: vpml_size = 0
+    integer(c_int) :: vpml_diffuse(2) = 0
   end type osr_sim_desc
 
   ! current smoothing (vdf/os-vdf-smooth.f90): the reference's filter type codes and the

@@ -114,9 +114,10 @@ class Simulation:
     """A resident simulation on the GPU (one per grid; species share the grid)."""
 
     def __init__(self, dims, nx, dx, dt, order, tile=None, plain=False, kernel=0,
-                 xbnd="periodic", reorder=0, vpml_size=0):
+                 xbnd="periodic", reorder=0, vpml_size=0, vpml_diffuse=(False, False)):
         """xbnd: "periodic"; "open" or "lindman" for Lindman absorbing field walls, "vpml"
-        for perfectly matched layers of vpml_size cells (0: 8), both with absorbing
+        for perfectly matched layers of vpml_size cells (0: 8; vpml_diffuse: lower, upper
+        wall mix the split B3 pair, 2D), both with absorbing
         particles at both x ends (fused path, nx a multiple of the tile width)."""
         d = SimDesc()
         d.path = 1 if plain else 0
@@ -124,6 +125,7 @@ class Simulation:
         d.xbnd = {"periodic": OSR_BND_PERIODIC, "open": OSR_BND_OPEN,
                   "lindman": OSR_BND_LINDMAN, "vpml": OSR_BND_VPML}[xbnd]
         d.vpml_size = vpml_size
+        d.vpml_diffuse[0], d.vpml_diffuse[1] = int(vpml_diffuse[0]), int(vpml_diffuse[1])
         d.reorder = reorder  # in-tile particle order restored every `reorder` steps
         d.dims = dims
         for i in range(3):

@@ -9,6 +9,7 @@ with absorbing particles (spec/os-spec-boundary.f03:476-513) in the resident pat
   FP32 rounding of the oracle's squares.
 * 3D: an oblique packet (all six split components in play), orders 2 and 3, thickness 8
   and 12: bit-exact fields.
+* vpml_diffuse (2D): bit-exact fields for orders 2 and 3.
 * A laser launched from the lower wall into a VPML-walled box: bit-exact fields.
 * Plasma: against the FP64 oracle, particle counts to the few particles that leave
   within FP32 rounding of a wall, energy histories to 1e-4 plus their share.
@@ -24,13 +25,14 @@ from tests.pic_inputs import oblique_packet, oblique_packet_3d, thermal_plasma, 
 pytestmark = pytest.mark.gpu
 
 
-def _fields_case(e, b, nx, ny, dx, dt, steps, order=2, n=8, tile=(16, 16), zpulse=None):
+def _fields_case(e, b, nx, ny, dx, dt, steps, order=2, n=8, tile=(16, 16), zpulse=None,
+                 diffuse=(False, False)):
     e, b = e.astype(np.float32), b.astype(np.float32)
     ref = O.Sim(2, (nx, ny), (dx, dx), dt, order, prec="f32", open_x=True, xwall="vpml",
-                vpml_size=n)
+                vpml_size=n, vpml_diffuse=diffuse)
     g = ref.grid
     with oa.Simulation(2, (nx, ny), (dx, dx), dt, order, tile=tile, xbnd="vpml",
-                       vpml_size=n) as sim:
+                       vpml_size=n, vpml_diffuse=diffuse) as sim:
         if zpulse:
             sim.add_zpulse_wall(**zpulse)
             ref.add_zpulse_wall(**zpulse)
@@ -67,6 +69,17 @@ def test_vpml_orders_and_thickness(order, n):
     nx, ny, dx, dt = 64, 16, 0.2, 0.12
     e, b = oblique_packet(nx, ny, dx, m=1, direction=-1)
     _fields_case(e, b, nx, ny, dx, dt, 80, order=order, n=n, tile=(8, 8))
+
+
+@pytest.mark.parametrize("order,diffuse", [(2, (True, True)), (2, (True, False)),
+                                           (3, (False, True))])
+def test_vpml_diffuse_bitexact(order, diffuse):
+    """vpml_diffuse, orders 2 and 3 (order 1 is refused)."""
+    print(f"case diffuse order={order} walls={diffuse}", flush=True)
+    nx, ny, dx, dt = 64, 16, 0.2, 0.12
+    for direction in (1, -1):
+        e, b = oblique_packet(nx, ny, dx, m=1, direction=direction)
+        _fields_case(e, b, nx, ny, dx, dt, 80, order=order, tile=(8, 8), diffuse=diffuse)
 
 
 def test_vpml_laser_wall_bitexact():
@@ -157,3 +170,9 @@ def test_vpml_3d_plasma_vs_fp64_oracle():
 def test_vpml_rejects():
     with pytest.raises(OsrError):
         oa.Simulation(2, (32, 16), (0.2, 0.2), 0.1, 2, xbnd="vpml", vpml_size=3)
+    with pytest.raises(OsrError):
+        oa.Simulation(3, (16, 8, 8), (0.2,) * 3, 0.05, 2, xbnd="vpml", vpml_diffuse=(True, True))
+    with pytest.raises(OsrError):
+        oa.Simulation(2, (32, 16), (0.2, 0.2), 0.1, 2, xbnd="open", vpml_diffuse=(True, False))
+    with pytest.raises(OsrError):
+        oa.Simulation(2, (32, 16), (0.2, 0.2), 0.1, 1, xbnd="vpml", vpml_diffuse=(True, True))
