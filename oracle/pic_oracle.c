@@ -22,7 +22,11 @@
  * fused Boris advance/deposit against the reference's own AVX2 kernels compiled from
  * simd/ (oracle/_ref) on identical inputs; deposit against discrete continuity with the
  * independent rho deposit; Vay against the exact ExB drift; Yee against the closed-form
- * numerical dispersion of a vacuum plane wave and energy conservation.
+ * numerical dispersion of a vacuum plane wave and energy conservation; Lindman walls
+ * against the closed-form Mur reflection; VPML walls against the Yee lattice continued
+ * through the wall (attenuation off) and their measured reflections.  Parity unpinned:
+ * the last bit of the VPML attenuation coefficients against a reference binary, whose
+ * expf and FMA contraction (-Ofast -march=native) the restatement cannot see.
  */
 #include <math.h>
 #include <stdint.h>
