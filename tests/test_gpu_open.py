@@ -124,18 +124,19 @@ def test_open_orders_vs_oracles(order, pusher):
     assert np.max(np.abs(fld - rfld) / fscale) <= tol
 
 
-def test_open_walls_with_relayouts():
+@pytest.mark.parametrize("xbnd", ["open", "vpml"])
+def test_open_walls_with_relayouts(xbnd):
     """A dense beam crossing into empty tiles overflows tile capacities, so steps halt and
     the host re-lays the particles out from the tiles and the pending movers; movers
-    pushed through a wall in those steps must stay removed.  Counts and energies against
-    the FP64 oracle, as without overflows."""
+    pushed through a wall in those steps must stay removed, and the wall fields advance
+    once per step.  Counts and energies against the FP64 oracle, as without overflows."""
     rng = np.random.default_rng(47)
     nx, dx, dt, steps = (32, 32), (0.1, 0.1), 0.07, 30
     ix, x, u, q = random_particles(rng, 2, nx, 6000, uth=0.05, ufl=(2.0, 1.5, 0.0), q=-0.02,
                                    lo=[17, 1], hi=[24, 8])
-    ref = O.Sim(2, nx, dx, dt, 2, open_x=True)
+    ref = O.Sim(2, nx, dx, dt, 2, open_x=True, xwall="vpml" if xbnd == "vpml" else "lindman")
     ref.add_species(-1.0, 0, ix, x, u, q)
-    with oa.Simulation(2, nx, dx, dt, 2, tile=(8, 8), xbnd="open") as sim:
+    with oa.Simulation(2, nx, dx, dt, 2, tile=(8, 8), xbnd=xbnd) as sim:
         sid = sim.add_species(-1.0)
         sim.set_particles(sid, ix, x, u, q)
         sim.run(steps)
