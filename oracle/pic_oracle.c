@@ -21,7 +21,11 @@
  * Pinning (see the tests): splines/wl against closed forms and partition of unity; the
  * fused Boris advance/deposit against the reference's own AVX2 kernels compiled from
  * simd/ (oracle/_ref) on identical inputs; deposit against discrete continuity with the
- * independent rho deposit; Vay against the exact ExB drift; Yee against the closed-form
+ * independent rho deposit; Vay against the exact ExB drift, the exact force-free motion
+ * with a momentum component along B (the gamma solve's (u'.tau)^2 term) and the exact
+ * acceleration along B; the step order (push, energy rows, fold, Yee) against the energy
+ * conservation of a cold Langmuir oscillation, O((w_p dt)^2) only when the field and
+ * kinetic rows are time-aligned; Yee against the closed-form
  * numerical dispersion of a vacuum plane wave and energy conservation; Lindman walls
  * against the closed-form Mur reflection; VPML walls against the Yee lattice continued
  * through the wall (attenuation off) and their measured reflections.  Parity unpinned:

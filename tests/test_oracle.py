@@ -174,6 +174,101 @@ def test_exb_drift(pusher):
         assert err > 1e-6  # the Boris pusher does not preserve the relativistic drift
 
 
+def _vay_numpy(u, e, b, rqm, dt, drop_ustar=False):
+    """Vay momentum update of spec/os-spec-push.f03:706-769 written out in numpy (FP64):
+    half acceleration plus the rotation with the old velocity, the second half
+    acceleration, the gamma of the new momentum from the quadratic of the implicit
+    rotation, then the rotation.  drop_ustar removes the (u' . tau)^2 term of that gamma
+    (a planted mistake, to show the pin below detects it)."""
+    tem = 0.5 * dt / rqm
+    e, b = tem * e, tem * b
+    v = u / np.sqrt(1 + (u * u).sum(-1, keepdims=True))
+    w = u + 2 * e + np.cross(v, b)
+    ustar = (w * b).sum(-1, keepdims=True)
+    sigma = 1 + (w * w).sum(-1, keepdims=True) - (b * b).sum(-1, keepdims=True)
+    extra = 0.0 if drop_ustar else ustar * ustar
+    gam = np.sqrt(0.5 * (sigma + np.sqrt(sigma * sigma + 4 * ((b * b).sum(-1, keepdims=True) + extra))))
+    t = b / gam
+    s = 1 / (1 + (t * t).sum(-1, keepdims=True))
+    return s * (w + (w * t).sum(-1, keepdims=True) * t + np.cross(w, t))
+
+
+def test_vay_force_free_with_momentum_along_b():
+    """E perpendicular to B and v = E x B / B^2 + v_par b: the Lorentz force vanishes, and
+    Vay's implicit rotation keeps u exactly.  The momentum has a component along B, so the
+    (u' . tau)^2 term of the gamma solve (os-spec-push.f03:723-731) is exercised: without
+    it the gamma, and then u, come out wrong."""
+    bvec = np.array([0.3, -0.4, 1.2])
+    bh = bvec / np.linalg.norm(bvec)
+    evec = 0.5 * np.cross(bh, [1.0, 0.0, 0.0])  # perpendicular to B
+    vdrift = np.cross(evec, bvec) / (bvec @ bvec)
+    v = vdrift + 0.55 * bh
+    assert v @ v < 1
+    u0 = v / np.sqrt(1 - v @ v)
+    u = u0[None, :].copy()
+    for _ in range(20):
+        O.dudt(1, -1.0, 0.2, u, np.ones(1), evec[None, :], bvec[None, :])
+    assert np.abs(u[0] - u0).max() < 1e-12
+    # the numpy statement reproduces the oracle on random inputs ...
+    rng = np.random.default_rng(11)
+    ur, er, br = rng.normal(0, 1, (3, 32, 3))
+    uo = ur.copy()
+    O.dudt(1, -1.0, 0.3, uo, np.ones(32), er, br)
+    assert np.allclose(_vay_numpy(ur, er, br, -1.0, 0.3), uo, rtol=0, atol=1e-13)
+    # ... and with the ustar term dropped it breaks the force-free motion
+    up = u0[None, :].copy()
+    for _ in range(20):
+        up = _vay_numpy(up, evec[None, :], bvec[None, :], -1.0, 0.2, drop_ustar=True)
+    assert np.abs(up[0] - u0).max() > 1e-5  # measured 7.6e-4
+
+
+def test_vay_acceleration_along_b():
+    """E, B and u parallel: no rotation, u(n+1) = u(n) + 2 tem E, and the energy sum is the
+    time-centred q |u_c|^2 / (gamma_c + 1) of u_c = (u(n) + u(n+1)) / 2
+    (os-spec-push.f03:772-792)."""
+    bh = np.array([2.0, -1.0, 2.0]) / 3.0
+    u0 = 0.7 * bh
+    e, b = 0.8 * bh, 1.5 * bh
+    u = u0[None, :].copy()
+    ene = O.dudt(1, -1.0, 0.1, u, np.ones(1), e[None, :], b[None, :])
+    tem = 0.5 * 0.1 / -1.0
+    assert np.allclose(u[0], u0 + 2 * tem * e, rtol=0, atol=1e-15)
+    uc = 0.5 * (u0 + u[0])
+    assert ene == pytest.approx((uc @ uc) / (np.sqrt(1 + uc @ uc) + 1), rel=1e-14)
+
+
+def test_history_rows_are_time_aligned():
+    """Cold Langmuir oscillation (immobile ions, u_x = u0 sin(kx)): energy flows between
+    the electrons and E_x at the plasma frequency.  With the reference's step order (push
+    with E, B at integer time; the time-centred kinetic energy of row n; field energy of
+    the same E, B; then fold and Yee, os-simulation.f03:588-661) the total of row n is
+    conserved to O((w_p dt)^2) of the exchanged energy.  A field row taken after the Yee
+    step, or kinetic and field rows one step apart, leave an O(w_p dt) oscillation: the
+    planted shifts below must fail the bound."""
+    nx, dx, dt, ppc = (64, 4), (0.1, 0.1), 0.07, (4, 4)
+    cells = np.stack(np.meshgrid(np.arange(1, nx[0] + 1), np.arange(1, nx[1] + 1),
+                                 indexing="ij"), -1).reshape(-1, 2)
+    ix = np.repeat(cells, ppc[0] * ppc[1], 0).astype(np.int32)
+    lat = np.stack(np.meshgrid((np.arange(ppc[0]) + 0.5) / ppc[0] - 0.5,
+                               (np.arange(ppc[1]) + 0.5) / ppc[1] - 0.5, indexing="ij"),
+                   -1).reshape(-1, 2)
+    x = np.tile(lat, (len(cells), 1))
+    k = 2 * np.pi / (nx[0] * dx[0])
+    u = np.zeros((len(ix), 3))
+    u[:, 0] = 0.01 * np.sin(k * (ix[:, 0] - 0.5 + x[:, 0]) * dx[0])
+    q = np.full(len(ix), -1.0 / 16)
+    s = O.Sim(2, nx, dx, dt, 2)
+    s.add_species(-1.0, 0, ix, x, u, q)
+    s.step(200)  # about 2.2 plasma periods
+    fld = np.array(s.fld_hist).sum(1)
+    kin = np.array(s.par_hist)[:, 0]
+    exchanged = kin.max() - kin.min()
+    assert fld.max() > 0.5 * exchanged  # the energy did flow into the field
+    assert np.ptp(fld + kin) < 0.01 * exchanged  # measured 1.6e-3 (w_p dt = 0.07)
+    for shifted in (fld[1:] + kin[:-1], fld[:-1] + kin[1:]):
+        assert np.ptp(shifted) > 0.1 * exchanged  # measured 0.14
+
+
 # Yee staggering: along axis a, E_a and B_b (b != a) sit half a cell from E_b, B_a
 HALF = {0: ((True, False, False), (False, True, True)),
         1: ((False, True, False), (True, False, True)),
