@@ -3,10 +3,13 @@
 //
 // Kernels are not executed.  Each launch validates what the kernel would touch: every
 // work item's descriptor, every array a descriptor points to (live, large enough, tile
-// offsets consistent, tile counts within the push limit).  The particle generator and the
-// step finalisation are emulated (particles placed on their cells, step counter, history
-// rows = step + column, parking at the step target), and the fill can inject tile
-// overflows at random so that the re-layout path runs.
+// offsets consistent, tile counts within the push limit); kernels without a dedicated
+// check have every pointer argument checked.  The checks run when the launch is enqueued
+// (to record the allocations it touches) and again when its stream executes it.  The
+// particle generator and the step finalisation are emulated at execution (particles
+// placed on their cells, step counter, history rows = step + column, parking at the step
+// target), and the fill can inject tile overflows at random so that the re-layout path
+// runs.
 #define OSR_ENSEMBLE_HOST_CHECK 1
 #include "mock_hip.h"
 #include "../../osiris_amd/csrc/capi.hip"
@@ -19,7 +22,10 @@ extern "C" double g_mock_overflow_p;
 double g_mock_overflow_p = 0.0;
 extern "C" double g_mock_notready_p;
 double g_mock_notready_p = 0.0;
+extern "C" double g_mock_progress_p;
+double g_mock_progress_p = 0.0;
 extern "C" long long mock_errors() { return mock::nerr(); }
+extern "C" long long mock_order_errors() { return mock::nordered(); }
 extern "C" long long mock_live() { return (long long)mock::live().size(); }
 
 static void check_grid(const GridDesc& g) {
@@ -65,10 +71,11 @@ void osr_mock_exec(const void* k, dim3 g, const SpecDesc* d, const int2* w) {
   for (unsigned b = 0; b < g.x; b++) {
     const int2 wk = w[b];
     if (!mock::valid(&d[wk.x], sizeof(SpecDesc))) { mock::err("desc index", &d[wk.x], sizeof(SpecDesc)); continue; }
+    mock::check(&d[wk.x], sizeof(SpecDesc), "desc");
     if (wk.y < 0 || wk.y >= d[wk.x].g.ntiles) mock::err("tile index", d, (size_t)wk.y);
     if (seen.insert(wk.x).second) check_spec(d[wk.x]);
   }
-  if (k == (const void*)tile_fill<256, 4096, 32768>) {
+  if (mock::effects() && k == (const void*)tile_fill<256, 4096, 32768>) {
     std::uniform_real_distribution<double> U(0, 1);
     for (int i : seen) {
       Ctrl* c = d[i].g.ctrl;
@@ -85,6 +92,7 @@ void osr_mock_exec(const void*, dim3 g, const FieldDesc* f, const int2* w) {
   for (unsigned b = 0; b < g.x; b++) {
     const int2 wk = w[b];
     if (!mock::valid(&f[wk.x], sizeof(FieldDesc))) { mock::err("fd index", &f[wk.x], 0); continue; }
+    mock::check(&f[wk.x], sizeof(FieldDesc), "fd");
     if ((long long)wk.y * 256 >= f[wk.x].g.ncell + 255) mock::err("yee block", f, (size_t)wk.y);
     if (seen.insert(wk.x).second) check_field(f[wk.x]);
   }
@@ -93,8 +101,10 @@ void osr_mock_exec(const void*, dim3 g, const FieldDesc* f, const int* w) {
   mock::check(w, (size_t)g.x * 4, "fin work");
   for (unsigned b = 0; b < g.x; b++) {
     if (!mock::valid(&f[w[b]], sizeof(FieldDesc))) { mock::err("fin fd", &f[w[b]], 0); continue; }
+    mock::check(&f[w[b]], sizeof(FieldDesc), "fin fd");
     const FieldDesc& fd = f[w[b]];
     check_field(fd);
+    if (!mock::effects()) continue;
     Ctrl* c = fd.g.ctrl;
     if (!mock::valid(c, sizeof(Ctrl)) || c->halt >= 2) continue;
     const long long step = c->steps;
@@ -107,7 +117,9 @@ void osr_mock_exec(const void*, dim3 g, const FieldDesc* f, const int* w) {
   }
 }
 void osr_mock_exec(const void*, dim3, const SpecDesc* d, int sid, GenArgs a) {
+  mock::check(&d[sid], sizeof(SpecDesc), "gen desc");
   check_spec(d[sid]);
+  if (!mock::effects()) return;
   const SpecDesc& D = d[sid];
   const GridDesc& g = D.g;
   for (int tile = 0; tile < g.ntiles; tile++) {
@@ -138,3 +150,44 @@ static void mock_segv(int) {
   _exit(99);
 }
 __attribute__((constructor)) static void mock_init() { signal(SIGSEGV, mock_segv); signal(SIGBUS, mock_segv); }
+
+// Planted stream-order mistakes, to show that the checks above catch them: returns the
+// number of violations reported for (1) a pool block freed on the allocation stream while
+// a kernel queued on another stream still uses it, (2) a host-synchronous copy into a
+// block with queued kernel work, (3) a kernel on another stream touching a pool block
+// before its allocating stream executed the allocation.  Expected: 3.
+__global__ void mock_touch_kernel(float* p) { (void)p; }
+extern "C" long long mock_selftest_planted() {
+  const long long before = mock::nordered();
+  hipStream_t a, b;
+  hipStreamCreateWithFlags(&a, hipStreamNonBlocking);
+  hipStreamCreateWithFlags(&b, hipStreamNonBlocking);
+  const double keep_p = g_mock_progress_p;
+  g_mock_progress_p = 0.0;  // nothing advances unless synchronised
+  void* p = nullptr;
+  hipMallocAsync(&p, 4096, a);
+  hipStreamSynchronize(a);
+  hipLaunchKernelGGL(mock_touch_kernel, dim3(1), dim3(64), 0, b, (float*)p);
+  hipFreeAsync(p, a);
+  hipStreamSynchronize(a);  // (1)
+  hipStreamSynchronize(b);
+  void* q = nullptr;
+  hipMalloc(&q, 4096);
+  hipLaunchKernelGGL(mock_touch_kernel, dim3(1), dim3(64), 0, b, (float*)q);
+  float h[4] = {0, 0, 0, 0};
+  hipMemcpy(q, h, sizeof(h), hipMemcpyHostToDevice);  // (2)
+  hipStreamSynchronize(b);
+  hipFree(q);
+  void* r = nullptr;
+  hipLaunchKernelGGL(mock_touch_kernel, dim3(1), dim3(64), 0, a, (float*)nullptr);  // a busy
+  hipMallocAsync(&r, 4096, a);
+  hipLaunchKernelGGL(mock_touch_kernel, dim3(1), dim3(64), 0, b, (float*)r);
+  hipStreamSynchronize(b);  // (3)
+  hipStreamSynchronize(a);
+  hipFreeAsync(r, a);
+  hipStreamSynchronize(a);
+  hipStreamDestroy(a);
+  hipStreamDestroy(b);
+  g_mock_progress_p = keep_p;
+  return mock::nordered() - before;
+}

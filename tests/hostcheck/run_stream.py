@@ -16,6 +16,8 @@ import osiris_amd._lib as L  # noqa: E402
 
 def main():
     lib_path, n, overflow_p, notready_p = sys.argv[1], int(sys.argv[2]), float(sys.argv[3]), float(sys.argv[4])
+    progress_p = float(sys.argv[7]) if len(sys.argv) > 7 else 0.3
+    realtime = len(sys.argv) > 8 and sys.argv[8] == "realtime"
     sides = tuple(int(v) for v in sys.argv[5].split(","))
     ppcs = tuple(int(v) for v in sys.argv[6].split(","))
     L.LIB_PATH = lib_path
@@ -23,12 +25,16 @@ def main():
     raw = C.CDLL(lib_path)
     C.c_double.in_dll(raw, "g_mock_overflow_p").value = overflow_p
     C.c_double.in_dll(raw, "g_mock_notready_p").value = notready_p
+    C.c_double.in_dll(raw, "g_mock_progress_p").value = progress_p
     raw.mock_errors.restype = C.c_longlong
+    raw.mock_order_errors.restype = C.c_longlong
     raw.mock_live.restype = C.c_longlong
     from osiris_amd.ensemble import poisson_jobs, serve
 
     stream = poisson_jobs(n=n, sides=sides, ppcs=ppcs, steps=(20, 400))
-    res = serve(stream, realtime=False, chunk_steps=8, keep=True)
+    if realtime:  # arrivals compressed 100x so that admissions land between chunks
+        stream = [(t / 100.0, j) for t, j in stream]
+    res = serve(stream, realtime=realtime, chunk_steps=8, keep=True)
     bad_hist = 0
     for k, (fld, par) in res["histories"].items():
         nsteps = stream[k][1].nsteps
@@ -37,7 +43,8 @@ def main():
         if fld.shape != (nsteps, 6) or not np.array_equal(fld[:, 0], rows) or \
                 not np.array_equal(par[:, 0], rows + 6):
             bad_hist += 1
-    print(json.dumps({"jobs": res["jobs"], "errors": raw.mock_errors(), "live": raw.mock_live(),
+    print(json.dumps({"jobs": res["jobs"], "errors": raw.mock_errors(),
+                      "order_errors": raw.mock_order_errors(), "live": raw.mock_live(),
                       "bad_histories": bad_hist}))
 
 

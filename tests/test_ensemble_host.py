@@ -35,9 +35,9 @@ def mock_lib():
     return out
 
 
-def _run(lib, n, overflow_p, notready_p, sides, ppcs):
+def _run(lib, n, overflow_p, notready_p, sides, ppcs, progress_p=0.3, mode="saturated"):
     r = subprocess.run([sys.executable, os.path.join(HC, "run_stream.py"), lib, str(n),
-                        str(overflow_p), str(notready_p), sides, ppcs],
+                        str(overflow_p), str(notready_p), sides, ppcs, str(progress_p), mode],
                        capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-3000:]
     return json.loads(r.stdout.strip().splitlines()[-1])
@@ -45,9 +45,29 @@ def _run(lib, n, overflow_p, notready_p, sides, ppcs):
 
 def test_ensemble_host_logic_with_relayouts(mock_lib):
     res = _run(mock_lib, 40, 0.01, 0.5, "64,128", "4,16")
-    assert res == {"jobs": 40, "errors": 0, "live": 0, "bad_histories": 0}
+    assert res == {"jobs": 40, "errors": 0, "order_errors": 0, "live": 0, "bad_histories": 0}
 
 
 def test_ensemble_host_logic_large_jobs(mock_lib):
     res = _run(mock_lib, 24, 0.002, 0.3, "128,256", "4,16,64")
-    assert res == {"jobs": 24, "errors": 0, "live": 0, "bad_histories": 0}
+    assert res == {"jobs": 24, "errors": 0, "order_errors": 0, "live": 0, "bad_histories": 0}
+
+
+def test_stream_order_checker_catches_planted_mistakes(mock_lib):
+    """The mock runtime reports each planted stream-order mistake (a pool block freed while
+    another stream still has queued work on it, a host-synchronous copy racing queued
+    work, a pool block used by another stream before its allocation executed)."""
+    r = subprocess.run([sys.executable, "-c",
+                        "import ctypes as C, sys; L = C.CDLL(sys.argv[1]); "
+                        "L.mock_selftest_planted.restype = C.c_longlong; "
+                        "print(L.mock_selftest_planted())", mock_lib],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert int(r.stdout.strip().splitlines()[-1]) == 3
+
+
+def test_ensemble_host_logic_realtime_arrivals(mock_lib):
+    """Arrivals spread over the run (admissions while chunks are queued) with slow stream
+    progress, so that frees, re-layouts and graph rebuilds meet queued work."""
+    res = _run(mock_lib, 48, 0.01, 0.7, "64,128", "4,16", progress_p=0.05, mode="realtime")
+    assert res == {"jobs": 48, "errors": 0, "order_errors": 0, "live": 0, "bad_histories": 0}
