@@ -214,11 +214,18 @@ def run_c3(args, oa):
 
 
 # c4: laser-plasma interaction between open x walls (emf_bound "lindman", spec_bound
-# "absorbing", zpulse_wall from x-min), periodic y
-C4 = dict(dx=(0.2, 0.5), dt=0.14, density=0.01, ramp_cells=100, gap_cells=100, ppc=(4, 2),
-          laser=dict(a0=2.0, omega0=10.0, pol=90.0, tenv_type="sin2", tenv_rise=10.0,
-                     tenv_flat=10.0, tenv_fall=10.0, per_type="gaussian", per_w0=20.0,
-                     per_focus=40.0))
+# "absorbing", zpulse_wall from x-min), periodic y.  Units: the laser frequency is the
+# normalising frequency (omega0 = 1), so lengths are in c/omega0, times in 1/omega0 and a
+# density of 0.01 is 0.01 n_c (plasma frequency 0.1 omega0).  dx = 0.2 resolves the
+# wavelength 2 pi with 31 cells and dt = 0.14 the period with 45 steps.  The pulse:
+# a0 = 2, linear polarisation in the simulation plane (E2), gaussian temporal envelope of
+# duration 30 (exp(-2 ((t - 90) / 30)^2), cut at +-3 durations), gaussian beam of waist 20
+# centred on the box (y = 128) and focused at the top of the ramp (x = 20).  Electrons:
+# 8 ppc cold, density rising linearly from the x-min wall over 100 cells, then flat.
+C4 = dict(dx=(0.2, 0.5), dt=0.14, density=0.01, ramp_cells=100, gap_cells=0, ppc=(4, 2),
+          laser=dict(a0=2.0, omega0=1.0, pol=0.0, tenv_type="gaussian", tenv_duration=30.0,
+                     tenv_range=180.0, per_type="gaussian", per_w0=20.0, per_focus=20.0,
+                     per_center=(128.0,)))
 
 
 def c4_particles(nx, gap, ramp, ppc, density):
@@ -240,20 +247,20 @@ def c4_particles(nx, gap, ramp, ppc, density):
     return ix, x, u, q
 
 
-def oracle_check_c4(oa, steps=128, xwall="lindman"):
-    """The c4 physics (cold ramped plasma, a0 = 2 sin^2 laser with a gaussian profile from
-    x-min, Lindman or VPML x walls, absorbing particles) on a 256 x 32 grid with a 40-cell
-    gap and ramp: GPU against the FP64 oracle on the same particles, over the steps in
-    which the laser enters the plasma.  Later the lattice-loaded cold plasma goes unstable
-    and amplifies rounding differences: the FP32 and FP64 oracles part by 6e-6 at step 136
-    and by 3e-3 at step 152, so those steps say nothing about the arithmetic."""
+def oracle_check_c4(oa, steps=256, xwall="lindman"):
+    """The c4 physics (cold ramped plasma from the x-min wall, a0 = 2 laser with gaussian
+    temporal envelope and gaussian beam from x-min, omega0 = 1, Lindman or VPML x walls,
+    absorbing particles) on a 256 x 32 grid (51.2 x 16 c/omega0) with a 40-cell ramp, a
+    shorter pulse (duration 4, range 24) and waist 4 centred on the box: GPU against the
+    FP64 oracle on the same particles, over the steps in which the laser enters the plasma
+    (the FP32 oracle's own deviation is reported beside)."""
     from oracle import oracle as O
 
     nx = (256, 32)
     dx, dt = C4["dx"], C4["dt"]
-    laser = dict(C4["laser"], tenv_rise=4.0, tenv_flat=4.0, tenv_fall=4.0, per_w0=4.0,
-                 per_focus=12.0)
-    ix, x, u, q = c4_particles(nx, 40, 40, C4["ppc"], C4["density"])
+    laser = dict(C4["laser"], tenv_duration=4.0, tenv_range=24.0, per_w0=4.0, per_focus=4.0,
+                 per_center=(8.0,))
+    ix, x, u, q = c4_particles(nx, 0, 40, C4["ppc"], C4["density"])
     with oa.Simulation(2, nx, dx, dt, 2, tile=(32, 16), xbnd=xwall) as sim:
         sid = sim.add_species(-1.0)
         sim.set_particles(sid, ix, x, u, q)
@@ -281,8 +288,9 @@ def oracle_check_c4(oa, steps=128, xwall="lindman"):
 
     err_gpu, err_f32 = rel(par), rel(refs["f32"][1])
     err_fld = float(np.max(np.abs(fld - rfld) / fscale))
-    return {"workload": ("c4 physics on 256x32 (gap and ramp 40 cells, sin^2 4/4/4, w0=4, "
-                         f"{xwall} walls), {len(q)} particles"),
+    return {"workload": ("c4 physics on 256x32 (omega0=1, ramp 40 cells from the x-min wall, "
+                         f"gaussian envelope duration 4 range 24, w0=4, {xwall} walls), "
+                         f"{len(q)} particles"),
             "steps": steps, "reference": "FP64 CPU oracle (restatement of the reference)",
             "particles_left": {"gpu": int(n_gpu), "oracle": int(n_ref)},
             "max_rel_err_particle_energy": err_gpu,
@@ -294,15 +302,16 @@ def oracle_check_c4(oa, steps=128, xwall="lindman"):
 
 
 def run_c4(args, oa):
-    """Laser-plasma interaction: 2048 x 512 cells, dx = 0.2, dy = 0.5, dt = 0.14, cold
-    electrons (8 ppc) at 0.01 n_c behind a 100-cell vacuum gap and a 100-cell linear ramp,
-    a0 = 2 laser (omega0 = 10, sin^2 10/10/10, gaussian w0 = 20 focused at x = 40) launched
-    from x-min, Lindman (or, with --xwall vpml, 8-cell VPML) x walls with absorbing
-    particles, periodic y, quadratic, Boris."""
+    """Laser-plasma interaction: 2048 x 512 cells, dx = 0.2, dy = 0.5 c/omega0, dt = 0.14
+    /omega0, cold electrons (8 ppc) at 0.01 n_c behind a 100-cell linear ramp from the
+    x-min wall, a0 = 2 laser (omega0 = 1, gaussian envelope of duration 30, gaussian beam
+    of waist 20 focused at x = 20) launched from x-min, Lindman (or, with --xwall vpml,
+    8-cell VPML) x walls with absorbing particles, periodic y, quadratic, Boris."""
     nx = (2048, 512)
     dx, dt = C4["dx"], C4["dt"]
     tile = tuple(args.tile) if args.tile != [16, 16] else (32, 16)
-    steps = min(args.steps, 1500)
+    # the job is 1500 steps: the warm-up steps are its first ones, the rest are timed
+    steps = min(args.steps or 1500, 1500 - args.warmup)
     ix, x, u, q = c4_particles(nx, C4["gap_cells"], C4["ramp_cells"], C4["ppc"], C4["density"])
     sim = oa.Simulation(2, nx, dx, dt, 2, tile=tile, xbnd=args.xwall)
     sid = sim.add_species(-1.0)
@@ -335,8 +344,9 @@ def run_c4(args, oa):
         "warmup": args.warmup, "ms_per_step": el * 1e3 / steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (lattice-loaded cold plasma, host generated)",
-        "config": {"workload": ("2D laser-plasma 2048x512, dx=0.2, dy=0.5, dt=0.14, 0.01 n_c "
-                                "with 100-cell ramp, 8 ppc cold, a0=2 zpulse_wall from x-min, "
+        "config": {"workload": ("2D laser-plasma 2048x512, dx=0.2, dy=0.5 c/w0, dt=0.14/w0, "
+                                "0.01 n_c with 100-cell ramp, 8 ppc cold, a0=2 zpulse_wall "
+                                "from x-min (w0=1, gaussian envelope 30/w0, waist 20 c/w0), "
                                 f"{'Lindman' if args.xwall == 'lindman' else 'VPML (8 cells)'} "
                                 "x walls + absorbing particles, periodic y, quadratic"),
                    "particles": [n0, n1], "cells": cells, "tile": list(tile),
@@ -361,7 +371,9 @@ def run_c4(args, oa):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: c2 200, c3 100, c4 the rest of the 1500-step "
+                         "job after the warm-up)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--nx", type=int, nargs=2, default=[1024, 1024])
     ap.add_argument("--ppc", type=int, nargs=2, default=[8, 4],
@@ -412,6 +424,8 @@ def main():
                           "GB/s (read+write)": 2 * nbytes / ms / 1e6}), flush=True)
         return
 
+    if args.steps is None and args.workload != "c4":
+        args.steps = 100 if args.workload == "c3" else 200
     if args.workload == "c3":
         run_c3(args, oa)
         return

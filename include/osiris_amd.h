@@ -127,10 +127,10 @@ typedef struct {
   int order;      /* interpolation / deposit order 1..3 */
   int tile[3];    /* tile size in cells; 0 = default (2D 16x16, 3D 8x8x4) */
   int path;       /* OSR_PATH_FUSED (default) or OSR_PATH_PLAIN (2D only) */
-  int kernel;     /* fused push kernel: 0 auto, 1 generic (runtime tile shape), 2 2D
-                     compile-time tile shape with a virtual-particle queue per wave
-                     (tiles 8x8, 16x16, 32x16); both accumulate the current in int64
-                     fixed point in LDS */
+  int kernel;     /* fused push kernel: 0 auto, 1 generic (runtime tile shape), 2
+                     compile-time tile shape (2D tiles 8x8, 16x16, 32x16; 3D tile 8x8x4);
+                     all accumulate the current in int64 fixed point in LDS and give
+                     bit-identical results */
   int xbnd;       /* x walls: OSR_BND_PERIODIC (0), OSR_BND_LINDMAN (1, also named
                      OSR_BND_OPEN) or OSR_BND_VPML (2) */
   int reorder;    /* fused path: restore the (rank in cell, cell) particle order inside the

@@ -21,6 +21,7 @@
 #include "k_loader.hip"
 #include "k_tiles.hip"
 #include "k_push2d.hip"
+#include "k_push3d.hip"
 #include "k_plain.hip"
 
 using namespace osr;
@@ -513,8 +514,8 @@ int layout_upload(osr_sim* s, Species& p, const HostParts& hp) {
     for (int64_t k = b0; k < b1; k++) {
       int ic[3];
       cell_coords(s, hp.cell[byt[k]], ic);
-      const int lc = (ic[0] % s->g.tx) + s->g.tx * ((ic[1] % s->g.ty) +
-                                                     s->g.ty * (s->dims == 3 ? ic[2] % s->g.tz : 0));
+      const int lc = cell_key(ic[0] % s->g.tx, ic[1] % s->g.ty, s->dims == 3 ? ic[2] % s->g.tz : 0,
+                              s->dims, s->g.tx, s->g.ty, s->dims == 3 ? s->g.tz : 1);
       crank[k - b0] = ccount[lc]++;
       maxc = std::max(maxc, ccount[lc]);
     }
@@ -531,8 +532,8 @@ int layout_upload(osr_sim* s, Species& p, const HostParts& hp) {
       for (int64_t k = b0; k < b1; k++) {
         int ic[3];
         cell_coords(s, hp.cell[byt[k]], ic);
-        lcv[k - b0] = (ic[0] % s->g.tx) + s->g.tx * ((ic[1] % s->g.ty) +
-                                                      s->g.ty * (s->dims == 3 ? ic[2] % s->g.tz : 0));
+        lcv[k - b0] = cell_key(ic[0] % s->g.tx, ic[1] % s->g.ty, s->dims == 3 ? ic[2] % s->g.tz : 0,
+                               s->dims, s->g.tx, s->g.ty, s->dims == 3 ? s->g.tz : 1);
         cstart[lcv[k - b0] + 1]++;
       }
       for (int c = 0; c < tcell; c++) cstart[c + 1] += cstart[c];
@@ -681,11 +682,119 @@ void sort_by_pid(HostParts& hp) {
   hp = std::move(o);
 }
 
+// Fill kernel and its launch shape (4 waves per tile; LDS lists for up to 4096 incoming
+// particles and 32768 holes left in a tile's tail).
+typedef void (*fill_fn)(const SpecDesc*, const int2*);
+struct FillKernel {
+  fill_fn f;
+  int nt;
+};
+FillKernel pick_fill() { return {tile_fill<256, 4096, 32768>, 256}; }
+
+// Device re-layout of species `sid` after a halted step (a tile fill overflowed): the
+// tiles that overflowed get their count after the step plus half of it again as
+// capacity (no tile shrinks), every tile's particles and pending mover lists move to the
+// new arrays on the device (regrow_move), and the fill of the overflowed tiles runs again
+// there.  *ok = false when that cannot finish the step (a tile's incoming or tail lists
+// exceed the fill kernel's LDS tables, or slots would exceed int32): the species is then
+// left in the halted state, possibly in the larger layout, for the host re-layout.
+int regrow_species(osr_sim* s, int sid, bool* ok) {
+  Species& p = *s->sp[sid];
+  const int nt = s->ntiles;
+  *ok = false;
+  std::vector<int> cnt, inc, need;
+  TRY(download(cnt, p.cnt, nt));
+  TRY(download(inc, p.inc, nt));
+  TRY(download(need, p.newcnt, nt));
+  std::vector<int2> over;
+  for (int t = 0; t < nt; t++)
+    if (inc[t] < 0) {
+      if (need[t] < 0) return OSR_OK;
+      over.push_back(make_int2(sid, t));
+    }
+  if (over.empty()) {
+    *ok = true;
+    return OSR_OK;
+  }
+  std::vector<int> cap(nt), off(nt);
+  int64_t total = 0;
+  for (int t = 0; t < nt; t++) {
+    int64_t c = p.h_cap[t];
+    if (inc[t] < 0) {
+      const int64_t r = need[t];
+      c = std::max<int64_t>(c, r + std::max<int64_t>(r / 2, 64) + 64 * kMaxPushWaves);
+    }
+    c = (c + 63) / 64 * 64;
+    if (total + c > INT32_MAX) return OSR_OK;
+    cap[t] = (int)c;
+    off[t] = (int)total;
+    total += c;
+  }
+  const size_t n = (size_t)total;
+  const bool three = s->dims == 3;
+  DBuf* arr[19] = {&p.cell, &p.x, &p.y, &p.z, &p.ux, &p.uy, &p.uz, &p.q, &p.pid,
+                   &p.mcell, &p.mx, &p.my, &p.mz, &p.mux, &p.muy, &p.muz, &p.mq, &p.mpid,
+                   &p.midx};
+  DBuf nb[19];
+  RegrowArrays src{}, dst{};
+  for (int a = 0; a < 19; a++) {
+    const bool dummy = !three && (a == 3 || a == 12);  // z and mz of a 2D species
+    TRY(nb[a].alloc(dummy ? 4 : 4 * n));
+    src.a[a] = arr[a]->as<uint32_t>();
+    dst.a[a] = nb[a].as<uint32_t>();
+  }
+  DBuf d_off, d_work;
+  TRY(d_off.alloc(4 * (size_t)nt));
+  TRY(d_work.alloc(sizeof(int2) * over.size()));
+  HIPC(hipMemcpy(d_off.p, off.data(), 4 * (size_t)nt, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(d_work.p, over.data(), sizeof(int2) * over.size(), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(regrow_move, dim3((unsigned)nt), dim3(256), 0, s->stream,
+                     s->d_spec.as<SpecDesc>(), sid, src, dst, d_off.as<int>());
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(s->stream));
+  for (int a = 0; a < 19; a++) arr[a]->swap(nb[a]);  // the old arrays go with nb
+  HIPC(hipMemcpy(p.off.p, off.data(), 4 * (size_t)nt, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(p.cap.p, cap.data(), 4 * (size_t)nt, hipMemcpyHostToDevice));
+  p.h_off = off;
+  p.h_cap = cap;
+  p.cap_total = total;
+  TRY(upload_descs(s));
+  // the fill of the overflowed tiles, on the new layout (the halt flag cleared for it)
+  Ctrl c{};
+  HIPC(hipMemcpy(&c, s->ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost));
+  Ctrl z = c;
+  z.halt = 0;
+  HIPC(hipMemcpy(s->ctrl.p, &z, sizeof(Ctrl), hipMemcpyHostToDevice));
+  const FillKernel fk = pick_fill();
+  hipLaunchKernelGGL(fk.f, dim3((unsigned)over.size()), dim3(fk.nt), 0, s->stream,
+                     s->d_spec.as<SpecDesc>(), d_work.as<int2>());
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(s->stream));
+  Ctrl after{};
+  HIPC(hipMemcpy(&after, s->ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost));
+  HIPC(hipMemcpy(s->ctrl.p, &c, sizeof(Ctrl), hipMemcpyHostToDevice));
+  if (after.halt != 0 || after.error != 0) return OSR_OK;
+  TRY(download(cnt, p.cnt, nt));
+  int64_t np = 0;
+  for (int v : cnt) np += v;
+  p.np = np;
+  *ok = true;
+  return OSR_OK;
+}
+
+// Re-layout of every species.  After a halted step (`pending`: the fill overflowed) the
+// device re-layout is tried first and the host one (collect + layout_upload) is the
+// fallback; otherwise the particles are laid out again from the host.
 int relayout(osr_sim* s, bool pending) {
-  for (auto& p : s->sp) {
+  for (size_t i = 0; i < s->sp.size(); i++) {
+    if (pending) {
+      bool ok = false;
+      TRY(regrow_species(s, (int)i, &ok));
+      if (ok) continue;
+    }
     HostParts hp;
-    TRY(collect(s, *p, pending, hp));
-    TRY(layout_upload(s, *p, hp));
+    TRY(collect(s, *s->sp[i], pending, hp));
+    TRY(layout_upload(s, *s->sp[i], hp));
   }
   TRY(upload_descs(s));
   s->relayouts++;
@@ -716,23 +825,52 @@ push_fn pick_push2d_t(int order, int pusher, int tx, int ty) {
   return nullptr;
 }
 
+// 3D kernels with compile-time tile shape (static LDS); nullptr if not instantiated
+push_fn pick_push3d_t(int order, int pusher, int tx, int ty, int tz) {
+#define OSR_T3(O, P, X, Y, Z)                                               \
+  if (order == O && pusher == P && tx == X && ty == Y && tz == Z)           \
+    return push_deposit_3d_t<O, P, X, Y, Z>;
+#define OSR_TT3(X, Y, Z)                                                     \
+  OSR_T3(1, 0, X, Y, Z) OSR_T3(2, 0, X, Y, Z) OSR_T3(3, 0, X, Y, Z)          \
+  OSR_T3(1, 1, X, Y, Z) OSR_T3(2, 1, X, Y, Z) OSR_T3(3, 1, X, Y, Z)
+  OSR_TT3(8, 8, 4)
+#undef OSR_TT3
+#undef OSR_T3
+  return nullptr;
+}
+
+// The fused push kernel of a simulation (or batch) kind: the compile-time tile variant
+// when one is instantiated and the generic kernel was not asked for, else the generic
+// kernel with `lds` bytes of dynamic LDS.  Sets the threads per workgroup and the dynamic
+// LDS of the launch.
+push_fn pick_push_kernel(int dims, int order, int pusher, int kernel, const int* tile,
+                         size_t lds, size_t* lds_launch, int* nt) {
+  push_fn f = nullptr;
+  *lds_launch = lds;
+  *nt = kNT;
+  if (kernel != 1) {
+    if (dims == 2) {
+      f = pick_push2d_t(order, pusher, tile[0], tile[1]);
+      if (f) *nt = push2d_t_threads(tile[0], tile[1]);
+    } else {
+      f = pick_push3d_t(order, pusher, tile[0], tile[1], tile[2]);
+    }
+    if (f) *lds_launch = 0;
+  }
+  if (!f) f = pick_push(dims, order, pusher);
+  return f;
+}
+
 // One push launch for species first .. first+count-1 (same pusher): the work list holds
 // their (species, tile) items back to back, so the workgroups of the later species fill
 // the tail of the earlier one's.  The species only share the read-only fields and the
 // integer current, so the result does not depend on the grouping.
 int launch_push(osr_sim* s, int first, int count = 1) {
   const int pusher = s->sp[first]->pusher;
-  push_fn f = nullptr;
-  size_t lds = s->lds_push;
+  const int tile[3] = {s->g.tx, s->g.ty, s->g.tz};
+  size_t lds = 0;
   int nt = kNT;
-  if (s->dims == 2 && s->desc.kernel != 1) {
-    f = pick_push2d_t(s->order, pusher, s->g.tx, s->g.ty);
-    if (f) {
-      lds = 0;
-      nt = push2d_t_threads(s->g.tx, s->g.ty);
-    }
-  }
-  if (!f) f = pick_push(s->dims, s->order, pusher);
+  push_fn f = pick_push_kernel(s->dims, s->order, pusher, s->desc.kernel, tile, s->lds_push, &lds, &nt);
   if (!f) return fail(OSR_ERR_ARG, "no push kernel for dims=%d order=%d", s->dims, s->order);
   if (nt != 64 * s->push_waves) return fail(OSR_ERR_STATE, "push wave count mismatch");
   hipLaunchKernelGGL(f, dim3((unsigned)(count * s->ntiles)), dim3(nt), lds, s->stream,
@@ -753,14 +891,6 @@ int launch_pushes(osr_sim* s) {
   return OSR_OK;
 }
 
-// Fill kernel and its launch shape (4 waves per tile; LDS lists for up to 4096 incoming
-// particles and 32768 holes left in a tile's tail).
-typedef void (*fill_fn)(const SpecDesc*, const int2*);
-struct FillKernel {
-  fill_fn f;
-  int nt;
-};
-FillKernel pick_fill() { return {tile_fill<256, 4096, 32768>, 256}; }
 
 // the in-tile particle order restored every s->reorder steps (the kernel itself checks the
 // step and returns at once on the other steps)

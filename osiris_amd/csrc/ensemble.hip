@@ -272,17 +272,9 @@ int ens_build(osr_ensemble* e, EnsBatch* b) {
   int r = OSR_OK;
   for (int p = 0; p < 2 && r == OSR_OK; p++) {
     if (!b->npush[p]) continue;
-    push_fn f = nullptr;
-    size_t lds = b->lds;
+    size_t lds = 0;
     int nt = kNT;
-    if (b->dims == 2 && b->kernel != 1) {
-      f = pick_push2d_t(b->order, p, b->tile[0], b->tile[1]);
-      if (f) {
-        lds = 0;
-        nt = push2d_t_threads(b->tile[0], b->tile[1]);
-      }
-    }
-    if (!f) f = pick_push(b->dims, b->order, p);
+    push_fn f = pick_push_kernel(b->dims, b->order, p, b->kernel, b->tile, b->lds, &lds, &nt);
     if (!f) {
       r = fail(OSR_ERR_ARG, "no push kernel for dims=%d order=%d", b->dims, b->order);
       break;

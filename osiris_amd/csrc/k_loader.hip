@@ -7,9 +7,9 @@
 // seed, particle id and draw index), so the result does not depend on the launch shape.
 // Charges: q = sign(rqm) * density / ppc (os-psource-std.f03:846).
 //
-// Slot p of a tile holds the (p / ncell_tile)-th particle of local cell p % ncell_tile:
-// consecutive lanes of a wave work on different cells, which spreads their LDS current
-// atomics over different banks and addresses.
+// Slot p of a tile holds the (p / ncell_tile)-th particle of the cell of key
+// p % ncell_tile (cell_key): consecutive lanes of a wave work on different cells, which
+// spreads their LDS current atomics over different banks and addresses.
 #include "osr_device.hpp"
 #include "osr_types.hpp"
 
@@ -65,9 +65,18 @@ __global__ __launch_bounds__(256) void generate_particles(const SpecDesc* __rest
   const int nct = wx * wy * wz;
   const int n = d.cnt[tile];
   const int base = d.off[tile];
+  // full tiles follow the in-tile cell order (cell_key), partial ones are walked x first
+  const bool full = wx == g.tx && wy == g.ty && wz == (g.dims == 3 ? g.tz : 1);
   for (int p = blockIdx.y * 256 + threadIdx.x; p < n; p += 256 * gridDim.y) {
     const int c = p % nct, k = p / nct;
-    const int lx = c % wx, ly = (c / wx) % wy, lz = c / (wx * wy);
+    int lx, ly, lz;
+    if (full) {
+      key_cell(c, g.dims, g.tx, g.ty, g.dims == 3 ? g.tz : 1, lx, ly, lz);
+    } else {
+      lx = c % wx;
+      ly = (c / wx) % wy;
+      lz = c / (wx * wy);
+    }
     const int ix = tx0 + lx, iy = ty0 + ly, iz = tz0 + lz;
     const long long gcell = ix + (long long)interior_nx(g) * (iy + (long long)g.ny * iz);
     const uint64_t id = (uint64_t)gcell * (uint64_t)a.ppc_tot + (uint64_t)k;

@@ -671,6 +671,9 @@ __global__ __launch_bounds__(NT) void tile_fill(const SpecDesc* __restrict__ des
   if (I > INC_MAX || nn > d.cap[tile] - 64 * NW || H - I > TAIL_BITS) {
     if (threadIdx.x == 0) {
       d.inc[tile] = -1;
+      // the count the tile needs after this step, or -1 if more capacity would not help
+      // (the incoming or tail lists exceed this kernel's LDS tables)
+      d.newcnt[tile] = (I > INC_MAX || H - I > TAIL_BITS) ? -1 : nn;
       atomicOr(&g.ctrl->halt, 1);
     }
     return;
@@ -717,6 +720,43 @@ __global__ __launch_bounds__(NT) void tile_fill(const SpecDesc* __restrict__ des
   if (threadIdx.x == 0) {
     d.cnt[tile] = nn;
     d.inc[tile] = I;
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Device re-layout after a step whose fill overflowed a tile: one workgroup per tile
+// copies the tile's particle slots [0, cnt) and its pending mover lists (the valid
+// entries of each wave's region) from the old arrays to the same relative slots of the
+// tile in the new, larger arrays.  Counts, mover counts and strides are per tile and stay
+// valid; the fill of the overflowed tiles then runs again on the new layout.
+// ------------------------------------------------------------------------------------
+struct RegrowArrays {
+  uint32_t* a[19];  // particle arrays (cell x y z ux uy uz q pid), mover arrays (mcell mx
+                    // my mz mux muy muz mq mpid midx)
+};
+__global__ __launch_bounds__(256) void regrow_move(const SpecDesc* __restrict__ descs, int sid,
+                                                   RegrowArrays src, RegrowArrays dst,
+                                                   const int* __restrict__ new_off) {
+  const SpecDesc& d = descs[sid];
+  const int tile = blockIdx.x;
+  const int o0 = d.off[tile], o1 = new_off[tile];
+  const int n = d.cnt[tile];
+  const int NW = d.waves, stride = d.mv_stride[tile];
+  const bool three = d.g.dims == 3;
+  for (int a = 0; a < 9; a++) {
+    if (a == 3 && !three) continue;
+    const uint32_t* S = src.a[a] + o0;
+    uint32_t* D = dst.a[a] + o1;
+    for (int i = threadIdx.x; i < n; i += 256) D[i] = S[i];
+  }
+  for (int w = 0; w < NW; w++) {
+    const int m = d.mv_cnt[tile * NW + w];
+    for (int a = 9; a < 19; a++) {
+      if (a == 12 && !three) continue;
+      const uint32_t* S = src.a[a] + o0 + w * stride;
+      uint32_t* D = dst.a[a] + o1 + w * stride;
+      for (int i = threadIdx.x; i < m; i += 256) D[i] = S[i];
+    }
   }
 }
 
@@ -778,7 +818,7 @@ __global__ __launch_bounds__(NT) void tile_reorder(const SpecDesc* __restrict__ 
             lz >= (dims == 3 ? g.tz : 1))
           s_abort = 1;
         else
-          lc = lx + g.tx * (ly + g.ty * lz);
+          lc = cell_key(lx, ly, lz, dims, g.tx, g.ty, dims == 3 ? g.tz : 1);
       }
       // lanes holding the same cell: one ballot per bit of the cell index
       unsigned long long same = __ballot(lc >= 0);

@@ -130,4 +130,38 @@ __host__ __device__ inline int tile_of(const GridDesc& g, uint32_t c) {
   return ix / g.tx + g.ntx * (iy / g.ty + g.nty * (iz / g.tz));
 }
 
+// Order of the cells of a tile in the particle layout: the loader places particle k of
+// a cell at slot k * ncell + key, the in-tile reorder restores (rank within the cell,
+// key) order, and the host layout uses the same keys.  Consecutive lanes of a wave
+// therefore hold consecutive keys.  3D tiles whose sides are multiples of (4, 4, 2) are
+// walked in blocks of 4 x 4 x 2 cells (x fastest inside a block, blocks x fastest), so
+// that the 32 lanes of a half wave hold one compact block and their LDS stencil reads
+// spread over the banks (with the 12-float row and 144-float plane pitches of the 8x8x4
+// kernel, lx + 12 ly + 144 lz covers 32 distinct banks for the cells of one block).
+// Other tiles are walked x fastest.  The key only orders particles: any order gives the
+// same results.
+__host__ __device__ inline bool blocked_tile(int dims, int tx, int ty, int tz) {
+  return dims == 3 && tx % 4 == 0 && ty % 4 == 0 && tz % 2 == 0;
+}
+__host__ __device__ inline int cell_key(int lx, int ly, int lz, int dims, int tx, int ty, int tz) {
+  if (!blocked_tile(dims, tx, ty, tz)) return lx + tx * (ly + ty * lz);
+  const int blk = (lx >> 2) + (tx >> 2) * ((ly >> 2) + (ty >> 2) * (lz >> 1));
+  return 32 * blk + (lx & 3) + 4 * (ly & 3) + 16 * (lz & 1);
+}
+// inverse of cell_key over a full tile
+__host__ __device__ inline void key_cell(int key, int dims, int tx, int ty, int tz, int& lx, int& ly,
+                                         int& lz) {
+  if (!blocked_tile(dims, tx, ty, tz)) {
+    lx = key % tx;
+    ly = (key / tx) % ty;
+    lz = key / (tx * ty);
+    return;
+  }
+  const int blk = key >> 5, w = key & 31;
+  const int bx = blk % (tx >> 2), by = (blk / (tx >> 2)) % (ty >> 2), bz = blk / ((tx >> 2) * (ty >> 2));
+  lx = 4 * bx + (w & 3);
+  ly = 4 * by + ((w >> 2) & 3);
+  lz = 2 * bz + (w >> 4);
+}
+
 }  // namespace osr

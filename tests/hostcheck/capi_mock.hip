@@ -82,6 +82,7 @@ void osr_mock_exec(const void* k, dim3 g, const SpecDesc* d, const int2* w) {
       if (c->halt == 0 && U(g_rng) < g_mock_overflow_p) {
         c->halt = 1;
         d[i].inc[0] = -1;
+        d[i].newcnt[0] = d[i].cnt[0];  // capacity would help: the device re-layout runs
       }
     }
   }

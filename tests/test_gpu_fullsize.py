@@ -70,13 +70,15 @@ def test_full_size_invariants(name, make, tiles, steps):
 
 @pytest.mark.parametrize("xbnd", ["lindman", "vpml"])
 def test_laser_plasma_2048x512_full_size(xbnd):
-    """The laser-plasma workload at full size (2048 x 512 cells, 8 M cold electrons behind a
-    ramp, a0 = 2 laser from the lower x wall, Lindman or VPML walls, absorbing particles),
-    300 steps: a repeated run is bit-identical, and another tile shape (guard band and
-    mover traffic differ) gives identical fields, surviving particles and counts."""
+    """The laser-plasma workload at full size (2048 x 512 cells, 8 M cold electrons at
+    0.01 n_c behind a 100-cell ramp from the x-min wall, a0 = 2 laser with omega0 = 1 from
+    that wall, Lindman or VPML walls, absorbing particles), all 1500 steps of the job: a
+    repeated run is bit-identical, and another tile shape (guard band and mover traffic
+    differ) gives identical fields, surviving particles and counts."""
     from bench import C4, c4_particles
 
-    ix, x, u, q = c4_particles((2048, 512), 100, 100, C4["ppc"], C4["density"])
+    ix, x, u, q = c4_particles((2048, 512), C4["gap_cells"], C4["ramp_cells"], C4["ppc"],
+                               C4["density"])
     out = []
     for tile in ((32, 16), (32, 16), (16, 16)):
         print(f"case c4 xbnd={xbnd} tile={tile}", flush=True)
@@ -84,7 +86,7 @@ def test_laser_plasma_2048x512_full_size(xbnd):
             sid = s.add_species(-1.0)
             s.set_particles(sid, ix, x, u, q)
             s.add_zpulse_wall(**C4["laser"])
-            s.run(300)
+            s.run(1500)
             out.append((s.get_fields(gcl=(2, 2), gcu=(3, 3)), s.get_particles(sid), s.count(sid),
                         s.history()))
     (f0, p0, n0, h0), (f1, p1, n1, h1), (f2, p2, n2, h2) = out

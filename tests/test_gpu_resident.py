@@ -302,14 +302,15 @@ def test_tile_shape_and_kernel_independence():
 
 
 def test_tile_shape_independence_3d():
-    """3D (cubic, Vay, magnetized): particles and fields do not depend on the tile shape;
-    the current is summed in integers whatever order the queued virtual particles are
-    deposited in."""
+    """3D (cubic, Vay, magnetized): particles and fields do not depend on the tile shape or
+    the kernel variant (compile-time 8x8x4 tile with padded LDS pitches and blocked cell
+    order, or the generic runtime-shape kernel); the current is summed in integers
+    whatever order the queued virtual particles are deposited in."""
     nx, dx, dt = (24, 16, 16), (0.1, 0.1, 0.1), 0.05
     out = []
-    for tile in ((8, 8, 4), (4, 4, 4), (8, 4, 8)):
-        print(f"case tile={tile}", flush=True)
-        with oa.Simulation(3, nx, dx, dt, 3, tile=tile) as sim:
+    for tile, kernel in (((8, 8, 4), 0), ((8, 8, 4), 1), ((4, 4, 4), 0), ((8, 4, 8), 0)):
+        print(f"case tile={tile} kernel={kernel}", flush=True)
+        with oa.Simulation(3, nx, dx, dt, 3, tile=tile, kernel=kernel) as sim:
             sid = sim.add_species(-1.0, oa.OSR_VAY)
             sim.generate(sid, (2, 2, 2), density=1.0, uth=(0.1,) * 3, seed=5)
             sim.fill_fields(b0=(0.0, 0.0, 0.5))
