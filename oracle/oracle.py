@@ -77,6 +77,8 @@ def _lib(prec: str):
         L.orc_es_solver_2d.argtypes = [P, P, P, P]
         L.orc_update_periodic_from.argtypes = [P, P, C.c_int, C.c_int, C.c_int]
         L.orc_es_solver_2d_beam.argtypes = [P, P, P, C.c_float, C.c_int, P, P]
+        L.orc_es_solver_3d_beam.argtypes = [P, P, P, C.c_float, C.c_int, P, P]
+        L.orc_zpulse_box_add.argtypes = [P, P, C.c_int, P, P, P]
         L._real = np.float64 if prec == "f64" else np.float32
         _LIBS[prec] = L
     return _LIBS[prec]
@@ -235,6 +237,90 @@ def zpulse_wall_params(nx, dx, xmin=(0.0, 0.0, 0.0), **kw):
     if p.backward:
         p.focus = -p.focus
     return p
+
+
+class ZpulseBoxParams(C.Structure):
+    """orc_zpulse_box of pic_oracle.c (parameters after read_input_zpulse)."""
+    _fields_ = [("a0", C.c_double), ("omega0", C.c_double), ("phase0", C.c_double),
+                ("pol", C.c_double), ("pol_type", C.c_int), ("backward", C.c_int),
+                ("lon_type", C.c_int), ("per_type", C.c_int), ("tem_mode", C.c_int * 2),
+                ("div_corr", C.c_int), ("lon_start", C.c_double), ("rise", C.c_double),
+                ("flat", C.c_double), ("fall", C.c_double), ("lon_x0", C.c_double),
+                ("duration", C.c_double), ("range", C.c_double), ("center", C.c_double * 2),
+                ("w0", C.c_double), ("focus", C.c_double), ("xmin", C.c_double * 3)]
+
+
+ZPULSE_BOX_DEFAULTS = dict(a0=1.0, omega0=10.0, phase=0.0, pol_type=0, pol=90.0,
+                           propagation="forward", lon_type="polynomial", lon_start=None,
+                           lon_fwhm=-1.0, lon_rise=0.0, lon_flat=0.0, lon_fall=0.0,
+                           lon_duration=0.0, lon_x0=None, lon_range=None, per_type="plane",
+                           per_center=None, per_w0=0.0, per_fwhm=0.0, per_focus=None,
+                           per_tem_mode=0, no_div_corr=False)
+
+
+def zpulse_box_params(nx, dx, xmin=(0.0, 0.0, 0.0), **kw):
+    """read_input_zpulse (zpulse/os-zpulse-std.f03:157-805) for an in-box pulse along x1
+    of a 2D or 3D box (len(nx)) without boost, tilt or chirp: the reference's defaults
+    (:246-302) updated by kw; plane pulses get no divergence correction (init_zpulse
+    :884-887).  Returns ZpulseBoxParams."""
+    unknown = set(kw) - set(ZPULSE_BOX_DEFAULTS)
+    if unknown:
+        raise ValueError(f"unknown zpulse parameters {sorted(unknown)}")
+    a = dict(ZPULSE_BOX_DEFAULTS, **kw)
+    pi_180 = 0.0174532925199432957692369  # os-math.f90:15
+    p = ZpulseBoxParams()
+    p.a0, p.omega0 = a["a0"], a["omega0"]
+    p.phase0 = a["phase"] * pi_180
+    if a["pol_type"] not in (-1, 0, 1):
+        raise ValueError("pol_type must be -1, 0 or 1")
+    p.pol_type = a["pol_type"]
+    p.pol = a["pol"] * pi_180
+    p.backward = {"forward": 0, "backward": 1}[a["propagation"]]
+    dims = len(nx)
+    xmax = [0.0] * 3
+    for d in range(dims):
+        p.xmin[d] = xmin[d] if d < len(xmin) else 0.0
+        xmax[d] = p.xmin[d] + nx[d] * dx[d]
+    p.lon_type = {"polynomial": 0, "sin2": 1, "gaussian": 2}[a["lon_type"]]
+    if p.lon_type < 2:
+        if a["lon_fwhm"] > 0:
+            p.rise, p.flat, p.fall = a["lon_fwhm"] / 2.0, 0.0, a["lon_fwhm"] / 2.0
+        else:
+            p.rise, p.flat, p.fall = a["lon_rise"], a["lon_flat"], a["lon_fall"]
+        if a["lon_start"] is None:
+            raise ValueError("polynomial and sin2 envelopes need lon_start")
+        p.lon_start = a["lon_start"]
+    else:
+        p.duration = (a["lon_fwhm"] / np.sqrt(2.0 * np.log(2.0)) if a["lon_fwhm"] > 0
+                      else a["lon_duration"])
+        if a["lon_range"] is None or a["lon_x0"] is None:
+            raise ValueError("a gaussian envelope needs lon_range and lon_x0")
+        p.range, p.lon_x0 = a["lon_range"], a["lon_x0"]
+    p.per_type = {"plane": 0, "gaussian": 1, "hermite": 1}[a["per_type"]]
+    if p.per_type == 1:
+        modes = np.atleast_1d(a["per_tem_mode"])
+        for d in range(dims - 1):
+            p.tem_mode[d] = int(modes[d]) if d < len(modes) else 0
+        p.w0 = a["per_fwhm"] / np.sqrt(4.0 * np.log(2.0)) if a["per_fwhm"] > 0 else a["per_w0"]
+        if p.w0 <= 0 or a["per_focus"] is None:
+            raise ValueError("a gaussian beam needs per_w0 (or per_fwhm) and per_focus")
+        p.focus = a["per_focus"]
+    cen = a["per_center"]
+    cen = [None] * (dims - 1) if cen is None else list(np.atleast_1d(cen))
+    for d in range(dims - 1):
+        c = cen[d] if d < len(cen) else None
+        p.center[d] = 0.5 * (p.xmin[d + 1] + xmax[d + 1]) if c is None else c
+    p.div_corr = 0 if (p.per_type == 0 or a["no_div_corr"]) else 1
+    return p
+
+
+def zpulse_box_add(grid: "Grid", p, order, dx, e, b, prec="f64"):
+    """launch_zpulse (os-zpulse-std.f03:2350-2469) of an in-box pulse: adds the pulse
+    (divergence-corrected unless plane or no_div_corr) to the guarded grids e, b."""
+    L = _lib(prec)
+    g = grid.c()
+    dxa = np.array((list(dx) + [1.0, 1.0])[:3], np.float64)
+    L.orc_zpulse_box_add(C.byref(g), C.byref(p), int(order), _p(dxa), _p(e), _p(b))
 
 
 def ref_available(prec: str = "s") -> bool:
@@ -648,28 +734,36 @@ class Sim:
 
     def init_fields(self, k, ufl1=0.0):
         """init_fields of species k (spec/os-spec-initfields.f90:38-100): rho deposited
-        and folded; ufl1 = 0: the 2D direct-sum ES field added to E; otherwise the beam
-        field of a species drifting along x with u1 = ufl1 (es_solver_2d_beam) added to E
-        and B."""
-        if self.dims != 2:
-            raise ValueError("the reference's ES solver is 2D only")
+        and folded; ufl1 = 0: the 2D direct-sum ES field added to E (the reference has no
+        3D form); otherwise the beam field of a species drifting along x with u1 = ufl1
+        (es_solver_2d_beam, es_solver_3d_beam) added to E and B."""
+        if self.dims != 2 and ufl1 == 0:
+            raise ValueError("the reference's ES solver (ufl(1) = 0) is 2D only")
         sp = self.species[k]
         rho = deposit_rho(self.grid, self.order, sp.ix, sp.x, sp.q, self.prec)
         L = _lib(self.prec)
         g = self.grid.c()
         L.orc_update_periodic_from(C.byref(g), _p(rho), 1, 1, 1 if self.open_x else 0)
         ef = self.grid.zeros(3, self.real)
-        dxa = np.array(list(self.dx) + [1.0], np.float64)
+        dxa = np.array((list(self.dx) + [1.0, 1.0])[:3], np.float64)
         if ufl1 == 0:
             L.orc_es_solver_2d(C.byref(g), _p(dxa), _p(rho), _p(ef))
             self.grid.interior(self.e)[...] += self.grid.interior(ef)
             return ef
         bf = self.grid.zeros(3, self.real)
-        L.orc_es_solver_2d_beam(C.byref(g), _p(dxa), _p(rho), float(np.float32(ufl1)),
-                                int(self.open_x), _p(ef), _p(bf))
+        beam = L.orc_es_solver_2d_beam if self.dims == 2 else L.orc_es_solver_3d_beam
+        beam(C.byref(g), _p(dxa), _p(rho), float(np.float32(ufl1)), int(self.open_x), _p(ef),
+             _p(bf))
         self.e += ef  # add(emf%e, ef), add(emf%b, bf): every cell
         self.b += bf
         return ef, bf
+
+    def add_zpulse(self, xmin=(0.0, 0.0, 0.0), **kw):
+        """An in-box zpulse section (direction 1) launched now, before the first step:
+        its field is added to E and B."""
+        p = zpulse_box_params(self.nx, self.dx, xmin, **kw)
+        zpulse_box_add(self.grid, p, self.order, self.dx, self.e, self.b, self.prec)
+        return p
 
     def add_zpulse_wall(self, xmin=(0.0, 0.0, 0.0), **kw):
         """A zpulse_wall section (direction 1); needs open x walls (os-zpulse-wall.f03:

@@ -1800,3 +1800,257 @@ void orc_es_solver_2d_beam(const orc_grid *g, const double *dx, const real *rho,
     ef[3 * c + 2] = 0;
   }
 }
+
+/* es_solver_3d_beam (emf/os-emf-es-solver.f03:795-1003), one node, the 3D branch of
+ * es_solver_beam (:240-287) that init_fields takes for a species drifting along x
+ * (spec/os-spec-initfields.f90:77-92): each source cell (ri1, ri2, ri3) with nonzero
+ * charge adds, on its own x slice ei1 = ri1 and for every (ei2, ei3) of the array
+ * (guards included; sources in the order ri3, ri2, ri1 outer to inner), the field of an
+ * infinite wire, rho * charge_norm * d / r^2 with charge_norm = dx2 dx3 / gamma / 2 / pi:
+ * E2 at (y + 1/2, z) with d = (ei2 - ri2 + 1/2) dx2, r^2 = d^2 + ((ei3 - ri3) dx3)^2, and
+ * E3 at (y, z + 1/2) symmetrically; each term in FP64, accumulated in the working
+ * precision.  After the E guard refresh: B1 = 0, B2 = -0.5 u1 (E3(i) + E3(i+1)),
+ * B3 = 0.5 u1 (E2(i) + E2(i+1)) (the sums in the working precision); then E2, E3 *= gamma,
+ * E1 = 0.  gamma = sqrt(u1^2 + 1) with u1 squared in the particle precision (float).
+ * open_x: the guards along x are not refreshed (E beyond the upper wall stays zero).
+ * ef, bf are three-component grids (overwritten). */
+void orc_es_solver_3d_beam(const orc_grid *g, const double *dx, const real *rho, float u1,
+                           int open_x, real *ef, real *bf) {
+  const int nx = g->nx[0], ny = g->nx[1], nz = g->nx[2];
+  const long nc = orc_grid_cells(g);
+  const double pi = 3.14159265358979323846;
+  memset(ef, 0, sizeof(real) * 3 * nc);
+  memset(bf, 0, sizeof(real) * 3 * nc);
+  float u1sq = u1 * u1;
+  double gamma = sqrt((double)u1sq + 1.0);
+  double charge_norm = dx[1] * dx[2] / gamma / 2 / pi;
+  for (int ri3 = 1; ri3 <= nz; ri3++)
+    for (int ri2 = 1; ri2 <= ny; ri2++)
+      for (int ri1 = 1; ri1 <= nx; ri1++) {
+        real r = rho[cidx(g, ri1, ri2, ri3)];
+        if (r == 0) continue;
+        for (int ei3 = 1 - g->gcl[2]; ei3 <= nz + g->gcu[2]; ei3++) {
+          double dx3c = (double)(ei3 - ri3);
+          double dx3s = dx3c * dx[2];
+          double dx3s_h = (dx3c + 0.5) * dx[2];
+          for (int ei2 = 1 - g->gcl[1]; ei2 <= ny + g->gcu[1]; ei2++) {
+            double dx2c = (double)(ei2 - ri2);
+            double dx2s = dx2c * dx[1];
+            double dx2s_h = (dx2c + 0.5) * dx[1];
+            real *e = &FLD(ef, g, 0, ri1, ei2, ei3);
+            double r_2 = dx2s_h * dx2s_h + dx3s * dx3s;
+            e[1] = (real)((double)e[1] + (double)r * charge_norm * dx2s_h / r_2);
+            r_2 = dx2s * dx2s + dx3s_h * dx3s_h;
+            e[2] = (real)((double)e[2] + (double)r * charge_norm * dx3s_h / r_2);
+          }
+        }
+      }
+  update_periodic_from(g, ef, 3, 0, open_x ? 1 : 0);
+  for (int ei3 = 1 - g->gcl[2]; ei3 <= nz + g->gcu[2]; ei3++)
+    for (int ei2 = 1 - g->gcl[1]; ei2 <= ny + g->gcu[1]; ei2++)
+      for (int ei1 = 1; ei1 <= nx; ei1++) {
+        real s3 = FLD(ef, g, 2, ei1, ei2, ei3) + FLD(ef, g, 2, ei1 + 1, ei2, ei3);
+        real s2 = FLD(ef, g, 1, ei1, ei2, ei3) + FLD(ef, g, 1, ei1 + 1, ei2, ei3);
+        FLD(bf, g, 0, ei1, ei2, ei3) = 0;
+        FLD(bf, g, 1, ei1, ei2, ei3) = (real)(-(0.5 * (double)u1 * (double)s3));
+        FLD(bf, g, 2, ei1, ei2, ei3) = (real)(0.5 * (double)u1 * (double)s2);
+      }
+  update_periodic_from(g, bf, 3, 0, open_x ? 1 : 0);
+  for (long c = 0; c < nc; c++) {
+    ef[3 * c] = 0;
+    ef[3 * c + 1] = (real)((double)ef[3 * c + 1] * gamma);
+    ef[3 * c + 2] = (real)((double)ef[3 * c + 2] * gamma);
+  }
+}
+
+/* ------------------------------------------------------------------------------------
+ * In-box laser pulse (zpulse, zpulse/os-zpulse-std.f03), launched once into the fields
+ * along x1 (direction 1), without boost, tilt or chirp, one node.  orc_zpulse_box holds
+ * the parameters after read_input_zpulse (:157-805): angles in radians, envelope lengths
+ * resolved from fwhm, per_center defaulted to the box centre; lon_type 0 polynomial,
+ * 1 sin2, 2 gaussian; per_type 0 plane, 1 hermite_gaussian; div_corr = 0 for plane
+ * pulses or no_div_corr (init_zpulse :867-929).
+ * orc_zpulse_box_add is launch_zpulse (:2350-2469): the pulse field is built in zeroed
+ * grids (launch_zpulse_2d :1072-1212 / 3d :1218-1387), its divergence corrected
+ * (div_corr_zpulse_2d :1681-1835 / 3d :2010-2205) and added to E and B (every cell);
+ * for pol_type != 0 a second pulse with phase0 + sign(pi/2, pol_type) and pol + pi/2 is
+ * built and added the same way.  order: the interpolation order (2 shifts the grid
+ * origin by half a cell, :1107-1113).  e, b: three-component grids.
+ * ------------------------------------------------------------------------------------ */
+typedef struct {
+  double a0, omega0, phase0, pol;
+  int pol_type, backward, lon_type, per_type, tem_mode[2], div_corr;
+  double lon_start, rise, flat, fall, lon_x0, duration, range;
+  double center[2], w0, focus;
+  double xmin[3];
+} orc_zpulse_box;
+
+/* lon_envelope_zpulse :2480-2561 (t ignored without boost) */
+static double zb_lon(const orc_zpulse_box *p, double z) {
+  if (p->lon_type == 0)
+    return p->backward ? zp_poly(z - p->lon_start, p->rise, p->flat, p->fall)
+                       : zp_poly(p->lon_start - z, p->rise, p->flat, p->fall);
+  if (p->lon_type == 1)
+    return p->backward ? zp_sin2(z - p->lon_start, p->rise, p->flat, p->fall)
+                       : zp_sin2(p->lon_start - z, p->rise, p->flat, p->fall);
+  if (fabs(z - p->lon_x0) > p->range / 2) return 0.0;
+  double a = (z - p->lon_x0) / p->duration;
+  return exp(-2 * (a * a));
+}
+/* lon_center_zpulse :3055-3097 */
+static double zb_center(const orc_zpulse_box *p) {
+  if (p->lon_type == 2) return p->lon_x0;
+  return p->backward ? p->lon_start + p->rise + p->flat / 2.0
+                     : p->lon_start - p->rise - p->flat / 2.0;
+}
+/* per_envelope_2d_zpulse :2599-2753 at t = 0 (plane, hermite_gaussian) */
+static double zb_per2(const orc_zpulse_box *p, double phase0, double x1, double x2) {
+  double zc = zb_center(p), k = p->omega0, ps = p->backward ? -1.0 : 1.0, t = 0.0;
+  if (p->per_type == 0) return cos(k * (x1 - zc - ps * t) + phase0);
+  double z0 = k * (p->w0 * p->w0) * 0.5;
+  double rho = x2 - p->center[0], rho2 = rho * rho, z = x1 - p->focus;
+  double rwl2 = 1, curv = 0, gouy = 0;
+  if (z != 0) {
+    rwl2 = (z0 * z0) / ((z0 * z0) + (z * z));
+    curv = 0.5 * rho2 * z / ((z * z) + (z0 * z0));
+    gouy = (p->tem_mode[0] + 1) * atan2(z, z0);
+  }
+  double rwu = sqrt(2 * rwl2) / p->w0;
+  return sqrt(sqrt(rwl2)) * zp_hermite(p->tem_mode[0], rho * rwu) *
+         exp(-(rho2 * rwl2 / (p->w0 * p->w0))) *
+         cos(k * (x1 - zc - ps * t) + k * curv - gouy + phase0);
+}
+/* per_envelope_3d_zpulse :2760-3049 at t = 0 (plane, hermite_gaussian) */
+static double zb_per3(const orc_zpulse_box *p, double phase0, double x1, double x2, double x3) {
+  double zc = zb_center(p), k = p->omega0, ps = p->backward ? -1.0 : 1.0, t = 0.0;
+  double ox = x2 - p->center[0], oy = x3 - p->center[1];
+  if (p->per_type == 0) return cos(k * (x1 - zc - ps * t) + phase0);
+  double rho2 = ox * ox + oy * oy, z = x1 - p->focus, z0 = k * (p->w0 * p->w0) / 2;
+  double rwl2 = 1, curv = 0, gouy = 0;
+  if (z != 0) {
+    rwl2 = (z0 * z0) / ((z0 * z0) + (z * z));
+    curv = 0.5 * rho2 * z / ((z * z) + (z0 * z0));
+    gouy = (p->tem_mode[0] + p->tem_mode[1] + 1) * atan2(z, z0);
+  }
+  double rwu = sqrt(2 * rwl2) / p->w0;
+  return sqrt(rwl2) * zp_hermite(p->tem_mode[0], ox * rwu) * zp_hermite(p->tem_mode[1], oy * rwu) *
+         exp(-(rho2 * rwl2 / (p->w0 * p->w0))) *
+         cos(k * (x1 - zc - ps * t) + k * curv - gouy + phase0);
+}
+
+/* one pulse (phase0, pol) into zeroed grids eb (E) and bb (B) */
+static void zb_build(const orc_grid *g, const orc_zpulse_box *p, int order, const double *dx,
+                     double phase0, double pol, real *eb, real *bb) {
+  const int D = g->dims;
+  int lo[3] = {1, 1, 1}, hi[3] = {1, 1, 1};
+  for (int d = 0; d < D; d++) {
+    lo[d] = 1 - g->gcl[d];
+    hi[d] = g->nx[d] + g->gcu[d];
+  }
+  double cos_pol = cos(pol), sin_pol = sin(pol), amp = p->omega0 * p->a0;
+  double ps = p->backward ? -1.0 : 1.0;
+  double zmin = p->xmin[0], rmin = p->xmin[1], r2min = D == 3 ? p->xmin[2] : 0.0;
+  double dz_2 = dx[0] / 2.0, dr_2 = dx[1] / 2.0, dr2_2 = D == 3 ? dx[2] / 2.0 : 0.0;
+  if (order == 2) {
+    zmin = zmin + dz_2;
+    rmin = rmin + dr_2;
+    r2min = r2min + dr2_2;
+  }
+  for (int i1 = lo[0]; i1 <= hi[0]; i1++) {
+    double z = zmin + (double)(i1 - 1) * dx[0], z_2 = z + dz_2;
+    double lenv = amp * zb_lon(p, z), lenv_2 = amp * zb_lon(p, z_2);
+    if (D == 2) {
+      for (int i2 = lo[1]; i2 <= hi[1]; i2++) {
+        double r = rmin + (double)(i2 - 1) * dx[1], r_2 = r + dr_2;
+        FLD(bb, g, 0, i1, i2, 1) = 0;
+        FLD(bb, g, 1, i1, i2, 1) = (real)(-(lenv_2 * zb_per2(p, phase0, z_2, r) * sin_pol * ps));
+        FLD(bb, g, 2, i1, i2, 1) = (real)(+(lenv_2 * zb_per2(p, phase0, z_2, r_2) * cos_pol * ps));
+        FLD(eb, g, 0, i1, i2, 1) = 0;
+        FLD(eb, g, 1, i1, i2, 1) = (real)(lenv * zb_per2(p, phase0, z, r_2) * cos_pol);
+        FLD(eb, g, 2, i1, i2, 1) = (real)(lenv * zb_per2(p, phase0, z, r) * sin_pol);
+      }
+      continue;
+    }
+    for (int i2 = lo[1]; i2 <= hi[1]; i2++) {
+      double r1 = rmin + (double)(i2 - 1) * dx[1], r1_2 = r1 + dr_2;
+      for (int i3 = lo[2]; i3 <= hi[2]; i3++) {
+        double r2 = r2min + (double)(i3 - 1) * dx[2], r2_2 = r2 + dr2_2;
+        FLD(bb, g, 1, i1, i2, i3) = FLD(bb, g, 1, i1, i2, i3) -
+                                    (real)(lenv_2 * zb_per3(p, phase0, z_2, r1, r2_2) * sin_pol * ps);
+        FLD(bb, g, 2, i1, i2, i3) = FLD(bb, g, 2, i1, i2, i3) +
+                                    (real)(lenv_2 * zb_per3(p, phase0, z_2, r1_2, r2) * cos_pol * ps);
+        FLD(eb, g, 1, i1, i2, i3) = FLD(eb, g, 1, i1, i2, i3) +
+                                    (real)(lenv * zb_per3(p, phase0, z, r1_2, r2) * cos_pol);
+        FLD(eb, g, 2, i1, i2, i3) = FLD(eb, g, 2, i1, i2, i3) +
+                                    (real)(lenv * zb_per3(p, phase0, z, r1, r2_2) * sin_pol);
+      }
+    }
+  }
+  if (!p->div_corr) return;
+  /* div_corr_zpulse_2d / 3d: E1 and B1 integrated from the upper x edge (one node: the
+   * loop over the other nodes' x range is empty and e1 = b1 = 0 at x = nx + 1) */
+  double x1min = p->xmin[0], x2min = p->xmin[1], x3min = D == 3 ? p->xmin[2] : 0.0;
+  (void)x2min;
+  (void)x3min;
+  if (order == 2) {
+    x1min = x1min + dx[0] * 0.5;
+    x2min = x2min + dx[1] * 0.5;
+    if (D == 3) x3min = x3min + dx[2] * 0.5;
+  }
+  const int n3 = D == 3 ? g->nx[2] : 1;
+  for (int i3 = 1; i3 <= n3; i3++)
+    for (int i2 = 1; i2 <= g->nx[1]; i2++) {
+      double e1 = 0.0, b1 = 0.0;
+      FLD(eb, g, 0, g->nx[0] + 1, i2, i3) = (real)e1;
+      FLD(bb, g, 0, g->nx[0] + 1, i2, i3) = (real)b1;
+      for (int i1 = g->nx[0]; i1 >= 1 - g->gcl[0]; i1--) {
+        if (zb_lon(p, x1min + i1 * dx[0]) > 0.0) {
+          double e2p = FLD(eb, g, 1, i1 + 1, i2, i3), e2m = FLD(eb, g, 1, i1 + 1, i2 - 1, i3);
+          if (D == 2) {
+            e1 = e1 + dx[0] * (e2p - e2m) / dx[1];
+          } else {
+            double e3p = FLD(eb, g, 2, i1 + 1, i2, i3), e3m = FLD(eb, g, 2, i1 + 1, i2, i3 - 1);
+            e1 = e1 + dx[0] * ((e2p - e2m) / dx[1] + (e3p - e3m) / dx[2]);
+          }
+          FLD(eb, g, 0, i1, i2, i3) = (real)e1;
+        } else {
+          e1 = 0.0;
+        }
+        if (zb_lon(p, x1min + (i1 - 0.5) * dx[0]) > 0.0) {
+          double b2p = FLD(bb, g, 1, i1, i2 + 1, i3), b2m = FLD(bb, g, 1, i1, i2, i3);
+          if (D == 2) {
+            b1 = b1 + dx[0] * (b2p - b2m) / dx[1];
+          } else {
+            double b3p = FLD(bb, g, 2, i1, i2, i3 + 1), b3m = FLD(bb, g, 2, i1, i2, i3);
+            b1 = b1 + dx[0] * ((b2p - b2m) / dx[1] + (b3p - b3m) / dx[2]);
+          }
+          FLD(bb, g, 0, i1, i2, i3) = (real)b1;
+        } else {
+          b1 = 0.0;
+        }
+      }
+    }
+}
+
+void orc_zpulse_box_add(const orc_grid *g, const orc_zpulse_box *p, int order, const double *dx,
+                        real *e, real *b) {
+  const long nc = orc_grid_cells(g);
+  real *eb = (real *)calloc((size_t)(3 * nc), sizeof(real));
+  real *bb = (real *)calloc((size_t)(3 * nc), sizeof(real));
+  double phase0 = p->phase0, pol = p->pol;
+  for (int k = 0; k < (p->pol_type != 0 ? 2 : 1); k++) {
+    if (k == 1) {
+      memset(eb, 0, sizeof(real) * 3 * nc);
+      memset(bb, 0, sizeof(real) * 3 * nc);
+      phase0 = phase0 + (p->pol_type > 0 ? ORC_PI_2 : -ORC_PI_2);
+      pol = pol + ORC_PI_2;
+    }
+    zb_build(g, p, order, dx, phase0, pol, eb, bb);
+    for (long c = 0; c < 3 * nc; c++) {
+      b[c] = b[c] + bb[c];
+      e[c] = e[c] + eb[c];
+    }
+  }
+  free(eb);
+  free(bb);
+}

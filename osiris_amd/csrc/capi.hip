@@ -691,9 +691,9 @@ struct FillKernel {
 };
 FillKernel pick_fill() { return {tile_fill<256, 4096, 32768>, 256}; }
 
-// Device re-layout of species `sid` after a halted step (a tile fill overflowed): the
-// tiles that overflowed get their count after the step plus half of it again as
-// capacity (no tile shrinks), every tile's particles and pending mover lists move to the
+// Device re-layout of species `sid` after a halted step (a tile fill overflowed): every
+// tile gets its count after the step plus half of it again as capacity (no tile
+// shrinks), every tile's particles and pending mover lists move to the
 // new arrays on the device (regrow_move), and the fill of the overflowed tiles runs again
 // there.  *ok = false when that cannot finish the step (a tile's incoming or tail lists
 // exceed the fill kernel's LDS tables, or slots would exceed int32): the species is then
@@ -719,11 +719,10 @@ int regrow_species(osr_sim* s, int sid, bool* ok) {
   std::vector<int> cap(nt), off(nt);
   int64_t total = 0;
   for (int t = 0; t < nt; t++) {
-    int64_t c = p.h_cap[t];
-    if (inc[t] < 0) {
-      const int64_t r = need[t];
-      c = std::max<int64_t>(c, r + std::max<int64_t>(r / 2, 64) + 64 * kMaxPushWaves);
-    }
+    // every tile gets room for half its count again (a density front that overflows one
+    // tile usually reaches its neighbours next); no tile shrinks
+    const int64_t r = inc[t] < 0 ? need[t] : cnt[t];
+    int64_t c = std::max<int64_t>(p.h_cap[t], r + std::max<int64_t>(r / 2, 64) + 64 * kMaxPushWaves);
     c = (c + 63) / 64 * 64;
     if (total + c > INT32_MAX) return OSR_OK;
     cap[t] = (int)c;
@@ -1726,11 +1725,13 @@ int osr_sim_gauss(osr_sim* s, double* res, double* rho_out, double* maxabs) {
 
 int osr_sim_init_fields(osr_sim* s, int sid, float ufl1) {
   TRY(check_sim(s));
-  if (s->dims != 2) return fail(OSR_ERR_ARG, "the electrostatic solver is 2D only");
+  if (s->dims != 2 && ufl1 == 0.0f)
+    return fail(OSR_ERR_ARG, "the electrostatic solver of a species at rest is 2D only (the "
+                             "reference's es_solver has no 3D form)");
   if (sid < 0 || sid >= (int)s->sp.size()) return fail(OSR_ERR_ARG, "bad species id");
   if (s->plain) return fail(OSR_ERR_STATE, "init_fields needs the fused path");
   const size_t nc = (size_t)s->g.ncell;
-  const long long nt = (long long)interior_nx(s->g) * s->g.ny;
+  const long long nt = (long long)interior_nx(s->g) * s->g.ny * s->g.nz;
   DBuf rho, charge;
   TRY(rho.alloc(8 * nc));
   TRY(charge.alloc(8 * (size_t)nt));
@@ -1750,6 +1751,20 @@ int osr_sim_init_fields(osr_sim* s, int sid, float ufl1) {
   if (ufl1 == 0.0f) {  // es_solver
     hipLaunchKernelGGL(es_direct_2d, dim3(nb), dim3(256), 0, s->stream, s->g, charge.as<double>(),
                        s->dx[0], s->dx[1], s->dx[0] * s->dx[1]);
+  } else if (s->dims == 3) {  // es_solver_3d_beam
+    const float u1sq = ufl1 * ufl1;
+    const double gamma = std::sqrt((double)u1sq + 1.0);
+    const double cn = s->dx[1] * s->dx[2] / gamma / 2 / 3.14159265358979323846;
+    DBuf e23u;
+    TRY(e23u.alloc(8 * (size_t)nt));
+    const long long ns = (long long)s->g.ny * s->g.nz;
+    hipLaunchKernelGGL(es_beam3d_e, dim3((unsigned)((ns + 255) / 256), (unsigned)interior_nx(s->g)),
+                       dim3(256), 0, s->stream, s->g, charge.as<double>(), cn, s->dx[1], s->dx[2],
+                       e23u.as<float>());
+    hipLaunchKernelGGL(es_beam3d_apply, dim3(nb), dim3(256), 0, s->stream, s->g, e23u.as<float>(),
+                       ufl1, gamma);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(s->stream));
   } else {  // es_solver_beam: gamma from u1^2 in the particle precision
     const float u1sq = ufl1 * ufl1;
     const double gamma = std::sqrt((double)u1sq + 1.0);

@@ -745,8 +745,8 @@ __global__ __launch_bounds__(256) void es_charge(GridDesc g, const unsigned long
                                                  double inv, double* charge) {
   const int nxi = interior_nx(g);
   const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (t >= (long long)nxi * g.ny) return;
-  const long long c = (t / nxi) * g.nx + t % nxi;
+  if (t >= (long long)nxi * g.ny * g.nz) return;
+  const long long c = (t / nxi) * g.nx + t % nxi;  // rows y + ny z of nx cells
   charge[t] = (double)(float)((double)(long long)rho[c] * inv);
 }
 
@@ -842,6 +842,90 @@ __global__ __launch_bounds__(256) void es_beam_apply(GridDesc g, const float* __
   g.E[2 * nc + c] = g.E[2 * nc + c] + 0.0f;
   g.B[c] = g.B[c] + 0.0f;
   g.B[nc + c] = g.B[nc + c] + 0.0f;
+  g.B[2 * nc + c] = g.B[2 * nc + c] + b3;
+}
+
+// es_solver_3d_beam (emf/os-emf-es-solver.f03:795-1003), a species drifting along x in
+// 3D: each cell of x slice i gets the field of the infinite wires of that slice's charge,
+// sources in the reference's order (z outer, y inner), each term
+// q * cn * d / r^2 in FP64 and accumulated in FP32 through FP64 sums: E2 at (y + 1/2, z),
+// E3 at (y, z + 1/2).  One workgroup per (slice, 256 targets); the slice's charges are
+// staged through LDS (every lane reads the same source: broadcast reads).  Writes the
+// unboosted E2, E3 to e23u [2][nz][ny][nxi].
+__global__ __launch_bounds__(256) void es_beam3d_e(GridDesc g, const double* __restrict__ charge,
+                                                   double cn, double d2, double d3, float* e23u) {
+  __shared__ double s_q[kEsChunk];
+  const int nxi = interior_nx(g), ny = g.ny, nz = g.nz;
+  const int i = blockIdx.y;
+  const long long ns = (long long)ny * nz;  // sources (and targets) of the slice
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  const bool live = t < ns;
+  const int ej = live ? (int)(t % ny) : 0, ek = live ? (int)(t / ny) : 0;
+  float a2 = 0.f, a3 = 0.f;
+  for (long long c0 = 0; c0 < ns; c0 += kEsChunk) {
+    const int m = (int)(ns - c0 < kEsChunk ? ns - c0 : kEsChunk);
+    __syncthreads();
+    for (int k = threadIdx.x; k < m; k += 256) s_q[k] = charge[(c0 + k) * nxi + i];
+    __syncthreads();
+    if (!live) continue;
+    int rj = (int)(c0 % ny), rk = (int)(c0 / ny);
+    for (int k = 0; k < m; k++) {
+      const double q = s_q[k];
+      if (q != 0) {
+        const double dx3c = (double)(ek - rk);
+        const double dx3s = dx3c * d3;
+        const double dx3s_h = (dx3c + 0.5) * d3;
+        const double dx2c = (double)(ej - rj);
+        const double dx2s = dx2c * d2;
+        const double dx2s_h = (dx2c + 0.5) * d2;
+        double r2 = dx2s_h * dx2s_h + dx3s * dx3s;
+        a2 = (float)((double)a2 + q * cn * dx2s_h / r2);
+        r2 = dx2s * dx2s + dx3s_h * dx3s_h;
+        a3 = (float)((double)a3 + q * cn * dx3s_h / r2);
+      }
+      if (++rj == ny) {
+        rj = 0;
+        rk++;
+      }
+    }
+  }
+  if (!live) return;
+  const long long nt = (long long)nxi * ns, o = t * nxi + i;
+  e23u[o] = a2;
+  e23u[nt + o] = a3;
+}
+
+// then B2 = -0.5 u1 (E3(i) + E3(i+1)), B3 = 0.5 u1 (E2(i) + E2(i+1)) (FP32 sums; the field
+// beyond an open upper wall is zero), E2, E3 *= gamma, E1 = B1 = 0, and the fields are
+// added to E and B (add(emf%e, ef), add(emf%b, bf)); zero components are added too
+__global__ __launch_bounds__(256) void es_beam3d_apply(GridDesc g, const float* __restrict__ e23u,
+                                                       float u1, double gamma) {
+  const int nxi = interior_nx(g);
+  const long long nt = (long long)nxi * g.ny * g.nz;
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= nt) return;
+  const int i = (int)(t % nxi);
+  const long long row = t / nxi;
+  const float e2 = e23u[t], e3 = e23u[nt + t];
+  float e2n, e3n;
+  if (i + 1 < nxi) {
+    e2n = e23u[t + 1];
+    e3n = e23u[nt + t + 1];
+  } else if (g.open_nx) {
+    e2n = e3n = 0.f;
+  } else {  // periodic image of x = 1
+    e2n = e23u[row * nxi];
+    e3n = e23u[nt + row * nxi];
+  }
+  const float s2 = e2 + e2n, s3 = e3 + e3n;
+  const float b2 = (float)(-(0.5 * (double)u1 * (double)s3));
+  const float b3 = (float)(0.5 * (double)u1 * (double)s2);
+  const long long nc = g.ncell, c = row * g.nx + i;
+  g.E[c] = g.E[c] + 0.0f;
+  g.E[nc + c] = g.E[nc + c] + (float)((double)e2 * gamma);
+  g.E[2 * nc + c] = g.E[2 * nc + c] + (float)((double)e3 * gamma);
+  g.B[c] = g.B[c] + 0.0f;
+  g.B[nc + c] = g.B[nc + c] + b2;
   g.B[2 * nc + c] = g.B[2 * nc + c] + b3;
 }
 

@@ -215,9 +215,9 @@ class Simulation:
                                     _p(None if b0 is None else np.array(b0, np.float64))))
 
     def init_fields(self, sid, ufl1=0.0):
-        """Add the initial field of species sid's charge (2D; before the first step):
-        the electrostatic field for ufl1 = 0, the beam field of a species drifting along x
-        with fluid momentum ufl1 otherwise."""
+        """Add the initial field of species sid's charge (before the first step):
+        the electrostatic field for ufl1 = 0 (2D), the beam field of a species drifting
+        along x with fluid momentum ufl1 otherwise (2D and 3D)."""
         check(lib().osr_sim_init_fields(self._h, sid, float(ufl1)))
 
     def add_zpulse_wall(self, xmin=(0.0, 0.0), **kw):

@@ -130,8 +130,70 @@ def test_beam_init_fields_vs_fp64_oracle(xbnd):
     assert np.max(np.abs(fld - rfld) / rfld.sum(1, keepdims=True)) <= 1e-4
 
 
-def test_init_fields_rejects_3d():
+def test_init_fields_rejects_3d_at_rest():
+    """The reference's es_solver has no 3D form for a species at rest (ufl(1) = 0)."""
     with oa.Simulation(3, (8, 8, 8), (0.1,) * 3, 0.05, 2, tile=(8, 8, 4)) as sim:
         sid = sim.add_species(-1.0)
         with pytest.raises(oa._lib.OsrError):
             sim.init_fields(sid)
+
+
+def _plasma3(rng, nx, ppc=8):
+    ix, x, u, q = thermal_plasma(rng, 3, nx, ppc, 0.05)
+    yc = (ix[:, 1] - 0.5 + x[:, 1]) / nx[1]
+    zc = (ix[:, 2] - 0.5 + x[:, 2]) / nx[2]
+    q = (q * (1.0 + 0.4 * np.sin(2 * np.pi * yc) * np.cos(2 * np.pi * zc))).astype(np.float32)
+    return ix, x, u, q
+
+
+def test_beam3d_field_bitexact_given_the_charge():
+    """es_solver_3d_beam: from the device's charge density the FP32 oracle's beam field
+    added to E and B equals the GPU's bit for bit."""
+    rng = np.random.default_rng(15)
+    nx, dx, u1 = (16, 24, 16), (0.1, 0.12, 0.15), 0.8
+    ix, x, u, q = _plasma3(rng, nx)
+    with oa.Simulation(3, nx, dx, 0.04, 2, tile=(8, 8, 4)) as sim:
+        sid = sim.add_species(-1.0)
+        sim.set_particles(sid, ix, x, u, q)
+        rho = sim.charge_density()
+        sim.init_fields(sid, u1)
+        ge, gb = sim.get_fields()
+    g = O.make_grid(3, nx, 2)
+    L = O._lib("f32")
+    r = g.zeros(1, np.float32)
+    g.interior(r)[..., 0] = rho.astype(np.float32)
+    ef = g.zeros(3, np.float32)
+    bf = g.zeros(3, np.float32)
+    L.orc_es_solver_3d_beam(C.byref(g.c()), O._p(np.array(dx, np.float64)), O._p(r),
+                            float(np.float32(u1)), 0, O._p(ef), O._p(bf))
+    we = np.zeros_like(ge) + g.interior(ef)
+    wb = np.zeros_like(gb) + g.interior(bf)
+    assert np.array_equal(ge.view(np.uint32), we.view(np.uint32))
+    assert np.array_equal(gb.view(np.uint32), wb.view(np.uint32))
+    assert np.abs(wb[..., 1]).max() > 0 and np.abs(wb[..., 2]).max() > 0
+
+
+@pytest.mark.parametrize("xbnd", ["periodic", "open"])
+def test_beam3d_init_fields_vs_fp64_oracle(xbnd):
+    rng = np.random.default_rng(16)
+    nx, dx, dt, steps, u1 = (16, 16, 16), (0.1, 0.1, 0.1), 0.04, 12, 0.6
+    ix, x, u, q = _plasma3(rng, nx)
+    u = u.copy()
+    u[:, 0] += u1
+    ref = O.Sim(3, nx, dx, dt, 2, open_x=(xbnd != "periodic"))
+    ref.add_species(-1.0, 0, ix, x, u, q)
+    ref.init_fields(0, u1)
+    with oa.Simulation(3, nx, dx, dt, 2, tile=(8, 8, 4), xbnd=xbnd) as sim:
+        sid = sim.add_species(-1.0)
+        sim.set_particles(sid, ix, x, u, q)
+        sim.init_fields(sid, u1)
+        ge, gb = sim.get_fields()
+        we, wb = ref.grid.interior(ref.e), ref.grid.interior(ref.b)
+        assert np.abs(ge - we).max() <= 1e-5 * np.abs(we).max()
+        assert np.abs(gb - wb).max() <= 1e-5 * np.abs(wb).max()
+        sim.run(steps)
+        ref.step(steps)
+        fld, par = sim.history()
+    rfld, rpar = np.array(ref.fld_hist), np.array(ref.par_hist)
+    assert np.max(np.abs(par - rpar) / np.abs(rpar)) <= 1e-4
+    assert np.max(np.abs(fld - rfld) / rfld.sum(1, keepdims=True)) <= 1e-4
