@@ -76,6 +76,25 @@ int osr_advance_deposit(int dims, int order, int pusher, int *ix, float *x, floa
                         float *j, const int *j_size, const int *j_offset, const double *dx,
                         double dt, double *ene);
 
+/* osr_advance_deposit on arrays already in device memory, with no device allocation.
+ * It has no counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Arguments and results are osr_advance_deposit's, except that ix, x, u, q,
+ * efield, bfield and j are DEVICE pointers (same layouts); emf_size, emf_offset, j_size,
+ * j_offset, dx and ene stay host pointers.  The working memory (int64 fixed-point
+ * current, per-block energy partials, range-check flag) is a fixed 64 MiB device array of
+ * the library, one per GPU for the life of the process: 8192 + 8 * ceil(np / 256) bytes
+ * (rounded up to 256) + 8 * 3 * prod(j_size) must fit in it, else OSR_ERR_NOMEM and
+ * nothing is touched.  Every call on one GPU shares that array, so calls that overlap on
+ * one GPU race and must be serialised.  Kernels run on the null stream; the call returns
+ * after they finished. */
+int osr_advance_deposit_noalloc(int dims, int order, int pusher, int *ix, float *x, float *u,
+                                const float *q, int i0, int i1, float rqm,
+                                const float *efield, const float *bfield,
+                                const int *emf_size, const int *emf_offset, float *j,
+                                const int *j_size, const int *j_offset, const double *dx,
+                                double dt, double *ene);
+
 /* Fortran-callable twins of FNAME(vadvance_deposit_*_sN): every argument by reference,
  * no return value.  On failure they print the error and exit(1), as the reference's
  * kernels do on a violated precondition (simd/os-spec-push-avx2.c:537-547). */

@@ -33,6 +33,7 @@ SIGNATURES = {
     "osr_version": (I, []),
     "osr_device_count": (I, []),
     "osr_advance_deposit": (I, [I, I, I, P, P, P, P, I, I, F, P, P, P, P, P, P, P, P, D, P]),
+    "osr_advance_deposit_noalloc": (I, [I, I, I, P, P, P, P, I, I, F, P, P, P, P, P, P, P, P, D, P]),
     "hip_vadvance_deposit_2d_s1_": (None, [P] * 17),
     "hip_vadvance_deposit_2d_s2_": (None, [P] * 17),
     "hip_vadvance_deposit_2d_s3_": (None, [P] * 17),

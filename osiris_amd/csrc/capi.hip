@@ -2460,3 +2460,4 @@ int osr_sim_set_smooth(osr_sim* s, const int* order, const float* coef) {
 }  // extern "C"
 
 #include "ensemble.hip"
+#include "noalloc.hip"

@@ -5,6 +5,7 @@ import ctypes as C
 import os
 import re
 
+import numpy as np
 import pytest
 
 import osiris_amd as oa
@@ -100,3 +101,21 @@ def test_struct_layouts_match_header(tmp_path):
         assert got[(cname, "size")] == C.sizeof(py), cname
         for f, _ in py._fields_:
             assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
+def test_noalloc_rejects_oversized_current_before_touching_the_gpu():
+    """osr_advance_deposit_noalloc refuses work whose current does not fit its fixed device
+    array, with a status and a message, before any device call (so this runs without a
+    GPU; the pointers are never dereferenced)."""
+    P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    big = np.array([300, 300, 300], np.int32)  # 3 * 300^3 int64 values: more than 64 MiB
+    off = np.array([2, 2, 2], np.int32)
+    dxa = np.array([0.1] * 3, np.float64)
+    ene = np.zeros(1, np.float64)
+    dummy = C.c_void_p(16)
+    st = oa.lib().osr_advance_deposit_noalloc(3, 2, oa.OSR_BORIS, dummy, dummy, dummy, dummy, 1,
+                                              10, -1.0, dummy, dummy, P(big), P(off), dummy,
+                                              P(big), P(off), P(dxa), 0.04, P(ene))
+    assert st == 3  # OSR_ERR_NOMEM
+    assert "more than its fixed" in oa.lib().osr_last_error().decode()
+    assert ene[0] == 0.0
