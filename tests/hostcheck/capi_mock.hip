@@ -141,6 +141,35 @@ void osr_mock_exec(const void*, dim3, const SpecDesc* d, int sid, GenArgs a) {
     }
   }
 }
+// regrow_move: the tile's particle slots and valid mover entries, old layout -> new
+void osr_mock_exec(const void*, dim3 g, const SpecDesc* d, int sid, RegrowArrays src,
+                   RegrowArrays dst, const int* new_off) {
+  mock::check(&d[sid], sizeof(SpecDesc), "regrow desc");
+  const SpecDesc& D = d[sid];
+  check_spec(D);
+  mock::check(new_off, 4 * (size_t)g.x, "regrow offsets");
+  if (!mock::effects()) return;
+  const bool three = D.g.dims == 3;
+  for (unsigned t = 0; t < g.x; t++) {
+    const int o0 = D.off[t], o1 = new_off[t], n = D.cnt[t], NW = D.waves, st = D.mv_stride[t];
+    for (int a = 0; a < 9; a++) {
+      if (a == 3 && !three) continue;
+      mock::check(src.a[a] + o0, 4 * (size_t)n, "regrow src");
+      mock::check(dst.a[a] + o1, 4 * (size_t)n, "regrow dst");
+      memcpy(dst.a[a] + o1, src.a[a] + o0, 4 * (size_t)n);
+    }
+    for (int w = 0; w < NW; w++) {
+      const int m = D.mv_cnt[t * NW + w];
+      for (int a = 9; a < 19; a++) {
+        if (a == 12 && !three) continue;
+        if (!m) continue;
+        mock::check(src.a[a] + o0 + w * st, 4 * (size_t)m, "regrow mover src");
+        mock::check(dst.a[a] + o1 + w * st, 4 * (size_t)m, "regrow mover dst");
+        memcpy(dst.a[a] + o1 + w * st, src.a[a] + o0 + w * st, 4 * (size_t)m);
+      }
+    }
+  }
+}
 }  // namespace osr
 #include <csignal>
 #include <unistd.h>

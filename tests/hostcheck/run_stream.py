@@ -14,6 +14,29 @@ import numpy as np  # noqa: E402
 import osiris_amd._lib as L  # noqa: E402
 
 
+def bursts(stream, seed):
+    """Submissions in random bursts between random numbers of scheduling passes (seeded):
+    admissions land at every point of the chunk cycle, independently of the wall clock."""
+    from osiris_amd.ensemble import Ensemble
+
+    rng = np.random.default_rng(seed)
+    ens = Ensemble(0, 0, 8)
+    ids, nxt = [], 0
+    try:
+        while True:
+            for _ in range(int(rng.integers(0, 4))):
+                if nxt < len(stream):
+                    ids.append(ens.submit(stream[nxt][1]))
+                    nxt += 1
+            for _ in range(int(rng.integers(1, 4))):
+                active, pending = ens.progress()
+            if nxt == len(stream) and active == 0 and pending == 0:
+                break
+        return {"jobs": len(ids), "histories": {k: ens.history(j) for k, j in enumerate(ids)}}
+    finally:
+        ens.close()
+
+
 def main():
     lib_path, n, overflow_p, notready_p = sys.argv[1], int(sys.argv[2]), float(sys.argv[3]), float(sys.argv[4])
     progress_p = float(sys.argv[7]) if len(sys.argv) > 7 else 0.3
@@ -32,9 +55,12 @@ def main():
     from osiris_amd.ensemble import poisson_jobs, serve
 
     stream = poisson_jobs(n=n, sides=sides, ppcs=ppcs, steps=(20, 400))
-    if realtime:  # arrivals compressed 100x so that admissions land between chunks
-        stream = [(t / 100.0, j) for t, j in stream]
-    res = serve(stream, realtime=realtime, chunk_steps=8, keep=True)
+    if len(sys.argv) > 8 and sys.argv[8] == "bursts":
+        res = bursts(stream, int(sys.argv[9]) if len(sys.argv) > 9 else 1)
+    else:
+        if realtime:  # arrivals compressed 100x so that admissions land between chunks
+            stream = [(t / 100.0, j) for t, j in stream]
+        res = serve(stream, realtime=realtime, chunk_steps=8, keep=True)
     bad_hist = 0
     for k, (fld, par) in res["histories"].items():
         nsteps = stream[k][1].nsteps

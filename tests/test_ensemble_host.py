@@ -35,9 +35,11 @@ def mock_lib():
     return out
 
 
-def _run(lib, n, overflow_p, notready_p, sides, ppcs, progress_p=0.3, mode="saturated"):
+def _run(lib, n, overflow_p, notready_p, sides, ppcs, progress_p=0.3, mode="saturated",
+         seed=1):
     r = subprocess.run([sys.executable, os.path.join(HC, "run_stream.py"), lib, str(n),
-                        str(overflow_p), str(notready_p), sides, ppcs, str(progress_p), mode],
+                        str(overflow_p), str(notready_p), sides, ppcs, str(progress_p), mode,
+                        str(seed)],
                        capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-3000:]
     return json.loads(r.stdout.strip().splitlines()[-1])
@@ -70,4 +72,14 @@ def test_ensemble_host_logic_realtime_arrivals(mock_lib):
     """Arrivals spread over the run (admissions while chunks are queued) with slow stream
     progress, so that frees, re-layouts and graph rebuilds meet queued work."""
     res = _run(mock_lib, 48, 0.01, 0.7, "64,128", "4,16", progress_p=0.05, mode="realtime")
+    assert res == {"jobs": 48, "errors": 0, "order_errors": 0, "live": 0, "bad_histories": 0}
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_ensemble_host_logic_random_bursts(mock_lib, seed):
+    """Submissions in seeded random bursts between scheduling passes, with injected tile
+    overflows (device re-layouts, and host re-layouts where the repeated fill overflows
+    again) and slow stream progress."""
+    res = _run(mock_lib, 48, 0.02, 0.7, "64,128", "4,16", progress_p=0.05, mode="bursts",
+               seed=seed)
     assert res == {"jobs": 48, "errors": 0, "order_errors": 0, "live": 0, "bad_histories": 0}
