@@ -491,6 +491,40 @@ int osr_sim_add_zpulse_wall(osr_sim *sim, const osr_zpulse_wall *zp);
 int osr_sim_zpulse_column(osr_sim *sim, int k, int second, double t, float *b12, float *b3,
                           int *active);
 
+/* In-box laser pulse: the zpulse input section (zpulse/os-zpulse-std.f03,
+ * read_input_zpulse :157-805) for a pulse along x1 (direction 1), without boost, tilt,
+ * chirp or math envelopes.  The pulse is launched at once (launch_zpulse :2350-2469,
+ * launch_time 0): its field (launch_zpulse_2d / 3d :1072-1387) is divergence-corrected
+ * unless plane or no_div_corr (div_corr_zpulse_2d / 3d :1681-2205) and added to E and B;
+ * pol_type != 0 adds a second pulse shifted by 90 degrees in phase and polarisation.
+ * The reference launches it in the loop of the first step, before the push
+ * (os-simulation.f03:528-531, t >= launch_time = 0).  Defaults in (), as the
+ * reference's. */
+typedef struct {
+  double a0, omega0;      /* (1.0, 10.0) */
+  double phase;           /* degrees (0) */
+  double pol;             /* polarisation angle, degrees (90) */
+  int pol_type;           /* 0 linear, +1 / -1 circular (0) */
+  int propagation;        /* OSR_PROP_* (forward: towards +x) */
+  int lon_type;           /* OSR_TENV_* longitudinal envelope (polynomial) */
+  int per_type;           /* OSR_PER_* (plane) */
+  double lon_start;       /* front of a polynomial / sin2 envelope (unset) */
+  double lon_fwhm;        /* > 0 overrides rise/flat/fall or duration (-1) */
+  double lon_rise, lon_flat, lon_fall;      /* (0, 0, 0) */
+  double lon_duration, lon_x0, lon_range;   /* gaussian envelope (0, unset, unset) */
+  double per_center[2];   /* transverse centre, y (and z) (unset: middle of the box) */
+  double per_w0, per_fwhm, per_focus;       /* gaussian beam waist, fwhm, focus (0,0,unset) */
+  int per_tem_mode[2];    /* Hermite modes 0..7 along y (and z) (0, 0) */
+  int no_div_corr;        /* 1: no longitudinal divergence correction (0) */
+  double xmin[3];         /* lower corner of the box (g_space xmin) */
+} osr_zpulse;
+
+/* The reference's defaults (os-zpulse-std.f03:246-302), box corner at the origin. */
+void osr_zpulse_defaults(osr_zpulse *zp);
+/* Launches the pulse into the simulation's fields now (before the first step).  With open
+ * x walls the pulse's values on the x guard cells go to the guard band as well. */
+int osr_sim_add_zpulse(osr_sim *sim, const osr_zpulse *zp);
+
 /* ---------------------------------------------------------------------------------
  * Measurement helper (no counterpart in the reference): streams `bytes` bytes through a
  * plain device copy kernel `reps` times and returns the mean kernel time in *ms; used

@@ -80,6 +80,8 @@ SIGNATURES = {
     "osr_sim_init_fields": (I, [P, I, C.c_float]),
     "osr_sim_add_zpulse_wall": (I, [P, P]),
     "osr_sim_zpulse_column": (I, [P, I, I, C.c_double, P, P, P]),
+    "osr_zpulse_defaults": (None, [P]),
+    "osr_sim_add_zpulse": (I, [P, P]),
     "osr_copy_probe": (I, [I64, I, P]),
 }
 
@@ -94,6 +96,18 @@ class ZpulseWall(C.Structure):
                 ("per_center", C.c_double * 2), ("per_w0", C.c_double), ("per_fwhm", C.c_double),
                 ("per_focus", C.c_double), ("per_tem_mode", C.c_int * 2),
                 ("launch_time", C.c_double), ("xmin", C.c_double * 3)]
+
+
+class ZpulseBox(C.Structure):
+    """osr_zpulse (in-box pulse)"""
+    _fields_ = [("a0", C.c_double), ("omega0", C.c_double), ("phase", C.c_double),
+                ("pol", C.c_double), ("pol_type", C.c_int), ("propagation", C.c_int),
+                ("lon_type", C.c_int), ("per_type", C.c_int), ("lon_start", C.c_double),
+                ("lon_fwhm", C.c_double), ("lon_rise", C.c_double), ("lon_flat", C.c_double),
+                ("lon_fall", C.c_double), ("lon_duration", C.c_double), ("lon_x0", C.c_double),
+                ("lon_range", C.c_double), ("per_center", C.c_double * 2), ("per_w0", C.c_double),
+                ("per_fwhm", C.c_double), ("per_focus", C.c_double), ("per_tem_mode", C.c_int * 2),
+                ("no_div_corr", C.c_int), ("xmin", C.c_double * 3)]
 
 
 OSR_TENV = {"polynomial": 0, "sin2": 1, "gaussian": 2}

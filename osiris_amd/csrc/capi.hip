@@ -2419,6 +2419,84 @@ int osr_sim_add_zpulse_wall(osr_sim* s, const osr_zpulse_wall* zp) {
   return OSR_OK;
 }
 
+void osr_zpulse_defaults(osr_zpulse* zp) {
+  if (!zp) return;
+  *zp = osr_zpulse{};
+  zp->a0 = 1.0;
+  zp->omega0 = 10.0;
+  zp->phase = 0.0;
+  zp->pol = 90.0;
+  zp->pol_type = 0;
+  zp->propagation = OSR_PROP_FORWARD;
+  zp->lon_type = OSR_TENV_POLYNOMIAL;
+  zp->per_type = OSR_PER_PLANE;
+  zp->lon_start = OSR_ZP_UNSET;
+  zp->lon_fwhm = -1.0;
+  zp->lon_x0 = zp->lon_range = OSR_ZP_UNSET;
+  zp->per_center[0] = zp->per_center[1] = OSR_ZP_UNSET;
+  zp->per_focus = OSR_ZP_UNSET;
+}
+
+// launch_zpulse (os-zpulse-std.f03:2350-2469) of an in-box pulse: built on the host in the
+// reference's guarded layout (osr_zp::box_build) and added to E and B, cell by cell in
+// FP32 (add(emf%b, b_pulse), add(emf%e, e_pulse)); a circular pulse adds its second half
+// after the first.  Interior cells always; with open x walls also the x guard cells, which
+// the ring's guard band holds.
+int osr_sim_add_zpulse(osr_sim* s, const osr_zpulse* zp) {
+  TRY(check_sim(s));
+  if (!zp) return fail(OSR_ERR_ARG, "null argument");
+  const int nxi = interior_nx(s->g);
+  const int nx[3] = {nxi, s->g.ny, s->g.nz};
+  osr_zp::Box w;
+  const char* err = osr_zp::resolve_box(*zp, s->dims, nx, s->dx, w);
+  if (err) return fail(OSR_ERR_ARG, "zpulse: %s", err);
+  // guard cells of the reference's fields (get_min_gc_part, particles/os-particles.f03:
+  // 321-360): linear (1, 2), quadratic (2, 3), cubic (3, 4)
+  const int gl = s->order, gu = s->order + 1;
+  osr_zp::GGrid g{s->dims, {nx[0], nx[1], s->dims == 3 ? nx[2] : 1}, {gl, gl, s->dims == 3 ? gl : 0},
+                  {gu, gu, s->dims == 3 ? gu : 0}};
+  const size_t nc = (size_t)s->g.ncell, gn = g.cells();
+  std::vector<float> pe(3 * nc), pb(3 * nc), eb(3 * gn), bb(3 * gn);
+  HIPC(hipStreamSynchronize(s->stream));
+  HIPC(hipMemcpy(pe.data(), s->E.p, 12 * nc, hipMemcpyDeviceToHost));
+  HIPC(hipMemcpy(pb.data(), s->B.p, 12 * nc, hipMemcpyDeviceToHost));
+  const int ring = s->g.nx;
+  double phase0 = w.phase0, pol = w.pol;
+  for (int k = 0; k < (w.pol_type != 0 ? 2 : 1); k++) {
+    if (k == 1) {  // :2431-2437
+      phase0 = phase0 + (w.pol_type > 0 ? osr_zp::kPi2 : -osr_zp::kPi2);
+      pol = pol + osr_zp::kPi2;
+    }
+    std::fill(eb.begin(), eb.end(), 0.0f);
+    std::fill(bb.begin(), bb.end(), 0.0f);
+    osr_zp::box_build(w, g, s->order, s->dx, phase0, pol, eb.data(), bb.data());
+    for (int cz = 0; cz < s->g.nz; cz++)
+      for (int cy = 0; cy < s->g.ny; cy++)
+        for (int cx = 0; cx < ring; cx++) {
+          int i;  // Fortran x index of the ring column
+          if (cx < nxi)
+            i = cx + 1;
+          else if (!s->g.open_nx)
+            continue;
+          else if (cx < nxi + gu)
+            i = cx + 1;  // x = nx + 1 .. nx + gu
+          else if (cx >= ring - gl)
+            i = cx - ring + 1;  // x = 1 - gl .. 0
+          else
+            continue;
+          const size_t c = (size_t)cx + (size_t)ring * (cy + (size_t)s->g.ny * cz);
+          for (int m = 0; m < 3; m++) {
+            const size_t r = g.at(m, i, cy + 1, cz + 1);
+            pb[m * nc + c] = pb[m * nc + c] + bb[r];
+            pe[m * nc + c] = pe[m * nc + c] + eb[r];
+          }
+        }
+  }
+  HIPC(hipMemcpy(s->E.p, pe.data(), 12 * nc, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(s->B.p, pb.data(), 12 * nc, hipMemcpyHostToDevice));
+  return restore_walls(s);
+}
+
 int osr_sim_zpulse_column(osr_sim* s, int k, int second, double t, float* b12, float* b3,
                           int* active) {
   TRY(check_sim(s));

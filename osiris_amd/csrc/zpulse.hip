@@ -257,4 +257,247 @@ inline const char* resolve(const osr_zpulse_wall& p, int dims, const int* nx, co
   return nullptr;
 }
 
+// ------------------------------------------------------------------------------------
+// In-box pulse (zpulse, zpulse/os-zpulse-std.f03): launched once into the fields along x1,
+// no boost, tilt or chirp, one node.  Box holds the parameters after read_input_zpulse
+// (:157-805); the fields are built in the reference's guarded layout (f(3, 1-gcl:nx+gcu,
+// ...), components interleaved) by launch_zpulse_2d / 3d (:1072-1387), in FP64 rounded
+// to float, and corrected by div_corr_zpulse_2d / 3d (:1681-2205).
+// ------------------------------------------------------------------------------------
+struct Box {
+  double a0, omega0, phase0, pol;
+  int pol_type, backward, lon_type, per_type, tem_mode[2], div_corr;
+  double lon_start, rise, flat, fall, lon_x0, duration, range;
+  double center[2], w0, focus;
+  double xmin[3];
+};
+
+// lon_envelope_zpulse :2480-2561 (the time is ignored without boost)
+inline double box_lon(const Box& p, double z) {
+  switch (p.lon_type) {
+    case OSR_TENV_POLYNOMIAL:
+      return p.backward ? fenv_poly(z - p.lon_start, p.rise, p.flat, p.fall)
+                        : fenv_poly(p.lon_start - z, p.rise, p.flat, p.fall);
+    case OSR_TENV_SIN2:
+      return p.backward ? fenv_sin2(z - p.lon_start, p.rise, p.flat, p.fall)
+                        : fenv_sin2(p.lon_start - z, p.rise, p.flat, p.fall);
+    default: {
+      if (std::fabs(z - p.lon_x0) > p.range / 2) return 0.0;
+      const double a = (z - p.lon_x0) / p.duration;
+      return std::exp(-2 * (a * a));
+    }
+  }
+}
+
+// lon_center_zpulse :3055-3097
+inline double box_center(const Box& p) {
+  if (p.lon_type == OSR_TENV_GAUSSIAN) return p.lon_x0;
+  return p.backward ? p.lon_start + p.rise + p.flat / 2.0 : p.lon_start - p.rise - p.flat / 2.0;
+}
+
+// per_envelope_2d_zpulse / per_envelope_3d_zpulse at time 0 (plane, hermite_gaussian)
+inline double box_per2(const Box& p, double phase0, double x1, double x2) {
+  const double zc = box_center(p), k = p.omega0, ps = p.backward ? -1.0 : 1.0, t = 0.0;
+  if (p.per_type == OSR_PER_PLANE) return std::cos(k * (x1 - zc - ps * t) + phase0);
+  const double z0 = k * (p.w0 * p.w0) * 0.5;
+  const double rho = x2 - p.center[0], rho2 = rho * rho, z = x1 - p.focus;
+  double rWl2 = 1, curv = 0, gouy = 0;
+  if (z != 0) {
+    rWl2 = (z0 * z0) / ((z0 * z0) + (z * z));
+    curv = 0.5 * rho2 * z / ((z * z) + (z0 * z0));
+    gouy = (p.tem_mode[0] + 1) * std::atan2(z, z0);
+  }
+  const double rWu = std::sqrt(2 * rWl2) / p.w0;
+  return std::sqrt(std::sqrt(rWl2)) * hermite(p.tem_mode[0], rho * rWu) *
+         std::exp(-(rho2 * rWl2 / (p.w0 * p.w0))) *
+         std::cos(k * (x1 - zc - ps * t) + k * curv - gouy + phase0);
+}
+inline double box_per3(const Box& p, double phase0, double x1, double x2, double x3) {
+  const double zc = box_center(p), k = p.omega0, ps = p.backward ? -1.0 : 1.0, t = 0.0;
+  const double ox = x2 - p.center[0], oy = x3 - p.center[1];
+  if (p.per_type == OSR_PER_PLANE) return std::cos(k * (x1 - zc - ps * t) + phase0);
+  const double rho2 = (ox * ox) + (oy * oy), z = x1 - p.focus, z0 = k * (p.w0 * p.w0) / 2;
+  double rWl2 = 1, curv = 0, gouy = 0;
+  if (z != 0) {
+    rWl2 = (z0 * z0) / ((z0 * z0) + (z * z));
+    curv = 0.5 * rho2 * z / ((z * z) + (z0 * z0));
+    gouy = (p.tem_mode[0] + p.tem_mode[1] + 1) * std::atan2(z, z0);
+  }
+  const double rWu = std::sqrt(2 * rWl2) / p.w0;
+  return std::sqrt(rWl2) * hermite(p.tem_mode[0], ox * rWu) * hermite(p.tem_mode[1], oy * rWu) *
+         std::exp(-(rho2 * rWl2 / (p.w0 * p.w0))) *
+         std::cos(k * (x1 - zc - ps * t) + k * curv - gouy + phase0);
+}
+
+// Guarded host grid of 3 interleaved components: nx interior cells, gcl / gcu guards.
+struct GGrid {
+  int dims, nx[3], gcl[3], gcu[3];
+  long ld(int d) const { return d < dims ? (long)nx[d] + gcl[d] + gcu[d] : 1; }
+  size_t cells() const { return (size_t)ld(0) * ld(1) * ld(2); }
+  // component c of Fortran cell (i, j, k)
+  size_t at(int c, int i, int j, int k) const {
+    size_t o = (size_t)(i - 1 + gcl[0]) + (size_t)ld(0) * (size_t)(j - 1 + gcl[1]);
+    if (dims == 3) o += (size_t)ld(0) * ld(1) * (size_t)(k - 1 + gcl[2]);
+    return 3 * o + c;
+  }
+};
+
+// One pulse (phase0, pol) into the zeroed arrays e, b (launch_zpulse_2d / 3d, then
+// div_corr_zpulse_2d / 3d when p.div_corr).  order 2 shifts the origin by half a cell.
+inline void box_build(const Box& p, const GGrid& g, int order, const double* dx, double phase0,
+                      double pol, float* e, float* b) {
+  const int D = g.dims;
+  int lo[3] = {1, 1, 1}, hi[3] = {1, 1, 1};
+  for (int d = 0; d < D; d++) {
+    lo[d] = 1 - g.gcl[d];
+    hi[d] = g.nx[d] + g.gcu[d];
+  }
+  const double cos_pol = std::cos(pol), sin_pol = std::sin(pol), amp = p.omega0 * p.a0;
+  const double ps = p.backward ? -1.0 : 1.0;
+  double zmin = p.xmin[0], rmin = p.xmin[1], r2min = D == 3 ? p.xmin[2] : 0.0;
+  const double dz_2 = dx[0] / 2.0, dr_2 = dx[1] / 2.0, dr2_2 = D == 3 ? dx[2] / 2.0 : 0.0;
+  if (order == 2) {
+    zmin = zmin + dz_2;
+    rmin = rmin + dr_2;
+    r2min = r2min + dr2_2;
+  }
+  for (int i1 = lo[0]; i1 <= hi[0]; i1++) {
+    const double z = zmin + (double)(i1 - 1) * dx[0], z_2 = z + dz_2;
+    const double lenv = amp * box_lon(p, z), lenv_2 = amp * box_lon(p, z_2);
+    if (D == 2) {  // :1127-1147, fields assigned
+      for (int i2 = lo[1]; i2 <= hi[1]; i2++) {
+        const double r = rmin + (double)(i2 - 1) * dx[1], r_2 = r + dr_2;
+        b[g.at(0, i1, i2, 1)] = 0.0f;
+        b[g.at(1, i1, i2, 1)] = (float)(-(lenv_2 * box_per2(p, phase0, z_2, r) * sin_pol * ps));
+        b[g.at(2, i1, i2, 1)] = (float)(+(lenv_2 * box_per2(p, phase0, z_2, r_2) * cos_pol * ps));
+        e[g.at(0, i1, i2, 1)] = 0.0f;
+        e[g.at(1, i1, i2, 1)] = (float)(lenv * box_per2(p, phase0, z, r_2) * cos_pol);
+        e[g.at(2, i1, i2, 1)] = (float)(lenv * box_per2(p, phase0, z, r) * sin_pol);
+      }
+      continue;
+    }
+    for (int i2 = lo[1]; i2 <= hi[1]; i2++) {  // :1288-1321, fields accumulated
+      const double r1 = rmin + (double)(i2 - 1) * dx[1], r1_2 = r1 + dr_2;
+      for (int i3 = lo[2]; i3 <= hi[2]; i3++) {
+        const double r2 = r2min + (double)(i3 - 1) * dx[2], r2_2 = r2 + dr2_2;
+        float& b2 = b[g.at(1, i1, i2, i3)];
+        float& b3 = b[g.at(2, i1, i2, i3)];
+        float& e2 = e[g.at(1, i1, i2, i3)];
+        float& e3 = e[g.at(2, i1, i2, i3)];
+        b2 = b2 - (float)(lenv_2 * box_per3(p, phase0, z_2, r1, r2_2) * sin_pol * ps);
+        b3 = b3 + (float)(lenv_2 * box_per3(p, phase0, z_2, r1_2, r2) * cos_pol * ps);
+        e2 = e2 + (float)(lenv * box_per3(p, phase0, z, r1_2, r2) * cos_pol);
+        e3 = e3 + (float)(lenv * box_per3(p, phase0, z, r1, r2_2) * sin_pol);
+      }
+    }
+  }
+  if (!p.div_corr) return;
+  // E1 and B1 integrated from the upper x edge, line by line over the interior (y, z);
+  // one node: the other nodes' x range is empty, so e1 = b1 = 0 at x = nx + 1
+  double x1min = p.xmin[0];
+  if (order == 2) x1min = x1min + dx[0] * 0.5;
+  const int n3 = D == 3 ? g.nx[2] : 1;
+  for (int i3 = 1; i3 <= n3; i3++)
+    for (int i2 = 1; i2 <= g.nx[1]; i2++) {
+      double e1 = 0.0, b1 = 0.0;
+      e[g.at(0, g.nx[0] + 1, i2, i3)] = (float)e1;
+      b[g.at(0, g.nx[0] + 1, i2, i3)] = (float)b1;
+      for (int i1 = g.nx[0]; i1 >= 1 - g.gcl[0]; i1--) {
+        if (box_lon(p, x1min + i1 * dx[0]) > 0.0) {
+          const double e2p = e[g.at(1, i1 + 1, i2, i3)], e2m = e[g.at(1, i1 + 1, i2 - 1, i3)];
+          if (D == 2) {
+            e1 = e1 + dx[0] * (e2p - e2m) / dx[1];
+          } else {
+            const double e3p = e[g.at(2, i1 + 1, i2, i3)], e3m = e[g.at(2, i1 + 1, i2, i3 - 1)];
+            e1 = e1 + dx[0] * ((e2p - e2m) / dx[1] + (e3p - e3m) / dx[2]);
+          }
+          e[g.at(0, i1, i2, i3)] = (float)e1;
+        } else {
+          e1 = 0.0;
+        }
+        if (box_lon(p, x1min + (i1 - 0.5) * dx[0]) > 0.0) {
+          const double b2p = b[g.at(1, i1, i2 + 1, i3)], b2m = b[g.at(1, i1, i2, i3)];
+          if (D == 2) {
+            b1 = b1 + dx[0] * (b2p - b2m) / dx[1];
+          } else {
+            const double b3p = b[g.at(2, i1, i2, i3 + 1)], b3m = b[g.at(2, i1, i2, i3)];
+            b1 = b1 + dx[0] * ((b2p - b2m) / dx[1] + (b3p - b3m) / dx[2]);
+          }
+          b[g.at(0, i1, i2, i3)] = (float)b1;
+        } else {
+          b1 = 0.0;
+        }
+      }
+    }
+}
+
+// read_input_zpulse for direction 1 (:246-805) and init_zpulse (:867-929): resolved
+// parameters, or an error text
+inline const char* resolve_box(const osr_zpulse& p, int dims, const int* nx, const double* dx,
+                               Box& w) {
+  w = Box{};
+  if (p.pol_type < -1 || p.pol_type > 1) return "pol_type must be -1, 0 or +1";
+  if (p.propagation != OSR_PROP_FORWARD && p.propagation != OSR_PROP_BACKWARD)
+    return "propagation must be forward or backward";
+  w.a0 = p.a0;
+  w.omega0 = p.omega0;
+  w.phase0 = p.phase * kPi180;
+  w.pol_type = p.pol_type;
+  w.pol = p.pol != 0.0 ? p.pol * kPi180 : 0.0;
+  w.backward = p.propagation == OSR_PROP_BACKWARD;
+  double xmax[3] = {0, 0, 0};
+  for (int d = 0; d < dims; d++) {
+    w.xmin[d] = p.xmin[d];
+    xmax[d] = p.xmin[d] + nx[d] * dx[d];
+  }
+  w.lon_type = p.lon_type;
+  switch (p.lon_type) {
+    case OSR_TENV_POLYNOMIAL:
+    case OSR_TENV_SIN2:
+      if (p.lon_fwhm > 0.0) {
+        w.rise = p.lon_fwhm / 2.0;
+        w.flat = 0.0;
+        w.fall = p.lon_fwhm / 2.0;
+      } else {
+        w.rise = p.lon_rise;
+        w.flat = p.lon_flat;
+        w.fall = p.lon_fall;
+      }
+      if (p.lon_start == OSR_ZP_UNSET) return "polynomial and sin2 envelopes need lon_start";
+      w.lon_start = p.lon_start;
+      break;
+    case OSR_TENV_GAUSSIAN:
+      w.duration = p.lon_fwhm > 0.0 ? p.lon_fwhm / std::sqrt(2.0 * std::log(2.0)) : p.lon_duration;
+      if (p.lon_range == OSR_ZP_UNSET) return "a gaussian envelope needs lon_range";
+      if (p.lon_x0 == OSR_ZP_UNSET) return "a gaussian envelope needs lon_x0";
+      w.range = p.lon_range;
+      w.lon_x0 = p.lon_x0;
+      break;
+    default:
+      return "lon_type must be polynomial, sin2 or gaussian";
+  }
+  w.per_type = p.per_type;
+  switch (p.per_type) {
+    case OSR_PER_PLANE:
+      break;
+    case OSR_PER_HERMITE:
+      for (int d = 0; d < dims - 1; d++) {
+        if (p.per_tem_mode[d] < 0 || p.per_tem_mode[d] > 7) return "per_tem_mode must be 0..7";
+        w.tem_mode[d] = p.per_tem_mode[d];
+      }
+      w.w0 = p.per_fwhm > 0.0 ? p.per_fwhm / std::sqrt(4.0 * std::log(2.0)) : p.per_w0;
+      if (w.w0 <= 0.0) return "a gaussian beam needs per_w0 or per_fwhm > 0";
+      if (p.per_focus == OSR_ZP_UNSET) return "a gaussian beam needs per_focus";
+      w.focus = p.per_focus;
+      break;
+    default:
+      return "per_type must be plane or hermite";
+  }
+  for (int d = 0; d < dims - 1; d++)  // :741-758
+    w.center[d] = p.per_center[d] == OSR_ZP_UNSET ? 0.5 * (w.xmin[d + 1] + xmax[d + 1])
+                                                   : p.per_center[d];
+  w.div_corr = (w.per_type != OSR_PER_PLANE && !p.no_div_corr) ? 1 : 0;
+  return nullptr;
+}
+
 }  // namespace osr_zp

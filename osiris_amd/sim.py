@@ -15,7 +15,8 @@ import ctypes as C
 import numpy as np
 
 from ._lib import (OSR_BND_LINDMAN, OSR_BND_OPEN, OSR_BND_PERIODIC, OSR_BND_VPML, OSR_BORIS,
-                   OSR_PER, OSR_PROP, OSR_SMOOTH, OSR_TENV, OSR_VAY, SimDesc, SmoothParams, ZpulseWall, check, lib)
+                   OSR_PER, OSR_PROP, OSR_SMOOTH, OSR_TENV, OSR_VAY, SimDesc, SmoothParams,
+                   ZpulseBox, ZpulseWall, check, lib)
 
 
 def _p(a):
@@ -240,6 +241,28 @@ class Simulation:
         for i, v in enumerate(xmin):
             z.xmin[i] = v
         check(lib().osr_sim_add_zpulse_wall(self._h, C.byref(z)))
+
+    def add_zpulse(self, xmin=(0.0, 0.0, 0.0), **kw):
+        """An in-box laser pulse along x (the zpulse input section: a0, omega0, phase, pol,
+        pol_type, propagation, lon_type, lon_start, lon_fwhm, lon_rise, lon_flat, lon_fall,
+        lon_duration, lon_x0, lon_range, per_type, per_center, per_w0, per_fwhm, per_focus,
+        per_tem_mode, no_div_corr, with the reference's defaults) launched into the fields
+        now, before the first step."""
+        z = ZpulseBox()
+        lib().osr_zpulse_defaults(C.byref(z))
+        names = {"propagation": OSR_PROP, "lon_type": OSR_TENV, "per_type": OSR_PER}
+        for k, v in kw.items():
+            if k not in {f[0] for f in ZpulseBox._fields_} - {"xmin"}:
+                raise ValueError(f"unknown zpulse parameter {k}")
+            if k in ("per_center", "per_tem_mode"):  # per transverse direction
+                v = list(v) if isinstance(v, (tuple, list)) else [v]
+                for i, vi in enumerate(v):
+                    getattr(z, k)[i] = vi
+                continue
+            setattr(z, k, names[k][v] if k in names else (int(v) if k == "no_div_corr" else v))
+        for i, v in enumerate(xmin):
+            z.xmin[i] = v
+        check(lib().osr_sim_add_zpulse(self._h, C.byref(z)))
 
     def zpulse_column(self, k, t, second=False):
         """(b12, b3, active): what pulse k adds to B2, B3 of its column (plane in 3D, shape

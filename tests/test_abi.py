@@ -79,7 +79,8 @@ def test_struct_layouts_match_header(tmp_path):
         pytest.skip("no C compiler")
     structs = {"osr_sim_desc": _lib.SimDesc, "osr_ens_desc": _lib.EnsDesc,
                "osr_job_species": _lib.JobSpecies, "osr_job_desc": _lib.JobDesc,
-               "osr_job_info": _lib.JobInfo}
+               "osr_job_info": _lib.JobInfo, "osr_zpulse_wall": _lib.ZpulseWall,
+               "osr_zpulse": _lib.ZpulseBox}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "osiris_amd.h"',
              'int main(void) {']
     for cname, py in structs.items():
