@@ -170,12 +170,30 @@ def run_c3(args, oa):
     """3D periodic magnetized thermal plasma: 256^3 cells, dx = 0.1, dt = 0.05, 8 ppc
     (2x2x2, uniform random), u_th = 0.1, uniform B0 = 0.5 z, cubic splines, Vay pusher."""
     nx, dx, dt = (256, 256, 256), (0.1, 0.1, 0.1), 0.05
-    tile = tuple(args.tile) if len(args.tile) == 3 else (8, 8, 4)
     steps = min(args.steps, 100)
-    sim = oa.Simulation(3, nx, dx, dt, 3, tile=tile, reorder=args.reorder)
-    sid = sim.add_species(-1.0, oa.OSR_VAY)
-    sim.generate(sid, (2, 2, 2), density=1.0, uth=(0.1, 0.1, 0.1), seed=3, random_pos=True)
-    sim.fill_fields((0.0, 0.0, 0.0), (0.0, 0.0, 0.5))
+
+    def make(cfg):
+        s = oa.Simulation(3, nx, dx, dt, 3, tile=tuple(cfg["tile"]), reorder=args.reorder,
+                          kernel=cfg.get("kernel", 0))
+        sid = s.add_species(-1.0, oa.OSR_VAY)
+        s.generate(sid, (2, 2, 2), density=1.0, uth=(0.1, 0.1, 0.1), seed=3, random_pos=True)
+        s.fill_fields((0.0, 0.0, 0.0), (0.0, 0.0, 0.5))
+        return s
+
+    tune = None
+    if len(args.tile) == 3 or not args.autotune:
+        cfg = {"tile": tuple(args.tile) if len(args.tile) == 3 else (8, 8, 4), "kernel": 0}
+    else:
+        from osiris_amd.autotune import label, tune as autotune
+
+        cfg, tune = autotune(make, "3d-o3-vay-256^3-ppc8", dims=3, warmup=2, steps=3,
+                             use_cache=False)
+        log("autotune (ms/step): " + ", ".join(f"{k}: {v:.3f}" for k, v in tune.items())
+            + f" -> {label(cfg)}")
+    sim = make(cfg)
+    lc = sim.launch_config
+    tile = lc["tile"]
+    sid = 0
     n_total = sim.count(sid)
     cells = int(np.prod(nx))
     log(f"workload c3: {n_total} particles, footprint {sim.footprint / 2**30:.2f} GiB")
@@ -201,10 +219,14 @@ def run_c3(args, oa):
         "config": {"workload": ("3D periodic magnetized thermal plasma 256^3, dx=0.1, "
                                 "dt=0.05, 8 ppc, u_th=0.1, B0=0.5 z, cubic splines, Vay"),
                    "particles": n_total, "cells": cells, "tile": list(tile),
-                   "reorder": args.reorder, "relayouts": relayouts},
+                   "push_waves": lc["push_waves"], "compiled_tile": lc["compiled_tile"],
+                   "autotune_ms_per_step": tune, "reorder": args.reorder,
+                   "relayouts": relayouts},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "push_deposit_3d<3,1,256>", "kernel_ms": t_launch * 1e3,
+                     "kernel": (f"push_deposit_3d_t<3,1,{tile[0]},{tile[1]},{tile[2]}>"
+                                if lc["compiled_tile"] else "push_deposit_3d<3,1,256>"),
+                     "kernel_ms": t_launch * 1e3,
                      "algorithmic_bytes": bytes_launch},
         "kernels_ms_per_step": kt,
     }
@@ -389,6 +411,8 @@ def main():
     ap.add_argument("--reorder", type=int, default=0,
                     help="in-tile particle order restored every N steps (0 = library default: "
                          "4 in 3D, off in 2D; negative = off)")
+    ap.add_argument("--push-waves", type=int, default=0,
+                    help="waves per push workgroup without autotuning (0 = the tile's default)")
     ap.add_argument("--autotune", action="store_true", default=True)
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
     ap.add_argument("--plain-steps", type=int, default=20,
@@ -437,24 +461,28 @@ def main():
     dx, dt = (0.05, 0.05), 0.035
     ppc_beam = int(np.prod(args.ppc))
 
-    def make(tile, plain=False):
-        s = oa.Simulation(2, nx, dx, dt, args.order, tile=tuple(tile), plain=plain)
+    def make(cfg, plain=False):
+        s = oa.Simulation(2, nx, dx, dt, args.order, tile=tuple(cfg["tile"]), plain=plain,
+                          push_waves=0 if plain else cfg.get("push_waves", 0),
+                          kernel=0 if plain else cfg.get("kernel", 0))
         for k, ufl in enumerate((0.2, -0.2)):
             sid = s.add_species(-1.0, oa.OSR_BORIS)
             s.generate(sid, args.ppc, density=0.5, ufl=(ufl, 0.0, 0.0), uth=(0.01, 0.01, 0.01),
                        seed=1000 + k, random_pos=True)
         return s
 
-    tile = tuple(args.tile[:2])
+    cfg = {"tile": tuple(args.tile[:2]), "push_waves": args.push_waves, "kernel": 0}
     tune = None
     if args.autotune:
-        from osiris_amd.autotune import tune_tile
+        from osiris_amd.autotune import label, tune as autotune
 
         key = f"2d-o{args.order}-boris-{nx[0]}x{nx[1]}-ppc{2 * ppc_beam}-beams"
-        tile, tune = tune_tile(make, key, dims=2, use_cache=False)
-        log("autotune (ms/step): " + ", ".join(f"{'x'.join(map(str, t))}: {v:.3f}"
-                                                for t, v in tune.items()) + f" -> {tile}")
-    sim = make(tile)
+        cfg, tune = autotune(make, key, dims=2, use_cache=False)
+        log("autotune (ms/step): " + ", ".join(f"{k}: {v:.3f}" for k, v in tune.items())
+            + f" -> {label(cfg)}")
+    tile = cfg["tile"]
+    sim = make(cfg)
+    waves = sim.push_waves
     n_total = sum(sim.count(s) for s in range(2))
     cells = nx[0] * nx[1]
     log(f"workload: {nx[0]}x{nx[1]} cells, {n_total} particles, footprint "
@@ -476,14 +504,15 @@ def main():
     bytes_launch = n_total * BYTES_PER_PARTICLE + cells * BYTES_PER_CELL_PUSH
     achieved = bytes_launch / t_launch / 1e9
     log("kernel ms/step: " + ", ".join(f"{k} {v:.4f}" for k, v in kt.items()))
-    kname = f"push_deposit_2d_t<{args.order},0,{tile[0]},{tile[1]}>"
+    kname = f"push_deposit_2d_t<{args.order},0,{tile[0]},{tile[1]},{waves}>"
     traffic = None
     if os.path.exists(args.traffic):
         try:
             with open(args.traffic) as f:
                 tj = json.load(f)
             # counters measured on the same kernel instantiation and launch shape only
-            if tuple(tj.get("tile", ())) == tuple(tile) and tj.get("species_per_launch") == nsp:
+            if (tuple(tj.get("tile", ())) == tuple(tile) and tj.get("push_waves") == waves
+                    and tj.get("species_per_launch") == nsp):
                 traffic = tj.get("hbm_bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
@@ -493,7 +522,7 @@ def main():
 
     plain = None
     if args.plain_steps > 0:
-        ps = make(tile, plain=True)
+        ps = make(cfg, plain=True)
         ps.run(2)
         t0 = time.perf_counter()
         ps.run(args.plain_steps)
@@ -528,8 +557,8 @@ def main():
             "particles": n_total,
             "cells": cells,
             "tile": list(tile),
-            "autotune_ms_per_step": ({"x".join(map(str, t)): v for t, v in tune.items()}
-                                     if tune else None),
+            "push_waves": waves,
+            "autotune_ms_per_step": tune,
             "relayouts": relayouts,
         },
         "roofline": {

@@ -163,6 +163,9 @@ typedef struct {
                      split B3 pair before each wall E update, lower / upper x wall
                      (vpml_diffuse(:, 1), default .false., os-emf-boundary.f90:231;
                      os-emf-vpml.f03:977-1006) */
+  int push_waves; /* fused path: waves per push workgroup, 0 = default (8 for 2D
+                     compile-time tiles of 512 cells or more, else 4); 4 or 8 for the 2D
+                     compile-time tiles 16x16 and 32x16.  Changes no result */
 } osr_sim_desc;
 
 /* x boundaries of a resident simulation.  Both open types have absorbing particles
@@ -240,6 +243,10 @@ int osr_sim_relayouts(osr_sim *sim, int64_t *count);
 int osr_sim_time_kernels(osr_sim *sim, int nsteps, double *ms);
 /* Bytes of device memory held by a simulation. */
 int osr_sim_footprint(osr_sim *sim, int64_t *bytes);
+/* Launch configuration the fused push resolved to (no counterpart in the reference):
+ * tile[3] in cells, waves per workgroup, and *compiled_tile = 1 when the compile-time
+ * tile variant runs, 0 for the generic kernel (or the plain path). */
+int osr_sim_launch_config(osr_sim *sim, int *tile, int *waves, int *compiled_tile);
 
 /* ---------------------------------------------------------------------------------
  * 4. Ensemble serving (no counterpart in the reference, which runs one simulation per

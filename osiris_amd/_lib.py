@@ -58,6 +58,7 @@ SIGNATURES = {
     "osr_sim_relayouts": (I, [P, P]),
     "osr_sim_time_kernels": (I, [P, I, P]),
     "osr_sim_footprint": (I, [P, P]),
+    "osr_sim_launch_config": (I, [P, P, P, P]),
     "osr_ens_create": (I, [P, P]),
     "osr_ens_destroy": (I, [P]),
     "osr_ens_submit": (I, [P, P, P]),
@@ -120,7 +121,7 @@ class SimDesc(C.Structure):
                 ("dt", C.c_double), ("order", C.c_int), ("tile", C.c_int * 3),
                 ("path", C.c_int), ("kernel", C.c_int), ("xbnd", C.c_int),
                 ("reorder", C.c_int), ("vpml_size", C.c_int),
-                ("vpml_diffuse", C.c_int * 2)]
+                ("vpml_diffuse", C.c_int * 2), ("push_waves", C.c_int)]
 
 
 class SmoothParams(C.Structure):

@@ -1,20 +1,45 @@
-"""Tile-shape autotuner for the fused path.
+"""Launch-configuration autotuner for the fused path.
 
 The push+deposit kernel trades LDS footprint (halo overhead, per-wave current tiles,
 occupancy) against the number of particles that leave their tile each step (tile
-perimeter).  The best tile depends on the order, the particles per cell and the drift,
-so it is measured: each candidate runs a few warm-up steps and then a few steps timed
-with HIP events (Simulation.time_kernels), and the fastest full step wins.  Results are
-cached per workload key in a JSON file.
+perimeter), and the waves of a workgroup share its staged field and current tiles.  The
+best configuration depends on the order, the particles per cell and the drift, so it is
+measured: each candidate (tile shape, waves per workgroup, push-kernel variant) runs a
+few warm-up steps and then a few steps timed with HIP events (Simulation.time_kernels);
+the fastest full step wins.  Every candidate gives bit-identical results, so the choice
+is purely one of speed.  Results are cached per workload key in a JSON file.
+
+The reference has no counterpart; its nearest knob is the fixed particle-chunk size of
+its CPU push (p_cache_size, source/os-param.f90:227).
 """
 from __future__ import annotations
 
 import json
 import os
 
-DEFAULT_2D = ((16, 16), (32, 16), (8, 8))
-DEFAULT_3D = ((8, 8, 4), (8, 8, 8), (4, 4, 4))
+# (tile, waves per workgroup, kernel): kernel 0 = compile-time tile variant where one is
+# instantiated, 1 = generic kernel (runtime tile shape, any tile that fits the LDS)
+DEFAULT_2D = (
+    {"tile": (16, 16), "push_waves": 4, "kernel": 0},
+    {"tile": (16, 16), "push_waves": 8, "kernel": 0},
+    {"tile": (32, 16), "push_waves": 4, "kernel": 0},
+    {"tile": (32, 16), "push_waves": 8, "kernel": 0},
+    {"tile": (8, 8), "push_waves": 4, "kernel": 0},
+)
+DEFAULT_3D = (
+    {"tile": (8, 8, 4), "push_waves": 4, "kernel": 0},
+    {"tile": (8, 8, 4), "push_waves": 4, "kernel": 1},
+    {"tile": (4, 4, 4), "push_waves": 4, "kernel": 1},
+    {"tile": (8, 4, 4), "push_waves": 4, "kernel": 1},
+    {"tile": (8, 8, 8), "push_waves": 4, "kernel": 1},
+)
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "osiris_amd", "autotune.json")
+
+
+def label(cfg) -> str:
+    """'16x16/w8/k0' for a candidate configuration."""
+    return (f"{'x'.join(map(str, cfg['tile']))}/w{cfg.get('push_waves', 4)}"
+            f"/k{cfg.get('kernel', 0)}")
 
 
 def _load(path):
@@ -34,32 +59,40 @@ def _save(path, data):
         pass
 
 
-def tune_tile(make_sim, key, candidates=None, dims=2, warmup=3, steps=5, cache=CACHE,
-              use_cache=True):
-    """Return (best_tile, {tile: ms_per_step}).
+def tune(make_sim, key, candidates=None, dims=2, warmup=3, steps=5, cache=CACHE,
+         use_cache=True):
+    """Return (best_config, {label: ms_per_step}).
 
-    make_sim(tile) must return a populated osiris_amd.Simulation (closed by the tuner).
+    make_sim(config) must return a populated osiris_amd.Simulation built with the
+    config's tile, push_waves and kernel (the tuner closes it).  A candidate whose
+    simulation cannot be created (tile larger than the grid or the LDS, no kernel with
+    that many waves) is skipped.
     """
     db = _load(cache) if cache else {}
     if use_cache and key in db:
         ent = db[key]
-        return tuple(ent["tile"]), {tuple(map(int, k.split("x"))): v for k, v in ent["ms"].items()}
+        best = dict(ent["best"])
+        best["tile"] = tuple(best["tile"])
+        return best, dict(ent["ms"])
     cands = candidates or (DEFAULT_2D if dims == 2 else DEFAULT_3D)
-    timings = {}
-    for t in cands:
+    timings, by_label = {}, {}
+    for cfg in cands:
         try:
-            sim = make_sim(t)
-        except Exception:  # tile not usable for this grid (e.g. LDS too large)
+            sim = make_sim(cfg)
+        except Exception:  # noqa: BLE001 - candidate not usable for this workload
             continue
         try:
             sim.run(warmup)
-            timings[tuple(t)] = sim.time_kernels(steps)["step"]
+            timings[label(cfg)] = sim.time_kernels(steps)["step"]
+            by_label[label(cfg)] = cfg
         finally:
             sim.close()
     if not timings:
-        raise RuntimeError("autotune: no candidate tile worked")
-    best = min(timings, key=timings.get)
+        raise RuntimeError("autotune: no candidate configuration worked")
+    best_label = min(timings, key=timings.get)
+    best = dict(by_label[best_label])
+    best["tile"] = tuple(best["tile"])
     if cache:
-        db[key] = {"tile": list(best), "ms": {"x".join(map(str, k)): v for k, v in timings.items()}}
+        db[key] = {"best": {**best, "tile": list(best["tile"])}, "ms": timings}
         _save(cache, db)
     return best, timings

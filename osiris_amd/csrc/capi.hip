@@ -812,13 +812,16 @@ push_fn pick_push(int dims, int order, int pusher) {
   return nullptr;
 }
 
-// 2D kernels with compile-time tile shape (static LDS); nullptr if not instantiated
-push_fn pick_push2d_t(int order, int pusher, int tx, int ty) {
-#define OSR_T(O, P, X, Y) \
-  if (order == O && pusher == P && tx == X && ty == Y) return push_deposit_2d_t<O, P, X, Y>;
-#define OSR_TT(X, Y) \
-  OSR_T(1, 0, X, Y) OSR_T(2, 0, X, Y) OSR_T(3, 0, X, Y) OSR_T(1, 1, X, Y) OSR_T(2, 1, X, Y) OSR_T(3, 1, X, Y)
-  OSR_TT(8, 8) OSR_TT(16, 16) OSR_TT(32, 16)
+// 2D kernels with compile-time tile shape (static LDS) and `waves` waves per workgroup;
+// nullptr if not instantiated
+push_fn pick_push2d_t(int order, int pusher, int tx, int ty, int waves) {
+#define OSR_T(O, P, X, Y, W)                                                 \
+  if (order == O && pusher == P && tx == X && ty == Y && waves == W)         \
+    return push_deposit_2d_t<O, P, X, Y, W>;
+#define OSR_TT(X, Y, W)                                                      \
+  OSR_T(1, 0, X, Y, W) OSR_T(2, 0, X, Y, W) OSR_T(3, 0, X, Y, W)             \
+  OSR_T(1, 1, X, Y, W) OSR_T(2, 1, X, Y, W) OSR_T(3, 1, X, Y, W)
+  OSR_TT(8, 8, 4) OSR_TT(16, 16, 4) OSR_TT(16, 16, 8) OSR_TT(32, 16, 4) OSR_TT(32, 16, 8)
 #undef OSR_TT
 #undef OSR_T
   return nullptr;
@@ -839,18 +842,18 @@ push_fn pick_push3d_t(int order, int pusher, int tx, int ty, int tz) {
 }
 
 // The fused push kernel of a simulation (or batch) kind: the compile-time tile variant
-// when one is instantiated and the generic kernel was not asked for, else the generic
-// kernel with `lds` bytes of dynamic LDS.  Sets the threads per workgroup and the dynamic
-// LDS of the launch.
+// with `waves` waves per workgroup when one is instantiated and the generic kernel was
+// not asked for, else the generic kernel (4 waves) with `lds` bytes of dynamic LDS.  Sets
+// the threads per workgroup and the dynamic LDS of the launch.
 push_fn pick_push_kernel(int dims, int order, int pusher, int kernel, const int* tile,
-                         size_t lds, size_t* lds_launch, int* nt) {
+                         int waves, size_t lds, size_t* lds_launch, int* nt) {
   push_fn f = nullptr;
   *lds_launch = lds;
   *nt = kNT;
   if (kernel != 1) {
     if (dims == 2) {
-      f = pick_push2d_t(order, pusher, tile[0], tile[1]);
-      if (f) *nt = push2d_t_threads(tile[0], tile[1]);
+      f = pick_push2d_t(order, pusher, tile[0], tile[1], waves);
+      if (f) *nt = 64 * waves;
     } else {
       f = pick_push3d_t(order, pusher, tile[0], tile[1], tile[2]);
     }
@@ -869,7 +872,8 @@ int launch_push(osr_sim* s, int first, int count = 1) {
   const int tile[3] = {s->g.tx, s->g.ty, s->g.tz};
   size_t lds = 0;
   int nt = kNT;
-  push_fn f = pick_push_kernel(s->dims, s->order, pusher, s->desc.kernel, tile, s->lds_push, &lds, &nt);
+  push_fn f = pick_push_kernel(s->dims, s->order, pusher, s->desc.kernel, tile, s->push_waves,
+                               s->lds_push, &lds, &nt);
   if (!f) return fail(OSR_ERR_ARG, "no push kernel for dims=%d order=%d", s->dims, s->order);
   if (nt != 64 * s->push_waves) return fail(OSR_ERR_STATE, "push wave count mismatch");
   hipLaunchKernelGGL(f, dim3((unsigned)(count * s->ntiles)), dim3(nt), lds, s->stream,
@@ -1233,8 +1237,15 @@ int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
   if (s->lds_push > 160 * 1024) return fail(OSR_ERR_ARG, "tile too large for LDS");
   s->reorder = desc->reorder > 0 ? desc->reorder : (desc->reorder == 0 && dims == 3 ? 4 : 0);
   s->push_waves = kWaves;
-  if (!s->plain && dims == 2 && desc->kernel != 1 && pick_push2d_t(s->order, 0, g.tx, g.ty))
-    s->push_waves = push2d_t_threads(g.tx, g.ty) / 64;
+  if (desc->push_waves != 0 && desc->push_waves != 4 && desc->push_waves != 8)
+    return fail(OSR_ERR_ARG, "push_waves must be 0, 4 or 8");
+  if (!s->plain && dims == 2 && desc->kernel != 1) {
+    const int w = desc->push_waves > 0 ? desc->push_waves : push2d_default_waves(g.tx, g.ty);
+    if (pick_push2d_t(s->order, 0, g.tx, g.ty, w)) s->push_waves = w;
+  }
+  if (desc->push_waves > 0 && desc->push_waves != s->push_waves)
+    return fail(OSR_ERR_ARG, "no push kernel with %d waves per workgroup for this tile and kernel",
+                desc->push_waves);
   HIPC(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   const size_t nc = (size_t)g.ncell;
   TRY(s->E.alloc(3 * nc * 4));
@@ -1802,6 +1813,20 @@ int osr_sim_footprint(osr_sim* s, int64_t* bytes) {
     for (const DBuf* d : all) b += d->n;
   }
   *bytes = (int64_t)b;
+  return OSR_OK;
+}
+
+int osr_sim_launch_config(osr_sim* s, int* tile, int* waves, int* compiled_tile) {
+  TRY(check_sim(s));
+  if (!tile || !waves || !compiled_tile) return fail(OSR_ERR_ARG, "null argument");
+  tile[0] = s->g.tx;
+  tile[1] = s->g.ty;
+  tile[2] = s->g.tz;
+  *waves = s->push_waves;
+  *compiled_tile = 0;
+  if (!s->plain && s->desc.kernel != 1)
+    *compiled_tile = (s->dims == 2 ? pick_push2d_t(s->order, 0, s->g.tx, s->g.ty, s->push_waves)
+                                   : pick_push3d_t(s->order, 0, s->g.tx, s->g.ty, s->g.tz)) != nullptr;
   return OSR_OK;
 }
 

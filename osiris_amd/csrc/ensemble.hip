@@ -34,7 +34,7 @@ struct EnsJob {
 };
 
 struct EnsBatch {
-  int dims = 2, order = 2, tile[3] = {1, 1, 1}, kernel = 0;
+  int dims = 2, order = 2, tile[3] = {1, 1, 1}, kernel = 0, waves = 4;
   hipStream_t stream = nullptr;
   hipEvent_t ev = nullptr;
   std::vector<EnsJob*> jobs;     // jobs stepped by the graph
@@ -186,7 +186,8 @@ int ens_admit(osr_ensemble* e, EnsJob* j) {
   EnsBatch* b = nullptr;
   for (auto& bb : e->batches)
     if (bb->dims == s->dims && bb->order == s->order && bb->tile[0] == s->g.tx &&
-        bb->tile[1] == s->g.ty && bb->tile[2] == s->g.tz && bb->kernel == jd.sim.kernel) {
+        bb->tile[1] == s->g.ty && bb->tile[2] == s->g.tz && bb->kernel == jd.sim.kernel &&
+        bb->waves == s->push_waves) {
       b = bb.get();
       break;
     }
@@ -198,6 +199,7 @@ int ens_admit(osr_ensemble* e, EnsJob* j) {
     nb->tile[1] = s->g.ty;
     nb->tile[2] = s->g.tz;
     nb->kernel = jd.sim.kernel;
+    nb->waves = s->push_waves;
     nb->lds = s->lds_push;
     HIPC(hipStreamCreateWithFlags(&nb->stream, hipStreamNonBlocking));
     HIPC(hipEventCreateWithFlags(&nb->ev, hipEventDisableTiming));
@@ -274,7 +276,8 @@ int ens_build(osr_ensemble* e, EnsBatch* b) {
     if (!b->npush[p]) continue;
     size_t lds = 0;
     int nt = kNT;
-    push_fn f = pick_push_kernel(b->dims, b->order, p, b->kernel, b->tile, b->lds, &lds, &nt);
+    push_fn f = pick_push_kernel(b->dims, b->order, p, b->kernel, b->tile, b->waves, b->lds,
+                                 &lds, &nt);
     if (!f) {
       r = fail(OSR_ERR_ARG, "no push kernel for dims=%d order=%d", b->dims, b->order);
       break;

@@ -1,7 +1,8 @@
 // k_push2d.hip -- fused 2D push + deposit with compile-time tile shape.
 //
-// One workgroup (4 waves, 8 for tiles of 512 cells or more) per (species, tile).  Compared with the generic kernel in
-// k_tiles.hip:
+// One workgroup of NW waves (4 or 8) per (species, tile); by default 8 for tiles of 512
+// cells or more, whose LDS field and current tiles are then shared by twice as many
+// waves.  Compared with the generic kernel in k_tiles.hip:
 //  * TX/TY are template parameters, so every LDS stencil offset is an immediate;
 //  * particle, mover and field arrays are accessed through global-address-space
 //    pointers: uniform base + 32-bit offset, no per-array 64-bit address arithmetic, and
@@ -48,12 +49,12 @@ constexpr int kCurPad = OSR_CUR_PAD;
 // periodic wrap of a cell index at most one period outside [0, n)
 OSR_DEV int wrap1(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
-template <int ORDER, int PUSHER, int TX, int TY>
-__global__ __launch_bounds__(push2d_t_threads(TX, TY), 4) void push_deposit_2d_t(const SpecDesc* __restrict__ descs,
-                                                         const int2* __restrict__ work) {
+template <int ORDER, int PUSHER, int TX, int TY, int NW>
+__global__ __launch_bounds__(64 * NW, 4) void push_deposit_2d_t(const SpecDesc* __restrict__ descs,
+                                                                const int2* __restrict__ work) {
   using O = Ord<ORDER>;
   constexpr int NP = O::NP;
-  constexpr int NT = push2d_t_threads(TX, TY), NW = NT / 64;
+  constexpr int NT = 64 * NW;
   // staged field tile GX x GY and current tile JX x JY, stored with row pitches GS / JS
   constexpr int GX = TX + O::GLO + O::GHI, GY = TY + O::GLO + O::GHI;
   constexpr int JX = TX + O::JLO + O::JHI, JY = TY + O::JLO + O::JHI;
@@ -326,13 +327,15 @@ __global__ __launch_bounds__(push2d_t_threads(TX, TY), 4) void push_deposit_2d_t
   }
 }
 
-#define OSR_INST2(O, P, X, Y) \
-  template __global__ void push_deposit_2d_t<O, P, X, Y>(const SpecDesc*, const int2*);
-#define OSR_INST2T(X, Y)                                                      \
-  OSR_INST2(1, 0, X, Y) OSR_INST2(2, 0, X, Y) OSR_INST2(3, 0, X, Y)           \
-  OSR_INST2(1, 1, X, Y) OSR_INST2(2, 1, X, Y) OSR_INST2(3, 1, X, Y)
-OSR_INST2T(8, 8)
-OSR_INST2T(16, 16)
-OSR_INST2T(32, 16)
+#define OSR_INST2(O, P, X, Y, W) \
+  template __global__ void push_deposit_2d_t<O, P, X, Y, W>(const SpecDesc*, const int2*);
+#define OSR_INST2T(X, Y, W)                                                            \
+  OSR_INST2(1, 0, X, Y, W) OSR_INST2(2, 0, X, Y, W) OSR_INST2(3, 0, X, Y, W)           \
+  OSR_INST2(1, 1, X, Y, W) OSR_INST2(2, 1, X, Y, W) OSR_INST2(3, 1, X, Y, W)
+OSR_INST2T(8, 8, 4)
+OSR_INST2T(16, 16, 4)
+OSR_INST2T(16, 16, 8)
+OSR_INST2T(32, 16, 4)
+OSR_INST2T(32, 16, 8)
 
 }  // namespace osr

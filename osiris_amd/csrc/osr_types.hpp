@@ -24,14 +24,16 @@ namespace osr {
 
 constexpr int kMaxSpecies = 8;
 // Push workgroups have 4 waves, or 8 for the 2D kernels with compile-time tiles of at least
-// 512 cells: their field and current tiles in LDS are then shared by twice as many waves,
-// so two workgroups (16 waves) fit in a CU's LDS where three 4-wave ones (12 waves) did.
+// 512 cells (by default; 4 or 8 on request for 16x16 and 32x16): their field and current
+// tiles in LDS are then shared by twice as many waves, so two workgroups (16 waves) fit in
+// a CU's LDS where three 4-wave ones (12 waves) did.
 constexpr int kMaxPushWaves = 8;
 // capacity of the per-wave virtual-particle queue of the 3D push (pending < 64 plus up to
 // 64 appended per VP level)
 constexpr int kVpQueue3 = 128;
-__host__ __device__ constexpr int push2d_t_threads(int tx, int ty) {
-  return tx * ty >= 512 ? 512 : 256;
+// default waves per workgroup of the 2D compile-time tile kernels
+__host__ __device__ constexpr int push2d_default_waves(int tx, int ty) {
+  return tx * ty >= 512 ? 8 : 4;
 }
 
 // Control block, one per simulation (device memory).

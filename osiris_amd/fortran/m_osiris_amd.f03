@@ -39,6 +39,7 @@ module m_osiris_amd
     integer(c_int) :: reorder = 0
     integer(c_int) :: vpml_size = 0
     integer(c_int) :: vpml_diffuse(2) = 0
+    integer(c_int) :: push_waves = 0
   end type osr_sim_desc
 
   ! current smoothing (vdf/os-vdf-smooth.f90): the reference's filter type codes and the
@@ -93,7 +94,8 @@ module m_osiris_amd
   public :: osr_sim_create, osr_sim_destroy, osr_sim_add_species, osr_species_set, &
             osr_species_count, osr_species_get, osr_species_generate, osr_fields_set, &
             osr_fields_get, osr_fields_fill, osr_sim_run, osr_sim_steps, osr_sim_history, &
-            osr_sim_gauss, osr_sim_relayouts, osr_sim_time_kernels, osr_sim_footprint
+            osr_sim_gauss, osr_sim_relayouts, osr_sim_time_kernels, osr_sim_footprint, &
+            osr_sim_launch_config
   public :: osr_smooth_setup, osr_smooth_vdf, osr_sim_set_smooth
   public :: osr_sim_write_hist, osr_sim_save_field
   public :: osr_sim_add_zpulse_wall, osr_sim_zpulse_column, osr_sim_init_fields
@@ -311,6 +313,13 @@ module m_osiris_amd
       type(c_ptr), value :: sim
       integer(c_int64_t), intent(out) :: bytes
       integer(c_int) :: osr_sim_footprint
+    end function
+    function osr_sim_launch_config(sim, tile, waves, compiled_tile) &
+        bind(c, name="osr_sim_launch_config")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: sim
+      integer(c_int), intent(out) :: tile(3), waves, compiled_tile
+      integer(c_int) :: osr_sim_launch_config
     end function
 
     ! ---- 5. current smoothing

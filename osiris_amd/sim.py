@@ -115,7 +115,8 @@ class Simulation:
     """A resident simulation on the GPU (one per grid; species share the grid)."""
 
     def __init__(self, dims, nx, dx, dt, order, tile=None, plain=False, kernel=0,
-                 xbnd="periodic", reorder=0, vpml_size=0, vpml_diffuse=(False, False)):
+                 xbnd="periodic", reorder=0, vpml_size=0, vpml_diffuse=(False, False),
+                 push_waves=0):
         """xbnd: "periodic"; "open" or "lindman" for Lindman absorbing field walls, "vpml"
         for perfectly matched layers of vpml_size cells (0: 8; vpml_diffuse: lower, upper
         wall mix the split B3 pair, 2D), both with absorbing
@@ -128,6 +129,7 @@ class Simulation:
         d.vpml_size = vpml_size
         d.vpml_diffuse[0], d.vpml_diffuse[1] = int(vpml_diffuse[0]), int(vpml_diffuse[1])
         d.reorder = reorder  # in-tile particle order restored every `reorder` steps
+        d.push_waves = push_waves  # waves per push workgroup (0: the tile's default)
         d.dims = dims
         for i in range(3):
             d.nx[i] = nx[i] if i < dims else 1
@@ -341,6 +343,19 @@ class Simulation:
         check(lib().osr_sim_time_kernels(self._h, int(nsteps), _p(ms)))
         return {"push_deposit": ms[0], "tile_fill": ms[1], "yee": ms[2], "finalize": ms[3],
                 "step": ms[4]}
+
+    @property
+    def launch_config(self):
+        """{'tile', 'push_waves', 'compiled_tile'} the fused push resolved to."""
+        t = (C.c_int * 3)()
+        w, k = C.c_int(), C.c_int()
+        check(lib().osr_sim_launch_config(self._h, t, C.byref(w), C.byref(k)))
+        return {"tile": tuple(t[:self.dims]), "push_waves": w.value,
+                "compiled_tile": bool(k.value)}
+
+    @property
+    def push_waves(self):
+        return self.launch_config["push_waves"]
 
     @property
     def footprint(self):
