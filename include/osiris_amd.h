@@ -238,8 +238,10 @@ int osr_sim_relayouts(osr_sim *sim, int64_t *count);
 
 /* Per-kernel timing of nsteps steps, measured with HIP events on the simulation's
  * stream (the steps are really advanced).  ms[k] = mean duration per step of kernel
- * class k: 0 push+deposit (all species), 1 tile fill, 2 Yee (3 sub-steps), 3 energy
- * finalisation; ms[4] = mean time per step from the first to the last event; ms[5] = 0. */
+ * class k: 0 push+deposit (all species), 1 tile fill and re-order, 2 field advance
+ * (smoothing, Yee sub-steps, wall updates), 3 step end (energy reduction; with Lindman
+ * walls it runs in the last wall kernel and this is 0); ms[4] = mean time per step from
+ * the first to the last event; ms[5] = 0. */
 int osr_sim_time_kernels(osr_sim *sim, int nsteps, double *ms);
 /* Bytes of device memory held by a simulation. */
 int osr_sim_footprint(osr_sim *sim, int64_t *bytes);

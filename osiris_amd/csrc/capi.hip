@@ -61,6 +61,7 @@ static int fail(int code, const char* fmt, ...) {
 namespace {
 
 constexpr int kNT = 256;  // threads per workgroup of the push kernels
+constexpr int kGraphSteps = 4;  // steps per replay of the captured step graph
 constexpr int kWaves = kNT / 64;
 
 // Device allocations made while g_alloc_stream is set (ensemble jobs) come from the
@@ -159,6 +160,8 @@ struct osr_sim {
   int ntiles = 0;
   int nblk = 0;  // Yee blocks of 256 cells
   int nblk_e = 0;  // energy workgroups of the first B half step (kEneBlk blocks each)
+  int nwork_e = 0;  // its work items: the nblk_e blocks, then (Lindman walls) the blocks of
+                    // 256 rows of the lower wall's first B update, as (0, -1 - block)
   double dx[3] = {1, 1, 1};
   double dt = 0;
   DBuf E, B, J, ctrl, fe_part, hist;
@@ -167,7 +170,8 @@ struct osr_sim {
   DBuf d_spec, d_work, d_fd, d_ywork, d_ework, d_fwork;
   FieldDesc fd{};
   hipStream_t stream = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  hipGraphExec_t gexec = nullptr;    // one step
+  hipGraphExec_t gexec_k = nullptr;  // kGraphSteps steps
   bool graph_ok = false;
   int64_t steps = 0;
   int64_t relayouts = 0;
@@ -952,23 +956,24 @@ int launch_yee(osr_sim* s) {
   }
   if (s->g.open_nx) {  // advance_yee with Lindman x walls (os-emf-solver-yee.f03:542-568)
     const unsigned nb = (unsigned)ceil_div(s->g.ny * s->g.nz, 256);
+    const unsigned nbf = (unsigned)ceil_div(s->g.ny * s->g.nz, kFinNT);
+    // the work list of the first B half step carries the wall's first B update
     if (s->dims == 2) {
-      hipLaunchKernelGGL((yee_dbdt<2, true>), dim3(s->nblk_e), dim3(256), 0, s->stream, fd, we);
-      hipLaunchKernelGGL((wall_b<1, 2>), dim3(nb), dim3(256), 0, s->stream, fd);
+      hipLaunchKernelGGL((yee_dbdt<2, true>), dim3(s->nwork_e), dim3(256), 0, s->stream, fd, we);
       hipLaunchKernelGGL((yee_dedt<2>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
       hipLaunchKernelGGL(wall_e, dim3(nb), dim3(256), 0, s->stream, fd);
       hipLaunchKernelGGL((yee_dbdt<2, false>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
-      hipLaunchKernelGGL((wall_b<2, 2>), dim3(nb), dim3(256), 0, s->stream, fd);
+      hipLaunchKernelGGL((wall_b2_end<2>), dim3(nbf), dim3(kFinNT), 0, s->stream, fd);
     } else {
-      hipLaunchKernelGGL((yee_dbdt<3, true>), dim3(s->nblk_e), dim3(256), 0, s->stream, fd, we);
-      hipLaunchKernelGGL((wall_b<1, 3>), dim3(nb), dim3(256), 0, s->stream, fd);
+      hipLaunchKernelGGL((yee_dbdt<3, true>), dim3(s->nwork_e), dim3(256), 0, s->stream, fd, we);
       hipLaunchKernelGGL((yee_dedt<3>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
       hipLaunchKernelGGL(wall_e, dim3(nb), dim3(256), 0, s->stream, fd);
       hipLaunchKernelGGL((yee_dbdt<3, false>), dim3(s->nblk), dim3(256), 0, s->stream, fd, w);
-      hipLaunchKernelGGL((wall_b<2, 3>), dim3(nb), dim3(256), 0, s->stream, fd);
+      hipLaunchKernelGGL((wall_b2_end<3>), dim3(nbf), dim3(kFinNT), 0, s->stream, fd);
     }
+    // the last wall kernel also restored the wall values and ended the step
     HIPC(hipGetLastError());
-    return launch_wall_restore(s);
+    return OSR_OK;
   }
   if (s->dims == 2) {
     hipLaunchKernelGGL((yee_dbdt<2, true>), dim3(s->nblk_e), dim3(256), 0, s->stream, fd, we);
@@ -983,7 +988,10 @@ int launch_yee(osr_sim* s) {
   return OSR_OK;
 }
 
+// End of the step: history row, step counter, halt protocol.  With Lindman walls the last
+// wall kernel of launch_yee has done it.
 int launch_finalize(osr_sim* s) {
+  if (s->g.open_nx && !s->vp_n) return OSR_OK;
   hipLaunchKernelGGL(finalize_step, dim3(1), dim3(kFinNT), 0, s->stream, s->d_fd.as<FieldDesc>(),
                      s->d_fwork.as<int>());
   HIPC(hipGetLastError());
@@ -1080,24 +1088,37 @@ int enqueue_step(osr_sim* s) {
   return OSR_OK;
 }
 
-int build_graph(osr_sim* s) {
-  if (s->gexec) {
-    (void)hipGraphExecDestroy(s->gexec);
-    s->gexec = nullptr;
-  }
-  s->graph_ok = false;
+// Capture of `nsteps` consecutive steps into one executable graph.  A replay boundary
+// costs a few microseconds of idle device time that the kernels of one graph do not pay
+// between them, so steps are replayed kGraphSteps at a time.  The kernels read the step
+// and the halt flag on the device, so a step that halts leaves the later steps of the
+// same replay idle, as separate replays would be.
+int capture_steps(osr_sim* s, int nsteps, hipGraphExec_t* out) {
   hipGraph_t graph = nullptr;
   HIPC(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-  int r = enqueue_step(s);
+  int r = OSR_OK;
+  for (int k = 0; k < nsteps && r == OSR_OK; k++) r = enqueue_step(s);
   hipError_t e = hipStreamEndCapture(s->stream, &graph);
   if (r != OSR_OK) {
     if (graph) (void)hipGraphDestroy(graph);
     return r;
   }
   if (e != hipSuccess) return fail(OSR_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-  e = hipGraphInstantiate(&s->gexec, graph, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return fail(OSR_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  return OSR_OK;
+}
+
+int build_graph(osr_sim* s) {
+  for (hipGraphExec_t* g : {&s->gexec, &s->gexec_k})
+    if (*g) {
+      (void)hipGraphExecDestroy(*g);
+      *g = nullptr;
+    }
+  s->graph_ok = false;
+  TRY(capture_steps(s, 1, &s->gexec));
+  TRY(capture_steps(s, kGraphSteps, &s->gexec_k));
   s->graph_ok = true;
   return OSR_OK;
 }
@@ -1284,13 +1305,17 @@ int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
   TRY(s->d_spec.alloc(sizeof(SpecDesc) * kMaxSpecies));
   TRY(s->d_fd.alloc(sizeof(FieldDesc)));
   TRY(s->d_ywork.alloc(sizeof(int2) * (size_t)s->nblk));
-  TRY(s->d_ework.alloc(sizeof(int2) * (size_t)s->nblk_e));
+  const int nwall_e = (g.open_nx && !vp_n) ? ceil_div(g.ny * g.nz, 256) : 0;
+  s->nwork_e = s->nblk_e + nwall_e;
+  TRY(s->d_ework.alloc(sizeof(int2) * (size_t)s->nwork_e));
   TRY(s->d_fwork.alloc(sizeof(int)));
   {
     std::vector<int2> yw((size_t)s->nblk);
     for (int b = 0; b < s->nblk; b++) yw[b] = make_int2(0, b);
     HIPC(hipMemcpy(s->d_ywork.p, yw.data(), sizeof(int2) * yw.size(), hipMemcpyHostToDevice));
-    HIPC(hipMemcpy(s->d_ework.p, yw.data(), sizeof(int2) * (size_t)s->nblk_e, hipMemcpyHostToDevice));
+    std::vector<int2> ew(yw.begin(), yw.begin() + s->nblk_e);
+    for (int b = 0; b < nwall_e; b++) ew.push_back(make_int2(0, -1 - b));
+    HIPC(hipMemcpy(s->d_ework.p, ew.data(), sizeof(int2) * ew.size(), hipMemcpyHostToDevice));
     int z = 0;
     HIPC(hipMemcpy(s->d_fwork.p, &z, sizeof(int), hipMemcpyHostToDevice));
   }
@@ -1312,6 +1337,7 @@ int osr_sim_destroy(osr_sim* s) {
   if (!s) return OSR_OK;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->gexec_k) (void)hipGraphExecDestroy(s->gexec_k);
   if (s->h_ctrl) (void)hipHostFree(s->h_ctrl);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
@@ -1586,7 +1612,9 @@ int osr_sim_run(osr_sim* s, int nsteps) {
       nb = std::min(nb, zp_rows(s, zp_launches(s)));
       TRY(zp_fill(s, s->steps, nb));
     }
-    for (int i = 0; i < nb; i++) HIPC(hipGraphLaunch(s->gexec, s->stream));
+    int i = 0;
+    for (; i + kGraphSteps <= nb; i += kGraphSteps) HIPC(hipGraphLaunch(s->gexec_k, s->stream));
+    for (; i < nb; i++) HIPC(hipGraphLaunch(s->gexec, s->stream));
     Ctrl c;
     TRY(read_ctrl(s, c));
     if (c.error & 1) return fail(OSR_ERR_LAYOUT, "particle found outside its tile");
@@ -1855,6 +1883,7 @@ int osr_sim_time_kernels(osr_sim* s, int nsteps, double* ms) {
       TRY(launch_reorder(s));
     }
     HIPC(hipEventRecord(ev[2], s->stream));
+    TRY(launch_smooth(s));
     TRY(launch_yee(s));
     HIPC(hipEventRecord(ev[3], s->stream));
     TRY(launch_finalize(s));

@@ -80,6 +80,10 @@ OSR_DEV int wrp(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 // FP64 block reduction is paid once per 1024 cells
 constexpr int kEneBlk = 4;
 
+// one row of the Lindman wall B update (defined with the wall kernels below)
+template <int STEP, int DIMS, bool RESTORE = false>
+OSR_DEV void wall_b_row(const FieldDesc& fd, long long r);
+
 template <int DIMS, bool ENERGY>
 __global__ __launch_bounds__(256) void yee_dbdt(const FieldDesc* __restrict__ fds,
                                                 const int2* __restrict__ work) {
@@ -88,55 +92,91 @@ __global__ __launch_bounds__(256) void yee_dbdt(const FieldDesc* __restrict__ fd
   const int2 wk = work[blockIdx.x];
   const FieldDesc& fd = fds[wk.x];
   const GridDesc& g = fd.g;
+  if (ENERGY && wk.y < 0) {
+    // a block of rows of the lower Lindman wall: its B update after the first half step
+    // (wall_b<1>) reads E and its own buffer only, neither of which this kernel writes
+    wall_b_row<1, DIMS>(fd, (long long)(-1 - wk.y) * 256 + threadIdx.x);
+    return;
+  }
   if (g.ctrl->halt >= 2) return;
   const long long nc = g.ncell;
   const float d1 = fd.bdt[0], d2 = fd.bdt[1], d3 = fd.bdt[2];
+  const int nx = g.nx, ny = g.ny;
+  // open x walls: the reference advances B on x = 0 .. nx+1 only (dbdt_2d loops), so the
+  // guard band keeps its values elsewhere; the energy sums the interior
+  const int nxi = interior_nx(g);
+  const float* __restrict__ E1 = g.E;
+  const float* __restrict__ E2 = g.E + nc;
+  const float* __restrict__ E3 = g.E + 2 * nc;
+  float* __restrict__ B1 = g.B;
+  float* __restrict__ B2 = g.B + nc;
+  float* __restrict__ B3 = g.B + 2 * nc;
+  // every load of the workgroup's NB cells per thread is issued before the first store
+  // (E and B are distinct arrays)
+  long long idx[NB];
+  int ii[NB];
+  float e1[NB], e2[NB], e3[NB], b1[NB], b2[NB], b3[NB];
+  float e3j[NB], e3i[NB], e2i[NB], e1j[NB], e2k[NB], e1k[NB];
+#pragma unroll
+  for (int qb = 0; qb < NB; qb++) {
+    idx[qb] = ((long long)wk.y * NB + qb) * 256 + threadIdx.x;
+    ii[qb] = 0;
+    e1[qb] = e2[qb] = e3[qb] = b1[qb] = b2[qb] = b3[qb] = 0.f;
+    e3j[qb] = e3i[qb] = e2i[qb] = e1j[qb] = e2k[qb] = e1k[qb] = 0.f;
+    if (idx[qb] >= nc) continue;
+    const long long id = idx[qb];
+    const int i = (int)(id % nx);
+    const int j = (int)((id / nx) % ny);
+    const long long ip = id - i + wrp(i + 1, nx);
+    const long long jp = id + (long long)(wrp(j + 1, ny) - j) * nx;
+    ii[qb] = i;
+    e1[qb] = E1[id];
+    e2[qb] = E2[id];
+    e3[qb] = E3[id];
+    b1[qb] = B1[id];
+    b2[qb] = B2[id];
+    b3[qb] = B3[id];
+    e3j[qb] = E3[jp];
+    e3i[qb] = E3[ip];
+    e2i[qb] = E2[ip];
+    e1j[qb] = E1[jp];
+    if (DIMS == 3) {
+      const int nz = g.nz;
+      const int k = (int)(id / ((long long)nx * ny));
+      const long long kp = id + (long long)(wrp(k + 1, nz) - k) * nx * ny;
+      e2k[qb] = E2[kp];
+      e1k[qb] = E1[kp];
+    }
+  }
   double acc[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
   for (int qb = 0; qb < NB; qb++) {
-    const long long idx = ((long long)wk.y * NB + qb) * 256 + threadIdx.x;
-    if (idx >= nc) break;
-    const int nx = g.nx, ny = g.ny;
-    const int i = (int)(idx % nx);
-    const int j = (int)((idx / nx) % ny);
-    const long long ip = idx - i + wrp(i + 1, nx);
-    const long long jp = idx + (long long)(wrp(j + 1, ny) - j) * nx;
-    const float* E1 = g.E;
-    const float* E2 = g.E + nc;
-    const float* E3 = g.E + 2 * nc;
-    float* B1 = g.B;
-    float* B2 = g.B + nc;
-    float* B3 = g.B + 2 * nc;
-    const float e1 = E1[idx], e2 = E2[idx], e3 = E3[idx];
-    float b1 = B1[idx], b2 = B2[idx], b3 = B3[idx];
-    // open x walls: the reference advances B on x = 0 .. nx+1 only (dbdt_2d loops), so
-    // the guard band keeps its values elsewhere; the energy sums the interior
-    const int nxi = interior_nx(g);
+    if (idx[qb] >= nc) break;
+    const int i = ii[qb];
     const bool live = i <= nxi || i == nx - 1;
     if (ENERGY && i < nxi) {
-      acc[0] += (double)b1 * b1;
-      acc[1] += (double)b2 * b2;
-      acc[2] += (double)b3 * b3;
-      acc[3] += (double)e1 * e1;
-      acc[4] += (double)e2 * e2;
-      acc[5] += (double)e3 * e3;
+      acc[0] += (double)b1[qb] * b1[qb];
+      acc[1] += (double)b2[qb] * b2[qb];
+      acc[2] += (double)b3[qb] * b3[qb];
+      acc[3] += (double)e1[qb] * e1[qb];
+      acc[4] += (double)e2[qb] * e2[qb];
+      acc[5] += (double)e3[qb] * e3[qb];
     }
+    float n1, n2, n3;
     if (DIMS == 2) {
-      b1 = b1 - d2 * (E3[jp] - e3);
-      b2 = b2 + d1 * (E3[ip] - e3);
-      b3 = b3 - d1 * (E2[ip] - e2) + d2 * (E1[jp] - e1);
+      n1 = b1[qb] - d2 * (e3j[qb] - e3[qb]);
+      n2 = b2[qb] + d1 * (e3i[qb] - e3[qb]);
+      n3 = b3[qb] - d1 * (e2i[qb] - e2[qb]) + d2 * (e1j[qb] - e1[qb]);
     } else {
-      const int nz = g.nz;
-      const int k = (int)(idx / ((long long)nx * ny));
-      const long long kp = idx + (long long)(wrp(k + 1, nz) - k) * nx * ny;
-      b1 = b1 - d2 * (E3[jp] - e3) + d3 * (E2[kp] - e2);
-      b2 = b2 + d1 * (E3[ip] - e3) - d3 * (E1[kp] - e1);
-      b3 = b3 - d1 * (E2[ip] - e2) + d2 * (E1[jp] - e1);
+      n1 = b1[qb] - d2 * (e3j[qb] - e3[qb]) + d3 * (e2k[qb] - e2[qb]);
+      n2 = b2[qb] + d1 * (e3i[qb] - e3[qb]) - d3 * (e1k[qb] - e1[qb]);
+      n3 = b3[qb] - d1 * (e2i[qb] - e2[qb]) + d2 * (e1j[qb] - e1[qb]);
     }
     if (live) {
-      B1[idx] = b1;
-      B2[idx] = b2;
-      B3[idx] = b3;
+      const long long id = idx[qb];
+      B1[id] = n1;
+      B2[id] = n2;
+      B3[id] = n3;
     }
   }
   if (ENERGY) {
@@ -243,14 +283,18 @@ OSR_DEV float* wall_at(const FieldDesc& fd, int k, long long R) { return fd.wall
 // nz+1), guard cells that nothing refreshes between the start of the step and the second
 // B half step, so they still hold E3(0, 1, z) and E2(0, y, 1) of the start of the step
 // (saved by wall_e).  E1 on the wall is always zero.
-template <int STEP, int DIMS>
-__global__ __launch_bounds__(256) void wall_b(const FieldDesc* __restrict__ fds) {
-  const FieldDesc& fd = *fds;
+// RESTORE (after the second B half step): the same thread then puts its row's wall values
+// back into the box, as wall_restore does.  The B update reads E only at x = 0 and x = 1;
+// there the restore rewrites the values the box already holds (wall_e left E2, E3 at x = 0
+// equal to their buffers, E1 there is never advanced and stays zero, and no kernel since
+// wall_e writes E).  The writes that change the box (B at x = 0, E1 at x = nx+1) touch
+// cells no B update reads, so the rows need no ordering and one launch does both.
+template <int STEP, int DIMS, bool RESTORE>
+OSR_DEV void wall_b_row(const FieldDesc& fd, long long r) {
   const GridDesc& g = fd.g;
-  if (g.ctrl->halt >= 2) return;
+  const int halt = g.ctrl->halt;  // checked before the first store, the loads in flight
   const int ny = g.ny, nz = g.nz, nx = g.nx;
   const long long R = (long long)ny * nz;
-  const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
   if (r >= R) return;
   const int j = (int)(r % ny), k = (int)(r / ny);
   const long long nc = g.ncell;
@@ -265,24 +309,55 @@ __global__ __launch_bounds__(256) void wall_b(const FieldDesc* __restrict__ fds)
   const float e3j = (STEP == 2 && j == ny - 1) ? stale3[k] : E3[cj];
   float* bb = wall_at(fd, 6, R);
   const float d1 = fd.wall_dt[0], d2 = fd.wall_dt[1], d3 = fd.wall_dt[2];
+  float n0, n1, n2;
   if (DIMS == 2) {
-    bb[r] = bb[r] - d2 * (e3j - E3[c0]);
-    bb[R + r] = bb[R + r] + d1 * (E3[c1] - E3[c0]);
-    bb[2 * R + r] = bb[2 * R + r] - d1 * (E2[c1] - E2[c0]) + d2 * (E1[cj] - E1[c0]);
+    n0 = bb[r] - d2 * (e3j - E3[c0]);
+    n1 = bb[R + r] + d1 * (E3[c1] - E3[c0]);
+    n2 = bb[2 * R + r] - d1 * (E2[c1] - E2[c0]) + d2 * (E1[cj] - E1[c0]);
   } else {
     const long long rk = j + (long long)(k + 1 == nz ? 0 : k + 1) * ny;  // row z+1
     const long long ck = rk * nx + nx - 1;
     const float e2k = (STEP == 2 && k == nz - 1) ? stale2[j] : E2[ck];
-    bb[r] = bb[r] - d2 * (e3j - E3[c0]) + d3 * (e2k - E2[c0]);
-    bb[R + r] = bb[R + r] - d3 * (E1[ck] - E1[c0]) + d1 * (E3[c1] - E3[c0]);
-    bb[2 * R + r] = bb[2 * R + r] - d1 * (E2[c1] - E2[c0]) + d2 * (E1[cj] - E1[c0]);
+    n0 = bb[r] - d2 * (e3j - E3[c0]) + d3 * (e2k - E2[c0]);
+    n1 = bb[R + r] - d3 * (E1[ck] - E1[c0]) + d1 * (E3[c1] - E3[c0]);
+    n2 = bb[2 * R + r] - d1 * (E2[c1] - E2[c0]) + d2 * (E1[cj] - E1[c0]);
   }
+  float el[3], eu[3];
+  const long long row = r * nx;
+  const int nxi = g.open_nx;
+  if (RESTORE) {
+    const float* ebl = wall_at(fd, 0, R);
+    const float* ebu = wall_at(fd, 9, R);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      el[c] = ebl[(2 * c) * R + r];
+      eu[c] = ebu[(2 * c) * R + r];
+    }
+  }
+  if (halt >= 2) return;
+  bb[r] = n0;
+  bb[R + r] = n1;
+  bb[2 * R + r] = n2;
+  if (RESTORE) {
+    const float nb[3] = {n0, n1, n2};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      g.E[c * nc + row + nx - 1] = el[c];
+      g.B[c * nc + row + nx - 1] = nb[c];
+      g.E[c * nc + row + nxi] = eu[c];
+    }
+  }
+}
+
+template <int STEP, int DIMS, bool RESTORE = false>
+__global__ __launch_bounds__(256) void wall_b(const FieldDesc* __restrict__ fds) {
+  wall_b_row<STEP, DIMS, RESTORE>(*fds, (long long)blockIdx.x * 256 + threadIdx.x);
 }
 
 __global__ __launch_bounds__(256) void wall_e(const FieldDesc* __restrict__ fds) {
   const FieldDesc& fd = *fds;
   const GridDesc& g = fd.g;
-  if (g.ctrl->halt >= 2) return;
+  const int halt = g.ctrl->halt;  // checked before the first store, the loads in flight
   const int ny = g.ny, nz = g.nz, nx = g.nx, nxi = g.open_nx;
   const long long R = (long long)ny * nz;
   const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -294,28 +369,34 @@ __global__ __launch_bounds__(256) void wall_e(const FieldDesc* __restrict__ fds)
   float* ebu = wall_at(fd, 9, R);
   float* stale3 = wall_at(fd, 15, R);
   float* stale2 = stale3 + nz;
-  if (j == 0) stale3[k] = ebl[(2 * 2 + 0) * R + r];  // E3(0, 1, z) before this update
-  if (k == 0) stale2[j] = ebl[(2 * 1 + 0) * R + r];  // E2(0, y, 1)
   const long long row = r * nx;
+  // lower wall: E(0) = buf(1) + dc (buf(0) - E(1)); buffers <- E(0), E(1)
+  // upper wall: E(nx+2) = buf(nx+1) + dc (buf(nx+2) - E(nx+1)); buffers <- E(nx+1), E(nx+2)
+  float lb0[3], lb1[3], e1v[3], ub1[3], ub2[3], env[3];
+#pragma unroll
+  for (int p = 1; p <= 2; p++) {
+    const float* E = g.E + p * nc + row;
+    lb0[p] = ebl[(2 * p + 0) * R + r];
+    lb1[p] = ebl[(2 * p + 1) * R + r];
+    e1v[p] = E[0];
+    ub1[p] = ebu[(2 * p + 0) * R + r];
+    ub2[p] = ebu[(2 * p + 1) * R + r];
+    env[p] = E[nxi];
+  }
+  if (halt >= 2) return;
+  if (j == 0) stale3[k] = lb0[2];  // E3(0, 1, z) before this update
+  if (k == 0) stale2[j] = lb0[1];  // E2(0, y, 1)
 #pragma unroll
   for (int p = 1; p <= 2; p++) {
     float* E = g.E + p * nc + row;
-    // lower wall: E(0) = buf(1) + dc (buf(0) - E(1)); buffers <- E(0), E(1)
-    float* b0 = ebl + (2 * p + 0) * R + r;
-    float* b1 = ebl + (2 * p + 1) * R + r;
-    const float e1 = E[0];
-    const float v = *b1 + dc * (*b0 - e1);
+    const float v = lb1[p] + dc * (lb0[p] - e1v[p]);
     E[nx - 1] = v;
-    *b0 = v;
-    *b1 = e1;
-    // upper wall: E(nx+2) = buf(nx+1) + dc (buf(nx+2) - E(nx+1)); buffers <- E(nx+1), E(nx+2)
-    float* u1 = ebu + (2 * p + 0) * R + r;
-    float* u2 = ebu + (2 * p + 1) * R + r;
-    const float en = E[nxi];
-    const float w = *u1 + dc * (*u2 - en);
+    ebl[(2 * p + 0) * R + r] = v;
+    ebl[(2 * p + 1) * R + r] = e1v[p];
+    const float w = ub1[p] + dc * (ub2[p] - env[p]);
     E[nxi + 1] = w;
-    *u1 = en;
-    *u2 = w;
+    ebu[(2 * p + 0) * R + r] = env[p];
+    ebu[(2 * p + 1) * R + r] = w;
   }
 }
 
@@ -515,12 +596,14 @@ __global__ __launch_bounds__(256) void vpml_update(const FieldDesc* __restrict__
 __global__ __launch_bounds__(256) void laser_wall(const FieldDesc* __restrict__ fds) {
   const FieldDesc& fd = *fds;
   const GridDesc& g = fd.g;
-  if (g.ctrl->halt >= 2) return;
+  const int halt = g.ctrl->halt;  // loaded together with the step and the table start
+  const long long step = g.ctrl->steps, first = *fd.las_first;
+  if (halt >= 2) return;
   const int nx = g.nx;
   const long long R = (long long)g.ny * g.nz;  // rows (y, z) of the wall column
   const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
   if (r >= R) return;
-  const long long row = g.ctrl->steps - *fd.las_first;
+  const long long row = step - first;
   if (row < 0 || row >= fd.las_rows) return;
   const long long nc = g.ncell;
   for (int k = 0; k < fd.las_k; k++) {
@@ -557,84 +640,119 @@ OSR_DEV void kmerge(double& s, double& c, double s2, double c2) {
 // One block per simulation: compensated reduction of the step's partial sums into one
 // history row, then the step counter and the halt protocol.  The order of every addition
 // is fixed (strided per-thread sums, then a fixed shuffle tree per wave and the waves in
-// order), so the row is reproducible bit for bit.  The six field columns are reduced in
-// one pass (each thread reads the six contiguous partials of its blocks), and the loads
-// of several blocks are issued before they are accumulated.
+// order), so the row is reproducible bit for bit.  All 6 + nspec columns go through the
+// reduction together: the six field columns in one pass over the blocks (each thread
+// reads the six contiguous partials of its blocks, several blocks' loads issued before
+// they are accumulated), the shuffle trees of all columns interleaved, and the final
+// merge of the 16 wave sums done by one thread per column, so the dependent FP64 chains
+// of the columns overlap instead of following one another.
 constexpr int kFinNT = 1024;
+constexpr int kFinCols = 6 + kMaxSpecies;
 
-// block-wide compensated merge of (s, c) in a fixed order; the result is valid in thread 0
-OSR_DEV void block_kmerge(double& s, double& c, double* w_s, double* w_c) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int off = 32; off > 0; off >>= 1) {
-    const double s2 = __shfl_down(s, off), c2 = __shfl_down(c, off);
-    kmerge(s, c, s2, c2);
-  }
-  if (lane == 0) {
-    w_s[wave] = s;
-    w_c[wave] = c;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0)
-    for (int w = 1; w < kFinNT / 64; w++) kmerge(s, c, w_s[w], w_c[w]);
-  __syncthreads();
-}
-
-__global__ __launch_bounds__(kFinNT) void finalize_step(const FieldDesc* __restrict__ fds,
-                                                        const int* __restrict__ work) {
-  __shared__ double w_s[kFinNT / 64], w_c[kFinNT / 64];
-  const FieldDesc& fd = fds[work[blockIdx.x]];
+// the reduction and the step-end protocol, run by one workgroup of kFinNT threads
+OSR_DEV void finalize_body(const FieldDesc& fd) {
+  __shared__ double w_s[kFinNT / 64][kFinCols], w_c[kFinNT / 64][kFinCols];
   const GridDesc& g = fd.g;
   Ctrl* ctrl = g.ctrl;
-  if (ctrl->halt >= 2) return;
+  // read with the partial sums; a halted simulation writes nothing (the reduction itself
+  // only reads, so it need not wait for the flag)
+  const int halt = ctrl->halt;
   const long long step = ctrl->steps;
-  const int ncol = 6 + fd.nspec;
-  constexpr int U = 4;  // blocks loaded ahead per thread
-  {
-    double s[6] = {0, 0, 0, 0, 0, 0}, c[6] = {0, 0, 0, 0, 0, 0};
-    for (int b0 = threadIdx.x; b0 < fd.nblk; b0 += kFinNT * U) {
-      double v[U][6];
+  const int nspec = fd.nspec;
+  const int ncol = 6 + nspec;
+  constexpr int U = 2;  // blocks loaded ahead per thread (the order of the adds is the
+                       // same for any U)
+  double s[kFinCols], c[kFinCols];
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int b = b0 + u * kFinNT;
+  for (int col = 0; col < kFinCols; col++) s[col] = c[col] = 0.0;
+  for (int b0 = threadIdx.x; b0 < fd.nblk; b0 += kFinNT * U) {
+    double v[U][6];
 #pragma unroll
-        for (int col = 0; col < 6; col++)
-          v[u][col] = b < fd.nblk ? fd.fe_part[(long long)b * 6 + col] : 0.0;
-      }
+    for (int u = 0; u < U; u++) {
+      const int b = b0 + u * kFinNT;
 #pragma unroll
-      for (int u = 0; u < U; u++)
-        if (b0 + u * kFinNT < fd.nblk)
-#pragma unroll
-          for (int col = 0; col < 6; col++) kadd(s[col], c[col], v[u][col]);
+      for (int col = 0; col < 6; col++)
+        v[u][col] = b < fd.nblk ? fd.fe_part[(long long)b * 6 + col] : 0.0;
     }
-    for (int col = 0; col < 6; col++) {
-      block_kmerge(s[col], c[col], w_s, w_c);
-      if (threadIdx.x == 0 && step < fd.hist_cap)
-        fd.hist[step * ncol + col] = (s[col] + c[col]) * fd.fe_scale;
-    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (b0 + u * kFinNT < fd.nblk)
+#pragma unroll
+        for (int col = 0; col < 6; col++) kadd(s[col], c[col], v[u][col]);
   }
-  for (int sp = 0; sp < fd.nspec; sp++) {
+#pragma unroll
+  for (int sp = 0; sp < kMaxSpecies; sp++) {
+    if (sp >= nspec) break;
     const double* ep = fd.ep[sp];
     const int n = fd.ep_n[sp];
-    double s = 0.0, c = 0.0;
     for (int t0 = threadIdx.x; t0 < n; t0 += kFinNT * U) {
       double v[U];
 #pragma unroll
       for (int u = 0; u < U; u++) v[u] = t0 + u * kFinNT < n ? ep[t0 + u * kFinNT] : 0.0;
 #pragma unroll
       for (int u = 0; u < U; u++)
-        if (t0 + u * kFinNT < n) kadd(s, c, v[u]);
+        if (t0 + u * kFinNT < n) kadd(s[6 + sp], c[6 + sp], v[u]);
     }
-    block_kmerge(s, c, w_s, w_c);
-    if (threadIdx.x == 0 && step < fd.hist_cap)
-      fd.hist[step * ncol + 6 + sp] = (s + c) * fd.ep_scale[sp];
   }
+  // per wave: the fixed shuffle tree, all columns at each level
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int col = 0; col < kFinCols; col++) {
+      if (col >= ncol) break;
+      const double s2 = __shfl_down(s[col], off), c2 = __shfl_down(c[col], off);
+      kmerge(s[col], c[col], s2, c2);
+    }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int col = 0; col < kFinCols; col++) {
+      w_s[wave][col] = s[col];
+      w_c[wave][col] = c[col];
+    }
+  }
+  __syncthreads();
+  if (halt >= 2) return;  // uniform over the block, after its last barrier
+  // the waves in order, one thread per column
+  if (threadIdx.x < ncol) {
+    const int col = threadIdx.x;
+    double ss = w_s[0][col], cc = w_c[0][col];
+    for (int w = 1; w < kFinNT / 64; w++) kmerge(ss, cc, w_s[w][col], w_c[w][col]);
+    if (step < fd.hist_cap)
+      fd.hist[step * ncol + col] = (ss + cc) * (col < 6 ? fd.fe_scale : fd.ep_scale[col - 6]);
+  }
+  __syncthreads();  // every read of the control block above precedes its update
   if (threadIdx.x == 0) {
     ctrl->steps = step + 1;
-    if (ctrl->halt == 1)
+    if (halt == 1)
       ctrl->halt = 2;
     else if (ctrl->target > 0 && step + 1 >= ctrl->target)
       ctrl->halt = 3;
   }
+}
+
+__global__ __launch_bounds__(kFinNT) void finalize_step(const FieldDesc* __restrict__ fds,
+                                                        const int* __restrict__ work) {
+  finalize_body(fds[work[blockIdx.x]]);
+}
+
+// Lindman walls: the lower wall's second B update with the restore of the wall values
+// (wall_b<2, DIMS, true>), kFinNT rows per workgroup, and the end of the step in the
+// workgroup that finishes last (a ticket on Ctrl::done), so no kernel launch follows.
+// Every workgroup has read the halt flag before it takes its ticket, so the flag changes
+// only once none of them looks at it any more; the reduction reads partial sums of
+// earlier kernels only.
+template <int DIMS>
+__global__ __launch_bounds__(kFinNT) void wall_b2_end(const FieldDesc* __restrict__ fds) {
+  __shared__ int s_last;
+  const FieldDesc& fd = *fds;
+  wall_b_row<2, DIMS, true>(fd, (long long)blockIdx.x * kFinNT + threadIdx.x);
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = atomicAdd(&fd.g.ctrl->done, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!s_last) return;
+  if (threadIdx.x == 0) fd.g.ctrl->done = 0;
+  finalize_body(fd);
 }
 
 // ------------------------------------------------------------------------------------
@@ -934,6 +1052,8 @@ template __global__ void yee_dbdt<2, false>(const FieldDesc*, const int2*);
 template __global__ void yee_dbdt<3, true>(const FieldDesc*, const int2*);
 template __global__ void yee_dbdt<3, false>(const FieldDesc*, const int2*);
 template __global__ void yee_dedt<2>(const FieldDesc*, const int2*);
+template __global__ void wall_b2_end<2>(const FieldDesc*);
+template __global__ void wall_b2_end<3>(const FieldDesc*);
 template __global__ void vpml_update<0, 2>(const FieldDesc*);
 template __global__ void vpml_update<1, 2>(const FieldDesc*);
 template __global__ void vpml_update<0, 3>(const FieldDesc*);

@@ -43,6 +43,9 @@ struct Ctrl {
   int error;         // sticky: 1 = particle found outside its tile (layout invariant broken)
   long long steps;   // completed steps
   long long target;  // park (halt = 3) once steps reaches target; 0 = no target
+  unsigned done;     // workgroups of a step-ending kernel that have finished (the last one
+                     // ends the step and sets it back to 0)
+  unsigned pad;
 };
 
 struct GridDesc {

@@ -341,6 +341,8 @@ class Simulation:
     def time_kernels(self, nsteps):
         ms = np.zeros(6)
         check(lib().osr_sim_time_kernels(self._h, int(nsteps), _p(ms)))
+        # tile_fill runs beside yee (the field advance with its walls); finalize is the
+        # step end after both
         return {"push_deposit": ms[0], "tile_fill": ms[1], "yee": ms[2], "finalize": ms[3],
                 "step": ms[4]}
 

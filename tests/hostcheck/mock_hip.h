@@ -373,6 +373,12 @@ inline hipError_t m_hipEventSynchronize(hipEvent_t e) {
   }
   return hipSuccess;
 }
+// a stream waiting on an event: the mock runs the event's stream up to the record (an
+// order the device may also take); in capture the launches are recorded in issue order
+inline hipError_t m_hipStreamWaitEvent(hipStream_t, hipEvent_t e, unsigned) {
+  if (!mock::capturing()) m_hipEventSynchronize(e);
+  return hipSuccess;
+}
 extern "C" double g_mock_notready_p;
 inline hipError_t m_hipEventQuery(hipEvent_t e) {
   mock::tick();
@@ -443,6 +449,7 @@ inline hipError_t m_hipGraphLaunch(hipGraphExec_t e, hipStream_t s) {
 #define hipStreamCreateWithFlags m_hipStreamCreateWithFlags
 #define hipStreamDestroy m_hipStreamDestroy
 #define hipStreamSynchronize m_hipStreamSynchronize
+#define hipStreamWaitEvent m_hipStreamWaitEvent
 #define hipDeviceSynchronize m_hipDeviceSynchronize
 #define hipEventCreate m_hipEventCreate
 #define hipEventCreateWithFlags m_hipEventCreateWithFlags

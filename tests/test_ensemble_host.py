@@ -28,10 +28,14 @@ def mock_lib():
         os.path.join(ROOT, "osiris_amd", "csrc", f)
         for f in os.listdir(os.path.join(ROOT, "osiris_amd", "csrc"))]
     if not os.path.exists(out) or os.path.getmtime(out) < max(os.path.getmtime(s) for s in srcs):
+        # each process builds its own file and renames it into place, so parallel test
+        # workers never load a library another one is still writing
+        tmp = f"{out}.{os.getpid()}.tmp"
         subprocess.run([HIPCC, "--offload-arch=gfx950", "-O1", "-g", "-std=c++17", "-fPIC",
                         "-shared", "-rdynamic", "-Xclang", "-target-feature", "-Xclang",
-                        "-packed-fp32-ops", "-I", os.path.join(ROOT, "include"), "-o", out,
+                        "-packed-fp32-ops", "-I", os.path.join(ROOT, "include"), "-o", tmp,
                         os.path.join(HC, "capi_mock.hip")], check=True)
+        os.replace(tmp, out)
     return out
 
 
