@@ -87,6 +87,29 @@ module m_osiris_amd
     real(c_double) :: xmin(3) = 0.0_c_double
   end type osr_zpulse_wall
 
+  ! the in-box pulse (zpulse section) along x1, launched at set-up; defaults as the
+  ! reference's and osr_zpulse_defaults
+  type, bind(c), public :: osr_zpulse
+    real(c_double) :: a0 = 1.0_c_double, omega0 = 10.0_c_double
+    real(c_double) :: phase = 0.0_c_double
+    real(c_double) :: pol = 90.0_c_double
+    integer(c_int) :: pol_type = 0
+    integer(c_int) :: propagation = OSR_PROP_FORWARD
+    integer(c_int) :: lon_type = OSR_TENV_POLYNOMIAL
+    integer(c_int) :: per_type = OSR_PER_PLANE
+    real(c_double) :: lon_start = -huge(1.0_c_double)
+    real(c_double) :: lon_fwhm = -1.0_c_double
+    real(c_double) :: lon_rise = 0.0_c_double, lon_flat = 0.0_c_double, lon_fall = 0.0_c_double
+    real(c_double) :: lon_duration = 0.0_c_double
+    real(c_double) :: lon_x0 = -huge(1.0_c_double), lon_range = -huge(1.0_c_double)
+    real(c_double) :: per_center(2) = -huge(1.0_c_double)
+    real(c_double) :: per_w0 = 0.0_c_double, per_fwhm = 0.0_c_double
+    real(c_double) :: per_focus = -huge(1.0_c_double)
+    integer(c_int) :: per_tem_mode(2) = 0
+    integer(c_int) :: no_div_corr = 0
+    real(c_double) :: xmin(3) = 0.0_c_double
+  end type osr_zpulse
+
   public :: osr_version, osr_device_count, osr_error_message
   public :: osr_advance_deposit, osr_yee_advance
   public :: hip_vadvance_deposit_2d_s1, hip_vadvance_deposit_2d_s2, hip_vadvance_deposit_2d_s3
@@ -98,7 +121,8 @@ module m_osiris_amd
             osr_sim_launch_config
   public :: osr_smooth_setup, osr_smooth_vdf, osr_sim_set_smooth
   public :: osr_sim_write_hist, osr_sim_save_field
-  public :: osr_sim_add_zpulse_wall, osr_sim_zpulse_column, osr_sim_init_fields
+  public :: osr_sim_add_zpulse_wall, osr_sim_zpulse_column, osr_sim_init_fields, &
+            osr_sim_add_zpulse
 
   interface
     function osr_last_error() bind(c, name="osr_last_error")
@@ -378,6 +402,12 @@ module m_osiris_amd
       type(c_ptr), value :: sim
       type(osr_zpulse_wall), intent(in) :: zp
       integer(c_int) :: osr_sim_add_zpulse_wall
+    end function
+    function osr_sim_add_zpulse(sim, zp) bind(c, name="osr_sim_add_zpulse")
+      import :: c_int, c_ptr, osr_zpulse
+      type(c_ptr), value :: sim
+      type(osr_zpulse), intent(in) :: zp
+      integer(c_int) :: osr_sim_add_zpulse
     end function
     function osr_sim_zpulse_column(sim, k, second, t, b12, b3, active) &
         bind(c, name="osr_sim_zpulse_column")

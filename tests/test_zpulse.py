@@ -138,8 +138,9 @@ def test_library_defaults_are_the_reference_defaults():
 
 
 def test_fortran_binding_layout():
-    """The bind(c) osr_zpulse_wall of m_osiris_amd has the C size and, with its default
-    initialisers, the values osr_zpulse_wall_defaults writes (host code, no GPU)."""
+    """The bind(c) osr_zpulse_wall and osr_zpulse of m_osiris_amd have the C sizes and,
+    with their default initialisers, the values osr_zpulse_wall_defaults and
+    osr_zpulse_defaults write (host code, no GPU)."""
     import os
     import subprocess
 
@@ -152,6 +153,8 @@ def test_fortran_binding_layout():
     assert r.returncode == 0, r.stdout + r.stderr
     lines = r.stdout.splitlines()
     assert lines[0] == f"size {C.sizeof(L.ZpulseWall)}"
-    f = [ln.split(None, 1)[1] for ln in lines[1:] if ln.startswith("fortran")]
-    c = [ln.split(None, 1)[1] for ln in lines[1:] if ln.startswith("c ")]
-    assert len(f) == 2 and f == c
+    assert f"size2 {C.sizeof(L.ZpulseBox)}" in lines
+    for ft, ct in (("fortran ", "c "), ("fortran2 ", "c2 ")):
+        f = [ln.split(None, 1)[1] for ln in lines if ln.startswith(ft)]
+        c = [ln.split(None, 1)[1] for ln in lines if ln.startswith(ct)]
+        assert len(f) == 2 and f == c
