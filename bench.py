@@ -208,6 +208,18 @@ def run_c3(args, oa):
     t_launch = kt["push_deposit"] * 1e-3
     bytes_launch = n_total * BYTES_PER_PARTICLE_3D + cells * BYTES_PER_CELL_PUSH
     achieved = bytes_launch / t_launch / 1e9
+    traffic = None
+    tj_path = os.path.join(ROOT, "profiles", "traffic_push_deposit_3d_c3.json")
+    if os.path.exists(tj_path):
+        try:
+            with open(tj_path) as f:
+                tj = json.load(f)
+            # counters measured on the same kernel instantiation and launch shape only
+            if (tuple(tj.get("tile", ())) == tuple(tile) and lc["compiled_tile"]
+                    and tj.get("push_waves") == lc["push_waves"]):
+                traffic = tj.get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
     relayouts = sim.relayouts
     sim.close()
     out = {
@@ -223,7 +235,7 @@ def run_c3(args, oa):
                    "autotune_ms_per_step": tune, "reorder": args.reorder,
                    "relayouts": relayouts},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": (f"push_deposit_3d_t<3,1,{tile[0]},{tile[1]},{tile[2]}>"
                                 if lc["compiled_tile"] else "push_deposit_3d<3,1,256>"),
                      "kernel_ms": t_launch * 1e3,
@@ -376,7 +388,7 @@ def run_c4(args, oa):
                    "field_energy_last": float(fld[-1].sum()),
                    "particle_energy_last": float(par[-1].sum())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "push_deposit_2d_t<2,0,%d,%d>" % tile, "kernel_ms": t_launch * 1e3,
                      "algorithmic_bytes": bytes_launch},
         "kernels_ms_per_step": kt,
