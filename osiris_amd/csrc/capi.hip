@@ -188,6 +188,19 @@ struct osr_sim {
   DBuf zp_tab, zp_mask, zp_first;  // FieldDesc::las_* (kZpRows steps per fill)
   Ctrl* h_ctrl = nullptr;  // pinned mirror
   int64_t desc_version = 0;  // bumped whenever the device descriptors change
+
+  osr_sim() = default;
+  osr_sim(const osr_sim&) = delete;
+  osr_sim& operator=(const osr_sim&) = delete;
+  // Also runs when osr_sim_create fails part way: the stream, graphs and pinned mirror
+  // go with the handle; the device buffers are released by their own destructors.
+  ~osr_sim() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (gexec_k) (void)hipGraphExecDestroy(gexec_k);
+    if (h_ctrl) (void)hipHostFree(h_ctrl);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
 };
 
 namespace {
@@ -1351,13 +1364,7 @@ int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
 }
 
 int osr_sim_destroy(osr_sim* s) {
-  if (!s) return OSR_OK;
-  if (s->stream) (void)hipStreamSynchronize(s->stream);
-  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
-  if (s->gexec_k) (void)hipGraphExecDestroy(s->gexec_k);
-  if (s->h_ctrl) (void)hipHostFree(s->h_ctrl);
-  if (s->stream) (void)hipStreamDestroy(s->stream);
-  delete s;
+  delete s;  // ~osr_sim waits for the stream and releases everything
   return OSR_OK;
 }
 

@@ -274,7 +274,7 @@ __global__ __launch_bounds__(256, 2) void push_deposit_3d_t(const SpecDesc* __re
       dep_vp3<ORDER>(sJ, JN, wx, wy, wz, vp[0].x0, vp[0].y0, vp[0].z0, vp[0].x1, vp[0].y1,
                      vp[0].z1, vp[0].q, nb0, JX, JXY, jn1, jn2, jn3, jscale);
     const unsigned long long below = (1ull << lane) - 1ull;
-#pragma unroll
+#pragma unroll 1
     for (int v = 1; v < 4; v++) {
       const unsigned long long m = __ballot(v < nvp);
       if (m == 0ull) break;
