@@ -471,29 +471,32 @@ OSR_DEV VP3 mk3(float x0, float y0, float z0, float x1, float y1, float z1, floa
 
 OSR_DEV int split3d(float xo, float yo, float zo, float xn, float yn, float zn, int d1, int d2,
                     int d3, float q, VP3* vp) {
-  const int cross = (d1 != 0) + 2 * (d2 != 0) + 4 * (d3 != 0);
+  const bool cx = d1 != 0, cy = d2 != 0, cz = d3 != 0;
+  const int cross = cx + 2 * cy + 4 * cz;
   float xint, yint, zint, xint2, yint2, zint2, delta;
   const float f1 = (float)d1, f2 = (float)d2, f3 = (float)d3;
-  switch (cross) {
-  case 0:
+  if (cross == 0) {
     vp[0] = mk3(xo, yo, zo, xn, yn, zn, q, 0, 0, 0);
     return 1;
-  case 1:
-    xint = 0.5f * f1;
-    delta = (xint - xo) / (xn - xo);
-    yint = yo + (yn - yo) * delta;
-    zint = zo + (zn - zo) * delta;
+  }
+  if (cross == 1 || cross == 2 || cross == 4) {
+    // One face crossed (cases 1, 2 and 4 of the reference): one division on the crossed
+    // axis and the same interpolation of the other two, so a wave whose lanes cross
+    // different faces runs one path instead of three.  Each field is the expression of
+    // its case (a shift of 0 leaves a coordinate exactly as the case leaves it).
+    const float fk = cx ? f1 : (cy ? f2 : f3);
+    const float ok = cx ? xo : (cy ? yo : zo), nk = cx ? xn : (cy ? yn : zn);
+    const float tint = 0.5f * fk;
+    delta = (tint - ok) / (nk - ok);
+    xint = cx ? tint : xo + (xn - xo) * delta;
+    yint = cy ? tint : yo + (yn - yo) * delta;
+    zint = cz ? tint : zo + (zn - zo) * delta;
     vp[0] = mk3(xo, yo, zo, xint, yint, zint, q, 0, 0, 0);
-    vp[1] = mk3(-xint, yint, zint, xn - f1, yn, zn, q, d1, 0, 0);
+    vp[1] = mk3(cx ? -xint : xint, cy ? -yint : yint, cz ? -zint : zint, xn - f1, yn - f2,
+                zn - f3, q, d1, d2, d3);
     return 2;
-  case 2:
-    yint = 0.5f * f2;
-    delta = (yint - yo) / (yn - yo);
-    xint = xo + (xn - xo) * delta;
-    zint = zo + (zn - zo) * delta;
-    vp[0] = mk3(xo, yo, zo, xint, yint, zint, q, 0, 0, 0);
-    vp[1] = mk3(xint, -yint, zint, xn, yn - f2, zn, q, 0, d2, 0);
-    return 2;
+  }
+  switch (cross) {
   case 3:
     xint = 0.5f * f1;
     delta = (xint - xo) / (xn - xo);
@@ -517,14 +520,6 @@ OSR_DEV int split3d(float xo, float yo, float zo, float xn, float yn, float zn, 
       vp[2] = mk3(-xint, yint - f2, zint, xn - f1, yn - f2, zn, q, d1, d2, 0);
     }
     return 3;
-  case 4:
-    zint = 0.5f * f3;
-    delta = (zint - zo) / (zn - zo);
-    xint = xo + (xn - xo) * delta;
-    yint = yo + (yn - yo) * delta;
-    vp[0] = mk3(xo, yo, zo, xint, yint, zint, q, 0, 0, 0);
-    vp[1] = mk3(xint, yint, -zint, xn, yn, zn - f3, q, 0, 0, d3);
-    return 2;
   case 5:
     xint = 0.5f * f1;
     delta = (xint - xo) / (xn - xo);
