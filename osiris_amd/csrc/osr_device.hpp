@@ -15,6 +15,49 @@ namespace osr {
 
 OSR_DEV float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
+// ------------------------------------------------------------------------------------
+// Correctly rounded FP32 reciprocal, quotient and square root for the operands of the
+// push path.  Each starts
+// from the hardware estimate (v_rcp_f32, v_rsq_f32, accurate to about 1 ulp) and
+// corrects it with fused multiply-adds, after Markstein (P. Markstein, "Computation of
+// elementary functions on the IBM RISC System/6000 processor", IBM J. Res. Dev. 34,
+// 1990; and "IA-64 and Elementary Functions", 2000, ch. 8):
+//  * rcp_rn: one Newton step y = r + r (1 - b r) from the estimate r;
+//  * div_rn: q = a y, the exact remainder a - b q, and q + (a - b q) y.  With y the
+//    correctly rounded 1/b this is the correctly rounded a/b;
+//  * sqrt_rn: g = x y and h = y/2 from the estimate y of 1/sqrt(x), one coupled Newton
+//    step on both, then g + (x - g g) h.
+// The sequences are correctly rounded on the ranges below, checked bit for bit against
+// the IEEE operations on the device: rcp_rn for every normal b with a normal reciprocal;
+// sqrt_rn for every x >= 2^-100 (below about 2^-102 the remainder underflows); div_rn
+// for every pair of mantissas, which stands for every pair of normal operands with
+// |a| >= 2^-79 (the remainder then stays normal) whose quotient is normal, and for +0
+// numerators (a -0 numerator gives +0).  The push path
+// uses them only there: sqrt_rn on 1 + u^2, div_rn on numerators that are tem, 1, 2 or
+// +-1/2 - x0 (at least 2^-25 in magnitude, or exactly +0).  The kinetic energy
+// u^2 / (gamma + 1), whose numerator can be denormal, and Vay's inner roots keep the
+// IEEE operations.  The compiler's IEEE sequences (-fhip-fp32-correctly-rounded-divide-
+// sqrt) also scale operands for the denormal and overflow ranges; the shorter forms
+// leave that out.
+// ------------------------------------------------------------------------------------
+OSR_DEV float rcp_rn(float b) {
+  const float r = __builtin_amdgcn_rcpf(b);
+  return fma_(fma_(-b, r, 1.0f), r, r);
+}
+OSR_DEV float div_rn(float a, float b) {
+  const float y = rcp_rn(b);
+  const float q = a * y;
+  return fma_(fma_(-b, q, a), y, q);
+}
+OSR_DEV float sqrt_rn(float x) {
+  const float y = __builtin_amdgcn_rsqf(x);
+  float g = x * y, h = 0.5f * y;
+  const float r = fma_(-g, h, 0.5f);
+  g = fma_(g, r, g);
+  h = fma_(h, r, h);
+  return fma_(fma_(-g, g, x), h, g);
+}
+
 // Stencil parameters per interpolation order (emf/os-emf-interpolate.f03:385-461,
 // spec/os-spec-current.f03:2495-2533).  LP: lower stencil point of spline/wl.
 template <int ORDER> struct Ord {
@@ -114,9 +157,9 @@ OSR_DEV float boris(float tem, float e1, float e2, float e3, float b1, float b2,
   e3 = e3 * tem;
   float ut1 = u1 + e1, ut2 = u2 + e2, ut3 = u3 + e3;
   float utsq = fma_(ut3, ut3, fma_(ut2, ut2, ut1 * ut1));
-  float gamma = __builtin_sqrtf(utsq + 1.0f);
-  float ene = utsq / (gamma + 1.0f);
-  float tg = tem / gamma;
+  float gamma = sqrt_rn(utsq + 1.0f);
+  float ene = utsq / (gamma + 1.0f);  // IEEE: utsq can be denormal
+  float tg = div_rn(tem, gamma);
   b1 = b1 * tg;
   b2 = b2 * tg;
   b3 = b3 * tg;
@@ -129,7 +172,7 @@ OSR_DEV float boris(float tem, float e1, float e2, float e3, float b1, float b2,
   float otsq = fma_(b1, b1, 1.0f);
   otsq = fma_(b2, b2, otsq);
   otsq = fma_(b3, b3, otsq);
-  otsq = 2.0f / otsq;
+  otsq = div_rn(2.0f, otsq);
   b1 = b1 * otsq;
   b2 = b2 * otsq;
   b3 = b3 * otsq;
@@ -153,7 +196,7 @@ OSR_DEV float vay(float tem, float e1, float e2, float e3, float b1, float b2, f
                   float& u1, float& u2, float& u3) {
   e1 *= tem; e2 *= tem; e3 *= tem;
   b1 *= tem; b2 *= tem; b3 *= tem;
-  float rgam = 1.0f / __builtin_sqrtf(1.0f + u1 * u1 + u2 * u2 + u3 * u3);
+  float rgam = rcp_rn(sqrt_rn(1.0f + u1 * u1 + u2 * u2 + u3 * u3));
   float v1 = u1 * rgam, v2 = u2 * rgam, v3 = u3 * rgam;
   float bpsq = b1 * b1 + b2 * b2 + b3 * b3;
   float ut1 = u1 + e1 + v2 * b3 - v3 * b2;
@@ -165,7 +208,7 @@ OSR_DEV float vay(float tem, float e1, float e2, float e3, float b1, float b2, f
   rgam = 1.0f / __builtin_sqrtf(0.5f * (sigma + __builtin_sqrtf(sigma * sigma +
                                                                  4.0f * (bpsq + ustar * ustar))));
   b1 *= rgam; b2 *= rgam; b3 *= rgam;
-  float spar = 1.0f / (1.0f + b1 * b1 + b2 * b2 + b3 * b3);
+  float spar = rcp_rn(1.0f + b1 * b1 + b2 * b2 + b3 * b3);
   float uptp = w1 * b1 + w2 * b2 + w3 * b3;
   float o1 = u1, o2 = u2, o3 = u3;
   u1 = spar * (w1 + uptp * b1 + w2 * b3 - w3 * b2);
@@ -173,8 +216,8 @@ OSR_DEV float vay(float tem, float e1, float e2, float e3, float b1, float b2, f
   u3 = spar * (w3 + uptp * b3 + w1 * b2 - w2 * b1);
   float uc1 = 0.5f * (o1 + u1), uc2 = 0.5f * (o2 + u2), uc3 = 0.5f * (o3 + u3);
   float uc2s = uc1 * uc1 + uc2 * uc2 + uc3 * uc3;
-  float gc = __builtin_sqrtf(uc2s + 1.0f);
-  return uc2s / (gc + 1.0f);
+  float gc = sqrt_rn(uc2s + 1.0f);
+  return uc2s / (gc + 1.0f);  // IEEE: uc2s can be denormal
 }
 
 template <int PUSHER>
@@ -186,7 +229,7 @@ OSR_DEV float push(float tem, float e1, float e2, float e3, float b1, float b2, 
 
 // 1/gamma: simd/os-spec-push-avx2.c:156-161 (VRGAMMA)
 OSR_DEV float rgamma(float u1, float u2, float u3) {
-  return 1.0f / __builtin_sqrtf(fma_(u3, u3, fma_(u2, u2, fma_(u1, u1, 1.0f))));
+  return rcp_rn(sqrt_rn(fma_(u3, u3, fma_(u2, u2, fma_(u1, u1, 1.0f)))));
 }
 
 // Cell shift keeping x in [-0.5, 0.5): spec/os-spec-push.f03:1332-1359, vmntrim
@@ -279,27 +322,27 @@ OSR_DEV int split2d(float xo, float yo, float xn, float yn, int d1, int d2, floa
   int n = 1;
   if (cross == 1) {
     const float xint = 0.5f * f1;
-    const float delta = (xint - xo) / (xn - xo);
+    const float delta = div_rn(xint - xo, xn - xo);
     const float yint = yo + (yn - yo) * delta;
     a = {xo, yo, xint, yint, q, vz * delta, 0, 0};
     b = {-xint, yint, xn - f1, yn, q, vz * (1.0f - delta), d1, 0};
     n = 2;
   } else if (cross == 2) {
     const float yint = 0.5f * f2;
-    const float delta = (yint - yo) / (yn - yo);
+    const float delta = div_rn(yint - yo, yn - yo);
     const float xint = xo + (xn - xo) * delta;
     a = {xo, yo, xint, yint, q, vz * delta, 0, 0};
     b = {xint, -yint, xn, yn - f2, q, vz * (1.0f - delta), 0, d2};
     n = 2;
   } else if (cross == 3) {
     const float xint = 0.5f * f1;
-    float delta = (xint - xo) / (xn - xo);
+    float delta = div_rn(xint - xo, xn - xo);
     const float yint = yo + (yn - yo) * delta;
     if (yint >= -0.5f && yint < 0.5f) {
       const float vzint = vz * delta;
       const float vzint2 = vz * (1.0f - delta);
       const float yint2 = 0.5f * f2;
-      delta = (yint2 - yint) / (yn - yint);
+      delta = div_rn(yint2 - yint, yn - yint);
       const float xint2 = -xint + (xn - xint) * delta;
       a = {xo, yo, xint, yint, q, vzint, 0, 0};
       b = {-xint, yint, xint2, yint2, q, vzint2 * delta, d1, 0};
@@ -307,7 +350,7 @@ OSR_DEV int split2d(float xo, float yo, float xn, float yn, int d1, int d2, floa
     } else {
       const float vzint = vz * delta;
       const float yint2 = 0.5f * f2;
-      delta = (yint2 - yo) / (yint - yo);
+      delta = div_rn(yint2 - yo, yint - yo);
       const float xint2 = xo + (xint - xo) * delta;
       a = {xo, yo, xint2, yint2, q, vzint * delta, 0, 0};
       b = {xint2, -yint2, xint, yint - f2, q, vzint * (1.0f - delta), 0, d2};
@@ -336,7 +379,7 @@ OSR_DEV int split2d_fast(float xo, float yo, float xn, float yn, int d1, int d2,
     const float xint = 0.5f * f1, yface = 0.5f * f2;
     const float num = cx ? xint - xo : yface - yo;
     const float den = cx ? xn - xo : yn - yo;
-    const float delta = num / den;
+    const float delta = div_rn(num, den);
     const float o0 = cx ? yo : xo, o1 = cx ? yn : xn;
     const float oint = o0 + (o1 - o0) * delta;  // yint (x face) or xint (y face)
     a.x1 = cx ? xint : oint;
@@ -355,14 +398,14 @@ OSR_DEV int split2d_fast(float xo, float yo, float xn, float yn, int d1, int d2,
       if (yint >= -0.5f && yint < 0.5f) {
         const float vzint = vz * delta;
         const float vzint2 = vz * (1.0f - delta);
-        const float delta2 = (yint2 - yint) / (yn - yint);
+        const float delta2 = div_rn(yint2 - yint, yn - yint);
         const float xint2 = -xint + (xn - xint) * delta2;
         a = {xo, yo, xint, yint, q, vzint, 0, 0};
         b = {-xint, yint, xint2, yint2, q, vzint2 * delta2, d1, 0};
         c = {xint2, -yint2, xn - f1, yn - f2, q, vzint2 * (1.0f - delta2), d1, d2};
       } else {
         const float vzint = vz * delta;
-        const float delta2 = (yint2 - yo) / (yint - yo);
+        const float delta2 = div_rn(yint2 - yo, yint - yo);
         const float xint2 = xo + (xint - xo) * delta2;
         a = {xo, yo, xint2, yint2, q, vzint * delta2, 0, 0};
         b = {xint2, -yint2, xint, yint - f2, q, vzint * (1.0f - delta2), 0, d2};
@@ -487,7 +530,7 @@ OSR_DEV int split3d(float xo, float yo, float zo, float xn, float yn, float zn, 
     const float fk = cx ? f1 : (cy ? f2 : f3);
     const float ok = cx ? xo : (cy ? yo : zo), nk = cx ? xn : (cy ? yn : zn);
     const float tint = 0.5f * fk;
-    delta = (tint - ok) / (nk - ok);
+    delta = div_rn(tint - ok, nk - ok);
     xint = cx ? tint : xo + (xn - xo) * delta;
     yint = cy ? tint : yo + (yn - yo) * delta;
     zint = cz ? tint : zo + (zn - zo) * delta;
@@ -499,12 +542,12 @@ OSR_DEV int split3d(float xo, float yo, float zo, float xn, float yn, float zn, 
   switch (cross) {
   case 3:
     xint = 0.5f * f1;
-    delta = (xint - xo) / (xn - xo);
+    delta = div_rn(xint - xo, xn - xo);
     yint = yo + (yn - yo) * delta;
     zint = zo + (zn - zo) * delta;
     if (yint >= -0.5f && yint < 0.5f) {
       yint2 = 0.5f * f2;
-      delta = (yint2 - yint) / (yn - yint);
+      delta = div_rn(yint2 - yint, yn - yint);
       xint2 = -xint + (xn - xint) * delta;
       zint2 = zint + (zn - zint) * delta;
       vp[0] = mk3(xo, yo, zo, xint, yint, zint, q, 0, 0, 0);
@@ -512,7 +555,7 @@ OSR_DEV int split3d(float xo, float yo, float zo, float xn, float yn, float zn, 
       vp[2] = mk3(xint2, -yint2, zint2, xn - f1, yn - f2, zn, q, d1, d2, 0);
     } else {
       yint2 = 0.5f * f2;
-      delta = (yint2 - yo) / (yint - yo);
+      delta = div_rn(yint2 - yo, yint - yo);
       xint2 = xo + (xint - xo) * delta;
       zint2 = zo + (zint - zo) * delta;
       vp[0] = mk3(xo, yo, zo, xint2, yint2, zint2, q, 0, 0, 0);
@@ -522,19 +565,19 @@ OSR_DEV int split3d(float xo, float yo, float zo, float xn, float yn, float zn, 
     return 3;
   case 5:
     xint = 0.5f * f1;
-    delta = (xint - xo) / (xn - xo);
+    delta = div_rn(xint - xo, xn - xo);
     yint = yo + (yn - yo) * delta;
     zint = zo + (zn - zo) * delta;
     zint2 = 0.5f * f3;
     if (zint >= -0.5f && zint < 0.5f) {
-      delta = (zint2 - zint) / (zn - zint);
+      delta = div_rn(zint2 - zint, zn - zint);
       xint2 = -xint + (xn - xint) * delta;
       yint2 = yint + (yn - yint) * delta;
       vp[0] = mk3(xo, yo, zo, xint, yint, zint, q, 0, 0, 0);
       vp[1] = mk3(-xint, yint, zint, xint2, yint2, zint2, q, d1, 0, 0);
       vp[2] = mk3(xint2, yint2, -zint2, xn - f1, yn, zn - f3, q, d1, 0, d3);
     } else {
-      delta = (zint2 - zo) / (zint - zo);
+      delta = div_rn(zint2 - zo, zint - zo);
       xint2 = xo + (xint - xo) * delta;
       yint2 = yo + (yint - yo) * delta;
       vp[0] = mk3(xo, yo, zo, xint2, yint2, zint2, q, 0, 0, 0);
@@ -544,19 +587,19 @@ OSR_DEV int split3d(float xo, float yo, float zo, float xn, float yn, float zn, 
     return 3;
   case 6:
     yint = 0.5f * f2;
-    delta = (yint - yo) / (yn - yo);
+    delta = div_rn(yint - yo, yn - yo);
     xint = xo + (xn - xo) * delta;
     zint = zo + (zn - zo) * delta;
     zint2 = 0.5f * f3;
     if (zint >= -0.5f && zint < 0.5f) {
-      delta = (zint2 - zint) / (zn - zint);
+      delta = div_rn(zint2 - zint, zn - zint);
       xint2 = xint + (xn - xint) * delta;
       yint2 = -yint + (yn - yint) * delta;
       vp[0] = mk3(xo, yo, zo, xint, yint, zint, q, 0, 0, 0);
       vp[1] = mk3(xint, -yint, zint, xint2, yint2, zint2, q, 0, d2, 0);
       vp[2] = mk3(xint2, yint2, -zint2, xn, yn - f2, zn - f3, q, 0, d2, d3);
     } else {
-      delta = (zint2 - zo) / (zint - zo);
+      delta = div_rn(zint2 - zo, zint - zo);
       xint2 = xo + (xint - xo) * delta;
       yint2 = yo + (yint - yo) * delta;
       vp[0] = mk3(xo, yo, zo, xint2, yint2, zint2, q, 0, 0, 0);
@@ -566,12 +609,12 @@ OSR_DEV int split3d(float xo, float yo, float zo, float xn, float yn, float zn, 
     return 3;
   default: {  // 7: x, y and z crossings
     xint = 0.5f * f1;
-    delta = (xint - xo) / (xn - xo);
+    delta = div_rn(xint - xo, xn - xo);
     yint = yo + (yn - yo) * delta;
     zint = zo + (zn - zo) * delta;
     if (yint >= -0.5f && yint < 0.5f) {
       yint2 = 0.5f * f2;
-      delta = (yint2 - yint) / (yn - yint);
+      delta = div_rn(yint2 - yint, yn - yint);
       xint2 = -xint + (xn - xint) * delta;
       zint2 = zint + (zn - zint) * delta;
       vp[0] = mk3(xo, yo, zo, xint, yint, zint, q, 0, 0, 0);
@@ -579,7 +622,7 @@ OSR_DEV int split3d(float xo, float yo, float zo, float xn, float yn, float zn, 
       vp[2] = mk3(xint2, -yint2, zint2, xn - f1, yn - f2, zn, q, d1, d2, 0);
     } else {
       yint2 = 0.5f * f2;
-      delta = (yint2 - yo) / (yint - yo);
+      delta = div_rn(yint2 - yo, yint - yo);
       xint2 = xo + (xint - xo) * delta;
       zint2 = zo + (zint - zo) * delta;
       vp[0] = mk3(xo, yo, zo, xint2, yint2, zint2, q, 0, 0, 0);
@@ -596,7 +639,7 @@ OSR_DEV int split3d(float xo, float yo, float zo, float xn, float yn, float zn, 
     if (!(c0 || c1 || c2)) return 3;
     const int s = c0 ? 0 : (c1 ? 1 : 2);
     const VP3 o = pick(vp, s);
-    delta = (zint - o.z0) / (o.z1 - o.z0);
+    delta = div_rn(zint - o.z0, o.z1 - o.z0);
     xint = o.x0 + (o.x1 - o.x0) * delta;
     yint = o.y0 + (o.y1 - o.y0) * delta;
     vp[3] = mk3(xint, yint, -zint, o.x1, o.y1, o.z1 - f3, o.q, o.di, o.dj, o.dk + d3);
