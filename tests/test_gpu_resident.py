@@ -3,7 +3,7 @@ drop-in path.  Tolerances (FP32 GPU vs FP64 oracle):
 
 * cell indices: identical for every particle that does not end within FP32 rounding of
   a cell edge during the compared steps;
-* energy histories: 1e-4 relative (the north-star tolerance) over 20-50 steps;
+* energy histories: 1e-4 relative (the FP32 path's stated tolerance against FP64) over 20-50 steps;
 * Gauss law: the change of div E - rho stays at FP32 round-off of E (1e-5 of max|rho|).
 """
 import numpy as np
@@ -120,7 +120,8 @@ def test_thermal_2d_vs_oracle(order):
 
 
 def test_two_stream_2d_vs_oracle():
-    """Counter-streaming beams (config 2 physics at 64x64): energy histories and moments."""
+    """Counter-streaming beams (the 1024x1024 two-stream workload's physics at 64x64):
+    energy histories and moments."""
     rng = np.random.default_rng(44)
     nx, dx, dt = (64, 64), (0.05, 0.05), 0.035
     sp = []
@@ -131,7 +132,7 @@ def test_two_stream_2d_vs_oracle():
 
 
 def test_c1_thermal_128_full_run_vs_fp64_oracle():
-    """The reference's CPU-runnable configuration at full size: 2D periodic thermal
+    """The reference's CPU-runnable workload at full size: 2D periodic thermal
     plasma, 128x128 cells, dx = 0.1, dt = 0.07, 16 ppc at uniform random positions,
     u_th = 0.05, linear splines, Boris, 500 steps.  FP32 on the GPU against the FP64
     oracle: energy histories within 1e-4 (the FP32 restatement of the oracle stays within
@@ -146,7 +147,8 @@ def test_c1_thermal_128_full_run_vs_fp64_oracle():
 
 
 def test_magnetized_3d_vay_vs_oracle():
-    """Config 3 physics at 16^3: cubic, Vay, uniform B0 = 0.5 z."""
+    """The 256^3 magnetized thermal plasma's physics at 16^3: cubic, Vay, uniform
+    B0 = 0.5 z."""
     rng = np.random.default_rng(45)
     nx, dx, dt = (16, 16, 16), (0.1, 0.1, 0.1), 0.05
     ix, x, u, q = thermal_plasma(rng, 3, nx, 2, 0.1)
