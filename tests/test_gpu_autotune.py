@@ -100,3 +100,29 @@ def test_push_waves_requests_are_checked():
         _two_stream({"tile": (8, 8), "push_waves": 8})
     with pytest.raises(oa.OsrError, match="no push kernel with 8 waves"):
         _two_stream({"tile": (16, 16), "push_waves": 8, "kernel": 1})
+
+
+@pytest.mark.parametrize("order,pusher", [(1, oa.OSR_BORIS), (3, oa.OSR_BORIS),
+                                          (1, oa.OSR_VAY), (2, oa.OSR_VAY), (3, oa.OSR_VAY)])
+def test_waves_change_no_result_every_order_and_pusher(order, pusher):
+    """Every compile-time 2D instantiation with 4 and with 8 waves per workgroup (16x16
+    and 32x16 tiles) gives the bits of the generic kernel, for each order and pusher."""
+    nx, dx, dt = (64, 48), (0.05, 0.05), 0.035
+    ref = None
+    for tile, waves, kernel in (((16, 16), 0, 1), ((16, 16), 4, 0), ((16, 16), 8, 0),
+                                ((32, 16), 4, 0), ((32, 16), 8, 0)):
+        print(f"case order={order} pusher={pusher} tile={tile} waves={waves} kernel={kernel}",
+              flush=True)
+        with oa.Simulation(2, nx, dx, dt, order, tile=tile, push_waves=waves,
+                           kernel=kernel) as sim:
+            for k, ufl in enumerate((0.2, -0.2)):
+                sid = sim.add_species(-1.0, pusher)
+                sim.generate(sid, (4, 4), density=0.5, ufl=(ufl, 0, 0), uth=(0.05,) * 3,
+                             seed=71 + k)
+            sim.fill_fields(b0=(0.1, 0.0, 0.3))
+            sim.run(20)
+            st = _state(sim, 2)
+        if ref is None:
+            ref = st
+        else:
+            _assert_same(ref, st)
