@@ -81,6 +81,9 @@ def cpu_baseline(nx, ppc_total, n_total, order, dx, dt, seconds, threads):
 
     if not O.ref_available("s"):
         return None
+    # the -Ofast build (the reference's production optimisation level) when present
+    fast = O.ref_available("s_fast")
+    flags = "gcc -Ofast -mavx2 -mfma" if fast else "gcc -O2 -mavx2 -mfma"
     rng = np.random.default_rng(12345)
     rows = max(1, min(nx[1], 4_194_304 // (nx[0] * ppc_total)))
     g = O.make_grid(2, nx, order)
@@ -98,7 +101,7 @@ def cpu_baseline(nx, ppc_total, n_total, order, dx, dt, seconds, threads):
     u[: n // 2, 0] += 0.2
     u[n // 2:, 0] -= 0.2
     q = np.full(n, -1.0 / ppc_total, np.float32)
-    timer = O.RefSimdTimer(order, ix, x, u, q, -1.0, e, b, glo, dx, dt, threads)
+    timer = O.RefSimdTimer(order, ix, x, u, q, -1.0, e, b, glo, dx, dt, threads, fast=fast)
     timer.step()  # warm
     reps, t0 = 0, time.perf_counter()
     while True:
@@ -122,7 +125,7 @@ def cpu_baseline(nx, ppc_total, n_total, order, dx, dt, seconds, threads):
         "cores": threads,
         "kind": "reference",
         "sample": (f"reference FNAME(vadvance_deposit_2d_s{order}) AVX2 FP32 "
-                   f"(simd/os-spec-push-avx2.c, gcc -O2 -mavx2 -mfma) on {n} particles "
+                   f"(simd/os-spec-push-avx2.c, {flags}) on {n} particles "
                    f"({nx[0]}x{rows} cells x {ppc_total} ppc, two beams), {threads} threads "
                    f"with private J + reduction, {reps} steps in {el:.1f} s; plus FP32 Yee "
                    f"(oracle) on the full {nx[0]}x{nx[1]} grid ({t_yee * 1e3:.1f} ms); "
@@ -603,6 +606,8 @@ def main():
         "plain_path": plain,
     }
     if not args.no_cpu:
+        # at most 16 threads: the host cores allotted to one GPU (os.cpu_count() reports
+        # the whole node)
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         cb = cpu_baseline(nx, 2 * ppc_beam, n_total, args.order, dx, dt, args.cpu_seconds,
                           threads)

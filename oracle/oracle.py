@@ -338,6 +338,8 @@ def _cpu_has_avx2() -> bool:
 
 
 def _ref(prec: str):
+    """The reference's AVX2 kernels built by `make -C oracle ref`: prec "s" (FP32) or "d"
+    (FP64) with -O2, or "s_fast" (FP32 with -Ofast, the build the CPU baseline times)."""
     key = "ref_" + prec
     if key not in _LIBS:
         path = os.path.join(_HERE, "_ref", f"libosiris_simd_{prec}.so")
@@ -558,7 +560,8 @@ def ref_vadvance_deposit(dims, order, ix, x, u, q, rqm, e, b, emf_lo, j, j_lo, d
     double) because the reference writes dummy particles past i1
     (os-spec-push-avx2.c:523-542).  Returns (ix, x, u, j, ene)."""
     L = _ref(prec)
-    real = np.float32 if prec == "s" else np.float64
+    single = prec.startswith("s")
+    real = np.float32 if single else np.float64
     fn = getattr(L, f"vadvance_deposit_{dims}d_s{order}_")
     n = len(q)
     pad = n + 16
@@ -579,7 +582,7 @@ def ref_vadvance_deposit(dims, order, ix, x, u, q, rqm, e, b, emf_lo, j, j_lo, d
     j_off = np.array([j_lo] * dims, np.int32)
     i0 = C.c_int(1)
     i1 = C.c_int(n)
-    rq = C.c_float(rqm) if prec == "s" else C.c_double(rqm)
+    rq = C.c_float(rqm) if single else C.c_double(rqm)
     dxa = np.array(list(dx), np.float64)
     dtc = C.c_double(dt)
     ene = C.c_double(0.0)
@@ -807,10 +810,10 @@ class RefSimdTimer:
     the call, ~1% of its time): positions and momenta keep evolving, and no particle
     can drift past the guard cells, which the kernel does not check."""
 
-    def __init__(self, order, ix, x, u, q, rqm, e, b, glo, dx, dt, threads):
+    def __init__(self, order, ix, x, u, q, rqm, e, b, glo, dx, dt, threads, fast=False):
         import threading  # noqa: F401
 
-        self.L = _ref("s")
+        self.L = _ref("s_fast" if fast else "s")
         self.fn = getattr(self.L, f"vadvance_deposit_2d_s{order}_")
         self.threads = threads
         n = len(q)

@@ -58,6 +58,32 @@ def test_oracle_matches_reference_simd_golden(dims, order, prec):
     assert abs(ene - rene) <= (1e-13 if prec == "d" else 1e-6) * abs(rene)
 
 
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_fast_reference_build_computes_the_same_push(order):
+    """The -Ofast build of the reference's FP32 2D kernel, the one the CPU baseline times,
+    pushes the golden inputs as the -O2 build does: same cells, positions and momenta to
+    FP32 reassociation (a few ulp), same current and energy to 2e-6."""
+    if not (O.ref_available("s") and O.ref_available("s_fast")):
+        pytest.skip("reference AVX2 kernels not built (oracle/_ref)")
+    G = np.load(GOLD)
+    key = f"2d_s{order}"
+    dt, dx0, lo = G[key + "_par"]
+    lo = int(lo)
+    g = O.make_grid(2, (6, 6), order)
+    args = [G[key + "_in_" + a].astype(np.int32 if a == "ix" else np.float32)
+            for a in ("ix", "x", "u", "q")]
+    e = G[key + "_in_e"].astype(np.float32)
+    b = G[key + "_in_b"].astype(np.float32)
+    out = [O.ref_vadvance_deposit(2, order, *args, -1.0, e, b, lo, g.zeros(3, np.float32), lo,
+                                  (dx0, dx0), dt, prec=p) for p in ("s", "s_fast")]
+    (ix0, x0, u0, j0, e0), (ix1, x1, u1, j1, e1) = out
+    assert np.array_equal(ix0, ix1)
+    assert np.abs(x1 - x0).max() <= 4e-7
+    assert np.abs(u1 - u0).max() <= 4e-7 * max(1.0, np.abs(u0).max())
+    assert np.abs(j1 - j0).max() <= 2e-6 * np.abs(j0).max()
+    assert abs(e1 - e0) <= 1e-6 * abs(e0)
+
+
 def test_reference_fp32_3d_cubic_deposit_discrepancy_is_documented():
     """The FP32 AVX2 3D cubic reference deposit differs from the FP64 one (known bug of
     the reference); everything else of that kernel agrees."""
