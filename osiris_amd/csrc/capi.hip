@@ -1271,7 +1271,8 @@ int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
   }
   g.ny = desc->nx[1];
   g.nz = dims == 3 ? desc->nx[2] : 1;
-  g.ncell = (long long)g.nx * g.ny * g.nz;
+  g.ncell = (long long)g.nx * g.ny * g.nz;  // < 2^32 within the extents accepted above
+  set_grid_divisors(g);
   g.tx = tile[0];
   g.ty = tile[1];
   g.tz = tile[2];
@@ -2128,6 +2129,7 @@ int osr_yee_advance(int dims, const int* nx, const int* ld, const int* gcl, cons
     if (nx[d] < 1 || gcl[d] < 1 || ld[d] < nx[d] + gcl[d] + 2)
       return fail(OSR_ERR_ARG, "Yee needs 1 lower and 2 upper guard cells");
   const size_t nc = (size_t)nxs[0] * nxs[1] * nxs[2];
+  if (nc >= (1ull << 32)) return fail(OSR_ERR_ARG, "grid of 2^32 cells or more");
   std::vector<float> pe(3 * nc), pb(3 * nc), pj(3 * nc);
   const size_t ld0 = ld[0], ld1 = ld[1];
   auto ref = [&](int i, int j, int k) {
@@ -2162,6 +2164,7 @@ int osr_yee_advance(int dims, const int* nx, const int* ld, const int* gcl, cons
   f.g.ny = nxs[1];
   f.g.nz = nxs[2];
   f.g.ncell = (long long)nc;
+  set_grid_divisors(f.g);
   f.g.E = dE.as<float>();
   f.g.B = dB.as<float>();
   f.g.ctrl = dctrl.as<Ctrl>();

@@ -76,6 +76,24 @@ constexpr int kWallEntries = 15;
 
 OSR_DEV int wrp(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
+// coordinates of cell c of the grid (c < ncell < 2^32): x, and the row y + ny z split into
+// y and z, with the grid's multiply-high divisors
+OSR_DEV void cell_xyz(const GridDesc& g, long long c, int& i, int& j, int& k) {
+  const uint32_t u = (uint32_t)c;
+  const uint32_t row = g.div_nx.div(u);
+  const uint32_t z = g.div_ny.div(row);
+  i = (int)(u - row * (uint32_t)g.nx);
+  j = (int)(row - z * (uint32_t)g.ny);
+  k = (int)z;
+}
+// a wall row r = y + ny z (r < ny nz) split into y and z
+OSR_DEV void row_yz(const GridDesc& g, long long r, int& j, int& k) {
+  const uint32_t u = (uint32_t)r;
+  const uint32_t z = g.div_ny.div(u);
+  j = (int)(u - z * (uint32_t)g.ny);
+  k = (int)z;
+}
+
 // 256-cell blocks per workgroup of the B half step that also sums the field energy: the
 // FP64 block reduction is paid once per 1024 cells
 constexpr int kEneBlk = 4;
@@ -125,8 +143,8 @@ __global__ __launch_bounds__(256) void yee_dbdt(const FieldDesc* __restrict__ fd
     e3j[qb] = e3i[qb] = e2i[qb] = e1j[qb] = e2k[qb] = e1k[qb] = 0.f;
     if (idx[qb] >= nc) continue;
     const long long id = idx[qb];
-    const int i = (int)(id % nx);
-    const int j = (int)((id / nx) % ny);
+    int i, j, k;
+    cell_xyz(g, id, i, j, k);
     const long long ip = id - i + wrp(i + 1, nx);
     const long long jp = id + (long long)(wrp(j + 1, ny) - j) * nx;
     ii[qb] = i;
@@ -142,7 +160,6 @@ __global__ __launch_bounds__(256) void yee_dbdt(const FieldDesc* __restrict__ fd
     e1j[qb] = E1[jp];
     if (DIMS == 3) {
       const int nz = g.nz;
-      const int k = (int)(id / ((long long)nx * ny));
       const long long kp = id + (long long)(wrp(k + 1, nz) - k) * nx * ny;
       e2k[qb] = E2[kp];
       e1k[qb] = E1[kp];
@@ -207,8 +224,8 @@ __global__ __launch_bounds__(256) void yee_dedt(const FieldDesc* __restrict__ fd
   if (idx >= nc) return;
   const float d1 = fd.edt[0], d2 = fd.edt[1], d3 = fd.edt[2], dtif = fd.dtif;
   const int nx = g.nx, ny = g.ny;
-  const int i = (int)(idx % nx);
-  const int j = (int)((idx / nx) % ny);
+  int i, j, k;
+  cell_xyz(g, idx, i, j, k);
   const long long im = idx - i + wrp(i - 1, nx);
   const long long jm = idx + (long long)(wrp(j - 1, ny) - j) * nx;
   float* E1 = g.E;
@@ -255,7 +272,6 @@ __global__ __launch_bounds__(256) void yee_dedt(const FieldDesc* __restrict__ fd
     e3 = e3 - dtif * j3 + d1 * (b2 - B2[im]) - d2 * (b1 - B1[jm]);
   } else {
     const int nz = g.nz;
-    const int k = (int)(idx / ((long long)nx * ny));
     const long long km = idx + (long long)(wrp(k - 1, nz) - k) * nx * ny;
     e1 = e1 - dtif * j1 + d2 * (b3 - B3[jm]) - d3 * (b2 - B2[km]);
     e2 = e2 - dtif * j2 + d3 * (b1 - B1[km]) - d1 * (b3 - B3[im]);
@@ -296,7 +312,8 @@ OSR_DEV void wall_b_row(const FieldDesc& fd, long long r) {
   const int ny = g.ny, nz = g.nz, nx = g.nx;
   const long long R = (long long)ny * nz;
   if (r >= R) return;
-  const int j = (int)(r % ny), k = (int)(r / ny);
+  int j, k;
+  row_yz(g, r, j, k);
   const long long nc = g.ncell;
   const float* E1 = g.E;
   const float* E2 = g.E + nc;
@@ -362,7 +379,8 @@ __global__ __launch_bounds__(256) void wall_e(const FieldDesc* __restrict__ fds)
   const long long R = (long long)ny * nz;
   const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
   if (r >= R) return;
-  const int j = (int)(r % ny), k = (int)(r / ny);
+  int j, k;
+  row_yz(g, r, j, k);
   const long long nc = g.ncell;
   const float dc = fd.wall_dc;
   float* ebl = wall_at(fd, 0, R);
@@ -492,7 +510,8 @@ __global__ __launch_bounds__(256) void vpml_update(const FieldDesc* __restrict__
   const long long r = t % R;
   const int l = (int)((t / R) % n);
   const int wl = (int)(t / (R * n));
-  const int j = (int)(r % ny), k = (int)(r / ny);
+  int j, k;
+  row_yz(g, r, j, k);
   const long long nc = g.ncell;
   // ring columns of the wall's interface cells: x = 1 and 0 (lower), nx+1 and nx+2 (upper)
   const int btv = wl == 0 ? n - 1 : 0, vtb = wl == 0 ? n - 2 : 1;
@@ -660,23 +679,62 @@ OSR_DEV void finalize_body(const FieldDesc& fd) {
   const long long step = ctrl->steps;
   const int nspec = fd.nspec;
   const int ncol = 6 + nspec;
-  constexpr int U = 2;  // blocks loaded ahead per thread (the order of the adds is the
-                       // same for any U)
+  constexpr int U = 2;  // items per column loaded ahead per thread (the order of the adds
+                       // is the same for any U)
   double s[kFinCols], c[kFinCols];
 #pragma unroll
   for (int col = 0; col < kFinCols; col++) s[col] = c[col] = 0.0;
-  for (int b0 = threadIdx.x; b0 < fd.nblk; b0 += kFinNT * U) {
+  const int tid = threadIdx.x;
+  const int nblk = fd.nblk;
+  // first pass: a thread's first U items of the field columns and of the first kPre
+  // species, all loads issued before the first addition (one memory round trip instead
+  // of one per column group); kPre bounds the registers the loads ahead take
+  constexpr int kPre = 2;
+  {
+    double vf[U][6], vs[kPre][U];
+    int epn[kPre];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int b = tid + u * kFinNT;
+#pragma unroll
+      for (int col = 0; col < 6; col++)
+        vf[u][col] = b < nblk ? fd.fe_part[(long long)b * 6 + col] : 0.0;
+    }
+#pragma unroll
+    for (int sp = 0; sp < kPre; sp++) {
+      epn[sp] = sp < nspec ? fd.ep_n[sp] : 0;
+      const double* ep = fd.ep[sp];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int t = tid + u * kFinNT;
+        vs[sp][u] = t < epn[sp] ? ep[t] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (tid + u * kFinNT < nblk)
+#pragma unroll
+        for (int col = 0; col < 6; col++) kadd(s[col], c[col], vf[u][col]);
+#pragma unroll
+    for (int sp = 0; sp < kPre; sp++)
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (tid + u * kFinNT < epn[sp]) kadd(s[6 + sp], c[6 + sp], vs[sp][u]);
+  }
+  // the remaining items of every column (many cell blocks or tiles, more species), in
+  // order
+  for (int b0 = tid + U * kFinNT; b0 < nblk; b0 += kFinNT * U) {
     double v[U][6];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int b = b0 + u * kFinNT;
 #pragma unroll
       for (int col = 0; col < 6; col++)
-        v[u][col] = b < fd.nblk ? fd.fe_part[(long long)b * 6 + col] : 0.0;
+        v[u][col] = b < nblk ? fd.fe_part[(long long)b * 6 + col] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < U; u++)
-      if (b0 + u * kFinNT < fd.nblk)
+      if (b0 + u * kFinNT < nblk)
 #pragma unroll
         for (int col = 0; col < 6; col++) kadd(s[col], c[col], v[u][col]);
   }
@@ -685,7 +743,7 @@ OSR_DEV void finalize_body(const FieldDesc& fd) {
     if (sp >= nspec) break;
     const double* ep = fd.ep[sp];
     const int n = fd.ep_n[sp];
-    for (int t0 = threadIdx.x; t0 < n; t0 += kFinNT * U) {
+    for (int t0 = tid + (sp < kPre ? U * kFinNT : 0); t0 < n; t0 += kFinNT * U) {
       double v[U];
 #pragma unroll
       for (int u = 0; u < U; u++) v[u] = t0 + u * kFinNT < n ? ep[t0 + u * kFinNT] : 0.0;
@@ -835,14 +893,13 @@ __global__ __launch_bounds__(256) void gauss_residual(GridDesc g, const unsigned
   if (idx >= g.ncell) return;
   const long long nc = g.ncell;
   const int nx = g.nx, ny = g.ny;
-  const int i = (int)(idx % nx);
-  const int j = (int)((idx / nx) % ny);
+  int i, j, k;
+  cell_xyz(g, idx, i, j, k);
   const long long im = idx - i + wrp(i - 1, nx);
   const long long jm = idx + (long long)(wrp(j - 1, ny) - j) * nx;
   double dv = ((double)g.E[idx] - (double)g.E[im]) * r1 +
               ((double)g.E[nc + idx] - (double)g.E[nc + jm]) * r2;
   if (g.dims == 3) {
-    const int k = (int)(idx / ((long long)nx * ny));
     const long long km = idx + (long long)(wrp(k - 1, g.nz) - k) * nx * ny;
     dv += ((double)g.E[2 * nc + idx] - (double)g.E[2 * nc + km]) * r3;
   }

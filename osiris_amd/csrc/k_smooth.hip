@@ -39,18 +39,20 @@ __global__ __launch_bounds__(256) void smooth_pass(const FieldDesc* __restrict__
   const int nx = g.nx, ny = g.ny;
   const int dir = fd.sm_dir[pass], n = fd.sm_n[pass];
   const float* s = fd.sm_coef[pass] + 15;
+  int ci, cj, ck;
+  cell_xyz(g, idx, ci, cj, ck);
   int c, len;
   long long st;
   if (dir == 0) {
-    c = (int)(idx % nx);
+    c = ci;
     len = nx;
     st = 1;
   } else if (dir == 1) {
-    c = (int)((idx / nx) % ny);
+    c = cj;
     len = ny;
     st = nx;
   } else {
-    c = (int)(idx / ((long long)nx * ny));
+    c = ck;
     len = g.nz;
     st = (long long)nx * ny;
   }

@@ -52,6 +52,38 @@ OSR_DEV int block_rank(bool flag, int* s_wtot, int& total) {
   return before + r;
 }
 
+// block_rank for PF flags per thread at once, ranked in the order (p, thread): item p of
+// thread t comes after every item p' < p and after item p of the threads below t.  One
+// LDS exchange (two barriers) for all PF; all threads must call it.
+template <int NT, int PF>
+OSR_DEV void block_rank_multi(const bool (&flag)[PF], int (*s_cnt)[NT / 64], int (&rank)[PF],
+                              int& total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  int own[PF];
+#pragma unroll
+  for (int p = 0; p < PF; p++) {
+    const unsigned long long m = __ballot(flag[p]);
+    own[p] = __popcll(m & below);
+    if (lane == 0) s_cnt[p][wave] = __popcll(m);
+  }
+  __syncthreads();
+  int base = 0;
+#pragma unroll
+  for (int p = 0; p < PF; p++) {
+    int before = 0, tp = 0;
+    for (int w = 0; w < NT / 64; w++) {
+      const int c = s_cnt[p][w];
+      if (w < wave) before += c;
+      tp += c;
+    }
+    rank[p] = base + before + own[p];
+    base += tp;
+  }
+  total = base;
+  __syncthreads();
+}
+
 // ------------------------------------------------------------------------------------
 // 2D push + deposit
 // ------------------------------------------------------------------------------------
@@ -592,6 +624,8 @@ OSR_DEV int seg_find(const int* ofs, int n, int i) {
   return lo;
 }
 
+constexpr int kFillPF = 8;  // candidate movers per thread whose cells are loaded together
+
 template <int NT, int INC_MAX, int TAIL_BITS>
 __global__ __launch_bounds__(NT) void tile_fill(const SpecDesc* __restrict__ descs,
                                                 const int2* __restrict__ work) {
@@ -601,6 +635,7 @@ __global__ __launch_bounds__(NT) void tile_fill(const SpecDesc* __restrict__ des
   __shared__ int s_lofs[MAXL + 1];
   __shared__ int s_lbase[MAXL];
   __shared__ int s_wtot[NT / 64];
+  __shared__ int s_pcnt[kFillPF][NT / 64];
   __shared__ int s_hpre[17];
   __shared__ int s_inc[INC_MAX];
   __shared__ unsigned int s_bits[TAIL_BITS / 32];
@@ -651,20 +686,33 @@ __global__ __launch_bounds__(NT) void tile_fill(const SpecDesc* __restrict__ des
   const bool absorb = (tile % g.ntx) * g.tx >= interior_nx(g);
   const int T = absorb ? 0 : s_lofs[L];
   const int H = s_hpre[NW];
-  // pass 1: collect the incoming movers
+  // pass 1: collect the incoming movers, in candidate order.  Candidates are taken PF NT
+  // at a time, candidate c0 + p NT + thread as item p of the thread, and the cells of all
+  // of them are loaded before any is ranked: one memory round trip per PF NT candidates
+  // (a tile's neighbours hold some 8 x 200 movers on the 2D two-stream workload).
   int I = 0;
-  for (int c0 = 0; c0 < T; c0 += NT) {
-    const int i = c0 + threadIdx.x;
-    int src = -1;
-    bool f = false;
-    if (i < T) {
-      const int l = seg_find(s_lofs, L, i);
-      src = s_lbase[l] + (i - s_lofs[l]);
-      f = tile_of(g, d.mcell[src]) == tile;
+  for (int c0 = 0; c0 < T; c0 += NT * kFillPF) {
+    uint32_t cell[kFillPF];
+    int src[kFillPF];
+#pragma unroll
+    for (int p = 0; p < kFillPF; p++) {
+      const int i = c0 + p * NT + threadIdx.x;
+      src[p] = -1;
+      cell[p] = 0u;
+      if (i < T) {
+        const int l = seg_find(s_lofs, L, i);
+        src[p] = s_lbase[l] + (i - s_lofs[l]);
+        cell[p] = d.mcell[src[p]];
+      }
     }
-    int tot;
-    const int r = block_rank<NT>(f, s_wtot, tot);
-    if (f && I + r < INC_MAX) s_inc[I + r] = src;
+    bool f[kFillPF];
+#pragma unroll
+    for (int p = 0; p < kFillPF; p++) f[p] = src[p] >= 0 && tile_of(g, cell[p]) == tile;
+    int r[kFillPF], tot;
+    block_rank_multi<NT, kFillPF>(f, s_pcnt, r, tot);
+#pragma unroll
+    for (int p = 0; p < kFillPF; p++)
+      if (f[p] && I + r[p] < INC_MAX) s_inc[I + r[p]] = src[p];
     I += tot;
   }
   const int nn = n - H + I;

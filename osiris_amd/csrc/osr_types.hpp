@@ -36,6 +36,31 @@ __host__ __device__ constexpr int push2d_default_waves(int tx, int ty) {
   return tx * ty >= 512 ? 8 : 4;
 }
 
+// Unsigned division of 32-bit values by a divisor fixed per grid, as a multiply-high and
+// two shifts: the round-up method of Granlund and Montgomery ("Division by invariant
+// integers using multiplication", PLDI 1994, fig. 4.1), exact for every dividend below
+// 2^32 and every divisor 1 <= d <= 2^31.  With l = ceil(log2 d) and
+// m = floor(2^32 (2^l - d) / d) + 1:  q = (t + ((n - t) >> 1)) >> (l - 1), t = mulhi(m, n)
+// (d = 1: m = 1, both shifts 0).  The field kernels split cell indices (< 2^32 for every
+// grid the library accepts) into coordinates with it instead of 64-bit divisions.
+struct FastDiv {
+  uint32_t m;
+  uint32_t s1, s2;
+  uint32_t d;
+  __host__ void set(uint32_t dv) {
+    uint32_t l = 0;
+    while ((1ull << l) < dv) l++;
+    d = dv;
+    m = (uint32_t)((((1ull << 32) * ((1ull << l) - dv)) / dv) + 1);
+    s1 = l > 0 ? 1 : 0;
+    s2 = l > 0 ? l - 1 : 0;
+  }
+  __host__ __device__ uint32_t div(uint32_t n) const {
+    const uint32_t t = (uint32_t)(((uint64_t)m * n) >> 32);
+    return (t + ((n - t) >> s1)) >> s2;
+  }
+};
+
 // Control block, one per simulation (device memory).
 struct Ctrl {
   int halt;          // 0 run, 1 a tile overflowed this step, 2 stopped at a step end,
@@ -64,7 +89,14 @@ struct GridDesc {
   float dt_dx[3];    // (float)(dt/dx)
   float jnorm[3];    // (float)(dx/dt/2) in 2D, (float)(dx/dt/3) in 3D
   Ctrl* ctrl;
+  FastDiv div_nx, div_ny;  // cell index c -> x = c mod nx, row c / nx; row -> y, z
 };
+
+// the divisors of a grid whose extents are set (host)
+__host__ inline void set_grid_divisors(GridDesc& g) {
+  g.div_nx.set((uint32_t)g.nx);
+  g.div_ny.set((uint32_t)g.ny);
+}
 
 struct SpecDesc {
   GridDesc g;
