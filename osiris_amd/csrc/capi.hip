@@ -125,6 +125,19 @@ struct DBuf {
   }
 };
 
+// Copy between host and device ordered on stream `st`: it runs after the work already
+// queued there (a download sees the kernels launched on `st` before it) and is complete
+// when the call returns, so kernels launched afterwards on any stream see an upload and
+// the host buffer may be reused.  A plain hipMemcpy runs on the null stream, which the
+// library's non-blocking streams do not wait for: a kernel launched on one of them right
+// after it may read the device memory as it was before the copy.
+int copy_on(hipStream_t st, void* dst, const void* src, size_t n, hipMemcpyKind kind) {
+  if (n == 0) return OSR_OK;
+  HIPC(hipMemcpyAsync(dst, src, n, kind, st));
+  HIPC(hipStreamSynchronize(st));
+  return OSR_OK;
+}
+
 struct Species {
   float rqm = -1.f;
   int pusher = OSR_BORIS;
@@ -407,9 +420,9 @@ void make_field_desc(const osr_sim* s, FieldDesc& f) {
 int upload_descs(osr_sim* s) {
   std::vector<SpecDesc> v(kMaxSpecies);
   make_spec_descs(s, v.data());
-  HIPC(hipMemcpy(s->d_spec.p, v.data(), sizeof(SpecDesc) * kMaxSpecies, hipMemcpyHostToDevice));
+  TRY(copy_on(s->stream, s->d_spec.p, v.data(), sizeof(SpecDesc) * kMaxSpecies, hipMemcpyHostToDevice));
   make_field_desc(s, s->fd);
-  HIPC(hipMemcpy(s->d_fd.p, &s->fd, sizeof(FieldDesc), hipMemcpyHostToDevice));
+  TRY(copy_on(s->stream, s->d_fd.p, &s->fd, sizeof(FieldDesc), hipMemcpyHostToDevice));
   s->desc_version++;
   return OSR_OK;
 }
@@ -473,9 +486,9 @@ int alloc_species(osr_sim* s, Species& p, const std::vector<int>& cnt) {
     TRY(p.pl_vp.alloc(7 * 3 * 4 * n));
     TRY(p.pl_nvp.alloc(4 * n));
   }
-  HIPC(hipMemcpy(p.off.p, off.data(), 4 * nt, hipMemcpyHostToDevice));
-  HIPC(hipMemcpy(p.cnt.p, cnt.data(), 4 * nt, hipMemcpyHostToDevice));
-  HIPC(hipMemcpy(p.cap.p, cap.data(), 4 * nt, hipMemcpyHostToDevice));
+  TRY(copy_on(s->stream, p.off.p, off.data(), 4 * nt, hipMemcpyHostToDevice));
+  TRY(copy_on(s->stream, p.cnt.p, cnt.data(), 4 * nt, hipMemcpyHostToDevice));
+  TRY(copy_on(s->stream, p.cap.p, cap.data(), 4 * nt, hipMemcpyHostToDevice));
   HIPC(hipMemsetAsync(p.mv_cnt.p, 0, 4 * (size_t)nt * kMaxPushWaves, s->stream));
   HIPC(hipMemsetAsync(p.mv_stride.p, 0, 4 * (size_t)nt, s->stream));
   HIPC(hipMemsetAsync(p.epart.p, 0, 8 * (size_t)p.epart_n, s->stream));
@@ -490,10 +503,8 @@ int alloc_species(osr_sim* s, Species& p, const std::vector<int>& cnt) {
 }
 
 template <class T>
-int upload(DBuf& b, const std::vector<T>& v) {
-  if (v.empty()) return OSR_OK;
-  HIPC(hipMemcpy(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
-  return OSR_OK;
+int upload(hipStream_t st, DBuf& b, const std::vector<T>& v) {
+  return copy_on(st, b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
 }
 
 // Lay a host particle list out in tiles (order inside a tile: round robin over the
@@ -571,15 +582,15 @@ int layout_upload(osr_sim* s, Species& p, const HostParts& hp) {
       q[o] = hp.q[i];
     }
   }
-  TRY(upload(p.cell, cell));
-  TRY(upload(p.pid, pid));
-  TRY(upload(p.x, x));
-  TRY(upload(p.y, y));
-  if (s->dims == 3) TRY(upload(p.z, z));
-  TRY(upload(p.ux, ux));
-  TRY(upload(p.uy, uy));
-  TRY(upload(p.uz, uz));
-  TRY(upload(p.q, q));
+  TRY(upload(s->stream, p.cell, cell));
+  TRY(upload(s->stream, p.pid, pid));
+  TRY(upload(s->stream, p.x, x));
+  TRY(upload(s->stream, p.y, y));
+  if (s->dims == 3) TRY(upload(s->stream, p.z, z));
+  TRY(upload(s->stream, p.ux, ux));
+  TRY(upload(s->stream, p.uy, uy));
+  TRY(upload(s->stream, p.uz, uz));
+  TRY(upload(s->stream, p.q, q));
   double qa = 0.0, qm = 0.0;
   for (size_t i = 0; i < n; i++) {
     qa += std::fabs((double)hp.q[i]);
@@ -591,10 +602,9 @@ int layout_upload(osr_sim* s, Species& p, const HostParts& hp) {
 }
 
 template <class T>
-int download(std::vector<T>& v, const DBuf& b, size_t n) {
+int download(hipStream_t st, std::vector<T>& v, const DBuf& b, size_t n) {
   v.resize(n);
-  if (n) HIPC(hipMemcpy(v.data(), b.p, sizeof(T) * n, hipMemcpyDeviceToHost));
-  return OSR_OK;
+  return copy_on(st, v.data(), b.p, sizeof(T) * n, hipMemcpyDeviceToHost);
 }
 
 // Collect the valid particles of a species.  `pending`: the last push moved leavers
@@ -604,33 +614,33 @@ int collect(osr_sim* s, Species& p, bool pending, HostParts& hp) {
   const int W = s->push_waves;  // wave count of the push that filled the mover buffers
   const size_t C = (size_t)p.cap_total;
   std::vector<int> cnt, mvc, mvs, midx;
-  TRY(download(cnt, p.cnt, nt));
+  TRY(download(s->stream, cnt, p.cnt, nt));
   std::vector<uint32_t> cell, pid;
   std::vector<float> x, y, z, ux, uy, uz, q;
-  TRY(download(cell, p.cell, C));
-  TRY(download(pid, p.pid, C));
-  TRY(download(x, p.x, C));
-  TRY(download(y, p.y, C));
-  if (s->dims == 3) TRY(download(z, p.z, C));
-  TRY(download(ux, p.ux, C));
-  TRY(download(uy, p.uy, C));
-  TRY(download(uz, p.uz, C));
-  TRY(download(q, p.q, C));
+  TRY(download(s->stream, cell, p.cell, C));
+  TRY(download(s->stream, pid, p.pid, C));
+  TRY(download(s->stream, x, p.x, C));
+  TRY(download(s->stream, y, p.y, C));
+  if (s->dims == 3) TRY(download(s->stream, z, p.z, C));
+  TRY(download(s->stream, ux, p.ux, C));
+  TRY(download(s->stream, uy, p.uy, C));
+  TRY(download(s->stream, uz, p.uz, C));
+  TRY(download(s->stream, q, p.q, C));
   std::vector<uint32_t> mcell, mpid;
   std::vector<float> mx, my, mz, mux, muy, muz, mq;
   if (pending) {
-    TRY(download(mvc, p.mv_cnt, (size_t)nt * W));
-    TRY(download(mvs, p.mv_stride, nt));
-    TRY(download(midx, p.midx, C));
-    TRY(download(mcell, p.mcell, C));
-    TRY(download(mpid, p.mpid, C));
-    TRY(download(mx, p.mx, C));
-    TRY(download(my, p.my, C));
-    if (s->dims == 3) TRY(download(mz, p.mz, C));
-    TRY(download(mux, p.mux, C));
-    TRY(download(muy, p.muy, C));
-    TRY(download(muz, p.muz, C));
-    TRY(download(mq, p.mq, C));
+    TRY(download(s->stream, mvc, p.mv_cnt, (size_t)nt * W));
+    TRY(download(s->stream, mvs, p.mv_stride, nt));
+    TRY(download(s->stream, midx, p.midx, C));
+    TRY(download(s->stream, mcell, p.mcell, C));
+    TRY(download(s->stream, mpid, p.mpid, C));
+    TRY(download(s->stream, mx, p.mx, C));
+    TRY(download(s->stream, my, p.my, C));
+    if (s->dims == 3) TRY(download(s->stream, mz, p.mz, C));
+    TRY(download(s->stream, mux, p.mux, C));
+    TRY(download(s->stream, muy, p.muy, C));
+    TRY(download(s->stream, muz, p.muz, C));
+    TRY(download(s->stream, mq, p.mq, C));
   }
   hp = HostParts();
   hp.reserve((size_t)p.np);
@@ -652,7 +662,7 @@ int collect(osr_sim* s, Species& p, bool pending, HostParts& hp) {
   // after a halted step, tiles whose fill ran have inc >= 0 and already hold their
   // incoming movers; the others still have their holes and their movers are pending
   std::vector<int> inc;
-  if (pending) TRY(download(inc, p.inc, nt));
+  if (pending) TRY(download(s->stream, inc, p.inc, nt));
   std::vector<char> hole;
   for (int t = 0; t < nt; t++) {
     const int base = p.h_off[t];
@@ -720,9 +730,9 @@ int regrow_species(osr_sim* s, int sid, bool* ok) {
   const int nt = s->ntiles;
   *ok = false;
   std::vector<int> cnt, inc, need;
-  TRY(download(cnt, p.cnt, nt));
-  TRY(download(inc, p.inc, nt));
-  TRY(download(need, p.newcnt, nt));
+  TRY(download(s->stream, cnt, p.cnt, nt));
+  TRY(download(s->stream, inc, p.inc, nt));
+  TRY(download(s->stream, need, p.newcnt, nt));
   std::vector<int2> over;
   for (int t = 0; t < nt; t++)
     if (inc[t] < 0) {
@@ -762,35 +772,35 @@ int regrow_species(osr_sim* s, int sid, bool* ok) {
   DBuf d_off, d_work;
   TRY(d_off.alloc(4 * (size_t)nt));
   TRY(d_work.alloc(sizeof(int2) * over.size()));
-  HIPC(hipMemcpy(d_off.p, off.data(), 4 * (size_t)nt, hipMemcpyHostToDevice));
-  HIPC(hipMemcpy(d_work.p, over.data(), sizeof(int2) * over.size(), hipMemcpyHostToDevice));
+  TRY(copy_on(s->stream, d_off.p, off.data(), 4 * (size_t)nt, hipMemcpyHostToDevice));
+  TRY(copy_on(s->stream, d_work.p, over.data(), sizeof(int2) * over.size(), hipMemcpyHostToDevice));
   hipLaunchKernelGGL(regrow_move, dim3((unsigned)nt), dim3(256), 0, s->stream,
                      s->d_spec.as<SpecDesc>(), sid, src, dst, d_off.as<int>());
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(s->stream));
   for (int a = 0; a < 19; a++) arr[a]->swap(nb[a]);  // the old arrays go with nb
-  HIPC(hipMemcpy(p.off.p, off.data(), 4 * (size_t)nt, hipMemcpyHostToDevice));
-  HIPC(hipMemcpy(p.cap.p, cap.data(), 4 * (size_t)nt, hipMemcpyHostToDevice));
+  TRY(copy_on(s->stream, p.off.p, off.data(), 4 * (size_t)nt, hipMemcpyHostToDevice));
+  TRY(copy_on(s->stream, p.cap.p, cap.data(), 4 * (size_t)nt, hipMemcpyHostToDevice));
   p.h_off = off;
   p.h_cap = cap;
   p.cap_total = total;
   TRY(upload_descs(s));
   // the fill of the overflowed tiles, on the new layout (the halt flag cleared for it)
   Ctrl c{};
-  HIPC(hipMemcpy(&c, s->ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost));
+  TRY(copy_on(s->stream, &c, s->ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost));
   Ctrl z = c;
   z.halt = 0;
-  HIPC(hipMemcpy(s->ctrl.p, &z, sizeof(Ctrl), hipMemcpyHostToDevice));
+  TRY(copy_on(s->stream, s->ctrl.p, &z, sizeof(Ctrl), hipMemcpyHostToDevice));
   const FillKernel fk = pick_fill();
   hipLaunchKernelGGL(fk.f, dim3((unsigned)over.size()), dim3(fk.nt), 0, s->stream,
                      s->d_spec.as<SpecDesc>(), d_work.as<int2>());
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(s->stream));
   Ctrl after{};
-  HIPC(hipMemcpy(&after, s->ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost));
-  HIPC(hipMemcpy(s->ctrl.p, &c, sizeof(Ctrl), hipMemcpyHostToDevice));
+  TRY(copy_on(s->stream, &after, s->ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost));
+  TRY(copy_on(s->stream, s->ctrl.p, &c, sizeof(Ctrl), hipMemcpyHostToDevice));
   if (after.halt != 0 || after.error != 0) return OSR_OK;
-  TRY(download(cnt, p.cnt, nt));
+  TRY(download(s->stream, cnt, p.cnt, nt));
   int64_t np = 0;
   for (int v : cnt) np += v;
   p.np = np;
@@ -1094,9 +1104,9 @@ int zp_fill(osr_sim* s, int64_t first, int n) {
       }
   }
   const long long f = first;
-  HIPC(hipMemcpy(s->zp_tab.p, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice));
-  HIPC(hipMemcpy(s->zp_mask.p, mask.data(), sizeof(int) * mask.size(), hipMemcpyHostToDevice));
-  HIPC(hipMemcpy(s->zp_first.p, &f, sizeof(f), hipMemcpyHostToDevice));
+  TRY(copy_on(s->stream, s->zp_tab.p, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice));
+  TRY(copy_on(s->stream, s->zp_mask.p, mask.data(), sizeof(int) * mask.size(), hipMemcpyHostToDevice));
+  TRY(copy_on(s->stream, s->zp_first.p, &f, sizeof(f), hipMemcpyHostToDevice));
   return OSR_OK;
 }
 
@@ -1160,7 +1170,7 @@ int ensure_hist(osr_sim* s, int64_t rows) {
   DBuf nb;
   TRY(nb.alloc(sizeof(double) * (size_t)cap * ncol));
   if (s->hist_cap > 0 && s->hist.p)
-    HIPC(hipMemcpy(nb.p, s->hist.p, sizeof(double) * (size_t)s->hist_cap * ncol,
+    TRY(copy_on(s->stream, nb.p, s->hist.p, sizeof(double) * (size_t)s->hist_cap * ncol,
                    hipMemcpyDeviceToDevice));
   s->hist.swap(nb);
   s->hist_cap = cap;
@@ -1343,12 +1353,12 @@ int osr_sim_create(const osr_sim_desc* desc, osr_sim** out) {
   {
     std::vector<int2> yw((size_t)s->nblk);
     for (int b = 0; b < s->nblk; b++) yw[b] = make_int2(0, b);
-    HIPC(hipMemcpy(s->d_ywork.p, yw.data(), sizeof(int2) * yw.size(), hipMemcpyHostToDevice));
+    TRY(copy_on(s->stream, s->d_ywork.p, yw.data(), sizeof(int2) * yw.size(), hipMemcpyHostToDevice));
     std::vector<int2> ew(yw.begin(), yw.begin() + s->nblk_e);
     for (int b = 0; b < nwall_e; b++) ew.push_back(make_int2(0, -1 - b));
-    HIPC(hipMemcpy(s->d_ework.p, ew.data(), sizeof(int2) * ew.size(), hipMemcpyHostToDevice));
+    TRY(copy_on(s->stream, s->d_ework.p, ew.data(), sizeof(int2) * ew.size(), hipMemcpyHostToDevice));
     int z = 0;
-    HIPC(hipMemcpy(s->d_fwork.p, &z, sizeof(int), hipMemcpyHostToDevice));
+    TRY(copy_on(s->stream, s->d_fwork.p, &z, sizeof(int), hipMemcpyHostToDevice));
   }
   update_jscale(s.get());
   TRY(ensure_hist(s.get(), 1024));
@@ -1386,7 +1396,7 @@ int osr_sim_add_species(osr_sim* s, float rqm, int pusher, int* sid) {
   std::vector<int2> w((size_t)ns * s->ntiles);
   for (int i = 0; i < ns; i++)
     for (int t = 0; t < s->ntiles; t++) w[(size_t)i * s->ntiles + t] = make_int2(i, xcd_tile(s->dims, t, s->ntiles));
-  HIPC(hipMemcpy(s->d_work.p, w.data(), sizeof(int2) * w.size(), hipMemcpyHostToDevice));
+  TRY(copy_on(s->stream, s->d_work.p, w.data(), sizeof(int2) * w.size(), hipMemcpyHostToDevice));
   // history rows gain a column: restart the history buffer
   s->hist.release();
   s->hist_cap = 0;
@@ -1538,8 +1548,8 @@ static int fields_xfer(osr_sim* s, float* e, float* b, const int* ld, const int*
   std::vector<float> pe(3 * nc), pb(3 * nc);
   if (set && s->g.open_nx) {  // the guard band keeps its values
     HIPC(hipStreamSynchronize(s->stream));
-    HIPC(hipMemcpy(pe.data(), s->E.p, 12 * nc, hipMemcpyDeviceToHost));
-    HIPC(hipMemcpy(pb.data(), s->B.p, 12 * nc, hipMemcpyDeviceToHost));
+    TRY(copy_on(s->stream, pe.data(), s->E.p, 12 * nc, hipMemcpyDeviceToHost));
+    TRY(copy_on(s->stream, pb.data(), s->B.p, 12 * nc, hipMemcpyDeviceToHost));
   }
   const long ld0 = ld[0], ld1 = ld[1];
   auto ref = [&](int i, int j, int k) -> size_t {
@@ -1559,13 +1569,13 @@ static int fields_xfer(osr_sim* s, float* e, float* b, const int* ld, const int*
           }
         }
     HIPC(hipStreamSynchronize(s->stream));
-    if (e) HIPC(hipMemcpy(s->E.p, pe.data(), 12 * nc, hipMemcpyHostToDevice));
-    if (b) HIPC(hipMemcpy(s->B.p, pb.data(), 12 * nc, hipMemcpyHostToDevice));
+    if (e) TRY(copy_on(s->stream, s->E.p, pe.data(), 12 * nc, hipMemcpyHostToDevice));
+    if (b) TRY(copy_on(s->stream, s->B.p, pb.data(), 12 * nc, hipMemcpyHostToDevice));
     return restore_walls(s);
   }
   HIPC(hipStreamSynchronize(s->stream));
-  HIPC(hipMemcpy(pe.data(), s->E.p, 12 * nc, hipMemcpyDeviceToHost));
-  HIPC(hipMemcpy(pb.data(), s->B.p, 12 * nc, hipMemcpyDeviceToHost));
+  TRY(copy_on(s->stream, pe.data(), s->E.p, 12 * nc, hipMemcpyDeviceToHost));
+  TRY(copy_on(s->stream, pb.data(), s->B.p, 12 * nc, hipMemcpyDeviceToHost));
   // every array cell (guards included) gets its periodic image; with open x walls the x
   // guard cells come from the guard band of the ring
   const int ldk = D == 3 ? ld[2] : 1;
@@ -1606,7 +1616,7 @@ int osr_fields_fill(osr_sim* s, const double* e0, const double* b0) {
     const double* f = w == 0 ? e0 : b0;
     if (!f) continue;
     for (int m = 0; m < 3; m++) std::fill(v.begin() + m * nc, v.begin() + (m + 1) * nc, (float)f[m]);
-    HIPC(hipMemcpy(w == 0 ? s->E.p : s->B.p, v.data(), 12 * nc, hipMemcpyHostToDevice));
+    TRY(copy_on(s->stream, w == 0 ? s->E.p : s->B.p, v.data(), 12 * nc, hipMemcpyHostToDevice));
   }
   return restore_walls(s);
 }
@@ -1615,7 +1625,7 @@ int osr_fields_fill(osr_sim* s, const double* e0, const double* b0) {
 static int refresh_counts(osr_sim* s) {
   for (auto& p : s->sp) {
     std::vector<int> cnt((size_t)s->ntiles);
-    HIPC(hipMemcpy(cnt.data(), p->cnt.p, sizeof(int) * cnt.size(), hipMemcpyDeviceToHost));
+    TRY(copy_on(s->stream, cnt.data(), p->cnt.p, sizeof(int) * cnt.size(), hipMemcpyDeviceToHost));
     int64_t n = 0;
     for (int c : cnt) n += c;
     p->np = n;
@@ -1651,7 +1661,7 @@ int osr_sim_run(osr_sim* s, int nsteps) {
       TRY(relayout(s, true));
       Ctrl z = c;
       z.halt = 0;
-      HIPC(hipMemcpy(s->ctrl.p, &z, sizeof(Ctrl), hipMemcpyHostToDevice));
+      TRY(copy_on(s->stream, s->ctrl.p, &z, sizeof(Ctrl), hipMemcpyHostToDevice));
       TRY(build_graph(s));
       batch = 8;
     } else if (done != nb) {
@@ -1680,7 +1690,7 @@ int osr_sim_history(osr_sim* s, int64_t first, int64_t count, double* fld, doubl
   std::vector<double> rows((size_t)count * ncol);
   HIPC(hipStreamSynchronize(s->stream));
   if (count)
-    HIPC(hipMemcpy(rows.data(), s->hist.as<double>() + first * ncol,
+    TRY(copy_on(s->stream, rows.data(), s->hist.as<double>() + first * ncol,
                    sizeof(double) * rows.size(), hipMemcpyDeviceToHost));
   for (int64_t r = 0; r < count; r++) {
     if (fld)
@@ -1715,7 +1725,7 @@ int osr_sim_save_field(osr_sim* s, int quant, const char* ms_dir, const double* 
   std::vector<float> v(nc);
   HIPC(hipStreamSynchronize(s->stream));
   const DBuf& src = quant < 3 ? s->E : s->B;
-  HIPC(hipMemcpy(v.data(), src.as<float>() + (size_t)(quant % 3) * nc, 4 * nc,
+  TRY(copy_on(s->stream, v.data(), src.as<float>() + (size_t)(quant % 3) * nc, 4 * nc,
                  hipMemcpyDeviceToHost));
   if (s->g.open_nx) {  // interior columns of the x ring only
     const int nxi = s->g.open_nx, nxr = s->g.nx;
@@ -1774,14 +1784,14 @@ int osr_sim_gauss(osr_sim* s, double* res, double* rho_out, double* maxabs) {
   HIPC(hipGetLastError());
   std::vector<double> h(nc);
   HIPC(hipStreamSynchronize(s->stream));
-  HIPC(hipMemcpy(h.data(), dres.p, 8 * nc, hipMemcpyDeviceToHost));
+  TRY(copy_on(s->stream, h.data(), dres.p, 8 * nc, hipMemcpyDeviceToHost));
   double m = 0;
   for (double v : h) m = std::max(m, std::fabs(v));
   if (maxabs) *maxabs = m;
   if (res) std::memcpy(res, h.data(), 8 * nc);
   if (rho_out) {
     std::vector<long long> r(nc);
-    HIPC(hipMemcpy(r.data(), rho.p, 8 * nc, hipMemcpyDeviceToHost));
+    TRY(copy_on(s->stream, r.data(), rho.p, 8 * nc, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < nc; i++) rho_out[i] = (double)r[i] / scale;
   }
   return OSR_OK;
@@ -1923,7 +1933,7 @@ int osr_sim_time_kernels(osr_sim* s, int nsteps, double* ms) {
       TRY(relayout(s, true));
       Ctrl z = c;
       z.halt = 0;
-      HIPC(hipMemcpy(s->ctrl.p, &z, sizeof(Ctrl), hipMemcpyHostToDevice));
+      TRY(copy_on(s->stream, s->ctrl.p, &z, sizeof(Ctrl), hipMemcpyHostToDevice));
       continue;  // a re-laid-out step is not representative
     }
     float t;
@@ -2492,9 +2502,9 @@ int osr_sim_add_zpulse_wall(osr_sim* s, const osr_zpulse_wall* zp) {
   TRY(s->zp_tab.alloc(sizeof(float) * (size_t)rows * K * 2 * s->g.ny * s->g.nz));
   TRY(s->zp_mask.alloc(sizeof(int) * (size_t)rows * K));
   TRY(s->zp_first.alloc(sizeof(long long)));
-  HIPC(hipMemset(s->zp_mask.p, 0, sizeof(int) * (size_t)rows * K));
+  HIPC(hipMemsetAsync(s->zp_mask.p, 0, sizeof(int) * (size_t)rows * K, s->stream));
   const long long none = -(1ll << 62);  // no table yet: every step falls outside it
-  HIPC(hipMemcpy(s->zp_first.p, &none, sizeof(none), hipMemcpyHostToDevice));
+  TRY(copy_on(s->stream, s->zp_first.p, &none, sizeof(none), hipMemcpyHostToDevice));
   TRY(upload_descs(s));
   s->graph_ok = false;
   return OSR_OK;
@@ -2539,8 +2549,8 @@ int osr_sim_add_zpulse(osr_sim* s, const osr_zpulse* zp) {
   const size_t nc = (size_t)s->g.ncell, gn = g.cells();
   std::vector<float> pe(3 * nc), pb(3 * nc), eb(3 * gn), bb(3 * gn);
   HIPC(hipStreamSynchronize(s->stream));
-  HIPC(hipMemcpy(pe.data(), s->E.p, 12 * nc, hipMemcpyDeviceToHost));
-  HIPC(hipMemcpy(pb.data(), s->B.p, 12 * nc, hipMemcpyDeviceToHost));
+  TRY(copy_on(s->stream, pe.data(), s->E.p, 12 * nc, hipMemcpyDeviceToHost));
+  TRY(copy_on(s->stream, pb.data(), s->B.p, 12 * nc, hipMemcpyDeviceToHost));
   const int ring = s->g.nx;
   double phase0 = w.phase0, pol = w.pol;
   for (int k = 0; k < (w.pol_type != 0 ? 2 : 1); k++) {
@@ -2573,8 +2583,8 @@ int osr_sim_add_zpulse(osr_sim* s, const osr_zpulse* zp) {
           }
         }
   }
-  HIPC(hipMemcpy(s->E.p, pe.data(), 12 * nc, hipMemcpyHostToDevice));
-  HIPC(hipMemcpy(s->B.p, pb.data(), 12 * nc, hipMemcpyHostToDevice));
+  TRY(copy_on(s->stream, s->E.p, pe.data(), 12 * nc, hipMemcpyHostToDevice));
+  TRY(copy_on(s->stream, s->B.p, pb.data(), 12 * nc, hipMemcpyHostToDevice));
   return restore_walls(s);
 }
 

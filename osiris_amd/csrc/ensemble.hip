@@ -168,8 +168,7 @@ int ens_admit(osr_ensemble* e, EnsJob* j) {
   TRY(ensure_hist(s, jd.nsteps));
   Ctrl c{};
   c.target = jd.nsteps;
-  HIPC(hipStreamSynchronize(s->stream));
-  HIPC(hipMemcpy(s->ctrl.p, &c, sizeof(Ctrl), hipMemcpyHostToDevice));
+  TRY(copy_on(s->stream, s->ctrl.p, &c, sizeof(Ctrl), hipMemcpyHostToDevice));
   int64_t bytes = 0;
   TRY(osr_sim_footprint(s, &bytes));
   int64_t np = 0;
@@ -239,7 +238,8 @@ int ens_build(osr_ensemble* e, EnsBatch* b) {
   auto put = [&](DBuf& d, const void* src, size_t bytes) -> int {
     if (bytes == 0) return OSR_OK;
     if (d.n < bytes) TRY(d.alloc(bytes + bytes / 2));
-    HIPC(hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice));
+    // on the batch's stream: the step graph launched there next reads these arrays
+    TRY(copy_on(b->stream, d.p, src, bytes, hipMemcpyHostToDevice));
     return OSR_OK;
   };
   TRY(put(b->spec, specs.data(), sizeof(SpecDesc) * specs.size()));
@@ -351,13 +351,13 @@ int ens_collect(osr_ensemble* e, EnsBatch* b) {
       TRY(relayout(s, true));
       Ctrl z = c;
       z.halt = done ? 3 : 0;
-      HIPC(hipMemcpy(s->ctrl.p, &z, sizeof(Ctrl), hipMemcpyHostToDevice));
+      TRY(copy_on(s->stream, s->ctrl.p, &z, sizeof(Ctrl), hipMemcpyHostToDevice));
       if (s->desc_version != v) b->dirty = true;
     }
     if (done) {
       j->hist.resize((size_t)j->desc.nsteps * j->ncol);
-      HIPC(hipMemcpy(j->hist.data(), s->hist.p, sizeof(double) * j->hist.size(),
-                     hipMemcpyDeviceToHost));
+      TRY(copy_on(s->stream, j->hist.data(), s->hist.p, sizeof(double) * j->hist.size(),
+                      hipMemcpyDeviceToHost));
       (void)osr_sim_destroy(s);
       j->sim = nullptr;
       e->used -= j->bytes;
@@ -419,8 +419,11 @@ int osr_ens_create(const osr_ens_desc* desc, osr_ensemble** out) {
   *out = nullptr;
 #ifndef OSR_ENSEMBLE_HOST_CHECK
   // Switched off: a long mixed serving run (256 jobs, grids up to 512^2, up to 2000
-  // steps) ended in a GPU memory fault whose cause is not identified yet.  The scheduler
-  // itself is exercised on the CPU (host-logic harness over a validating mock runtime).
+  // steps) ended in a GPU memory fault.  The cause found on the CPU, descriptor and work-
+  // list uploads on the null stream that the batch streams did not wait for (a batch
+  // graph could read the descriptors of retired jobs), is fixed by the stream-ordered
+  // copies (copy_on); the fix has not run on the GPU yet.  The scheduler is exercised on
+  // the CPU (host-logic harness over a validating mock runtime).
   return fail(OSR_ERR_STATE,
               "ensemble execution is switched off: a GPU memory fault in long mixed runs is "
               "under investigation");

@@ -185,7 +185,8 @@ __attribute__((constructor)) static void mock_init() { signal(SIGSEGV, mock_segv
 // number of violations reported for (1) a pool block freed on the allocation stream while
 // a kernel queued on another stream still uses it, (2) a host-synchronous copy into a
 // block with queued kernel work, (3) a kernel on another stream touching a pool block
-// before its allocating stream executed the allocation.  Expected: 3.
+// before its allocating stream executed the allocation, (4) a kernel on a stream reading
+// a block that a host-synchronous copy (null stream) wrote just before.  Expected: 4.
 __global__ void mock_touch_kernel(float* p) { (void)p; }
 extern "C" long long mock_selftest_planted() {
   const long long before = mock::nordered();
@@ -216,6 +217,12 @@ extern "C" long long mock_selftest_planted() {
   hipStreamSynchronize(a);
   hipFreeAsync(r, a);
   hipStreamSynchronize(a);
+  void* t = nullptr;
+  hipMalloc(&t, 4096);
+  hipMemcpy(t, h, sizeof(h), hipMemcpyHostToDevice);
+  hipLaunchKernelGGL(mock_touch_kernel, dim3(1), dim3(64), 0, b, (float*)t);  // (4)
+  hipStreamSynchronize(b);
+  hipFree(t);
   hipStreamDestroy(a);
   hipStreamDestroy(b);
   g_mock_progress_p = keep_p;

@@ -22,7 +22,10 @@
 //     copy, only after its allocating stream has executed the allocation;
 //   * a host-synchronous copy or fill (null stream, which does not order with
 //     non-blocking streams) touches no block that a stream still has queued work on;
-//   * pinned host memory is not freed while a queued copy targets it.
+//   * pinned host memory is not freed while a queued copy targets it;
+//   * work enqueued on a stream touches no block written by a host-synchronous copy or
+//     fill (null stream) since the last device-wide synchronisation: the streams do not
+//     wait for the null stream, so such work may not see the write.
 // Graph capture records kernel launches for replay (each replay enqueues them again).
 // Kernel launches are handed to osr_mock_exec overloads (capi_mock.hip), which validate
 // every descriptor pointer the kernel would dereference and report, through mock::check,
@@ -134,8 +137,21 @@ inline void exec_one(hipStream_t s) {
 inline void drain(hipStream_t s) {
   while (!streams()[s].q.empty()) exec_one(s);
 }
+// blocks written by host-synchronous copies or fills on the null stream since the last
+// device-wide synchronisation: the library's streams do not wait for the null stream, so
+// work enqueued on one of them may not see such a write
+inline std::set<uintptr_t>& null_written() { static std::set<uintptr_t> w; return w; }
 inline void drain_all() {
   for (auto& kv : streams()) drain(kv.first);
+  null_written().clear();
+}
+// work enqueued on stream s touching `touch`
+inline void check_null_order(hipStream_t s, const std::set<uintptr_t>& touch) {
+  if (!s) return;  // the null stream itself is ordered with its copies
+  for (uintptr_t b : touch)
+    if (null_written().count(b))
+      order_err("stream work touches a block written by a null-stream copy it is not ordered after",
+                (void*)b);
 }
 inline void tick() {
   std::uniform_real_distribution<double> U(0, 1);
@@ -174,6 +190,7 @@ inline void enqueue_launch(const Launch& L, hipStream_t s) {
   effects() = false;
   o.touch = touch_of(L.validate);
   effects() = true;
+  check_null_order(s, o.touch);
   o.run = L.run;
   enqueue(s, std::move(o));
 }
@@ -274,7 +291,11 @@ inline void host_access(const void* p, size_t n, const char* what) {
 inline hipError_t m_hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind k) {
   mock::tick();
   if (mock::capturing()) mock::err("synchronous copy during capture", d, n);
-  if (k == hipMemcpyHostToDevice || k == hipMemcpyDeviceToDevice) host_access(d, n, "memcpy dst");
+  if (k == hipMemcpyHostToDevice || k == hipMemcpyDeviceToDevice) {
+    host_access(d, n, "memcpy dst");
+    auto it = mock::find(d, n);
+    if (it != mock::live().end()) mock::null_written().insert(it->first);
+  }
   if (k == hipMemcpyDeviceToHost || k == hipMemcpyDeviceToDevice) host_access(s, n, "memcpy src");
   memcpy(d, s, n);
   return hipSuccess;
@@ -293,6 +314,7 @@ inline hipError_t m_hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKi
   };
   mock::Op o;
   o.touch = mock::touch_of(validate);
+  mock::check_null_order(st, o.touch);
   o.run = [=]() {
     validate();
     memcpy(d, s, n);
@@ -303,6 +325,8 @@ inline hipError_t m_hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKi
 inline hipError_t m_hipMemset(void* d, int v, size_t n) {
   mock::tick();
   host_access(d, n, "memset");
+  auto it = mock::find(d, n);
+  if (it != mock::live().end()) mock::null_written().insert(it->first);
   memset(d, v, n);
   return hipSuccess;
 }
@@ -311,6 +335,7 @@ inline hipError_t m_hipMemsetAsync(void* d, int v, size_t n, hipStream_t st) {
   if (mock::capturing()) mock::err("memset in capture", d, n);
   mock::Op o;
   o.touch = mock::touch_of([=]() { mock::check(d, n, "memsetAsync"); });
+  mock::check_null_order(st, o.touch);
   o.run = [=]() {
     mock::check(d, n, "memsetAsync");
     if (mock::valid(d, n)) memset(d, v, n);

@@ -62,14 +62,15 @@ def test_ensemble_host_logic_large_jobs(mock_lib):
 def test_stream_order_checker_catches_planted_mistakes(mock_lib):
     """The mock runtime reports each planted stream-order mistake (a pool block freed while
     another stream still has queued work on it, a host-synchronous copy racing queued
-    work, a pool block used by another stream before its allocation executed)."""
+    work, a pool block used by another stream before its allocation executed, a kernel on
+    a stream reading what a null-stream copy just wrote)."""
     r = subprocess.run([sys.executable, "-c",
                         "import ctypes as C, sys; L = C.CDLL(sys.argv[1]); "
                         "L.mock_selftest_planted.restype = C.c_longlong; "
                         "print(L.mock_selftest_planted())", mock_lib],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr[-2000:]
-    assert int(r.stdout.strip().splitlines()[-1]) == 3
+    assert int(r.stdout.strip().splitlines()[-1]) == 4
 
 
 def test_ensemble_host_logic_realtime_arrivals(mock_lib):
