@@ -98,6 +98,18 @@ void osr_mock_exec(const void*, dim3 g, const FieldDesc* f, const int2* w) {
     if (seen.insert(wk.x).second) check_field(f[wk.x]);
   }
 }
+// the step end (history row = step + column, step counter, halt protocol)
+static void emulate_step_end(const FieldDesc& fd) {
+  Ctrl* c = fd.g.ctrl;
+  if (!mock::valid(c, sizeof(Ctrl)) || c->halt >= 2) return;
+  const long long step = c->steps;
+  const int ncol = 6 + fd.nspec;
+  if (step < fd.hist_cap)
+    for (int col = 0; col < ncol; col++) fd.hist[step * ncol + col] = (double)step + col;
+  c->steps = step + 1;
+  if (c->halt == 1) c->halt = 2;
+  else if (c->target > 0 && step + 1 >= c->target) c->halt = 3;
+}
 void osr_mock_exec(const void*, dim3 g, const FieldDesc* f, const int* w) {
   mock::check(w, (size_t)g.x * 4, "fin work");
   for (unsigned b = 0; b < g.x; b++) {
@@ -105,17 +117,23 @@ void osr_mock_exec(const void*, dim3 g, const FieldDesc* f, const int* w) {
     mock::check(&f[w[b]], sizeof(FieldDesc), "fin fd");
     const FieldDesc& fd = f[w[b]];
     check_field(fd);
-    if (!mock::effects()) continue;
-    Ctrl* c = fd.g.ctrl;
-    if (!mock::valid(c, sizeof(Ctrl)) || c->halt >= 2) continue;
-    const long long step = c->steps;
-    const int ncol = 6 + fd.nspec;
-    if (step < fd.hist_cap)
-      for (int col = 0; col < ncol; col++) fd.hist[step * ncol + col] = (double)step + col;
-    c->steps = step + 1;
-    if (c->halt == 1) c->halt = 2;
-    else if (c->target > 0 && step + 1 >= c->target) c->halt = 3;
+    if (mock::effects()) emulate_step_end(fd);
   }
+}
+// kernels of one simulation's walls and lasers (one field descriptor): the descriptor and
+// every array it names are checked; the Lindman step end (wall_b2_end) also ends the step
+void osr_mock_exec(const void* k, dim3, const FieldDesc* f) {
+  if (!mock::valid(f, sizeof(FieldDesc))) { mock::err("wall fd", f, 0); return; }
+  mock::check(f, sizeof(FieldDesc), "wall fd");
+  check_field(*f);
+  if (f->wall) mock::check(f->wall, 4, "wall buffers");
+  if (f->las_k) {
+    mock::check(f->las_tab, 4, "laser table");
+    mock::check(f->las_mask, 4, "laser mask");
+    mock::check(f->las_first, 8, "laser first row");
+  }
+  if (mock::effects() && (k == (const void*)wall_b2_end<2> || k == (const void*)wall_b2_end<3>))
+    emulate_step_end(*f);
 }
 void osr_mock_exec(const void*, dim3, const SpecDesc* d, int sid, GenArgs a) {
   mock::check(&d[sid], sizeof(SpecDesc), "gen desc");

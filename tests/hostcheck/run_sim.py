@@ -2,7 +2,8 @@
 prints one JSON line of results (invoked by tests/test_ensemble_host.py in a subprocess):
 creation, particle generation, runs of several lengths through the captured step graphs,
 device re-layouts after injected tile overflows, the per-kernel timing, history and
-particle read-back, and destruction.  The mock's step end writes history row r, column c
+particle read-back, and destruction; periodic grids, and Lindman walls with a laser wall
+(its table refilled before every batch of steps).  The mock's step end writes history row r, column c
 as r + c, so the rows read back show whether every step was counted once."""
 import ctypes as C
 import json
@@ -31,13 +32,18 @@ def main():
 
     rng = np.random.default_rng(seed)
     bad_rows = relayouts = steps = 0
-    for nx, order, tile in (((64, 48), 2, (16, 16)), ((96, 64), 1, (32, 16)),
-                            ((64, 64), 3, (16, 16))):
-        with Simulation(2, nx, (0.1, 0.1), 0.07, order, tile=tile) as s:
+    for nx, order, tile, xbnd in (((64, 48), 2, (16, 16), "periodic"),
+                                  ((96, 64), 1, (32, 16), "periodic"),
+                                  ((64, 64), 3, (16, 16), "periodic"),
+                                  ((64, 32), 2, (16, 16), "lindman")):
+        with Simulation(2, nx, (0.1, 0.1), 0.07, order, tile=tile, xbnd=xbnd) as s:
             for _ in range(2):
                 sid = s.add_species(-1.0)
                 s.generate(sid, (2, 2), density=1.0, uth=(0.05,) * 3,
                            seed=int(rng.integers(1, 1 << 30)))
+            if xbnd != "periodic":  # a laser wall: its table is refilled before each batch
+                s.add_zpulse_wall(a0=1.0, omega0=10.0, tenv_type="sin2", tenv_rise=2.0,
+                                  tenv_flat=0.0, tenv_fall=2.0)
             for n in (5, 13, 40):
                 s.run(n)
             s.time_kernels(3)

@@ -92,15 +92,15 @@ def test_ensemble_host_logic_random_bursts(mock_lib, seed):
 
 @pytest.mark.parametrize("seed", [1, 2])
 def test_simulation_host_logic_with_relayouts(mock_lib, seed):
-    """A single simulation's own path over the mock runtime: runs through the captured
-    step graphs, device re-layouts after injected overflows, per-kernel timing, history
-    and particle read-back.  No kernel or copy touches a dead or unordered block, every
+    """A single simulation's own path over the mock runtime (periodic, and Lindman walls
+    with a laser wall): runs through the captured step graphs, device re-layouts after
+    injected overflows, per-kernel timing, history and particle read-back.  No kernel or copy touches a dead or unordered block, every
     step is counted once (history rows), and all memory is returned."""
     r = subprocess.run([sys.executable, os.path.join(HC, "run_sim.py"), mock_lib, "0.02", "0.3",
                         str(seed)], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-3000:]
     res = json.loads(r.stdout.strip().splitlines()[-1])
-    assert res["steps"] == 3 * (5 + 13 + 40 + 3), res  # runs + the timed steps, per sim
+    assert res["steps"] == 4 * (5 + 13 + 40 + 3), res  # runs + the timed steps, per sim
     assert res["relayouts"] > 0, res
     assert {k: res[k] for k in ("bad_rows", "errors", "order_errors", "live")} == {
         "bad_rows": 0, "errors": 0, "order_errors": 0, "live": 0}, res
