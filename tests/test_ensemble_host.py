@@ -104,3 +104,12 @@ def test_simulation_host_logic_with_relayouts(mock_lib, seed):
     assert res["relayouts"] > 0, res
     assert {k: res[k] for k in ("bad_rows", "errors", "order_errors", "live")} == {
         "bad_rows": 0, "errors": 0, "order_errors": 0, "live": 0}, res
+
+
+def test_ensemble_host_logic_256_job_stream(mock_lib):
+    """The serving workload's job count (256 Poisson arrivals; smaller grids and step
+    counts than the GPU stream, so that the mock's host memory holds them): hundreds of
+    admissions, retirements, re-layouts and batch rebuilds while other batches have queued
+    work, with no stale or unordered access and every history complete."""
+    res = _run(mock_lib, 256, 0.01, 0.5, "64,128", "4,16", progress_p=0.1, mode="realtime")
+    assert res == {"jobs": 256, "errors": 0, "order_errors": 0, "live": 0, "bad_histories": 0}
