@@ -715,6 +715,9 @@ __global__ __launch_bounds__(NT) void tile_fill(const SpecDesc* __restrict__ des
       if (f[p] && I + r[p] < INC_MAX) s_inc[I + r[p]] = src[p];
     I += tot;
   }
+  // every wave's entries of s_inc are in LDS before any wave reads them in pass 2 (the
+  // hole count below has barriers only when the tile loses more movers than it gains)
+  __syncthreads();
   const int nn = n - H + I;
   if (I > INC_MAX || nn > d.cap[tile] - 64 * NW || H - I > TAIL_BITS) {
     if (threadIdx.x == 0) {
