@@ -419,11 +419,12 @@ int osr_ens_create(const osr_ens_desc* desc, osr_ensemble** out) {
   *out = nullptr;
 #ifndef OSR_ENSEMBLE_HOST_CHECK
   // Switched off: a long mixed serving run (256 jobs, grids up to 512^2, up to 2000
-  // steps) ended in a GPU memory fault.  The cause found on the CPU, descriptor and work-
-  // list uploads on the null stream that the batch streams did not wait for (a batch
-  // graph could read the descriptors of retired jobs), is fixed by the stream-ordered
-  // copies (copy_on); the fix has not run on the GPU yet.  The scheduler is exercised on
-  // the CPU (host-logic harness over a validating mock runtime).
+  // steps) ended in a GPU memory fault.  Two causes were found and fixed: descriptor and
+  // work-list uploads on the null stream that the batch streams did not wait for (a batch
+  // graph could read the descriptors of retired jobs; now stream-ordered, copy_on), and a
+  // missing barrier in the tile fill (a stale LDS index could read out of range).  The
+  // fixes have not run on the GPU yet.  The scheduler is exercised on the CPU (host-logic
+  // harness over a validating mock runtime).
   return fail(OSR_ERR_STATE,
               "ensemble execution is switched off: a GPU memory fault in long mixed runs is "
               "under investigation");

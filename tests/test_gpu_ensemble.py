@@ -10,10 +10,11 @@ from osiris_amd._lib import OsrError
 pytestmark = [
     pytest.mark.gpu,
     pytest.mark.skip(reason="ensemble execution is switched off: a 256-job mixed serving run "
-                            "ended in a GPU memory fault; the cause found on the CPU (uploads on "
-                            "the null stream not ordered with the batch streams) is fixed, but "
-                            "the fix has not had its GPU run yet; the scheduler is covered on "
-                            "the CPU by tests/test_ensemble_host.py"),
+                            "ended in a GPU memory fault; the two causes found (uploads on the "
+                            "null stream not ordered with the batch streams, a missing barrier "
+                            "in the tile fill) are fixed, but the fixes have not had their GPU "
+                            "run yet; the scheduler is covered on the CPU by "
+                            "tests/test_ensemble_host.py"),
 ]
 
 
