@@ -70,10 +70,10 @@ def test_full_size_invariants(name, make, tiles, steps):
 
 @pytest.mark.parametrize("xbnd", [
     pytest.param("lindman", marks=pytest.mark.skip(
-        reason="left the GPU faulted (after reporting a particle outside its tile) with "
-               "DPP-based FP64 wave reductions in the push, the energy B half step and the "
-               "step-end reduction; those reductions were taken out, the cause is not shown, "
-               "so this case stays off the GPU until it is")),
+        reason="left the GPU faulted (after reporting a particle outside its tile) in two "
+               "runs, with DPP-based FP64 wave reductions since taken out; the cause found "
+               "afterwards, a missing barrier in the tile fill (a stale LDS index), is fixed, "
+               "and this case stays off the GPU until the fix has had its run")),
     "vpml"])
 def test_laser_plasma_2048x512_full_size(xbnd):
     """The laser-plasma workload at full size (2048 x 512 cells, 8 M cold electrons at
